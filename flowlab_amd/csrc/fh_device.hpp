@@ -1,0 +1,232 @@
+// Device-side helpers shared by every kernel of libflowhip (gfx950).
+//
+// MAC layout on the device is the reference's host layout
+// (src/core/grid.ts:1-36, 118-141): p/label nx*ny stride nx, u (nx+1)*ny
+// stride nx+1, v nx*(ny+1) stride nx, x fastest -- so a wavefront of 64
+// consecutive threads along x reads 64 consecutive words.
+//
+// Positions are carried as (integer cell base, float offset) in CELL units
+// (world / h).  A backtrace moves a point by at most a few cells, so the
+// offset keeps ~2^-22 of a cell of precision at any grid size, where a
+// plain float world coordinate loses it as the grid grows (2^-10 cell at
+// 8192 cells).  Sampling semantics are grid.ts:173-250 exactly.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fh {
+
+enum : uint8_t { FLUID = 0, AIR = 1, SOLID = 2 };
+
+// grid.ts:159-161: off-grid counts as solid inside a stencil.
+__device__ __forceinline__ bool solid_or_outside(const uint8_t* __restrict__ l, int nx, int ny,
+                                                 int i, int j) {
+  return i < 0 || j < 0 || i >= nx || j >= ny || l[i + j * nx] == SOLID;
+}
+
+// boundaries.ts:23-25: in-domain AND solid.
+__device__ __forceinline__ bool is_solid_cell(const uint8_t* __restrict__ l, int nx, int ny,
+                                              int i, int j) {
+  return i >= 0 && j >= 0 && i < nx && j < ny && l[i + j * nx] == SOLID;
+}
+
+struct Pos {
+  int i;    // integer cell base
+  float f;  // offset in cells
+};
+
+struct Axis {
+  int i0;
+  float f;
+};
+
+// grid.ts:173-177 for pos = base + off: clamp pos to [0, count-1], then
+// i0 = min(floor(pos), count-2), f = pos - i0.
+__device__ __forceinline__ Axis clamped_axis(int base, float off, int count) {
+  float fl = floorf(off);
+  int i0 = base + (int)fl;
+  float f = off - fl;
+  if (i0 < 0) {
+    i0 = 0;
+    f = 0.f;
+  } else if (i0 > count - 2) {
+    i0 = count - 2;
+    f = 1.f;
+  }
+  return {i0, f};
+}
+
+// grid.ts:180-191
+__device__ __forceinline__ float bilerp(float v00, float v10, float v01, float v11, float fx,
+                                        float fy) {
+  float lo = v00 + fx * (v10 - v00);
+  float hi = v01 + fx * (v11 - v01);
+  return lo + fy * (hi - lo);
+}
+
+// Gather of the four stencil values of a field with row stride `s`.
+struct Quad {
+  float a, b, c, d;
+};
+
+__device__ __forceinline__ Quad gather4(const float* __restrict__ q, int s, Axis ax, Axis ay) {
+  const float* r0 = q + ax.i0 + ay.i0 * s;
+  return {r0[0], r0[1], r0[s], r0[s + 1]};
+}
+
+// grid.ts:197-208: u at (i, j+1/2).
+__device__ __forceinline__ float sample_u(const float* __restrict__ u, int nx, int ny, Pos x,
+                                          Pos y) {
+  Axis ax = clamped_axis(x.i, x.f, nx + 1);
+  Axis ay = clamped_axis(y.i, y.f - 0.5f, ny);
+  Quad g = gather4(u, nx + 1, ax, ay);
+  return bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
+}
+
+// grid.ts:214-225: v at (i+1/2, j).
+__device__ __forceinline__ float sample_v(const float* __restrict__ v, int nx, int ny, Pos x,
+                                          Pos y) {
+  Axis ax = clamped_axis(x.i, x.f - 0.5f, nx);
+  Axis ay = clamped_axis(y.i, y.f, ny + 1);
+  Quad g = gather4(v, nx, ax, ay);
+  return bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
+}
+
+// grid.ts:239-250 on a cell-centred grid of nx*ny; x/y in that grid's cells.
+__device__ __forceinline__ float sample_p(const float* __restrict__ q, int nx, int ny, Pos x,
+                                          Pos y) {
+  Axis ax = clamped_axis(x.i, x.f - 0.5f, nx);
+  Axis ay = clamped_axis(y.i, y.f - 0.5f, ny);
+  Quad g = gather4(q, nx, ax, ay);
+  return bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
+}
+
+// grid.ts:268-270
+__device__ __forceinline__ float clamp_to_quad(float q, Quad g) {
+  float lo = fminf(fminf(g.a, g.b), fminf(g.c, g.d));
+  float hi = fmaxf(fmaxf(g.a, g.b), fmaxf(g.c, g.d));
+  return fminf(fmaxf(q, lo), hi);
+}
+
+__device__ __forceinline__ Pos shift(Pos p, float d) { return {p.i, p.f + d}; }
+
+// advect.ts:41-55 in cell units: c = dt/h.
+//   mid  = P - c/2 * (u0, v0);   prev = P - c * (u(mid), v(mid))
+__device__ __forceinline__ void backtrace(const float* __restrict__ uc,
+                                          const float* __restrict__ vc, int nx, int ny, Pos x,
+                                          Pos y, float u0, float v0, float c, Pos* bx, Pos* by) {
+  Pos mx = shift(x, -0.5f * c * u0);
+  Pos my = shift(y, -0.5f * c * v0);
+  *bx = shift(x, -c * sample_u(uc, nx, ny, mx, my));
+  *by = shift(y, -c * sample_v(vc, nx, ny, mx, my));
+}
+
+// ---- reductions ---------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+}  // namespace fh
+
+namespace fh {
+
+// Deterministic grid-wide reduction of NS doubles: each block reduces its
+// threads' values (wavefront shuffles, then across waves in LDS), writes one
+// partial per slot, and the last block to finish sums the partials in block
+// order and writes out[0..NS).  The counter is left at zero for the next
+// launch.  Block size must be a multiple of 64 and at most 1024.
+template <int NS>
+__device__ void grid_sum(double (&val)[NS], double* partials, unsigned* counter, double* out) {
+  __shared__ double sh[NS][16];
+  __shared__ bool last;
+  const int nthreads = blockDim.x * blockDim.y;
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nblocks = gridDim.x * gridDim.y;
+  const int bid = blockIdx.x + blockIdx.y * gridDim.x;
+  const int lane = tid & 63, wid = tid >> 6, nw = nthreads >> 6;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    double v = wave_sum(val[s]);
+    if (lane == 0) sh[s][wid] = v;
+  }
+  __syncthreads();
+  if (tid == 0) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      double t = 0;
+      for (int w = 0; w < nw; w++) t += sh[s][w];
+      partials[(size_t)bid * NS + s] = t;
+    }
+    __threadfence();
+    unsigned done = atomicAdd(counter, 1u);
+    last = (done == (unsigned)nblocks - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const volatile double* vp = partials;
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    double t = 0;
+    for (int b = tid; b < nblocks; b += nthreads) t += vp[(size_t)b * NS + s];
+    t = wave_sum(t);
+    __syncthreads();
+    if (lane == 0) sh[s][wid] = t;
+    __syncthreads();
+    if (tid == 0) {
+      double a = 0;
+      for (int w = 0; w < nw; w++) a += sh[s][w];
+      out[s] = a;
+    }
+  }
+  if (tid == 0) *counter = 0u;
+}
+
+// Same pattern for a max of non-negative floats.
+__device__ inline void grid_max(float val, float* partials, unsigned* counter, float* out) {
+  __shared__ float sh[16];
+  __shared__ bool last;
+  const int nthreads = blockDim.x * blockDim.y;
+  const int tid = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nblocks = gridDim.x * gridDim.y;
+  const int bid = blockIdx.x + blockIdx.y * gridDim.x;
+  const int lane = tid & 63, wid = tid >> 6, nw = nthreads >> 6;
+  float v = wave_max(val);
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  if (tid == 0) {
+    float t = 0.f;
+    for (int w = 0; w < nw; w++) t = fmaxf(t, sh[w]);
+    partials[bid] = t;
+    __threadfence();
+    unsigned done = atomicAdd(counter, 1u);
+    last = (done == (unsigned)nblocks - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const volatile float* vp = partials;
+  float t = 0.f;
+  for (int b = tid; b < nblocks; b += nthreads) t = fmaxf(t, vp[b]);
+  t = wave_max(t);
+  __syncthreads();
+  if (lane == 0) sh[wid] = t;
+  __syncthreads();
+  if (tid == 0) {
+    float a = 0.f;
+    for (int w = 0; w < nw; w++) a = fmaxf(a, sh[w]);
+    out[0] = a;
+    *counter = 0u;
+  }
+}
+
+}  // namespace fh

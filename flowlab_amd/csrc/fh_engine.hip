@@ -1,0 +1,1005 @@
+// libflowhip engine: device state of one MAC grid and the C ABI of
+// include/flowhip.h.  One engine = one grid on one HIP stream; nothing here
+// synchronises except where a value must reach the host (solver
+// convergence checks, readbacks, timings).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/flowhip.h"
+#include "fh_device.hpp"
+#include "fh_launch.hpp"
+
+using namespace fh;
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const std::string& s) { g_err = s; }
+
+struct Level {
+  int nx, ny;
+  uint8_t* lbl;  // level 0 aliases the engine's label array
+  float* x;
+  float* b;
+};
+
+// pressureMultigrid.ts:65-78 with a configurable coarsest side.
+int level_count(int nx, int ny, int coarsest) {
+  int n = 1;
+  while (nx > coarsest || ny > coarsest) {
+    nx = (nx + 1) / 2;
+    ny = (ny + 1) / 2;
+    n++;
+  }
+  return n;
+}
+
+}  // namespace
+
+struct fh_engine {
+  int nx, ny, ds, dnx, dny;
+  double h;
+  hipStream_t s = nullptr;
+  // fields
+  float *u = nullptr, *v = nullptr, *u2 = nullptr, *v2 = nullptr, *uh = nullptr, *vh = nullptr;
+  float *dye = nullptr, *dye2 = nullptr, *dyeh = nullptr, *w = nullptr;
+  uint8_t* lbl = nullptr;
+  double *p = nullptr, *p2 = nullptr, *div = nullptr;
+  // multigrid: levels for coarsest side 4 (reference); the defect-correction
+  // hierarchy (coarsest side 16) is a prefix of it.
+  std::vector<Level> lv;
+  int nlev_ref = 0, nlev_dc = 0;
+  double* M = nullptr;  // coarsest-level pseudo-inverse, n*n
+  int* cells = nullptr;
+  int ncoarse = 0;
+  bool labels_dirty = true;
+  // reductions
+  double* partials = nullptr;
+  unsigned* counter = nullptr;
+  double* sums = nullptr;
+  float* fpartials = nullptr;
+  float* fmax = nullptr;
+  double* h_sums = nullptr;  // pinned
+  float* h_fmax = nullptr;   // pinned
+  // patch / forces
+  float* patch = nullptr;
+  int pi0 = 0, pj0 = 0, pnx = 0, pny = 0, has_patch = 0;
+  double buoyancy = 0, confinement = 0;
+  StirBox stir{};
+  // state
+  bool busy = false, lost = false;
+  std::vector<double> hist;
+  bool timing = false;
+  hipEvent_t ev[FH_NUM_PHASES + 1] = {};
+  bool ev_valid = false;
+  std::vector<float> stage;
+  // per-kernel tracing
+  int trace_kid = 0;
+  std::vector<hipEvent_t> trace_ev;
+  int trace_n = 0;
+};
+
+namespace {
+
+int fail_hip(fh_engine* e, hipError_t err, const char* what) {
+  char buf[256];
+  snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(err));
+  set_err(buf);
+  if (e && (err == hipErrorIllegalAddress || err == hipErrorLaunchFailure ||
+            err == hipErrorECCNotCorrectable || err == hipErrorAssert))
+    e->lost = true;
+  return e && e->lost ? FH_ERR_DEVICE_LOST : FH_ERR_HIP;
+}
+
+#define HIPCHK(call)                                    \
+  do {                                                  \
+    hipError_t _e = (call);                             \
+    if (_e != hipSuccess) return fail_hip(e, _e, #call); \
+  } while (0)
+
+// Reentrancy / device-loss guard around every engine entry point.
+struct Guard {
+  fh_engine* e;
+  int status = FH_OK;
+  Guard(fh_engine* eng, const char* what) : e(eng) {
+    if (!e) {
+      set_err(std::string(what) + ": null engine");
+      status = FH_ERR_BAD_ARG;
+    } else if (e->lost) {
+      set_err("GPU device lost");
+      status = FH_ERR_DEVICE_LOST;
+    } else if (e->busy) {
+      set_err(std::string(what) + " is not re-entrant");
+      status = FH_ERR_BUSY;
+    } else {
+      e->busy = true;
+    }
+  }
+  ~Guard() {
+    if (status == FH_OK) e->busy = false;
+  }
+};
+
+#define GUARD(name)            \
+  Guard _g(e, name);           \
+  if (_g.status != FH_OK) return _g.status
+
+int check_launch(fh_engine* e, const char* what) {
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return fail_hip(e, err, what);
+  return FH_OK;
+}
+
+#define LAUNCHED(what)                       \
+  do {                                       \
+    int _st = check_launch(e, what);         \
+    if (_st != FH_OK) return _st;            \
+  } while (0)
+
+size_t n_u(const fh_engine* e) { return (size_t)(e->nx + 1) * e->ny; }
+size_t n_v(const fh_engine* e) { return (size_t)e->nx * (e->ny + 1); }
+size_t n_c(const fh_engine* e) { return (size_t)e->nx * e->ny; }
+size_t n_d(const fh_engine* e) { return (size_t)e->dnx * e->dny; }
+
+constexpr int kTraceMax = 8192;
+
+// Brackets one launch of the traced kernel with a pair of events.
+struct Trace {
+  fh_engine* e;
+  bool on;
+  Trace(fh_engine* eng, int kid) : e(eng), on(eng->trace_kid == kid && eng->trace_n < kTraceMax) {
+    if (!on) return;
+    size_t need = 2 * (size_t)(e->trace_n + 1);
+    while (e->trace_ev.size() < need) {
+      hipEvent_t ev;
+      if (hipEventCreate(&ev) != hipSuccess) {
+        on = false;
+        return;
+      }
+      e->trace_ev.push_back(ev);
+    }
+    (void)hipEventRecord(e->trace_ev[2 * e->trace_n], e->s);
+  }
+  ~Trace() {
+    if (!on) return;
+    (void)hipEventRecord(e->trace_ev[2 * e->trace_n + 1], e->s);
+    e->trace_n++;
+  }
+};
+
+
+template <class T>
+int dalloc(fh_engine* e, T** p, size_t n) {
+  hipError_t err = hipMalloc((void**)p, n * sizeof(T) + 256);
+  if (err != hipSuccess) return fail_hip(e, err, "hipMalloc");
+  err = hipMemset(*p, 0, n * sizeof(T) + 256);
+  if (err != hipSuccess) return fail_hip(e, err, "hipMemset");
+  return FH_OK;
+}
+
+// f64 host -> f32 device
+int up32(fh_engine* e, float* dst, const double* src, size_t n) {
+  e->stage.resize(n);
+  for (size_t k = 0; k < n; k++) e->stage[k] = (float)src[k];
+  HIPCHK(hipMemcpyAsync(dst, e->stage.data(), n * sizeof(float), hipMemcpyHostToDevice, e->s));
+  HIPCHK(hipStreamSynchronize(e->s));
+  return FH_OK;
+}
+
+int down32(fh_engine* e, double* dst, const float* src, size_t n) {
+  e->stage.resize(n);
+  HIPCHK(hipMemcpyAsync(e->stage.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, e->s));
+  HIPCHK(hipStreamSynchronize(e->s));
+  for (size_t k = 0; k < n; k++) dst[k] = (double)e->stage[k];
+  return FH_OK;
+}
+
+template <class T>
+void swap_ptr(T*& a, T*& b) {
+  T* t = a;
+  a = b;
+  b = t;
+}
+
+// ---- coarsest-level pseudo-inverse (host, FP64) ---------------------------
+// Unknowns: fluid cells with at least one non-solid neighbour.  A fluid
+// component that touches no Air is pure Neumann; it is regularised by
+// (1/n) 1 1^T and the component mean is projected out of the result, so
+// M b is the minimum-norm least-squares solution A^+ b on every component.
+void build_pseudo_inverse(int nx, int ny, const std::vector<uint8_t>& lab, std::vector<double>& M,
+                          std::vector<int>& cells) {
+  auto so = [&](int i, int j) {
+    return i < 0 || j < 0 || i >= nx || j >= ny || lab[i + j * nx] == SOLID;
+  };
+  const int di[4] = {1, -1, 0, 0}, dj[4] = {0, 0, 1, -1};
+  std::vector<int> map(nx * ny, -1);
+  cells.clear();
+  for (int k = 0; k < nx * ny; k++) {
+    int i = k % nx, j = k / nx;
+    if (lab[k] != FLUID) continue;
+    int cnt = 0;
+    for (int t = 0; t < 4; t++) cnt += !so(i + di[t], j + dj[t]);
+    if (cnt > 0) {
+      map[k] = (int)cells.size();
+      cells.push_back(k);
+    }
+  }
+  int n = (int)cells.size();
+  std::vector<int> comp(n, -1), stack;
+  std::vector<int> neumann, size;
+  for (int a = 0; a < n; a++) {
+    if (comp[a] >= 0) continue;
+    int c = (int)neumann.size();
+    int anchored = 0, sz = 0;
+    stack.assign(1, a);
+    comp[a] = c;
+    while (!stack.empty()) {
+      int b = stack.back();
+      stack.pop_back();
+      sz++;
+      int k = cells[b], i = k % nx, j = k / nx;
+      for (int t = 0; t < 4; t++) {
+        int ii = i + di[t], jj = j + dj[t];
+        if (ii < 0 || jj < 0 || ii >= nx || jj >= ny) continue;
+        int kk = ii + jj * nx;
+        if (lab[kk] == AIR) anchored = 1;
+        int m = map[kk];
+        if (m >= 0 && comp[m] < 0) {
+          comp[m] = c;
+          stack.push_back(m);
+        }
+      }
+    }
+    neumann.push_back(!anchored);
+    size.push_back(sz);
+  }
+  std::vector<double> A((size_t)n * n, 0.0);
+  for (int a = 0; a < n; a++) {
+    int k = cells[a], i = k % nx, j = k / nx;
+    for (int t = 0; t < 4; t++) {
+      int ii = i + di[t], jj = j + dj[t];
+      if (so(ii, jj)) continue;
+      A[(size_t)a * n + a] += 1.0;
+      int m = map[ii + jj * nx];
+      if (m >= 0) A[(size_t)a * n + m] -= 1.0;
+    }
+  }
+  for (int a = 0; a < n; a++)
+    for (int b = 0; b < n; b++)
+      if (comp[a] == comp[b] && neumann[comp[a]]) A[(size_t)a * n + b] += 1.0 / size[comp[a]];
+  // Cholesky A = L L^T (lower triangle in place)
+  for (int c = 0; c < n; c++) {
+    double sdiag = A[(size_t)c * n + c];
+    for (int t = 0; t < c; t++) sdiag -= A[(size_t)c * n + t] * A[(size_t)c * n + t];
+    double d = std::sqrt(sdiag);
+    A[(size_t)c * n + c] = d;
+    for (int r = c + 1; r < n; r++) {
+      double q = A[(size_t)r * n + c];
+      for (int t = 0; t < c; t++) q -= A[(size_t)r * n + t] * A[(size_t)c * n + t];
+      A[(size_t)r * n + c] = q / d;
+    }
+  }
+  // inverse column by column
+  M.assign((size_t)n * n, 0.0);
+  std::vector<double> y(n);
+  for (int col = 0; col < n; col++) {
+    for (int r = 0; r < n; r++) {
+      double sacc = (r == col) ? 1.0 : 0.0;
+      for (int t = 0; t < r; t++) sacc -= A[(size_t)r * n + t] * y[t];
+      y[r] = sacc / A[(size_t)r * n + r];
+    }
+    for (int r = n - 1; r >= 0; r--) {
+      double sacc = y[r];
+      for (int t = r + 1; t < n; t++) sacc -= A[(size_t)t * n + r] * y[t];
+      y[r] = sacc / A[(size_t)r * n + r];
+    }
+    for (int r = 0; r < n; r++) M[(size_t)r * n + col] = y[r];
+  }
+  // project out component means on Neumann components
+  int ncomp = (int)neumann.size();
+  std::vector<double> mean(ncomp);
+  for (int col = 0; col < n; col++) {
+    std::fill(mean.begin(), mean.end(), 0.0);
+    for (int a = 0; a < n; a++)
+      if (neumann[comp[a]]) mean[comp[a]] += M[(size_t)a * n + col];
+    for (int a = 0; a < n; a++)
+      if (neumann[comp[a]]) M[(size_t)a * n + col] -= mean[comp[a]] / size[comp[a]];
+  }
+}
+
+// Coarsen the label chain and rebuild the coarsest-level operator; zero
+// every multigrid unknown so no stale value from an old geometry survives.
+int refresh_levels(fh_engine* e) {
+  for (size_t l = 1; l < e->lv.size(); l++) {
+    launch_coarsen_labels(e->s, e->lv[l - 1].lbl, e->lv[l - 1].nx, e->lv[l - 1].ny, e->lv[l].lbl,
+                          e->lv[l].nx, e->lv[l].ny);
+    LAUNCHED("coarsen_labels");
+  }
+  for (auto& L : e->lv) {
+    HIPCHK(hipMemsetAsync(L.x, 0, (size_t)L.nx * L.ny * sizeof(float), e->s));
+  }
+  Level& C = e->lv[e->nlev_dc - 1];
+  std::vector<uint8_t> lab((size_t)C.nx * C.ny);
+  HIPCHK(hipMemcpyAsync(lab.data(), C.lbl, lab.size(), hipMemcpyDeviceToHost, e->s));
+  HIPCHK(hipStreamSynchronize(e->s));
+  std::vector<double> M;
+  std::vector<int> cells;
+  build_pseudo_inverse(C.nx, C.ny, lab, M, cells);
+  int n = (int)cells.size();
+  if (n > 1024) {
+    set_err("coarsest level too large for the direct solve");
+    return FH_ERR_BAD_ARG;
+  }
+  if (e->M) HIPCHK(hipFree(e->M));
+  if (e->cells) HIPCHK(hipFree(e->cells));
+  e->M = nullptr;
+  e->cells = nullptr;
+  e->ncoarse = n;
+  if (n > 0) {
+    HIPCHK(hipMalloc((void**)&e->M, M.size() * sizeof(double)));
+    HIPCHK(hipMalloc((void**)&e->cells, cells.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(e->M, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->cells, cells.data(), cells.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  e->labels_dirty = false;
+  return FH_OK;
+}
+
+int read_sums(fh_engine* e) {
+  HIPCHK(hipMemcpyAsync(e->h_sums, e->sums, 4 * sizeof(double), hipMemcpyDeviceToHost, e->s));
+  HIPCHK(hipStreamSynchronize(e->s));
+  return FH_OK;
+}
+
+void rb(fh_engine* e, int l, int color, float omega, float scale, int first) {
+  Level& L = e->lv[l];
+  Trace t(e, l == 0 ? FH_K_RB_SWEEP_FINE : -1);
+  launch_rb_sweep(e->s, L.x, L.b, L.lbl, L.nx, L.ny, color, omega, scale, first);
+}
+
+// One V-cycle on levels [l, nlev) -- pressureMultigrid.ts:253-275.
+// `zero` says the level's unknown is zero on entry, so the first colour
+// half-sweep skips the reads and clears the other colour.
+int vcycle(fh_engine* e, int l, int nlev, bool dc, bool zero) {
+  Level& L = e->lv[l];
+  if (l == nlev - 1) {
+    if (dc) {
+      if (e->ncoarse > 0) launch_coarse_solve(e->s, L.b, L.x, e->M, e->cells, e->ncoarse);
+    } else {
+      for (int k = 0; k < 8; k++) {
+        rb(e, l, 0, 1.f, -1.f, zero && k == 0);
+        rb(e, l, 1, 1.f, -1.f, 0);
+      }
+    }
+    LAUNCHED("coarsest");
+    return FH_OK;
+  }
+  for (int k = 0; k < 2; k++) {
+    rb(e, l, 0, 1.f, -1.f, zero && k == 0);
+    rb(e, l, 1, 1.f, -1.f, 0);
+  }
+  Level& N = e->lv[l + 1];
+  {
+    Trace t(e, l == 0 ? FH_K_RESTRICT_FINE : -1);
+    launch_residual_restrict(e->s, L.x, L.b, L.lbl, L.nx, L.ny, N.b, N.x, N.nx, N.ny);
+  }
+  LAUNCHED("pre-smooth/restrict");
+  int st = vcycle(e, l + 1, nlev, dc, true);
+  if (st != FH_OK) return st;
+  launch_prolong(e->s, L.x, L.lbl, L.nx, L.ny, N.x, N.nx, N.ny);
+  for (int k = 0; k < 2; k++) {
+    rb(e, l, 0, 1.f, -1.f, 0);
+    rb(e, l, 1, 1.f, -1.f, 0);
+  }
+  LAUNCHED("prolong/post-smooth");
+  return FH_OK;
+}
+
+double rel_resid(const double* s, double scale, bool scaled_r) {
+  // sums = {sum r^2, n_r, sum div^2, n_fluid}; the defect-correction kernel
+  // reports r in code units (b - A p), the norms kernel in divergence units.
+  if (s[1] <= 0 || s[2] <= 0) return 0.0;
+  double rr = std::sqrt(s[0] / s[1]);
+  if (scaled_r) rr /= std::fabs(scale);
+  return rr / std::sqrt(s[2] / s[3]);
+}
+
+// PressureSolver.solve on the resident p (FP64), div (FP64), label.
+int solve(fh_engine* e, int solver, double scale, int iterations, double omega, double tol,
+          int* out_iters) {
+  int N = e->nx * e->ny;
+  if (e->labels_dirty) {
+    int st = refresh_levels(e);
+    if (st != FH_OK) return st;
+  }
+  e->hist.clear();
+  Level& L0 = e->lv[0];
+  if (solver == FH_SOLVER_MG_DC) {
+    // r = b - A p (no update) + norms; the zero-divergence guard of
+    // pressureMultigrid.ts:294-300 reads the same pass.
+    launch_dc_update(e->s, e->p, e->p2, nullptr, e->div, e->lbl, L0.b, e->nx, e->ny, scale,
+                     e->partials, e->counter, e->sums);
+    LAUNCHED("dc_update");
+    swap_ptr(e->p, e->p2);
+    int st = read_sums(e);
+    if (st != FH_OK) return st;
+    if (e->h_sums[2] == 0.0) {
+      HIPCHK(hipMemsetAsync(e->p, 0, N * sizeof(double), e->s));
+      *out_iters = 0;
+      return FH_OK;
+    }
+    double rel = rel_resid(e->h_sums, scale, true);
+    e->hist.push_back(rel);
+    int used = iterations;
+    for (int c = 0; c < iterations; c++) {
+      st = vcycle(e, 0, e->nlev_dc, true, true);
+      if (st != FH_OK) return st;
+      {
+        Trace t(e, FH_K_DC_UPDATE);
+        launch_dc_update(e->s, e->p, e->p2, L0.x, e->div, e->lbl, L0.b, e->nx, e->ny, scale,
+                         e->partials, e->counter, e->sums);
+      }
+      LAUNCHED("dc_update");
+      swap_ptr(e->p, e->p2);
+      if (tol > 0 || c == iterations - 1) {
+        st = read_sums(e);
+        if (st != FH_OK) return st;
+        rel = rel_resid(e->h_sums, scale, true);
+        e->hist.push_back(rel);
+        if (tol > 0 && rel <= tol) {
+          used = c + 1;
+          break;
+        }
+      }
+    }
+    *out_iters = used;
+    return FH_OK;
+  }
+  // FP32 solvers on a single-precision copy of p (level-0 unknown).
+  launch_norms(e->s, nullptr, e->p, e->div, e->lbl, e->nx, e->ny, scale, e->partials,
+               e->counter, e->sums);
+  LAUNCHED("norms");
+  int st = read_sums(e);
+  if (st != FH_OK) return st;
+  if (e->h_sums[2] == 0.0) {
+    HIPCHK(hipMemsetAsync(e->p, 0, N * sizeof(double), e->s));
+    *out_iters = 0;
+    return FH_OK;
+  }
+  double divrms = std::sqrt(e->h_sums[2] / e->h_sums[3]);
+  double target = tol > 0 ? tol * divrms : -1;
+  launch_f64_to_f32(e->s, e->p, L0.x, N);
+  int used = iterations;
+  if (solver == FH_SOLVER_RBSOR) {
+    launch_rhs32(e->s, e->div, L0.b, N, 1.0);
+    for (int k = 0; k < iterations; k++) {
+      rb(e, 0, 0, (float)omega, (float)scale, 0);
+      rb(e, 0, 1, (float)omega, (float)scale, 0);
+      LAUNCHED("rb_sweep");
+      if (target >= 0 && k % 12 == 11) {
+        launch_norms(e->s, L0.x, nullptr, e->div, e->lbl, e->nx, e->ny, scale, e->partials,
+                     e->counter, e->sums);
+        st = read_sums(e);
+        if (st != FH_OK) return st;
+        if (e->h_sums[1] > 0 && std::sqrt(e->h_sums[0] / e->h_sums[1]) <= target) {
+          used = k + 1;
+          break;
+        }
+      }
+    }
+  } else if (solver == FH_SOLVER_MG) {
+    launch_rhs32(e->s, e->div, L0.b, N, -scale);
+    for (int c = 0; c < iterations; c++) {
+      st = vcycle(e, 0, e->nlev_ref, false, false);
+      if (st != FH_OK) return st;
+      if (target >= 0) {
+        launch_norms(e->s, L0.x, nullptr, e->div, e->lbl, e->nx, e->ny, scale, e->partials,
+                     e->counter, e->sums);
+        st = read_sums(e);
+        if (st != FH_OK) return st;
+        if (e->h_sums[1] > 0 && std::sqrt(e->h_sums[0] / e->h_sums[1]) <= target) {
+          used = c + 1;
+          break;
+        }
+      }
+    }
+  } else {
+    set_err("unknown solver");
+    return FH_ERR_BAD_ARG;
+  }
+  launch_f32_to_f64(e->s, L0.x, e->p, N);
+  LAUNCHED("solve");
+  *out_iters = used;
+  return FH_OK;
+}
+
+int advect_velocity(fh_engine* e, int scheme, double dt) {
+  float c = (float)(dt / e->h);
+  if (scheme == FH_MACCORMACK) {
+    {
+      Trace t(e, FH_K_ADVECT_FACES);
+      launch_advect_faces(e->s, e->u, e->v, e->u, e->v, e->uh, e->vh, e->lbl, e->nx, e->ny, c);
+    }
+    Trace t(e, FH_K_MACCORMACK_FACES);
+    launch_maccormack_faces(e->s, e->u, e->v, e->uh, e->vh, e->u2, e->v2, e->lbl, e->nx, e->ny,
+                            c);
+  } else {
+    Trace t(e, FH_K_ADVECT_FACES);
+    launch_advect_faces(e->s, e->u, e->v, e->u, e->v, e->u2, e->v2, e->lbl, e->nx, e->ny, c);
+  }
+  LAUNCHED("advect_velocity");
+  swap_ptr(e->u, e->u2);
+  swap_ptr(e->v, e->v2);
+  return FH_OK;
+}
+
+int forces(fh_engine* e, double dt) {
+  float eps = (float)e->confinement;
+  float gain = (float)(e->buoyancy * dt);
+  if (eps != 0.f) launch_vorticity(e->s, e->u, e->v, e->w, e->nx, e->ny, (float)(1.0 / e->h));
+  if (e->stir.active || gain != 0.f || eps != 0.f)
+    launch_forces(e->s, e->u, e->v, e->lbl, e->dye, e->w, e->nx, e->ny, e->ds, (float)e->h,
+                  (float)dt, gain, eps, e->stir);
+  e->stir.active = 0;  // one frame's displacement (simulation.ts:69-72)
+  LAUNCHED("forces");
+  return FH_OK;
+}
+
+int project(fh_engine* e, double grad_scale) {
+  launch_subtract_gradient(e->s, e->u, e->v, e->p, e->lbl, e->nx, e->ny, grad_scale);
+  launch_outflow(e->s, e->u, e->v, e->lbl, e->nx, e->ny);
+  LAUNCHED("project");
+  return FH_OK;
+}
+
+int dye_phase(fh_engine* e, int scheme, double dt, const double keep[3]) {
+  float c = (float)(dt / e->h);
+  float k0 = keep ? (float)keep[0] : 1.f, k1 = keep ? (float)keep[1] : 1.f,
+        k2 = keep ? (float)keep[2] : 1.f;
+  if (scheme == FH_MACCORMACK) {
+    {
+      Trace t(e, FH_K_ADVECT_DYE);
+      launch_advect_dye(e->s, e->u, e->v, e->dye, e->dyeh, e->lbl, e->nx, e->ny, e->ds, c, 1.f,
+                        1.f, 1.f);
+    }
+    Trace t(e, FH_K_MACCORMACK_DYE);
+    launch_maccormack_dye(e->s, e->u, e->v, e->dye, e->dyeh, e->dye2, e->lbl, e->nx, e->ny, e->ds,
+                          c, k0, k1, k2);
+  } else {
+    Trace t(e, FH_K_ADVECT_DYE);
+    launch_advect_dye(e->s, e->u, e->v, e->dye, e->dye2, e->lbl, e->nx, e->ny, e->ds, c, k0, k1,
+                      k2);
+  }
+  LAUNCHED("advect_dye");
+  swap_ptr(e->dye, e->dye2);
+  if (e->has_patch) {
+    launch_dye_patch(e->s, e->dye, e->patch, e->dnx, e->dny, e->pi0, e->pj0, e->pnx, e->pny);
+    LAUNCHED("dye_patch");
+  }
+  return FH_OK;
+}
+
+int mark(fh_engine* e, int phase) {
+  if (e->timing) HIPCHK(hipEventRecord(e->ev[phase], e->s));
+  return FH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fh_last_error(void) { return g_err.c_str(); }
+
+int fh_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+fh_engine* fh_create(int nx, int ny, int dye_scale, double h) {
+  if (nx < 2 || ny < 2 || dye_scale < 1 || !(h > 0)) {
+    set_err("fh_create: need nx, ny >= 2, dye_scale >= 1, h > 0");
+    return nullptr;
+  }
+  if (fh_device_count() < 1) {
+    set_err("fh_create: no HIP device");
+    return nullptr;
+  }
+  fh_engine* e = new fh_engine();
+  e->nx = nx;
+  e->ny = ny;
+  e->ds = dye_scale;
+  e->dnx = nx * dye_scale;
+  e->dny = ny * dye_scale;
+  e->h = h;
+  auto bail = [&]() {
+    fh_destroy(e);
+    return (fh_engine*)nullptr;
+  };
+  if (hipStreamCreate(&e->s) != hipSuccess) {
+    set_err("fh_create: hipStreamCreate failed");
+    return bail();
+  }
+  size_t nu = n_u(e), nv = n_v(e), nc = n_c(e), nd = n_d(e);
+  if (dalloc(e, &e->u, nu) || dalloc(e, &e->v, nv) || dalloc(e, &e->u2, nu) ||
+      dalloc(e, &e->v2, nv) || dalloc(e, &e->uh, nu) || dalloc(e, &e->vh, nv) ||
+      dalloc(e, &e->dye, 3 * nd) || dalloc(e, &e->dye2, 3 * nd) || dalloc(e, &e->dyeh, 3 * nd) ||
+      dalloc(e, &e->w, nc) || dalloc(e, &e->lbl, nc) || dalloc(e, &e->p, nc) ||
+      dalloc(e, &e->p2, nc) || dalloc(e, &e->div, nc))
+    return bail();
+  e->nlev_ref = level_count(nx, ny, 4);
+  e->nlev_dc = level_count(nx, ny, 16);
+  int lnx = nx, lny = ny;
+  for (int l = 0; l < e->nlev_ref; l++) {
+    Level L{lnx, lny, nullptr, nullptr, nullptr};
+    size_t n = (size_t)lnx * lny;
+    if (l == 0)
+      L.lbl = e->lbl;
+    else if (dalloc(e, &L.lbl, n))
+      return bail();
+    if (dalloc(e, &L.x, n) || dalloc(e, &L.b, n)) return bail();
+    e->lv.push_back(L);
+    lnx = (lnx + 1) / 2;
+    lny = (lny + 1) / 2;
+  }
+  size_t nblk = (size_t)((nx + 63) / 64) * ((ny + 3) / 4);
+  if (dalloc(e, &e->partials, 4 * nblk) || dalloc(e, &e->counter, 1) || dalloc(e, &e->sums, 4) ||
+      dalloc(e, &e->fpartials, 1024) || dalloc(e, &e->fmax, 1))
+    return bail();
+  if (hipHostMalloc((void**)&e->h_sums, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&e->h_fmax, sizeof(float), hipHostMallocDefault) != hipSuccess) {
+    set_err("fh_create: hipHostMalloc failed");
+    return bail();
+  }
+  for (int k = 0; k <= FH_NUM_PHASES; k++)
+    if (hipEventCreate(&e->ev[k]) != hipSuccess) {
+      set_err("fh_create: hipEventCreate failed");
+      return bail();
+    }
+  return e;
+}
+
+void fh_destroy(fh_engine* e) {
+  if (!e) return;
+  if (e->s) (void)hipStreamSynchronize(e->s);
+  float* fbufs[] = {e->u, e->v, e->u2, e->v2, e->uh, e->vh, e->dye, e->dye2, e->dyeh, e->w,
+                    e->patch, e->fpartials, e->fmax};
+  for (float* p : fbufs)
+    if (p) (void)hipFree(p);
+  double* dbufs[] = {e->p, e->p2, e->div, e->M, e->partials, e->sums};
+  for (double* p : dbufs)
+    if (p) (void)hipFree(p);
+  if (e->lbl) (void)hipFree(e->lbl);
+  if (e->cells) (void)hipFree(e->cells);
+  if (e->counter) (void)hipFree(e->counter);
+  for (size_t l = 0; l < e->lv.size(); l++) {
+    if (l > 0 && e->lv[l].lbl) (void)hipFree(e->lv[l].lbl);
+    if (e->lv[l].x) (void)hipFree(e->lv[l].x);
+    if (e->lv[l].b) (void)hipFree(e->lv[l].b);
+  }
+  if (e->h_sums) (void)hipHostFree(e->h_sums);
+  if (e->h_fmax) (void)hipHostFree(e->h_fmax);
+  for (int k = 0; k <= FH_NUM_PHASES; k++)
+    if (e->ev[k]) (void)hipEventDestroy(e->ev[k]);
+  for (hipEvent_t ev : e->trace_ev) (void)hipEventDestroy(ev);
+  if (e->s) (void)hipStreamDestroy(e->s);
+  delete e;
+}
+
+int fh_upload(fh_engine* e, const double* u, const double* v, const double* p,
+              const uint8_t* label, const double* dye) {
+  GUARD("HipStepper.upload()");
+  int st;
+  if (u && (st = up32(e, e->u, u, n_u(e)))) return st;
+  if (v && (st = up32(e, e->v, v, n_v(e)))) return st;
+  if (dye && (st = up32(e, e->dye, dye, 3 * n_d(e)))) return st;
+  if (p) HIPCHK(hipMemcpy(e->p, p, n_c(e) * sizeof(double), hipMemcpyHostToDevice));
+  if (label) {
+    HIPCHK(hipMemcpy(e->lbl, label, n_c(e), hipMemcpyHostToDevice));
+    e->labels_dirty = true;
+  }
+  return FH_OK;
+}
+
+int fh_set_labels(fh_engine* e, const uint8_t* label) {
+  GUARD("HipStepper.setLabels()");
+  if (!label) {
+    set_err("setLabels: null label");
+    return FH_ERR_BAD_ARG;
+  }
+  HIPCHK(hipMemcpy(e->lbl, label, n_c(e), hipMemcpyHostToDevice));
+  launch_commit_labels(e->s, e->p, e->u, e->v, e->dye, e->lbl, e->nx, e->ny, (int)n_d(e));
+  LAUNCHED("commit_labels");
+  HIPCHK(hipStreamSynchronize(e->s));
+  e->labels_dirty = true;
+  return FH_OK;
+}
+
+int fh_set_dye_patch(fh_engine* e, int i0, int j0, int pnx, int pny, const float* data) {
+  GUARD("HipStepper.setDyePatch()");
+  if (e->patch) HIPCHK(hipFree(e->patch));
+  e->patch = nullptr;
+  e->has_patch = 0;
+  if (!data) return FH_OK;
+  if (pnx <= 0 || pny <= 0) {
+    set_err("setDyePatch: empty patch");
+    return FH_ERR_BAD_ARG;
+  }
+  size_t n = (size_t)4 * pnx * pny;
+  HIPCHK(hipMalloc((void**)&e->patch, n * sizeof(float)));
+  HIPCHK(hipMemcpy(e->patch, data, n * sizeof(float), hipMemcpyHostToDevice));
+  e->pi0 = i0;
+  e->pj0 = j0;
+  e->pnx = pnx;
+  e->pny = pny;
+  e->has_patch = 1;
+  return FH_OK;
+}
+
+int fh_set_buoyancy(fh_engine* e, double w) {
+  GUARD("HipStepper.setBuoyancy()");
+  e->buoyancy = w;
+  return FH_OK;
+}
+
+int fh_set_confinement(fh_engine* e, double eps) {
+  GUARD("HipStepper.setConfinement()");
+  e->confinement = eps;
+  return FH_OK;
+}
+
+int fh_set_stir(fh_engine* e, int active, double x, double y, double r, double dx, double dy) {
+  GUARD("HipStepper.setStir()");
+  StirBox& s = e->stir;
+  s.active = active ? 1 : 0;
+  if (!active) return FH_OK;
+  if (!(r > 0)) {
+    set_err("setStir: radius must be positive");
+    return FH_ERR_BAD_ARG;
+  }
+  // stir.ts:49-62 box, then cell units for the kernel
+  double h = e->h, reach = 3 * r;
+  int c0 = (int)std::floor((x - reach) / h), c1 = (int)std::ceil((x + reach) / h);
+  int r0 = (int)std::floor((y - reach) / h), r1 = (int)std::ceil((y + reach) / h);
+  auto cl = [](int a, int lo, int hi) { return a < lo ? lo : a > hi ? hi : a; };
+  int nx = e->nx, ny = e->ny;
+  s.ui0 = cl(c0, 1, nx - 1);
+  s.ui1 = cl(c1, 1, nx - 1);
+  s.uj0 = cl(r0, 0, ny - 1);
+  s.uj1 = cl(r1, 0, ny - 1);
+  s.vi0 = cl(c0, 0, nx - 1);
+  s.vi1 = cl(c1, 0, nx - 1);
+  s.vj0 = cl(r0, 1, ny - 1);
+  s.vj1 = cl(r1, 1, ny - 1);
+  s.x = (float)(x / h);
+  s.y = (float)(y / h);
+  s.r = (float)(r / h);
+  s.dx = (float)dx;
+  s.dy = (float)dy;
+  return FH_OK;
+}
+
+int fh_step(fh_engine* e, int scheme, double dt, double scale, double grad_scale,
+            const double keep[3], int solver, int iterations, double omega, double tol,
+            int* out_iters) {
+  GUARD("HipStepper.step()");
+  if (!(dt > 0) || !out_iters) {
+    set_err("step: dt must be positive and out_iters non-null");
+    return FH_ERR_BAD_ARG;
+  }
+  int st;
+  if ((st = mark(e, 0))) return st;
+  if ((st = advect_velocity(e, scheme, dt))) return st;
+  if ((st = mark(e, 1))) return st;
+  if ((st = forces(e, dt))) return st;
+  if ((st = mark(e, 2))) return st;
+  launch_divergence(e->s, e->u, e->v, e->div, e->nx, e->ny, 1.0 / e->h);
+  LAUNCHED("divergence");
+  if ((st = mark(e, 3))) return st;
+  if ((st = solve(e, solver, scale, iterations, omega, tol, out_iters))) return st;
+  if ((st = mark(e, 4))) return st;
+  if ((st = project(e, grad_scale))) return st;
+  if ((st = mark(e, 5))) return st;
+  if ((st = dye_phase(e, scheme, dt, keep))) return st;
+  if ((st = mark(e, 6))) return st;
+  e->ev_valid = e->timing;
+  return FH_OK;
+}
+
+int fh_read_velocity(fh_engine* e, double* u, double* v) {
+  GUARD("HipStepper.readVelocity()");
+  int st;
+  if (u && (st = down32(e, u, e->u, n_u(e)))) return st;
+  if (v && (st = down32(e, v, e->v, n_v(e)))) return st;
+  return FH_OK;
+}
+
+int fh_read_dye(fh_engine* e, double* dye) {
+  GUARD("HipStepper.readDye()");
+  if (!dye) return FH_ERR_BAD_ARG;
+  return down32(e, dye, e->dye, 3 * n_d(e));
+}
+
+int fh_read_pressure(fh_engine* e, double* p) {
+  GUARD("HipStepper.readPressure()");
+  if (!p) return FH_ERR_BAD_ARG;
+  HIPCHK(hipMemcpyAsync(p, e->p, n_c(e) * sizeof(double), hipMemcpyDeviceToHost, e->s));
+  HIPCHK(hipStreamSynchronize(e->s));
+  return FH_OK;
+}
+
+int fh_read_divergence(fh_engine* e, double* div) {
+  GUARD("HipStepper.readDivergence()");
+  if (!div) return FH_ERR_BAD_ARG;
+  HIPCHK(hipMemcpyAsync(div, e->div, n_c(e) * sizeof(double), hipMemcpyDeviceToHost, e->s));
+  HIPCHK(hipStreamSynchronize(e->s));
+  return FH_OK;
+}
+
+int fh_max_face_speed(fh_engine* e, double* out) {
+  GUARD("HipStepper.maxFaceSpeed()");
+  if (!out) return FH_ERR_BAD_ARG;
+  launch_max_speed(e->s, e->u, e->v, e->nx, e->ny, e->fpartials, e->counter, e->fmax);
+  LAUNCHED("max_speed");
+  HIPCHK(hipMemcpyAsync(e->h_fmax, e->fmax, sizeof(float), hipMemcpyDeviceToHost, e->s));
+  HIPCHK(hipStreamSynchronize(e->s));
+  *out = (double)*e->h_fmax;
+  return FH_OK;
+}
+
+int fh_solve_pressure(fh_engine* e, int solver, double* p, const double* div,
+                      const uint8_t* label, double scale, int iterations, double omega,
+                      double tol, int* out_iters) {
+  GUARD("HipPressureSolver.solve()");
+  if (!p || !div || !label || !out_iters) {
+    set_err("solve: null argument");
+    return FH_ERR_BAD_ARG;
+  }
+  size_t nc = n_c(e);
+  HIPCHK(hipMemcpy(e->p, p, nc * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(e->div, div, nc * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<uint8_t> cur(nc);
+  HIPCHK(hipMemcpy(cur.data(), e->lbl, nc, hipMemcpyDeviceToHost));
+  if (memcmp(cur.data(), label, nc) != 0) {
+    HIPCHK(hipMemcpy(e->lbl, label, nc, hipMemcpyHostToDevice));
+    e->labels_dirty = true;
+  }
+  int st = solve(e, solver, scale, iterations, omega, tol, out_iters);
+  if (st != FH_OK) return st;
+  HIPCHK(hipMemcpyAsync(p, e->p, nc * sizeof(double), hipMemcpyDeviceToHost, e->s));
+  HIPCHK(hipStreamSynchronize(e->s));
+  return FH_OK;
+}
+
+int fh_advect_velocity(fh_engine* e, int scheme, const double* u_in, const double* v_in,
+                       double* u_out, double* v_out, const uint8_t* label, double dt) {
+  GUARD("HipAdvector.velocity()");
+  if (!u_in || !v_in || !u_out || !v_out || !label) return FH_ERR_BAD_ARG;
+  int st;
+  if ((st = up32(e, e->u, u_in, n_u(e))) || (st = up32(e, e->v, v_in, n_v(e)))) return st;
+  HIPCHK(hipMemcpy(e->lbl, label, n_c(e), hipMemcpyHostToDevice));
+  e->labels_dirty = true;
+  if ((st = advect_velocity(e, scheme, dt))) return st;
+  if ((st = down32(e, u_out, e->u, n_u(e))) || (st = down32(e, v_out, e->v, n_v(e)))) return st;
+  return FH_OK;
+}
+
+int fh_advect_dye(fh_engine* e, int scheme, const double* u, const double* v, const double* q_in,
+                  double* q_out, const uint8_t* label, double dt) {
+  GUARD("HipAdvector.dye()");
+  if (!u || !v || !q_in || !q_out || !label) return FH_ERR_BAD_ARG;
+  int st;
+  if ((st = up32(e, e->u, u, n_u(e))) || (st = up32(e, e->v, v, n_v(e))) ||
+      (st = up32(e, e->dye, q_in, 3 * n_d(e))))
+    return st;
+  HIPCHK(hipMemcpy(e->lbl, label, n_c(e), hipMemcpyHostToDevice));
+  e->labels_dirty = true;
+  int had_patch = e->has_patch;
+  e->has_patch = 0;
+  st = dye_phase(e, scheme, dt, nullptr);
+  e->has_patch = had_patch;
+  if (st) return st;
+  return down32(e, q_out, e->dye, 3 * n_d(e));
+}
+
+int fh_phase_advect_velocity(fh_engine* e, int scheme, double dt) {
+  GUARD("phase.advectVelocity()");
+  return advect_velocity(e, scheme, dt);
+}
+
+int fh_phase_forces(fh_engine* e, double dt) {
+  GUARD("phase.forces()");
+  return forces(e, dt);
+}
+
+int fh_phase_divergence(fh_engine* e) {
+  GUARD("phase.divergence()");
+  launch_divergence(e->s, e->u, e->v, e->div, e->nx, e->ny, 1.0 / e->h);
+  LAUNCHED("divergence");
+  return FH_OK;
+}
+
+int fh_phase_solve(fh_engine* e, int solver, double scale, int iterations, double omega,
+                   double tol, int* out_iters) {
+  GUARD("phase.solve()");
+  if (!out_iters) return FH_ERR_BAD_ARG;
+  return solve(e, solver, scale, iterations, omega, tol, out_iters);
+}
+
+int fh_phase_project(fh_engine* e, double grad_scale) {
+  GUARD("phase.project()");
+  return project(e, grad_scale);
+}
+
+int fh_phase_dye(fh_engine* e, int scheme, double dt, const double keep[3]) {
+  GUARD("phase.dye()");
+  return dye_phase(e, scheme, dt, keep);
+}
+
+int fh_solve_history(fh_engine* e, double* out, int max) {
+  if (!e || !out) return 0;
+  int n = (int)e->hist.size() < max ? (int)e->hist.size() : max;
+  for (int k = 0; k < n; k++) out[k] = e->hist[k];
+  return n;
+}
+
+int fh_set_timing(fh_engine* e, int on) {
+  GUARD("setTiming()");
+  e->timing = on != 0;
+  e->ev_valid = false;
+  return FH_OK;
+}
+
+int fh_get_timings(fh_engine* e, double* ms) {
+  GUARD("getTimings()");
+  if (!ms) return FH_ERR_BAD_ARG;
+  for (int k = 0; k < FH_NUM_PHASES; k++) ms[k] = 0;
+  if (!e->ev_valid) return FH_OK;
+  HIPCHK(hipEventSynchronize(e->ev[6]));
+  for (int k = 0; k < 6; k++) {
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e->ev[k], e->ev[k + 1]));
+    ms[k] = t;
+  }
+  float tot = 0;
+  HIPCHK(hipEventElapsedTime(&tot, e->ev[0], e->ev[6]));
+  ms[FH_PHASE_TOTAL] = tot;
+  return FH_OK;
+}
+
+int fh_trace_kernel(fh_engine* e, int kernel) {
+  GUARD("traceKernel()");
+  HIPCHK(hipStreamSynchronize(e->s));
+  e->trace_kid = kernel;
+  e->trace_n = 0;
+  return FH_OK;
+}
+
+int fh_trace_stats(fh_engine* e, int* count, double* total_ms) {
+  GUARD("traceStats()");
+  if (!count || !total_ms) return FH_ERR_BAD_ARG;
+  HIPCHK(hipStreamSynchronize(e->s));
+  double tot = 0;
+  for (int k = 0; k < e->trace_n; k++) {
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e->trace_ev[2 * k], e->trace_ev[2 * k + 1]));
+    tot += t;
+  }
+  *count = e->trace_n;
+  *total_ms = tot;
+  return FH_OK;
+}
+
+int fh_synchronize(fh_engine* e) {
+  GUARD("synchronize()");
+  HIPCHK(hipStreamSynchronize(e->s));
+  return FH_OK;
+}
+
+}  // extern "C"

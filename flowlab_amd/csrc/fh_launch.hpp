@@ -1,0 +1,90 @@
+// Host-side launchers of libflowhip's kernels (one translation unit each for
+// advection, projection/forces and the pressure solvers).  Every launcher
+// enqueues on `s` and returns immediately; none synchronises.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fh {
+
+struct StirBox {
+  int active;
+  float x, y, r, dx, dy;  // cell units (world / h)
+  int ui0, ui1, uj0, uj1;  // inclusive u-face box (stir.ts:398-405)
+  int vi0, vi1, vj0, vj1;  // inclusive v-face box (stir.ts:406-413)
+};
+
+// ---- advection (fh_advect.hip) -------------------------------------------
+// Semi-Lagrangian pass over every u and v face: out = src at the RK2
+// backtrace along (uc, vc) with c = dt/h.  advect.ts:72-112.
+void launch_advect_faces(hipStream_t s, const float* uc, const float* vc, const float* us,
+                         const float* vs, float* uo, float* vo, const uint8_t* lbl, int nx, int ny,
+                         float c);
+// MacCormack reverse pass fused with the correction and limiter.
+// advect.ts:177-210.
+void launch_maccormack_faces(hipStream_t s, const float* un, const float* vn, const float* uh,
+                             const float* vh, float* uo, float* vo, const uint8_t* lbl, int nx,
+                             int ny, float c);
+// Three dye channels on one backtrace, scaled by keep[c] at the write.
+// advect.ts:241-270 (+ the decay of simulation.ts:569-576).
+void launch_advect_dye(hipStream_t s, const float* u, const float* v, const float* q, float* qo,
+                       const uint8_t* lbl, int nx, int ny, int ds, float c, float k0, float k1,
+                       float k2);
+// advect.ts:291-311 for three channels, times keep[c].
+void launch_maccormack_dye(hipStream_t s, const float* u, const float* v, const float* qn,
+                           const float* qh, float* qo, const uint8_t* lbl, int nx, int ny, int ds,
+                           float c, float k0, float k1, float k2);
+
+// ---- forces, divergence, projection (fh_project.hip) ----------------------
+void launch_vorticity(hipStream_t s, const float* u, const float* v, float* w, int nx, int ny,
+                      float inv_h);
+void launch_forces(hipStream_t s, float* u, float* v, const uint8_t* lbl, const float* heat,
+                   const float* w, int nx, int ny, int ds, float h, float dt, float gain,
+                   float eps, StirBox stir);
+void launch_divergence(hipStream_t s, const float* u, const float* v, double* div, int nx, int ny,
+                       double inv_h);
+void launch_subtract_gradient(hipStream_t s, float* u, float* v, const double* p,
+                              const uint8_t* lbl, int nx, int ny, double grad_scale);
+void launch_outflow(hipStream_t s, float* u, float* v, const uint8_t* lbl, int nx, int ny);
+void launch_dye_patch(hipStream_t s, float* dye, const float* patch, int dnx, int dny, int i0,
+                      int j0, int pnx, int pny);
+void launch_commit_labels(hipStream_t s, double* p, float* u, float* v, float* dye,
+                          const uint8_t* lbl, int nx, int ny, int dye_plane);
+void launch_max_speed(hipStream_t s, const float* u, const float* v, int nx, int ny,
+                      float* partials, unsigned* counter, float* out);
+
+// ---- pressure (fh_mg.hip) --------------------------------------------------
+// One colour half-sweep: x = (1-w) x + w (sum - scale*b)/count on fluid cells
+// of colour (i+j)&1 == color.  pressureRedBlack.ts:32-72.  first != 0 treats
+// x as zero on entry and writes both colours (the first sweep of a V-cycle
+// on a zero correction).
+void launch_rb_sweep(hipStream_t s, float* x, const float* b, const uint8_t* lbl, int nx, int ny,
+                     int color, float omega, float scale, int first);
+// Coarse b = sum of child residuals, coarse x = 0.  pressureMultigrid.ts:117-178.
+void launch_residual_restrict(hipStream_t s, const float* x, const float* b, const uint8_t* lbl,
+                              int nx, int ny, float* bc, float* xc, int cnx, int cny);
+// x += bilinear(e) on fluid cells.  pressureMultigrid.ts:192-207.
+void launch_prolong(hipStream_t s, float* x, const uint8_t* lbl, int nx, int ny, const float* e,
+                    int cnx, int cny);
+void launch_coarsen_labels(hipStream_t s, const uint8_t* f, int fnx, int fny, uint8_t* c, int cnx,
+                           int cny);
+// x = M b on the coarsest level (FP64 dense apply of the pseudo-inverse).
+void launch_coarse_solve(hipStream_t s, const float* b, float* x, const double* M,
+                         const int* cells, int n);
+// out = (float)(factor * div)
+void launch_rhs32(hipStream_t s, const double* div, float* out, int n, double factor);
+void launch_f64_to_f32(hipStream_t s, const double* a, float* out, int n);
+void launch_f32_to_f64(hipStream_t s, const float* a, double* out, int n);
+// Defect-correction update (FP64): pn = po + e;  r = b - A pn with
+// b = -scale*div;  r32 = (float) r;  sums[0..3] = {sum r^2 (fluid, count>0),
+// n of those, sum div^2 (fluid), n fluid}.  e may be null (no update).
+void launch_dc_update(hipStream_t s, const double* po, double* pn, const float* e,
+                      const double* div, const uint8_t* lbl, float* r32, int nx, int ny,
+                      double scale, double* partials, unsigned* counter, double* sums);
+// Norms of the current FP32 or FP64 pressure: sums as in launch_dc_update,
+// r computed as div + (count p - sum)/scale (pressure.ts:102-142).
+void launch_norms(hipStream_t s, const float* p32, const double* p64, const double* div,
+                  const uint8_t* lbl, int nx, int ny, double scale, double* partials,
+                  unsigned* counter, double* sums);
+
+}  // namespace fh

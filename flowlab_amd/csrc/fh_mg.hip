@@ -1,0 +1,302 @@
+// Pressure solvers: red-black SOR sweeps, the geometric-multigrid transfer
+// operators and the FP64 defect-correction update with its residual norm
+// (src/core/pressureRedBlack.ts, pressureMultigrid.ts, pressure.ts:102-161).
+//
+// Every level solves  count*x - sum(x over non-solid neighbours) = b  on
+// fluid cells, with Air neighbours contributing x = 0 and off-grid/Solid
+// neighbours absent (pressureMultigrid.ts:22-42).
+#include "fh_device.hpp"
+#include "fh_launch.hpp"
+
+namespace fh {
+
+constexpr int BX = 64, BY = 4;
+
+static inline dim3 grid2(int w, int h) { return dim3((w + BX - 1) / BX, (h + BY - 1) / BY); }
+
+// pressureRedBlack.ts:32-72.  Thread t of row j owns the cell pair
+// (2t, 2t+1) and updates the one of this colour, so every lane is busy and a
+// wavefront covers 128 contiguous cells.  Neighbour order +x, -x, +y, -y.
+__global__ __launch_bounds__(256) void k_rb_sweep(float* __restrict__ x,
+                                                  const float* __restrict__ b,
+                                                  const uint8_t* __restrict__ lbl, int nx, int ny,
+                                                  int color, float omega, float scale, int first) {
+  int t = blockIdx.x * BX + threadIdx.x;
+  int j = blockIdx.y * BY + threadIdx.y;
+  if (j >= ny) return;
+  int i = 2 * t + ((j + color) & 1);
+  if (first) {
+    int io = 2 * t + ((j + color + 1) & 1);
+    if (io < nx && lbl[io + j * nx] == FLUID) x[io + j * nx] = 0.f;
+  }
+  if (i >= nx) return;
+  int k = i + j * nx;
+  if (lbl[k] != FLUID) return;
+  int cnt = 0;
+  float sum = 0.f;
+  if (!solid_or_outside(lbl, nx, ny, i + 1, j)) {
+    cnt++;
+    if (!first) sum += x[k + 1];
+  }
+  if (!solid_or_outside(lbl, nx, ny, i - 1, j)) {
+    cnt++;
+    if (!first) sum += x[k - 1];
+  }
+  if (!solid_or_outside(lbl, nx, ny, i, j + 1)) {
+    cnt++;
+    if (!first) sum += x[k + nx];
+  }
+  if (!solid_or_outside(lbl, nx, ny, i, j - 1)) {
+    cnt++;
+    if (!first) sum += x[k - nx];
+  }
+  if (cnt == 0) return;
+  float pgs = (sum - scale * b[k]) / (float)cnt;
+  float old = first ? 0.f : x[k];
+  x[k] = (1.f - omega) * old + omega * pgs;
+}
+
+// pressureMultigrid.ts:117-151 at one cell; 0 off the fluid.
+__device__ __forceinline__ float residual_at(const float* __restrict__ x,
+                                             const float* __restrict__ b,
+                                             const uint8_t* __restrict__ lbl, int nx, int ny,
+                                             int i, int j) {
+  int k = i + j * nx;
+  if (lbl[k] != FLUID) return 0.f;
+  int cnt = 0;
+  float sum = 0.f;
+  if (!solid_or_outside(lbl, nx, ny, i + 1, j)) { cnt++; sum += x[k + 1]; }
+  if (!solid_or_outside(lbl, nx, ny, i - 1, j)) { cnt++; sum += x[k - 1]; }
+  if (!solid_or_outside(lbl, nx, ny, i, j + 1)) { cnt++; sum += x[k + nx]; }
+  if (!solid_or_outside(lbl, nx, ny, i, j - 1)) { cnt++; sum += x[k - nx]; }
+  if (cnt == 0) return 0.f;
+  return b[k] - ((float)cnt * x[k] - sum);
+}
+
+// pressureMultigrid.ts:117-178 fused: the residual is never stored, each
+// coarse cell sums its (up to four) children's residuals directly.
+__global__ __launch_bounds__(256) void k_residual_restrict(
+    const float* __restrict__ x, const float* __restrict__ b, const uint8_t* __restrict__ lbl,
+    int nx, int ny, float* __restrict__ bc, float* __restrict__ xc, int cnx, int cny) {
+  int I = blockIdx.x * BX + threadIdx.x;
+  int J = blockIdx.y * BY + threadIdx.y;
+  if (I >= cnx || J >= cny) return;
+  float s = 0.f;
+#pragma unroll
+  for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+    for (int di = 0; di < 2; di++) {
+      int i = 2 * I + di, j = 2 * J + dj;
+      if (i < nx && j < ny) s += residual_at(x, b, lbl, nx, ny, i, j);
+    }
+  bc[I + J * cnx] = s;
+  xc[I + J * cnx] = 0.f;
+}
+
+// pressureMultigrid.ts:192-207
+__global__ __launch_bounds__(256) void k_prolong(float* __restrict__ x,
+                                                 const uint8_t* __restrict__ lbl, int nx, int ny,
+                                                 const float* __restrict__ e, int cnx, int cny) {
+  int i = blockIdx.x * BX + threadIdx.x;
+  int j = blockIdx.y * BY + threadIdx.y;
+  if (i >= nx || j >= ny) return;
+  int k = i + j * nx;
+  if (lbl[k] != FLUID) return;
+  int J = j >> 1, I = i >> 1;
+  int J2 = (j & 1) ? min(J + 1, cny - 1) : max(J - 1, 0);
+  int I2 = (i & 1) ? min(I + 1, cnx - 1) : max(I - 1, 0);
+  x[k] += 0.5625f * e[I + J * cnx] + 0.1875f * (e[I2 + J * cnx] + e[I + J2 * cnx]) +
+          0.0625f * e[I2 + J2 * cnx];
+}
+
+// pressureMultigrid.ts:92-110: Air > Fluid > Solid
+__global__ __launch_bounds__(256) void k_coarsen_labels(const uint8_t* __restrict__ f, int fnx,
+                                                        int fny, uint8_t* __restrict__ c, int cnx,
+                                                        int cny) {
+  int I = blockIdx.x * BX + threadIdx.x;
+  int J = blockIdx.y * BY + threadIdx.y;
+  if (I >= cnx || J >= cny) return;
+  bool air = false, fluid = false;
+  for (int dj = 0; dj < 2; dj++)
+    for (int di = 0; di < 2; di++) {
+      int i = 2 * I + di, j = 2 * J + dj;
+      if (i >= fnx || j >= fny) continue;
+      uint8_t l = f[i + j * fnx];
+      air = air || l == AIR;
+      fluid = fluid || l == FLUID;
+    }
+  c[I + J * cnx] = air ? AIR : fluid ? FLUID : SOLID;
+}
+
+// Coarsest level: x[cells[a]] = sum_b M[a][b] b[cells[b]] in FP64, one
+// workgroup; the right-hand side is staged in LDS.
+__global__ __launch_bounds__(256) void k_coarse_solve(const float* __restrict__ bc,
+                                                      float* __restrict__ xc,
+                                                      const double* __restrict__ M,
+                                                      const int* __restrict__ cells, int n) {
+  __shared__ double rhs[1024];
+  for (int a = threadIdx.x; a < n; a += blockDim.x) rhs[a] = (double)bc[cells[a]];
+  __syncthreads();
+  for (int a = threadIdx.x; a < n; a += blockDim.x) {
+    const double* row = M + (size_t)a * n;
+    double s = 0.0;
+    for (int q = 0; q < n; q++) s += row[q] * rhs[q];
+    xc[cells[a]] = (float)s;
+  }
+}
+
+__global__ void k_rhs32(const double* __restrict__ div, float* __restrict__ out, long n,
+                        double factor) {
+  long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (k < n) out[k] = (float)(factor * div[k]);
+}
+
+__global__ void k_f64_to_f32(const double* __restrict__ a, float* __restrict__ out, long n) {
+  long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (k < n) out[k] = (float)a[k];
+}
+
+__global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ out, long n) {
+  long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (k < n) out[k] = (double)a[k];
+}
+
+// FP64 defect-correction update, one thread per cell:
+//   pn = po + e  (e = 0 off the fluid),  r = b - (count*pn - sum pn),
+//   b = -scale*div,  r32 = (float) r  (the next V-cycle's right-hand side).
+// Reads po and e at the 5-point stencil; pn goes to a second buffer so no
+// thread reads a neighbour that another thread already advanced.
+__global__ __launch_bounds__(256) void k_dc_update(
+    const double* __restrict__ po, double* __restrict__ pn, const float* __restrict__ e,
+    const double* __restrict__ div, const uint8_t* __restrict__ lbl, float* __restrict__ r32,
+    int nx, int ny, double scale, double* partials, unsigned* counter, double* sums) {
+  int i = blockIdx.x * BX + threadIdx.x;
+  int j = blockIdx.y * BY + threadIdx.y;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < nx && j < ny) {
+    int k = i + j * nx;
+    double pk = po[k] + (e ? (double)e[k] : 0.0);
+    pn[k] = pk;
+    double r = 0.0;
+    if (lbl[k] == FLUID) {
+      double d = div[k];
+      acc[2] = d * d;
+      acc[3] = 1.0;
+      int cnt = 0;
+      double sum = 0.0;
+      if (!solid_or_outside(lbl, nx, ny, i + 1, j)) {
+        cnt++;
+        sum += po[k + 1] + (e ? (double)e[k + 1] : 0.0);
+      }
+      if (!solid_or_outside(lbl, nx, ny, i - 1, j)) {
+        cnt++;
+        sum += po[k - 1] + (e ? (double)e[k - 1] : 0.0);
+      }
+      if (!solid_or_outside(lbl, nx, ny, i, j + 1)) {
+        cnt++;
+        sum += po[k + nx] + (e ? (double)e[k + nx] : 0.0);
+      }
+      if (!solid_or_outside(lbl, nx, ny, i, j - 1)) {
+        cnt++;
+        sum += po[k - nx] + (e ? (double)e[k - nx] : 0.0);
+      }
+      if (cnt > 0) {
+        r = -scale * d - ((double)cnt * pk - sum);
+        acc[0] = r * r;
+        acc[1] = 1.0;
+      }
+    }
+    r32[k] = (float)r;
+  }
+  grid_sum<4>(acc, partials, counter, sums);
+}
+
+// pressure.ts:102-161 norms of a pressure held in FP32 (p32) or FP64 (p64):
+// sums = {sum rr^2, n, sum div^2, n_fluid}, rr = div + (count p - sum)/scale.
+// Neighbour order +x, +y, -x, -y as in rmsRemainingDivergence.
+__global__ __launch_bounds__(256) void k_norms(const float* __restrict__ p32,
+                                               const double* __restrict__ p64,
+                                               const double* __restrict__ div,
+                                               const uint8_t* __restrict__ lbl, int nx, int ny,
+                                               double scale, double* partials, unsigned* counter,
+                                               double* sums) {
+  int i = blockIdx.x * BX + threadIdx.x;
+  int j = blockIdx.y * BY + threadIdx.y;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (i < nx && j < ny) {
+    int k = i + j * nx;
+    if (lbl[k] == FLUID) {
+      double d = div[k];
+      acc[2] = d * d;
+      acc[3] = 1.0;
+      auto P = [&](int q) { return p32 ? (double)p32[q] : p64[q]; };
+      int cnt = 0;
+      double sum = 0.0;
+      if (!solid_or_outside(lbl, nx, ny, i + 1, j)) { cnt++; sum += P(k + 1); }
+      if (!solid_or_outside(lbl, nx, ny, i, j + 1)) { cnt++; sum += P(k + nx); }
+      if (!solid_or_outside(lbl, nx, ny, i - 1, j)) { cnt++; sum += P(k - 1); }
+      if (!solid_or_outside(lbl, nx, ny, i, j - 1)) { cnt++; sum += P(k - nx); }
+      if (cnt > 0) {
+        double rr = d + ((double)cnt * P(k) - sum) / scale;
+        acc[0] = rr * rr;
+        acc[1] = 1.0;
+      }
+    }
+  }
+  grid_sum<4>(acc, partials, counter, sums);
+}
+
+void launch_rb_sweep(hipStream_t s, float* x, const float* b, const uint8_t* lbl, int nx, int ny,
+                     int color, float omega, float scale, int first) {
+  k_rb_sweep<<<grid2((nx + 1) / 2, ny), dim3(BX, BY), 0, s>>>(x, b, lbl, nx, ny, color, omega,
+                                                              scale, first);
+}
+
+void launch_residual_restrict(hipStream_t s, const float* x, const float* b, const uint8_t* lbl,
+                              int nx, int ny, float* bc, float* xc, int cnx, int cny) {
+  k_residual_restrict<<<grid2(cnx, cny), dim3(BX, BY), 0, s>>>(x, b, lbl, nx, ny, bc, xc, cnx,
+                                                               cny);
+}
+
+void launch_prolong(hipStream_t s, float* x, const uint8_t* lbl, int nx, int ny, const float* e,
+                    int cnx, int cny) {
+  k_prolong<<<grid2(nx, ny), dim3(BX, BY), 0, s>>>(x, lbl, nx, ny, e, cnx, cny);
+}
+
+void launch_coarsen_labels(hipStream_t s, const uint8_t* f, int fnx, int fny, uint8_t* c, int cnx,
+                           int cny) {
+  k_coarsen_labels<<<grid2(cnx, cny), dim3(BX, BY), 0, s>>>(f, fnx, fny, c, cnx, cny);
+}
+
+void launch_coarse_solve(hipStream_t s, const float* b, float* x, const double* M,
+                         const int* cells, int n) {
+  k_coarse_solve<<<1, 256, 0, s>>>(b, x, M, cells, n);
+}
+
+void launch_rhs32(hipStream_t s, const double* div, float* out, int n, double factor) {
+  k_rhs32<<<(n + 255) / 256, 256, 0, s>>>(div, out, n, factor);
+}
+
+void launch_f64_to_f32(hipStream_t s, const double* a, float* out, int n) {
+  k_f64_to_f32<<<(n + 255) / 256, 256, 0, s>>>(a, out, n);
+}
+
+void launch_f32_to_f64(hipStream_t s, const float* a, double* out, int n) {
+  k_f32_to_f64<<<(n + 255) / 256, 256, 0, s>>>(a, out, n);
+}
+
+void launch_dc_update(hipStream_t s, const double* po, double* pn, const float* e,
+                      const double* div, const uint8_t* lbl, float* r32, int nx, int ny,
+                      double scale, double* partials, unsigned* counter, double* sums) {
+  k_dc_update<<<grid2(nx, ny), dim3(BX, BY), 0, s>>>(po, pn, e, div, lbl, r32, nx, ny, scale,
+                                                     partials, counter, sums);
+}
+
+void launch_norms(hipStream_t s, const float* p32, const double* p64, const double* div,
+                  const uint8_t* lbl, int nx, int ny, double scale, double* partials,
+                  unsigned* counter, double* sums) {
+  k_norms<<<grid2(nx, ny), dim3(BX, BY), 0, s>>>(p32, p64, div, lbl, nx, ny, scale, partials,
+                                                 counter, sums);
+}
+
+}  // namespace fh

@@ -1,0 +1,275 @@
+"""The reference's three engine seams, backed by libflowhip on the GPU.
+
+  HipStepper         implements GpuStep        (src/core/simulation.ts:35-84)
+  HipPressureSolver  implements PressureSolver (src/core/pressureSolver.ts:21-41)
+  HipAdvector        implements Advector       (src/core/advector.ts:28-63)
+
+Each owns one fh_engine (one grid on one HIP stream).  Arguments mean what
+they mean in the reference; errors are raised with the reference's messages
+(a re-entrant call -> "<Class>.<method>() is not re-entrant", a lost device
+-> "GPU device lost").  Host fields are numpy float64 arrays in the
+reference's layouts, labels uint8.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import hip
+from .hip import check, dptr, fptr, u8ptr
+
+
+class HipEngine:
+    """One device-resident MAC grid (nx x ny, cell size h, dye refined by
+    dye_scale).  The low-level handle every seam below wraps."""
+
+    def __init__(self, nx: int, ny: int, dye_scale: int = 1, h: float | None = None):
+        self.lib = hip.lib()
+        self.nx, self.ny, self.ds = nx, ny, dye_scale
+        self.h = 1.0 / ny if h is None else h
+        ptr = self.lib.fh_create(nx, ny, dye_scale, self.h)
+        if not ptr:
+            raise hip.HipError(hip.FH_ERR_NO_DEVICE, hip.last_error())
+        self._e = C.c_void_p(ptr)
+        self.dnx, self.dny = nx * dye_scale, ny * dye_scale
+
+    def close(self):
+        if getattr(self, "_e", None):
+            self.lib.fh_destroy(self._e)
+            self._e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # sizes
+    @property
+    def n_u(self):
+        return (self.nx + 1) * self.ny
+
+    @property
+    def n_v(self):
+        return self.nx * (self.ny + 1)
+
+    @property
+    def n_c(self):
+        return self.nx * self.ny
+
+    @property
+    def n_d(self):
+        return self.dnx * self.dny
+
+    # ---- state ----------------------------------------------------------
+    def upload(self, u=None, v=None, p=None, label=None, dye=None):
+        check(self.lib.fh_upload(self._e, dptr(u), dptr(v), dptr(p), u8ptr(label), dptr(dye)))
+
+    def set_labels(self, label: np.ndarray):
+        check(self.lib.fh_set_labels(self._e, u8ptr(label)))
+
+    def set_dye_patch(self, patch):
+        if patch is None:
+            check(self.lib.fh_set_dye_patch(self._e, 0, 0, 0, 0, None))
+        else:
+            data = np.ascontiguousarray(patch.data, np.float32)
+            check(self.lib.fh_set_dye_patch(self._e, patch.i0, patch.j0, patch.nx, patch.ny,
+                                            fptr(data)))
+
+    def set_buoyancy(self, w: float):
+        check(self.lib.fh_set_buoyancy(self._e, float(w)))
+
+    def set_confinement(self, eps: float):
+        check(self.lib.fh_set_confinement(self._e, float(eps)))
+
+    def set_stir(self, stir):
+        if stir is None:
+            check(self.lib.fh_set_stir(self._e, 0, 0, 0, 0, 0, 0))
+        else:
+            x, y, r, dx, dy = stir
+            check(self.lib.fh_set_stir(self._e, 1, x, y, r, dx, dy))
+
+    def read_velocity(self, u=None, v=None):
+        u = np.empty(self.n_u) if u is None else u
+        v = np.empty(self.n_v) if v is None else v
+        check(self.lib.fh_read_velocity(self._e, dptr(u), dptr(v)))
+        return u, v
+
+    def read_dye(self, dye=None):
+        dye = np.empty(3 * self.n_d) if dye is None else dye
+        check(self.lib.fh_read_dye(self._e, dptr(dye)))
+        return dye
+
+    def read_pressure(self):
+        p = np.empty(self.n_c)
+        check(self.lib.fh_read_pressure(self._e, dptr(p)))
+        return p
+
+    def read_divergence(self):
+        d = np.empty(self.n_c)
+        check(self.lib.fh_read_divergence(self._e, dptr(d)))
+        return d
+
+    def max_face_speed(self) -> float:
+        out = C.c_double()
+        check(self.lib.fh_max_face_speed(self._e, C.byref(out)))
+        return out.value
+
+    # ---- the fused step and its phases -----------------------------------
+    def step(self, scheme, dt, scale, grad_scale, keep=(1.0, 1.0, 1.0), solver="mg-dc",
+             iterations=100, omega=1.0, tol=1e-6) -> int:
+        k = (C.c_double * 3)(*keep)
+        it = C.c_int()
+        check(self.lib.fh_step(self._e, hip.SCHEMES[scheme], dt, scale, grad_scale,
+                               C.cast(k, hip._dp), hip.SOLVERS[solver], iterations, omega, tol,
+                               C.byref(it)))
+        return it.value
+
+    def phase_advect_velocity(self, scheme, dt):
+        check(self.lib.fh_phase_advect_velocity(self._e, hip.SCHEMES[scheme], dt))
+
+    def phase_forces(self, dt):
+        check(self.lib.fh_phase_forces(self._e, dt))
+
+    def phase_divergence(self):
+        check(self.lib.fh_phase_divergence(self._e))
+
+    def phase_solve(self, solver, scale, iterations, omega=1.0, tol=0.0) -> int:
+        it = C.c_int()
+        check(self.lib.fh_phase_solve(self._e, hip.SOLVERS[solver], scale, iterations, omega, tol,
+                                      C.byref(it)))
+        return it.value
+
+    def phase_project(self, grad_scale):
+        check(self.lib.fh_phase_project(self._e, grad_scale))
+
+    def phase_dye(self, scheme, dt, keep=(1.0, 1.0, 1.0)):
+        k = (C.c_double * 3)(*keep)
+        check(self.lib.fh_phase_dye(self._e, hip.SCHEMES[scheme], dt, C.cast(k, hip._dp)))
+
+    def solve_history(self, max_entries=512) -> np.ndarray:
+        out = np.zeros(max_entries)
+        n = self.lib.fh_solve_history(self._e, dptr(out), max_entries)
+        return out[:n]
+
+    def set_timing(self, on: bool):
+        check(self.lib.fh_set_timing(self._e, 1 if on else 0))
+
+    def timings(self) -> dict:
+        ms = np.zeros(len(hip.PHASES))
+        check(self.lib.fh_get_timings(self._e, dptr(ms)))
+        return dict(zip(hip.PHASES, ms.tolist()))
+
+    def synchronize(self):
+        check(self.lib.fh_synchronize(self._e))
+
+    def trace_kernel(self, name: str | None):
+        check(self.lib.fh_trace_kernel(self._e, 0 if name is None else hip.KERNELS[name]))
+
+    def trace_stats(self):
+        n, ms = C.c_int(), C.c_double()
+        check(self.lib.fh_trace_stats(self._e, C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+
+class HipStepper:
+    """GpuStep (simulation.ts:35-84) on libflowhip: the whole advect ->
+    project -> dye stretch as one device pass with u, v, p and dye resident.
+
+    step() returns the solver's iteration count; on return f.u and f.v hold
+    the step's velocity (read back, as the fused WebGPU step does,
+    stepGpu.ts:546-604) and f.p is NOT updated; the dye stays resident and
+    is read with read_dye (simulation.ts:29-33, 75-83)."""
+
+    name = "hip-fused"
+
+    def __init__(self, nx, ny, dye_scale=1, h=None, solver="mg-dc", readback=True):
+        self.engine = HipEngine(nx, ny, dye_scale, h)
+        self.solver = solver
+        self.readback = readback
+        self._dirty = True
+
+    def step(self, f, scheme, dt, scale, grad_scale, dye_keep, perf=None, iterations=100,
+             omega=1.0, tol=1e-6) -> int:
+        e = self.engine
+        if self._dirty:
+            e.upload(f.u, f.v, f.p, f.label, np.concatenate(f.dye))
+            self._dirty = False
+        iters = e.step(scheme, dt, scale, grad_scale, dye_keep, self.solver, iterations, omega,
+                       tol)
+        if self.readback:
+            e.read_velocity(f.u, f.v)
+        if perf is not None:
+            perf.record(e)
+        return iters
+
+    def invalidate(self):
+        self._dirty = True
+
+    def set_dye_patch(self, patch):
+        self.engine.set_dye_patch(patch)
+
+    def set_buoyancy(self, w):
+        self.engine.set_buoyancy(w)
+
+    def set_confinement(self, eps):
+        self.engine.set_confinement(eps)
+
+    def set_stir(self, stir):
+        self.engine.set_stir(None if stir is None else
+                             (stir.x, stir.y, stir.r, stir.dx, stir.dy) if hasattr(stir, "x")
+                             else stir)
+
+    def set_labels(self, label):
+        self.engine.set_labels(label)
+
+    def read_dye(self, dye):
+        buf = self.engine.read_dye()
+        n = self.engine.n_d
+        for c in range(3):
+            dye[c][:] = buf[c * n:(c + 1) * n]
+
+    def max_face_speed(self) -> float:
+        return self.engine.max_face_speed()
+
+
+class HipPressureSolver:
+    """PressureSolver (pressureSolver.ts:21-41): solve() fills p (warm start
+    in, answer out) and returns sweeps or V-cycles used."""
+
+    def __init__(self, nx, ny, method="mg-dc", h=None):
+        self.engine = HipEngine(nx, ny, 1, h)
+        self.method = method
+        self.name = {"rbsor": "hip-rbsor", "mg": "hip-mg", "mg-dc": "hip-mg-dc"}[method]
+
+    def solve(self, g, p, div, label, scale, iterations, omega, tol) -> int:
+        it = C.c_int()
+        check(self.engine.lib.fh_solve_pressure(self.engine._e, hip.SOLVERS[self.method],
+                                                dptr(p), dptr(div), u8ptr(label), scale,
+                                                int(iterations), omega, tol, C.byref(it)))
+        return it.value
+
+
+class HipAdvector:
+    """Advector (advector.ts:28-63).  velocity() runs before dye() within a
+    step and labels do not change between them (advector.ts:21-24)."""
+
+    name = "hip"
+
+    def __init__(self, nx, ny, dye_scale=1, h=None):
+        self.engine = HipEngine(nx, ny, dye_scale, h)
+
+    def velocity(self, g, scheme, u_in, v_in, u_out, v_out, label, dt):
+        check(self.engine.lib.fh_advect_velocity(self.engine._e, hip.SCHEMES[scheme], dptr(u_in),
+                                                 dptr(v_in), dptr(u_out), dptr(v_out),
+                                                 u8ptr(label), dt))
+
+    def dye(self, g, scheme, u, v, q_in, q_out, label, dt, dg=None):
+        n = self.engine.n_d
+        qi = np.concatenate([np.asarray(q, np.float64) for q in q_in])
+        qo = np.empty(3 * n)
+        check(self.engine.lib.fh_advect_dye(self.engine._e, hip.SCHEMES[scheme], dptr(u), dptr(v),
+                                            dptr(qi), dptr(qo), u8ptr(label), dt))
+        for c in range(len(q_out)):
+            q_out[c][:] = qo[c * n:(c + 1) * n]

@@ -1,0 +1,138 @@
+"""ctypes binding of libflowhip.so (include/flowhip.h).
+
+Loads the in-tree library and fails loudly when it is missing: there is no
+CPU fallback anywhere in flowlab_amd.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libflowhip.so")
+
+FH_OK, FH_ERR_BAD_ARG, FH_ERR_BUSY, FH_ERR_DEVICE_LOST, FH_ERR_NO_DEVICE, FH_ERR_HIP = range(6)
+SEMI_LAGRANGIAN, MACCORMACK = 0, 1
+SOLVER_RBSOR, SOLVER_MG, SOLVER_MG_DC = 1, 2, 3
+PHASES = ("advect", "forces", "divergence", "solve", "project", "dye", "total")
+
+SCHEMES = {"semiLagrangian": SEMI_LAGRANGIAN, "macCormack": MACCORMACK}
+SOLVERS = {"rbsor": SOLVER_RBSOR, "mg": SOLVER_MG, "mg-dc": SOLVER_MG_DC}
+
+# Every exported symbol of include/flowhip.h: (name, restype, argtypes).
+_dp = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+_i, _d = C.c_int, C.c_double
+
+SIGNATURES = [
+    ("fh_last_error", C.c_char_p, []),
+    ("fh_device_count", _i, []),
+    ("fh_create", _vp, [_i, _i, _i, _d]),
+    ("fh_destroy", None, [_vp]),
+    ("fh_upload", _i, [_vp, _dp, _dp, _dp, _u8p, _dp]),
+    ("fh_set_labels", _i, [_vp, _u8p]),
+    ("fh_set_dye_patch", _i, [_vp, _i, _i, _i, _i, _fp]),
+    ("fh_set_buoyancy", _i, [_vp, _d]),
+    ("fh_set_confinement", _i, [_vp, _d]),
+    ("fh_set_stir", _i, [_vp, _i, _d, _d, _d, _d, _d]),
+    ("fh_step", _i, [_vp, _i, _d, _d, _d, _dp, _i, _i, _d, _d, _ip]),
+    ("fh_read_velocity", _i, [_vp, _dp, _dp]),
+    ("fh_read_dye", _i, [_vp, _dp]),
+    ("fh_read_pressure", _i, [_vp, _dp]),
+    ("fh_read_divergence", _i, [_vp, _dp]),
+    ("fh_max_face_speed", _i, [_vp, _dp]),
+    ("fh_solve_pressure", _i, [_vp, _i, _dp, _dp, _u8p, _d, _i, _d, _d, _ip]),
+    ("fh_advect_velocity", _i, [_vp, _i, _dp, _dp, _dp, _dp, _u8p, _d]),
+    ("fh_advect_dye", _i, [_vp, _i, _dp, _dp, _dp, _dp, _u8p, _d]),
+    ("fh_phase_advect_velocity", _i, [_vp, _i, _d]),
+    ("fh_phase_forces", _i, [_vp, _d]),
+    ("fh_phase_divergence", _i, [_vp]),
+    ("fh_phase_solve", _i, [_vp, _i, _d, _i, _d, _d, _ip]),
+    ("fh_phase_project", _i, [_vp, _d]),
+    ("fh_phase_dye", _i, [_vp, _i, _d, _dp]),
+    ("fh_solve_history", _i, [_vp, _dp, _i]),
+    ("fh_set_timing", _i, [_vp, _i]),
+    ("fh_get_timings", _i, [_vp, _dp]),
+    ("fh_synchronize", _i, [_vp]),
+    ("fh_trace_kernel", _i, [_vp, _i]),
+    ("fh_trace_stats", _i, [_vp, _ip, _dp]),
+]
+
+KERNELS = {"rb_sweep_fine": 1, "advect_faces": 2, "maccormack_faces": 3, "advect_dye": 4,
+           "maccormack_dye": 5, "dc_update": 6, "restrict_fine": 7}
+
+
+class HipError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+
+
+def load() -> C.CDLL:
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`"
+            " (flowlab_amd has no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_lib: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        _lib = load()
+    return _lib
+
+
+def last_error() -> str:
+    return (lib().fh_last_error() or b"").decode()
+
+
+def check(status: int) -> None:
+    """Status -> exception, with the reference's messages: a re-entrant call
+    says '<Class>.<method>() is not re-entrant' and a lost device says
+    'GPU device lost' (src/gpu/stepGpu.ts:402-403)."""
+    if status == FH_OK:
+        return
+    msg = last_error()
+    if status == FH_ERR_DEVICE_LOST:
+        msg = "GPU device lost"
+    raise HipError(status, msg)
+
+
+def f64(a: np.ndarray) -> np.ndarray:
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+        raise TypeError("expected a C-contiguous float64 array")
+    return a
+
+
+def dptr(a):
+    return None if a is None else f64(a).ctypes.data_as(_dp)
+
+
+def u8ptr(a):
+    if a is None:
+        return None
+    if not (a.dtype == np.uint8 and a.flags.c_contiguous):
+        raise TypeError("expected a C-contiguous uint8 label array")
+    return a.ctypes.data_as(_u8p)
+
+
+def fptr(a):
+    if a is None:
+        return None
+    if not (a.dtype == np.float32 and a.flags.c_contiguous):
+        raise TypeError("expected a C-contiguous float32 array")
+    return a.ctypes.data_as(_fp)

@@ -1,0 +1,171 @@
+/*
+ * libflowhip -- C ABI of the MI355X (gfx950) projection-method step.
+ *
+ * This is the drop-in boundary: plain pointers and sizes, no framework
+ * types.  A host binding (the N-API addon under flowlab_amd/node/, the
+ * Python ctypes layer in flowlab_amd/hip.py) wraps it into the reference's
+ * three engine seams:
+ *
+ *   GpuStep        src/core/simulation.ts:35-84     -> fh_step, fh_upload,
+ *                  fh_set_dye_patch, fh_set_buoyancy, fh_set_stir,
+ *                  fh_set_labels, fh_read_dye (+ fh_read_velocity, which the
+ *                  fused path's readback performs, src/gpu/stepGpu.ts:546-604)
+ *   PressureSolver src/core/pressureSolver.ts:21-41 -> fh_solve_pressure
+ *   Advector       src/core/advector.ts:28-63       -> fh_advect_velocity,
+ *                  fh_advect_dye
+ *
+ * Host fields are the reference's Float64Array layouts (src/core/grid.ts:
+ * 84-141): p/label/div nx*ny, u (nx+1)*ny, v nx*(ny+1), dye 3 planes of
+ * (nx*s)*(ny*s) for dye scale s, all row-major with x fastest.  Labels are
+ * Fluid 0, Air 1, Solid 2 (grid.ts:41-45).  On the device velocities and dye
+ * are FP32; the pressure, its right-hand side and every norm are FP64.
+ *
+ * Every entry point returns an int status (FH_OK = 0) except fh_create
+ * (NULL on failure, as device.ts:22-45 returns null) and the accessors noted.
+ * fh_last_error() describes the last failure on the calling thread.
+ * Calls on one engine must not overlap: an overlapping call returns
+ * FH_ERR_BUSY (the reference throws "<Class>.<method>() is not re-entrant",
+ * src/gpu/stepGpu.ts:402).  After a device fault every call returns
+ * FH_ERR_DEVICE_LOST (src/gpu/stepGpu.ts:403).
+ */
+#ifndef FLOWHIP_H
+#define FLOWHIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fh_engine fh_engine;
+
+enum {
+  FH_OK = 0,
+  FH_ERR_BAD_ARG = 1,
+  FH_ERR_BUSY = 2,
+  FH_ERR_DEVICE_LOST = 3,
+  FH_ERR_NO_DEVICE = 4,
+  FH_ERR_HIP = 5
+};
+
+/* src/core/advect.ts:21 */
+enum { FH_SEMI_LAGRANGIAN = 0, FH_MACCORMACK = 1 };
+
+/* Pressure solvers.  FH_SOLVER_RBSOR: red-black SOR, FP32, sweep budget and
+ * tol checked every 12 sweeps (pressureRedBlack.ts:88-113).  FH_SOLVER_MG:
+ * the reference V(2,2) multigrid, FP32, coarsened to <= 4 cells a side with
+ * 8 red-black pairs on the coarsest level, `iterations` = cycle budget
+ * (pressureMultigrid.ts:219-320).  FH_SOLVER_MG_DC: mixed-precision defect
+ * correction -- FP64 pressure, residual and norms, FP32 V(2,2) on the
+ * correction, coarsened to <= 16 cells a side with an FP64 direct solve;
+ * `iterations` = cycle cap, `tol` = relative residual target. */
+enum { FH_SOLVER_RBSOR = 1, FH_SOLVER_MG = 2, FH_SOLVER_MG_DC = 3 };
+
+/* Step phases reported by fh_get_timings (milliseconds of device time). */
+enum {
+  FH_PHASE_ADVECT = 0,
+  FH_PHASE_FORCES = 1,
+  FH_PHASE_DIVERGENCE = 2,
+  FH_PHASE_SOLVE = 3,
+  FH_PHASE_PROJECT = 4,
+  FH_PHASE_DYE = 5,
+  FH_PHASE_TOTAL = 6,
+  FH_NUM_PHASES = 7
+};
+
+const char* fh_last_error(void);
+int fh_device_count(void);
+
+/* Lifecycle.  h is the cell size (simulation.ts:365 uses 1/ny); dye_scale
+ * is the whole-number dye refinement (simulation.ts:358-372).  Replaces the
+ * GpuStepper / GpuMultigridSolver / GpuAdvector constructors and their
+ * destroy() (src/gpu/stepGpu.ts:672-689). */
+fh_engine* fh_create(int nx, int ny, int dye_scale, double h);
+void fh_destroy(fh_engine* e);
+
+/* GpuStep.invalidate() + the re-upload it triggers (stepGpu.ts:419-433):
+ * host truth for every field.  Any pointer may be NULL to leave that field. */
+int fh_upload(fh_engine* e, const double* u, const double* v, const double* p,
+              const uint8_t* label, const double* dye);
+/* GpuStep.setLabels (simulation.ts:74) followed by the device-side
+ * commitLabels of boundaries.ts:45-64. */
+int fh_set_labels(fh_engine* e, const uint8_t* label);
+/* GpuStep.setDyePatch (simulation.ts:53): data = 3 value planes + coverage,
+ * each pnx*pny (dye.ts:30-36); data == NULL clears the patch. */
+int fh_set_dye_patch(fh_engine* e, int i0, int j0, int pnx, int pny, const float* data);
+/* GpuStep.setBuoyancy (simulation.ts:60). */
+int fh_set_buoyancy(fh_engine* e, double w);
+/* Vorticity-confinement epsilon (no reference counterpart; 0 = off). */
+int fh_set_confinement(fh_engine* e, double eps);
+/* GpuStep.setStir (simulation.ts:72): consumed by the next step. */
+int fh_set_stir(fh_engine* e, int active, double x, double y, double r, double dx, double dy);
+
+/* GpuStep.step (simulation.ts:36-46): advect -> forces -> divergence ->
+ * solve -> subtract + outflow -> dye advect (x keep) -> dye patch, on the
+ * device, one stream.  keep[c] = exp(-dyeDecay_c dt).  Returns the solver's
+ * iteration count (sweeps or V-cycles) in *out_iters. */
+int fh_step(fh_engine* e, int scheme, double dt, double scale, double grad_scale,
+            const double keep[3], int solver, int iterations, double omega, double tol,
+            int* out_iters);
+int fh_read_velocity(fh_engine* e, double* u, double* v);
+/* GpuStep.readDye (simulation.ts:83). */
+int fh_read_dye(fh_engine* e, double* dye);
+/* The device pressure (a diagnostic the reference lacks: f.p is not
+ * updated by its fused step, simulation.ts:29-33). */
+int fh_read_pressure(fh_engine* e, double* p);
+int fh_read_divergence(fh_engine* e, double* div);
+/* Simulation.maxFaceSpeed (simulation.ts:451-459) on the device fields. */
+int fh_max_face_speed(fh_engine* e, double* out);
+
+/* PressureSolver.solve (pressureSolver.ts:21-41): p in/out (warm start),
+ * div, label as host arrays.  Returns sweeps/V-cycles in *out_iters. */
+int fh_solve_pressure(fh_engine* e, int solver, double* p, const double* div,
+                      const uint8_t* label, double scale, int iterations, double omega,
+                      double tol, int* out_iters);
+
+/* Advector.velocity / Advector.dye (advector.ts:28-63) on host arrays.
+ * fh_advect_dye takes and returns all 3 channels (3 planes). */
+int fh_advect_velocity(fh_engine* e, int scheme, const double* u_in, const double* v_in,
+                       double* u_out, double* v_out, const uint8_t* label, double dt);
+int fh_advect_dye(fh_engine* e, int scheme, const double* u, const double* v,
+                  const double* q_in, double* q_out, const uint8_t* label, double dt);
+
+/* The fused step's phases on the resident fields, one call each, so every
+ * kernel can be diffed against the FP64 oracle from identical state. */
+int fh_phase_advect_velocity(fh_engine* e, int scheme, double dt);
+int fh_phase_forces(fh_engine* e, double dt);
+int fh_phase_divergence(fh_engine* e);
+int fh_phase_solve(fh_engine* e, int solver, double scale, int iterations, double omega,
+                   double tol, int* out_iters);
+int fh_phase_project(fh_engine* e, double grad_scale);
+int fh_phase_dye(fh_engine* e, int scheme, double dt, const double keep[3]);
+
+/* Relative residual (rmsRemainingDivergence / fluidDivRms, pressure.ts:
+ * 102-161) before and after every cycle of the last FH_SOLVER_MG_DC solve;
+ * returns the number of entries written (<= max). */
+int fh_solve_history(fh_engine* e, double* out, int max);
+
+/* Per-phase device milliseconds of the last fh_step when timing is on. */
+int fh_set_timing(fh_engine* e, int on);
+int fh_get_timings(fh_engine* e, double* ms /* FH_NUM_PHASES */);
+int fh_synchronize(fh_engine* e);
+
+/* Live per-kernel timing on the engine's stream: with a kernel selected,
+ * every launch of it is bracketed by HIP events (up to 8192 launches);
+ * fh_trace_stats returns the launch count and the summed device ms. */
+enum {
+  FH_K_NONE = 0,
+  FH_K_RB_SWEEP_FINE = 1,     /* red-black half-sweep on the finest level */
+  FH_K_ADVECT_FACES = 2,      /* semi-Lagrangian / MacCormack forward, faces */
+  FH_K_MACCORMACK_FACES = 3,  /* MacCormack reverse+correct, faces */
+  FH_K_ADVECT_DYE = 4,
+  FH_K_MACCORMACK_DYE = 5,
+  FH_K_DC_UPDATE = 6,         /* FP64 defect-correction update + norm */
+  FH_K_RESTRICT_FINE = 7      /* residual+restrict from the finest level */
+};
+int fh_trace_kernel(fh_engine* e, int kernel);
+int fh_trace_stats(fh_engine* e, int* count, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
