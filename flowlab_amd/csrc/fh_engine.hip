@@ -25,9 +25,14 @@ void set_err(const std::string& s) { g_err = s; }
 struct Level {
   int nx, ny;
   uint8_t* lbl;  // level 0 aliases the engine's label array
-  float* x;
+  float* x;      // the level's unknown after a cycle
   float* b;
+  float* xa;     // pre-smoothed unknown (tiled kernels read halos of x
+                 // while writing interiors, so they ping-pong x <-> xa)
 };
+
+// LDS one workgroup may use for the coarse sub-cycle.
+constexpr size_t kCoarseLds = 65536;
 
 // pressureMultigrid.ts:65-78 with a configurable coarsest side.
 int level_count(int nx, int ny, int coarsest) {
@@ -55,6 +60,7 @@ struct fh_engine {
   // hierarchy (coarsest side 16) is a prefix of it.
   std::vector<Level> lv;
   int nlev_ref = 0, nlev_dc = 0;
+  int lc_ref = 0, lc_dc = 0;  // first level run by the one-workgroup sub-cycle
   double* M = nullptr;  // coarsest-level pseudo-inverse, n*n
   int* cells = nullptr;
   int ncoarse = 0;
@@ -323,6 +329,7 @@ int refresh_levels(fh_engine* e) {
   }
   for (auto& L : e->lv) {
     HIPCHK(hipMemsetAsync(L.x, 0, (size_t)L.nx * L.ny * sizeof(float), e->s));
+    HIPCHK(hipMemsetAsync(L.xa, 0, (size_t)L.nx * L.ny * sizeof(float), e->s));
   }
   Level& C = e->lv[e->nlev_dc - 1];
   std::vector<uint8_t> lab((size_t)C.nx * C.ny);
@@ -363,41 +370,56 @@ void rb(fh_engine* e, int l, int color, float omega, float scale, int first) {
   launch_rb_sweep(e->s, L.x, L.b, L.lbl, L.nx, L.ny, color, omega, scale, first);
 }
 
-// One V-cycle on levels [l, nlev) -- pressureMultigrid.ts:253-275.
-// `zero` says the level's unknown is zero on entry, so the first colour
-// half-sweep skips the reads and clears the other colour.
-int vcycle(fh_engine* e, int l, int nlev, bool dc, bool zero) {
-  Level& L = e->lv[l];
-  if (l == nlev - 1) {
-    if (dc) {
-      if (e->ncoarse > 0) launch_coarse_solve(e->s, L.b, L.x, e->M, e->cells, e->ncoarse);
-    } else {
-      for (int k = 0; k < 8; k++) {
-        rb(e, l, 0, 1.f, -1.f, zero && k == 0);
-        rb(e, l, 1, 1.f, -1.f, 0);
-      }
-    }
-    LAUNCHED("coarsest");
-    return FH_OK;
+CoarseLevels coarse_levels(const fh_engine* e, int lc, int nlev) {
+  CoarseLevels C{};
+  C.n = nlev - lc;
+  for (int k = 0; k < C.n; k++) {
+    C.nx[k] = e->lv[lc + k].nx;
+    C.ny[k] = e->lv[lc + k].ny;
+    C.lbl[k] = e->lv[lc + k].lbl;
   }
-  for (int k = 0; k < 2; k++) {
-    rb(e, l, 0, 1.f, -1.f, zero && k == 0);
-    rb(e, l, 1, 1.f, -1.f, 0);
+  return C;
+}
+
+// First level from which the rest of the hierarchy fits one workgroup's LDS.
+int coarse_start(const fh_engine* e, int nlev) {
+  for (int l = 0; l < nlev; l++)
+    if (nlev - l <= kMaxCoarse && coarse_vcycle_lds_bytes(coarse_levels(e, l, nlev)) <= kCoarseLds)
+      return l;
+  return nlev - 1;
+}
+
+// One V(2,2) cycle (pressureMultigrid.ts:253-275) on levels [0, nlev):
+// levels below lc with the LDS-tiled pre-smooth+restrict and
+// prolong+post-smooth kernels, levels from lc in one workgroup.  warm: level
+// 0 starts from its stored x (the reference mode's pressure warm start);
+// otherwise every level's unknown starts at zero.
+int vcycle(fh_engine* e, int nlev, bool dc, bool warm) {
+  int lc = dc ? e->lc_dc : e->lc_ref;
+  for (int l = 0; l < lc; l++) {
+    Level& L = e->lv[l];
+    Level& N = e->lv[l + 1];
+    bool zero = !(l == 0 && warm);
+    Trace t(e, l == 0 ? FH_K_SMOOTH_RESTRICT_FINE : -1);
+    launch_smooth_restrict(e->s, zero ? nullptr : L.x, L.xa, L.b, L.lbl, L.nx, L.ny, N.b, N.nx,
+                           N.ny, zero ? 1 : 0);
   }
-  Level& N = e->lv[l + 1];
+  LAUNCHED("smooth_restrict");
   {
-    Trace t(e, l == 0 ? FH_K_RESTRICT_FINE : -1);
-    launch_residual_restrict(e->s, L.x, L.b, L.lbl, L.nx, L.ny, N.b, N.x, N.nx, N.ny);
+    Trace t(e, FH_K_COARSE_VCYCLE);
+    launch_coarse_vcycle(e->s, coarse_levels(e, lc, nlev), e->lv[lc].b,
+                         (lc == 0 && warm) ? e->lv[0].x : nullptr, e->lv[lc].x,
+                         dc ? e->M : nullptr, dc ? e->cells : nullptr, dc ? e->ncoarse : 0,
+                         dc ? 1 : 0);
   }
-  LAUNCHED("pre-smooth/restrict");
-  int st = vcycle(e, l + 1, nlev, dc, true);
-  if (st != FH_OK) return st;
-  launch_prolong(e->s, L.x, L.lbl, L.nx, L.ny, N.x, N.nx, N.ny);
-  for (int k = 0; k < 2; k++) {
-    rb(e, l, 0, 1.f, -1.f, 0);
-    rb(e, l, 1, 1.f, -1.f, 0);
+  LAUNCHED("coarse_vcycle");
+  for (int l = lc - 1; l >= 0; l--) {
+    Level& L = e->lv[l];
+    Level& N = e->lv[l + 1];
+    Trace t(e, l == 0 ? FH_K_PROLONG_SMOOTH_FINE : -1);
+    launch_prolong_smooth(e->s, L.xa, L.x, L.b, L.lbl, L.nx, L.ny, N.x, N.nx, N.ny);
   }
-  LAUNCHED("prolong/post-smooth");
+  LAUNCHED("prolong_smooth");
   return FH_OK;
 }
 
@@ -438,7 +460,7 @@ int solve(fh_engine* e, int solver, double scale, int iterations, double omega, 
     e->hist.push_back(rel);
     int used = iterations;
     for (int c = 0; c < iterations; c++) {
-      st = vcycle(e, 0, e->nlev_dc, true, true);
+      st = vcycle(e, e->nlev_dc, true, false);
       if (st != FH_OK) return st;
       {
         Trace t(e, FH_K_DC_UPDATE);
@@ -496,7 +518,7 @@ int solve(fh_engine* e, int solver, double scale, int iterations, double omega, 
   } else if (solver == FH_SOLVER_MG) {
     launch_rhs32(e->s, e->div, L0.b, N, -scale);
     for (int c = 0; c < iterations; c++) {
-      st = vcycle(e, 0, e->nlev_ref, false, false);
+      st = vcycle(e, e->nlev_ref, false, true);
       if (st != FH_OK) return st;
       if (target >= 0) {
         launch_norms(e->s, L0.x, nullptr, e->div, e->lbl, e->nx, e->ny, scale, e->partials,
@@ -637,17 +659,19 @@ fh_engine* fh_create(int nx, int ny, int dye_scale, double h) {
   e->nlev_dc = level_count(nx, ny, 16);
   int lnx = nx, lny = ny;
   for (int l = 0; l < e->nlev_ref; l++) {
-    Level L{lnx, lny, nullptr, nullptr, nullptr};
+    Level L{lnx, lny, nullptr, nullptr, nullptr, nullptr};
     size_t n = (size_t)lnx * lny;
     if (l == 0)
       L.lbl = e->lbl;
     else if (dalloc(e, &L.lbl, n))
       return bail();
-    if (dalloc(e, &L.x, n) || dalloc(e, &L.b, n)) return bail();
+    if (dalloc(e, &L.x, n) || dalloc(e, &L.b, n) || dalloc(e, &L.xa, n)) return bail();
     e->lv.push_back(L);
     lnx = (lnx + 1) / 2;
     lny = (lny + 1) / 2;
   }
+  e->lc_ref = coarse_start(e, e->nlev_ref);
+  e->lc_dc = coarse_start(e, e->nlev_dc);
   size_t nblk = (size_t)((nx + 63) / 64) * ((ny + 3) / 4);
   if (dalloc(e, &e->partials, 4 * nblk) || dalloc(e, &e->counter, 1) || dalloc(e, &e->sums, 4) ||
       dalloc(e, &e->fpartials, 1024) || dalloc(e, &e->fmax, 1))
@@ -682,6 +706,7 @@ void fh_destroy(fh_engine* e) {
     if (l > 0 && e->lv[l].lbl) (void)hipFree(e->lv[l].lbl);
     if (e->lv[l].x) (void)hipFree(e->lv[l].x);
     if (e->lv[l].b) (void)hipFree(e->lv[l].b);
+    if (e->lv[l].xa) (void)hipFree(e->lv[l].xa);
   }
   if (e->h_sums) (void)hipHostFree(e->h_sums);
   if (e->h_fmax) (void)hipHostFree(e->h_fmax);

@@ -87,4 +87,30 @@ void launch_norms(hipStream_t s, const float* p32, const double* p64, const doub
                   const uint8_t* lbl, int nx, int ny, double scale, double* partials,
                   unsigned* counter, double* sums);
 
+// ---- fused multigrid (fh_mgfused.hip) ----------------------------------------
+constexpr int kMaxCoarse = 12;
+struct CoarseLevels {
+  int n;
+  int nx[kMaxCoarse];
+  int ny[kMaxCoarse];
+  const uint8_t* lbl[kMaxCoarse];
+};
+// Pre-smoothing (2 red-black sweeps, omega 1) + residual + restriction of a
+// level to the next one's right-hand side bc.  zero: the unknown starts at 0
+// (xin unread).  xin and xout must differ.
+void launch_smooth_restrict(hipStream_t s, const float* xin, float* xout, const float* b,
+                            const uint8_t* lbl, int nx, int ny, float* bc, int cnx, int cny,
+                            int zero);
+// Prolongation of the coarse correction e and post-smoothing (2 sweeps).
+void launch_prolong_smooth(hipStream_t s, const float* xin, float* xout, const float* b,
+                           const uint8_t* lbl, int nx, int ny, const float* e, int cnx, int cny);
+// The sub-cycle of levels L[0..n) in one workgroup: b0 is the first level's
+// right-hand side, x0in its warm start (NULL = zero), x0out its result.
+// direct: FP64 dense solve on the last level, else 8 red-black pairs.
+void launch_coarse_vcycle(hipStream_t s, const CoarseLevels& L, const float* b0,
+                          const float* x0in, float* x0out, const double* M, const int* cells,
+                          int ncoarse, int direct);
+size_t coarse_vcycle_lds_bytes(const CoarseLevels& L);
+void launch_finalize_sum4(hipStream_t s, const double* partials, int nblocks, double* out);
+
 }  // namespace fh

@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void k_dc_update(
     }
     r32[k] = (float)r;
   }
-  grid_sum<4>(acc, partials, counter, sums);
+  block_sum_store<4>(acc, partials);
 }
 
 // pressure.ts:102-161 norms of a pressure held in FP32 (p32) or FP64 (p64):
@@ -243,7 +243,26 @@ __global__ __launch_bounds__(256) void k_norms(const float* __restrict__ p32,
       }
     }
   }
-  grid_sum<4>(acc, partials, counter, sums);
+  block_sum_store<4>(acc, partials);
+}
+
+// One block sums NS-slot partials in block order (deterministic).
+__global__ __launch_bounds__(256) void k_finalize_sum4(const double* __restrict__ partials,
+                                                       int nblocks, double* __restrict__ out) {
+  __shared__ double sh[4][4];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  for (int s = 0; s < 4; s++) {
+    double t = 0.0;
+    for (int b = tid; b < nblocks; b += 256) t += partials[(size_t)b * 4 + s];
+    t = wave_sum(t);
+    if (lane == 0) sh[s][wid] = t;
+  }
+  __syncthreads();
+  if (tid < 4) out[tid] = ((sh[tid][0] + sh[tid][1]) + sh[tid][2]) + sh[tid][3];
+}
+
+void launch_finalize_sum4(hipStream_t s, const double* partials, int nblocks, double* out) {
+  k_finalize_sum4<<<1, 256, 0, s>>>(partials, nblocks, out);
 }
 
 void launch_rb_sweep(hipStream_t s, float* x, const float* b, const uint8_t* lbl, int nx, int ny,
@@ -288,15 +307,18 @@ void launch_f32_to_f64(hipStream_t s, const float* a, double* out, int n) {
 void launch_dc_update(hipStream_t s, const double* po, double* pn, const float* e,
                       const double* div, const uint8_t* lbl, float* r32, int nx, int ny,
                       double scale, double* partials, unsigned* counter, double* sums) {
-  k_dc_update<<<grid2(nx, ny), dim3(BX, BY), 0, s>>>(po, pn, e, div, lbl, r32, nx, ny, scale,
-                                                     partials, counter, sums);
+  dim3 g = grid2(nx, ny);
+  k_dc_update<<<g, dim3(BX, BY), 0, s>>>(po, pn, e, div, lbl, r32, nx, ny, scale, partials,
+                                         counter, sums);
+  launch_finalize_sum4(s, partials, g.x * g.y, sums);
 }
 
 void launch_norms(hipStream_t s, const float* p32, const double* p64, const double* div,
                   const uint8_t* lbl, int nx, int ny, double scale, double* partials,
                   unsigned* counter, double* sums) {
-  k_norms<<<grid2(nx, ny), dim3(BX, BY), 0, s>>>(p32, p64, div, lbl, nx, ny, scale, partials,
-                                                 counter, sums);
+  dim3 g = grid2(nx, ny);
+  k_norms<<<g, dim3(BX, BY), 0, s>>>(p32, p64, div, lbl, nx, ny, scale, partials, counter, sums);
+  launch_finalize_sum4(s, partials, g.x * g.y, sums);
 }
 
 }  // namespace fh

@@ -232,7 +232,19 @@ __global__ __launch_bounds__(256) void k_max_speed(const float* __restrict__ u,
     m = fmaxf(m, fabsf(u[k]));
   for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < nv; k += stride)
     m = fmaxf(m, fabsf(v[k]));
-  grid_max(m, partials, counter, out);
+  block_max_store(m, partials);
+}
+
+__global__ __launch_bounds__(256) void k_finalize_max(const float* __restrict__ partials,
+                                                      int nblocks, float* __restrict__ out) {
+  __shared__ float sh[4];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  float t = 0.f;
+  for (int b = tid; b < nblocks; b += 256) t = fmaxf(t, partials[b]);
+  t = wave_max(t);
+  if (lane == 0) sh[wid] = t;
+  __syncthreads();
+  if (tid == 0) out[0] = fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
 }
 
 void launch_vorticity(hipStream_t s, const float* u, const float* v, float* w, int nx, int ny,
@@ -277,6 +289,7 @@ void launch_max_speed(hipStream_t s, const float* u, const float* v, int nx, int
                       float* partials, unsigned* counter, float* out) {
   k_max_speed<<<1024, 256, 0, s>>>(u, v, (long)(nx + 1) * ny, (long)nx * (ny + 1), partials,
                                    counter, out);
+  k_finalize_max<<<1, 256, 0, s>>>(partials, 1024, out);
 }
 
 }  // namespace fh

@@ -64,7 +64,8 @@ SIGNATURES = [
 ]
 
 KERNELS = {"rb_sweep_fine": 1, "advect_faces": 2, "maccormack_faces": 3, "advect_dye": 4,
-           "maccormack_dye": 5, "dc_update": 6, "restrict_fine": 7}
+           "maccormack_dye": 5, "dc_update": 6, "restrict_fine": 7,
+           "smooth_restrict_fine": 8, "prolong_smooth_fine": 9, "coarse_vcycle": 10}
 
 
 class HipError(RuntimeError):

@@ -160,7 +160,10 @@ enum {
   FH_K_ADVECT_DYE = 4,
   FH_K_MACCORMACK_DYE = 5,
   FH_K_DC_UPDATE = 6,         /* FP64 defect-correction update + norm */
-  FH_K_RESTRICT_FINE = 7      /* residual+restrict from the finest level */
+  FH_K_RESTRICT_FINE = 7,     /* residual+restrict from the finest level */
+  FH_K_SMOOTH_RESTRICT_FINE = 8, /* fused pre-smooth + restrict, finest level */
+  FH_K_PROLONG_SMOOTH_FINE = 9,  /* fused prolong + post-smooth, finest level */
+  FH_K_COARSE_VCYCLE = 10        /* one-workgroup sub-cycle of the coarse levels */
 };
 int fh_trace_kernel(fh_engine* e, int kernel);
 int fh_trace_stats(fh_engine* e, int* count, double* total_ms);
