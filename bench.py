@@ -35,7 +35,12 @@ HBM_PEAK_GBS = 8000.0
 #   maccormack_faces: 2 x (carrier u,v 8 + hat 4 + u^n 4 + out 4 + label 1).
 #   advect_faces: 2 x (carrier 8 + out 4 + label 1).
 #   dc_update: e 4 + p 8 read + 8 write + div 8 + label 1 + r32 4.
+#   smooth_restrict_fine: b 4 + label 1 + x 4 written + coarse b 1 (x starts
+#     at zero, so it is not read).
+#   prolong_smooth_fine: x 4 + b 4 + label 1 + coarse e 1 + x 4 written.
 ALGO_BYTES_PER_CELL = {
+    "smooth_restrict_fine": 10.0,
+    "prolong_smooth_fine": 14.0,
     "rb_sweep_fine": 6.5,
     "maccormack_faces": 42.0,
     "advect_faces": 26.0,

@@ -14,6 +14,15 @@
 #include "fh_device.hpp"
 #include "fh_launch.hpp"
 
+namespace fh {
+// fh_unet.hip
+int engine_capture_hook(fh_unet* u, hipStream_t s, const float* r32, const uint8_t* lbl,
+                        const double* p_prev, const double* p_final, long n, int H, int W,
+                        bool begin);
+int engine_warm_start(fh_unet* u, hipStream_t s, const float* r32, const uint8_t* lbl, double* p,
+                      int H, int W, float* stats2);
+}  // namespace fh
+
 using namespace fh;
 
 namespace {
@@ -89,6 +98,14 @@ struct fh_engine {
   int trace_kid = 0;
   std::vector<hipEvent_t> trace_ev;
   int trace_n = 0;
+  // learned warm start: training-sample capture and inference
+  fh_unet* capture = nullptr;
+  fh_unet* warm = nullptr;
+  double* psave = nullptr;
+  float* wstats = nullptr;
+  // device time of the last pressure solve
+  hipEvent_t solve_ev[2] = {};
+  bool solve_ev_valid = false;
 };
 
 namespace {
@@ -433,8 +450,28 @@ double rel_resid(const double* s, double scale, bool scaled_r) {
 }
 
 // PressureSolver.solve on the resident p (FP64), div (FP64), label.
+int solve_impl(fh_engine* e, int solver, double scale, int iterations, double omega,
+               double tol, int* out_iters);
+
+// Every solve is bracketed by two events: fh_last_solve_ms reports its
+// device time (label refresh, warm start and all cycles included).
 int solve(fh_engine* e, int solver, double scale, int iterations, double omega, double tol,
           int* out_iters) {
+  e->solve_ev_valid = false;
+  if (!e->solve_ev[0]) {
+    HIPCHK(hipEventCreate(&e->solve_ev[0]));
+    HIPCHK(hipEventCreate(&e->solve_ev[1]));
+  }
+  HIPCHK(hipEventRecord(e->solve_ev[0], e->s));
+  int st = solve_impl(e, solver, scale, iterations, omega, tol, out_iters);
+  if (st != FH_OK) return st;
+  HIPCHK(hipEventRecord(e->solve_ev[1], e->s));
+  e->solve_ev_valid = true;
+  return FH_OK;
+}
+
+int solve_impl(fh_engine* e, int solver, double scale, int iterations, double omega,
+               double tol, int* out_iters) {
   int N = e->nx * e->ny;
   if (e->labels_dirty) {
     int st = refresh_levels(e);
@@ -455,6 +492,23 @@ int solve(fh_engine* e, int solver, double scale, int iterations, double omega, 
       HIPCHK(hipMemsetAsync(e->p, 0, N * sizeof(double), e->s));
       *out_iters = 0;
       return FH_OK;
+    }
+    if (e->capture) {
+      HIPCHK(hipMemcpyAsync(e->psave, e->p, N * sizeof(double), hipMemcpyDeviceToDevice, e->s));
+      st = engine_capture_hook(e->capture, e->s, L0.b, e->lbl, e->psave, nullptr, N, e->ny,
+                               e->nx, true);
+      if (st != FH_OK) return st;
+    }
+    if (e->warm) {
+      // learned warm start: p += net correction, then the defect again
+      st = engine_warm_start(e->warm, e->s, L0.b, e->lbl, e->p, e->ny, e->nx, e->wstats);
+      if (st != FH_OK) return st;
+      launch_dc_update(e->s, e->p, e->p2, nullptr, e->div, e->lbl, L0.b, e->nx, e->ny, scale,
+                       e->partials, e->counter, e->sums);
+      LAUNCHED("dc_update");
+      swap_ptr(e->p, e->p2);
+      st = read_sums(e);
+      if (st != FH_OK) return st;
     }
     double rel = rel_resid(e->h_sums, scale, true);
     e->hist.push_back(rel);
@@ -481,6 +535,11 @@ int solve(fh_engine* e, int solver, double scale, int iterations, double omega, 
       }
     }
     *out_iters = used;
+    if (e->capture) {
+      st = engine_capture_hook(e->capture, e->s, nullptr, nullptr, e->psave, e->p, N, e->ny,
+                               e->nx, false);
+      if (st != FH_OK) return st;
+    }
     return FH_OK;
   }
   // FP32 solvers on a single-precision copy of p (level-0 unknown).
@@ -702,6 +761,10 @@ void fh_destroy(fh_engine* e) {
   if (e->lbl) (void)hipFree(e->lbl);
   if (e->cells) (void)hipFree(e->cells);
   if (e->counter) (void)hipFree(e->counter);
+  if (e->psave) (void)hipFree(e->psave);
+  if (e->wstats) (void)hipFree(e->wstats);
+  for (hipEvent_t ev : e->solve_ev)
+    if (ev) (void)hipEventDestroy(ev);
   for (size_t l = 0; l < e->lv.size(); l++) {
     if (l > 0 && e->lv[l].lbl) (void)hipFree(e->lv[l].lbl);
     if (e->lv[l].x) (void)hipFree(e->lv[l].x);
@@ -995,6 +1058,32 @@ int fh_get_timings(fh_engine* e, double* ms) {
   float tot = 0;
   HIPCHK(hipEventElapsedTime(&tot, e->ev[0], e->ev[6]));
   ms[FH_PHASE_TOTAL] = tot;
+  return FH_OK;
+}
+
+int fh_last_solve_ms(fh_engine* e, double* ms) {
+  GUARD("lastSolveMs()");
+  if (!ms) return FH_ERR_BAD_ARG;
+  *ms = 0;
+  if (!e->solve_ev_valid) return FH_OK;
+  HIPCHK(hipEventSynchronize(e->solve_ev[1]));
+  float t = 0;
+  HIPCHK(hipEventElapsedTime(&t, e->solve_ev[0], e->solve_ev[1]));
+  *ms = t;
+  return FH_OK;
+}
+
+int fh_set_capture(fh_engine* e, fh_unet* u) {
+  GUARD("setCapture()");
+  if (u && !e->psave) HIPCHK(hipMalloc((void**)&e->psave, n_c(e) * sizeof(double)));
+  e->capture = u;
+  return FH_OK;
+}
+
+int fh_set_warm_start(fh_engine* e, fh_unet* u) {
+  GUARD("setWarmStart()");
+  if (u && !e->wstats) HIPCHK(hipMalloc((void**)&e->wstats, 64));
+  e->warm = u;
   return FH_OK;
 }
 

@@ -1,0 +1,352 @@
+// Kernels of the learned warm start: a 4-level convolutional U-Net with
+// BF16 weights and activations, FP32 accumulation on MFMA.
+//
+// Tensors are channels-last, [N][H][W][C] with C a multiple of 8, so the 8
+// consecutive K values one MFMA lane needs (8 channels of one 3x3 tap) are
+// one 16-byte load.  A 3x3 "same" convolution is an implicit GEMM
+//   D[co][px] = sum_k W[co][k] * X[px + tap(k)][ci(k)],  k = tap*Cin + ci,
+// with M = output channels, N = pixels, K = 9*Cin padded to 32.  One
+// V_MFMA_F32_16X16X32_BF16 multiplies a 16x32 weight block by a 32x16 patch
+// block; the D layout (4 consecutive channels of one pixel per lane) makes
+// the epilogue store 8 contiguous bytes per lane.
+#include "fh_device.hpp"
+#include "fh_unet.hpp"
+
+namespace fh {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16x8 load8(const bf16_t* p) {
+  uint4 v = *reinterpret_cast<const uint4*>(p);
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0)); }
+
+// Round-to-nearest-even float -> bf16 (finite inputs).
+__device__ __forceinline__ bf16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float bf2f(bf16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// ---- conv3x3 forward --------------------------------------------------------
+// Workgroup = 4 waves; wave w computes row y = blockIdx.y*4 + w, pixels
+// x0 .. x0+63 (four 16-pixel MFMA tiles), all COUT channels.
+template <int COUT>
+__global__ __launch_bounds__(256) void k_conv3x3(const bf16_t* __restrict__ X, int cin,
+                                                 const bf16_t* __restrict__ Wt, int kpad,
+                                                 const float* __restrict__ bias, int N, int H,
+                                                 int Wd, int relu, bf16_t* __restrict__ Y,
+                                                 float* __restrict__ Yf) {
+  constexpr int CT = COUT / 16;  // channel tiles
+  constexpr int PT = 4;          // pixel tiles per wave
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n = blockIdx.z;
+  const int y = blockIdx.y * 4 + wave;
+  const int x0 = blockIdx.x * 64;
+  if (y >= H) return;
+  const int j = lane & 15, g = lane >> 4;
+  f32x4 acc[PT][CT];
+#pragma unroll
+  for (int p = 0; p < PT; p++)
+#pragma unroll
+    for (int t = 0; t < CT; t++) acc[p][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int kreal = 9 * cin;
+  const size_t plane = (size_t)H * Wd;
+  const bf16_t* Xn = X + (size_t)n * plane * cin;
+  for (int k0 = 0; k0 < kpad; k0 += 32) {
+    const int k = k0 + 8 * g;
+    bf16x8 b[PT];
+    if (k < kreal) {
+      const int tap = k / cin, ci = k - tap * cin;
+      const int yy = y + tap / 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+      for (int p = 0; p < PT; p++) {
+        int xx = x0 + 16 * p + j + dx;
+        b[p] = (yy >= 0 && yy < H && xx >= 0 && xx < Wd)
+                   ? load8(Xn + ((size_t)yy * Wd + xx) * cin + ci)
+                   : zero8();
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < PT; p++) b[p] = zero8();
+    }
+#pragma unroll
+    for (int t = 0; t < CT; t++) {
+      bf16x8 a = load8(Wt + (size_t)(16 * t + j) * kpad + k);
+#pragma unroll
+      for (int p = 0; p < PT; p++)
+        acc[p][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[p], acc[p][t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < PT; p++) {
+    int x = x0 + 16 * p + j;
+    if (x >= Wd) continue;
+    size_t pix = (size_t)n * plane + (size_t)y * Wd + x;
+#pragma unroll
+    for (int t = 0; t < CT; t++) {
+      int co = 16 * t + 4 * g;
+      float v[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        v[r] = acc[p][t][r] + (bias ? bias[co + r] : 0.f);
+        if (relu) v[r] = fmaxf(v[r], 0.f);
+      }
+      if (Yf) {
+        if (co == 0) Yf[pix] = v[0];
+      } else {
+        uint2 packed;
+        packed.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+        packed.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+        *reinterpret_cast<uint2*>(Y + pix * COUT + co) = packed;
+      }
+    }
+  }
+}
+
+void launch_conv3x3(hipStream_t s, const bf16_t* X, int cin, const bf16_t* Wt, int kpad,
+                    const float* bias, int cout, int N, int H, int W, int relu, bf16_t* Y,
+                    float* Yf) {
+  dim3 g((W + 63) / 64, (H + 3) / 4, N);
+  switch (cout) {
+    case 16: k_conv3x3<16><<<g, 256, 0, s>>>(X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); break;
+    case 32: k_conv3x3<32><<<g, 256, 0, s>>>(X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); break;
+    case 48: k_conv3x3<48><<<g, 256, 0, s>>>(X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); break;
+    case 64: k_conv3x3<64><<<g, 256, 0, s>>>(X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); break;
+    case 96: k_conv3x3<96><<<g, 256, 0, s>>>(X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); break;
+    case 128: k_conv3x3<128><<<g, 256, 0, s>>>(X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); break;
+    default: break;
+  }
+}
+
+// ---- weight-gradient ----------------------------------------------------------
+// dW[co][k] = sum_px dY[px][co] * X[px + tap(k)][ci(k)],  db[co] = sum dY.
+// One workgroup owns 16 channels x 64 K values and a slice of the pixels;
+// thread t accumulates 4 consecutive k of one channel in FP32 and the slice
+// partials are added into dW with atomics (FP32).
+__global__ __launch_bounds__(256) void k_conv3x3_wgrad(const bf16_t* __restrict__ X, int cin,
+                                                       const bf16_t* __restrict__ dY, int cout,
+                                                       int N, int H, int Wd, int kstride,
+                                                       float* __restrict__ dW,
+                                                       float* __restrict__ db, int slices) {
+  const int t = threadIdx.x;
+  const int co = blockIdx.y * 16 + (t >> 4);
+  const int k = blockIdx.x * 64 + (t & 15) * 4;
+  const int kreal = 9 * cin;
+  const long total = (long)N * H * Wd;
+  const long per = (total + slices - 1) / slices;
+  const long p0 = blockIdx.z * per, p1 = min(total, p0 + per);
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f, ab = 0.f;
+  const bool kin = k < kreal && co < cout;
+  int tap = 0, ci = 0, dy = 0, dx = 0;
+  if (kin) {
+    tap = k / cin;
+    ci = k - tap * cin;
+    dy = tap / 3 - 1;
+    dx = tap % 3 - 1;
+  }
+  const bool dob = (k == 0) && co < cout;
+  if (kin || dob) {
+    for (long p = p0; p < p1; p++) {
+      int x = (int)(p % Wd);
+      long q = p / Wd;
+      int y = (int)(q % H);
+      int n = (int)(q / H);
+      float gy = bf2f(dY[p * cout + co]);
+      if (dob) ab += gy;
+      if (!kin) continue;
+      int yy = y + dy, xx = x + dx;
+      if (yy < 0 || yy >= H || xx < 0 || xx >= Wd) continue;
+      const bf16_t* xp = X + (((size_t)n * H + yy) * Wd + xx) * cin + ci;
+      uint2 v = *reinterpret_cast<const uint2*>(xp);
+      a0 += gy * __uint_as_float(v.x << 16);
+      a1 += gy * __uint_as_float(v.x & 0xffff0000u);
+      a2 += gy * __uint_as_float(v.y << 16);
+      a3 += gy * __uint_as_float(v.y & 0xffff0000u);
+    }
+  }
+  if (kin) {
+    float* w = dW + (size_t)co * kstride + k;
+    atomicAdd(w + 0, a0);
+    atomicAdd(w + 1, a1);
+    atomicAdd(w + 2, a2);
+    atomicAdd(w + 3, a3);
+  }
+  if (dob) atomicAdd(db + co, ab);
+}
+
+void launch_conv3x3_wgrad(hipStream_t s, const bf16_t* X, int cin, const bf16_t* dY, int cout,
+                          int N, int H, int W, int kstride, float* dW, float* db) {
+  long total = (long)N * H * W;
+  int slices = (int)min(256L, max(1L, total / 2048));
+  dim3 g((9 * cin + 63) / 64, (cout + 15) / 16, slices);
+  k_conv3x3_wgrad<<<g, 256, 0, s>>>(X, cin, dY, cout, N, H, W, kstride, dW, db, slices);
+}
+
+// ---- elementwise ----------------------------------------------------------------
+
+// 2x2 average pool, [N][H][W][C] -> [N][H/2][W/2][C]
+__global__ void k_pool2(const bf16_t* __restrict__ X, bf16_t* __restrict__ Y, int N, int H, int W,
+                        int C) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int h = H / 2, w = W / 2;
+  long total = (long)N * h * w * C;
+  if (i >= total) return;
+  int c = (int)(i % C);
+  long q = i / C;
+  int x = (int)(q % w);
+  q /= w;
+  int y = (int)(q % h);
+  int n = (int)(q / h);
+  const bf16_t* b = X + (((size_t)n * H + 2 * y) * W + 2 * x) * C + c;
+  float s = (bf2f(b[0]) + bf2f(b[C])) + (bf2f(b[(size_t)W * C]) + bf2f(b[(size_t)W * C + C]));
+  Y[i] = f2bf(0.25f * s);
+}
+
+// d(pool): dX (+)= dY/4 broadcast over each 2x2 block.
+__global__ void k_pool2_bwd(const bf16_t* __restrict__ dY, bf16_t* __restrict__ dX, int N, int H,
+                            int W, int C, int accumulate) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long total = (long)N * H * W * C;
+  if (i >= total) return;
+  int c = (int)(i % C);
+  long q = i / C;
+  int x = (int)(q % W);
+  q /= W;
+  int y = (int)(q % H);
+  int n = (int)(q / H);
+  int h = H / 2, w = W / 2;
+  float d = 0.25f * bf2f(dY[(((size_t)n * h + y / 2) * w + x / 2) * C + c]);
+  if (accumulate) d += bf2f(dX[i]);
+  dX[i] = f2bf(d);
+}
+
+// out[N][H][W][Ca+Cb] = concat(upsample2(a [N][H/2][W/2][Ca]), b [N][H][W][Cb])
+__global__ void k_up_concat(const bf16_t* __restrict__ a, int Ca, const bf16_t* __restrict__ b,
+                            int Cb, bf16_t* __restrict__ out, int N, int H, int W) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int C = Ca + Cb;
+  long total = (long)N * H * W * C;
+  if (i >= total) return;
+  int c = (int)(i % C);
+  long pix = i / C;
+  int x = (int)(pix % W);
+  long q = pix / W;
+  int y = (int)(q % H);
+  int n = (int)(q / H);
+  if (c < Ca)
+    out[i] = a[(((size_t)n * (H / 2) + y / 2) * (W / 2) + x / 2) * Ca + c];
+  else
+    out[i] = b[pix * Cb + (c - Ca)];
+}
+
+// backward of k_up_concat: da = sum over each 2x2 block, db = slice.
+__global__ void k_up_concat_bwd_a(const bf16_t* __restrict__ dout, int Ca, int Cb,
+                                  bf16_t* __restrict__ da, int N, int H, int W) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  int h = H / 2, w = W / 2, C = Ca + Cb;
+  long total = (long)N * h * w * Ca;
+  if (i >= total) return;
+  int c = (int)(i % Ca);
+  long q = i / Ca;
+  int x = (int)(q % w);
+  q /= w;
+  int y = (int)(q % h);
+  int n = (int)(q / h);
+  const bf16_t* d = dout + (((size_t)n * H + 2 * y) * W + 2 * x) * C + c;
+  float s = (bf2f(d[0]) + bf2f(d[C])) + (bf2f(d[(size_t)W * C]) + bf2f(d[(size_t)W * C + C]));
+  da[i] = f2bf(s);
+}
+
+// db (the skip branch) accumulates: skip tensors also feed the next level's
+// pool, so their gradient is the sum of both uses.
+__global__ void k_up_concat_bwd_b(const bf16_t* __restrict__ dout, int Ca, int Cb,
+                                  bf16_t* __restrict__ db, int N, int H, int W, int accumulate) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long total = (long)N * H * W * Cb;
+  if (i >= total) return;
+  int c = (int)(i % Cb);
+  long pix = i / Cb;
+  float v = bf2f(dout[pix * (Ca + Cb) + Ca + c]);
+  if (accumulate) v += bf2f(db[i]);
+  db[i] = f2bf(v);
+}
+
+// dY *= (Y > 0)
+__global__ void k_relu_bwd(bf16_t* __restrict__ dY, const bf16_t* __restrict__ Y, long n) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (!(bf2f(Y[i]) > 0.f)) dY[i] = 0;
+}
+
+// Weight layout transforms.  Forward layout: Wt[co][kpad], k = tap*cin+ci.
+// Backward-data layout: Wb[ci][kpadT], k' = tap'*cout+co, tap' = 8 - tap.
+__global__ void k_pack_weights(const float* __restrict__ w, int cin, int cout, int kpad,
+                               bf16_t* __restrict__ Wt, int kpadT, bf16_t* __restrict__ Wb) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int total = cout * 9 * cin;
+  if (i >= total) return;
+  int ci = i % cin;
+  int tap = (i / cin) % 9;
+  int co = i / (9 * cin);
+  bf16_t v = f2bf(w[i]);  // master layout: [co][tap][ci]
+  Wt[(size_t)co * kpad + tap * cin + ci] = v;
+  if (Wb) Wb[(size_t)ci * kpadT + (8 - tap) * cout + co] = v;
+}
+
+// Adam (Kingma & Ba 2015) on FP32 master weights with bias correction.
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, long n, float lr, float b1, float b2, float eps,
+                       float c1, float c2, float grad_scale) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float gi = g[i] * grad_scale;
+  float mi = b1 * m[i] + (1.f - b1) * gi;
+  float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+static inline int nb(long n) { return (int)((n + 255) / 256); }
+
+void launch_pool2(hipStream_t s, const bf16_t* X, bf16_t* Y, int N, int H, int W, int C) {
+  long n = (long)N * (H / 2) * (W / 2) * C;
+  k_pool2<<<nb(n), 256, 0, s>>>(X, Y, N, H, W, C);
+}
+void launch_pool2_bwd(hipStream_t s, const bf16_t* dY, bf16_t* dX, int N, int H, int W, int C,
+                      int accumulate) {
+  long n = (long)N * H * W * C;
+  k_pool2_bwd<<<nb(n), 256, 0, s>>>(dY, dX, N, H, W, C, accumulate);
+}
+void launch_up_concat(hipStream_t s, const bf16_t* a, int Ca, const bf16_t* b, int Cb, bf16_t* out,
+                      int N, int H, int W) {
+  long n = (long)N * H * W * (Ca + Cb);
+  k_up_concat<<<nb(n), 256, 0, s>>>(a, Ca, b, Cb, out, N, H, W);
+}
+void launch_up_concat_bwd(hipStream_t s, const bf16_t* dout, int Ca, int Cb, bf16_t* da,
+                          bf16_t* db, int N, int H, int W, int accumulate_b) {
+  long na = (long)N * (H / 2) * (W / 2) * Ca;
+  k_up_concat_bwd_a<<<nb(na), 256, 0, s>>>(dout, Ca, Cb, da, N, H, W);
+  long nbb = (long)N * H * W * Cb;
+  k_up_concat_bwd_b<<<nb(nbb), 256, 0, s>>>(dout, Ca, Cb, db, N, H, W, accumulate_b);
+}
+void launch_relu_bwd(hipStream_t s, bf16_t* dY, const bf16_t* Y, long n) {
+  k_relu_bwd<<<nb(n), 256, 0, s>>>(dY, Y, n);
+}
+void launch_pack_weights(hipStream_t s, const float* w, int cin, int cout, int kpad, bf16_t* Wt,
+                         int kpadT, bf16_t* Wb) {
+  int n = cout * 9 * cin;
+  k_pack_weights<<<nb(n), 256, 0, s>>>(w, cin, cout, kpad, Wt, kpadT, Wb);
+}
+void launch_adam(hipStream_t s, float* w, const float* g, float* m, float* v, long n, float lr,
+                 float b1, float b2, float eps, float c1, float c2, float grad_scale) {
+  k_adam<<<nb(n), 256, 0, s>>>(w, g, m, v, n, lr, b1, b2, eps, c1, c2, grad_scale);
+}
+
+}  // namespace fh

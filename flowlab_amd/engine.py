@@ -164,6 +164,19 @@ class HipEngine:
     def synchronize(self):
         check(self.lib.fh_synchronize(self._e))
 
+    def set_capture(self, unet):
+        """Append every defect-correction solve to unet's training set."""
+        check(self.lib.fh_set_capture(self._e, None if unet is None else unet._u))
+
+    def set_warm_start(self, unet):
+        """Learned warm start for defect-correction solves (None = p_prev)."""
+        check(self.lib.fh_set_warm_start(self._e, None if unet is None else unet._u))
+
+    def last_solve_ms(self) -> float:
+        ms = C.c_double()
+        check(self.lib.fh_last_solve_ms(self._e, C.byref(ms)))
+        return ms.value
+
     def trace_kernel(self, name: str | None):
         check(self.lib.fh_trace_kernel(self._e, 0 if name is None else hip.KERNELS[name]))
 

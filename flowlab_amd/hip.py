@@ -61,6 +61,23 @@ SIGNATURES = [
     ("fh_synchronize", _i, [_vp]),
     ("fh_trace_kernel", _i, [_vp, _i]),
     ("fh_trace_stats", _i, [_vp, _ip, _dp]),
+    ("fh_unet_create", _vp, [C.c_ulonglong]),
+    ("fh_unet_destroy", None, [_vp]),
+    ("fh_unet_last_error", C.c_char_p, []),
+    ("fh_unet_num_params", _i, []),
+    ("fh_unet_get_params", _i, [_vp, _fp, _i]),
+    ("fh_unet_set_params", _i, [_vp, _fp, _i]),
+    ("fh_unet_set_scale", _i, [_vp, _d, _d]),
+    ("fh_unet_get_scale", _i, [_vp, _dp, _dp]),
+    ("fh_unet_forward", _i, [_vp, _i, _i, _i, _fp, _fp]),
+    ("fh_unet_backward", _i, [_vp, _i, _i, _i, _fp, _fp, _fp]),
+    ("fh_unet_dataset_reserve", _i, [_vp, _i, _i, _i]),
+    ("fh_unet_dataset_size", _i, [_vp]),
+    ("fh_unet_calibrate", _i, [_vp, _i]),
+    ("fh_unet_train", _i, [_vp, _i, _i, _d, C.c_ulonglong, _dp]),
+    ("fh_set_capture", _i, [_vp, _vp]),
+    ("fh_last_solve_ms", _i, [_vp, _dp]),
+    ("fh_set_warm_start", _i, [_vp, _vp]),
 ]
 
 KERNELS = {"rb_sweep_fine": 1, "advect_faces": 2, "maccormack_faces": 3, "advect_dye": 4,

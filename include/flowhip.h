@@ -168,6 +168,47 @@ enum {
 int fh_trace_kernel(fh_engine* e, int kernel);
 int fh_trace_stats(fh_engine* e, int* count, double* total_ms);
 
+/* ---- learned warm start (no reference counterpart) ---------------------
+ * A 4-level convolutional U-Net (3x3 convs, channels 16-32-64-64, BF16
+ * weights and activations, FP32 accumulation on MFMA) predicts a correction
+ * to the previous pressure; the defect-correction solve then runs its
+ * V-cycles to the tolerance as before, so plain multigrid stays the judge.
+ * Parameters: per layer weights [cout][3][3][cin] then bias [cout], FP32,
+ * for the 15 layers e0a e0b e1a e1b e2a e2b b3a b3b d2a d2b d1a d1b d0a d0b
+ * head.  Network input: [N][H][W][8] (channels r/s_r, solid, air, p/s_p,
+ * then zeros); output [N][H][W].  H and W must be multiples of 8. */
+typedef struct fh_unet fh_unet;
+fh_unet* fh_unet_create(unsigned long long seed);
+void fh_unet_destroy(fh_unet* u);
+const char* fh_unet_last_error(void);
+int fh_unet_num_params(void);
+int fh_unet_get_params(fh_unet* u, float* out, int n);
+int fh_unet_set_params(fh_unet* u, const float* in, int n);
+/* Output scale: warm start = p_prev + s_r * c0 * (H W / area0) * net. */
+int fh_unet_set_scale(fh_unet* u, double c0, double area0);
+int fh_unet_get_scale(fh_unet* u, double* c0, double* area0);
+int fh_unet_forward(fh_unet* u, int N, int H, int W, const float* in, float* out);
+/* Parameter gradient of sum(dout .* net(in)). */
+int fh_unet_backward(fh_unet* u, int N, int H, int W, const float* in, const float* dout,
+                     float* dparams);
+/* Training set on the device: every FH_SOLVER_MG_DC solve of an engine with
+ * capture on (fh_set_capture) appends (defect, labels, p_prev, p* - p_prev)
+ * until `capacity` samples of H x W are held. */
+int fh_unet_dataset_reserve(fh_unet* u, int capacity, int H, int W);
+int fh_unet_dataset_size(fh_unet* u);
+/* Sets c0 from the captured samples' RMS correction / defect ratio. */
+int fh_unet_calibrate(fh_unet* u, int samples);
+/* `steps` Adam steps (lr; beta 0.9/0.999) on batches of `batch`; per-step
+ * masked MSE in loss_out (may be NULL). */
+int fh_unet_train(fh_unet* u, int steps, int batch, double lr, unsigned long long seed,
+                  double* loss_out);
+int fh_set_capture(fh_engine* e, fh_unet* u);
+/* Device milliseconds of the last pressure solve (fh_step's or
+ * fh_solve_pressure's), warm start and every cycle included. */
+int fh_last_solve_ms(fh_engine* e, double* ms);
+/* NULL = start from the previous pressure (the reference's warm start). */
+int fh_set_warm_start(fh_engine* e, fh_unet* u);
+
 #ifdef __cplusplus
 }
 #endif

@@ -504,3 +504,52 @@ def _reset_impl(sim: "OracleSim", g, scene):
 
 
 OracleSim.reset = _reset_impl
+
+
+# ---- learned warm start (oracle/unet.c, build-defined) ----------------------
+
+_sig("fo_unet_num_params", _i)
+_sig("fo_unet_forward", None, _i, _i, _i, _dp, _dp, _dp, _i)
+_sig("fo_unet_backward", None, _i, _i, _i, _dp, _dp, _dp, _dp, _i)
+
+UNET_LAYERS = [(8, 16, 1), (16, 16, 1), (16, 32, 1), (32, 32, 1), (32, 64, 1), (64, 64, 1),
+               (64, 64, 1), (64, 64, 1), (128, 64, 1), (64, 64, 1), (96, 32, 1), (32, 32, 1),
+               (48, 16, 1), (16, 16, 1), (16, 16, 0)]
+
+
+def unet_num_params():
+    return _lib.fo_unet_num_params()
+
+
+def unet_forward(params, x, emulate_bf16=False):
+    """x: [N][H][W][8] -> [N][H][W]"""
+    params, x = _f64(params), _f64(x)
+    N, H, W, _ = x.shape
+    out = np.empty((N, H, W))
+    _lib.fo_unet_forward(N, H, W, _p(params), _p(x), _p(out), int(emulate_bf16))
+    return out
+
+
+def unet_backward(params, x, dout, emulate_bf16=False):
+    """Gradient of sum(dout * net(x)) wrt the parameters."""
+    params, x, dout = _f64(params), _f64(x), _f64(dout)
+    N, H, W, _ = x.shape
+    g = np.empty(params.size)
+    _lib.fo_unet_backward(N, H, W, _p(params), _p(x), _p(dout), _p(g), int(emulate_bf16))
+    return g
+
+
+def unet_init(seed=0, head_scale=0.1):
+    """He-normal weights, zero biases (head: only output channel 0 live)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for li, (cin, cout, _) in enumerate(UNET_LAYERS):
+        std = np.sqrt(2.0 / (9 * cin))
+        w = rng.standard_normal((cout, 3, 3, cin)) * std
+        b = rng.standard_normal(cout) * 0.05
+        if li == len(UNET_LAYERS) - 1:
+            w[1:] = 0
+            b[1:] = 0
+            w *= head_scale
+        out += [w.ravel(), b]
+    return np.concatenate(out)

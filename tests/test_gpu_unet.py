@@ -1,0 +1,95 @@
+"""GPU checks of the learned warm start against the FP64 U-Net oracle.
+
+The oracle runs with BF16 rounding at the points where the GPU stores BF16
+(weights, activations, activation gradients), so the remaining difference is
+FP32-vs-FP64 accumulation plus rare one-ulp BF16 rounding flips; the bars
+below are set for that (relative L2 1e-2 on outputs and gradients, cosine
+0.999).  A wrong MFMA lane layout, tap offset or transposed weight fails by
+orders of magnitude more.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def net():
+    from flowlab_amd.learned import UNet
+    u = UNet(seed=11)
+    p = O.unet_init(4).astype(np.float32)
+    u.params = p
+    return u, p.astype(np.float64)
+
+
+def _input(N, H, W, seed):
+    rng = np.random.default_rng(seed)
+    x = np.zeros((N, H, W, 8))
+    x[..., 0] = rng.standard_normal((N, H, W))
+    x[..., 1] = rng.random((N, H, W)) < 0.2
+    x[..., 2] = rng.random((N, H, W)) < 0.05
+    x[..., 3] = rng.standard_normal((N, H, W))
+    return x
+
+
+@pytest.mark.parametrize("shape", [(1, 16, 16), (2, 32, 24), (1, 8, 72)])
+def test_unet_forward_matches_oracle(net, shape):
+    u, p = net
+    x = _input(*shape, seed=sum(shape))
+    got = u.forward(x)
+    want = O.unet_forward(p, x, emulate_bf16=True)
+    assert np.abs(want).max() > 1e-3
+    assert rel_l2(got, want) < 1e-2
+
+
+def test_unet_gradient_matches_oracle(net):
+    u, p = net
+    x = _input(2, 16, 24, seed=3)
+    dout = np.random.default_rng(4).standard_normal((2, 16, 24))
+    got = u.backward(x, dout).astype(np.float64)
+    want = O.unet_backward(p, x, dout, emulate_bf16=True)
+    cos = got @ want / np.linalg.norm(got) / np.linalg.norm(want)
+    assert cos > 0.999
+    assert rel_l2(got, want) < 3e-2
+    # per layer, so no layer's gradient can hide behind a large one
+    off = 0
+    for cin, cout, _ in O.UNET_LAYERS[:-1]:
+        n = cout * (9 * cin + 1)
+        assert rel_l2(got[off:off + n], want[off:off + n]) < 5e-2
+        off += n
+
+
+def test_capture_train_and_warm_start():
+    """Capture solves of a small scene, train a few steps (the loss falls),
+    and check a learned-warm-start solve still reaches 1e-6 by the oracle's
+    FP64 residual -- the multigrid stays the judge."""
+    from flowlab_amd import learned as Lrn
+    u = Lrn.UNet(seed=2)
+    n = 64
+    Lrn.generate(u, samples=24, n=n, steps_per_scene=12, seed=5, log=None)
+    assert u.samples == 24
+    u.calibrate(8)
+    c0, a0 = u.scale
+    assert c0 > 0 and a0 == n * n
+    loss = u.train(steps=40, batch=4, lr=2e-3, seed=3)
+    assert np.isfinite(loss).all()
+    assert loss[-10:].mean() < loss[:5].mean()
+    res = Lrn.evaluate(u, n=n, scenes=1, warm_steps=4, seed=77)
+    for k in ("zero", "previous", "learned"):
+        assert res[k]["vcycles"] >= 1
+    # judge: a warm-started solve converges to 1e-6 by the oracle
+    sim = Lrn._sim(n, 123)
+    for _ in range(3):
+        sim.step()
+    e = sim.stepper.engine
+    e.set_warm_start(u)
+    sim.step()
+    p, d = e.read_pressure(), e.read_divergence()
+    rel = O.relative_residual(n, n, p, d, sim.f.label, sim.g.h * sim.g.h / sim.dt)
+    assert rel <= 1e-6
