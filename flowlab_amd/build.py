@@ -56,7 +56,34 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    addon = build_node_addon(verbose)
+    if verbose and addon:
+        print(f"built {addon}")
     return LIB
+
+
+NODE_DIR = os.path.join(HERE, "node")
+ADDON = os.path.join(NODE_DIR, "flowhip.node")
+NODE_INCLUDE = os.environ.get("NODE_INCLUDE", "/usr/include/node")
+
+
+def build_node_addon(verbose: bool = False):
+    """The N-API addon (flowlab_amd/node/flowhip.node) over libflowhip.so,
+    built with the host C compiler against node_api.h; skipped when Node's
+    headers are absent."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        if verbose:
+            print(f"node_api.h not found under {NODE_INCLUDE}: N-API addon not built")
+        return None
+    tmp = ADDON + ".tmp"
+    cmd = ["gcc", "-O2", "-shared", "-fPIC", "-std=gnu11", f"-I{NODE_INCLUDE}",
+           "-DNODE_GYP_MODULE_NAME=flowhip", os.path.join(NODE_DIR, "flowhip_napi.c"),
+           "-o", tmp, f"-L{HERE}", "-lflowhip", "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"N-API addon build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, ADDON)
+    return ADDON
 
 
 if __name__ == "__main__":
