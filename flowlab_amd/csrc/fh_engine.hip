@@ -4,6 +4,7 @@
 // convergence checks, readbacks, timings).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -38,6 +39,7 @@ struct Level {
   float* b;
   float* xa;     // pre-smoothed unknown (tiled kernels read halos of x
                  // while writing interiors, so they ping-pong x <-> xa)
+  uint8_t* info; // per-cell Fluid flag + open-neighbour count (cell_info)
 };
 
 // LDS one workgroup may use for the coarse sub-cycle.
@@ -82,13 +84,19 @@ struct fh_engine {
   float* fmax = nullptr;
   double* h_sums = nullptr;  // pinned
   float* h_fmax = nullptr;   // pinned
+  // defect-correction loop: device stop flag, two pinned result slots (one
+  // per cycle in flight) and the events that mark their copies
+  int* done = nullptr;
+  double* h_res = nullptr;
+  hipEvent_t cyc_ev[2] = {};
   // patch / forces
   float* patch = nullptr;
   int pi0 = 0, pj0 = 0, pnx = 0, pny = 0, has_patch = 0;
   double buoyancy = 0, confinement = 0;
   StirBox stir{};
   // state
-  bool busy = false, lost = false;
+  std::atomic<bool> busy{false};  // calls may come from several host threads
+  bool lost = false;
   std::vector<double> hist;
   bool timing = false;
   hipEvent_t ev[FH_NUM_PHASES + 1] = {};
@@ -137,15 +145,16 @@ struct Guard {
     } else if (e->lost) {
       set_err("GPU device lost");
       status = FH_ERR_DEVICE_LOST;
-    } else if (e->busy) {
-      set_err(std::string(what) + " is not re-entrant");
-      status = FH_ERR_BUSY;
     } else {
-      e->busy = true;
+      bool expected = false;
+      if (!e->busy.compare_exchange_strong(expected, true)) {
+        set_err(std::string(what) + " is not re-entrant");
+        status = FH_ERR_BUSY;
+      }
     }
   }
   ~Guard() {
-    if (status == FH_OK) e->busy = false;
+    if (status == FH_OK) e->busy.store(false);
   }
 };
 
@@ -345,6 +354,8 @@ int refresh_levels(fh_engine* e) {
     LAUNCHED("coarsen_labels");
   }
   for (auto& L : e->lv) {
+    launch_cell_info(e->s, L.lbl, L.nx, L.ny, L.info);
+    LAUNCHED("cell_info");
     HIPCHK(hipMemsetAsync(L.x, 0, (size_t)L.nx * L.ny * sizeof(float), e->s));
     HIPCHK(hipMemsetAsync(L.xa, 0, (size_t)L.nx * L.ny * sizeof(float), e->s));
   }
@@ -376,7 +387,7 @@ int refresh_levels(fh_engine* e) {
 }
 
 int read_sums(fh_engine* e) {
-  HIPCHK(hipMemcpyAsync(e->h_sums, e->sums, 4 * sizeof(double), hipMemcpyDeviceToHost, e->s));
+  HIPCHK(hipMemcpyAsync(e->h_sums, e->sums, 6 * sizeof(double), hipMemcpyDeviceToHost, e->s));
   HIPCHK(hipStreamSynchronize(e->s));
   return FH_OK;
 }
@@ -393,7 +404,7 @@ CoarseLevels coarse_levels(const fh_engine* e, int lc, int nlev) {
   for (int k = 0; k < C.n; k++) {
     C.nx[k] = e->lv[lc + k].nx;
     C.ny[k] = e->lv[lc + k].ny;
-    C.lbl[k] = e->lv[lc + k].lbl;
+    C.info[k] = e->lv[lc + k].info;
   }
   return C;
 }
@@ -411,15 +422,15 @@ int coarse_start(const fh_engine* e, int nlev) {
 // prolong+post-smooth kernels, levels from lc in one workgroup.  warm: level
 // 0 starts from its stored x (the reference mode's pressure warm start);
 // otherwise every level's unknown starts at zero.
-int vcycle(fh_engine* e, int nlev, bool dc, bool warm) {
+int vcycle(fh_engine* e, int nlev, bool dc, bool warm, const int* done) {
   int lc = dc ? e->lc_dc : e->lc_ref;
   for (int l = 0; l < lc; l++) {
     Level& L = e->lv[l];
     Level& N = e->lv[l + 1];
     bool zero = !(l == 0 && warm);
     Trace t(e, l == 0 ? FH_K_SMOOTH_RESTRICT_FINE : -1);
-    launch_smooth_restrict(e->s, zero ? nullptr : L.x, L.xa, L.b, L.lbl, L.nx, L.ny, N.b, N.nx,
-                           N.ny, zero ? 1 : 0);
+    launch_smooth_restrict(e->s, zero ? nullptr : L.x, L.xa, L.b, L.info, L.nx, L.ny, N.b, N.nx,
+                           N.ny, zero ? 1 : 0, done);
   }
   LAUNCHED("smooth_restrict");
   {
@@ -427,14 +438,14 @@ int vcycle(fh_engine* e, int nlev, bool dc, bool warm) {
     launch_coarse_vcycle(e->s, coarse_levels(e, lc, nlev), e->lv[lc].b,
                          (lc == 0 && warm) ? e->lv[0].x : nullptr, e->lv[lc].x,
                          dc ? e->M : nullptr, dc ? e->cells : nullptr, dc ? e->ncoarse : 0,
-                         dc ? 1 : 0);
+                         dc ? 1 : 0, done);
   }
   LAUNCHED("coarse_vcycle");
   for (int l = lc - 1; l >= 0; l--) {
     Level& L = e->lv[l];
     Level& N = e->lv[l + 1];
     Trace t(e, l == 0 ? FH_K_PROLONG_SMOOTH_FINE : -1);
-    launch_prolong_smooth(e->s, L.xa, L.x, L.b, L.lbl, L.nx, L.ny, N.x, N.nx, N.ny);
+    launch_prolong_smooth(e->s, L.xa, L.x, L.b, L.info, L.nx, L.ny, N.x, N.nx, N.ny, done);
   }
   LAUNCHED("prolong_smooth");
   return FH_OK;
@@ -482,8 +493,9 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
   if (solver == FH_SOLVER_MG_DC) {
     // r = b - A p (no update) + norms; the zero-divergence guard of
     // pressureMultigrid.ts:294-300 reads the same pass.
+    launch_set_int(e->s, e->done, 0);
     launch_dc_update(e->s, e->p, e->p2, nullptr, e->div, e->lbl, L0.b, e->nx, e->ny, scale,
-                     e->partials, e->counter, e->sums);
+                     e->partials, e->sums, nullptr, 0.0);
     LAUNCHED("dc_update");
     swap_ptr(e->p, e->p2);
     int st = read_sums(e);
@@ -504,35 +516,53 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
       st = engine_warm_start(e->warm, e->s, L0.b, e->lbl, e->p, e->ny, e->nx, e->wstats);
       if (st != FH_OK) return st;
       launch_dc_update(e->s, e->p, e->p2, nullptr, e->div, e->lbl, L0.b, e->nx, e->ny, scale,
-                       e->partials, e->counter, e->sums);
+                       e->partials, e->sums, nullptr, 0.0);
       LAUNCHED("dc_update");
       swap_ptr(e->p, e->p2);
       st = read_sums(e);
       if (st != FH_OK) return st;
     }
-    double rel = rel_resid(e->h_sums, scale, true);
-    e->hist.push_back(rel);
+    e->hist.push_back(rel_resid(e->h_sums, scale, true));
+    // Cycles are enqueued one ahead of the host's convergence check: cycle
+    // c goes in before the host waits for cycle c-1's norm, so the GPU never
+    // idles on the host.  The norm kernel sets `done` on the device once the
+    // tolerance is met, and every kernel of a cycle enqueued after that
+    // returns at entry, so no extra cycle is applied.
     int used = iterations;
+    bool stopped = false;
     for (int c = 0; c < iterations; c++) {
-      st = vcycle(e, e->nlev_dc, true, false);
+      const int trace_mark = e->trace_n;
+      st = vcycle(e, e->nlev_dc, true, false, e->done);
       if (st != FH_OK) return st;
       {
         Trace t(e, FH_K_DC_UPDATE);
         launch_dc_update(e->s, e->p, e->p2, L0.x, e->div, e->lbl, L0.b, e->nx, e->ny, scale,
-                         e->partials, e->counter, e->sums);
+                         e->partials, e->sums, e->done, tol);
       }
       LAUNCHED("dc_update");
       swap_ptr(e->p, e->p2);
-      if (tol > 0 || c == iterations - 1) {
-        st = read_sums(e);
-        if (st != FH_OK) return st;
-        rel = rel_resid(e->h_sums, scale, true);
-        e->hist.push_back(rel);
-        if (tol > 0 && rel <= tol) {
-          used = c + 1;
+      HIPCHK(hipMemcpyAsync(e->h_res + 8 * (c & 1), e->sums, 6 * sizeof(double),
+                            hipMemcpyDeviceToHost, e->s));
+      HIPCHK(hipEventRecord(e->cyc_ev[c & 1], e->s));
+      if (c > 0 && tol > 0) {
+        HIPCHK(hipEventSynchronize(e->cyc_ev[(c - 1) & 1]));
+        const double* r = e->h_res + 8 * ((c - 1) & 1);
+        e->hist.push_back(r[4]);
+        if (r[5] != 0.0) {
+          used = c;  // cycle c was skipped on the device
+          // p/p2 were swapped for the skipped cycle's no-op update, and its
+          // kernels' trace records time only their early return
+          swap_ptr(e->p, e->p2);
+          e->trace_n = trace_mark;
+          stopped = true;
           break;
         }
       }
+    }
+    if (!stopped && iterations > 0) {
+      int c = iterations - 1;
+      HIPCHK(hipEventSynchronize(e->cyc_ev[c & 1]));
+      e->hist.push_back(e->h_res[8 * (c & 1) + 4]);
     }
     *out_iters = used;
     if (e->capture) {
@@ -543,8 +573,7 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
     return FH_OK;
   }
   // FP32 solvers on a single-precision copy of p (level-0 unknown).
-  launch_norms(e->s, nullptr, e->p, e->div, e->lbl, e->nx, e->ny, scale, e->partials,
-               e->counter, e->sums);
+  launch_norms(e->s, nullptr, e->p, e->div, e->lbl, e->nx, e->ny, scale, e->partials, e->sums);
   LAUNCHED("norms");
   int st = read_sums(e);
   if (st != FH_OK) return st;
@@ -565,7 +594,7 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
       LAUNCHED("rb_sweep");
       if (target >= 0 && k % 12 == 11) {
         launch_norms(e->s, L0.x, nullptr, e->div, e->lbl, e->nx, e->ny, scale, e->partials,
-                     e->counter, e->sums);
+                     e->sums);
         st = read_sums(e);
         if (st != FH_OK) return st;
         if (e->h_sums[1] > 0 && std::sqrt(e->h_sums[0] / e->h_sums[1]) <= target) {
@@ -577,11 +606,11 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
   } else if (solver == FH_SOLVER_MG) {
     launch_rhs32(e->s, e->div, L0.b, N, -scale);
     for (int c = 0; c < iterations; c++) {
-      st = vcycle(e, e->nlev_ref, false, true);
+      st = vcycle(e, e->nlev_ref, false, true, nullptr);
       if (st != FH_OK) return st;
       if (target >= 0) {
         launch_norms(e->s, L0.x, nullptr, e->div, e->lbl, e->nx, e->ny, scale, e->partials,
-                     e->counter, e->sums);
+                     e->sums);
         st = read_sums(e);
         if (st != FH_OK) return st;
         if (e->h_sums[1] > 0 && std::sqrt(e->h_sums[0] / e->h_sums[1]) <= target) {
@@ -718,13 +747,14 @@ fh_engine* fh_create(int nx, int ny, int dye_scale, double h) {
   e->nlev_dc = level_count(nx, ny, 16);
   int lnx = nx, lny = ny;
   for (int l = 0; l < e->nlev_ref; l++) {
-    Level L{lnx, lny, nullptr, nullptr, nullptr, nullptr};
+    Level L{lnx, lny, nullptr, nullptr, nullptr, nullptr, nullptr};
     size_t n = (size_t)lnx * lny;
     if (l == 0)
       L.lbl = e->lbl;
     else if (dalloc(e, &L.lbl, n))
       return bail();
-    if (dalloc(e, &L.x, n) || dalloc(e, &L.b, n) || dalloc(e, &L.xa, n)) return bail();
+    if (dalloc(e, &L.x, n) || dalloc(e, &L.b, n) || dalloc(e, &L.xa, n) || dalloc(e, &L.info, n))
+      return bail();
     e->lv.push_back(L);
     lnx = (lnx + 1) / 2;
     lny = (lny + 1) / 2;
@@ -732,16 +762,22 @@ fh_engine* fh_create(int nx, int ny, int dye_scale, double h) {
   e->lc_ref = coarse_start(e, e->nlev_ref);
   e->lc_dc = coarse_start(e, e->nlev_dc);
   size_t nblk = (size_t)((nx + 63) / 64) * ((ny + 3) / 4);
-  if (dalloc(e, &e->partials, 4 * nblk) || dalloc(e, &e->counter, 1) || dalloc(e, &e->sums, 4) ||
-      dalloc(e, &e->fpartials, 1024) || dalloc(e, &e->fmax, 1))
+  if (dalloc(e, &e->partials, 4 * nblk) || dalloc(e, &e->counter, 1) || dalloc(e, &e->sums, 8) ||
+      dalloc(e, &e->fpartials, 1024) || dalloc(e, &e->fmax, 1) || dalloc(e, &e->done, 1))
     return bail();
-  if (hipHostMalloc((void**)&e->h_sums, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+  if (hipHostMalloc((void**)&e->h_sums, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&e->h_res, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&e->h_fmax, sizeof(float), hipHostMallocDefault) != hipSuccess) {
     set_err("fh_create: hipHostMalloc failed");
     return bail();
   }
   for (int k = 0; k <= FH_NUM_PHASES; k++)
     if (hipEventCreate(&e->ev[k]) != hipSuccess) {
+      set_err("fh_create: hipEventCreate failed");
+      return bail();
+    }
+  for (hipEvent_t& ev : e->cyc_ev)
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
       set_err("fh_create: hipEventCreate failed");
       return bail();
     }
@@ -761,16 +797,21 @@ void fh_destroy(fh_engine* e) {
   if (e->lbl) (void)hipFree(e->lbl);
   if (e->cells) (void)hipFree(e->cells);
   if (e->counter) (void)hipFree(e->counter);
+  if (e->done) (void)hipFree(e->done);
   if (e->psave) (void)hipFree(e->psave);
   if (e->wstats) (void)hipFree(e->wstats);
   for (hipEvent_t ev : e->solve_ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (hipEvent_t ev : e->cyc_ev)
     if (ev) (void)hipEventDestroy(ev);
   for (size_t l = 0; l < e->lv.size(); l++) {
     if (l > 0 && e->lv[l].lbl) (void)hipFree(e->lv[l].lbl);
     if (e->lv[l].x) (void)hipFree(e->lv[l].x);
     if (e->lv[l].b) (void)hipFree(e->lv[l].b);
     if (e->lv[l].xa) (void)hipFree(e->lv[l].xa);
+    if (e->lv[l].info) (void)hipFree(e->lv[l].info);
   }
+  if (e->h_res) (void)hipHostFree(e->h_res);
   if (e->h_sums) (void)hipHostFree(e->h_sums);
   if (e->h_fmax) (void)hipHostFree(e->h_fmax);
   for (int k = 0; k <= FH_NUM_PHASES; k++)

@@ -77,15 +77,18 @@ void launch_f64_to_f32(hipStream_t s, const double* a, float* out, int n);
 void launch_f32_to_f64(hipStream_t s, const float* a, double* out, int n);
 // Defect-correction update (FP64): pn = po + e;  r = b - A pn with
 // b = -scale*div;  r32 = (float) r;  sums[0..3] = {sum r^2 (fluid, count>0),
-// n of those, sum div^2 (fluid), n fluid}.  e may be null (no update).
+// n of those, sum div^2 (fluid), n fluid}, sums[4] = relative residual,
+// sums[5] = 1 once it is <= tol (tol > 0), which also sets *done; with *done
+// already set nothing runs.  e may be null (no update), done may be null.
 void launch_dc_update(hipStream_t s, const double* po, double* pn, const float* e,
                       const double* div, const uint8_t* lbl, float* r32, int nx, int ny,
-                      double scale, double* partials, unsigned* counter, double* sums);
+                      double scale, double* partials, double* sums, int* done, double tol);
 // Norms of the current FP32 or FP64 pressure: sums as in launch_dc_update,
 // r computed as div + (count p - sum)/scale (pressure.ts:102-142).
 void launch_norms(hipStream_t s, const float* p32, const double* p64, const double* div,
                   const uint8_t* lbl, int nx, int ny, double scale, double* partials,
-                  unsigned* counter, double* sums);
+                  double* sums);
+void launch_set_int(hipStream_t s, int* p, int v);
 
 // ---- fused multigrid (fh_mgfused.hip) ----------------------------------------
 constexpr int kMaxCoarse = 12;
@@ -93,23 +96,27 @@ struct CoarseLevels {
   int n;
   int nx[kMaxCoarse];
   int ny[kMaxCoarse];
-  const uint8_t* lbl[kMaxCoarse];
+  const uint8_t* info[kMaxCoarse];  // per-level cell info (launch_cell_info)
 };
+// info byte per cell: bit 7 fluid, bits 0..2 the count of non-solid
+// in-domain neighbours; 0 for non-fluid cells.
+void launch_cell_info(hipStream_t s, const uint8_t* lbl, int nx, int ny, uint8_t* info);
 // Pre-smoothing (2 red-black sweeps, omega 1) + residual + restriction of a
 // level to the next one's right-hand side bc.  zero: the unknown starts at 0
 // (xin unread).  xin and xout must differ.
 void launch_smooth_restrict(hipStream_t s, const float* xin, float* xout, const float* b,
-                            const uint8_t* lbl, int nx, int ny, float* bc, int cnx, int cny,
-                            int zero);
+                            const uint8_t* info, int nx, int ny, float* bc, int cnx, int cny,
+                            int zero, const int* done);
 // Prolongation of the coarse correction e and post-smoothing (2 sweeps).
 void launch_prolong_smooth(hipStream_t s, const float* xin, float* xout, const float* b,
-                           const uint8_t* lbl, int nx, int ny, const float* e, int cnx, int cny);
+                           const uint8_t* info, int nx, int ny, const float* e, int cnx, int cny,
+                           const int* done);
 // The sub-cycle of levels L[0..n) in one workgroup: b0 is the first level's
 // right-hand side, x0in its warm start (NULL = zero), x0out its result.
 // direct: FP64 dense solve on the last level, else 8 red-black pairs.
 void launch_coarse_vcycle(hipStream_t s, const CoarseLevels& L, const float* b0,
                           const float* x0in, float* x0out, const double* M, const int* cells,
-                          int ncoarse, int direct);
+                          int ncoarse, int direct, const int* done);
 size_t coarse_vcycle_lds_bytes(const CoarseLevels& L);
 void launch_finalize_sum4(hipStream_t s, const double* partials, int nblocks, double* out);
 

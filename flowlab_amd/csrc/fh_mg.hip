@@ -161,6 +161,16 @@ __global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ o
   if (k < n) out[k] = (double)a[k];
 }
 
+// Reduction grids are capped at ~1024 blocks (grid-stride over rows), so the
+// one-block finalize reads few partials.
+static inline dim3 reduce_grid(int nx, int ny) {
+  int gx = (nx + BX - 1) / BX;
+  int gy = (ny + BY - 1) / BY;
+  int cap = 1024 / gx;
+  if (cap < 1) cap = 1;
+  return dim3(gx, gy < cap ? gy : cap);
+}
+
 // FP64 defect-correction update, one thread per cell:
 //   pn = po + e  (e = 0 off the fluid),  r = b - (count*pn - sum pn),
 //   b = -scale*div,  r32 = (float) r  (the next V-cycle's right-hand side).
@@ -169,19 +179,20 @@ __global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ o
 __global__ __launch_bounds__(256) void k_dc_update(
     const double* __restrict__ po, double* __restrict__ pn, const float* __restrict__ e,
     const double* __restrict__ div, const uint8_t* __restrict__ lbl, float* __restrict__ r32,
-    int nx, int ny, double scale, double* partials, unsigned* counter, double* sums) {
-  int i = blockIdx.x * BX + threadIdx.x;
-  int j = blockIdx.y * BY + threadIdx.y;
+    int nx, int ny, double scale, double* partials, const int* __restrict__ done) {
+  if (done && *done) return;
+  const int i = blockIdx.x * BX + threadIdx.x;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  if (i < nx && j < ny) {
+  for (int j = blockIdx.y * BY + threadIdx.y; j < ny; j += gridDim.y * BY) {
+    if (i >= nx) break;
     int k = i + j * nx;
     double pk = po[k] + (e ? (double)e[k] : 0.0);
     pn[k] = pk;
     double r = 0.0;
     if (lbl[k] == FLUID) {
       double d = div[k];
-      acc[2] = d * d;
-      acc[3] = 1.0;
+      acc[2] += d * d;
+      acc[3] += 1.0;
       int cnt = 0;
       double sum = 0.0;
       if (!solid_or_outside(lbl, nx, ny, i + 1, j)) {
@@ -202,8 +213,8 @@ __global__ __launch_bounds__(256) void k_dc_update(
       }
       if (cnt > 0) {
         r = -scale * d - ((double)cnt * pk - sum);
-        acc[0] = r * r;
-        acc[1] = 1.0;
+        acc[0] += r * r;
+        acc[1] += 1.0;
       }
     }
     r32[k] = (float)r;
@@ -218,52 +229,74 @@ __global__ __launch_bounds__(256) void k_norms(const float* __restrict__ p32,
                                                const double* __restrict__ p64,
                                                const double* __restrict__ div,
                                                const uint8_t* __restrict__ lbl, int nx, int ny,
-                                               double scale, double* partials, unsigned* counter,
-                                               double* sums) {
-  int i = blockIdx.x * BX + threadIdx.x;
-  int j = blockIdx.y * BY + threadIdx.y;
+                                               double scale, double* partials) {
+  const int i = blockIdx.x * BX + threadIdx.x;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  if (i < nx && j < ny) {
+  for (int j = blockIdx.y * BY + threadIdx.y; j < ny; j += gridDim.y * BY) {
+    if (i >= nx) break;
     int k = i + j * nx;
-    if (lbl[k] == FLUID) {
-      double d = div[k];
-      acc[2] = d * d;
-      acc[3] = 1.0;
-      auto P = [&](int q) { return p32 ? (double)p32[q] : p64[q]; };
-      int cnt = 0;
-      double sum = 0.0;
-      if (!solid_or_outside(lbl, nx, ny, i + 1, j)) { cnt++; sum += P(k + 1); }
-      if (!solid_or_outside(lbl, nx, ny, i, j + 1)) { cnt++; sum += P(k + nx); }
-      if (!solid_or_outside(lbl, nx, ny, i - 1, j)) { cnt++; sum += P(k - 1); }
-      if (!solid_or_outside(lbl, nx, ny, i, j - 1)) { cnt++; sum += P(k - nx); }
-      if (cnt > 0) {
-        double rr = d + ((double)cnt * P(k) - sum) / scale;
-        acc[0] = rr * rr;
-        acc[1] = 1.0;
-      }
+    if (lbl[k] != FLUID) continue;
+    double d = div[k];
+    acc[2] += d * d;
+    acc[3] += 1.0;
+    auto P = [&](int q) { return p32 ? (double)p32[q] : p64[q]; };
+    int cnt = 0;
+    double sum = 0.0;
+    if (!solid_or_outside(lbl, nx, ny, i + 1, j)) { cnt++; sum += P(k + 1); }
+    if (!solid_or_outside(lbl, nx, ny, i, j + 1)) { cnt++; sum += P(k + nx); }
+    if (!solid_or_outside(lbl, nx, ny, i - 1, j)) { cnt++; sum += P(k - 1); }
+    if (!solid_or_outside(lbl, nx, ny, i, j - 1)) { cnt++; sum += P(k - nx); }
+    if (cnt > 0) {
+      double rr = d + ((double)cnt * P(k) - sum) / scale;
+      acc[0] += rr * rr;
+      acc[1] += 1.0;
     }
   }
   block_sum_store<4>(acc, partials);
 }
 
-// One block sums NS-slot partials in block order (deterministic).
-__global__ __launch_bounds__(256) void k_finalize_sum4(const double* __restrict__ partials,
-                                                       int nblocks, double* __restrict__ out) {
-  __shared__ double sh[4][4];
+// One block sums the 4-slot partials in block order (deterministic) into
+// out[0..3].  With tol > 0 it also writes out[4] = the relative residual
+// sqrt(out0/out1) / (|scale| sqrt(out2/out3)) (the defect-correction norm)
+// and sets *done (and out[5] = 1) once it is <= tol.
+__global__ __launch_bounds__(1024) void k_finalize_sum4(const double* __restrict__ partials,
+                                                        int nblocks, double* __restrict__ out,
+                                                        int* __restrict__ done, double scale,
+                                                        double tol) {
+  if (done && *done) return;
+  __shared__ double sh[4][16];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   for (int s = 0; s < 4; s++) {
     double t = 0.0;
-    for (int b = tid; b < nblocks; b += 256) t += partials[(size_t)b * 4 + s];
+    for (int b = tid; b < nblocks; b += 1024) t += partials[(size_t)b * 4 + s];
     t = wave_sum(t);
     if (lane == 0) sh[s][wid] = t;
   }
   __syncthreads();
-  if (tid < 4) out[tid] = ((sh[tid][0] + sh[tid][1]) + sh[tid][2]) + sh[tid][3];
+  if (tid == 0) {
+    double v[4];
+    for (int s = 0; s < 4; s++) {
+      double a = 0.0;
+      for (int w = 0; w < 16; w++) a += sh[s][w];
+      v[s] = a;
+      out[s] = a;
+    }
+    double rel = 0.0;
+    if (v[1] > 0 && v[2] > 0) rel = sqrt(v[0] / v[1]) / (fabs(scale) * sqrt(v[2] / v[3]));
+    out[4] = rel;
+    bool conv = tol > 0 && rel <= tol;
+    out[5] = conv ? 1.0 : 0.0;
+    if (done && conv) *done = 1;
+  }
 }
 
 void launch_finalize_sum4(hipStream_t s, const double* partials, int nblocks, double* out) {
-  k_finalize_sum4<<<1, 256, 0, s>>>(partials, nblocks, out);
+  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, nblocks, out, nullptr, 1.0, 0.0);
 }
+
+__global__ void k_set_int(int* p, int v) { *p = v; }
+
+void launch_set_int(hipStream_t s, int* p, int v) { k_set_int<<<1, 1, 0, s>>>(p, v); }
 
 void launch_rb_sweep(hipStream_t s, float* x, const float* b, const uint8_t* lbl, int nx, int ny,
                      int color, float omega, float scale, int first) {
@@ -306,18 +339,17 @@ void launch_f32_to_f64(hipStream_t s, const float* a, double* out, int n) {
 
 void launch_dc_update(hipStream_t s, const double* po, double* pn, const float* e,
                       const double* div, const uint8_t* lbl, float* r32, int nx, int ny,
-                      double scale, double* partials, unsigned* counter, double* sums) {
-  dim3 g = grid2(nx, ny);
-  k_dc_update<<<g, dim3(BX, BY), 0, s>>>(po, pn, e, div, lbl, r32, nx, ny, scale, partials,
-                                         counter, sums);
-  launch_finalize_sum4(s, partials, g.x * g.y, sums);
+                      double scale, double* partials, double* sums, int* done, double tol) {
+  dim3 g = reduce_grid(nx, ny);
+  k_dc_update<<<g, dim3(BX, BY), 0, s>>>(po, pn, e, div, lbl, r32, nx, ny, scale, partials, done);
+  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, g.x * g.y, sums, done, scale, tol);
 }
 
 void launch_norms(hipStream_t s, const float* p32, const double* p64, const double* div,
                   const uint8_t* lbl, int nx, int ny, double scale, double* partials,
-                  unsigned* counter, double* sums) {
-  dim3 g = grid2(nx, ny);
-  k_norms<<<g, dim3(BX, BY), 0, s>>>(p32, p64, div, lbl, nx, ny, scale, partials, counter, sums);
+                  double* sums) {
+  dim3 g = reduce_grid(nx, ny);
+  k_norms<<<g, dim3(BX, BY), 0, s>>>(p32, p64, div, lbl, nx, ny, scale, partials);
   launch_finalize_sum4(s, partials, g.x * g.y, sums);
 }
 

@@ -13,10 +13,14 @@
 // Temporal blocking: a 64x32 tile is loaded with a halo of H cells into LDS
 // and the 4 colour half-sweeps run there, the valid region shrinking by one
 // cell per half-sweep; each cell's update depends only on its 4 neighbours'
-// previous values, so the tile interior equals the global sweep bit for bit.
-// Unknowns at non-fluid cells are zero, so a neighbour sum over all four
-// neighbours in the reference order (+x, -x, +y, -y) equals the reference's
-// sum over non-solid neighbours exactly.
+// previous values, so the tile interior equals the global sweep.  Unknowns
+// at non-fluid cells are zero, so a neighbour sum over all four neighbours in
+// the reference order (+x, -x, +y, -y) equals the reference's sum over
+// non-solid neighbours exactly.
+//
+// Every kernel takes `done`: once the defect-correction loop has converged
+// (flag set on the device by the norm kernel), cycles the host enqueued
+// ahead of that knowledge return at once and change nothing.
 #include "fh_device.hpp"
 #include "fh_launch.hpp"
 
@@ -37,30 +41,61 @@ __device__ __forceinline__ uint8_t cell_info(const uint8_t* __restrict__ lbl, in
   return (uint8_t)(0x80 | c);
 }
 
-// One colour half-sweep with omega = 1 on an LDS region of LX x LY cells
-// (row pitch LX), updating cells of the colour whose LDS distance to the
-// region border is >= ring.  (gx0 + gy0) is the global parity of LDS (0,0).
-// zero: the unknown is zero on entry (neighbour sum = 0).
+__global__ void k_cell_info(const uint8_t* __restrict__ lbl, int nx, int ny,
+                            uint8_t* __restrict__ info) {
+  int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y;
+  if (i < nx && j < ny) info[i + j * nx] = cell_info(lbl, nx, ny, i, j);
+}
+
+// (sum + b) / count, with count in 1..4 (1/3 rounded: within an ulp of the
+// division).
+__device__ __forceinline__ float gs_update(float sum, float b, int cnt) {
+  float r = cnt == 1 ? 1.f : cnt == 2 ? 0.5f : cnt == 3 ? (1.f / 3.f) : 0.25f;
+  return (sum + b) * r;
+}
+
+// One colour half-sweep with omega = 1 on an LDS region of LX x LY cells,
+// updating cells of the colour whose LDS distance to the region border is
+// >= ring.  par0 = parity of LDS (0,0) in global coordinates.  zero: the
+// unknown is zero on entry (neighbour sum = 0).  LX is a template constant so
+// the pair index splits into (row, pair) with a multiply-shift.
+template <int LX, int LY>
 __device__ __forceinline__ void lds_half_sweep(float* sx, const float* sb, const uint8_t* si,
-                                               int LX, int LY, int ring, int color, int par0,
-                                               bool zero, int tid, int nthreads) {
-  int w = LX - 2 * ring, hgt = LY - 2 * ring;
-  if (w <= 0 || hgt <= 0) return;
-  int pairs = (w + 1) >> 1;
-  int total = pairs * hgt;
+                                               int ring, int color, int par0, bool zero, int tid,
+                                               int nthreads) {
+  constexpr int P = (LX + 1) / 2;  // pairs per row
+  const int r0 = ring, r1 = LY - 1 - ring;
+  const int total = P * (r1 - r0 + 1);
   for (int q = tid; q < total; q += nthreads) {
-    int r = q / pairs, pc = q - r * pairs;
-    int lj = ring + r;
-    int li0 = ring + 2 * pc;
-    // choose the pair member whose global parity is `color`
-    int li = li0 + (((li0 + lj + par0) & 1) != color);
-    if (li > LX - 1 - ring) continue;
+    int lj = r0 + q / P;
+    int pc = q - (lj - r0) * P;
+    int li = 2 * pc + ((lj + par0 + color) & 1);
+    if (li < ring || li > LX - 1 - ring) continue;
     int l = li + lj * LX;
-    uint8_t inf = si[l];
-    int cnt = inf & 7;
+    int cnt = si[l] & 7;
     if (cnt == 0) continue;
     float sum = zero ? 0.f : (((sx[l + 1] + sx[l - 1]) + sx[l + LX]) + sx[l - LX]);
-    sx[l] = (sum + sb[l]) / (float)cnt;
+    sx[l] = gs_update(sum, sb[l], cnt);
+  }
+}
+
+// Runtime-width variant for the one-workgroup coarse levels.
+__device__ __forceinline__ void lds_half_sweep_rt(float* sx, const float* sb, const uint8_t* si,
+                                                  int LX, int LY, int color, bool zero, int tid,
+                                                  int nthreads) {
+  const int w = LX - 2, P = (w + 1) >> 1;
+  const int total = P * (LY - 2);
+  for (int q = tid; q < total; q += nthreads) {
+    int r = q / P, pc = q - r * P;
+    int lj = 1 + r;
+    // padded (0,0) is global (-1,-1): global parity of (li, lj) = li + lj
+    int li = 1 + 2 * pc + (((1 + 2 * pc + lj + color) & 1) ? 1 : 0);
+    if (li > LX - 2) continue;
+    int l = li + lj * LX;
+    int cnt = si[l] & 7;
+    if (cnt == 0) continue;
+    float sum = zero ? 0.f : (((sx[l + 1] + sx[l - 1]) + sx[l + LX]) + sx[l - LX]);
+    sx[l] = gs_update(sum, sb[l], cnt);
   }
 }
 
@@ -84,9 +119,11 @@ constexpr int AX = TX + 2 * HA, AY = TY + 2 * HA;
 __global__ __launch_bounds__(NT) void k_smooth_restrict(const float* __restrict__ xin,
                                                         float* __restrict__ xout,
                                                         const float* __restrict__ b,
-                                                        const uint8_t* __restrict__ lbl, int nx,
+                                                        const uint8_t* __restrict__ info, int nx,
                                                         int ny, float* __restrict__ bc, int cnx,
-                                                        int cny, int zero) {
+                                                        int cny, int zero,
+                                                        const int* __restrict__ done) {
+  if (done && *done) return;
   __shared__ float sx[AX * AY];
   __shared__ float sb[AX * AY];
   __shared__ uint8_t si[AX * AY];
@@ -100,12 +137,12 @@ __global__ __launch_bounds__(NT) void k_smooth_restrict(const float* __restrict_
     int g = gi + gj * nx;
     sx[q] = (in && !zero) ? xin[g] : 0.f;
     sb[q] = in ? b[g] : 0.f;
-    si[q] = cell_info(lbl, nx, ny, gi, gj);
+    si[q] = in ? info[g] : 0;
   }
   __syncthreads();
   const int par0 = (gx0 + gy0) & 1;
   for (int s = 1; s <= 4; s++) {
-    lds_half_sweep(sx, sb, si, AX, AY, s, (s - 1) & 1, par0, zero && s == 1, tid, NT);
+    lds_half_sweep<AX, AY>(sx, sb, si, s, (s - 1) & 1, par0, zero && s == 1, tid, NT);
     __syncthreads();
   }
   // coarse cells of this tile: children (2I+di, 2J+dj) are tile-interior
@@ -141,9 +178,11 @@ constexpr int EX = BXW / 2 + 3, EY = BYH / 2 + 3;
 __global__ __launch_bounds__(NT) void k_prolong_smooth(const float* __restrict__ xin,
                                                        float* __restrict__ xout,
                                                        const float* __restrict__ b,
-                                                       const uint8_t* __restrict__ lbl, int nx,
+                                                       const uint8_t* __restrict__ info, int nx,
                                                        int ny, const float* __restrict__ e,
-                                                       int cnx, int cny) {
+                                                       int cnx, int cny,
+                                                       const int* __restrict__ done) {
+  if (done && *done) return;
   __shared__ float sx[BXW * BYH];
   __shared__ float sb[BXW * BYH];
   __shared__ uint8_t si[BXW * BYH];
@@ -165,7 +204,7 @@ __global__ __launch_bounds__(NT) void k_prolong_smooth(const float* __restrict__
     int g = gi + gj * nx;
     sx[q] = in ? xin[g] : 0.f;
     sb[q] = in ? b[g] : 0.f;
-    si[q] = cell_info(lbl, nx, ny, gi, gj);
+    si[q] = in ? info[g] : 0;
   }
   __syncthreads();
   for (int q = tid; q < BXW * BYH; q += NT) {
@@ -184,7 +223,7 @@ __global__ __launch_bounds__(NT) void k_prolong_smooth(const float* __restrict__
   __syncthreads();
   const int par0 = (gx0 + gy0) & 1;
   for (int s = 1; s <= 4; s++) {
-    lds_half_sweep(sx, sb, si, BXW, BYH, s, (s - 1) & 1, par0, false, tid, NT);
+    lds_half_sweep<BXW, BYH>(sx, sb, si, s, (s - 1) & 1, par0, false, tid, NT);
     __syncthreads();
   }
   for (int q = tid; q < TX * TY; q += NT) {
@@ -207,7 +246,8 @@ __global__ __launch_bounds__(NC) void k_coarse_vcycle(CoarseLevels L, const floa
                                                       float* __restrict__ x0out,
                                                       const double* __restrict__ M,
                                                       const int* __restrict__ cells, int ncoarse,
-                                                      int direct) {
+                                                      int direct, const int* __restrict__ done) {
+  if (done && *done) return;
   extern __shared__ unsigned char smem[];
   const int tid = threadIdx.x;
   float* X[kMaxCoarse];
@@ -231,18 +271,17 @@ __global__ __launch_bounds__(NC) void k_coarse_vcycle(CoarseLevels L, const floa
       int lj = q / px, li = q - lj * px;
       int gi = li - 1, gj = lj - 1;
       bool in = gi >= 0 && gj >= 0 && gi < nx && gj < ny;
-      I8[l][q] = cell_info(L.lbl[l], nx, ny, gi, gj);
+      I8[l][q] = in ? L.info[l][gi + gj * nx] : 0;
       X[l][q] = (l == 0 && in && x0in) ? x0in[gi + gj * nx] : 0.f;
       B[l][q] = (l == 0 && in) ? b0[gi + gj * nx] : 0.f;
     }
   }
   __syncthreads();
-  // the padded region's (0,0) is global (-1,-1): parity offset 0
   for (int l = 0; l < L.n - 1; l++) {
     int px = L.nx[l] + 2, py = L.ny[l] + 2;
     bool zero = (l > 0) || !x0in;
     for (int s = 1; s <= 4; s++) {
-      lds_half_sweep(X[l], B[l], I8[l], px, py, 1, (s - 1) & 1, 0, zero && s == 1, tid, NC);
+      lds_half_sweep_rt(X[l], B[l], I8[l], px, py, (s - 1) & 1, zero && s == 1, tid, NC);
       __syncthreads();
     }
     int cnx = L.nx[l + 1], cny = L.ny[l + 1], cpx = cnx + 2;
@@ -282,7 +321,7 @@ __global__ __launch_bounds__(NC) void k_coarse_vcycle(CoarseLevels L, const floa
       }
     } else {
       for (int s = 0; s < 16; s++) {
-        lds_half_sweep(X[l], B[l], I8[l], px, py, 1, s & 1, 0, zero && s == 0, tid, NC);
+        lds_half_sweep_rt(X[l], B[l], I8[l], px, py, s & 1, zero && s == 0, tid, NC);
         __syncthreads();
       }
     }
@@ -305,7 +344,7 @@ __global__ __launch_bounds__(NC) void k_coarse_vcycle(CoarseLevels L, const floa
     }
     __syncthreads();
     for (int s = 1; s <= 4; s++) {
-      lds_half_sweep(X[l], B[l], I8[l], px, py, 1, (s - 1) & 1, 0, false, tid, NC);
+      lds_half_sweep_rt(X[l], B[l], I8[l], px, py, (s - 1) & 1, false, tid, NC);
       __syncthreads();
     }
   }
@@ -327,24 +366,30 @@ size_t coarse_vcycle_lds_bytes(const CoarseLevels& L) {
   return b;
 }
 
+void launch_cell_info(hipStream_t s, const uint8_t* lbl, int nx, int ny, uint8_t* info) {
+  dim3 g((nx + 63) / 64, (ny + 3) / 4);
+  k_cell_info<<<g, dim3(64, 4), 0, s>>>(lbl, nx, ny, info);
+}
+
 void launch_smooth_restrict(hipStream_t s, const float* xin, float* xout, const float* b,
-                            const uint8_t* lbl, int nx, int ny, float* bc, int cnx, int cny,
-                            int zero) {
+                            const uint8_t* info, int nx, int ny, float* bc, int cnx, int cny,
+                            int zero, const int* done) {
   dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY);
-  k_smooth_restrict<<<g, NT, 0, s>>>(xin, xout, b, lbl, nx, ny, bc, cnx, cny, zero);
+  k_smooth_restrict<<<g, NT, 0, s>>>(xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done);
 }
 
 void launch_prolong_smooth(hipStream_t s, const float* xin, float* xout, const float* b,
-                           const uint8_t* lbl, int nx, int ny, const float* e, int cnx, int cny) {
+                           const uint8_t* info, int nx, int ny, const float* e, int cnx, int cny,
+                           const int* done) {
   dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY);
-  k_prolong_smooth<<<g, NT, 0, s>>>(xin, xout, b, lbl, nx, ny, e, cnx, cny);
+  k_prolong_smooth<<<g, NT, 0, s>>>(xin, xout, b, info, nx, ny, e, cnx, cny, done);
 }
 
 void launch_coarse_vcycle(hipStream_t s, const CoarseLevels& L, const float* b0,
                           const float* x0in, float* x0out, const double* M, const int* cells,
-                          int ncoarse, int direct) {
+                          int ncoarse, int direct, const int* done) {
   size_t lds = coarse_vcycle_lds_bytes(L);
-  k_coarse_vcycle<<<1, NC, lds, s>>>(L, b0, x0in, x0out, M, cells, ncoarse, direct);
+  k_coarse_vcycle<<<1, NC, lds, s>>>(L, b0, x0in, x0out, M, cells, ncoarse, direct, done);
 }
 
 }  // namespace fh
