@@ -57,8 +57,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nx", type=int, default=2048)
     ap.add_argument("--ny", type=int, default=1024)
-    ap.add_argument("--kernel", default="rb_sweep_fine", help="kernel traced for the roofline")
+    ap.add_argument("--kernel", default="dc_update", help="kernel traced for the roofline")
     ap.add_argument("--cpu-steps", type=int, default=4, help="oracle steps timed (0 = skip)")
+    ap.add_argument("--batch-scenes", type=int, default=0,
+                    help="scenes of the batched decaying-turbulence run (0 = skip; the "
+                         "batched kernels are switched off in this build)")
+    ap.add_argument("--batch-steps", type=int, default=20)
     return ap.parse_args()
 
 
@@ -95,6 +99,51 @@ def run_gpu(args):
                 n_launch=n_launch, k_ms=k_ms, g=g)
 
 
+def run_batched(args):
+    """64 decaying-turbulence scenes of uneven size (256^2 .. 1024^2) in one
+    batch: tiles autotuned, each step one captured hipGraph, MG-DC to 1e-6,
+    dt per scene at CFL 0.5."""
+    from flowlab_amd import scenes as S
+    from flowlab_amd.batch import HipBatch
+
+    scenes = S.turbulence_batch(args.batch_scenes)
+    b = HipBatch([g for g, _ in scenes])
+    for s, (_, sc) in enumerate(scenes):
+        b.reset(s, sc)
+    kw = dict(scheme="macCormack", cfl=0.5, dt_max=1.0, tol=1e-6, max_cycles=100,
+              graph_cycles=8)
+    b.step(**kw)
+    tune = b.tune(reps=5)
+    for _ in range(max(args.warmup - 1, 1)):
+        b.step(**kw)
+    iters, res_max, dev_ms = [], 0.0, 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.batch_steps):
+        b.step(**kw)
+        _, it, res = b.results()
+        iters.append(float(it.mean()))
+        res_max = max(res_max, float(res.max()))
+        dev_ms += b.last_step_ms()
+    elapsed = time.perf_counter() - t0
+    cells = b.cells
+    sizes = sorted({(g.nx, g.ny) for g, _ in scenes})
+    return {
+        "workload": f"{len(scenes)} decaying-turbulence scenes, sizes {sizes} drawn by "
+                    "SplitMix64(20240612), square-obstacle lattices, k^-3 velocity RMS 1, "
+                    "MacCormack, MG-DC to 1e-6, CFL 0.5",
+        "cells": cells,
+        "steps": args.batch_steps,
+        "value": cells * args.batch_steps / elapsed,
+        "unit": "cell-steps/s",
+        "ms_per_step": elapsed * 1e3 / args.batch_steps,
+        "device_ms_per_step": dev_ms / args.batch_steps,
+        "vcycles_per_solve_mean": sum(iters) / len(iters),
+        "max_relative_residual": res_max,
+        "tiles": tune["tiles"],
+        "graph": "one hipGraph per step (8 cycles), further cycles one graph each",
+    }
+
+
 def run_cpu_baseline(args):
     from flowlab_amd import scenes as S
     from oracle import pyoracle as O
@@ -121,6 +170,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     r = run_gpu(args)
+    batched = run_batched(args) if args.batch_scenes > 0 else None
     nx, ny, K = args.nx, args.ny, args.steps
     cells = nx * ny
     value = cells * K / r["elapsed"]
@@ -166,6 +216,8 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
     }
+    if batched is not None:
+        out["batched"] = batched
     if args.cpu_steps > 0:
         v, secs = run_cpu_baseline(args)
         out["cpu_baseline"] = {

@@ -84,10 +84,12 @@ struct fh_engine {
   float* fmax = nullptr;
   double* h_sums = nullptr;  // pinned
   float* h_fmax = nullptr;   // pinned
-  // defect-correction loop: device stop flag, two pinned result slots (one
-  // per cycle in flight) and the events that mark their copies
+  // defect-correction loop: device stop flag, two host result slots (one per
+  // cycle in flight) that the norm kernel writes, and the events that mark
+  // those kernels' completion
   int* done = nullptr;
-  double* h_res = nullptr;
+  double* h_res = nullptr;  // mapped, coherent; d_res is its device address
+  double* d_res = nullptr;
   hipEvent_t cyc_ev[2] = {};
   // patch / forces
   float* patch = nullptr;
@@ -239,6 +241,10 @@ void swap_ptr(T*& a, T*& b) {
   b = t;
 }
 
+}  // namespace
+
+namespace fh {
+
 // ---- coarsest-level pseudo-inverse (host, FP64) ---------------------------
 // Unknowns: fluid cells with at least one non-solid neighbour.  A fluid
 // component that touches no Air is pure Neumann; it is regularised by
@@ -344,6 +350,10 @@ void build_pseudo_inverse(int nx, int ny, const std::vector<uint8_t>& lab, std::
       if (neumann[comp[a]]) M[(size_t)a * n + col] -= mean[comp[a]] / size[comp[a]];
   }
 }
+
+}  // namespace fh
+
+namespace {
 
 // Coarsen the label chain and rebuild the coarsest-level operator; zero
 // every multigrid unknown so no stale value from an old geometry survives.
@@ -494,8 +504,8 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
     // r = b - A p (no update) + norms; the zero-divergence guard of
     // pressureMultigrid.ts:294-300 reads the same pass.
     launch_set_int(e->s, e->done, 0);
-    launch_dc_update(e->s, e->p, e->p2, nullptr, e->div, e->lbl, L0.b, e->nx, e->ny, scale,
-                     e->partials, e->sums, nullptr, 0.0);
+    launch_dc_update(e->s, e->p, e->p2, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
+                     e->partials, e->sums, nullptr, nullptr, 0.0);
     LAUNCHED("dc_update");
     swap_ptr(e->p, e->p2);
     int st = read_sums(e);
@@ -515,8 +525,8 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
       // learned warm start: p += net correction, then the defect again
       st = engine_warm_start(e->warm, e->s, L0.b, e->lbl, e->p, e->ny, e->nx, e->wstats);
       if (st != FH_OK) return st;
-      launch_dc_update(e->s, e->p, e->p2, nullptr, e->div, e->lbl, L0.b, e->nx, e->ny, scale,
-                       e->partials, e->sums, nullptr, 0.0);
+      launch_dc_update(e->s, e->p, e->p2, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
+                       e->partials, e->sums, nullptr, nullptr, 0.0);
       LAUNCHED("dc_update");
       swap_ptr(e->p, e->p2);
       st = read_sums(e);
@@ -536,13 +546,13 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
       if (st != FH_OK) return st;
       {
         Trace t(e, FH_K_DC_UPDATE);
-        launch_dc_update(e->s, e->p, e->p2, L0.x, e->div, e->lbl, L0.b, e->nx, e->ny, scale,
-                         e->partials, e->sums, e->done, tol);
+        launch_dc_correct(e->s, e->p, e->p2, L0.x, e->div, L0.info, L0.b, e->nx, e->ny, scale,
+                          e->partials, e->done);
       }
+      launch_dc_norm(e->s, e->partials, e->nx, e->ny, e->sums, e->d_res + 8 * (c & 1), e->done,
+                     scale, tol);
       LAUNCHED("dc_update");
       swap_ptr(e->p, e->p2);
-      HIPCHK(hipMemcpyAsync(e->h_res + 8 * (c & 1), e->sums, 6 * sizeof(double),
-                            hipMemcpyDeviceToHost, e->s));
       HIPCHK(hipEventRecord(e->cyc_ev[c & 1], e->s));
       if (c > 0 && tol > 0) {
         HIPCHK(hipEventSynchronize(e->cyc_ev[(c - 1) & 1]));
@@ -766,7 +776,9 @@ fh_engine* fh_create(int nx, int ny, int dye_scale, double h) {
       dalloc(e, &e->fpartials, 1024) || dalloc(e, &e->fmax, 1) || dalloc(e, &e->done, 1))
     return bail();
   if (hipHostMalloc((void**)&e->h_sums, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&e->h_res, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&e->h_res, 16 * sizeof(double),
+                    hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&e->d_res, e->h_res, 0) != hipSuccess ||
       hipHostMalloc((void**)&e->h_fmax, sizeof(float), hipHostMallocDefault) != hipSuccess) {
     set_err("fh_create: hipHostMalloc failed");
     return bail();

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace fh {
 
 struct StirBox {
@@ -80,9 +82,17 @@ void launch_f32_to_f64(hipStream_t s, const float* a, double* out, int n);
 // n of those, sum div^2 (fluid), n fluid}, sums[4] = relative residual,
 // sums[5] = 1 once it is <= tol (tol > 0), which also sets *done; with *done
 // already set nothing runs.  e may be null (no update), done may be null.
+// info is the level's cell_info table (launch_cell_info).
 void launch_dc_update(hipStream_t s, const double* po, double* pn, const float* e,
-                      const double* div, const uint8_t* lbl, float* r32, int nx, int ny,
-                      double scale, double* partials, double* sums, int* done, double tol);
+                      const double* div, const uint8_t* info, float* r32, int nx, int ny,
+                      double scale, double* partials, double* sums, double* host_sums, int* done,
+                      double tol);
+// The two kernels of launch_dc_update, for callers that time them apart.
+void launch_dc_correct(hipStream_t s, const double* po, double* pn, const float* e,
+                       const double* div, const uint8_t* info, float* r32, int nx, int ny,
+                       double scale, double* partials, const int* done);
+void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
+                    double* host_sums, int* done, double scale, double tol);
 // Norms of the current FP32 or FP64 pressure: sums as in launch_dc_update,
 // r computed as div + (count p - sum)/scale (pressure.ts:102-142).
 void launch_norms(hipStream_t s, const float* p32, const double* p64, const double* div,
@@ -118,6 +128,13 @@ void launch_coarse_vcycle(hipStream_t s, const CoarseLevels& L, const float* b0,
                           const float* x0in, float* x0out, const double* M, const int* cells,
                           int ncoarse, int direct, const int* done);
 size_t coarse_vcycle_lds_bytes(const CoarseLevels& L);
+// out[0..3] = sums of 4-slot block partials (nblocks <= 8192).
 void launch_finalize_sum4(hipStream_t s, const double* partials, int nblocks, double* out);
+
+// Host, FP64: dense pseudo-inverse M (n x n, row-major) of the coarsest
+// level's operator over its unknown cells (fluid with an open neighbour),
+// with Neumann components regularised and mean-projected (fh_engine.hip).
+void build_pseudo_inverse(int nx, int ny, const std::vector<uint8_t>& lab, std::vector<double>& M,
+                          std::vector<int>& cells);
 
 }  // namespace fh

@@ -161,12 +161,15 @@ __global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ o
   if (k < n) out[k] = (double)a[k];
 }
 
-// Reduction grids are capped at ~1024 blocks (grid-stride over rows), so the
-// one-block finalize reads few partials.
+// Reduction grids are capped at kMaxPartials blocks (grid-stride over rows
+// beyond that), so the one-block finalize reads at most kMaxPartials/1024
+// partials per thread, all in flight together.
+constexpr int kMaxPartials = 8192;
+
 static inline dim3 reduce_grid(int nx, int ny) {
   int gx = (nx + BX - 1) / BX;
   int gy = (ny + BY - 1) / BY;
-  int cap = 1024 / gx;
+  int cap = kMaxPartials / gx;
   if (cap < 1) cap = 1;
   return dim3(gx, gy < cap ? gy : cap);
 }
@@ -175,44 +178,46 @@ static inline dim3 reduce_grid(int nx, int ny) {
 //   pn = po + e  (e = 0 off the fluid),  r = b - (count*pn - sum pn),
 //   b = -scale*div,  r32 = (float) r  (the next V-cycle's right-hand side).
 // Reads po and e at the 5-point stencil; pn goes to a second buffer so no
-// thread reads a neighbour that another thread already advanced.
+// thread reads a neighbour that another thread already advanced.  The
+// neighbour set comes from the level's info byte (open-neighbour mask and
+// count), so every load of a cell is issued at once, unconditionally, from
+// clamped in-domain indices, and closed neighbours add 0 (sum order +x, -x,
+// +y, -y, as in the reference's residual).
 __global__ __launch_bounds__(256) void k_dc_update(
     const double* __restrict__ po, double* __restrict__ pn, const float* __restrict__ e,
-    const double* __restrict__ div, const uint8_t* __restrict__ lbl, float* __restrict__ r32,
+    const double* __restrict__ div, const uint8_t* __restrict__ info, float* __restrict__ r32,
     int nx, int ny, double scale, double* partials, const int* __restrict__ done) {
   if (done && *done) return;
   const int i = blockIdx.x * BX + threadIdx.x;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int j = blockIdx.y * BY + threadIdx.y; j < ny; j += gridDim.y * BY) {
     if (i >= nx) break;
-    int k = i + j * nx;
-    double pk = po[k] + (e ? (double)e[k] : 0.0);
-    pn[k] = pk;
+    const int k = i + j * nx;
+    const int kE = i + 1 < nx ? k + 1 : k, kW = i > 0 ? k - 1 : k;
+    const int kN = j + 1 < ny ? k + nx : k, kS = j > 0 ? k - nx : k;
+    const int f = info[k];
+    const double d = div[k];
+    double pc = po[k], pE = po[kE], pW = po[kW], pN = po[kN], pS = po[kS];
+    if (e) {
+      pc += (double)e[k];
+      pE += (double)e[kE];
+      pW += (double)e[kW];
+      pN += (double)e[kN];
+      pS += (double)e[kS];
+    }
+    pn[k] = pc;
     double r = 0.0;
-    if (lbl[k] == FLUID) {
-      double d = div[k];
+    if (f & 0x80) {
       acc[2] += d * d;
       acc[3] += 1.0;
-      int cnt = 0;
-      double sum = 0.0;
-      if (!solid_or_outside(lbl, nx, ny, i + 1, j)) {
-        cnt++;
-        sum += po[k + 1] + (e ? (double)e[k + 1] : 0.0);
-      }
-      if (!solid_or_outside(lbl, nx, ny, i - 1, j)) {
-        cnt++;
-        sum += po[k - 1] + (e ? (double)e[k - 1] : 0.0);
-      }
-      if (!solid_or_outside(lbl, nx, ny, i, j + 1)) {
-        cnt++;
-        sum += po[k + nx] + (e ? (double)e[k + nx] : 0.0);
-      }
-      if (!solid_or_outside(lbl, nx, ny, i, j - 1)) {
-        cnt++;
-        sum += po[k - nx] + (e ? (double)e[k - nx] : 0.0);
-      }
+      const int cnt = f & 7;
       if (cnt > 0) {
-        r = -scale * d - ((double)cnt * pk - sum);
+        double sum = 0.0;
+        sum += (f & 0x08) ? pE : 0.0;
+        sum += (f & 0x10) ? pW : 0.0;
+        sum += (f & 0x20) ? pN : 0.0;
+        sum += (f & 0x40) ? pS : 0.0;
+        r = -scale * d - ((double)cnt * pc - sum);
         acc[0] += r * r;
         acc[1] += 1.0;
       }
@@ -258,18 +263,38 @@ __global__ __launch_bounds__(256) void k_norms(const float* __restrict__ p32,
 // One block sums the 4-slot partials in block order (deterministic) into
 // out[0..3].  With tol > 0 it also writes out[4] = the relative residual
 // sqrt(out0/out1) / (|scale| sqrt(out2/out3)) (the defect-correction norm)
-// and sets *done (and out[5] = 1) once it is <= tol.
+// and sets *done (and out[5] = 1) once it is <= tol.  host_out, if given,
+// is a mapped pinned host buffer that receives the same 6 values, so the host
+// reads them after an event without a copy on the stream.  nblocks <=
+// kMaxPartials.
 __global__ __launch_bounds__(1024) void k_finalize_sum4(const double* __restrict__ partials,
                                                         int nblocks, double* __restrict__ out,
+                                                        double* __restrict__ host_out,
                                                         int* __restrict__ done, double scale,
                                                         double tol) {
   if (done && *done) return;
   __shared__ double sh[4][16];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  constexpr int K = kMaxPartials / 1024;
+  double2 lo[K], hi[K];
+  const double2* p2 = reinterpret_cast<const double2*>(partials);
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int b = tid + k * 1024;
+    lo[k] = b < nblocks ? p2[2 * b] : make_double2(0.0, 0.0);
+    hi[k] = b < nblocks ? p2[2 * b + 1] : make_double2(0.0, 0.0);
+  }
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    acc[0] += lo[k].x;
+    acc[1] += lo[k].y;
+    acc[2] += hi[k].x;
+    acc[3] += hi[k].y;
+  }
+#pragma unroll
   for (int s = 0; s < 4; s++) {
-    double t = 0.0;
-    for (int b = tid; b < nblocks; b += 1024) t += partials[(size_t)b * 4 + s];
-    t = wave_sum(t);
+    double t = wave_sum(acc[s]);
     if (lane == 0) sh[s][wid] = t;
   }
   __syncthreads();
@@ -286,12 +311,17 @@ __global__ __launch_bounds__(1024) void k_finalize_sum4(const double* __restrict
     out[4] = rel;
     bool conv = tol > 0 && rel <= tol;
     out[5] = conv ? 1.0 : 0.0;
+    if (host_out) {
+      for (int s = 0; s < 4; s++) host_out[s] = v[s];
+      host_out[4] = rel;
+      host_out[5] = out[5];
+    }
     if (done && conv) *done = 1;
   }
 }
 
 void launch_finalize_sum4(hipStream_t s, const double* partials, int nblocks, double* out) {
-  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, nblocks, out, nullptr, 1.0, 0.0);
+  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, nblocks, out, nullptr, nullptr, 1.0, 0.0);
 }
 
 __global__ void k_set_int(int* p, int v) { *p = v; }
@@ -338,11 +368,25 @@ void launch_f32_to_f64(hipStream_t s, const float* a, double* out, int n) {
 }
 
 void launch_dc_update(hipStream_t s, const double* po, double* pn, const float* e,
-                      const double* div, const uint8_t* lbl, float* r32, int nx, int ny,
-                      double scale, double* partials, double* sums, int* done, double tol) {
+                      const double* div, const uint8_t* info, float* r32, int nx, int ny,
+                      double scale, double* partials, double* sums, double* host_sums, int* done,
+                      double tol) {
+  launch_dc_correct(s, po, pn, e, div, info, r32, nx, ny, scale, partials, done);
+  launch_dc_norm(s, partials, nx, ny, sums, host_sums, done, scale, tol);
+}
+
+void launch_dc_correct(hipStream_t s, const double* po, double* pn, const float* e,
+                       const double* div, const uint8_t* info, float* r32, int nx, int ny,
+                       double scale, double* partials, const int* done) {
   dim3 g = reduce_grid(nx, ny);
-  k_dc_update<<<g, dim3(BX, BY), 0, s>>>(po, pn, e, div, lbl, r32, nx, ny, scale, partials, done);
-  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, g.x * g.y, sums, done, scale, tol);
+  k_dc_update<<<g, dim3(BX, BY), 0, s>>>(po, pn, e, div, info, r32, nx, ny, scale, partials,
+                                         done);
+}
+
+void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
+                    double* host_sums, int* done, double scale, double tol) {
+  dim3 g = reduce_grid(nx, ny);
+  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, g.x * g.y, sums, host_sums, done, scale, tol);
 }
 
 void launch_norms(hipStream_t s, const float* p32, const double* p64, const double* div,

@@ -1,0 +1,464 @@
+// Multigrid tile bodies shared by the single-grid kernels (fh_mgfused.hip)
+// and the batched ones (fh_batch.hip): LDS-tiled pre-smooth + restrict,
+// prolong + post-smooth, and the one-workgroup coarse sub-cycle of the
+// V(2,2) cycle (pressureMultigrid.ts:253-275).  The tile shape is a
+// template parameter so the autotuner can pick it per level size; temporal
+// blocking makes every tiling give the same bits.
+#pragma once
+#include "fh_device.hpp"
+#include "fh_launch.hpp"
+
+namespace fh {
+
+constexpr int NT = 256;  // threads per tile workgroup
+constexpr int HA = 5;    // halo of the pre-smooth+restrict tile (4 half-sweeps + residual)
+constexpr int HB = 4;    // halo of the prolong+post-smooth tile (4 half-sweeps)
+
+// info byte of a cell: bit 7 = fluid; bits 3..6 = which neighbours are
+// non-solid and in the domain (+x, -x, +y, -y); bits 0..2 = how many (the
+// equation's diagonal).  0 off the domain and on non-fluid cells.
+__device__ __forceinline__ uint8_t cell_info(const uint8_t* __restrict__ lbl, int nx, int ny,
+                                             int i, int j) {
+  if (i < 0 || j < 0 || i >= nx || j >= ny) return 0;
+  if (lbl[i + j * nx] != FLUID) return 0;
+  int e = !solid_or_outside(lbl, nx, ny, i + 1, j);
+  int w = !solid_or_outside(lbl, nx, ny, i - 1, j);
+  int n = !solid_or_outside(lbl, nx, ny, i, j + 1);
+  int s = !solid_or_outside(lbl, nx, ny, i, j - 1);
+  return (uint8_t)(0x80 | (e << 3) | (w << 4) | (n << 5) | (s << 6) | (e + w + n + s));
+}
+
+// (sum + b) / count, with count in 1..4 (1/3 rounded: within an ulp of the
+// division).
+__device__ __forceinline__ float gs_update(float sum, float b, int cnt) {
+  float r = cnt == 1 ? 1.f : cnt == 2 ? 0.5f : cnt == 3 ? (1.f / 3.f) : 0.25f;
+  return (sum + b) * r;
+}
+
+// One colour half-sweep with omega = 1 on an LDS region of LX x LY cells,
+// updating cells of the colour whose LDS distance to the region border is
+// >= ring.  par0 = parity of LDS (0,0) in global coordinates.  zero: the
+// unknown is zero on entry (neighbour sum = 0).  Cells of one colour only
+// read cells of the other, so each thread reads all of its cells' operands
+// (fully unrolled, LX/LY are template constants) before writing any: the LDS
+// reads are in flight together instead of one cell after another.
+template <int LX, int LY>
+__device__ __forceinline__ void lds_half_sweep(float* sx, const float* sb, const uint8_t* si,
+                                               int ring, int color, int par0, bool zero,
+                                               int tid) {
+  constexpr int P = (LX + 1) / 2;  // pairs per row
+  constexpr int K = (P * (LY - 2) + NT - 1) / NT;
+  const int r0 = ring, r1 = LY - 1 - ring;
+  const int total = P * (r1 - r0 + 1);
+  int l[K];
+  bool act[K];
+  float nb[K], bb[K];
+  uint8_t ib[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int q = tid + k * NT;
+    int lj = r0 + q / P;
+    int pc = q - (lj - r0) * P;
+    int li = 2 * pc + ((lj + par0 + color) & 1);
+    act[k] = q < total && li >= ring && li <= LX - 1 - ring;
+    l[k] = act[k] ? li + lj * LX : LX + 1;  // any interior cell: reads stay in the array
+    ib[k] = si[l[k]];
+    bb[k] = sb[l[k]];
+    nb[k] = zero ? 0.f
+                 : (((sx[l[k] + 1] + sx[l[k] - 1]) + sx[l[k] + LX]) + sx[l[k] - LX]);
+  }
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int cnt = ib[k] & 7;
+    if (act[k] && cnt != 0) sx[l[k]] = gs_update(nb[k], bb[k], cnt);
+  }
+}
+
+// Runtime-width variant for the one-workgroup coarse levels (arrays padded
+// by a ring of zero cells), in chunks of 4 cells per thread with the reads of
+// a chunk issued before its writes.
+__device__ __forceinline__ void lds_half_sweep_rt(float* sx, const float* sb, const uint8_t* si,
+                                                  int LX, int LY, int color, bool zero, int tid,
+                                                  int nthreads) {
+  const int w = LX - 2, P = (w + 1) >> 1;
+  const int total = P * (LY - 2);
+  for (int base = 0; base < total; base += 4 * nthreads) {
+    int l[4];
+    bool act[4];
+    float nb[4], bb[4];
+    uint8_t ib[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      act[k] = false;
+      ib[k] = 0;
+      l[k] = 0;
+      nb[k] = bb[k] = 0.f;
+      if (base + k * nthreads >= total) continue;  // uniform: nothing of this slot is left
+      int q = base + tid + k * nthreads;
+      int r = q / P, pc = q - r * P;
+      int lj = 1 + r;
+      // padded (0,0) is global (-1,-1): global parity of (li, lj) = li + lj
+      int li = 1 + 2 * pc + (((1 + 2 * pc + lj + color) & 1) ? 1 : 0);
+      act[k] = q < total && li <= LX - 2;
+      l[k] = act[k] ? li + lj * LX : LX + 1;
+      ib[k] = si[l[k]];
+      bb[k] = sb[l[k]];
+      nb[k] = zero ? 0.f
+                   : (((sx[l[k] + 1] + sx[l[k] - 1]) + sx[l[k] + LX]) + sx[l[k] - LX]);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      int cnt = ib[k] & 7;
+      if (act[k] && cnt != 0) sx[l[k]] = gs_update(nb[k], bb[k], cnt);
+    }
+  }
+}
+
+// Loads an LX x LY window with global origin (gx0, gy0) into LDS (zeros off
+// the domain; x = 0 when xin is null).  Every thread first issues all of its
+// global loads into registers and only then writes LDS, so one thread waits
+// for memory once instead of once per element: with the trip count known at
+// compile time the loop unrolls and the loads are in flight together.
+//
+// The defect-correction loop's `done` flag is read alongside: returns false
+// (before any LDS write, uniformly for the workgroup) if it is set, so the
+// flag costs no memory round trip of its own.
+template <int LX, int LY>
+__device__ __forceinline__ bool stage_tile(float* sx, float* sb, uint8_t* si,
+                                           const float* __restrict__ xin,
+                                           const float* __restrict__ b,
+                                           const uint8_t* __restrict__ info, int nx, int ny,
+                                           int gx0, int gy0, int tid,
+                                           const int* __restrict__ done) {
+  const int stop = done ? *done : 0;
+  constexpr int N = LX * LY, K = (N + NT - 1) / NT;
+  float rx[K], rb[K];
+  uint8_t ri[K];
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int q = tid + k * NT;
+    int lj = q / LX, li = q - lj * LX;
+    int gi = gx0 + li, gj = gy0 + lj;
+    bool in = q < N && gi >= 0 && gj >= 0 && gi < nx && gj < ny;
+    int g = in ? gi + gj * nx : 0;
+    rx[k] = (in && xin) ? xin[g] : 0.f;
+    rb[k] = in ? b[g] : 0.f;
+    ri[k] = in ? info[g] : 0;
+  }
+  if (stop) return false;
+#pragma unroll
+  for (int k = 0; k < K; k++) {
+    int q = tid + k * NT;
+    if (q < N) {
+      sx[q] = rx[k];
+      sb[q] = rb[k];
+      si[q] = ri[k];
+    }
+  }
+  return true;
+}
+
+// r = b - (count x - sum) at an LDS cell (0 off the fluid or if count = 0).
+__device__ __forceinline__ float lds_residual(const float* sx, const float* sb, const uint8_t* si,
+                                              int LX, int l) {
+  int cnt = si[l] & 7;
+  if (cnt == 0) return 0.f;
+  float sum = ((sx[l + 1] + sx[l - 1]) + sx[l + LX]) + sx[l - LX];
+  return sb[l] - ((float)cnt * sx[l] - sum);
+}
+
+// ---- k_smooth_restrict ------------------------------------------------------
+// Pre-smoothing (2 sweeps) of level (nx,ny) and restriction of the residual
+// to the coarse right-hand side bc (cnx,cny).  x is read from xin unless
+// zero (then the unknown starts at 0), written to xout (xin != xout: halos
+// are read from global memory while other tiles write their interiors).
+// One TX x TY tile (tile index bx, by) per call; TX and TY even, so the
+// children of a coarse cell never straddle tiles.  A caller that runs
+// several tiles per workgroup puts a barrier between calls (LDS reuse).
+template <int TX, int TY>
+__device__ __forceinline__ void smooth_restrict_tile(const float* __restrict__ xin,
+                                                     float* __restrict__ xout,
+                                                     const float* __restrict__ b,
+                                                     const uint8_t* __restrict__ info, int nx,
+                                                     int ny, float* __restrict__ bc, int cnx,
+                                                     int cny, int zero,
+                                                     const int* __restrict__ done, int bx,
+                                                     int by) {
+  constexpr int AX = TX + 2 * HA, AY = TY + 2 * HA;
+  __shared__ float sx[AX * AY];
+  __shared__ float sb[AX * AY];
+  __shared__ uint8_t si[AX * AY];
+  // Opaque per call: keeps the compiler from hoisting tid-only index
+  // arithmetic out of a caller's persistent task loop (which would hold
+  // all of it in registers across tasks).
+  int tid = threadIdx.x;
+  __asm__ volatile("" : "+v"(tid));
+  const int ox = bx * TX, oy = by * TY;
+  const int gx0 = ox - HA, gy0 = oy - HA;
+  if (!stage_tile<AX, AY>(sx, sb, si, zero ? nullptr : xin, b, info, nx, ny, gx0, gy0, tid, done))
+    return;
+  __syncthreads();
+  const int par0 = (gx0 + gy0) & 1;
+  for (int s = 1; s <= 4; s++) {
+    lds_half_sweep<AX, AY>(sx, sb, si, s, (s - 1) & 1, par0, zero && s == 1, tid);
+    __syncthreads();
+  }
+  // coarse cells of this tile: children (2I+di, 2J+dj) are tile-interior
+  const int CX = TX / 2, CY = TY / 2;
+  for (int q = tid; q < CX * CY; q += NT) {
+    int cj = q / CX, ci = q - cj * CX;
+    int I = ox / 2 + ci, J = oy / 2 + cj;
+    if (I >= cnx || J >= cny) continue;
+    float s = 0.f;
+#pragma unroll
+    for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+      for (int di = 0; di < 2; di++) {
+        int gi = 2 * I + di, gj = 2 * J + dj;
+        if (gi < nx && gj < ny) s += lds_residual(sx, sb, si, AX, (gi - gx0) + (gj - gy0) * AX);
+      }
+    bc[I + J * cnx] = s;
+  }
+  for (int q = tid; q < TX * TY; q += NT) {
+    int lj = q / TX, li = q - lj * TX;
+    int gi = ox + li, gj = oy + lj;
+    if (gi < nx && gj < ny) xout[gi + gj * nx] = sx[(li + HA) + (lj + HA) * AX];
+  }
+}
+
+// ---- k_prolong_smooth --------------------------------------------------------
+// x += bilinear(e) on fluid cells (pressureMultigrid.ts:192-207), then 2
+// post-smoothing sweeps.  Reads xin, writes xout (distinct buffers).
+template <int TX, int TY>
+__device__ __forceinline__ void prolong_smooth_tile(const float* __restrict__ xin,
+                                                    float* __restrict__ xout,
+                                                    const float* __restrict__ b,
+                                                    const uint8_t* __restrict__ info, int nx,
+                                                    int ny, const float* __restrict__ e, int cnx,
+                                                    int cny, const int* __restrict__ done, int bx,
+                                                    int by) {
+  constexpr int BXW = TX + 2 * HB, BYH = TY + 2 * HB;
+  constexpr int EX = BXW / 2 + 3, EY = BYH / 2 + 3;
+  __shared__ float sx[BXW * BYH];
+  __shared__ float sb[BXW * BYH];
+  __shared__ uint8_t si[BXW * BYH];
+  __shared__ float se[EX * EY];
+  // Opaque per call: keeps the compiler from hoisting tid-only index
+  // arithmetic out of a caller's persistent task loop (which would hold
+  // all of it in registers across tasks).
+  int tid = threadIdx.x;
+  __asm__ volatile("" : "+v"(tid));
+  const int ox = bx * TX, oy = by * TY;
+  const int gx0 = ox - HB, gy0 = oy - HB;
+  // coarse window: parents I = gi>>1 and their +-1 neighbours
+  const int ex0 = (gx0 >> 1) - 1, ey0 = (gy0 >> 1) - 1;
+  {
+    constexpr int K = (EX * EY + NT - 1) / NT;
+    float re[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      int q = tid + k * NT;
+      int ej = q / EX, ei = q - ej * EX;
+      int I = ex0 + ei, J = ey0 + ej;
+      bool in = q < EX * EY && I >= 0 && J >= 0 && I < cnx && J < cny;
+      re[k] = in ? e[I + J * cnx] : 0.f;
+    }
+    if (!stage_tile<BXW, BYH>(sx, sb, si, xin, b, info, nx, ny, gx0, gy0, tid, done)) return;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      int q = tid + k * NT;
+      if (q < EX * EY) se[q] = re[k];
+    }
+  }
+  __syncthreads();
+  for (int q = tid; q < BXW * BYH; q += NT) {
+    if (!(si[q] & 0x80)) continue;
+    int lj = q / BXW, li = q - lj * BXW;
+    int gi = gx0 + li, gj = gy0 + lj;
+    int I = gi >> 1, J = gj >> 1;
+    int I2 = (gi & 1) ? min(I + 1, cnx - 1) : max(I - 1, 0);
+    int J2 = (gj & 1) ? min(J + 1, cny - 1) : max(J - 1, 0);
+    float e00 = se[(I - ex0) + (J - ey0) * EX];
+    float e10 = se[(I2 - ex0) + (J - ey0) * EX];
+    float e01 = se[(I - ex0) + (J2 - ey0) * EX];
+    float e11 = se[(I2 - ex0) + (J2 - ey0) * EX];
+    sx[q] += 0.5625f * e00 + 0.1875f * (e10 + e01) + 0.0625f * e11;
+  }
+  __syncthreads();
+  const int par0 = (gx0 + gy0) & 1;
+  for (int s = 1; s <= 4; s++) {
+    lds_half_sweep<BXW, BYH>(sx, sb, si, s, (s - 1) & 1, par0, false, tid);
+    __syncthreads();
+  }
+  for (int q = tid; q < TX * TY; q += NT) {
+    int lj = q / TX, li = q - lj * TX;
+    int gi = ox + li, gj = oy + lj;
+    if (gi < nx && gj < ny) xout[gi + gj * nx] = sx[(li + HB) + (lj + HB) * BXW];
+  }
+}
+
+// ---- k_coarse_vcycle -----------------------------------------------------------
+// The whole sub-cycle from level lc down, in one workgroup's LDS.  Each
+// level's arrays are padded by a one-cell ring of zeros (pitch nx+2) so the
+// half-sweep's neighbour reads never leave the level.
+constexpr int NC = 1024;
+constexpr int KC = 4;  // padded cells staged per thread per pass
+
+__device__ __forceinline__ int padded(int nx, int ny) { return (nx + 2) * (ny + 2); }
+
+// Body of the sub-cycle for a workgroup of NC threads; smem is the dynamic
+// LDS of coarse_vcycle_lds_bytes(L).
+__device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
+                                                   const float* __restrict__ b0,
+                                                   const float* __restrict__ x0in,
+                                                   float* __restrict__ x0out,
+                                                   const double* __restrict__ M,
+                                                   const int* __restrict__ cells, int ncoarse,
+                                                   int direct, unsigned char* smem) {
+  const int tid = threadIdx.x;
+  float* X[kMaxCoarse];
+  float* B[kMaxCoarse];
+  uint8_t* I8[kMaxCoarse];
+  {
+    unsigned char* p = smem;
+    for (int l = 0; l < L.n; l++) {
+      int n = padded(L.nx[l], L.ny[l]);
+      X[l] = (float*)p;
+      p += n * sizeof(float);
+      B[l] = (float*)p;
+      p += n * sizeof(float);
+      I8[l] = (uint8_t*)p;
+      p += (n + 3) & ~3;
+    }
+  }
+  {
+    // All levels' padded cells as one index range, KC per thread, loads
+    // issued together before the LDS writes (one memory wait per thread).
+    int total = 0;
+    for (int l = 0; l < L.n; l++) total += padded(L.nx[l], L.ny[l]);
+    for (int base = 0; base < total; base += KC * NC) {
+      float rx[KC], rb[KC];
+      uint8_t ri[KC];
+      int lvl[KC], lq[KC];
+#pragma unroll
+      for (int k = 0; k < KC; k++) {
+        int t = base + tid + k * NC;
+        int l = 0, q = t;
+        while (l < L.n - 1 && q >= padded(L.nx[l], L.ny[l])) {
+          q -= padded(L.nx[l], L.ny[l]);
+          l++;
+        }
+        int nx = L.nx[l], ny = L.ny[l], px = nx + 2;
+        int lj = q / px, li = q - lj * px;
+        int gi = li - 1, gj = lj - 1;
+        bool in = t < total && gi >= 0 && gj >= 0 && gi < nx && gj < ny;
+        int g = in ? gi + gj * nx : 0;
+        ri[k] = in ? L.info[l][g] : 0;
+        rx[k] = (l == 0 && in && x0in) ? x0in[g] : 0.f;
+        rb[k] = (l == 0 && in) ? b0[g] : 0.f;
+        lvl[k] = t < total ? l : -1;
+        lq[k] = q;
+      }
+#pragma unroll
+      for (int k = 0; k < KC; k++) {
+        if (lvl[k] < 0) continue;
+        I8[lvl[k]][lq[k]] = ri[k];
+        X[lvl[k]][lq[k]] = rx[k];
+        B[lvl[k]][lq[k]] = rb[k];
+      }
+    }
+  }
+  __syncthreads();
+  for (int l = 0; l < L.n - 1; l++) {
+    int px = L.nx[l] + 2, py = L.ny[l] + 2;
+    bool zero = (l > 0) || !x0in;
+    for (int s = 1; s <= 4; s++) {
+      lds_half_sweep_rt(X[l], B[l], I8[l], px, py, (s - 1) & 1, zero && s == 1, tid, NC);
+      __syncthreads();
+    }
+    int cnx = L.nx[l + 1], cny = L.ny[l + 1], cpx = cnx + 2;
+    for (int q = tid; q < cnx * cny; q += NC) {
+      int J = q / cnx, I = q - J * cnx;
+      float s = 0.f;
+      for (int dj = 0; dj < 2; dj++)
+        for (int di = 0; di < 2; di++) {
+          int gi = 2 * I + di, gj = 2 * J + dj;
+          if (gi < L.nx[l] && gj < L.ny[l])
+            s += lds_residual(X[l], B[l], I8[l], px, (gi + 1) + (gj + 1) * px);
+        }
+      B[l + 1][(I + 1) + (J + 1) * cpx] = s;
+    }
+    __syncthreads();
+  }
+  {
+    int l = L.n - 1;
+    int nx = L.nx[l], ny = L.ny[l], px = nx + 2, py = ny + 2;
+    bool zero = (l > 0) || !x0in;
+    if (direct) {
+      // x = M b in FP64, eight lanes per row; lane `part` takes columns
+      // part + 8m in chunks of 8, all 8 loads of a chunk in flight together.
+      for (int t = tid; t < 8 * ncoarse; t += NC) {
+        int a = t >> 3, part = t & 7;
+        const double* row = M + (size_t)a * ncoarse;
+        double s = 0.0;
+        for (int q0 = part; q0 < ncoarse; q0 += 64) {
+          double mv[8];
+          int cv[8];
+#pragma unroll
+          for (int m = 0; m < 8; m++) {
+            int q = q0 + 8 * m;
+            mv[m] = q < ncoarse ? row[q] : 0.0;
+            cv[m] = q < ncoarse ? cells[q] : 0;
+          }
+#pragma unroll
+          for (int m = 0; m < 8; m++)
+            if (q0 + 8 * m < ncoarse)
+              s += mv[m] * (double)B[l][(cv[m] % nx + 1) + (cv[m] / nx + 1) * px];
+        }
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        s += __shfl_xor(s, 4, 64);
+        if (part == 0) {
+          int c = cells[a];
+          X[l][(c % nx + 1) + (c / nx + 1) * px] = (float)s;
+        }
+      }
+    } else {
+      for (int s = 0; s < 16; s++) {
+        lds_half_sweep_rt(X[l], B[l], I8[l], px, py, s & 1, zero && s == 0, tid, NC);
+        __syncthreads();
+      }
+    }
+    __syncthreads();
+  }
+  for (int l = L.n - 2; l >= 0; l--) {
+    int nx = L.nx[l], ny = L.ny[l], px = nx + 2, py = ny + 2;
+    int cnx = L.nx[l + 1], cny = L.ny[l + 1], cpx = cnx + 2;
+    for (int q = tid; q < nx * ny; q += NC) {
+      int gj = q / nx, gi = q - gj * nx;
+      int lq = (gi + 1) + (gj + 1) * px;
+      if (!(I8[l][lq] & 0x80)) continue;
+      int I = gi >> 1, J = gj >> 1;
+      int I2 = (gi & 1) ? min(I + 1, cnx - 1) : max(I - 1, 0);
+      int J2 = (gj & 1) ? min(J + 1, cny - 1) : max(J - 1, 0);
+      const float* E = X[l + 1];
+      X[l][lq] += 0.5625f * E[(I + 1) + (J + 1) * cpx] +
+                  0.1875f * (E[(I2 + 1) + (J + 1) * cpx] + E[(I + 1) + (J2 + 1) * cpx]) +
+                  0.0625f * E[(I2 + 1) + (J2 + 1) * cpx];
+    }
+    __syncthreads();
+    for (int s = 1; s <= 4; s++) {
+      lds_half_sweep_rt(X[l], B[l], I8[l], px, py, (s - 1) & 1, false, tid, NC);
+      __syncthreads();
+    }
+  }
+  {
+    int nx = L.nx[0], ny = L.ny[0], px = nx + 2;
+    for (int q = tid; q < nx * ny; q += NC) {
+      int gj = q / nx, gi = q - gj * nx;
+      x0out[q] = X[0][(gi + 1) + (gj + 1) * px];
+    }
+  }
+}
+
+}  // namespace fh

@@ -1,0 +1,338 @@
+// Per-element bodies of the streaming stencil kernels: advection
+// (src/core/advect.ts), forces, divergence, gradient subtraction, outflow,
+// dye sources and label commits (src/core/{vorticity,buoyancy,stir,
+// divergence,subtractGradient,boundaries,dye}.ts).
+//
+// Each function computes the outputs owned by one thread index (i, j) of one
+// grid and returns at once outside its range.  The single-grid kernels map
+// (blockIdx, threadIdx) to (i, j); the batched kernels map a task of their
+// work list (scene, block) to the scene's fields and (i, j).  Both run this
+// same code, so a scene gives the same bits either way.
+#pragma once
+#include "fh_device.hpp"
+#include "fh_launch.hpp"
+
+namespace fh {
+
+// ---- advection (advect.ts) ---------------------------------------------------
+
+// advect.ts:72-112 -- one thread index covers u(i,j) when j<ny and v(i,j)
+// when i<nx, over (nx+1) x (ny+1).
+__device__ __forceinline__ void advect_faces_at(const float* __restrict__ uc,
+                                                const float* __restrict__ vc,
+                                                const float* __restrict__ us,
+                                                const float* __restrict__ vs,
+                                                float* __restrict__ uo, float* __restrict__ vo,
+                                                const uint8_t* __restrict__ lbl, int nx, int ny,
+                                                float c, int i, int j) {
+  if (i > nx || j > ny) return;
+  Pos bx, by;
+  if (j < ny) {
+    int k = i + j * (nx + 1);
+    if (solid_or_outside(lbl, nx, ny, i - 1, j) || solid_or_outside(lbl, nx, ny, i, j)) {
+      uo[k] = us[k];
+    } else {
+      Pos X{i, 0.f}, Y{j, 0.5f};
+      float v0 = sample_v(vc, nx, ny, X, Y);
+      backtrace(uc, vc, nx, ny, X, Y, uc[k], v0, c, &bx, &by);
+      uo[k] = sample_u(us, nx, ny, bx, by);
+    }
+  }
+  if (i < nx) {
+    int k = i + j * nx;
+    if (solid_or_outside(lbl, nx, ny, i, j - 1) || solid_or_outside(lbl, nx, ny, i, j)) {
+      vo[k] = vs[k];
+    } else {
+      Pos X{i, 0.5f}, Y{j, 0.f};
+      float u0 = sample_u(uc, nx, ny, X, Y);
+      backtrace(uc, vc, nx, ny, X, Y, u0, vc[k], c, &bx, &by);
+      vo[k] = sample_v(vs, nx, ny, bx, by);
+    }
+  }
+}
+
+// advect.ts:177-210 fused: the reverse pass (carrier u^n, source uHat, -dt)
+// is evaluated for this face only, then corrected = uHat + (u^n - uBar)/2 is
+// clamped to the u^n stencil at the forward backtrace.  Each face reads only
+// its own uBar, so fusing changes no value.
+__device__ __forceinline__ void maccormack_faces_at(const float* __restrict__ un,
+                                                    const float* __restrict__ vn,
+                                                    const float* __restrict__ uh,
+                                                    const float* __restrict__ vh,
+                                                    float* __restrict__ uo, float* __restrict__ vo,
+                                                    const uint8_t* __restrict__ lbl, int nx,
+                                                    int ny, float c, int i, int j) {
+  if (i > nx || j > ny) return;
+  Pos rx, ry, fx, fy;
+  if (j < ny) {
+    int k = i + j * (nx + 1);
+    if (solid_or_outside(lbl, nx, ny, i - 1, j) || solid_or_outside(lbl, nx, ny, i, j)) {
+      uo[k] = un[k];
+    } else {
+      Pos X{i, 0.f}, Y{j, 0.5f};
+      float u0 = un[k];
+      float v0 = sample_v(vn, nx, ny, X, Y);
+      backtrace(un, vn, nx, ny, X, Y, u0, v0, -c, &rx, &ry);
+      float ubar = sample_u(uh, nx, ny, rx, ry);
+      float corrected = uh[k] + 0.5f * (u0 - ubar);
+      backtrace(un, vn, nx, ny, X, Y, u0, v0, c, &fx, &fy);
+      Axis ax = clamped_axis(fx.i, fx.f, nx + 1);
+      Axis ay = clamped_axis(fy.i, fy.f - 0.5f, ny);
+      uo[k] = clamp_to_quad(corrected, gather4(un, nx + 1, ax, ay));
+    }
+  }
+  if (i < nx) {
+    int k = i + j * nx;
+    if (solid_or_outside(lbl, nx, ny, i, j - 1) || solid_or_outside(lbl, nx, ny, i, j)) {
+      vo[k] = vn[k];
+    } else {
+      Pos X{i, 0.5f}, Y{j, 0.f};
+      float v0 = vn[k];
+      float u0 = sample_u(un, nx, ny, X, Y);
+      backtrace(un, vn, nx, ny, X, Y, u0, v0, -c, &rx, &ry);
+      float vbar = sample_v(vh, nx, ny, rx, ry);
+      float corrected = vh[k] + 0.5f * (v0 - vbar);
+      backtrace(un, vn, nx, ny, X, Y, u0, v0, c, &fx, &fy);
+      Axis ax = clamped_axis(fx.i, fx.f - 0.5f, nx);
+      Axis ay = clamped_axis(fy.i, fy.f, ny + 1);
+      vo[k] = clamp_to_quad(corrected, gather4(vn, nx, ax, ay));
+    }
+  }
+}
+
+// Cell (i,j) of the dye grid (ds x finer) in velocity-cell units.
+__device__ __forceinline__ Pos dye_pos(int i, int ds) {
+  return {i / ds, ((float)(i % ds) + 0.5f) / (float)ds};
+}
+
+// advect.ts:241-270 for three channels sharing one backtrace; out *= keep.
+__device__ __forceinline__ void advect_dye_at(const float* __restrict__ u,
+                                              const float* __restrict__ v,
+                                              const float* __restrict__ q, float* __restrict__ qo,
+                                              const uint8_t* __restrict__ lbl, int nx, int ny,
+                                              int ds, float c, float k0, float k1, float k2, int i,
+                                              int j) {
+  int dnx = nx * ds, dny = ny * ds;
+  if (i >= dnx || j >= dny) return;
+  size_t plane = (size_t)dnx * dny;
+  int k = i + j * dnx;
+  if (solid_or_outside(lbl, nx, ny, i / ds, j / ds)) {
+    qo[k] = q[k] * k0;
+    qo[k + plane] = q[k + plane] * k1;
+    qo[k + 2 * plane] = q[k + 2 * plane] * k2;
+    return;
+  }
+  Pos X = dye_pos(i, ds), Y = dye_pos(j, ds);
+  float u0 = sample_u(u, nx, ny, X, Y);
+  float v0 = sample_v(v, nx, ny, X, Y);
+  Pos bx, by;
+  backtrace(u, v, nx, ny, X, Y, u0, v0, c, &bx, &by);
+  Pos dx{i, 0.5f + ds * (bx.f - X.f)}, dy{j, 0.5f + ds * (by.f - Y.f)};
+  Axis ax = clamped_axis(dx.i, dx.f - 0.5f, dnx);
+  Axis ay = clamped_axis(dy.i, dy.f - 0.5f, dny);
+  Quad g0 = gather4(q, dnx, ax, ay);
+  Quad g1 = gather4(q + plane, dnx, ax, ay);
+  Quad g2 = gather4(q + 2 * plane, dnx, ax, ay);
+  qo[k] = bilerp(g0.a, g0.b, g0.c, g0.d, ax.f, ay.f) * k0;
+  qo[k + plane] = bilerp(g1.a, g1.b, g1.c, g1.d, ax.f, ay.f) * k1;
+  qo[k + 2 * plane] = bilerp(g2.a, g2.b, g2.c, g2.d, ax.f, ay.f) * k2;
+}
+
+// advect.ts:291-311 for three channels, fused as for the faces; out *= keep.
+__device__ __forceinline__ void maccormack_dye_at(const float* __restrict__ u,
+                                                  const float* __restrict__ v,
+                                                  const float* __restrict__ qn,
+                                                  const float* __restrict__ qh,
+                                                  float* __restrict__ qo,
+                                                  const uint8_t* __restrict__ lbl, int nx, int ny,
+                                                  int ds, float c, float k0, float k1, float k2,
+                                                  int i, int j) {
+  int dnx = nx * ds, dny = ny * ds;
+  if (i >= dnx || j >= dny) return;
+  size_t plane = (size_t)dnx * dny;
+  int k = i + j * dnx;
+  if (solid_or_outside(lbl, nx, ny, i / ds, j / ds)) {
+    qo[k] = qn[k] * k0;
+    qo[k + plane] = qn[k + plane] * k1;
+    qo[k + 2 * plane] = qn[k + 2 * plane] * k2;
+    return;
+  }
+  Pos X = dye_pos(i, ds), Y = dye_pos(j, ds);
+  float u0 = sample_u(u, nx, ny, X, Y);
+  float v0 = sample_v(v, nx, ny, X, Y);
+  Pos bx, by;
+  backtrace(u, v, nx, ny, X, Y, u0, v0, -c, &bx, &by);
+  Axis ax = clamped_axis(i, ds * (bx.f - X.f), dnx);
+  Axis ay = clamped_axis(j, ds * (by.f - Y.f), dny);
+  Quad r0 = gather4(qh, dnx, ax, ay);
+  Quad r1 = gather4(qh + plane, dnx, ax, ay);
+  Quad r2 = gather4(qh + 2 * plane, dnx, ax, ay);
+  float c0 = qh[k] + 0.5f * (qn[k] - bilerp(r0.a, r0.b, r0.c, r0.d, ax.f, ay.f));
+  float c1 = qh[k + plane] + 0.5f * (qn[k + plane] - bilerp(r1.a, r1.b, r1.c, r1.d, ax.f, ay.f));
+  float c2 = qh[k + 2 * plane] +
+             0.5f * (qn[k + 2 * plane] - bilerp(r2.a, r2.b, r2.c, r2.d, ax.f, ay.f));
+  backtrace(u, v, nx, ny, X, Y, u0, v0, c, &bx, &by);
+  ax = clamped_axis(i, ds * (bx.f - X.f), dnx);
+  ay = clamped_axis(j, ds * (by.f - Y.f), dny);
+  qo[k] = clamp_to_quad(c0, gather4(qn, dnx, ax, ay)) * k0;
+  qo[k + plane] = clamp_to_quad(c1, gather4(qn + plane, dnx, ax, ay)) * k1;
+  qo[k + 2 * plane] = clamp_to_quad(c2, gather4(qn + 2 * plane, dnx, ax, ay)) * k2;
+}
+
+// ---- forces and projection -------------------------------------------------------
+
+// vorticity.ts:9-16 at corner (i,j), times h.
+__device__ __forceinline__ float curl_h(const float* __restrict__ u, const float* __restrict__ v,
+                                        int nx, int i, int j) {
+  return (v[i + j * nx] - v[i - 1 + j * nx]) - (u[i + j * (nx + 1)] - u[i + (j - 1) * (nx + 1)]);
+}
+
+// vorticity.ts:24-42: cell-centred vorticity from the four clamped corners.
+__device__ __forceinline__ void vorticity_at(const float* __restrict__ u,
+                                             const float* __restrict__ v, float* __restrict__ w,
+                                             int nx, int ny, float inv_h, int i, int j) {
+  if (i >= nx || j >= ny) return;
+  int hiI = nx - 1, hiJ = ny - 1;
+  int j0 = min(max(j, 1), hiJ), j1 = min(max(j + 1, 1), hiJ);
+  int i0 = min(max(i, 1), hiI), i1 = min(max(i + 1, 1), hiI);
+  float s = curl_h(u, v, nx, i0, j0) * inv_h + curl_h(u, v, nx, i1, j0) * inv_h +
+            curl_h(u, v, nx, i0, j1) * inv_h + curl_h(u, v, nx, i1, j1) * inv_h;
+  w[i + j * nx] = 0.25f * s;
+}
+
+// Confinement force at cell (i,j), in velocity per unit time:
+//   a = |w|, g = grad a (clamped central differences),
+//   N = g / (|g| + 1e-4 a/h + 1e-30),  f = eps h (N_y w, -N_x w).
+// Evaluated in cell units (g*h), which is the same N.
+__device__ __forceinline__ void conf_force(const float* __restrict__ w, int nx, int ny, float h,
+                                           float eps, int i, int j, float* fx, float* fy) {
+  int ip = min(i + 1, nx - 1), im = max(i - 1, 0);
+  int jp = min(j + 1, ny - 1), jm = max(j - 1, 0);
+  float om = w[i + j * nx];
+  float a = fabsf(om);
+  float gx = (fabsf(w[ip + j * nx]) - fabsf(w[im + j * nx])) / (float)(ip - im);
+  float gy = (fabsf(w[i + jp * nx]) - fabsf(w[i + jm * nx])) / (float)(jp - jm);
+  float len = sqrtf(gx * gx + gy * gy);
+  float inv = 1.0f / (len + 1e-4f * a + 1e-30f * h);
+  *fx = eps * h * (gy * inv * om);
+  *fy = eps * h * (-(gx * inv) * om);
+}
+
+// Body forces on the faces, in the order the CPU step adds them: stir
+// (stir.ts:49-83), buoyancy on v (buoyancy.ts:70-88), then confinement
+// computed from the velocity before any force.  Face set: u i in [1,nx-1],
+// v j in [1,ny-1], skipping faces that touch an in-domain Solid.
+__device__ __forceinline__ void forces_at(float* __restrict__ u, float* __restrict__ v,
+                                          const uint8_t* __restrict__ lbl,
+                                          const float* __restrict__ heat,
+                                          const float* __restrict__ w, int nx, int ny, int ds,
+                                          float h, float dt, float gain, float eps,
+                                          const StirBox& st, int i, int j) {
+  if (i > nx || j > ny) return;
+  float invR2 = st.active ? 1.0f / (st.r * st.r) : 0.f;
+  if (j < ny && i >= 1 && i <= nx - 1 && !is_solid_cell(lbl, nx, ny, i - 1, j) &&
+      !is_solid_cell(lbl, nx, ny, i, j)) {
+    int k = i + j * (nx + 1);
+    float val = u[k];
+    if (st.active && i >= st.ui0 && i <= st.ui1 && j >= st.uj0 && j <= st.uj1) {
+      float dx = (float)i - st.x, dy = ((float)j + 0.5f) - st.y;
+      val += st.dx * expf(-(dx * dx + dy * dy) * invR2);
+    }
+    if (eps != 0.f) {
+      float fa, ga, fb, gb;
+      conf_force(w, nx, ny, h, eps, i - 1, j, &fa, &ga);
+      conf_force(w, nx, ny, h, eps, i, j, &fb, &gb);
+      val += dt * (0.5f * (fa + fb));
+    }
+    u[k] = val;
+  }
+  if (i < nx && j >= 1 && j <= ny - 1 && !is_solid_cell(lbl, nx, ny, i, j - 1) &&
+      !is_solid_cell(lbl, nx, ny, i, j)) {
+    int k = i + j * nx;
+    float val = v[k];
+    if (st.active && i >= st.vi0 && i <= st.vi1 && j >= st.vj0 && j <= st.vj1) {
+      float dx = ((float)i + 0.5f) - st.x, dy = (float)j - st.y;
+      val += st.dy * expf(-(dx * dx + dy * dy) * invR2);
+    }
+    if (gain != 0.f) {
+      // sampleP on the dye grid at the face ((i+1/2) h, j h)
+      Axis ax = clamped_axis(i * ds, 0.5f * ds - 0.5f, nx * ds);
+      Axis ay = clamped_axis(j * ds, -0.5f, ny * ds);
+      Quad g = gather4(heat, nx * ds, ax, ay);
+      val += gain * bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
+    }
+    if (eps != 0.f) {
+      float fa, ga, fb, gb;
+      conf_force(w, nx, ny, h, eps, i, j - 1, &fa, &ga);
+      conf_force(w, nx, ny, h, eps, i, j, &fb, &gb);
+      val += dt * (0.5f * (ga + gb));
+    }
+    v[k] = val;
+  }
+}
+
+// divergence.ts:17-25, evaluated in FP64 from the FP32 faces.
+__device__ __forceinline__ void divergence_at(const float* __restrict__ u,
+                                              const float* __restrict__ v,
+                                              double* __restrict__ div, int nx, int ny,
+                                              double inv_h, int i, int j) {
+  if (i >= nx || j >= ny) return;
+  const float* ur = u + j * (nx + 1);
+  double d = ((double)ur[i + 1] - (double)ur[i]) + (double)v[i + (j + 1) * nx] -
+             (double)v[i + j * nx];
+  div[i + j * nx] = inv_h * d;
+}
+
+// subtractGradient.ts:40-61 with an FP64 pressure.
+__device__ __forceinline__ void subtract_at(float* __restrict__ u, float* __restrict__ v,
+                                            const double* __restrict__ p,
+                                            const uint8_t* __restrict__ lbl, int nx, int ny,
+                                            double gs, int i, int j) {
+  if (i > nx || j > ny) return;
+  if (j < ny && i >= 1 && i <= nx - 1 && !solid_or_outside(lbl, nx, ny, i - 1, j) &&
+      !solid_or_outside(lbl, nx, ny, i, j)) {
+    int k = i + j * (nx + 1);
+    u[k] = (float)((double)u[k] - gs * (p[i + j * nx] - p[i - 1 + j * nx]));
+  }
+  if (i < nx && j >= 1 && j <= ny - 1 && !solid_or_outside(lbl, nx, ny, i, j - 1) &&
+      !solid_or_outside(lbl, nx, ny, i, j)) {
+    int k = i + j * nx;
+    v[k] = (float)((double)v[k] - gs * (p[i + j * nx] - p[i + (j - 1) * nx]));
+  }
+}
+
+// boundaries.ts:115-143: index t owns u row t and v column t, replaying the
+// reference's low-edge-then-high-edge order within each.
+__device__ __forceinline__ void outflow_at(float* __restrict__ u, float* __restrict__ v,
+                                           const uint8_t* __restrict__ lbl, int nx, int ny,
+                                           int t) {
+  int su = nx + 1;
+  if (t < ny) {
+    int j = t;
+    if (lbl[j * nx] == AIR) {
+      float o = fminf(u[1 + j * su], 0.f);
+      u[1 + j * su] = o;
+      u[j * su] = o;
+    }
+    if (lbl[nx - 1 + j * nx] == AIR) {
+      float o = fmaxf(u[nx - 1 + j * su], 0.f);
+      u[nx - 1 + j * su] = o;
+      u[nx + j * su] = o;
+    }
+  }
+  if (t < nx) {
+    int i = t;
+    if (lbl[i] == AIR) {
+      float o = fminf(v[i + nx], 0.f);
+      v[i + nx] = o;
+      v[i] = o;
+    }
+    if (lbl[i + (ny - 1) * nx] == AIR) {
+      float o = fmaxf(v[i + (ny - 1) * nx], 0.f);
+      v[i + (ny - 1) * nx] = o;
+      v[i + ny * nx] = o;
+    }
+  }
+}
+
+}  // namespace fh

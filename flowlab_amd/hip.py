@@ -78,6 +78,21 @@ SIGNATURES = [
     ("fh_set_capture", _i, [_vp, _vp]),
     ("fh_last_solve_ms", _i, [_vp, _dp]),
     ("fh_set_warm_start", _i, [_vp, _vp]),
+    ("fh_batch_last_error", C.c_char_p, []),
+    ("fh_batch_create", _vp, [_i, _ip, _ip, _dp]),
+    ("fh_batch_destroy", None, [_vp]),
+    ("fh_batch_upload", _i, [_vp, _i, _dp, _dp, _dp, _u8p, _dp]),
+    ("fh_batch_set_forces", _i, [_vp, _i, _d, _d, _dp]),
+    ("fh_batch_step", _i, [_vp, _i, _d, _d, _d, _d, _i, _i]),
+    ("fh_batch_results", _i, [_vp, _dp, _ip, _dp]),
+    ("fh_batch_read", _i, [_vp, _i, _dp, _dp, _dp, _dp]),
+    ("fh_batch_last_step_ms", _i, [_vp, _fp]),
+    ("fh_batch_cells", C.c_longlong, [_vp]),
+    ("fh_batch_num_tiles", _i, []),
+    ("fh_batch_tile_name", C.c_char_p, [_i]),
+    ("fh_batch_set_tile", _i, [_vp, _i, _i]),
+    ("fh_batch_get_tiles", _i, [_vp, _ip, _ip]),
+    ("fh_batch_tune", _i, [_vp, _i, _fp]),
 ]
 
 KERNELS = {"rb_sweep_fine": 1, "advect_faces": 2, "maccormack_faces": 3, "advect_dye": 4,
@@ -125,6 +140,16 @@ def check(status: int) -> None:
     if status == FH_OK:
         return
     msg = last_error()
+    if status == FH_ERR_DEVICE_LOST:
+        msg = "GPU device lost"
+    raise HipError(status, msg)
+
+
+def check_batch(status: int) -> None:
+    """check() for the fh_batch_* entry points (their own error string)."""
+    if status == FH_OK:
+        return
+    msg = (lib().fh_batch_last_error() or b"").decode()
     if status == FH_ERR_DEVICE_LOST:
         msg = "GPU device lost"
     raise HipError(status, msg)

@@ -11,6 +11,8 @@ from __future__ import annotations
 from dataclasses import dataclass, field
 from typing import Callable, Optional
 
+import math
+
 import numpy as np
 
 FLUID, AIR, SOLID = 0, 1, 2
@@ -241,3 +243,101 @@ def karman_street(nx=2048, ny=1024, seed=20240611, perturb=1e-3) -> tuple[Grid, 
 
     return g, SceneSpec(labels=all_labels(open_right(), solid_disk(0.25 * L, 0.5, 0.05)),
                         seed=seed_fn)
+
+
+# ---- decaying turbulence among obstacles (batched workload) ------------------
+
+_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64_array(seed: int, n: int, skip: int = 0) -> np.ndarray:
+    """Outputs skip+1 .. skip+n of SplitMix64(seed), vectorised: the k-th output
+    is the mixing function of seed + k*gamma (mod 2^64), which is what
+    SplitMix64.next_u64 computes one at a time."""
+    k = np.arange(skip + 1, skip + n + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & SplitMix64.MASK) + k * _GAMMA
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def splitmix64_uniforms(seed: int, n: int, lo=0.0, hi=1.0) -> np.ndarray:
+    """SplitMix64.uniform(lo, hi), n at once."""
+    return lo + (hi - lo) * ((splitmix64_array(seed, n) >> np.uint64(11)).astype(np.float64)
+                             * (1.0 / (1 << 53)))
+
+
+def square_lattice(spacing: int, side: int, oi: int, oj: int):
+    """Solid squares of `side` cells on a lattice of pitch `spacing` cells,
+    offset (oi, oj): cell (i, j) is Solid when (i - oi) mod spacing < side and
+    (j - oj) mod spacing < side.  Square obstacles as obstacles.ts:49-60
+    rasterises them, in index space."""
+    def f(g: Grid, label: np.ndarray):
+        L = label.reshape(g.ny, g.nx)
+        mi = ((np.arange(g.nx) - oi) % spacing) < side
+        mj = ((np.arange(g.ny) - oj) % spacing) < side
+        L[np.ix_(mj, mi)] = SOLID
+    return f
+
+
+def spectral_turbulence(seed: int, slope: float = -3.0, rms: float = 1.0):
+    """Random solenoidal velocity with energy spectrum E(k) ~ k^slope.
+
+    The stream function is a sine series on the corners, psi(i, j) =
+    sum_{m,n} a_mn sin(pi m i / nx) sin(pi n j / ny), so psi = 0 on the walls
+    and no face on the boundary carries flow; a_mn = xi_mn k^((slope - 3)/2)
+    with k = |(m/nx, n/ny)| and xi ~ U(-1, 1) from SplitMix64(seed), since in
+    2D E(k) ~ k^3 |psi(k)|^2.  The velocity is its discrete curl
+    (testFields.ts:40-54, divergence-free to rounding), scaled to RMS `rms`
+    over all faces."""
+    from scipy.fft import idstn
+
+    def f(g: Grid, u: np.ndarray, v: np.ndarray):
+        m = np.arange(1, g.nx, dtype=np.float64)[None, :] / g.nx
+        n = np.arange(1, g.ny, dtype=np.float64)[:, None] / g.ny
+        k = np.sqrt(m * m + n * n)
+        xi = splitmix64_uniforms(seed, (g.ny - 1) * (g.nx - 1), -1.0, 1.0).reshape(g.ny - 1,
+                                                                                 g.nx - 1)
+        a = xi * k ** ((slope - 3.0) / 2.0)
+        psi = np.zeros((g.ny + 1, g.nx + 1))
+        psi[1:-1, 1:-1] = idstn(a, type=1)
+        uu = np.zeros_like(u)
+        vv = np.zeros_like(v)
+        add_curl_of_stream(g, uu, vv, psi, 1.0)
+        s = math.sqrt((np.sum(uu * uu) + np.sum(vv * vv)) / (uu.size + vv.size))
+        if s > 0:
+            u += uu * (rms / s)
+            v += vv * (rms / s)
+    return f
+
+
+BATCH_SHAPES = ((256, 256), (512, 512), (1024, 512), (1024, 1024))
+
+
+def decaying_turbulence(nx: int, ny: int, seed: int) -> tuple[Grid, SceneSpec]:
+    """Closed box (walls all round) with a seeded lattice of square obstacles,
+    pitch U[32, 64] cells and area blockage U[0.25, 0.40], and a k^-3
+    solenoidal velocity of RMS 1 (spectral_turbulence), all from
+    SplitMix64(seed)."""
+    g = Grid(nx, ny, 1.0 / ny)
+    rng = SplitMix64(seed)
+    spacing = rng.randint(32, 64)
+    blockage = rng.uniform(0.25, 0.40)
+    side = max(1, int(round(spacing * math.sqrt(blockage))))
+    oi, oj = rng.randint(0, spacing - 1), rng.randint(0, spacing - 1)
+    vel_seed = rng.next_u64()
+    return g, SceneSpec(labels=square_lattice(spacing, side, oi, oj),
+                        seed=spectral_turbulence(vel_seed))
+
+
+def turbulence_batch(count: int = 64, seed: int = 20240612,
+                     shapes=BATCH_SHAPES) -> list[tuple[Grid, SceneSpec]]:
+    """`count` decaying-turbulence scenes with sizes drawn from `shapes` by
+    SplitMix64(seed); scene k uses the seed drawn after its shape."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        nx, ny = shapes[rng.randint(0, len(shapes) - 1)]
+        out.append(decaying_turbulence(nx, ny, rng.next_u64()))
+    return out

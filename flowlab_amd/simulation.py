@@ -230,3 +230,20 @@ def apply_dye_patch(dg: Grid, dye: list, patch: DyePatch) -> None:
         vs = val[j0 - patch.j0:j1 - patch.j0, i0 - patch.i0:i1 - patch.i0]
         D = dye[c].reshape(dg.ny, dg.nx)[j0:j1, i0:i1]
         D[m] += cs[m] * (vs[m] - D[m])
+
+
+def scene_fields(g: Grid, scene: SceneSpec) -> Fields:
+    """The host fields Simulation.reset builds (simulation.ts:388-416) for a
+    scene without a dye source: labels, velocity seed, dye seed, then
+    commitLabels.  For callers that own no stepper (batches)."""
+    n = g.nx * g.ny
+    f = Fields(p=np.zeros(n), u=np.zeros((g.nx + 1) * g.ny), v=np.zeros(g.nx * (g.ny + 1)),
+               label=np.full(n, FLUID, np.uint8), dye=[np.zeros(n) for _ in range(3)])
+    if scene.labels:
+        scene.labels(g, f.label)
+    if scene.seed:
+        scene.seed(g, f.u, f.v)
+    if scene.dye:
+        scene.dye(g, f.dye)
+    commit_labels(g, f)
+    return f

@@ -209,6 +209,49 @@ int fh_last_solve_ms(fh_engine* e, double* ms);
 /* NULL = start from the previous pressure (the reference's warm start). */
 int fh_set_warm_start(fh_engine* e, fh_unet* u);
 
+/* ---- Batched scenes ------------------------------------------------------
+ * No counterpart in the reference, which steps one grid per Simulation
+ * (simulation.ts:467-587); its nearest relative is the one-submit recording
+ * of a fused step (stepGpu.ts:441-542).  A batch holds up to 255 independent
+ * grids of any sizes (dyeScale 1, fixed at creation) and advances all of
+ * them with one launch per phase: each scene takes dt from its own CFL
+ * (simulation.ts:477-479) and solves its pressure by FP64 defect-correction
+ * multigrid (FH_SOLVER_MG_DC) to its own convergence.  Per-scene fields are
+ * uploaded and read in the reference's host layout, FP64, as fh_upload. */
+typedef struct fh_batch fh_batch;
+const char* fh_batch_last_error(void);
+/* NULL on failure (fh_batch_last_error says why). */
+fh_batch* fh_batch_create(int nscenes, const int* nx, const int* ny, const double* h);
+void fh_batch_destroy(fh_batch* b);
+/* Any pointer may be NULL (field unchanged).  Labels rebuild the scene's
+ * multigrid hierarchy; call with labels at least once per scene. */
+int fh_batch_upload(fh_batch* b, int scene, const double* u, const double* v, const double* p,
+                    const uint8_t* label, const double* dye);
+/* Buoyancy weight, confinement epsilon, dye decay rates (decay may be NULL). */
+int fh_batch_set_forces(fh_batch* b, int scene, double buoyancy, double confinement,
+                        const double* decay);
+/* One step of every scene: dt = min(dt_max, cfl h / max face speed),
+ * scale = rho h^2/dt, gradScale = dt/(rho h).  Pressure: V-cycles until the
+ * relative residual <= tol (tol <= 0: exactly max_cycles).  graph_cycles > 0
+ * replays the step with that many cycles as one captured hipGraph (further
+ * cycles, if needed, one graph each); 0 launches the kernels directly. */
+int fh_batch_step(fh_batch* b, int scheme, double cfl, double dt_max, double rho, double tol,
+                  int max_cycles, int graph_cycles);
+/* Per scene of the last step (arrays of nscenes, any may be NULL). */
+int fh_batch_results(fh_batch* b, double* dt, int* iters, double* resid);
+int fh_batch_read(fh_batch* b, int scene, double* u, double* v, double* p, double* dye);
+int fh_batch_last_step_ms(fh_batch* b, float* ms);
+long long fh_batch_cells(fh_batch* b);
+/* Multigrid tile shapes: fh_batch_num_tiles() candidates by name; level -1
+ * sets every tiled level.  Results do not depend on the shape. */
+int fh_batch_num_tiles(void);
+const char* fh_batch_tile_name(int shape);
+int fh_batch_set_tile(fh_batch* b, int level, int shape);
+int fh_batch_get_tiles(fh_batch* b, int* shapes, int* nlevels);
+/* Autotuner: per tiled level, times every candidate over the batch (reps
+ * passes each) and keeps the fastest; ms_out gets nlevels x num_tiles ms. */
+int fh_batch_tune(fh_batch* b, int reps, float* ms_out);
+
 #ifdef __cplusplus
 }
 #endif

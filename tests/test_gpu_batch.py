@@ -1,0 +1,150 @@
+"""Batched scenes on the GPU (fh_batch_*): one launch per phase for scenes of
+uneven size, device-side dt and convergence, a captured hipGraph step and the
+tile-shape autotuner.
+
+The batch runs the single-grid engine's per-cell code, so each scene must
+equal the same scene stepped alone bit for bit, and both are held to the
+FP64 oracle at the per-kernel bar (relative L2 <= 1e-5 in FP32) with every
+pressure solve at 1e-6 by the oracle's own residual.
+"""
+import numpy as np
+import pytest
+
+from flowlab_amd import scenes as S
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+# The batched kernels took a GPU memory aperture fault on their first device
+# run (k_b_maccormack_faces reported, step 0 of the first test below) and the
+# cause is not yet shown; fh_batch_step and fh_batch_tune refuse to launch
+# them until it is.  These tests stay in place for that run.
+OFF = pytest.mark.skip(reason="batched kernels switched off: an unexplained GPU memory fault "
+                              "on their first run")
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def small_scenes():
+    """Uneven sizes, closed boxes with obstacles and one open outlet."""
+    return [S.decaying_turbulence(64, 64, 11), S.decaying_turbulence(128, 64, 12),
+            S.karman_street(96, 48), S.decaying_turbulence(64, 128, 13)]
+
+
+def make_batch(scenes):
+    from flowlab_amd.batch import HipBatch
+    b = HipBatch([g for g, _ in scenes])
+    fields = [b.reset(s, sc) for s, (_, sc) in enumerate(scenes)]
+    return b, fields
+
+
+@OFF
+def test_batch_equals_single_engine_bitwise():
+    from flowlab_amd.simulation import Simulation
+    scenes = small_scenes()
+    b, _ = make_batch(scenes)
+    sims = []
+    for g, sc in scenes:
+        sim = Simulation(g.nx, g.ny, dict(scheme="macCormack", cfl_target=0.5, dt_max=1.0,
+                                         solver="mg-dc", pressure_iters=60, tol=TOL,
+                                         residual_every=0))
+        sim.reset(sc)
+        sims.append(sim)
+    for n in range(3):
+        print(f"step {n}", flush=True)
+        b.step(cfl=0.5, dt_max=1.0, tol=TOL, max_cycles=60, graph_cycles=0)
+        dt, iters, res = b.results()
+        for s, sim in enumerate(sims):
+            sim.step()
+            u, v, p, dye = b.read(s)
+            assert dt[s] == sim.dt
+            assert iters[s] == sim.iters, (s, iters[s], sim.iters)
+            assert np.array_equal(u, sim.f.u) and np.array_equal(v, sim.f.v), s
+            assert np.array_equal(p, sim.stepper.engine.read_pressure()), s
+            assert res[s] <= TOL
+
+
+@OFF
+def test_batch_matches_oracle():
+    scenes = small_scenes()
+    b, _ = make_batch(scenes)
+    prm = O.step_params(scheme="macCormack", solver="mg-dc", pressure_iters=60, tol=TOL,
+                        cfl_target=0.5, dt_max=1.0, residual_every=0)
+    osims = []
+    for g, sc in scenes:
+        o = O.OracleSim(g.nx, g.ny)
+        o.reset(g, sc)
+        osims.append(o)
+    for n in range(2):
+        b.step(cfl=0.5, dt_max=1.0, tol=TOL, max_cycles=60, graph_cycles=4)
+        dt, iters, res = b.results()
+        for s, (g, _) in enumerate(scenes):
+            osims[s].step(prm)
+            u, v, p, _ = b.read(s)
+            assert abs(dt[s] - prm.dt_used) <= 1e-6 * prm.dt_used
+            assert abs(iters[s] - prm.iters) <= 1, (s, iters[s], prm.iters)
+            assert rel_l2(u, osims[s].u) < 1e-5, s
+            assert rel_l2(v, osims[s].v) < 1e-4, s
+            assert res[s] <= TOL
+
+
+@OFF
+def test_graph_replays_equal_eager_steps():
+    """hipGraph replays (with eager host work between them) give exactly the
+    eager launches' fields, step after step; 3 cycles in the graph leave most
+    scenes to the one-cycle graph, so both captured pieces are exercised."""
+    scenes = small_scenes()
+    a, _ = make_batch(scenes)
+    g, _ = make_batch(scenes)
+    for n in range(4):
+        print(f"step {n}", flush=True)
+        a.step(tol=TOL, graph_cycles=0)
+        g.step(tol=TOL, graph_cycles=3)
+        ra, rg = a.results(), g.results()
+        assert np.array_equal(ra[1], rg[1]) and np.array_equal(ra[0], rg[0])
+        for s in range(len(scenes)):
+            for x, y in zip(a.read(s)[:3], g.read(s)[:3]):
+                assert np.array_equal(x, y), (n, s)
+
+
+@OFF
+def test_tile_shapes_and_autotuner_do_not_change_results():
+    from flowlab_amd.batch import tile_names
+    scenes = small_scenes()[:2]
+    ref = None
+    for shape in tile_names():
+        b, _ = make_batch(scenes)
+        b.set_tile(shape)
+        assert set(b.tiles()) == {shape}
+        for _ in range(2):
+            b.step(tol=TOL, graph_cycles=0)
+        out = [b.read(s)[:3] for s in range(len(scenes))]
+        if ref is None:
+            ref = out
+        else:
+            for x, y in zip(ref, out):
+                for a_, b_ in zip(x, y):
+                    assert np.array_equal(a_, b_), shape
+    b, _ = make_batch(scenes)
+    rep = b.tune(reps=2)
+    assert len(rep["tiles"]) >= 1 and all(t in tile_names() for t in rep["tiles"])
+    assert all(v > 0 for lvl in rep["ms"] for v in lvl.values())
+    for _ in range(2):
+        b.step(tol=TOL, graph_cycles=0)
+    for x, y in zip(ref, [b.read(s)[:3] for s in range(len(scenes))]):
+        for a_, b_ in zip(x, y):
+            assert np.array_equal(a_, b_)
+
+
+def test_batch_rejects_bad_arguments():
+    from flowlab_amd import hip
+    from flowlab_amd.batch import HipBatch
+    b = HipBatch([S.Grid(32, 32, 1 / 32)])
+    with pytest.raises(hip.HipError):
+        b.step(cfl=-1.0)
+    with pytest.raises(hip.HipError):
+        b.set_tile(99)
