@@ -66,7 +66,9 @@ def parse():
     return ap.parse_args()
 
 
-def run_gpu(args):
+def run_gpu(args, unet=None, trace=True):
+    """The Karman workload: W warm-up steps, then K timed steps.  unet: solve
+    every pressure from the learned warm start instead of the previous p."""
     from flowlab_amd import scenes as S
     from flowlab_amd.simulation import Simulation
 
@@ -77,10 +79,13 @@ def run_gpu(args):
     sim = Simulation(nx, ny, params, readback=False)
     sim.reset(scene)
     e = sim.stepper.engine
+    if unet is not None:
+        e.set_warm_start(unet)
     for _ in range(args.warmup):
         sim.step()
     e.set_timing(True)
-    e.trace_kernel(args.kernel)
+    if trace:
+        e.trace_kernel(args.kernel)
     e.synchronize()
     solve_ms, iters, hist_max = [], [], 0.0
     t0 = time.perf_counter()
@@ -92,11 +97,37 @@ def run_gpu(args):
         hist_max = max(hist_max, float(e.solve_history()[-1]))
     e.synchronize()
     elapsed = time.perf_counter() - t0
-    n_launch, k_ms = e.trace_stats()
-    e.trace_kernel(None)
+    n_launch, k_ms, bracket_ms = 0, 0.0, 0.0
+    if trace:
+        n_launch, k_ms = e.trace_stats()
+        bracket_ms = e.trace_overhead(200)
+        e.trace_kernel(None)
     e.set_timing(False)
+    e.set_warm_start(None)
     return dict(sim=sim, elapsed=elapsed, solve_ms=solve_ms, iters=iters, hist_max=hist_max,
-                n_launch=n_launch, k_ms=k_ms, g=g)
+                n_launch=n_launch, k_ms=k_ms, bracket_ms=bracket_ms, g=g)
+
+
+def run_learned(args, plain):
+    """The same workload with the trained U-Net warm start (weights in
+    flowlab_amd/weights), against the plain run's solves."""
+    from flowlab_amd import learned as L
+    if not os.path.exists(L.WEIGHTS):
+        return None
+    u = L.UNet()
+    u.load(L.WEIGHTS)
+    r = run_gpu(args, unet=u, trace=False)
+    K = args.steps
+    ms_l = sum(r["solve_ms"]) / K
+    ms_p = sum(plain["solve_ms"]) / K
+    return {
+        "ms_per_solve": ms_l,
+        "vcycles_per_solve": sum(r["iters"]) / K,
+        "max_relative_residual": r["hist_max"],
+        "solves_per_s_vs_plain": ms_p / ms_l,
+        "cell_steps_per_s": args.nx * args.ny * K / r["elapsed"],
+        "weights": os.path.relpath(L.WEIGHTS, ROOT),
+    }
 
 
 def run_batched(args):
@@ -170,11 +201,14 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     r = run_gpu(args)
+    learned = run_learned(args, r)
     batched = run_batched(args) if args.batch_scenes > 0 else None
     nx, ny, K = args.nx, args.ny, args.steps
     cells = nx * ny
     value = cells * K / r["elapsed"]
-    k_avg_ms = r["k_ms"] / max(r["n_launch"], 1)
+    # per-launch device time: event bracket around each launch, less the
+    # bracket's own cost (an empty event pair on the same stream)
+    k_avg_ms = r["k_ms"] / max(r["n_launch"], 1) - r["bracket_ms"]
     bytes_per_launch = ALGO_BYTES_PER_CELL[args.kernel] * cells
     achieved = bytes_per_launch / (k_avg_ms * 1e-3) / 1e9 if r["n_launch"] else None
     out = {
@@ -201,7 +235,7 @@ def main():
             "ms_per_solve": sum(r["solve_ms"]) / K,
             "vcycles_per_solve": sum(r["iters"]) / K,
             "max_relative_residual": r["hist_max"],
-            "learned_warm_start": None,
+            "learned_warm_start": learned,
         },
         "roofline": {
             "kernel": args.kernel,
@@ -212,6 +246,7 @@ def main():
             "frac": None if achieved is None else achieved / HBM_PEAK_GBS,
             "traffic": None,
             "avg_launch_ms": k_avg_ms,
+            "event_bracket_ms": r["bracket_ms"],
             "launches": r["n_launch"],
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },

@@ -20,8 +20,8 @@ namespace fh {
 int engine_capture_hook(fh_unet* u, hipStream_t s, const float* r32, const uint8_t* lbl,
                         const double* p_prev, const double* p_final, long n, int H, int W,
                         bool begin);
-int engine_warm_start(fh_unet* u, hipStream_t s, const float* r32, const uint8_t* lbl, double* p,
-                      int H, int W, float* stats2);
+int engine_warm_start(fh_unet* u, hipStream_t s, const float* r32, const uint8_t* lbl,
+                      const uint8_t* info, double* p, int H, int W, float* stats2);
 }  // namespace fh
 
 using namespace fh;
@@ -523,7 +523,8 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
     }
     if (e->warm) {
       // learned warm start: p += net correction, then the defect again
-      st = engine_warm_start(e->warm, e->s, L0.b, e->lbl, e->p, e->ny, e->nx, e->wstats);
+      st = engine_warm_start(e->warm, e->s, L0.b, e->lbl, L0.info, e->p, e->ny, e->nx,
+                             e->wstats);
       if (st != FH_OK) return st;
       launch_dc_update(e->s, e->p, e->p2, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
                        e->partials, e->sums, nullptr, nullptr, 0.0);
@@ -1160,6 +1161,30 @@ int fh_trace_stats(fh_engine* e, int* count, double* total_ms) {
   }
   *count = e->trace_n;
   *total_ms = tot;
+  return FH_OK;
+}
+
+// Device ms of an event pair with nothing between it, averaged over `reps`
+// pairs recorded back to back on the engine's stream: the bracket cost that
+// fh_trace_stats' per-launch times include.
+int fh_trace_overhead(fh_engine* e, int reps, double* ms) {
+  GUARD("traceOverhead()");
+  if (!ms || reps < 1) return FH_ERR_BAD_ARG;
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  double tot = 0;
+  for (int r = 0; r < reps; r++) {
+    HIPCHK(hipEventRecord(a, e->s));
+    HIPCHK(hipEventRecord(b, e->s));
+    HIPCHK(hipEventSynchronize(b));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, a, b));
+    tot += t;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *ms = tot / reps;
   return FH_OK;
 }
 

@@ -144,13 +144,43 @@ __global__ void k_head_grad(const float* __restrict__ dout, long n, bf16_t* __re
   dhead[i * 16] = f2bf_(dout[i]);
 }
 
-// p += s_r c o on fluid cells (FP64 pressure).
+// The warm start is a line search along the network output o: with the
+// defect r0 = b - A p_prev and q = A o (o counted as 0 off the fluid, where
+// the correction is 0), the step m = <r0, q> / <q, q> (at least 0) minimises
+// |b - A (p_prev + m o)| over fluid cells, so the warm-started residual is
+// never larger than the previous pressure's, and the output scale drops out.
+// k_warm_dots: block partial sums {r0.q, q.q} over fluid cells with an open
+// neighbour (info byte: bit 7 fluid, bits 3..6 open +x,-x,+y,-y, 0..2 count).
+__global__ __launch_bounds__(256) void k_warm_dots(const float* __restrict__ r0,
+                                                   const float* __restrict__ o,
+                                                   const uint8_t* __restrict__ info, int nx,
+                                                   long n, double* partials) {
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n;
+       k += (long)gridDim.x * blockDim.x) {
+    const int f = info[k];
+    if (!(f & 0x80) || (f & 7) == 0) continue;
+    // an open neighbour is in the domain, so its index is valid
+    float nb = 0.f;
+    if ((f & 0x08) && (info[k + 1] & 0x80)) nb += o[k + 1];
+    if ((f & 0x10) && (info[k - 1] & 0x80)) nb += o[k - 1];
+    if ((f & 0x20) && (info[k + nx] & 0x80)) nb += o[k + nx];
+    if ((f & 0x40) && (info[k - nx] & 0x80)) nb += o[k - nx];
+    const double q = (double)(f & 7) * (double)o[k] - (double)nb;
+    acc[0] += (double)r0[k] * q;
+    acc[1] += q * q;
+  }
+  block_sum_store<4>(acc, partials);
+}
+
+// p += m o on fluid cells, m from the finalized sums {r0.q, q.q}.
 __global__ void k_apply_warm(double* __restrict__ p, const float* __restrict__ o,
-                             const uint8_t* __restrict__ lbl, const float* __restrict__ stats,
-                             long n, float c) {
+                             const uint8_t* __restrict__ info, const double* __restrict__ sums,
+                             long n) {
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (lbl[i] == FLUID) p[i] += (double)stats[0] * (double)c * (double)o[i];
+  const double m = sums[1] > 0.0 ? fmax(sums[0] / sums[1], 0.0) : 0.0;
+  if (info[i] & 0x80) p[i] += m * (double)o[i];
 }
 
 // Host input [N][H][W][8] f32 -> bf16
@@ -435,8 +465,8 @@ namespace fh {
 int engine_capture_hook(fh_unet* u, hipStream_t s, const float* r32, const uint8_t* lbl,
                         const double* p_prev, const double* p_final, long n, int H, int W,
                         bool begin);
-int engine_warm_start(fh_unet* u, hipStream_t s, const float* r32, const uint8_t* lbl, double* p,
-                      int H, int W, float* stats2);
+int engine_warm_start(fh_unet* u, hipStream_t s, const float* r32, const uint8_t* lbl,
+                      const uint8_t* info, double* p, int H, int W, float* stats2);
 }  // namespace fh
 
 static thread_local std::string g_unet_err;
@@ -483,8 +513,8 @@ int engine_capture_hook(fh_unet* u, hipStream_t s, const float* r32, const uint8
   return FH_OK;
 }
 
-int engine_warm_start(fh_unet* u, hipStream_t s, const float* r32, const uint8_t* lbl, double* p,
-                      int H, int W, float* stats2) {
+int engine_warm_start(fh_unet* u, hipStream_t s, const float* r32, const uint8_t* lbl,
+                      const uint8_t* info, double* p, int H, int W, float* stats2) {
   UNet& net = u->net;
   if ((H % 8) || (W % 8)) return FH_OK;  // three 2x poolings need multiples of 8
   UCHK(unet_resize(net, 1, H, W));
@@ -495,7 +525,9 @@ int engine_warm_start(fh_unet* u, hipStream_t s, const float* r32, const uint8_t
   k_prep<double><<<g, 256, 0, s>>>(r32, p, lbl, nullptr, stats2, nullptr, n, H, W, c,
                                    net.act[0], nullptr, nullptr);
   unet_forward(net, s);
-  k_apply_warm<<<nblk(n), 256, 0, s>>>(p, net.out, lbl, stats2, n, c);
+  k_warm_dots<<<512, 256, 0, s>>>(r32, net.out, info, W, n, u->partials);
+  launch_finalize_sum4(s, u->partials, 512, u->sums);
+  k_apply_warm<<<nblk(n), 256, 0, s>>>(p, net.out, info, u->sums, n);
   UCHK(hipGetLastError());
   return FH_OK;
 }

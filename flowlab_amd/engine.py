@@ -185,6 +185,12 @@ class HipEngine:
         check(self.lib.fh_trace_stats(self._e, C.byref(n), C.byref(ms)))
         return n.value, ms.value
 
+    def trace_overhead(self, reps: int = 200) -> float:
+        """Device ms of an empty event bracket on this engine's stream."""
+        ms = C.c_double()
+        check(self.lib.fh_trace_overhead(self._e, reps, C.byref(ms)))
+        return ms.value
+
 
 class HipStepper:
     """GpuStep (simulation.ts:35-84) on libflowhip: the whole advect ->

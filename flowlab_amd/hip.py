@@ -61,6 +61,7 @@ SIGNATURES = [
     ("fh_synchronize", _i, [_vp]),
     ("fh_trace_kernel", _i, [_vp, _i]),
     ("fh_trace_stats", _i, [_vp, _ip, _dp]),
+    ("fh_trace_overhead", _i, [_vp, _i, _dp]),
     ("fh_unet_create", _vp, [C.c_ulonglong]),
     ("fh_unet_destroy", None, [_vp]),
     ("fh_unet_last_error", C.c_char_p, []),

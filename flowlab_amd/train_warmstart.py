@@ -38,7 +38,22 @@ def main(argv=None):
     ap.add_argument("--out", default=L.WEIGHTS)
     ap.add_argument("--report", default=None)
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--eval-only", action="store_true",
+                    help="load --out and only evaluate it")
     a = ap.parse_args(argv)
+
+    if a.eval_only:
+        u = L.UNet()
+        u.load(a.out)
+        evals = {}
+        for n in [int(s) for s in a.eval_sizes.split(",") if s]:
+            evals[str(n)] = L.evaluate(u, n, scenes=a.eval_scenes)
+            print(f"eval {n}^2: " + ", ".join(f"{k}: {v['vcycles']:.2f} cycles {v['ms']:.3f} ms"
+                                            for k, v in evals[str(n)].items()), flush=True)
+        if a.report:
+            with open(a.report, "w") as f:
+                json.dump(dict(weights=a.out, eval=evals), f, indent=1)
+        return
 
     t0 = time.perf_counter()
     u = L.UNet(seed=a.seed)

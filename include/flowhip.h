@@ -167,6 +167,9 @@ enum {
 };
 int fh_trace_kernel(fh_engine* e, int kernel);
 int fh_trace_stats(fh_engine* e, int* count, double* total_ms);
+/* Device ms of an empty event pair on the engine's stream (mean of reps):
+ * the part of each fh_trace_stats launch time that is the bracket itself. */
+int fh_trace_overhead(fh_engine* e, int reps, double* ms);
 
 /* ---- learned warm start (no reference counterpart) ---------------------
  * A 4-level convolutional U-Net (3x3 convs, channels 16-32-64-64, BF16

@@ -93,3 +93,28 @@ def test_capture_train_and_warm_start():
     p, d = e.read_pressure(), e.read_divergence()
     rel = O.relative_residual(n, n, p, d, sim.f.label, sim.g.h * sim.g.h / sim.dt)
     assert rel <= 1e-6
+    # the warm start is a residual-minimising step along the network output,
+    # so its defect is never above the previous pressure's (same inputs)
+    from flowlab_amd.engine import HipPressureSolver
+    sim2 = Lrn._sim(n, 321)
+    for _ in range(4):
+        sim2.step()
+    e2 = sim2.stepper.engine
+    p_prev = e2.read_pressure()
+    sim2.step()
+    div = e2.read_divergence()
+    label = sim2.f.label.copy()
+    scale = sim2.g.h * sim2.g.h / sim2.dt
+    solver = HipPressureSolver(n, n, "mg-dc")
+    first = {}
+    for name in ("previous", "learned"):
+        solver.engine.set_warm_start(u if name == "learned" else None)
+        p = p_prev.copy()
+        solver.solve(None, p, div, label, scale, 1, 1.0, 0)  # labels, buffers
+        p = p_prev.copy()
+        solver.solve(None, p, div, label, scale, 100, 1.0, 1e-6)
+        first[name] = float(solver.engine.solve_history()[0])
+        rel = O.relative_residual(n, n, p, div, label, scale)
+        assert rel <= 1e-6, name
+    solver.engine.set_warm_start(None)
+    assert first["learned"] <= first["previous"] * (1 + 1e-9), first
