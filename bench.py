@@ -63,7 +63,27 @@ def parse():
                     help="scenes of the batched decaying-turbulence run (0 = skip; the "
                          "batched kernels are switched off in this build)")
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--large-n", type=int, default=8192,
+                    help="side of the large single-domain plume (0 = skip)")
+    ap.add_argument("--large-steps", type=int, default=4)
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="first run 3 device-to-device copies of this many MiB (the known-size "
+                         "copy kernel that scales hardware traffic counters)")
     return ap.parse_args()
+
+
+PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_karman_2048x1024.json")
+
+
+def measured_traffic(kernel, nx, ny):
+    """HBM bytes per launch of `kernel` from the committed counter passes
+    (FETCH_SIZE and WRITE_SIZE, scaled by a known-size copy), for the
+    2048x1024 workload they were taken on; None otherwise."""
+    if (nx, ny) != (2048, 1024) or not os.path.exists(PMC_TRAFFIC):
+        return None
+    with open(PMC_TRAFFIC) as f:
+        k = json.load(f)["kernels"].get(kernel)
+    return None if k is None else k["traffic_bytes"]
 
 
 def run_gpu(args, unet=None, trace=True):
@@ -175,6 +195,67 @@ def run_batched(args):
     }
 
 
+LARGE_KERNELS = ("dc_update", "smooth_restrict_fine", "prolong_smooth_fine", "maccormack_faces",
+                 "advect_faces", "maccormack_dye", "advect_dye")
+
+
+def run_large(args):
+    """One n x n domain (default 8192^2, 67 M cells): closed box, 200 seeded
+    circular obstacles, 16 buoyant heat sources (buoyancy 1), MacCormack,
+    MG-DC to 1e-6.  Times K steps, then traces each large kernel for one
+    step and reports its algorithmic GB/s against the HBM peak."""
+    from flowlab_amd import scenes as S
+    from flowlab_amd.simulation import Simulation
+
+    n = args.large_n
+    g, scene = S.large_plume(n)
+    params = dict(scheme="macCormack", cfl_target=0.5, dt_max=0.5 * g.h, solver="mg-dc",
+                  pressure_iters=100, tol=1e-6, residual_every=0, buoyancy=1.0)
+    sim = Simulation(n, n, params, readback=False)
+    sim.reset(scene)
+    e = sim.stepper.engine
+    for _ in range(2):
+        sim.step()
+    e.set_timing(True)
+    e.synchronize()
+    solve_ms, iters, res = [], [], 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.large_steps):
+        sim.step()
+        solve_ms.append(e.timings()["solve"])
+        iters.append(sim.iters)
+        res = max(res, float(e.solve_history()[-1]))
+    e.synchronize()
+    elapsed = time.perf_counter() - t0
+    bracket = e.trace_overhead(200)
+    kernels = {}
+    for k in LARGE_KERNELS:
+        e.trace_kernel(k)
+        sim.step()
+        cnt, ms = e.trace_stats()
+        e.trace_kernel(None)
+        if cnt:
+            avg = ms / cnt - bracket
+            gbs = ALGO_BYTES_PER_CELL[k] * n * n / (avg * 1e-3) / 1e9
+            kernels[k] = {"avg_launch_ms": avg, "launches": cnt, "achieved_GBs": gbs,
+                          "frac": gbs / HBM_PEAK_GBS}
+    e.set_timing(False)
+    K = args.large_steps
+    return {
+        "workload": f"thermal plume {n}x{n}, closed box, 200 circular obstacles, 16 buoyant "
+                    "sources, MacCormack, MG-DC to 1e-6, SplitMix64(20240613)",
+        "cells": n * n,
+        "steps": K,
+        "value": n * n * K / elapsed,
+        "unit": "cell-steps/s",
+        "ms_per_step": elapsed * 1e3 / K,
+        "ms_per_solve": sum(solve_ms) / K,
+        "vcycles_per_solve": sum(iters) / K,
+        "max_relative_residual": res,
+        "kernels": kernels,
+    }
+
+
 def run_cpu_baseline(args):
     from flowlab_amd import scenes as S
     from oracle import pyoracle as O
@@ -200,9 +281,19 @@ def main():
               "(the multi-GPU decomposition is a design note, not a measured path)",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_mb > 0:
+        import torch
+        n = args.copy_mb * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     r = run_gpu(args)
     learned = run_learned(args, r)
     batched = run_batched(args) if args.batch_scenes > 0 else None
+    large = run_large(args) if args.large_n > 0 else None
     nx, ny, K = args.nx, args.ny, args.steps
     cells = nx * ny
     value = cells * K / r["elapsed"]
@@ -244,7 +335,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": None if achieved is None else achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.kernel, nx, ny),
+            "traffic_source": os.path.relpath(PMC_TRAFFIC, ROOT),
             "avg_launch_ms": k_avg_ms,
             "event_bracket_ms": r["bracket_ms"],
             "launches": r["n_launch"],
@@ -253,6 +345,8 @@ def main():
     }
     if batched is not None:
         out["batched"] = batched
+    if large is not None:
+        out["large"] = large
     if args.cpu_steps > 0:
         v, secs = run_cpu_baseline(args)
         out["cpu_baseline"] = {

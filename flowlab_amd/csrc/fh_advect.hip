@@ -19,32 +19,40 @@ __global__ __launch_bounds__(256) void k_advect_faces(
     const float* __restrict__ uc, const float* __restrict__ vc, const float* __restrict__ us,
     const float* __restrict__ vs, float* __restrict__ uo, float* __restrict__ vo,
     const uint8_t* __restrict__ lbl, int nx, int ny, float c) {
-  advect_faces_at(uc, vc, us, vs, uo, vo, lbl, nx, ny, c, blockIdx.x * BX + threadIdx.x,
-                  blockIdx.y * BY + threadIdx.y);
+  int bx, by;
+  xcd_block(&bx, &by);
+  advect_faces_at(uc, vc, us, vs, uo, vo, lbl, nx, ny, c,
+                  bx * BX + threadIdx.x, by * BY + threadIdx.y);
 }
 
 __global__ __launch_bounds__(256) void k_maccormack_faces(
     const float* __restrict__ un, const float* __restrict__ vn, const float* __restrict__ uh,
     const float* __restrict__ vh, float* __restrict__ uo, float* __restrict__ vo,
     const uint8_t* __restrict__ lbl, int nx, int ny, float c) {
-  maccormack_faces_at(un, vn, uh, vh, uo, vo, lbl, nx, ny, c, blockIdx.x * BX + threadIdx.x,
-                      blockIdx.y * BY + threadIdx.y);
+  int bx, by;
+  xcd_block(&bx, &by);
+  maccormack_faces_at(un, vn, uh, vh, uo, vo, lbl, nx, ny, c,
+                      bx * BX + threadIdx.x, by * BY + threadIdx.y);
 }
 
 __global__ __launch_bounds__(256) void k_advect_dye(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ q,
     float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx, int ny, int ds, float c,
     float k0, float k1, float k2) {
-  advect_dye_at(u, v, q, qo, lbl, nx, ny, ds, c, k0, k1, k2, blockIdx.x * BX + threadIdx.x,
-                blockIdx.y * BY + threadIdx.y);
+  int bx, by;
+  xcd_block(&bx, &by);
+  advect_dye_at(u, v, q, qo, lbl, nx, ny, ds, c, k0, k1, k2,
+                bx * BX + threadIdx.x, by * BY + threadIdx.y);
 }
 
 __global__ __launch_bounds__(256) void k_maccormack_dye(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ qn,
     const float* __restrict__ qh, float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx,
     int ny, int ds, float c, float k0, float k1, float k2) {
+  int bx, by;
+  xcd_block(&bx, &by);
   maccormack_dye_at(u, v, qn, qh, qo, lbl, nx, ny, ds, c, k0, k1, k2,
-                    blockIdx.x * BX + threadIdx.x, blockIdx.y * BY + threadIdx.y);
+                    bx * BX + threadIdx.x, by * BY + threadIdx.y);
 }
 
 void launch_advect_faces(hipStream_t s, const float* uc, const float* vc, const float* us,

@@ -121,6 +121,19 @@ __device__ __forceinline__ void backtrace(const float* __restrict__ uc,
   *by = shift(y, -c * sample_v(vc, nx, ny, mx, my));
 }
 
+// Workgroups of a launch are dispatched round-robin over the chip's 8 XCDs,
+// each with its own L2.  Remapping the dispatch index gives every XCD a
+// contiguous run of logical blocks (row-major over the block grid), so the
+// halos that neighbouring blocks share are fetched into one L2 instead of
+// several.  Identity when the block count is not a multiple of 8.
+__device__ __forceinline__ void xcd_block(int* bx, int* by) {
+  const int gx = (int)gridDim.x, n = gx * (int)gridDim.y;
+  int b = (int)blockIdx.x + (int)blockIdx.y * gx;
+  if ((n & 7) == 0) b = (b & 7) * (n >> 3) + (b >> 3);
+  *by = b / gx;
+  *bx = b - *by * gx;
+}
+
 // ---- reductions ---------------------------------------------------------
 
 __device__ __forceinline__ double wave_sum(double v) {

@@ -4,6 +4,7 @@
 // convergence checks, readbacks, timings).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -42,7 +43,10 @@ struct Level {
   uint8_t* info; // per-cell Fluid flag + open-neighbour count (cell_info)
 };
 
-// LDS one workgroup may use for the coarse sub-cycle.
+// LDS the one-workgroup coarse sub-cycle may use.  64 KB: with gfx950's
+// 160 KB the 128x64 level of a 2048x1024 grid would fit too, but one
+// workgroup then takes longer over it (measured +31 us) than the two tiled
+// launches it replaces (17 us).
 constexpr size_t kCoarseLds = 65536;
 
 // pressureMultigrid.ts:65-78 with a configurable coarsest side.
@@ -422,7 +426,8 @@ CoarseLevels coarse_levels(const fh_engine* e, int lc, int nlev) {
 // First level from which the rest of the hierarchy fits one workgroup's LDS.
 int coarse_start(const fh_engine* e, int nlev) {
   for (int l = 0; l < nlev; l++)
-    if (nlev - l <= kMaxCoarse && coarse_vcycle_lds_bytes(coarse_levels(e, l, nlev)) <= kCoarseLds)
+    if (nlev - l <= kMaxCoarse &&
+        coarse_vcycle_lds_bytes(coarse_levels(e, l, nlev)) <= kCoarseLds)
       return l;
   return nlev - 1;
 }

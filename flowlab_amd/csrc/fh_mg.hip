@@ -188,9 +188,11 @@ __global__ __launch_bounds__(256) void k_dc_update(
     const double* __restrict__ div, const uint8_t* __restrict__ info, float* __restrict__ r32,
     int nx, int ny, double scale, double* partials, const int* __restrict__ done) {
   if (done && *done) return;
-  const int i = blockIdx.x * BX + threadIdx.x;
+  int bx, by;
+  xcd_block(&bx, &by);
+  const int i = bx * BX + threadIdx.x;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int j = blockIdx.y * BY + threadIdx.y; j < ny; j += gridDim.y * BY) {
+  for (int j = by * BY + threadIdx.y; j < ny; j += gridDim.y * BY) {
     if (i >= nx) break;
     const int k = i + j * nx;
     const int kE = i + 1 < nx ? k + 1 : k, kW = i > 0 ? k - 1 : k;
@@ -235,9 +237,11 @@ __global__ __launch_bounds__(256) void k_norms(const float* __restrict__ p32,
                                                const double* __restrict__ div,
                                                const uint8_t* __restrict__ lbl, int nx, int ny,
                                                double scale, double* partials) {
-  const int i = blockIdx.x * BX + threadIdx.x;
+  int bx, by;
+  xcd_block(&bx, &by);
+  const int i = bx * BX + threadIdx.x;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int j = blockIdx.y * BY + threadIdx.y; j < ny; j += gridDim.y * BY) {
+  for (int j = by * BY + threadIdx.y; j < ny; j += gridDim.y * BY) {
     if (i >= nx) break;
     int k = i + j * nx;
     if (lbl[k] != FLUID) continue;

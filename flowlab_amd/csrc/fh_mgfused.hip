@@ -33,8 +33,9 @@ __global__ __launch_bounds__(NT) void k_smooth_restrict(const float* __restrict_
                                                         int ny, float* __restrict__ bc, int cnx,
                                                         int cny, int zero,
                                                         const int* __restrict__ done) {
-  smooth_restrict_tile<TX, TY>(xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done, blockIdx.x,
-                               blockIdx.y);
+  int bx, by;
+  xcd_block(&bx, &by);
+  smooth_restrict_tile<TX, TY>(xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done, bx, by);
 }
 
 template <int TX, int TY>
@@ -45,8 +46,9 @@ __global__ __launch_bounds__(NT) void k_prolong_smooth(const float* __restrict__
                                                        int ny, const float* __restrict__ e,
                                                        int cnx, int cny,
                                                        const int* __restrict__ done) {
-  prolong_smooth_tile<TX, TY>(xin, xout, b, info, nx, ny, e, cnx, cny, done, blockIdx.x,
-                              blockIdx.y);
+  int bx, by;
+  xcd_block(&bx, &by);
+  prolong_smooth_tile<TX, TY>(xin, xout, b, info, nx, ny, e, cnx, cny, done, bx, by);
 }
 
 __global__ __launch_bounds__(NC) void k_coarse_vcycle(CoarseLevels L, const float* __restrict__ b0,

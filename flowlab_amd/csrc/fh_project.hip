@@ -16,8 +16,9 @@ __global__ __launch_bounds__(256) void k_vorticity(const float* __restrict__ u,
                                                    const float* __restrict__ v,
                                                    float* __restrict__ w, int nx, int ny,
                                                    float inv_h) {
-  vorticity_at(u, v, w, nx, ny, inv_h, blockIdx.x * BX + threadIdx.x,
-               blockIdx.y * BY + threadIdx.y);
+  int bx, by;
+  xcd_block(&bx, &by);
+  vorticity_at(u, v, w, nx, ny, inv_h, bx * BX + threadIdx.x, by * BY + threadIdx.y);
 }
 
 __global__ __launch_bounds__(256) void k_forces(float* __restrict__ u, float* __restrict__ v,
@@ -26,24 +27,28 @@ __global__ __launch_bounds__(256) void k_forces(float* __restrict__ u, float* __
                                                 const float* __restrict__ w, int nx, int ny,
                                                 int ds, float h, float dt, float gain, float eps,
                                                 StirBox st) {
-  forces_at(u, v, lbl, heat, w, nx, ny, ds, h, dt, gain, eps, st, blockIdx.x * BX + threadIdx.x,
-            blockIdx.y * BY + threadIdx.y);
+  int bx, by;
+  xcd_block(&bx, &by);
+  forces_at(u, v, lbl, heat, w, nx, ny, ds, h, dt, gain, eps, st,
+            bx * BX + threadIdx.x, by * BY + threadIdx.y);
 }
 
 __global__ __launch_bounds__(256) void k_divergence(const float* __restrict__ u,
                                                     const float* __restrict__ v,
                                                     double* __restrict__ div, int nx, int ny,
                                                     double inv_h) {
-  divergence_at(u, v, div, nx, ny, inv_h, blockIdx.x * BX + threadIdx.x,
-                blockIdx.y * BY + threadIdx.y);
+  int bx, by;
+  xcd_block(&bx, &by);
+  divergence_at(u, v, div, nx, ny, inv_h, bx * BX + threadIdx.x, by * BY + threadIdx.y);
 }
 
 __global__ __launch_bounds__(256) void k_subtract(float* __restrict__ u, float* __restrict__ v,
                                                   const double* __restrict__ p,
                                                   const uint8_t* __restrict__ lbl, int nx, int ny,
                                                   double gs) {
-  subtract_at(u, v, p, lbl, nx, ny, gs, blockIdx.x * BX + threadIdx.x,
-              blockIdx.y * BY + threadIdx.y);
+  int bx, by;
+  xcd_block(&bx, &by);
+  subtract_at(u, v, p, lbl, nx, ny, gs, bx * BX + threadIdx.x, by * BY + threadIdx.y);
 }
 
 __global__ __launch_bounds__(256) void k_outflow(float* __restrict__ u, float* __restrict__ v,

@@ -341,3 +341,57 @@ def turbulence_batch(count: int = 64, seed: int = 20240612,
         nx, ny = shapes[rng.randint(0, len(shapes) - 1)]
         out.append(decaying_turbulence(nx, ny, rng.next_u64()))
     return out
+
+
+# ---- large single domain: many sources and obstacles -------------------------
+
+def _disks(n_cells_x: int, n_cells_y: int, h: float, cx, cy, r, value):
+    """Rasterise disks (world units) by cell centre into an (ny, nx) mask-like
+    array, each disk evaluated only over its bounding box; later disks win."""
+    out = np.zeros((n_cells_y, n_cells_x), np.float32)
+    for x0, y0, rr, v in zip(cx, cy, r, value):
+        i0 = max(0, int(np.floor((x0 - rr) / h)) - 1)
+        i1 = min(n_cells_x, int(np.ceil((x0 + rr) / h)) + 1)
+        j0 = max(0, int(np.floor((y0 - rr) / h)) - 1)
+        j1 = min(n_cells_y, int(np.ceil((y0 + rr) / h)) + 1)
+        if i0 >= i1 or j0 >= j1:
+            continue
+        x = (np.arange(i0, i1) + 0.5) * h
+        y = (np.arange(j0, j1) + 0.5) * h
+        m = (x[None, :] - x0) ** 2 + (y[:, None] - y0) ** 2 <= rr * rr
+        out[j0:j1, i0:i1][m] = v
+    return out
+
+
+def large_plume(n: int = 8192, seed: int = 20240613, sources: int = 16, obstacles: int = 200):
+    """Closed n x n box (walls all round) with `obstacles` seeded circular
+    solids (radius U[8, 48] cells) and `sources` buoyant heat disks (radius
+    U[16, 64] cells, T ~ U[0.5, 1] in dye channel 0, smoke 1 in channel 1)
+    held by one full-domain dye patch; all draws from SplitMix64(seed).
+    Sources are placed in the lower half, obstacles anywhere; a source cell
+    that is also Solid stays Solid (commitLabels clears its dye)."""
+    g = Grid(n, n, 1.0 / n)
+    rng = SplitMix64(seed)
+    ocx = [rng.uniform(0, 1) for _ in range(obstacles)]
+    ocy = [rng.uniform(0, 1) for _ in range(obstacles)]
+    orr = [rng.uniform(8, 48) * g.h for _ in range(obstacles)]
+    scx = [rng.uniform(0.05, 0.95) for _ in range(sources)]
+    scy = [rng.uniform(0.05, 0.5) for _ in range(sources)]
+    srr = [rng.uniform(16, 64) * g.h for _ in range(sources)]
+    sT = [rng.uniform(0.5, 1.0) for _ in range(sources)]
+
+    def labels(gg: Grid, label: np.ndarray):
+        m = _disks(gg.nx, gg.ny, gg.h, ocx, ocy, orr, [1.0] * obstacles)
+        label.reshape(gg.ny, gg.nx)[m > 0] = SOLID
+
+    def dye_source(dg: Grid, gg: Grid) -> DyePatch:
+        heat = _disks(dg.nx, dg.ny, dg.h, scx, scy, srr, sT)
+        cov = _disks(dg.nx, dg.ny, dg.h, scx, scy, srr, [1.0] * sources)
+        cells = dg.nx * dg.ny
+        data = np.zeros(4 * cells, np.float32)
+        data[:cells] = heat.ravel()
+        data[cells:2 * cells] = cov.ravel()
+        data[3 * cells:] = cov.ravel()
+        return DyePatch(0, 0, dg.nx, dg.ny, data)
+
+    return g, SceneSpec(labels=labels, dye_source=dye_source)
