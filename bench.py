@@ -86,7 +86,7 @@ def measured_traffic(kernel, nx, ny):
     return None if k is None else k["traffic_bytes"]
 
 
-def run_gpu(args, unet=None, trace=True):
+def run_gpu(args, unet=None, trace=True, tiles=None):
     """The Karman workload: W warm-up steps, then K timed steps.  unet: solve
     every pressure from the learned warm start instead of the previous p."""
     from flowlab_amd import scenes as S
@@ -103,6 +103,11 @@ def run_gpu(args, unet=None, trace=True):
         e.set_warm_start(unet)
     for _ in range(args.warmup):
         sim.step()
+    if tiles is None:
+        tiles = e.tune_tiles(reps=10)["tiles"]  # autotuned tile shape per level
+    else:
+        for level, name in enumerate(tiles):
+            e.set_tile(name, level)
     e.set_timing(True)
     if trace:
         e.trace_kernel(args.kernel)
@@ -125,7 +130,7 @@ def run_gpu(args, unet=None, trace=True):
     e.set_timing(False)
     e.set_warm_start(None)
     return dict(sim=sim, elapsed=elapsed, solve_ms=solve_ms, iters=iters, hist_max=hist_max,
-                n_launch=n_launch, k_ms=k_ms, bracket_ms=bracket_ms, g=g)
+                n_launch=n_launch, k_ms=k_ms, bracket_ms=bracket_ms, g=g, tiles=tiles)
 
 
 def run_learned(args, plain):
@@ -136,7 +141,7 @@ def run_learned(args, plain):
         return None
     u = L.UNet()
     u.load(L.WEIGHTS)
-    r = run_gpu(args, unet=u, trace=False)
+    r = run_gpu(args, unet=u, trace=False, tiles=plain["tiles"])
     K = args.steps
     ms_l = sum(r["solve_ms"]) / K
     ms_p = sum(plain["solve_ms"]) / K
@@ -321,6 +326,7 @@ def main():
             "grid": [nx, ny],
             "solver": "mg-dc",
             "tol": 1e-6,
+            "mg_tiles": r["tiles"],
         },
         "pressure": {
             "ms_per_solve": sum(r["solve_ms"]) / K,

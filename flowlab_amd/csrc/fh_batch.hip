@@ -422,15 +422,6 @@ __global__ __launch_bounds__(NC) void k_b_coarse(const BScene* sc) {
                      smem);
 }
 
-// Tile shapes the autotuner chooses from (all even, 256 threads).
-struct TileShape {
-  int tx, ty;
-  const char* name;
-};
-constexpr TileShape kTiles[] = {
-    {64, 32, "64x32"}, {32, 32, "32x32"}, {64, 16, "64x16"}, {128, 16, "128x16"}, {32, 16, "32x16"}};
-constexpr int kNumTiles = sizeof(kTiles) / sizeof(kTiles[0]);
-
 void launch_b_smooth_restrict(int shape, dim3 g, hipStream_t s, const BScene* sc,
                               const uint32_t* tasks, int n, int l) {
   switch (shape) {
@@ -578,7 +569,7 @@ dim3 pgrid(const fh_batch* b, int n) { return dim3(std::max(1, std::min(n, b->ca
 // Tile task lists of level l for the level's chosen shape.
 int build_tile_list(fh_batch* b, int l) {
   std::vector<uint32_t> v;
-  const TileShape& T = kTiles[b->tile[l]];
+  const TileShape& T = kTileShapes[b->tile[l]];
   for (int s = 0; s < b->ns; s++) {
     const BScene& S = b->h[s];
     if (l >= S.lc) continue;
@@ -703,9 +694,9 @@ extern "C" {
 
 const char* fh_batch_last_error(void) { return b_err.c_str(); }
 
-int fh_batch_num_tiles(void) { return kNumTiles; }
+int fh_batch_num_tiles(void) { return kNumTileShapes; }
 
-const char* fh_batch_tile_name(int t) { return (t >= 0 && t < kNumTiles) ? kTiles[t].name : ""; }
+const char* fh_batch_tile_name(int t) { return (t >= 0 && t < kNumTileShapes) ? kTileShapes[t].name : ""; }
 
 fh_batch* fh_batch_create(int nscenes, const int* nx, const int* ny, const double* h) {
   if (nscenes < 1 || nscenes > 255 || !nx || !ny || !h) {
@@ -1050,7 +1041,7 @@ int fh_batch_read(fh_batch* b, int scene, double* u, double* v, double* p, doubl
 // Tile shape of every tiled level (an index of fh_batch_tile_name).
 int fh_batch_set_tile(fh_batch* b, int level, int shape) {
   BGUARD("HipBatch.setTile()");
-  if (shape < 0 || shape >= kNumTiles || level < -1 || level >= kMaxLev) {
+  if (shape < 0 || shape >= kNumTileShapes || level < -1 || level >= kMaxLev) {
     b_err = "fh_batch_set_tile: bad level or shape";
     return FH_ERR_BAD_ARG;
   }
@@ -1091,7 +1082,7 @@ int fh_batch_tune(fh_batch* b, int reps, float* ms_out) {
   for (int l = 0; l < b->max_lc; l++) {
     float best = 1e30f;
     int best_t = 0;
-    for (int t = 0; t < kNumTiles; t++) {
+    for (int t = 0; t < kNumTileShapes; t++) {
       b->tile[l] = t;
       int st = build_tile_list(b, l);
       if (st != FH_OK) return st;
@@ -1111,7 +1102,7 @@ int fh_batch_tune(fh_batch* b, int reps, float* ms_out) {
       float ms = 0.f;
       BCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
       ms /= reps;
-      if (ms_out) ms_out[l * kNumTiles + t] = ms;
+      if (ms_out) ms_out[l * kNumTileShapes + t] = ms;
       if (ms < best) {
         best = ms;
         best_t = t;

@@ -15,6 +15,7 @@
 #include "../../include/flowhip.h"
 #include "fh_device.hpp"
 #include "fh_launch.hpp"
+#include "fh_mgtile.hpp"
 
 namespace fh {
 // fh_unet.hip
@@ -48,6 +49,7 @@ struct Level {
 // workgroup then takes longer over it (measured +31 us) than the two tiled
 // launches it replaces (17 us).
 constexpr size_t kCoarseLds = 65536;
+constexpr int kMaxTiledLevels = 16;
 
 // pressureMultigrid.ts:65-78 with a configurable coarsest side.
 int level_count(int nx, int ny, int coarsest) {
@@ -76,6 +78,7 @@ struct fh_engine {
   std::vector<Level> lv;
   int nlev_ref = 0, nlev_dc = 0;
   int lc_ref = 0, lc_dc = 0;  // first level run by the one-workgroup sub-cycle
+  int tile[kMaxTiledLevels] = {};  // tile shape per tiled level (kTileShapes)
   double* M = nullptr;  // coarsest-level pseudo-inverse, n*n
   int* cells = nullptr;
   int ncoarse = 0;
@@ -444,8 +447,8 @@ int vcycle(fh_engine* e, int nlev, bool dc, bool warm, const int* done) {
     Level& N = e->lv[l + 1];
     bool zero = !(l == 0 && warm);
     Trace t(e, l == 0 ? FH_K_SMOOTH_RESTRICT_FINE : -1);
-    launch_smooth_restrict(e->s, zero ? nullptr : L.x, L.xa, L.b, L.info, L.nx, L.ny, N.b, N.nx,
-                           N.ny, zero ? 1 : 0, done);
+    launch_smooth_restrict(e->s, e->tile[l], zero ? nullptr : L.x, L.xa, L.b, L.info, L.nx, L.ny,
+                           N.b, N.nx, N.ny, zero ? 1 : 0, done);
   }
   LAUNCHED("smooth_restrict");
   {
@@ -460,7 +463,8 @@ int vcycle(fh_engine* e, int nlev, bool dc, bool warm, const int* done) {
     Level& L = e->lv[l];
     Level& N = e->lv[l + 1];
     Trace t(e, l == 0 ? FH_K_PROLONG_SMOOTH_FINE : -1);
-    launch_prolong_smooth(e->s, L.xa, L.x, L.b, L.info, L.nx, L.ny, N.x, N.nx, N.ny, done);
+    launch_prolong_smooth(e->s, e->tile[l], L.xa, L.x, L.b, L.info, L.nx, L.ny, N.x, N.nx, N.ny,
+                          done);
   }
   LAUNCHED("prolong_smooth");
   return FH_OK;
@@ -1190,6 +1194,79 @@ int fh_trace_overhead(fh_engine* e, int reps, double* ms) {
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   *ms = tot / reps;
+  return FH_OK;
+}
+
+int fh_num_tile_shapes(void) { return kNumTileShapes; }
+
+const char* fh_tile_shape_name(int shape) {
+  return (shape >= 0 && shape < kNumTileShapes) ? kTileShapes[shape].name : "";
+}
+
+int fh_set_tile(fh_engine* e, int level, int shape) {
+  GUARD("setTile()");
+  if (shape < 0 || shape >= kNumTileShapes || level < -1 || level >= kMaxTiledLevels) {
+    set_err("fh_set_tile: bad level or shape");
+    return FH_ERR_BAD_ARG;
+  }
+  for (int l = 0; l < kMaxTiledLevels; l++)
+    if (level < 0 || level == l) e->tile[l] = shape;
+  return FH_OK;
+}
+
+// Tiled levels of the defect-correction hierarchy and their shapes.
+int fh_get_tiles(fh_engine* e, int* shapes, int* nlevels) {
+  GUARD("getTiles()");
+  if (nlevels) *nlevels = e->lc_dc;
+  if (shapes)
+    for (int l = 0; l < e->lc_dc; l++) shapes[l] = e->tile[l];
+  return FH_OK;
+}
+
+// Autotuner: for every tiled level of the defect-correction hierarchy, times
+// `reps` pre-smooth+restrict and prolong+post-smooth launches of each tile
+// shape and keeps the fastest.  Only multigrid scratch (level unknowns and
+// coarse right-hand sides) is touched; labels must be set (a solve or step
+// has run).  ms_out (optional): nlevels x fh_num_tile_shapes() ms per pair.
+int fh_tune_tiles(fh_engine* e, int reps, float* ms_out) {
+  GUARD("tuneTiles()");
+  if (reps < 1) reps = 1;
+  if (e->labels_dirty) {
+    int st = refresh_levels(e);
+    if (st != FH_OK) return st;
+  }
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  for (int l = 0; l < e->lc_dc && l < kMaxTiledLevels; l++) {
+    Level& L = e->lv[l];
+    Level& N = e->lv[l + 1];
+    float best = 1e30f;
+    int best_t = 0;
+    for (int t = 0; t < kNumTileShapes; t++) {
+      for (int r = -2; r < reps; r++) {
+        if (r == 0) HIPCHK(hipEventRecord(a, e->s));
+        launch_smooth_restrict(e->s, t, nullptr, L.xa, L.b, L.info, L.nx, L.ny, N.b, N.nx, N.ny, 1,
+                               nullptr);
+        launch_prolong_smooth(e->s, t, L.xa, L.x, L.b, L.info, L.nx, L.ny, N.x, N.nx, N.ny,
+                              nullptr);
+      }
+      HIPCHK(hipEventRecord(b, e->s));
+      HIPCHK(hipEventSynchronize(b));
+      LAUNCHED("tune_tiles");
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, a, b));
+      ms /= reps;
+      if (ms_out) ms_out[l * kNumTileShapes + t] = ms;
+      if (ms < best) {
+        best = ms;
+        best_t = t;
+      }
+    }
+    e->tile[l] = best_t;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
   return FH_OK;
 }
 

@@ -113,14 +113,15 @@ struct CoarseLevels {
 void launch_cell_info(hipStream_t s, const uint8_t* lbl, int nx, int ny, uint8_t* info);
 // Pre-smoothing (2 red-black sweeps, omega 1) + residual + restriction of a
 // level to the next one's right-hand side bc.  zero: the unknown starts at 0
-// (xin unread).  xin and xout must differ.
-void launch_smooth_restrict(hipStream_t s, const float* xin, float* xout, const float* b,
-                            const uint8_t* info, int nx, int ny, float* bc, int cnx, int cny,
-                            int zero, const int* done);
+// (xin unread).  xin and xout must differ.  shape: tile shape index
+// (kTileShapes, fh_mgtile.hpp); every shape gives the same bits.
+void launch_smooth_restrict(hipStream_t s, int shape, const float* xin, float* xout,
+                            const float* b, const uint8_t* info, int nx, int ny, float* bc, int cnx,
+                            int cny, int zero, const int* done);
 // Prolongation of the coarse correction e and post-smoothing (2 sweeps).
-void launch_prolong_smooth(hipStream_t s, const float* xin, float* xout, const float* b,
-                           const uint8_t* info, int nx, int ny, const float* e, int cnx, int cny,
-                           const int* done);
+void launch_prolong_smooth(hipStream_t s, int shape, const float* xin, float* xout,
+                           const float* b, const uint8_t* info, int nx, int ny, const float* e,
+                           int cnx, int cny, const int* done);
 // The sub-cycle of levels L[0..n) in one workgroup: b0 is the first level's
 // right-hand side, x0in its warm start (NULL = zero), x0out its result.
 // direct: FP64 dense solve on the last level, else 8 red-black pairs.

@@ -82,20 +82,44 @@ void launch_cell_info(hipStream_t s, const uint8_t* lbl, int nx, int ny, uint8_t
   k_cell_info<<<g, dim3(64, 4), 0, s>>>(lbl, nx, ny, info);
 }
 
-void launch_smooth_restrict(hipStream_t s, const float* xin, float* xout, const float* b,
-                            const uint8_t* info, int nx, int ny, float* bc, int cnx, int cny,
-                            int zero, const int* done) {
-  constexpr int TX = 64, TY = 32;
+template <int TX, int TY>
+static void sr_launch(hipStream_t s, const float* xin, float* xout, const float* b,
+                      const uint8_t* info, int nx, int ny, float* bc, int cnx, int cny, int zero,
+                      const int* done) {
   dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY);
   k_smooth_restrict<TX, TY><<<g, NT, 0, s>>>(xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done);
 }
 
-void launch_prolong_smooth(hipStream_t s, const float* xin, float* xout, const float* b,
-                           const uint8_t* info, int nx, int ny, const float* e, int cnx, int cny,
-                           const int* done) {
-  constexpr int TX = 64, TY = 32;
+template <int TX, int TY>
+static void ps_launch(hipStream_t s, const float* xin, float* xout, const float* b,
+                      const uint8_t* info, int nx, int ny, const float* e, int cnx, int cny,
+                      const int* done) {
   dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY);
   k_prolong_smooth<TX, TY><<<g, NT, 0, s>>>(xin, xout, b, info, nx, ny, e, cnx, cny, done);
+}
+
+void launch_smooth_restrict(hipStream_t s, int shape, const float* xin, float* xout,
+                            const float* b, const uint8_t* info, int nx, int ny, float* bc, int cnx,
+                            int cny, int zero, const int* done) {
+  switch (shape) {
+    case 1: sr_launch<32, 32>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
+    case 2: sr_launch<64, 16>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
+    case 3: sr_launch<128, 16>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
+    case 4: sr_launch<32, 16>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
+    default: sr_launch<64, 32>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
+  }
+}
+
+void launch_prolong_smooth(hipStream_t s, int shape, const float* xin, float* xout,
+                           const float* b, const uint8_t* info, int nx, int ny, const float* e,
+                           int cnx, int cny, const int* done) {
+  switch (shape) {
+    case 1: ps_launch<32, 32>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
+    case 2: ps_launch<64, 16>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
+    case 3: ps_launch<128, 16>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
+    case 4: ps_launch<32, 16>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
+    default: ps_launch<64, 32>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
+  }
 }
 
 void launch_coarse_vcycle(hipStream_t s, const CoarseLevels& L, const float* b0,

@@ -11,6 +11,16 @@
 namespace fh {
 
 constexpr int NT = 256;  // threads per tile workgroup
+
+// Tile shapes (output cells per workgroup, both sides even) the tiled
+// smoothers are instantiated for; the autotuners choose among them per level.
+struct TileShape {
+  int tx, ty;
+  const char* name;
+};
+constexpr TileShape kTileShapes[] = {
+    {64, 32, "64x32"}, {32, 32, "32x32"}, {64, 16, "64x16"}, {128, 16, "128x16"}, {32, 16, "32x16"}};
+constexpr int kNumTileShapes = sizeof(kTileShapes) / sizeof(kTileShapes[0]);
 constexpr int HA = 5;    // halo of the pre-smooth+restrict tile (4 half-sweeps + residual)
 constexpr int HB = 4;    // halo of the prolong+post-smooth tile (4 half-sweeps)
 

@@ -185,6 +185,32 @@ class HipEngine:
         check(self.lib.fh_trace_stats(self._e, C.byref(n), C.byref(ms)))
         return n.value, ms.value
 
+    def tile_names(self) -> list:
+        return [self.lib.fh_tile_shape_name(t).decode()
+                for t in range(self.lib.fh_num_tile_shapes())]
+
+    def set_tile(self, shape, level: int = -1):
+        if isinstance(shape, str):
+            shape = self.tile_names().index(shape)
+        check(self.lib.fh_set_tile(self._e, level, shape))
+
+    def tiles(self) -> list:
+        names = self.tile_names()
+        shapes = (C.c_int * 16)()
+        n = C.c_int()
+        check(self.lib.fh_get_tiles(self._e, shapes, C.byref(n)))
+        return [names[shapes[l]] for l in range(n.value)]
+
+    def tune_tiles(self, reps: int = 10) -> dict:
+        """Autotune the multigrid tile shape per level on this grid."""
+        names = self.tile_names()
+        n = len(self.tiles())
+        ms = np.zeros(max(n, 1) * len(names), np.float32)
+        check(self.lib.fh_tune_tiles(self._e, reps, ms.ctypes.data_as(C.POINTER(C.c_float))))
+        return {"tiles": self.tiles(),
+                "ms": [{names[t]: float(ms[l * len(names) + t]) for t in range(len(names))}
+                       for l in range(n)]}
+
     def trace_overhead(self, reps: int = 200) -> float:
         """Device ms of an empty event bracket on this engine's stream."""
         ms = C.c_double()

@@ -62,6 +62,11 @@ SIGNATURES = [
     ("fh_trace_kernel", _i, [_vp, _i]),
     ("fh_trace_stats", _i, [_vp, _ip, _dp]),
     ("fh_trace_overhead", _i, [_vp, _i, _dp]),
+    ("fh_num_tile_shapes", _i, []),
+    ("fh_tile_shape_name", C.c_char_p, [_i]),
+    ("fh_set_tile", _i, [_vp, _i, _i]),
+    ("fh_get_tiles", _i, [_vp, _ip, _ip]),
+    ("fh_tune_tiles", _i, [_vp, _i, _fp]),
     ("fh_unet_create", _vp, [C.c_ulonglong]),
     ("fh_unet_destroy", None, [_vp]),
     ("fh_unet_last_error", C.c_char_p, []),

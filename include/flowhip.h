@@ -171,6 +171,20 @@ int fh_trace_stats(fh_engine* e, int* count, double* total_ms);
  * the part of each fh_trace_stats launch time that is the bracket itself. */
 int fh_trace_overhead(fh_engine* e, int reps, double* ms);
 
+/* Tile shapes of the LDS-tiled multigrid smoothers.  No counterpart in the
+ * reference, whose WebGPU multigrid uses fixed 8x8 workgroups
+ * (pressureGpu.ts:37-44).  fh_num_tile_shapes() candidates, named by
+ * fh_tile_shape_name(); level -1 sets every tiled level.  Results do not
+ * depend on the shape (temporal blocking computes the global sweep). */
+int fh_num_tile_shapes(void);
+const char* fh_tile_shape_name(int shape);
+int fh_set_tile(fh_engine* e, int level, int shape);
+int fh_get_tiles(fh_engine* e, int* shapes, int* nlevels);
+/* Autotuner: times every shape on each tiled level of this engine's grid
+ * (reps launch pairs each) and keeps the fastest; needs labels set.
+ * ms_out (may be NULL): nlevels x fh_num_tile_shapes() ms. */
+int fh_tune_tiles(fh_engine* e, int reps, float* ms_out);
+
 /* ---- learned warm start (no reference counterpart) ---------------------
  * A 4-level convolutional U-Net (3x3 convs, channels 16-32-64-64, BF16
  * weights and activations, FP32 accumulation on MFMA) predicts a correction
