@@ -418,7 +418,7 @@ __global__ __launch_bounds__(NC) void k_b_coarse(const BScene* sc) {
   const BScene& S = sc[blockIdx.x];
   if (S.done) return;
   extern __shared__ unsigned char smem[];
-  coarse_vcycle_body(S.coarse, S.lv[S.lc].b, nullptr, S.lv[S.lc].x, S.M, S.cells, S.ncoarse, 1,
+  coarse_vcycle_body<NC>(S.coarse, S.lv[S.lc].b, nullptr, S.lv[S.lc].x, S.M, S.cells, S.ncoarse, 1,
                      smem);
 }
 
@@ -429,6 +429,8 @@ void launch_b_smooth_restrict(int shape, dim3 g, hipStream_t s, const BScene* sc
     case 1: k_b_smooth_restrict<32, 32><<<g, NT, 0, s>>>(sc, tasks, n, l); break;
     case 2: k_b_smooth_restrict<64, 16><<<g, NT, 0, s>>>(sc, tasks, n, l); break;
     case 3: k_b_smooth_restrict<128, 16><<<g, NT, 0, s>>>(sc, tasks, n, l); break;
+    case 5: k_b_smooth_restrict<16, 16><<<g, NT, 0, s>>>(sc, tasks, n, l); break;
+    case 6: k_b_smooth_restrict<32, 8><<<g, NT, 0, s>>>(sc, tasks, n, l); break;
     default: k_b_smooth_restrict<32, 16><<<g, NT, 0, s>>>(sc, tasks, n, l); break;
   }
 }
@@ -440,6 +442,8 @@ void launch_b_prolong_smooth(int shape, dim3 g, hipStream_t s, const BScene* sc,
     case 1: k_b_prolong_smooth<32, 32><<<g, NT, 0, s>>>(sc, tasks, n, l); break;
     case 2: k_b_prolong_smooth<64, 16><<<g, NT, 0, s>>>(sc, tasks, n, l); break;
     case 3: k_b_prolong_smooth<128, 16><<<g, NT, 0, s>>>(sc, tasks, n, l); break;
+    case 5: k_b_prolong_smooth<16, 16><<<g, NT, 0, s>>>(sc, tasks, n, l); break;
+    case 6: k_b_prolong_smooth<32, 8><<<g, NT, 0, s>>>(sc, tasks, n, l); break;
     default: k_b_prolong_smooth<32, 16><<<g, NT, 0, s>>>(sc, tasks, n, l); break;
   }
 }

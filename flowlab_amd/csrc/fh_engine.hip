@@ -79,6 +79,7 @@ struct fh_engine {
   int nlev_ref = 0, nlev_dc = 0;
   int lc_ref = 0, lc_dc = 0;  // first level run by the one-workgroup sub-cycle
   int tile[kMaxTiledLevels] = {};  // tile shape per tiled level (kTileShapes)
+  int coarse_group = 0;            // coarse sub-cycle workgroup (kCoarseGroups)
   double* M = nullptr;  // coarsest-level pseudo-inverse, n*n
   int* cells = nullptr;
   int ncoarse = 0;
@@ -453,7 +454,7 @@ int vcycle(fh_engine* e, int nlev, bool dc, bool warm, const int* done) {
   LAUNCHED("smooth_restrict");
   {
     Trace t(e, FH_K_COARSE_VCYCLE);
-    launch_coarse_vcycle(e->s, coarse_levels(e, lc, nlev), e->lv[lc].b,
+    launch_coarse_vcycle(e->s, e->coarse_group, coarse_levels(e, lc, nlev), e->lv[lc].b,
                          (lc == 0 && warm) ? e->lv[0].x : nullptr, e->lv[lc].x,
                          dc ? e->M : nullptr, dc ? e->cells : nullptr, dc ? e->ncoarse : 0,
                          dc ? 1 : 0, done);
@@ -1265,9 +1266,45 @@ int fh_tune_tiles(fh_engine* e, int reps, float* ms_out) {
     }
     e->tile[l] = best_t;
   }
+  // the one-workgroup coarse sub-cycle: its workgroup size
+  {
+    const int lc = e->lc_dc, nlev = e->nlev_dc;
+    float best = 1e30f;
+    for (int g = 0; g < kNumCoarseGroups; g++) {
+      for (int r = -2; r < reps; r++) {
+        if (r == 0) HIPCHK(hipEventRecord(a, e->s));
+        launch_coarse_vcycle(e->s, g, coarse_levels(e, lc, nlev), e->lv[lc].b, nullptr,
+                             e->lv[lc].x, e->M, e->cells, e->ncoarse, 1, nullptr);
+      }
+      HIPCHK(hipEventRecord(b, e->s));
+      HIPCHK(hipEventSynchronize(b));
+      LAUNCHED("tune_coarse");
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, a, b));
+      if (ms / reps < best) {
+        best = ms / reps;
+        e->coarse_group = g;
+      }
+    }
+  }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   return FH_OK;
+}
+
+// Workgroup size of the coarse sub-cycle: threads of kCoarseGroups[group].
+int fh_set_coarse_group(fh_engine* e, int group) {
+  GUARD("setCoarseGroup()");
+  if (group < 0 || group >= kNumCoarseGroups) {
+    set_err("fh_set_coarse_group: bad group");
+    return FH_ERR_BAD_ARG;
+  }
+  e->coarse_group = group;
+  return FH_OK;
+}
+
+int fh_coarse_group_threads(fh_engine* e) {
+  return e ? kCoarseGroups[e->coarse_group] : 0;
 }
 
 int fh_synchronize(fh_engine* e) {

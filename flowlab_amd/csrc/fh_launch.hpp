@@ -125,7 +125,8 @@ void launch_prolong_smooth(hipStream_t s, int shape, const float* xin, float* xo
 // The sub-cycle of levels L[0..n) in one workgroup: b0 is the first level's
 // right-hand side, x0in its warm start (NULL = zero), x0out its result.
 // direct: FP64 dense solve on the last level, else 8 red-black pairs.
-void launch_coarse_vcycle(hipStream_t s, const CoarseLevels& L, const float* b0,
+// group: workgroup size index (kCoarseGroups, fh_mgtile.hpp), same bits.
+void launch_coarse_vcycle(hipStream_t s, int group, const CoarseLevels& L, const float* b0,
                           const float* x0in, float* x0out, const double* M, const int* cells,
                           int ncoarse, int direct, const int* done);
 size_t coarse_vcycle_lds_bytes(const CoarseLevels& L);

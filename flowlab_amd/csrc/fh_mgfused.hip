@@ -51,15 +51,16 @@ __global__ __launch_bounds__(NT) void k_prolong_smooth(const float* __restrict__
   prolong_smooth_tile<TX, TY>(xin, xout, b, info, nx, ny, e, cnx, cny, done, bx, by);
 }
 
-__global__ __launch_bounds__(NC) void k_coarse_vcycle(CoarseLevels L, const float* __restrict__ b0,
-                                                      const float* __restrict__ x0in,
-                                                      float* __restrict__ x0out,
-                                                      const double* __restrict__ M,
-                                                      const int* __restrict__ cells, int ncoarse,
-                                                      int direct, const int* __restrict__ done) {
+template <int G>
+__global__ __launch_bounds__(G) void k_coarse_vcycle(CoarseLevels L, const float* __restrict__ b0,
+                                                     const float* __restrict__ x0in,
+                                                     float* __restrict__ x0out,
+                                                     const double* __restrict__ M,
+                                                     const int* __restrict__ cells, int ncoarse,
+                                                     int direct, const int* __restrict__ done) {
   if (done && *done) return;
   extern __shared__ unsigned char smem[];
-  coarse_vcycle_body(L, b0, x0in, x0out, M, cells, ncoarse, direct, smem);
+  coarse_vcycle_body<G>(L, b0, x0in, x0out, M, cells, ncoarse, direct, smem);
 }
 
 __global__ void k_cell_info(const uint8_t* __restrict__ lbl, int nx, int ny,
@@ -106,6 +107,8 @@ void launch_smooth_restrict(hipStream_t s, int shape, const float* xin, float* x
     case 2: sr_launch<64, 16>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
     case 3: sr_launch<128, 16>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
     case 4: sr_launch<32, 16>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
+    case 5: sr_launch<16, 16>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
+    case 6: sr_launch<32, 8>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
     default: sr_launch<64, 32>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
   }
 }
@@ -118,15 +121,28 @@ void launch_prolong_smooth(hipStream_t s, int shape, const float* xin, float* xo
     case 2: ps_launch<64, 16>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
     case 3: ps_launch<128, 16>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
     case 4: ps_launch<32, 16>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
+    case 5: ps_launch<16, 16>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
+    case 6: ps_launch<32, 8>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
     default: ps_launch<64, 32>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
   }
 }
 
-void launch_coarse_vcycle(hipStream_t s, const CoarseLevels& L, const float* b0,
+void launch_coarse_vcycle(hipStream_t s, int group, const CoarseLevels& L, const float* b0,
                           const float* x0in, float* x0out, const double* M, const int* cells,
                           int ncoarse, int direct, const int* done) {
   size_t lds = coarse_vcycle_lds_bytes(L);
-  k_coarse_vcycle<<<1, NC, lds, s>>>(L, b0, x0in, x0out, M, cells, ncoarse, direct, done);
+  switch (group) {
+    case 1:
+      k_coarse_vcycle<512><<<1, 512, lds, s>>>(L, b0, x0in, x0out, M, cells, ncoarse, direct, done);
+      break;
+    case 2:
+      k_coarse_vcycle<256><<<1, 256, lds, s>>>(L, b0, x0in, x0out, M, cells, ncoarse, direct, done);
+      break;
+    default:
+      k_coarse_vcycle<1024><<<1, 1024, lds, s>>>(L, b0, x0in, x0out, M, cells, ncoarse, direct,
+                                                 done);
+      break;
+  }
 }
 
 }  // namespace fh

@@ -19,7 +19,8 @@ struct TileShape {
   const char* name;
 };
 constexpr TileShape kTileShapes[] = {
-    {64, 32, "64x32"}, {32, 32, "32x32"}, {64, 16, "64x16"}, {128, 16, "128x16"}, {32, 16, "32x16"}};
+    {64, 32, "64x32"}, {32, 32, "32x32"}, {64, 16, "64x16"}, {128, 16, "128x16"},
+    {32, 16, "32x16"}, {16, 16, "16x16"}, {32, 8, "32x8"}};
 constexpr int kNumTileShapes = sizeof(kTileShapes) / sizeof(kTileShapes[0]);
 constexpr int HA = 5;    // halo of the pre-smooth+restrict tile (4 half-sweeps + residual)
 constexpr int HB = 4;    // halo of the prolong+post-smooth tile (4 half-sweeps)
@@ -311,13 +312,19 @@ __device__ __forceinline__ void prolong_smooth_tile(const float* __restrict__ xi
 // The whole sub-cycle from level lc down, in one workgroup's LDS.  Each
 // level's arrays are padded by a one-cell ring of zeros (pitch nx+2) so the
 // half-sweep's neighbour reads never leave the level.
-constexpr int NC = 1024;
-constexpr int KC = 4;  // padded cells staged per thread per pass
+// The workgroup size NC is a template parameter (the autotuner picks it): a
+// larger group has more lanes per phase, a smaller one cheaper barriers.  KC
+// = padded cells staged per thread per pass, so NC*KC covers the usual
+// coarse hierarchy in one pass.
+constexpr int NC = 1024;  // the batch's coarse workgroup
+constexpr int kCoarseGroups[] = {1024, 512, 256};
+constexpr int kNumCoarseGroups = 3;
 
 __device__ __forceinline__ int padded(int nx, int ny) { return (nx + 2) * (ny + 2); }
 
 // Body of the sub-cycle for a workgroup of NC threads; smem is the dynamic
 // LDS of coarse_vcycle_lds_bytes(L).
+template <int NC, int KC = 4096 / NC>
 __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
                                                    const float* __restrict__ b0,
                                                    const float* __restrict__ x0in,

@@ -201,13 +201,19 @@ class HipEngine:
         check(self.lib.fh_get_tiles(self._e, shapes, C.byref(n)))
         return [names[shapes[l]] for l in range(n.value)]
 
+    def set_coarse_group(self, group: int):
+        check(self.lib.fh_set_coarse_group(self._e, group))
+
+    def coarse_group_threads(self) -> int:
+        return int(self.lib.fh_coarse_group_threads(self._e))
+
     def tune_tiles(self, reps: int = 10) -> dict:
         """Autotune the multigrid tile shape per level on this grid."""
         names = self.tile_names()
         n = len(self.tiles())
         ms = np.zeros(max(n, 1) * len(names), np.float32)
         check(self.lib.fh_tune_tiles(self._e, reps, ms.ctypes.data_as(C.POINTER(C.c_float))))
-        return {"tiles": self.tiles(),
+        return {"tiles": self.tiles(), "coarse_threads": self.coarse_group_threads(),
                 "ms": [{names[t]: float(ms[l * len(names) + t]) for t in range(len(names))}
                        for l in range(n)]}
 

@@ -67,6 +67,8 @@ SIGNATURES = [
     ("fh_set_tile", _i, [_vp, _i, _i]),
     ("fh_get_tiles", _i, [_vp, _ip, _ip]),
     ("fh_tune_tiles", _i, [_vp, _i, _fp]),
+    ("fh_set_coarse_group", _i, [_vp, _i]),
+    ("fh_coarse_group_threads", _i, [_vp]),
     ("fh_unet_create", _vp, [C.c_ulonglong]),
     ("fh_unet_destroy", None, [_vp]),
     ("fh_unet_last_error", C.c_char_p, []),
