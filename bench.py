@@ -32,8 +32,9 @@ HBM_PEAK_GBS = 8000.0
 # kernel: every array element read or written once, neighbour reuse perfect.
 #   rb_sweep_fine: one colour half-sweep = half of a full red-black sweep's
 #     13 B/cell (x 4 read + 4 write, rhs 4, label 1).
-#   maccormack_faces: 2 x (carrier u,v 8 + hat 4 + u^n 4 + out 4 + label 1).
-#   advect_faces: 2 x (carrier 8 + out 4 + label 1).
+#   maccormack_faces: u^n, v^n 8 + forward-pass u, v 8 + out u, v 8 + label 1
+#     (u^n is both the carrier and the limiter's stencil: read once).
+#   advect_faces: carrier = source u, v 8 + out u, v 8 + label 1.
 #   dc_update: e 4 + p 8 read + 8 write + div 8 + label 1 + r32 4.
 #   smooth_restrict_fine: b 4 + label 1 + x 4 written + coarse b 1 (x starts
 #     at zero, so it is not read).
@@ -42,8 +43,8 @@ ALGO_BYTES_PER_CELL = {
     "smooth_restrict_fine": 10.0,
     "prolong_smooth_fine": 14.0,
     "rb_sweep_fine": 6.5,
-    "maccormack_faces": 42.0,
-    "advect_faces": 26.0,
+    "maccormack_faces": 25.0,
+    "advect_faces": 17.0,
     "maccormack_dye": 45.0,
     "advect_dye": 33.0,
     "dc_update": 33.0,

@@ -55,32 +55,239 @@ __global__ __launch_bounds__(256) void k_maccormack_dye(
                     bx * BX + threadIdx.x, by * BY + threadIdx.y);
 }
 
+// ---- LDS-staged variants ------------------------------------------------------
+// A 64x4 workgroup covers a 64x16 tile of thread indices, four rows per
+// thread.  The input fields are staged into LDS over the tile plus a halo of
+// AH = 2 cells on each side and one more column and row (WW x WH).
+//
+// Reach: a backtrace moves a point by at most D = |c| * Vw cells, Vw the
+// largest |u|, |v| in the staged velocity windows (the midpoint samples lie
+// inside the window when the endpoint bound holds, and bilinear samples never
+// exceed the values they blend).  For a thread index i in [x0, x0+63] the
+// lowest stencil column read is floor(x0 - 0.5 - D) >= x0 - 2 and the highest
+// floor(x0 + 63.5 + D) + 1 <= x0 + 66 when D <= 1.5, likewise for rows; the
+// index clamping of the sampling only moves indices towards the tile.  So
+// when D <= kReach (1.5 less a margin for rounding) every read of the tile is
+// in its window and the kernel reads LDS unchecked; otherwise (a long
+// backtrace, or a non-finite velocity) the whole workgroup reads global
+// memory exactly as the plain kernels do.  Same bits either way.
+constexpr int TX = 64, TY = 16, AH = 2;
+constexpr int WW = TX + 2 * AH + 1, WH = TY + 2 * AH + 1, WN = WW * WH;
+constexpr int kStage = (WN + 255) / 256;
+constexpr float kReach = 1.25f;
+
+struct Win {
+  const float* __restrict__ p;
+  int w, h;  // field extent; row stride is w
+};
+
+// max(|a|, |b|) that keeps a NaN.
+__device__ __forceinline__ float nan_max(float m, float x) {
+  x = fabsf(x);
+  return (x > m || x != x) ? x : m;
+}
+
+// Stages NF windows (all global loads first, then the LDS stores) and
+// returns this thread's nan_max over what it staged of windows 0 and 1 (the
+// velocity); elements outside a field are staged as 0.
+template <int NF>
+__device__ __forceinline__ float stage_windows(float (*lds)[WN], const Win (&f)[NF], int wx0,
+                                               int wy0, int tid) {
+  float v[NF][kStage];
+#pragma unroll
+  for (int r = 0; r < kStage; r++) {
+    const int e = tid + r * 256;
+    const int b = e / WW, a = e - b * WW;
+    const int i = wx0 + a, j = wy0 + b;
+#pragma unroll
+    for (int q = 0; q < NF; q++)
+      v[q][r] = (e < WN && i >= 0 && j >= 0 && i < f[q].w && j < f[q].h)
+                    ? f[q].p[i + j * f[q].w]
+                    : 0.f;
+  }
+  float m = 0.f;
+#pragma unroll
+  for (int r = 0; r < kStage; r++) {
+    const int e = tid + r * 256;
+    if (e < WN) {
+#pragma unroll
+      for (int q = 0; q < NF; q++) lds[q][e] = v[q][r];
+    }
+    m = nan_max(nan_max(m, v[0][r]), v[1][r]);
+  }
+  return m;
+}
+
+// Workgroup-uniform reach test: every thread passes its staged maximum, the
+// barrier that publishes the windows also publishes the four waves' maxima.
+__device__ __forceinline__ bool within_reach(float m, float c, int tid) {
+  __shared__ float wm[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = nan_max(m, __shfl_xor(m, o, 64));
+  if ((tid & 63) == 0) wm[tid >> 6] = m;
+  __syncthreads();
+  const float vw = nan_max(nan_max(nan_max(wm[0], wm[1]), wm[2]), wm[3]);
+  return fabsf(c) * vw <= kReach;
+}
+
+__device__ __forceinline__ LField window(const float (*lds)[WN], int q, int wx0, int wy0) {
+  return LField{(lds_cfp)lds[q], wx0, wy0, WW};
+}
+
+// advect.ts:72-112 with carrier == source (the only use: u and v advect
+// themselves), so two windows.
+__global__ __launch_bounds__(256) void k_advect_faces_lds(
+    const float* __restrict__ uc, const float* __restrict__ vc, float* __restrict__ uo,
+    float* __restrict__ vo, const uint8_t* __restrict__ lbl, int nx, int ny, float c) {
+  __shared__ float lds[2][WN];
+  int bx, by;
+  xcd_block(&bx, &by);
+  const int x0 = bx * TX, y0 = by * TY, wx0 = x0 - AH, wy0 = y0 - AH;
+  const int tid = threadIdx.x + threadIdx.y * TX;
+  const Win f[2] = {{uc, nx + 1, ny}, {vc, nx, ny + 1}};
+  const float m = stage_windows<2>(lds, f, wx0, wy0, tid);
+  const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
+  if (within_reach(m, c, tid)) {
+    const LField u = window(lds, 0, wx0, wy0), v = window(lds, 1, wx0, wy0);
+    for (int r = 0; r < TY / BY; r++) advect_faces_t(u, v, u, v, uo, vo, lbl, nx, ny, c, i, j + r * BY);
+  } else {
+    const GField u{uc, nx + 1}, v{vc, nx};
+    for (int r = 0; r < TY / BY; r++) advect_faces_t(u, v, u, v, uo, vo, lbl, nx, ny, c, i, j + r * BY);
+  }
+}
+
+// advect.ts:177-210 (fused reverse pass and clamp, as k_maccormack_faces).
+__global__ __launch_bounds__(256) void k_maccormack_faces_lds(
+    const float* __restrict__ un, const float* __restrict__ vn, const float* __restrict__ uh,
+    const float* __restrict__ vh, float* __restrict__ uo, float* __restrict__ vo,
+    const uint8_t* __restrict__ lbl, int nx, int ny, float c) {
+  __shared__ float lds[4][WN];
+  int bx, by;
+  xcd_block(&bx, &by);
+  const int x0 = bx * TX, y0 = by * TY, wx0 = x0 - AH, wy0 = y0 - AH;
+  const int tid = threadIdx.x + threadIdx.y * TX;
+  const Win f[4] = {{un, nx + 1, ny}, {vn, nx, ny + 1}, {uh, nx + 1, ny}, {vh, nx, ny + 1}};
+  const float m = stage_windows<4>(lds, f, wx0, wy0, tid);
+  const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
+  if (within_reach(m, c, tid)) {
+    const LField a = window(lds, 0, wx0, wy0), b = window(lds, 1, wx0, wy0),
+                 ah = window(lds, 2, wx0, wy0), bh = window(lds, 3, wx0, wy0);
+    for (int r = 0; r < TY / BY; r++)
+      maccormack_faces_t(a, b, ah, bh, uo, vo, lbl, nx, ny, c, i, j + r * BY);
+  } else {
+    const GField a{un, nx + 1}, b{vn, nx}, ah{uh, nx + 1}, bh{vh, nx};
+    for (int r = 0; r < TY / BY; r++)
+      maccormack_faces_t(a, b, ah, bh, uo, vo, lbl, nx, ny, c, i, j + r * BY);
+  }
+}
+
+// advect.ts:241-270, dye on the velocity grid (scale 1): windows of u, v and
+// the 3 channels.
+__global__ __launch_bounds__(256) void k_advect_dye_lds(
+    const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ q,
+    float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx, int ny, float c, float k0,
+    float k1, float k2) {
+  __shared__ float lds[5][WN];
+  int bx, by;
+  xcd_block(&bx, &by);
+  const int x0 = bx * TX, y0 = by * TY, wx0 = x0 - AH, wy0 = y0 - AH;
+  const int tid = threadIdx.x + threadIdx.y * TX;
+  const size_t plane = (size_t)nx * ny;
+  const Win f[5] = {{u, nx + 1, ny}, {v, nx, ny + 1}, {q, nx, ny}, {q + plane, nx, ny},
+                    {q + 2 * plane, nx, ny}};
+  const float m = stage_windows<5>(lds, f, wx0, wy0, tid);
+  const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
+  if (within_reach(m, c, tid)) {
+    const LField a = window(lds, 0, wx0, wy0), b = window(lds, 1, wx0, wy0),
+                 q0 = window(lds, 2, wx0, wy0), q1 = window(lds, 3, wx0, wy0),
+                 q2 = window(lds, 4, wx0, wy0);
+    for (int r = 0; r < TY / BY; r++)
+      advect_dye_t(a, b, q0, q1, q2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i, j + r * BY);
+  } else {
+    const GField a{u, nx + 1}, b{v, nx}, q0{q, nx}, q1{q + plane, nx}, q2{q + 2 * plane, nx};
+    for (int r = 0; r < TY / BY; r++)
+      advect_dye_t(a, b, q0, q1, q2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i, j + r * BY);
+  }
+}
+
+// advect.ts:291-311, dye on the velocity grid: windows of u, v, the 3
+// channels of q^n and of the forward pass.
+__global__ __launch_bounds__(256) void k_maccormack_dye_lds(
+    const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ qn,
+    const float* __restrict__ qh, float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx,
+    int ny, float c, float k0, float k1, float k2) {
+  __shared__ float lds[8][WN];
+  int bx, by;
+  xcd_block(&bx, &by);
+  const int x0 = bx * TX, y0 = by * TY, wx0 = x0 - AH, wy0 = y0 - AH;
+  const int tid = threadIdx.x + threadIdx.y * TX;
+  const size_t plane = (size_t)nx * ny;
+  const Win f[8] = {{u, nx + 1, ny},      {v, nx, ny + 1},          {qn, nx, ny},
+                    {qn + plane, nx, ny}, {qn + 2 * plane, nx, ny}, {qh, nx, ny},
+                    {qh + plane, nx, ny}, {qh + 2 * plane, nx, ny}};
+  const float m = stage_windows<8>(lds, f, wx0, wy0, tid);
+  const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
+  if (within_reach(m, c, tid)) {
+    const LField a = window(lds, 0, wx0, wy0), b = window(lds, 1, wx0, wy0),
+                 n0 = window(lds, 2, wx0, wy0), n1 = window(lds, 3, wx0, wy0),
+                 n2 = window(lds, 4, wx0, wy0), h0 = window(lds, 5, wx0, wy0),
+                 h1 = window(lds, 6, wx0, wy0), h2 = window(lds, 7, wx0, wy0);
+    for (int r = 0; r < TY / BY; r++)
+      maccormack_dye_t(a, b, n0, n1, n2, h0, h1, h2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i,
+                       j + r * BY);
+  } else {
+    const GField a{u, nx + 1}, b{v, nx}, n0{qn, nx}, n1{qn + plane, nx}, n2{qn + 2 * plane, nx},
+        h0{qh, nx}, h1{qh + plane, nx}, h2{qh + 2 * plane, nx};
+    for (int r = 0; r < TY / BY; r++)
+      maccormack_dye_t(a, b, n0, n1, n2, h0, h1, h2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i,
+                       j + r * BY);
+  }
+}
+
+static inline dim3 grid_t(int w, int h) { return dim3((w + TX - 1) / TX, (h + TY - 1) / TY); }
+
 void launch_advect_faces(hipStream_t s, const float* uc, const float* vc, const float* us,
                          const float* vs, float* uo, float* vo, const uint8_t* lbl, int nx, int ny,
-                         float c) {
-  k_advect_faces<<<grid2(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(uc, vc, us, vs, uo, vo, lbl, nx,
-                                                                ny, c);
+                         float c, bool lds) {
+  if (lds && us == uc && vs == vc)
+    k_advect_faces_lds<<<grid_t(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(uc, vc, uo, vo, lbl, nx,
+                                                                       ny, c);
+  else
+    k_advect_faces<<<grid2(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(uc, vc, us, vs, uo, vo, lbl,
+                                                                  nx, ny, c);
 }
 
 void launch_maccormack_faces(hipStream_t s, const float* un, const float* vn, const float* uh,
                              const float* vh, float* uo, float* vo, const uint8_t* lbl, int nx,
-                             int ny, float c) {
-  k_maccormack_faces<<<grid2(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(un, vn, uh, vh, uo, vo, lbl,
-                                                                    nx, ny, c);
+                             int ny, float c, bool lds) {
+  if (lds)
+    k_maccormack_faces_lds<<<grid_t(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(un, vn, uh, vh, uo,
+                                                                           vo, lbl, nx, ny, c);
+  else
+    k_maccormack_faces<<<grid2(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(un, vn, uh, vh, uo, vo,
+                                                                      lbl, nx, ny, c);
 }
 
 void launch_advect_dye(hipStream_t s, const float* u, const float* v, const float* q, float* qo,
                        const uint8_t* lbl, int nx, int ny, int ds, float c, float k0, float k1,
-                       float k2) {
-  k_advect_dye<<<grid2(nx * ds, ny * ds), dim3(BX, BY), 0, s>>>(u, v, q, qo, lbl, nx, ny, ds, c,
-                                                                k0, k1, k2);
+                       float k2, bool lds) {
+  if (lds && ds == 1)
+    k_advect_dye_lds<<<grid_t(nx, ny), dim3(BX, BY), 0, s>>>(u, v, q, qo, lbl, nx, ny, c, k0, k1,
+                                                             k2);
+  else
+    k_advect_dye<<<grid2(nx * ds, ny * ds), dim3(BX, BY), 0, s>>>(u, v, q, qo, lbl, nx, ny, ds,
+                                                                  c, k0, k1, k2);
 }
 
 void launch_maccormack_dye(hipStream_t s, const float* u, const float* v, const float* qn,
                            const float* qh, float* qo, const uint8_t* lbl, int nx, int ny, int ds,
-                           float c, float k0, float k1, float k2) {
-  k_maccormack_dye<<<grid2(nx * ds, ny * ds), dim3(BX, BY), 0, s>>>(u, v, qn, qh, qo, lbl, nx,
-                                                                    ny, ds, c, k0, k1, k2);
+                           float c, float k0, float k1, float k2, bool lds) {
+  if (lds && ds == 1)
+    k_maccormack_dye_lds<<<grid_t(nx, ny), dim3(BX, BY), 0, s>>>(u, v, qn, qh, qo, lbl, nx, ny,
+                                                                 c, k0, k1, k2);
+  else
+    k_maccormack_dye<<<grid2(nx * ds, ny * ds), dim3(BX, BY), 0, s>>>(u, v, qn, qh, qo, lbl, nx,
+                                                                      ny, ds, c, k0, k1, k2);
 }
 
 }  // namespace fh

@@ -121,6 +121,63 @@ __device__ __forceinline__ void backtrace(const float* __restrict__ uc,
   *by = shift(y, -c * sample_v(vc, nx, ny, mx, my));
 }
 
+// ---- field accessors ------------------------------------------------------
+// The sampling above, generic over where a field's values come from: GField
+// reads global memory (the same elements the pointer versions read); LField
+// reads a window of the field staged in LDS, for callers that have shown
+// every index they will read lies inside the window.
+struct GField {
+  const float* __restrict__ p;
+  int s;  // row stride
+  __device__ __forceinline__ float at(int i, int j) const { return p[i + j * s]; }
+  __device__ __forceinline__ Quad quad(int i, int j) const {
+    const float* r = p + i + j * s;
+    return {r[0], r[1], r[s], r[s + 1]};
+  }
+};
+
+typedef __attribute__((address_space(3))) const float* lds_cfp;
+
+struct LField {
+  lds_cfp l;    // window origin: field index (x0, y0)
+  int x0, y0, w;  // w = window row stride
+  __device__ __forceinline__ float at(int i, int j) const { return l[(i - x0) + (j - y0) * w]; }
+  __device__ __forceinline__ Quad quad(int i, int j) const {
+    lds_cfp r = l + (i - x0) + (j - y0) * w;
+    return {r[0], r[1], r[w], r[w + 1]};
+  }
+};
+
+template <class F>
+__device__ __forceinline__ Quad gather4f(const F& q, Axis ax, Axis ay) {
+  return q.quad(ax.i0, ay.i0);
+}
+
+template <class F>
+__device__ __forceinline__ float sample_uf(const F& u, int nx, int ny, Pos x, Pos y) {
+  Axis ax = clamped_axis(x.i, x.f, nx + 1);
+  Axis ay = clamped_axis(y.i, y.f - 0.5f, ny);
+  Quad g = gather4f(u, ax, ay);
+  return bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
+}
+
+template <class F>
+__device__ __forceinline__ float sample_vf(const F& v, int nx, int ny, Pos x, Pos y) {
+  Axis ax = clamped_axis(x.i, x.f - 0.5f, nx);
+  Axis ay = clamped_axis(y.i, y.f, ny + 1);
+  Quad g = gather4f(v, ax, ay);
+  return bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
+}
+
+template <class FU, class FV>
+__device__ __forceinline__ void backtracef(const FU& uc, const FV& vc, int nx, int ny, Pos x, Pos y,
+                                           float u0, float v0, float c, Pos* bx, Pos* by) {
+  Pos mx = shift(x, -0.5f * c * u0);
+  Pos my = shift(y, -0.5f * c * v0);
+  *bx = shift(x, -c * sample_uf(uc, nx, ny, mx, my));
+  *by = shift(y, -c * sample_vf(vc, nx, ny, mx, my));
+}
+
 // Workgroups of a launch are dispatched round-robin over the chip's 8 XCDs,
 // each with its own L2.  Remapping the dispatch index gives every XCD a
 // contiguous run of logical blocks (row-major over the block grid), so the

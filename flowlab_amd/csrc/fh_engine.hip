@@ -80,6 +80,7 @@ struct fh_engine {
   int lc_ref = 0, lc_dc = 0;  // first level run by the one-workgroup sub-cycle
   int tile[kMaxTiledLevels] = {};  // tile shape per tiled level (kTileShapes)
   int coarse_group = 0;            // coarse sub-cycle workgroup (kCoarseGroups)
+  bool advect_lds = true;          // LDS-staged advection kernels
   double* M = nullptr;  // coarsest-level pseudo-inverse, n*n
   int* cells = nullptr;
   int ncoarse = 0;
@@ -655,14 +656,16 @@ int advect_velocity(fh_engine* e, int scheme, double dt) {
   if (scheme == FH_MACCORMACK) {
     {
       Trace t(e, FH_K_ADVECT_FACES);
-      launch_advect_faces(e->s, e->u, e->v, e->u, e->v, e->uh, e->vh, e->lbl, e->nx, e->ny, c);
+      launch_advect_faces(e->s, e->u, e->v, e->u, e->v, e->uh, e->vh, e->lbl, e->nx, e->ny, c,
+                          e->advect_lds);
     }
     Trace t(e, FH_K_MACCORMACK_FACES);
     launch_maccormack_faces(e->s, e->u, e->v, e->uh, e->vh, e->u2, e->v2, e->lbl, e->nx, e->ny,
-                            c);
+                            c, e->advect_lds);
   } else {
     Trace t(e, FH_K_ADVECT_FACES);
-    launch_advect_faces(e->s, e->u, e->v, e->u, e->v, e->u2, e->v2, e->lbl, e->nx, e->ny, c);
+    launch_advect_faces(e->s, e->u, e->v, e->u, e->v, e->u2, e->v2, e->lbl, e->nx, e->ny, c,
+                        e->advect_lds);
   }
   LAUNCHED("advect_velocity");
   swap_ptr(e->u, e->u2);
@@ -697,15 +700,15 @@ int dye_phase(fh_engine* e, int scheme, double dt, const double keep[3]) {
     {
       Trace t(e, FH_K_ADVECT_DYE);
       launch_advect_dye(e->s, e->u, e->v, e->dye, e->dyeh, e->lbl, e->nx, e->ny, e->ds, c, 1.f,
-                        1.f, 1.f);
+                        1.f, 1.f, e->advect_lds);
     }
     Trace t(e, FH_K_MACCORMACK_DYE);
     launch_maccormack_dye(e->s, e->u, e->v, e->dye, e->dyeh, e->dye2, e->lbl, e->nx, e->ny, e->ds,
-                          c, k0, k1, k2);
+                          c, k0, k1, k2, e->advect_lds);
   } else {
     Trace t(e, FH_K_ADVECT_DYE);
     launch_advect_dye(e->s, e->u, e->v, e->dye, e->dye2, e->lbl, e->nx, e->ny, e->ds, c, k0, k1,
-                      k2);
+                      k2, e->advect_lds);
   }
   LAUNCHED("advect_dye");
   swap_ptr(e->dye, e->dye2);
@@ -1300,6 +1303,14 @@ int fh_set_coarse_group(fh_engine* e, int group) {
     return FH_ERR_BAD_ARG;
   }
   e->coarse_group = group;
+  return FH_OK;
+}
+
+// Advection kernels: LDS-staged windows (1, the default) or plain gathers from
+// global memory (0).  Same results either way.
+int fh_set_advect_staging(fh_engine* e, int on) {
+  GUARD("setAdvectStaging()");
+  e->advect_lds = on != 0;
   return FH_OK;
 }
 

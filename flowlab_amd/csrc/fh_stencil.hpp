@@ -15,9 +15,43 @@
 namespace fh {
 
 // ---- advection (advect.ts) ---------------------------------------------------
+// Bodies are generic over field accessors (GField / LField, fh_device.hpp);
+// the pointer versions read global memory and give the same bits.
 
 // advect.ts:72-112 -- one thread index covers u(i,j) when j<ny and v(i,j)
-// when i<nx, over (nx+1) x (ny+1).
+// when i<nx, over (nx+1) x (ny+1).  uc/vc carry, us/vs are advected.
+template <class CU, class CV, class SU, class SV>
+__device__ __forceinline__ void advect_faces_t(const CU& uc, const CV& vc, const SU& us,
+                                               const SV& vs, float* __restrict__ uo,
+                                               float* __restrict__ vo,
+                                               const uint8_t* __restrict__ lbl, int nx, int ny,
+                                               float c, int i, int j) {
+  if (i > nx || j > ny) return;
+  Pos bx, by;
+  if (j < ny) {
+    int k = i + j * (nx + 1);
+    if (solid_or_outside(lbl, nx, ny, i - 1, j) || solid_or_outside(lbl, nx, ny, i, j)) {
+      uo[k] = us.at(i, j);
+    } else {
+      Pos X{i, 0.f}, Y{j, 0.5f};
+      float v0 = sample_vf(vc, nx, ny, X, Y);
+      backtracef(uc, vc, nx, ny, X, Y, uc.at(i, j), v0, c, &bx, &by);
+      uo[k] = sample_uf(us, nx, ny, bx, by);
+    }
+  }
+  if (i < nx) {
+    int k = i + j * nx;
+    if (solid_or_outside(lbl, nx, ny, i, j - 1) || solid_or_outside(lbl, nx, ny, i, j)) {
+      vo[k] = vs.at(i, j);
+    } else {
+      Pos X{i, 0.5f}, Y{j, 0.f};
+      float u0 = sample_uf(uc, nx, ny, X, Y);
+      backtracef(uc, vc, nx, ny, X, Y, u0, vc.at(i, j), c, &bx, &by);
+      vo[k] = sample_vf(vs, nx, ny, bx, by);
+    }
+  }
+}
+
 __device__ __forceinline__ void advect_faces_at(const float* __restrict__ uc,
                                                 const float* __restrict__ vc,
                                                 const float* __restrict__ us,
@@ -25,36 +59,58 @@ __device__ __forceinline__ void advect_faces_at(const float* __restrict__ uc,
                                                 float* __restrict__ uo, float* __restrict__ vo,
                                                 const uint8_t* __restrict__ lbl, int nx, int ny,
                                                 float c, int i, int j) {
-  if (i > nx || j > ny) return;
-  Pos bx, by;
-  if (j < ny) {
-    int k = i + j * (nx + 1);
-    if (solid_or_outside(lbl, nx, ny, i - 1, j) || solid_or_outside(lbl, nx, ny, i, j)) {
-      uo[k] = us[k];
-    } else {
-      Pos X{i, 0.f}, Y{j, 0.5f};
-      float v0 = sample_v(vc, nx, ny, X, Y);
-      backtrace(uc, vc, nx, ny, X, Y, uc[k], v0, c, &bx, &by);
-      uo[k] = sample_u(us, nx, ny, bx, by);
-    }
-  }
-  if (i < nx) {
-    int k = i + j * nx;
-    if (solid_or_outside(lbl, nx, ny, i, j - 1) || solid_or_outside(lbl, nx, ny, i, j)) {
-      vo[k] = vs[k];
-    } else {
-      Pos X{i, 0.5f}, Y{j, 0.f};
-      float u0 = sample_u(uc, nx, ny, X, Y);
-      backtrace(uc, vc, nx, ny, X, Y, u0, vc[k], c, &bx, &by);
-      vo[k] = sample_v(vs, nx, ny, bx, by);
-    }
-  }
+  advect_faces_t(GField{uc, nx + 1}, GField{vc, nx}, GField{us, nx + 1}, GField{vs, nx}, uo, vo,
+                 lbl, nx, ny, c, i, j);
 }
 
 // advect.ts:177-210 fused: the reverse pass (carrier u^n, source uHat, -dt)
 // is evaluated for this face only, then corrected = uHat + (u^n - uBar)/2 is
 // clamped to the u^n stencil at the forward backtrace.  Each face reads only
 // its own uBar, so fusing changes no value.
+template <class NU, class NV, class HU, class HV>
+__device__ __forceinline__ void maccormack_faces_t(const NU& un, const NV& vn, const HU& uh,
+                                                   const HV& vh, float* __restrict__ uo,
+                                                   float* __restrict__ vo,
+                                                   const uint8_t* __restrict__ lbl, int nx, int ny,
+                                                   float c, int i, int j) {
+  if (i > nx || j > ny) return;
+  Pos rx, ry, fx, fy;
+  if (j < ny) {
+    int k = i + j * (nx + 1);
+    if (solid_or_outside(lbl, nx, ny, i - 1, j) || solid_or_outside(lbl, nx, ny, i, j)) {
+      uo[k] = un.at(i, j);
+    } else {
+      Pos X{i, 0.f}, Y{j, 0.5f};
+      float u0 = un.at(i, j);
+      float v0 = sample_vf(vn, nx, ny, X, Y);
+      backtracef(un, vn, nx, ny, X, Y, u0, v0, -c, &rx, &ry);
+      float ubar = sample_uf(uh, nx, ny, rx, ry);
+      float corrected = uh.at(i, j) + 0.5f * (u0 - ubar);
+      backtracef(un, vn, nx, ny, X, Y, u0, v0, c, &fx, &fy);
+      Axis ax = clamped_axis(fx.i, fx.f, nx + 1);
+      Axis ay = clamped_axis(fy.i, fy.f - 0.5f, ny);
+      uo[k] = clamp_to_quad(corrected, gather4f(un, ax, ay));
+    }
+  }
+  if (i < nx) {
+    int k = i + j * nx;
+    if (solid_or_outside(lbl, nx, ny, i, j - 1) || solid_or_outside(lbl, nx, ny, i, j)) {
+      vo[k] = vn.at(i, j);
+    } else {
+      Pos X{i, 0.5f}, Y{j, 0.f};
+      float v0 = vn.at(i, j);
+      float u0 = sample_uf(un, nx, ny, X, Y);
+      backtracef(un, vn, nx, ny, X, Y, u0, v0, -c, &rx, &ry);
+      float vbar = sample_vf(vh, nx, ny, rx, ry);
+      float corrected = vh.at(i, j) + 0.5f * (v0 - vbar);
+      backtracef(un, vn, nx, ny, X, Y, u0, v0, c, &fx, &fy);
+      Axis ax = clamped_axis(fx.i, fx.f - 0.5f, nx);
+      Axis ay = clamped_axis(fy.i, fy.f, ny + 1);
+      vo[k] = clamp_to_quad(corrected, gather4f(vn, ax, ay));
+    }
+  }
+}
+
 __device__ __forceinline__ void maccormack_faces_at(const float* __restrict__ un,
                                                     const float* __restrict__ vn,
                                                     const float* __restrict__ uh,
@@ -62,42 +118,8 @@ __device__ __forceinline__ void maccormack_faces_at(const float* __restrict__ un
                                                     float* __restrict__ uo, float* __restrict__ vo,
                                                     const uint8_t* __restrict__ lbl, int nx,
                                                     int ny, float c, int i, int j) {
-  if (i > nx || j > ny) return;
-  Pos rx, ry, fx, fy;
-  if (j < ny) {
-    int k = i + j * (nx + 1);
-    if (solid_or_outside(lbl, nx, ny, i - 1, j) || solid_or_outside(lbl, nx, ny, i, j)) {
-      uo[k] = un[k];
-    } else {
-      Pos X{i, 0.f}, Y{j, 0.5f};
-      float u0 = un[k];
-      float v0 = sample_v(vn, nx, ny, X, Y);
-      backtrace(un, vn, nx, ny, X, Y, u0, v0, -c, &rx, &ry);
-      float ubar = sample_u(uh, nx, ny, rx, ry);
-      float corrected = uh[k] + 0.5f * (u0 - ubar);
-      backtrace(un, vn, nx, ny, X, Y, u0, v0, c, &fx, &fy);
-      Axis ax = clamped_axis(fx.i, fx.f, nx + 1);
-      Axis ay = clamped_axis(fy.i, fy.f - 0.5f, ny);
-      uo[k] = clamp_to_quad(corrected, gather4(un, nx + 1, ax, ay));
-    }
-  }
-  if (i < nx) {
-    int k = i + j * nx;
-    if (solid_or_outside(lbl, nx, ny, i, j - 1) || solid_or_outside(lbl, nx, ny, i, j)) {
-      vo[k] = vn[k];
-    } else {
-      Pos X{i, 0.5f}, Y{j, 0.f};
-      float v0 = vn[k];
-      float u0 = sample_u(un, nx, ny, X, Y);
-      backtrace(un, vn, nx, ny, X, Y, u0, v0, -c, &rx, &ry);
-      float vbar = sample_v(vh, nx, ny, rx, ry);
-      float corrected = vh[k] + 0.5f * (v0 - vbar);
-      backtrace(un, vn, nx, ny, X, Y, u0, v0, c, &fx, &fy);
-      Axis ax = clamped_axis(fx.i, fx.f - 0.5f, nx);
-      Axis ay = clamped_axis(fy.i, fy.f, ny + 1);
-      vo[k] = clamp_to_quad(corrected, gather4(vn, nx, ax, ay));
-    }
-  }
+  maccormack_faces_t(GField{un, nx + 1}, GField{vn, nx}, GField{uh, nx + 1}, GField{vh, nx}, uo,
+                     vo, lbl, nx, ny, c, i, j);
 }
 
 // Cell (i,j) of the dye grid (ds x finer) in velocity-cell units.
@@ -106,39 +128,91 @@ __device__ __forceinline__ Pos dye_pos(int i, int ds) {
 }
 
 // advect.ts:241-270 for three channels sharing one backtrace; out *= keep.
+// q0..q2 are the channels as accessors over the dye grid (dnx x dny); the
+// output planes are qo, qo + plane, qo + 2 plane.
+template <class CU, class CV, class Q>
+__device__ __forceinline__ void advect_dye_t(const CU& u, const CV& v, const Q& q0, const Q& q1,
+                                             const Q& q2, float* __restrict__ qo,
+                                             const uint8_t* __restrict__ lbl, int nx, int ny,
+                                             int ds, float c, float k0, float k1, float k2, int i,
+                                             int j) {
+  int dnx = nx * ds, dny = ny * ds;
+  if (i >= dnx || j >= dny) return;
+  size_t plane = (size_t)dnx * dny;
+  int k = i + j * dnx;
+  if (solid_or_outside(lbl, nx, ny, i / ds, j / ds)) {
+    qo[k] = q0.at(i, j) * k0;
+    qo[k + plane] = q1.at(i, j) * k1;
+    qo[k + 2 * plane] = q2.at(i, j) * k2;
+    return;
+  }
+  Pos X = dye_pos(i, ds), Y = dye_pos(j, ds);
+  float u0 = sample_uf(u, nx, ny, X, Y);
+  float v0 = sample_vf(v, nx, ny, X, Y);
+  Pos bx, by;
+  backtracef(u, v, nx, ny, X, Y, u0, v0, c, &bx, &by);
+  Pos dx{i, 0.5f + ds * (bx.f - X.f)}, dy{j, 0.5f + ds * (by.f - Y.f)};
+  Axis ax = clamped_axis(dx.i, dx.f - 0.5f, dnx);
+  Axis ay = clamped_axis(dy.i, dy.f - 0.5f, dny);
+  Quad g0 = gather4f(q0, ax, ay);
+  Quad g1 = gather4f(q1, ax, ay);
+  Quad g2 = gather4f(q2, ax, ay);
+  qo[k] = bilerp(g0.a, g0.b, g0.c, g0.d, ax.f, ay.f) * k0;
+  qo[k + plane] = bilerp(g1.a, g1.b, g1.c, g1.d, ax.f, ay.f) * k1;
+  qo[k + 2 * plane] = bilerp(g2.a, g2.b, g2.c, g2.d, ax.f, ay.f) * k2;
+}
+
 __device__ __forceinline__ void advect_dye_at(const float* __restrict__ u,
                                               const float* __restrict__ v,
                                               const float* __restrict__ q, float* __restrict__ qo,
                                               const uint8_t* __restrict__ lbl, int nx, int ny,
                                               int ds, float c, float k0, float k1, float k2, int i,
                                               int j) {
+  const int dnx = nx * ds;
+  const size_t plane = (size_t)dnx * ny * ds;
+  advect_dye_t(GField{u, nx + 1}, GField{v, nx}, GField{q, dnx}, GField{q + plane, dnx},
+               GField{q + 2 * plane, dnx}, qo, lbl, nx, ny, ds, c, k0, k1, k2, i, j);
+}
+
+// advect.ts:291-311 for three channels, fused as for the faces; out *= keep.
+template <class CU, class CV, class Q>
+__device__ __forceinline__ void maccormack_dye_t(const CU& u, const CV& v, const Q& n0,
+                                                 const Q& n1, const Q& n2, const Q& h0,
+                                                 const Q& h1, const Q& h2, float* __restrict__ qo,
+                                                 const uint8_t* __restrict__ lbl, int nx, int ny,
+                                                 int ds, float c, float k0, float k1, float k2,
+                                                 int i, int j) {
   int dnx = nx * ds, dny = ny * ds;
   if (i >= dnx || j >= dny) return;
   size_t plane = (size_t)dnx * dny;
   int k = i + j * dnx;
   if (solid_or_outside(lbl, nx, ny, i / ds, j / ds)) {
-    qo[k] = q[k] * k0;
-    qo[k + plane] = q[k + plane] * k1;
-    qo[k + 2 * plane] = q[k + 2 * plane] * k2;
+    qo[k] = n0.at(i, j) * k0;
+    qo[k + plane] = n1.at(i, j) * k1;
+    qo[k + 2 * plane] = n2.at(i, j) * k2;
     return;
   }
   Pos X = dye_pos(i, ds), Y = dye_pos(j, ds);
-  float u0 = sample_u(u, nx, ny, X, Y);
-  float v0 = sample_v(v, nx, ny, X, Y);
+  float u0 = sample_uf(u, nx, ny, X, Y);
+  float v0 = sample_vf(v, nx, ny, X, Y);
   Pos bx, by;
-  backtrace(u, v, nx, ny, X, Y, u0, v0, c, &bx, &by);
-  Pos dx{i, 0.5f + ds * (bx.f - X.f)}, dy{j, 0.5f + ds * (by.f - Y.f)};
-  Axis ax = clamped_axis(dx.i, dx.f - 0.5f, dnx);
-  Axis ay = clamped_axis(dy.i, dy.f - 0.5f, dny);
-  Quad g0 = gather4(q, dnx, ax, ay);
-  Quad g1 = gather4(q + plane, dnx, ax, ay);
-  Quad g2 = gather4(q + 2 * plane, dnx, ax, ay);
-  qo[k] = bilerp(g0.a, g0.b, g0.c, g0.d, ax.f, ay.f) * k0;
-  qo[k + plane] = bilerp(g1.a, g1.b, g1.c, g1.d, ax.f, ay.f) * k1;
-  qo[k + 2 * plane] = bilerp(g2.a, g2.b, g2.c, g2.d, ax.f, ay.f) * k2;
+  backtracef(u, v, nx, ny, X, Y, u0, v0, -c, &bx, &by);
+  Axis ax = clamped_axis(i, ds * (bx.f - X.f), dnx);
+  Axis ay = clamped_axis(j, ds * (by.f - Y.f), dny);
+  Quad r0 = gather4f(h0, ax, ay);
+  Quad r1 = gather4f(h1, ax, ay);
+  Quad r2 = gather4f(h2, ax, ay);
+  float c0 = h0.at(i, j) + 0.5f * (n0.at(i, j) - bilerp(r0.a, r0.b, r0.c, r0.d, ax.f, ay.f));
+  float c1 = h1.at(i, j) + 0.5f * (n1.at(i, j) - bilerp(r1.a, r1.b, r1.c, r1.d, ax.f, ay.f));
+  float c2 = h2.at(i, j) + 0.5f * (n2.at(i, j) - bilerp(r2.a, r2.b, r2.c, r2.d, ax.f, ay.f));
+  backtracef(u, v, nx, ny, X, Y, u0, v0, c, &bx, &by);
+  ax = clamped_axis(i, ds * (bx.f - X.f), dnx);
+  ay = clamped_axis(j, ds * (by.f - Y.f), dny);
+  qo[k] = clamp_to_quad(c0, gather4f(n0, ax, ay)) * k0;
+  qo[k + plane] = clamp_to_quad(c1, gather4f(n1, ax, ay)) * k1;
+  qo[k + 2 * plane] = clamp_to_quad(c2, gather4f(n2, ax, ay)) * k2;
 }
 
-// advect.ts:291-311 for three channels, fused as for the faces; out *= keep.
 __device__ __forceinline__ void maccormack_dye_at(const float* __restrict__ u,
                                                   const float* __restrict__ v,
                                                   const float* __restrict__ qn,
@@ -147,36 +221,11 @@ __device__ __forceinline__ void maccormack_dye_at(const float* __restrict__ u,
                                                   const uint8_t* __restrict__ lbl, int nx, int ny,
                                                   int ds, float c, float k0, float k1, float k2,
                                                   int i, int j) {
-  int dnx = nx * ds, dny = ny * ds;
-  if (i >= dnx || j >= dny) return;
-  size_t plane = (size_t)dnx * dny;
-  int k = i + j * dnx;
-  if (solid_or_outside(lbl, nx, ny, i / ds, j / ds)) {
-    qo[k] = qn[k] * k0;
-    qo[k + plane] = qn[k + plane] * k1;
-    qo[k + 2 * plane] = qn[k + 2 * plane] * k2;
-    return;
-  }
-  Pos X = dye_pos(i, ds), Y = dye_pos(j, ds);
-  float u0 = sample_u(u, nx, ny, X, Y);
-  float v0 = sample_v(v, nx, ny, X, Y);
-  Pos bx, by;
-  backtrace(u, v, nx, ny, X, Y, u0, v0, -c, &bx, &by);
-  Axis ax = clamped_axis(i, ds * (bx.f - X.f), dnx);
-  Axis ay = clamped_axis(j, ds * (by.f - Y.f), dny);
-  Quad r0 = gather4(qh, dnx, ax, ay);
-  Quad r1 = gather4(qh + plane, dnx, ax, ay);
-  Quad r2 = gather4(qh + 2 * plane, dnx, ax, ay);
-  float c0 = qh[k] + 0.5f * (qn[k] - bilerp(r0.a, r0.b, r0.c, r0.d, ax.f, ay.f));
-  float c1 = qh[k + plane] + 0.5f * (qn[k + plane] - bilerp(r1.a, r1.b, r1.c, r1.d, ax.f, ay.f));
-  float c2 = qh[k + 2 * plane] +
-             0.5f * (qn[k + 2 * plane] - bilerp(r2.a, r2.b, r2.c, r2.d, ax.f, ay.f));
-  backtrace(u, v, nx, ny, X, Y, u0, v0, c, &bx, &by);
-  ax = clamped_axis(i, ds * (bx.f - X.f), dnx);
-  ay = clamped_axis(j, ds * (by.f - Y.f), dny);
-  qo[k] = clamp_to_quad(c0, gather4(qn, dnx, ax, ay)) * k0;
-  qo[k + plane] = clamp_to_quad(c1, gather4(qn + plane, dnx, ax, ay)) * k1;
-  qo[k + 2 * plane] = clamp_to_quad(c2, gather4(qn + 2 * plane, dnx, ax, ay)) * k2;
+  const int dnx = nx * ds;
+  const size_t plane = (size_t)dnx * ny * ds;
+  maccormack_dye_t(GField{u, nx + 1}, GField{v, nx}, GField{qn, dnx}, GField{qn + plane, dnx},
+                   GField{qn + 2 * plane, dnx}, GField{qh, dnx}, GField{qh + plane, dnx},
+                   GField{qh + 2 * plane, dnx}, qo, lbl, nx, ny, ds, c, k0, k1, k2, i, j);
 }
 
 // ---- forces and projection -------------------------------------------------------

@@ -204,6 +204,10 @@ class HipEngine:
     def set_coarse_group(self, group: int):
         check(self.lib.fh_set_coarse_group(self._e, group))
 
+    def set_advect_staging(self, on: bool):
+        """LDS-staged advection kernels (default) or plain global gathers."""
+        check(self.lib.fh_set_advect_staging(self._e, int(bool(on))))
+
     def coarse_group_threads(self) -> int:
         return int(self.lib.fh_coarse_group_threads(self._e))
 

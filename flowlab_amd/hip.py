@@ -68,6 +68,7 @@ SIGNATURES = [
     ("fh_get_tiles", _i, [_vp, _ip, _ip]),
     ("fh_tune_tiles", _i, [_vp, _i, _fp]),
     ("fh_set_coarse_group", _i, [_vp, _i]),
+    ("fh_set_advect_staging", _i, [_vp, _i]),
     ("fh_coarse_group_threads", _i, [_vp]),
     ("fh_unet_create", _vp, [C.c_ulonglong]),
     ("fh_unet_destroy", None, [_vp]),

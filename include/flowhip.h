@@ -189,6 +189,10 @@ int fh_tune_tiles(fh_engine* e, int reps, float* ms_out);
  * threads); fh_tune_tiles also picks it.  Results do not depend on it. */
 int fh_set_coarse_group(fh_engine* e, int group);
 int fh_coarse_group_threads(fh_engine* e);
+/* Advection kernels (advect.ts:72-112, 177-210, 241-270, 291-311): on != 0
+ * (the default) stages each workgroup's input windows through LDS; 0 gathers
+ * straight from global memory.  Results are the same bits either way. */
+int fh_set_advect_staging(fh_engine* e, int on);
 
 /* ---- learned warm start (no reference counterpart) ---------------------
  * A 4-level convolutional U-Net (3x3 convs, channels 16-32-64-64, BF16
