@@ -85,14 +85,36 @@ __device__ __forceinline__ void lds_half_sweep(float* sx, const float* sb, const
   }
 }
 
+// LDS-typed pointers for the one-workgroup coarse levels, whose arrays sit at
+// runtime offsets of one dynamic LDS block: with the address space explicit
+// the accesses are ds_read/ds_write rather than flat ones.
+typedef __attribute__((address_space(3))) float* lds_fp;
+typedef __attribute__((address_space(3))) uint8_t* lds_bp;
+
+// q / d for 0 <= q < 2^16 and 1 <= d < 2^16 by a multiply-high.  With
+// m = ceil(2^32 / d) = d m' ... : m d = 2^32 + e, 0 <= e < d, so
+// q m / 2^32 = q / d + q e / (d 2^32), and the second term is below
+// q / 2^32 < 2^-16 < 1/d, too small to carry floor(q / d) (whose fractional
+// part is at most 1 - 1/d) past the next integer.
+struct FastDiv {
+  unsigned m;
+  int d;
+  __device__ __forceinline__ explicit FastDiv(int dd)
+      : m(dd > 1 ? 0xFFFFFFFFu / (unsigned)dd + 1u : 0u), d(dd) {}
+  __device__ __forceinline__ int div(int q) const {
+    return d > 1 ? (int)__umulhi((unsigned)q, m) : q;
+  }
+};
+
 // Runtime-width variant for the one-workgroup coarse levels (arrays padded
 // by a ring of zero cells), in chunks of 4 cells per thread with the reads of
 // a chunk issued before its writes.
-__device__ __forceinline__ void lds_half_sweep_rt(float* sx, const float* sb, const uint8_t* si,
+__device__ __forceinline__ void lds_half_sweep_rt(lds_fp sx, const lds_fp sb, const lds_bp si,
                                                   int LX, int LY, int color, bool zero, int tid,
                                                   int nthreads) {
   const int w = LX - 2, P = (w + 1) >> 1;
   const int total = P * (LY - 2);
+  const FastDiv dP(P);
   for (int base = 0; base < total; base += 4 * nthreads) {
     int l[4];
     bool act[4];
@@ -106,7 +128,7 @@ __device__ __forceinline__ void lds_half_sweep_rt(float* sx, const float* sb, co
       nb[k] = bb[k] = 0.f;
       if (base + k * nthreads >= total) continue;  // uniform: nothing of this slot is left
       int q = base + tid + k * nthreads;
-      int r = q / P, pc = q - r * P;
+      int r = dP.div(q), pc = q - r * P;
       int lj = 1 + r;
       // padded (0,0) is global (-1,-1): global parity of (li, lj) = li + lj
       int li = 1 + 2 * pc + (((1 + 2 * pc + lj + color) & 1) ? 1 : 0);
@@ -170,8 +192,8 @@ __device__ __forceinline__ bool stage_tile(float* sx, float* sb, uint8_t* si,
 }
 
 // r = b - (count x - sum) at an LDS cell (0 off the fluid or if count = 0).
-__device__ __forceinline__ float lds_residual(const float* sx, const float* sb, const uint8_t* si,
-                                              int LX, int l) {
+template <class XP, class BP, class IP>
+__device__ __forceinline__ float lds_residual(XP sx, BP sb, IP si, int LX, int l) {
   int cnt = si[l] & 7;
   if (cnt == 0) return 0.f;
   float sum = ((sx[l + 1] + sx[l - 1]) + sx[l + LX]) + sx[l - LX];
@@ -322,6 +344,28 @@ constexpr int kNumCoarseGroups = 3;
 
 __device__ __forceinline__ int padded(int nx, int ny) { return (nx + 2) * (ny + 2); }
 
+__device__ __forceinline__ int clevel_bytes(int nx, int ny) {
+  const int n = padded(nx, ny);
+  return 8 * n + ((n + 3) & ~3);
+}
+
+// One coarse level's LDS arrays: x and b (pitch nx+2, padded by the zero
+// ring) and the info bytes, at the level's offset in the dynamic LDS block
+// (levels in order: x, b, info each, as coarse_vcycle_lds_bytes sizes them).
+struct CLevel {
+  lds_fp x, b;
+  lds_bp i;
+  int nx, ny, px, py;
+};
+
+__device__ __forceinline__ CLevel clevel(lds_bp smem, const CoarseLevels& L, int l) {
+  int off = 0;
+  for (int k = 0; k < l; k++) off += clevel_bytes(L.nx[k], L.ny[k]);
+  const int nx = L.nx[l], ny = L.ny[l], n = padded(nx, ny);
+  lds_bp p = smem + off;
+  return {(lds_fp)p, (lds_fp)(p + 4 * n), p + 8 * n, nx, ny, nx + 2, ny + 2};
+}
+
 // Body of the sub-cycle for a workgroup of NC threads; smem is the dynamic
 // LDS of coarse_vcycle_lds_bytes(L).
 template <int NC, int KC = 4096 / NC>
@@ -331,23 +375,9 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
                                                    float* __restrict__ x0out,
                                                    const double* __restrict__ M,
                                                    const int* __restrict__ cells, int ncoarse,
-                                                   int direct, unsigned char* smem) {
+                                                   int direct, unsigned char* smem_generic) {
   const int tid = threadIdx.x;
-  float* X[kMaxCoarse];
-  float* B[kMaxCoarse];
-  uint8_t* I8[kMaxCoarse];
-  {
-    unsigned char* p = smem;
-    for (int l = 0; l < L.n; l++) {
-      int n = padded(L.nx[l], L.ny[l]);
-      X[l] = (float*)p;
-      p += n * sizeof(float);
-      B[l] = (float*)p;
-      p += n * sizeof(float);
-      I8[l] = (uint8_t*)p;
-      p += (n + 3) & ~3;
-    }
-  }
+  const lds_bp smem = (lds_bp)smem_generic;
   {
     // All levels' padded cells as one index range, KC per thread, loads
     // issued together before the LDS writes (one memory wait per thread).
@@ -356,14 +386,16 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
     for (int base = 0; base < total; base += KC * NC) {
       float rx[KC], rb[KC];
       uint8_t ri[KC];
-      int lvl[KC], lq[KC];
+      int ox[KC], ob[KC], oi[KC];  // LDS byte offsets of the cell's x, b, info
 #pragma unroll
       for (int k = 0; k < KC; k++) {
         int t = base + tid + k * NC;
-        int l = 0, q = t;
-        while (l < L.n - 1 && q >= padded(L.nx[l], L.ny[l])) {
-          q -= padded(L.nx[l], L.ny[l]);
+        int l = 0, q = t, off = 0, n = padded(L.nx[0], L.ny[0]);
+        while (l < L.n - 1 && q >= n) {
+          q -= n;
+          off += 8 * n + ((n + 3) & ~3);
           l++;
+          n = padded(L.nx[l], L.ny[l]);
         }
         int nx = L.nx[l], ny = L.ny[l], px = nx + 2;
         int lj = q / px, li = q - lj * px;
@@ -373,47 +405,49 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
         ri[k] = in ? L.info[l][g] : 0;
         rx[k] = (l == 0 && in && x0in) ? x0in[g] : 0.f;
         rb[k] = (l == 0 && in) ? b0[g] : 0.f;
-        lvl[k] = t < total ? l : -1;
-        lq[k] = q;
+        ox[k] = t < total ? off + 4 * q : -1;
+        ob[k] = off + 4 * n + 4 * q;
+        oi[k] = off + 8 * n + q;
       }
 #pragma unroll
       for (int k = 0; k < KC; k++) {
-        if (lvl[k] < 0) continue;
-        I8[lvl[k]][lq[k]] = ri[k];
-        X[lvl[k]][lq[k]] = rx[k];
-        B[lvl[k]][lq[k]] = rb[k];
+        if (ox[k] < 0) continue;
+        smem[oi[k]] = ri[k];
+        *(lds_fp)(smem + ox[k]) = rx[k];
+        *(lds_fp)(smem + ob[k]) = rb[k];
       }
     }
   }
   __syncthreads();
   for (int l = 0; l < L.n - 1; l++) {
-    int px = L.nx[l] + 2, py = L.ny[l] + 2;
+    const CLevel F = clevel(smem, L, l), G = clevel(smem, L, l + 1);
     bool zero = (l > 0) || !x0in;
     for (int s = 1; s <= 4; s++) {
-      lds_half_sweep_rt(X[l], B[l], I8[l], px, py, (s - 1) & 1, zero && s == 1, tid, NC);
+      lds_half_sweep_rt(F.x, F.b, F.i, F.px, F.py, (s - 1) & 1, zero && s == 1, tid, NC);
       __syncthreads();
     }
-    int cnx = L.nx[l + 1], cny = L.ny[l + 1], cpx = cnx + 2;
-    for (int q = tid; q < cnx * cny; q += NC) {
-      int J = q / cnx, I = q - J * cnx;
+    const FastDiv dc(G.nx);
+    for (int q = tid; q < G.nx * G.ny; q += NC) {
+      int J = dc.div(q), I = q - J * G.nx;
       float s = 0.f;
       for (int dj = 0; dj < 2; dj++)
         for (int di = 0; di < 2; di++) {
           int gi = 2 * I + di, gj = 2 * J + dj;
-          if (gi < L.nx[l] && gj < L.ny[l])
-            s += lds_residual(X[l], B[l], I8[l], px, (gi + 1) + (gj + 1) * px);
+          if (gi < F.nx && gj < F.ny)
+            s += lds_residual(F.x, F.b, F.i, F.px, (gi + 1) + (gj + 1) * F.px);
         }
-      B[l + 1][(I + 1) + (J + 1) * cpx] = s;
+      G.b[(I + 1) + (J + 1) * G.px] = s;
     }
     __syncthreads();
   }
   {
-    int l = L.n - 1;
-    int nx = L.nx[l], ny = L.ny[l], px = nx + 2, py = ny + 2;
+    const int l = L.n - 1;
+    const CLevel C = clevel(smem, L, l);
     bool zero = (l > 0) || !x0in;
     if (direct) {
       // x = M b in FP64, eight lanes per row; lane `part` takes columns
       // part + 8m in chunks of 8, all 8 loads of a chunk in flight together.
+      const FastDiv dn(C.nx);
       for (int t = tid; t < 8 * ncoarse; t += NC) {
         int a = t >> 3, part = t & 7;
         const double* row = M + (size_t)a * ncoarse;
@@ -429,51 +463,54 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
           }
 #pragma unroll
           for (int m = 0; m < 8; m++)
-            if (q0 + 8 * m < ncoarse)
-              s += mv[m] * (double)B[l][(cv[m] % nx + 1) + (cv[m] / nx + 1) * px];
+            if (q0 + 8 * m < ncoarse) {
+              const int cj = dn.div(cv[m]), ci = cv[m] - cj * C.nx;
+              s += mv[m] * (double)C.b[(ci + 1) + (cj + 1) * C.px];
+            }
         }
         s += __shfl_xor(s, 1, 64);
         s += __shfl_xor(s, 2, 64);
         s += __shfl_xor(s, 4, 64);
         if (part == 0) {
-          int c = cells[a];
-          X[l][(c % nx + 1) + (c / nx + 1) * px] = (float)s;
+          const int c = cells[a], cj = dn.div(c), ci = c - cj * C.nx;
+          C.x[(ci + 1) + (cj + 1) * C.px] = (float)s;
         }
       }
     } else {
       for (int s = 0; s < 16; s++) {
-        lds_half_sweep_rt(X[l], B[l], I8[l], px, py, s & 1, zero && s == 0, tid, NC);
+        lds_half_sweep_rt(C.x, C.b, C.i, C.px, C.py, s & 1, zero && s == 0, tid, NC);
         __syncthreads();
       }
     }
     __syncthreads();
   }
   for (int l = L.n - 2; l >= 0; l--) {
-    int nx = L.nx[l], ny = L.ny[l], px = nx + 2, py = ny + 2;
-    int cnx = L.nx[l + 1], cny = L.ny[l + 1], cpx = cnx + 2;
-    for (int q = tid; q < nx * ny; q += NC) {
-      int gj = q / nx, gi = q - gj * nx;
-      int lq = (gi + 1) + (gj + 1) * px;
-      if (!(I8[l][lq] & 0x80)) continue;
+    const CLevel F = clevel(smem, L, l), G = clevel(smem, L, l + 1);
+    const FastDiv dn(F.nx);
+    for (int q = tid; q < F.nx * F.ny; q += NC) {
+      int gj = dn.div(q), gi = q - gj * F.nx;
+      int lq = (gi + 1) + (gj + 1) * F.px;
+      if (!(F.i[lq] & 0x80)) continue;
       int I = gi >> 1, J = gj >> 1;
-      int I2 = (gi & 1) ? min(I + 1, cnx - 1) : max(I - 1, 0);
-      int J2 = (gj & 1) ? min(J + 1, cny - 1) : max(J - 1, 0);
-      const float* E = X[l + 1];
-      X[l][lq] += 0.5625f * E[(I + 1) + (J + 1) * cpx] +
-                  0.1875f * (E[(I2 + 1) + (J + 1) * cpx] + E[(I + 1) + (J2 + 1) * cpx]) +
-                  0.0625f * E[(I2 + 1) + (J2 + 1) * cpx];
+      int I2 = (gi & 1) ? min(I + 1, G.nx - 1) : max(I - 1, 0);
+      int J2 = (gj & 1) ? min(J + 1, G.ny - 1) : max(J - 1, 0);
+      const lds_fp E = G.x;
+      F.x[lq] += 0.5625f * E[(I + 1) + (J + 1) * G.px] +
+                 0.1875f * (E[(I2 + 1) + (J + 1) * G.px] + E[(I + 1) + (J2 + 1) * G.px]) +
+                 0.0625f * E[(I2 + 1) + (J2 + 1) * G.px];
     }
     __syncthreads();
     for (int s = 1; s <= 4; s++) {
-      lds_half_sweep_rt(X[l], B[l], I8[l], px, py, (s - 1) & 1, false, tid, NC);
+      lds_half_sweep_rt(F.x, F.b, F.i, F.px, F.py, (s - 1) & 1, false, tid, NC);
       __syncthreads();
     }
   }
   {
-    int nx = L.nx[0], ny = L.ny[0], px = nx + 2;
-    for (int q = tid; q < nx * ny; q += NC) {
-      int gj = q / nx, gi = q - gj * nx;
-      x0out[q] = X[0][(gi + 1) + (gj + 1) * px];
+    const CLevel F = clevel(smem, L, 0);
+    const FastDiv dn(F.nx);
+    for (int q = tid; q < F.nx * F.ny; q += NC) {
+      int gj = dn.div(q), gi = q - gj * F.nx;
+      x0out[q] = F.x[(gi + 1) + (gj + 1) * F.px];
     }
   }
 }
