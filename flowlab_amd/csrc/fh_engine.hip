@@ -97,9 +97,11 @@ struct fh_engine {
   // cycle in flight) that the norm kernel writes, and the events that mark
   // those kernels' completion
   int* done = nullptr;
-  double* h_res = nullptr;  // mapped, coherent; d_res is its device address
+  double* h_res = nullptr;  // mapped, coherent; d_res is its device address: two cycle
+                            // slots of 8 and the initial pass's at 16
   double* d_res = nullptr;
-  hipEvent_t cyc_ev[2] = {};
+  hipEvent_t cyc_ev[3] = {};  // cycle slots 0, 1 and the initial pass (2)
+  double rho_first = -1.0;     // r_0 / r_init of the last solve (its first cycle's contraction)
   // patch / forces
   float* patch = nullptr;
   int pi0 = 0, pj0 = 0, pnx = 0, pny = 0, has_patch = 0;
@@ -472,6 +474,25 @@ int vcycle(fh_engine* e, int nlev, bool dc, bool warm, const int* done) {
   return FH_OK;
 }
 
+// One defect-correction cycle c: V-cycle on the correction, p += e with the
+// new residual and its norm (to host slot c & 1), then cycle event c & 1.
+int enqueue_dc_cycle(fh_engine* e, int c, double scale, double tol) {
+  Level& L0 = e->lv[0];
+  int st = vcycle(e, e->nlev_dc, true, false, e->done);
+  if (st != FH_OK) return st;
+  {
+    Trace t(e, FH_K_DC_UPDATE);
+    launch_dc_correct(e->s, e->p, e->p2, L0.x, e->div, L0.info, L0.b, e->nx, e->ny, scale,
+                      e->partials, e->done);
+  }
+  launch_dc_norm(e->s, e->partials, e->nx, e->ny, e->sums, e->d_res + 8 * (c & 1), e->done, scale,
+                 tol);
+  LAUNCHED("dc_update");
+  swap_ptr(e->p, e->p2);
+  HIPCHK(hipEventRecord(e->cyc_ev[c & 1], e->s));
+  return FH_OK;
+}
+
 double rel_resid(const double* s, double scale, bool scaled_r) {
   // sums = {sum r^2, n_r, sum div^2, n_fluid}; the defect-correction kernel
   // reports r in code units (b - A p), the norms kernel in divergence units.
@@ -512,65 +533,123 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
   e->hist.clear();
   Level& L0 = e->lv[0];
   if (solver == FH_SOLVER_MG_DC) {
-    // r = b - A p (no update) + norms; the zero-divergence guard of
-    // pressureMultigrid.ts:294-300 reads the same pass.
-    launch_set_int(e->s, e->done, 0);
-    launch_dc_update(e->s, e->p, e->p2, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
-                     e->partials, e->sums, nullptr, nullptr, 0.0);
-    LAUNCHED("dc_update");
-    swap_ptr(e->p, e->p2);
-    int st = read_sums(e);
-    if (st != FH_OK) return st;
-    if (e->h_sums[2] == 0.0) {
-      HIPCHK(hipMemsetAsync(e->p, 0, N * sizeof(double), e->s));
-      *out_iters = 0;
-      return FH_OK;
-    }
-    if (e->capture) {
-      HIPCHK(hipMemcpyAsync(e->psave, e->p, N * sizeof(double), hipMemcpyDeviceToDevice, e->s));
-      st = engine_capture_hook(e->capture, e->s, L0.b, e->lbl, e->psave, nullptr, N, e->ny,
-                               e->nx, true);
-      if (st != FH_OK) return st;
-    }
-    if (e->warm) {
-      // learned warm start: p += net correction, then the defect again
-      st = engine_warm_start(e->warm, e->s, L0.b, e->lbl, L0.info, e->p, e->ny, e->nx,
-                             e->wstats);
-      if (st != FH_OK) return st;
+    // r = b - A p (no update) + norms.  The zero-divergence guard of
+    // pressureMultigrid.ts:294-300 reads the same pass: on the device (the
+    // norm sets `done`, so the cycles enqueued behind it skip themselves)
+    // unless a capture or learned warm start needs the norm on the host now.
+    const bool sync_init = e->capture || e->warm;
+    int st;
+    if (sync_init) {
+      launch_set_int(e->s, e->done, 0);
       launch_dc_update(e->s, e->p, e->p2, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
                        e->partials, e->sums, nullptr, nullptr, 0.0);
       LAUNCHED("dc_update");
       swap_ptr(e->p, e->p2);
       st = read_sums(e);
       if (st != FH_OK) return st;
+      if (e->h_sums[2] == 0.0) {
+        HIPCHK(hipMemsetAsync(e->p, 0, N * sizeof(double), e->s));
+        *out_iters = 0;
+        return FH_OK;
+      }
+      if (e->capture) {
+        HIPCHK(hipMemcpyAsync(e->psave, e->p, N * sizeof(double), hipMemcpyDeviceToDevice, e->s));
+        st = engine_capture_hook(e->capture, e->s, L0.b, e->lbl, e->psave, nullptr, N, e->ny,
+                                 e->nx, true);
+        if (st != FH_OK) return st;
+      }
+      if (e->warm) {
+        // learned warm start: p += net correction, then the defect again
+        st = engine_warm_start(e->warm, e->s, L0.b, e->lbl, L0.info, e->p, e->ny, e->nx,
+                               e->wstats);
+        if (st != FH_OK) return st;
+        launch_dc_update(e->s, e->p, e->p2, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
+                         e->partials, e->sums, nullptr, nullptr, 0.0);
+        LAUNCHED("dc_update");
+        swap_ptr(e->p, e->p2);
+        st = read_sums(e);
+        if (st != FH_OK) return st;
+      }
+      e->hist.push_back(rel_resid(e->h_sums, scale, true));
+    } else {
+      launch_dc_correct(e->s, e->p, e->p2, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
+                        e->partials, nullptr);
+      launch_dc_norm_init(e->s, e->partials, e->nx, e->ny, e->sums, e->d_res + 16, e->done,
+                          scale);
+      LAUNCHED("dc_update");
+      swap_ptr(e->p, e->p2);
+      HIPCHK(hipEventRecord(e->cyc_ev[2], e->s));
     }
-    e->hist.push_back(rel_resid(e->h_sums, scale, true));
     // Cycles are enqueued one ahead of the host's convergence check: cycle
     // c goes in before the host waits for cycle c-1's norm, so the GPU never
     // idles on the host.  The norm kernel sets `done` on the device once the
     // tolerance is met, and every kernel of a cycle enqueued after that
-    // returns at entry, so no extra cycle is applied.
+    // returns at entry, so no extra cycle is applied.  Such a skipped cycle
+    // still costs its launches, so when cycle c-1's residual, predicted from
+    // the contraction measured so far (for the first cycle, the last solve's
+    // first-cycle contraction), is already within the tolerance, the host
+    // reads cycle c-1's norm before enqueuing cycle c instead.  A wrong
+    // guess costs either one skipped cycle or one host round trip, never a
+    // different result.
+    //
+    // At the top of iteration c, cycles 0..c-1 are enqueued and the norms of
+    // cycles 0..c-2 have been read.
     int used = iterations;
-    bool stopped = false;
+    bool stopped = false, init_pending = !sync_init;
+    const int trace_c0 = e->trace_n;
+    double r_init = sync_init ? e->hist.back() : -1.0;
+    double r_prev2 = -1.0, r_prev = r_init;  // the last two norms read
+    // The initial pass's norm, once its event (or any later one) has been
+    // waited for.  Returns true on zero divergence.
+    auto consume_init = [&]() -> bool {
+      if (!init_pending) return false;
+      init_pending = false;
+      const double* init = e->h_res + 16;
+      if (init[2] == 0.0) return true;
+      e->hist.push_back(rel_resid(init, scale, true));
+      r_init = r_prev = e->hist.back();
+      return false;
+    };
+    // Zero divergence: every enqueued cycle skipped itself; p = 0.
+    auto zero_solution = [&]() -> int {
+      e->trace_n = trace_c0;
+      e->hist.clear();
+      HIPCHK(hipMemsetAsync(e->p, 0, N * sizeof(double), e->s));
+      *out_iters = 0;
+      return FH_OK;
+    };
+    auto read_cycle = [&](int c) -> bool {  // true: cycle c met the tolerance
+      const double* r = e->h_res + 8 * (c & 1);
+      e->hist.push_back(r[4]);
+      r_prev2 = r_prev;
+      r_prev = r[4];
+      if (c == 0 && r_init > 0.0) e->rho_first = r[4] / r_init;
+      return r[5] != 0.0;
+    };
     for (int c = 0; c < iterations; c++) {
-      const int trace_mark = e->trace_n;
-      st = vcycle(e, e->nlev_dc, true, false, e->done);
-      if (st != FH_OK) return st;
-      {
-        Trace t(e, FH_K_DC_UPDATE);
-        launch_dc_correct(e->s, e->p, e->p2, L0.x, e->div, L0.info, L0.b, e->nx, e->ny, scale,
-                          e->partials, e->done);
+      bool prev_read = false;
+      if (c >= 1 && tol > 0) {
+        if (init_pending) {  // c == 1: cycle 0 keeps the GPU busy meanwhile
+          HIPCHK(hipEventSynchronize(e->cyc_ev[2]));
+          if (consume_init()) return zero_solution();
+        }
+        const double rho = c >= 2 ? r_prev / r_prev2 : e->rho_first;
+        if (r_prev > 0.0 && rho > 0.0 && r_prev * rho <= tol) {
+          HIPCHK(hipEventSynchronize(e->cyc_ev[(c - 1) & 1]));
+          if (read_cycle(c - 1)) {
+            used = c;
+            stopped = true;
+            break;
+          }
+          prev_read = true;
+        }
       }
-      launch_dc_norm(e->s, e->partials, e->nx, e->ny, e->sums, e->d_res + 8 * (c & 1), e->done,
-                     scale, tol);
-      LAUNCHED("dc_update");
-      swap_ptr(e->p, e->p2);
-      HIPCHK(hipEventRecord(e->cyc_ev[c & 1], e->s));
-      if (c > 0 && tol > 0) {
+      const int trace_mark = e->trace_n;
+      st = enqueue_dc_cycle(e, c, scale, tol);
+      if (st != FH_OK) return st;
+      if (c > 0 && tol > 0 && !prev_read) {
         HIPCHK(hipEventSynchronize(e->cyc_ev[(c - 1) & 1]));
-        const double* r = e->h_res + 8 * ((c - 1) & 1);
-        e->hist.push_back(r[4]);
-        if (r[5] != 0.0) {
+        if (read_cycle(c - 1)) {
           used = c;  // cycle c was skipped on the device
           // p/p2 were swapped for the skipped cycle's no-op update, and its
           // kernels' trace records time only their early return
@@ -584,7 +663,12 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
     if (!stopped && iterations > 0) {
       int c = iterations - 1;
       HIPCHK(hipEventSynchronize(e->cyc_ev[c & 1]));
+      if (consume_init()) return zero_solution();
       e->hist.push_back(e->h_res[8 * (c & 1) + 4]);
+    }
+    if (init_pending) {  // no cycle ran
+      HIPCHK(hipStreamSynchronize(e->s));
+      if (consume_init()) return zero_solution();
     }
     *out_iters = used;
     if (e->capture) {
@@ -790,7 +874,7 @@ fh_engine* fh_create(int nx, int ny, int dye_scale, double h) {
       dalloc(e, &e->fpartials, 1024) || dalloc(e, &e->fmax, 1) || dalloc(e, &e->done, 1))
     return bail();
   if (hipHostMalloc((void**)&e->h_sums, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&e->h_res, 16 * sizeof(double),
+      hipHostMalloc((void**)&e->h_res, 24 * sizeof(double),
                     hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
       hipHostGetDevicePointer((void**)&e->d_res, e->h_res, 0) != hipSuccess ||
       hipHostMalloc((void**)&e->h_fmax, sizeof(float), hipHostMallocDefault) != hipSuccess) {

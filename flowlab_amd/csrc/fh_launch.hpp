@@ -95,6 +95,12 @@ void launch_dc_correct(hipStream_t s, const double* po, double* pn, const float*
                        double scale, double* partials, const int* done);
 void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                     double* host_sums, int* done, double scale, double tol);
+// The norm of the pass before the first cycle: runs whatever *done holds and
+// sets *done (and sums[5]) to 1 when the divergence is zero, 0 otherwise
+// (pressureMultigrid.ts:294-300), so the cycles enqueued behind it skip
+// themselves without the host reading the norm first.
+void launch_dc_norm_init(hipStream_t s, const double* partials, int nx, int ny, double* sums,
+                         double* host_sums, int* done, double scale);
 // Norms of the current FP32 or FP64 pressure: sums as in launch_dc_update,
 // r computed as div + (count p - sum)/scale (pressure.ts:102-142).
 void launch_norms(hipStream_t s, const float* p32, const double* p64, const double* div,

@@ -275,8 +275,8 @@ __global__ __launch_bounds__(1024) void k_finalize_sum4(const double* __restrict
                                                         int nblocks, double* __restrict__ out,
                                                         double* __restrict__ host_out,
                                                         int* __restrict__ done, double scale,
-                                                        double tol) {
-  if (done && *done) return;
+                                                        double tol, int init) {
+  if (!init && done && *done) return;
   __shared__ double sh[4][16];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   constexpr int K = kMaxPartials / 1024;
@@ -313,19 +313,21 @@ __global__ __launch_bounds__(1024) void k_finalize_sum4(const double* __restrict
     double rel = 0.0;
     if (v[1] > 0 && v[2] > 0) rel = sqrt(v[0] / v[1]) / (fabs(scale) * sqrt(v[2] / v[3]));
     out[4] = rel;
-    bool conv = tol > 0 && rel <= tol;
-    out[5] = conv ? 1.0 : 0.0;
+    // init (the pass before the first cycle): *done is (re)set from the
+    // zero-divergence guard alone; otherwise it is set once rel <= tol.
+    bool stop = init ? v[2] == 0.0 : (tol > 0 && rel <= tol);
+    out[5] = stop ? 1.0 : 0.0;
     if (host_out) {
       for (int s = 0; s < 4; s++) host_out[s] = v[s];
       host_out[4] = rel;
       host_out[5] = out[5];
     }
-    if (done && conv) *done = 1;
+    if (done && (stop || init)) *done = stop ? 1 : 0;
   }
 }
 
 void launch_finalize_sum4(hipStream_t s, const double* partials, int nblocks, double* out) {
-  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, nblocks, out, nullptr, nullptr, 1.0, 0.0);
+  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, nblocks, out, nullptr, nullptr, 1.0, 0.0, 0);
 }
 
 __global__ void k_set_int(int* p, int v) { *p = v; }
@@ -390,7 +392,13 @@ void launch_dc_correct(hipStream_t s, const double* po, double* pn, const float*
 void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                     double* host_sums, int* done, double scale, double tol) {
   dim3 g = reduce_grid(nx, ny);
-  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, g.x * g.y, sums, host_sums, done, scale, tol);
+  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, g.x * g.y, sums, host_sums, done, scale, tol, 0);
+}
+
+void launch_dc_norm_init(hipStream_t s, const double* partials, int nx, int ny, double* sums,
+                         double* host_sums, int* done, double scale) {
+  dim3 g = reduce_grid(nx, ny);
+  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, g.x * g.y, sums, host_sums, done, scale, 0.0, 1);
 }
 
 void launch_norms(hipStream_t s, const float* p32, const double* p64, const double* div,

@@ -344,3 +344,38 @@ def test_full_size_karman_projection_property():
     div_after = sim.div.reshape(ny, nx)[interior]
     div_before = d.reshape(ny, nx)[interior]
     assert np.sqrt(np.mean(div_after ** 2)) < 1e-4 * max(np.sqrt(np.mean(div_before ** 2)), 1)
+
+
+@pytest.mark.parametrize("iterations,tol", [(60, 1e-6), (1, 0.0), (5, 0.0), (0, 1e-6)])
+def test_mg_dc_zero_divergence_guard(iterations, tol):
+    """pressureMultigrid.ts:294-300: zero divergence gives p = 0 and 0 cycles,
+    whatever the warm start, and the next solve on the engine runs normally."""
+    from flowlab_amd.engine import HipPressureSolver
+    nx, ny = 96, 64
+    lab, div = boundary_fixture(nx, ny)
+    sol = HipPressureSolver(nx, ny, "mg-dc")
+    p = np.linspace(-1.0, 1.0, nx * ny)  # a leftover warm start
+    it = sol.solve(None, p, np.zeros(nx * ny), lab, 0.02, iterations, 1.0, tol)
+    assert it == 0 and not p.any()
+    assert len(sol.engine.solve_history()) == 0
+    it = sol.solve(None, p, div, lab, 0.02, 60, 1.0, 1e-6)
+    assert 0 < it < 60
+    assert O.relative_residual(nx, ny, p, div, lab, 0.02) <= 1e-6
+
+
+def test_mg_dc_stops_at_the_first_cycle_within_tolerance():
+    """Cycle count is minimal however the host schedules the cycles: every
+    history entry but the last is above the tolerance, repeated warm-started
+    solves included."""
+    from flowlab_amd.engine import HipPressureSolver
+    nx, ny = 256, 192
+    lab, div = boundary_fixture(nx, ny)
+    sol = HipPressureSolver(nx, ny, "mg-dc")
+    p = np.zeros(nx * ny)
+    for k in range(4):
+        d = div * (1.0 + 0.05 * k)
+        it = sol.solve(None, p, d, lab, 0.02, 60, 1.0, 1e-6)
+        hist = sol.engine.solve_history()
+        assert len(hist) == it + 1
+        assert hist[-1] <= 1e-6 and all(h > 1e-6 for h in hist[:-1])
+        assert O.relative_residual(nx, ny, p, d, lab, 0.02) <= 1e-6
