@@ -109,12 +109,24 @@ def run_gpu(args, unet=None, trace=True, tiles=None):
     else:
         for level, name in enumerate(tiles):
             e.set_tile(name, level)
+    # Timed pass: K steps with nothing else in the loop (per-step phase
+    # timing would make the host wait for every step's end).
+    e.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.step()
+    e.synchronize()
+    elapsed = time.perf_counter() - t0
+    # Instrumented pass over the same K steps (the run is deterministic:
+    # reset, the same warm-up, then K steps with solve times, cycle counts,
+    # residuals and the roofline kernel's launch times recorded).
+    sim.reset(scene)
+    for _ in range(args.warmup):
+        sim.step()
     e.set_timing(True)
     if trace:
         e.trace_kernel(args.kernel)
-    e.synchronize()
     solve_ms, iters, hist_max = [], [], 0.0
-    t0 = time.perf_counter()
     for _ in range(args.steps):
         sim.step()
         ph = e.timings()
@@ -122,7 +134,6 @@ def run_gpu(args, unet=None, trace=True, tiles=None):
         iters.append(sim.iters)
         hist_max = max(hist_max, float(e.solve_history()[-1]))
     e.synchronize()
-    elapsed = time.perf_counter() - t0
     n_launch, k_ms, bracket_ms = 0, 0.0, 0.0
     if trace:
         n_launch, k_ms = e.trace_stats()
@@ -208,8 +219,9 @@ LARGE_KERNELS = ("dc_update", "smooth_restrict_fine", "prolong_smooth_fine", "ma
 def run_large(args):
     """One n x n domain (default 8192^2, 67 M cells): closed box, 200 seeded
     circular obstacles, 16 buoyant heat sources (buoyancy 1), MacCormack,
-    MG-DC to 1e-6.  Times K steps, then traces each large kernel for one
-    step and reports its algorithmic GB/s against the HBM peak."""
+    MG-DC to 1e-6.  Times K steps, replays them instrumented (solve times,
+    cycles), then traces each large kernel for one step and reports its
+    algorithmic GB/s against the HBM peak."""
     from flowlab_amd import scenes as S
     from flowlab_amd.simulation import Simulation
 
@@ -222,17 +234,23 @@ def run_large(args):
     e = sim.stepper.engine
     for _ in range(2):
         sim.step()
-    e.set_timing(True)
     e.synchronize()
-    solve_ms, iters, res = [], [], 0.0
     t0 = time.perf_counter()
+    for _ in range(args.large_steps):
+        sim.step()
+    e.synchronize()
+    elapsed = time.perf_counter() - t0
+    sim.reset(scene)  # the same steps again, instrumented
+    for _ in range(2):
+        sim.step()
+    e.set_timing(True)
+    solve_ms, iters, res = [], [], 0.0
     for _ in range(args.large_steps):
         sim.step()
         solve_ms.append(e.timings()["solve"])
         iters.append(sim.iters)
         res = max(res, float(e.solve_history()[-1]))
     e.synchronize()
-    elapsed = time.perf_counter() - t0
     bracket = e.trace_overhead(200)
     kernels = {}
     for k in LARGE_KERNELS:

@@ -93,6 +93,11 @@ struct fh_engine {
   float* fmax = nullptr;
   double* h_sums = nullptr;  // pinned
   float* h_fmax = nullptr;   // pinned
+  // Largest face speed of the resident velocity: every write of u/v bumps
+  // vel_epoch; a step enqueues the reduction of its final velocity (and the
+  // copy to h_fmax, then fmax_ev) and tags it with fmax_epoch.
+  long vel_epoch = 0, fmax_epoch = -1;
+  hipEvent_t fmax_ev = nullptr;
   // defect-correction loop: device stop flag, two host result slots (one per
   // cycle in flight) that the norm kernel writes, and the events that mark
   // those kernels' completion
@@ -754,6 +759,7 @@ int advect_velocity(fh_engine* e, int scheme, double dt) {
   LAUNCHED("advect_velocity");
   swap_ptr(e->u, e->u2);
   swap_ptr(e->v, e->v2);
+  e->vel_epoch++;
   return FH_OK;
 }
 
@@ -765,6 +771,7 @@ int forces(fh_engine* e, double dt) {
     launch_forces(e->s, e->u, e->v, e->lbl, e->dye, e->w, e->nx, e->ny, e->ds, (float)e->h,
                   (float)dt, gain, eps, e->stir);
   e->stir.active = 0;  // one frame's displacement (simulation.ts:69-72)
+  e->vel_epoch++;
   LAUNCHED("forces");
   return FH_OK;
 }
@@ -772,6 +779,7 @@ int forces(fh_engine* e, double dt) {
 int project(fh_engine* e, double grad_scale) {
   launch_subtract_gradient(e->s, e->u, e->v, e->p, e->lbl, e->nx, e->ny, grad_scale);
   launch_outflow(e->s, e->u, e->v, e->lbl, e->nx, e->ny);
+  e->vel_epoch++;
   LAUNCHED("project");
   return FH_OK;
 }
@@ -891,6 +899,10 @@ fh_engine* fh_create(int nx, int ny, int dye_scale, double h) {
       set_err("fh_create: hipEventCreate failed");
       return bail();
     }
+  if (hipEventCreateWithFlags(&e->fmax_ev, hipEventDisableTiming) != hipSuccess) {
+    set_err("fh_create: hipEventCreate failed");
+    return bail();
+  }
   return e;
 }
 
@@ -914,6 +926,7 @@ void fh_destroy(fh_engine* e) {
     if (ev) (void)hipEventDestroy(ev);
   for (hipEvent_t ev : e->cyc_ev)
     if (ev) (void)hipEventDestroy(ev);
+  if (e->fmax_ev) (void)hipEventDestroy(e->fmax_ev);
   for (size_t l = 0; l < e->lv.size(); l++) {
     if (l > 0 && e->lv[l].lbl) (void)hipFree(e->lv[l].lbl);
     if (e->lv[l].x) (void)hipFree(e->lv[l].x);
@@ -935,6 +948,7 @@ int fh_upload(fh_engine* e, const double* u, const double* v, const double* p,
               const uint8_t* label, const double* dye) {
   GUARD("HipStepper.upload()");
   int st;
+  e->vel_epoch++;
   if (u && (st = up32(e, e->u, u, n_u(e)))) return st;
   if (v && (st = up32(e, e->v, v, n_v(e)))) return st;
   if (dye && (st = up32(e, e->dye, dye, 3 * n_d(e)))) return st;
@@ -954,6 +968,7 @@ int fh_set_labels(fh_engine* e, const uint8_t* label) {
   }
   HIPCHK(hipMemcpy(e->lbl, label, n_c(e), hipMemcpyHostToDevice));
   launch_commit_labels(e->s, e->p, e->u, e->v, e->dye, e->lbl, e->nx, e->ny, (int)n_d(e));
+  e->vel_epoch++;
   LAUNCHED("commit_labels");
   HIPCHK(hipStreamSynchronize(e->s));
   e->labels_dirty = true;
@@ -1044,6 +1059,14 @@ int fh_step(fh_engine* e, int scheme, double dt, double scale, double grad_scale
   if ((st = solve(e, solver, scale, iterations, omega, tol, out_iters))) return st;
   if ((st = mark(e, 4))) return st;
   if ((st = project(e, grad_scale))) return st;
+  // The next step's CFL reads this velocity (simulation.ts:474-477); the dye
+  // phase below leaves it unchanged, so its reduction and copy go here and
+  // the host's wait for them overlaps the dye kernels.
+  launch_max_speed(e->s, e->u, e->v, e->nx, e->ny, e->fpartials, e->counter, e->fmax);
+  LAUNCHED("max_speed");
+  HIPCHK(hipMemcpyAsync(e->h_fmax, e->fmax, sizeof(float), hipMemcpyDeviceToHost, e->s));
+  HIPCHK(hipEventRecord(e->fmax_ev, e->s));
+  e->fmax_epoch = e->vel_epoch;
   if ((st = mark(e, 5))) return st;
   if ((st = dye_phase(e, scheme, dt, keep))) return st;
   if ((st = mark(e, 6))) return st;
@@ -1084,6 +1107,11 @@ int fh_read_divergence(fh_engine* e, double* div) {
 int fh_max_face_speed(fh_engine* e, double* out) {
   GUARD("HipStepper.maxFaceSpeed()");
   if (!out) return FH_ERR_BAD_ARG;
+  if (e->fmax_epoch == e->vel_epoch) {  // reduced by the last step, velocity unchanged since
+    HIPCHK(hipEventSynchronize(e->fmax_ev));
+    *out = (double)*e->h_fmax;
+    return FH_OK;
+  }
   launch_max_speed(e->s, e->u, e->v, e->nx, e->ny, e->fpartials, e->counter, e->fmax);
   LAUNCHED("max_speed");
   HIPCHK(hipMemcpyAsync(e->h_fmax, e->fmax, sizeof(float), hipMemcpyDeviceToHost, e->s));
@@ -1121,6 +1149,7 @@ int fh_advect_velocity(fh_engine* e, int scheme, const double* u_in, const doubl
   GUARD("HipAdvector.velocity()");
   if (!u_in || !v_in || !u_out || !v_out || !label) return FH_ERR_BAD_ARG;
   int st;
+  e->vel_epoch++;
   if ((st = up32(e, e->u, u_in, n_u(e))) || (st = up32(e, e->v, v_in, n_v(e)))) return st;
   HIPCHK(hipMemcpy(e->lbl, label, n_c(e), hipMemcpyHostToDevice));
   e->labels_dirty = true;
@@ -1134,6 +1163,7 @@ int fh_advect_dye(fh_engine* e, int scheme, const double* u, const double* v, co
   GUARD("HipAdvector.dye()");
   if (!u || !v || !q_in || !q_out || !label) return FH_ERR_BAD_ARG;
   int st;
+  e->vel_epoch++;
   if ((st = up32(e, e->u, u, n_u(e))) || (st = up32(e, e->v, v, n_v(e))) ||
       (st = up32(e, e->dye, q_in, 3 * n_d(e))))
     return st;

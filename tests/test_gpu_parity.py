@@ -379,3 +379,23 @@ def test_mg_dc_stops_at_the_first_cycle_within_tolerance():
         assert len(hist) == it + 1
         assert hist[-1] <= 1e-6 and all(h > 1e-6 for h in hist[:-1])
         assert O.relative_residual(nx, ny, p, d, lab, 0.02) <= 1e-6
+
+
+def test_step_max_face_speed_tracks_the_velocity():
+    """simulation.ts:451-459: the largest face speed of the CURRENT velocity.
+    A step reduces its final velocity for the next step's CFL; any later
+    write of the velocity (here an upload) must not see that value."""
+    from flowlab_amd.simulation import Simulation
+    nx, ny = 256, 128
+    g, scene = S.karman_street(nx, ny)
+    sim = Simulation(nx, ny, dict(scheme="macCormack", cfl_target=0.5, solver="mg-dc",
+                                  pressure_iters=60, tol=1e-6, residual_every=0), readback=True)
+    sim.reset(scene)
+    e = sim.stepper.engine
+    for _ in range(3):
+        sim.step()
+        exp = max(np.abs(sim.f.u).max(), np.abs(sim.f.v).max())  # read back, f32 values
+        assert e.max_face_speed() == exp
+    u2, v2 = 3.0 * sim.f.u, 3.0 * sim.f.v
+    e.upload(u2, v2, None, None, None)
+    assert e.max_face_speed() == max(np.abs(f32(u2)).max(), np.abs(f32(v2)).max())
