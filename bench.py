@@ -39,6 +39,14 @@ HBM_PEAK_GBS = 8000.0
 #   smooth_restrict_fine: b 4 + label 1 + x 4 written + coarse b 1 (x starts
 #     at zero, so it is not read).
 #   prolong_smooth_fine: x 4 + b 4 + label 1 + coarse e 1 + x 4 written.
+#   fcg_dots: z 4 + previous direction 4 + r 8 + info 1.
+#   fcg_update: z 4 + previous direction 4 + p 8 + r 8 + info 1 read;
+#     direction 4 + p 8 + r 8 + r32 4 written.
+SOLVER_NAMES = {
+    "mg-dc": "MG-DC V(2,2) to 16x8",
+    "mg-fcg": "one MG-DC V(2,2) cycle (to 16x8), then flexible CG preconditioned by it",
+}
+
 ALGO_BYTES_PER_CELL = {
     "smooth_restrict_fine": 10.0,
     "prolong_smooth_fine": 14.0,
@@ -48,6 +56,8 @@ ALGO_BYTES_PER_CELL = {
     "maccormack_dye": 45.0,
     "advect_dye": 33.0,
     "dc_update": 33.0,
+    "fcg_dots": 17.0,
+    "fcg_update": 49.0,
 }
 
 
@@ -58,7 +68,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--nx", type=int, default=2048)
     ap.add_argument("--ny", type=int, default=1024)
-    ap.add_argument("--kernel", default="dc_update", help="kernel traced for the roofline")
+    ap.add_argument("--kernel", default=None,
+                    help="kernel traced for the roofline (default: the solver's fine-level "
+                         "update, dc_update or fcg_update)")
+    ap.add_argument("--solver", default="mg-fcg", choices=sorted(SOLVER_NAMES),
+                    help="pressure solver of the GPU legs (tolerance 1e-6 either way)")
     ap.add_argument("--cpu-steps", type=int, default=4, help="oracle steps timed (0 = skip)")
     ap.add_argument("--batch-scenes", type=int, default=0,
                     help="scenes of the batched decaying-turbulence run (0 = skip; the "
@@ -70,7 +84,10 @@ def parse():
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (the known-size "
                          "copy kernel that scales hardware traffic counters)")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.kernel is None:
+        args.kernel = "fcg_update" if args.solver == "mg-fcg" else "dc_update"
+    return args
 
 
 PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_karman_2048x1024.json")
@@ -95,7 +112,7 @@ def run_gpu(args, unet=None, trace=True, tiles=None):
 
     nx, ny = args.nx, args.ny
     g, scene = S.karman_street(nx, ny)
-    params = dict(scheme="macCormack", cfl_target=0.5, solver="mg-dc", pressure_iters=100,
+    params = dict(scheme="macCormack", cfl_target=0.5, solver=args.solver, pressure_iters=100,
                   tol=1e-6, residual_every=0)
     sim = Simulation(nx, ny, params, readback=False)
     sim.reset(scene)
@@ -227,7 +244,7 @@ def run_large(args):
 
     n = args.large_n
     g, scene = S.large_plume(n)
-    params = dict(scheme="macCormack", cfl_target=0.5, dt_max=0.5 * g.h, solver="mg-dc",
+    params = dict(scheme="macCormack", cfl_target=0.5, dt_max=0.5 * g.h, solver=args.solver,
                   pressure_iters=100, tol=1e-6, residual_every=0, buoyancy=1.0)
     sim = Simulation(n, n, params, readback=False)
     sim.reset(scene)
@@ -267,7 +284,8 @@ def run_large(args):
     K = args.large_steps
     return {
         "workload": f"thermal plume {n}x{n}, closed box, 200 circular obstacles, 16 buoyant "
-                    "sources, MacCormack, MG-DC to 1e-6, SplitMix64(20240613)",
+                    f"sources, MacCormack, {SOLVER_NAMES[args.solver]}, tol 1e-6, "
+                    "SplitMix64(20240613)",
         "cells": n * n,
         "steps": K,
         "value": n * n * K / elapsed,
@@ -340,10 +358,10 @@ def main():
         "dtype": "f32 (advection, smoothing, transfers) + f64 (pressure, residual, norms)",
         "data": "synthetic",
         "config": {
-            "workload": f"karman vortex street {nx}x{ny}, MacCormack, MG-DC V(2,2) to 16x8, "
-                        "tol 1e-6, CFL 0.5",
+            "workload": f"karman vortex street {nx}x{ny}, MacCormack, "
+                        f"{SOLVER_NAMES[args.solver]}, tol 1e-6, CFL 0.5",
             "grid": [nx, ny],
-            "solver": "mg-dc",
+            "solver": args.solver,
             "tol": 1e-6,
             "mg_tiles": r["tiles"],
         },

@@ -182,13 +182,16 @@ __device__ __forceinline__ void backtracef(const FU& uc, const FV& vc, int nx, i
 // each with its own L2.  Remapping the dispatch index gives every XCD a
 // contiguous run of logical blocks (row-major over the block grid), so the
 // halos that neighbouring blocks share are fetched into one L2 instead of
-// several.  Identity when the block count is not a multiple of 8.
+// several.  With n = 8q + r blocks, XCD x runs the q + (x < r) dispatch
+// indices x, x+8, ...; the k-th of them becomes logical block
+// x q + min(x, r) + k, a bijection onto [0, n).
 __device__ __forceinline__ void xcd_block(int* bx, int* by) {
   const int gx = (int)gridDim.x, n = gx * (int)gridDim.y;
-  int b = (int)blockIdx.x + (int)blockIdx.y * gx;
-  if ((n & 7) == 0) b = (b & 7) * (n >> 3) + (b >> 3);
-  *by = b / gx;
-  *bx = b - *by * gx;
+  const int b = (int)blockIdx.x + (int)blockIdx.y * gx;
+  const int x = b & 7, q = n >> 3, r = n & 7;
+  const int l = x * q + (x < r ? x : r) + (b >> 3);
+  *by = l / gx;
+  *bx = l - *by * gx;
 }
 
 // ---- reductions ---------------------------------------------------------

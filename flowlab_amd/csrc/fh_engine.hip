@@ -73,6 +73,11 @@ struct fh_engine {
   float *dye = nullptr, *dye2 = nullptr, *dyeh = nullptr, *w = nullptr;
   uint8_t* lbl = nullptr;
   double *p = nullptr, *p2 = nullptr, *div = nullptr;
+  // flexible CG (allocated by the first such solve): FP64 residual, two FP32
+  // directions (ping-pong by cycle parity), {alpha, beta, <d, A d>}
+  double* rvec = nullptr;
+  float* dvec[2] = {nullptr, nullptr};
+  double* fcg = nullptr;
   // multigrid: levels for coarsest side 4 (reference); the defect-correction
   // hierarchy (coarsest side 16) is a prefix of it.
   std::vector<Level> lv;
@@ -480,21 +485,57 @@ int vcycle(fh_engine* e, int nlev, bool dc, bool warm, const int* done) {
 }
 
 // One defect-correction cycle c: V-cycle on the correction, p += e with the
-// new residual and its norm (to host slot c & 1), then cycle event c & 1.
-int enqueue_dc_cycle(fh_engine* e, int c, double scale, double tol) {
+// new residual (also to r64 when given) and its norm (to host slot c & 1),
+// then cycle event c & 1.
+int enqueue_dc_cycle(fh_engine* e, int c, double scale, double tol, double* r64 = nullptr) {
   Level& L0 = e->lv[0];
   int st = vcycle(e, e->nlev_dc, true, false, e->done);
   if (st != FH_OK) return st;
   {
     Trace t(e, FH_K_DC_UPDATE);
     launch_dc_correct(e->s, e->p, e->p2, L0.x, e->div, L0.info, L0.b, e->nx, e->ny, scale,
-                      e->partials, e->done);
+                      e->partials, e->done, r64);
   }
   launch_dc_norm(e->s, e->partials, e->nx, e->ny, e->sums, e->d_res + 8 * (c & 1), e->done, scale,
                  tol);
   LAUNCHED("dc_update");
   swap_ptr(e->p, e->p2);
   HIPCHK(hipEventRecord(e->cyc_ev[c & 1], e->s));
+  return FH_OK;
+}
+
+// One flexible-CG cycle c >= 1 (cycle 0 of the solve is a defect-correction
+// cycle): z = V-cycle(r32), then the CG step (p and r in place) and its norm
+// (to host slot c & 1), then cycle event c & 1.
+int enqueue_fcg_cycle(fh_engine* e, int c, double scale, double tol) {
+  Level& L0 = e->lv[0];
+  int st = vcycle(e, e->nlev_dc, true, false, e->done);
+  if (st != FH_OK) return st;
+  const int first = c == 1 ? 1 : 0;
+  {
+    Trace t(e, FH_K_FCG_DOTS);
+    launch_fcg_dots(e->s, L0.x, e->dvec[c & 1], e->rvec, L0.info, e->nx, e->ny, first,
+                    e->partials, e->fcg, e->done);
+  }
+  {
+    Trace t(e, FH_K_FCG_UPDATE);
+    launch_fcg_update(e->s, L0.x, e->dvec[c & 1], e->dvec[(c + 1) & 1], e->p, e->rvec, L0.info,
+                      L0.b, e->nx, e->ny, first, e->fcg, e->partials, e->done);
+  }
+  launch_fcg_norm(e->s, e->partials, e->nx, e->ny, e->sums, e->sums, e->d_res + 8 * (c & 1),
+                  e->done, scale, tol);
+  LAUNCHED("fcg_step");
+  HIPCHK(hipEventRecord(e->cyc_ev[c & 1], e->s));
+  return FH_OK;
+}
+
+int ensure_fcg(fh_engine* e) {
+  if (e->rvec) return FH_OK;
+  const size_t n = (size_t)e->nx * e->ny;
+  int st;
+  if ((st = dalloc(e, &e->rvec, n)) || (st = dalloc(e, &e->dvec[0], n)) ||
+      (st = dalloc(e, &e->dvec[1], n)) || (st = dalloc(e, &e->fcg, (size_t)16)))
+    return st;
   return FH_OK;
 }
 
@@ -537,7 +578,13 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
   }
   e->hist.clear();
   Level& L0 = e->lv[0];
-  if (solver == FH_SOLVER_MG_DC) {
+  if (solver == FH_SOLVER_MG_DC || solver == FH_SOLVER_MG_FCG) {
+    const bool fcg = solver == FH_SOLVER_MG_FCG;
+    if (fcg) {
+      int st0 = ensure_fcg(e);
+      if (st0 != FH_OK) return st0;
+    }
+    double* r64 = fcg ? e->rvec : nullptr;  // CG keeps the FP64 residual
     // r = b - A p (no update) + norms.  The zero-divergence guard of
     // pressureMultigrid.ts:294-300 reads the same pass: on the device (the
     // norm sets `done`, so the cycles enqueued behind it skip themselves)
@@ -546,8 +593,9 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
     int st;
     if (sync_init) {
       launch_set_int(e->s, e->done, 0);
-      launch_dc_update(e->s, e->p, e->p2, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
-                       e->partials, e->sums, nullptr, nullptr, 0.0);
+      launch_dc_correct(e->s, e->p, e->p2, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
+                        e->partials, nullptr, r64);
+      launch_dc_norm(e->s, e->partials, e->nx, e->ny, e->sums, nullptr, nullptr, scale, 0.0);
       LAUNCHED("dc_update");
       swap_ptr(e->p, e->p2);
       st = read_sums(e);
@@ -568,8 +616,9 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
         st = engine_warm_start(e->warm, e->s, L0.b, e->lbl, L0.info, e->p, e->ny, e->nx,
                                e->wstats);
         if (st != FH_OK) return st;
-        launch_dc_update(e->s, e->p, e->p2, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
-                         e->partials, e->sums, nullptr, nullptr, 0.0);
+        launch_dc_correct(e->s, e->p, e->p2, nullptr, e->div, L0.info, L0.b, e->nx, e->ny,
+                          scale, e->partials, nullptr, r64);
+        launch_dc_norm(e->s, e->partials, e->nx, e->ny, e->sums, nullptr, nullptr, scale, 0.0);
         LAUNCHED("dc_update");
         swap_ptr(e->p, e->p2);
         st = read_sums(e);
@@ -578,7 +627,7 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
       e->hist.push_back(rel_resid(e->h_sums, scale, true));
     } else {
       launch_dc_correct(e->s, e->p, e->p2, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
-                        e->partials, nullptr);
+                        e->partials, nullptr, r64);
       launch_dc_norm_init(e->s, e->partials, e->nx, e->ny, e->sums, e->d_res + 16, e->done,
                           scale);
       LAUNCHED("dc_update");
@@ -650,15 +699,17 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
         }
       }
       const int trace_mark = e->trace_n;
-      st = enqueue_dc_cycle(e, c, scale, tol);
+      st = (fcg && c > 0) ? enqueue_fcg_cycle(e, c, scale, tol)
+                          : enqueue_dc_cycle(e, c, scale, tol, r64);
       if (st != FH_OK) return st;
       if (c > 0 && tol > 0 && !prev_read) {
         HIPCHK(hipEventSynchronize(e->cyc_ev[(c - 1) & 1]));
         if (read_cycle(c - 1)) {
           used = c;  // cycle c was skipped on the device
-          // p/p2 were swapped for the skipped cycle's no-op update, and its
+          // p/p2 were swapped for the skipped cycle's no-op update (a CG
+          // step, c > 0 of FH_SOLVER_MG_FCG, updates p in place), and its
           // kernels' trace records time only their early return
-          swap_ptr(e->p, e->p2);
+          if (!(fcg && c > 0)) swap_ptr(e->p, e->p2);
           e->trace_n = trace_mark;
           stopped = true;
           break;
@@ -910,10 +961,10 @@ void fh_destroy(fh_engine* e) {
   if (!e) return;
   if (e->s) (void)hipStreamSynchronize(e->s);
   float* fbufs[] = {e->u, e->v, e->u2, e->v2, e->uh, e->vh, e->dye, e->dye2, e->dyeh, e->w,
-                    e->patch, e->fpartials, e->fmax};
+                    e->patch, e->fpartials, e->fmax, e->dvec[0], e->dvec[1]};
   for (float* p : fbufs)
     if (p) (void)hipFree(p);
-  double* dbufs[] = {e->p, e->p2, e->div, e->M, e->partials, e->sums};
+  double* dbufs[] = {e->p, e->p2, e->div, e->M, e->partials, e->sums, e->rvec, e->fcg};
   for (double* p : dbufs)
     if (p) (void)hipFree(p);
   if (e->lbl) (void)hipFree(e->lbl);

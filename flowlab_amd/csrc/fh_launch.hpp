@@ -92,7 +92,7 @@ void launch_dc_update(hipStream_t s, const double* po, double* pn, const float* 
 // The two kernels of launch_dc_update, for callers that time them apart.
 void launch_dc_correct(hipStream_t s, const double* po, double* pn, const float* e,
                        const double* div, const uint8_t* info, float* r32, int nx, int ny,
-                       double scale, double* partials, const int* done);
+                       double scale, double* partials, const int* done, double* r64 = nullptr);
 void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                     double* host_sums, int* done, double scale, double tol);
 // The norm of the pass before the first cycle: runs whatever *done holds and
@@ -101,6 +101,21 @@ void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, doubl
 // themselves without the host reading the norm first.
 void launch_dc_norm_init(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                          double* host_sums, int* done, double scale);
+// Flexible CG after the V-cycle has left z = B r in z (first: the first
+// step of a solve, beta = 0 and dprev unread).  launch_fcg_dots: the four
+// dot products and the coefficients, state = {alpha, beta, <d, A d>}.
+// launch_fcg_update: d = z + beta dprev into dnext, p += alpha d,
+// r -= alpha A d, r32 = (float) r, norm partials.  launch_fcg_norm: the
+// norm as launch_dc_norm writes it, with the divergence terms taken from
+// sums0 (the initial pass's sums).
+void launch_fcg_dots(hipStream_t s, const float* z, const float* dprev, const double* r,
+                     const uint8_t* info, int nx, int ny, int first, double* partials,
+                     double* state, const int* done);
+void launch_fcg_update(hipStream_t s, const float* z, const float* dprev, float* dnext, double* p,
+                       double* r, const uint8_t* info, float* r32, int nx, int ny, int first,
+                       const double* state, double* partials, const int* done);
+void launch_fcg_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
+                     const double* sums0, double* host_sums, int* done, double scale, double tol);
 // Norms of the current FP32 or FP64 pressure: sums as in launch_dc_update,
 // r computed as div + (count p - sum)/scale (pressure.ts:102-142).
 void launch_norms(hipStream_t s, const float* p32, const double* p64, const double* div,

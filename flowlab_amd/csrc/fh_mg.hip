@@ -186,7 +186,8 @@ static inline dim3 reduce_grid(int nx, int ny) {
 __global__ __launch_bounds__(256) void k_dc_update(
     const double* __restrict__ po, double* __restrict__ pn, const float* __restrict__ e,
     const double* __restrict__ div, const uint8_t* __restrict__ info, float* __restrict__ r32,
-    int nx, int ny, double scale, double* partials, const int* __restrict__ done) {
+    double* __restrict__ r64, int nx, int ny, double scale, double* partials,
+    const int* __restrict__ done) {
   if (done && *done) return;
   int bx, by;
   xcd_block(&bx, &by);
@@ -225,6 +226,155 @@ __global__ __launch_bounds__(256) void k_dc_update(
       }
     }
     r32[k] = (float)r;
+    if (r64) r64[k] = r;
+  }
+  block_sum_store<4>(acc, partials);
+}
+
+// ---- flexible conjugate gradients (FCG(1)) around the V-cycle -------------
+// The preconditioner B is one V(2,2) cycle from zero on r (FP32); it is not
+// symmetric (pre- and post-smoothing sweep in the same colour order, and
+// restriction is not the transpose of prolongation), so the direction is
+// A-orthogonalised against the previous one explicitly (Notay's flexible
+// CG with one stored direction) instead of by the Fletcher-Reeves ratio:
+//   z = B r;  beta = -<z, A d'> / <d', A d'>;  d = z + beta d';
+//   alpha = <d, r> / <d, A d>;  p += alpha d;  r -= alpha A d.
+// A solve's first cycle is a plain defect-correction cycle (p += B r, the
+// residual recomputed from p), so a solve that one V-cycle completes costs
+// what defect correction costs; CG starts from its result.  <d, r> is formed
+// explicitly rather than assumed 0 against d'.
+// A is the 5-point operator of the fluid cells with a non-zero count
+// (count x - sum of open neighbours; Air neighbours hold 0), symmetric, so
+// every product above follows from four dot products taken in one pass
+// after the V-cycle:
+//   s = {<z, r>, <z, A z>, <z, A d'>, <d', r>},
+//   <d, A d> = <z,Az> + 2 beta <z,Ad'> + beta^2 <d',Ad'>,  <d, r> = <z,r> + beta <d',r>,
+// and A d is formed on the fly in the update pass as A z + beta A d'.  The
+// vectors d and z are FP32, p and r FP64 with FP64 arithmetic throughout: p
+// and r are updated with the same (exactly converted) d, so the recursive r
+// stays the true residual b - A p to FP64 rounding.
+
+// (A x)_k for one cell from its info byte and the five stencil values.
+__device__ __forceinline__ double apply_a(int f, double c, double e, double w, double n,
+                                          double s) {
+  const int cnt = f & 7;
+  if (!(f & 0x80) || cnt == 0) return 0.0;
+  double sum = 0.0;
+  sum += (f & 0x08) ? e : 0.0;
+  sum += (f & 0x10) ? w : 0.0;
+  sum += (f & 0x20) ? n : 0.0;
+  sum += (f & 0x40) ? s : 0.0;
+  return (double)cnt * c - sum;
+}
+
+// Pass 1: partials of s (first: d' is not read and counts as 0).
+__global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
+                                                  const float* __restrict__ dprev,
+                                                  const double* __restrict__ r,
+                                                  const uint8_t* __restrict__ info, int nx,
+                                                  int ny, int first, double* partials,
+                                                  const int* __restrict__ done) {
+  if (done && *done) return;
+  int bx, by;
+  xcd_block(&bx, &by);
+  const int i = bx * BX + threadIdx.x;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = by * BY + threadIdx.y; j < ny; j += gridDim.y * BY) {
+    if (i >= nx) break;
+    const int k = i + j * nx;
+    const int kE = i + 1 < nx ? k + 1 : k, kW = i > 0 ? k - 1 : k;
+    const int kN = j + 1 < ny ? k + nx : k, kS = j > 0 ? k - nx : k;
+    const int f = info[k];
+    const double zc = z[k], rc = r[k];
+    const double az = apply_a(f, zc, z[kE], z[kW], z[kN], z[kS]);
+    acc[0] += zc * rc;
+    acc[1] += zc * az;
+    if (!first) {
+      const double dc = dprev[k];
+      const double ad = apply_a(f, dc, dprev[kE], dprev[kW], dprev[kN], dprev[kS]);
+      acc[2] += zc * ad;
+      acc[3] += dc * rc;
+    }
+  }
+  block_sum_store<4>(acc, partials);
+}
+
+// The step coefficients from pass 1's partials: state = {alpha, beta,
+// <d, A d>} (the last carried to the next iteration).  A non-positive or
+// non-finite <d, A d> (a direction the operator does not see) gives alpha 0.
+__global__ __launch_bounds__(1024) void k_fcg_coef(const double* __restrict__ partials,
+                                                   int nblocks, double* __restrict__ state,
+                                                   int first, const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ double sh[4][16];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = tid; b < nblocks; b += 1024)
+    for (int q = 0; q < 4; q++) acc[q] += partials[4 * b + q];
+  for (int q = 0; q < 4; q++) {
+    double t = wave_sum(acc[q]);
+    if (lane == 0) sh[q][wid] = t;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double v[4];
+    for (int q = 0; q < 4; q++) {
+      double a = 0.0;
+      for (int w = 0; w < 16; w++) a += sh[q][w];
+      v[q] = a;
+    }
+    const double dad_prev = state[2];
+    const double beta = (first || !(dad_prev > 0.0)) ? 0.0 : -v[2] / dad_prev;
+    const double dad = v[1] + 2.0 * beta * v[2] + beta * beta * dad_prev;
+    const double dr = v[0] + beta * v[3];
+    const double alpha = (dad > 0.0 && isfinite(dad)) ? dr / dad : 0.0;
+    state[0] = alpha;
+    state[1] = beta;
+    state[2] = dad;
+  }
+}
+
+// Pass 2: d = z + beta d';  p += alpha d;  r -= alpha (A z + beta A d');
+// r32 = (float) r;  partials {sum r^2, n} over the fluid cells with a
+// non-zero count (the divergence terms of the norm are the initial pass's).
+__global__ __launch_bounds__(256) void k_fcg_update(
+    const float* __restrict__ z, const float* __restrict__ dprev, float* __restrict__ dnext,
+    double* __restrict__ p, double* __restrict__ r, const uint8_t* __restrict__ info,
+    float* __restrict__ r32, const double* __restrict__ state, int nx, int ny, int first,
+    double* partials, const int* __restrict__ done) {
+  if (done && *done) return;
+  int bx, by;
+  xcd_block(&bx, &by);
+  const int i = bx * BX + threadIdx.x;
+  const double alpha = state[0], beta = first ? 0.0 : state[1];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int j = by * BY + threadIdx.y; j < ny; j += gridDim.y * BY) {
+    if (i >= nx) break;
+    const int k = i + j * nx;
+    const int kE = i + 1 < nx ? k + 1 : k, kW = i > 0 ? k - 1 : k;
+    const int kN = j + 1 < ny ? k + nx : k, kS = j > 0 ? k - nx : k;
+    const int f = info[k];
+    double dc = z[k], dE = z[kE], dW = z[kW], dN = z[kN], dS = z[kS];
+    if (!first) {
+      dc += beta * (double)dprev[k];
+      dE += beta * (double)dprev[kE];
+      dW += beta * (double)dprev[kW];
+      dN += beta * (double)dprev[kN];
+      dS += beta * (double)dprev[kS];
+    }
+    // the stored direction is FP32: p, r and A d all use its rounded values
+    const float d32 = (float)dc;
+    dnext[k] = d32;
+    const double ad = apply_a(f, (double)d32, (double)(float)dE, (double)(float)dW,
+                              (double)(float)dN, (double)(float)dS);
+    p[k] += alpha * (double)d32;
+    const double rn = r[k] - alpha * ad;
+    r[k] = rn;
+    r32[k] = (float)rn;
+    if ((f & 0x80) && (f & 7)) {
+      acc[0] += rn * rn;
+      acc[1] += 1.0;
+    }
   }
   block_sum_store<4>(acc, partials);
 }
@@ -275,7 +425,8 @@ __global__ __launch_bounds__(1024) void k_finalize_sum4(const double* __restrict
                                                         int nblocks, double* __restrict__ out,
                                                         double* __restrict__ host_out,
                                                         int* __restrict__ done, double scale,
-                                                        double tol, int init) {
+                                                        double tol, int init,
+                                                        const double* __restrict__ fixed23) {
   if (!init && done && *done) return;
   __shared__ double sh[4][16];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -308,8 +459,12 @@ __global__ __launch_bounds__(1024) void k_finalize_sum4(const double* __restrict
       double a = 0.0;
       for (int w = 0; w < 16; w++) a += sh[s][w];
       v[s] = a;
-      out[s] = a;
     }
+    if (fixed23) {  // the divergence terms, from the pass before the first cycle
+      v[2] = fixed23[2];
+      v[3] = fixed23[3];
+    }
+    for (int s = 0; s < 4; s++) out[s] = v[s];
     double rel = 0.0;
     if (v[1] > 0 && v[2] > 0) rel = sqrt(v[0] / v[1]) / (fabs(scale) * sqrt(v[2] / v[3]));
     out[4] = rel;
@@ -327,7 +482,8 @@ __global__ __launch_bounds__(1024) void k_finalize_sum4(const double* __restrict
 }
 
 void launch_finalize_sum4(hipStream_t s, const double* partials, int nblocks, double* out) {
-  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, nblocks, out, nullptr, nullptr, 1.0, 0.0, 0);
+  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, nblocks, out, nullptr, nullptr, 1.0, 0.0, 0,
+                                     nullptr);
 }
 
 __global__ void k_set_int(int* p, int v) { *p = v; }
@@ -383,22 +539,48 @@ void launch_dc_update(hipStream_t s, const double* po, double* pn, const float* 
 
 void launch_dc_correct(hipStream_t s, const double* po, double* pn, const float* e,
                        const double* div, const uint8_t* info, float* r32, int nx, int ny,
-                       double scale, double* partials, const int* done) {
+                       double scale, double* partials, const int* done, double* r64) {
   dim3 g = reduce_grid(nx, ny);
-  k_dc_update<<<g, dim3(BX, BY), 0, s>>>(po, pn, e, div, info, r32, nx, ny, scale, partials,
-                                         done);
+  k_dc_update<<<g, dim3(BX, BY), 0, s>>>(po, pn, e, div, info, r32, r64, nx, ny, scale,
+                                         partials, done);
 }
 
 void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                     double* host_sums, int* done, double scale, double tol) {
   dim3 g = reduce_grid(nx, ny);
-  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, g.x * g.y, sums, host_sums, done, scale, tol, 0);
+  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, g.x * g.y, sums, host_sums, done, scale, tol, 0,
+                                     nullptr);
 }
 
 void launch_dc_norm_init(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                          double* host_sums, int* done, double scale) {
   dim3 g = reduce_grid(nx, ny);
-  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, g.x * g.y, sums, host_sums, done, scale, 0.0, 1);
+  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, g.x * g.y, sums, host_sums, done, scale, 0.0, 1,
+                                     nullptr);
+}
+
+void launch_fcg_dots(hipStream_t s, const float* z, const float* dprev, const double* r,
+                     const uint8_t* info, int nx, int ny, int first, double* partials,
+                     double* state, const int* done) {
+  dim3 g = reduce_grid(nx, ny);
+  k_fcg_dots<<<g, dim3(BX, BY), 0, s>>>(z, dprev, r, info, nx, ny, first, partials, done);
+  k_fcg_coef<<<1, 1024, 0, s>>>(partials, g.x * g.y, state, first, done);
+}
+
+void launch_fcg_update(hipStream_t s, const float* z, const float* dprev, float* dnext, double* p,
+                       double* r, const uint8_t* info, float* r32, int nx, int ny, int first,
+                       const double* state, double* partials, const int* done) {
+  dim3 g = reduce_grid(nx, ny);
+  k_fcg_update<<<g, dim3(BX, BY), 0, s>>>(z, dprev, dnext, p, r, info, r32, state, nx, ny, first,
+                                          partials, done);
+}
+
+void launch_fcg_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
+                     const double* sums0, double* host_sums, int* done, double scale,
+                     double tol) {
+  dim3 g = reduce_grid(nx, ny);
+  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, g.x * g.y, sums, host_sums, done, scale, tol, 0,
+                                     sums0);
 }
 
 void launch_norms(hipStream_t s, const float* p32, const double* p64, const double* div,

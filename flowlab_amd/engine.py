@@ -296,7 +296,8 @@ class HipPressureSolver:
     def __init__(self, nx, ny, method="mg-dc", h=None):
         self.engine = HipEngine(nx, ny, 1, h)
         self.method = method
-        self.name = {"rbsor": "hip-rbsor", "mg": "hip-mg", "mg-dc": "hip-mg-dc"}[method]
+        self.name = {"rbsor": "hip-rbsor", "mg": "hip-mg", "mg-dc": "hip-mg-dc",
+                     "mg-fcg": "hip-mg-fcg"}[method]
 
     def solve(self, g, p, div, label, scale, iterations, omega, tol) -> int:
         it = C.c_int()

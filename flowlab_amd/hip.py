@@ -15,11 +15,12 @@ LIB_PATH = os.path.join(_HERE, "libflowhip.so")
 
 FH_OK, FH_ERR_BAD_ARG, FH_ERR_BUSY, FH_ERR_DEVICE_LOST, FH_ERR_NO_DEVICE, FH_ERR_HIP = range(6)
 SEMI_LAGRANGIAN, MACCORMACK = 0, 1
-SOLVER_RBSOR, SOLVER_MG, SOLVER_MG_DC = 1, 2, 3
+SOLVER_RBSOR, SOLVER_MG, SOLVER_MG_DC, SOLVER_MG_FCG = 1, 2, 3, 4
 PHASES = ("advect", "forces", "divergence", "solve", "project", "dye", "total")
 
 SCHEMES = {"semiLagrangian": SEMI_LAGRANGIAN, "macCormack": MACCORMACK}
-SOLVERS = {"rbsor": SOLVER_RBSOR, "mg": SOLVER_MG, "mg-dc": SOLVER_MG_DC}
+SOLVERS = {"rbsor": SOLVER_RBSOR, "mg": SOLVER_MG, "mg-dc": SOLVER_MG_DC,
+           "mg-fcg": SOLVER_MG_FCG}
 
 # Every exported symbol of include/flowhip.h: (name, restype, argtypes).
 _dp = C.POINTER(C.c_double)
@@ -106,7 +107,8 @@ SIGNATURES = [
 
 KERNELS = {"rb_sweep_fine": 1, "advect_faces": 2, "maccormack_faces": 3, "advect_dye": 4,
            "maccormack_dye": 5, "dc_update": 6, "restrict_fine": 7,
-           "smooth_restrict_fine": 8, "prolong_smooth_fine": 9, "coarse_vcycle": 10}
+           "smooth_restrict_fine": 8, "prolong_smooth_fine": 9, "coarse_vcycle": 10,
+           "fcg_dots": 11, "fcg_update": 12}
 
 
 class HipError(RuntimeError):

@@ -15,7 +15,7 @@ const path = require('path');
 const addon = require(path.join(__dirname, 'flowhip.node'));
 
 const SCHEME = { semiLagrangian: 0, macCormack: 1 };
-const SOLVER = { rbsor: 1, mg: 2, 'mg-dc': 3 };
+const SOLVER = { rbsor: 1, mg: 2, 'mg-dc': 3, 'mg-fcg': 4 };
 
 function ensure(handle, cls) {
   if (!handle) throw new Error(`${cls}: no GPU (libflowhip could not create an engine)`);
@@ -26,7 +26,7 @@ class HipStepper {
   /**
    * @param {{nx:number, ny:number, h:number}} g  the velocity grid
    * @param {number} dyeScale whole-number dye refinement
-   * @param {object} opts {solver: 'mg-dc'|'mg'|'rbsor', iterations, omega, tol}
+   * @param {object} opts {solver: 'mg-dc'|'mg-fcg'|'mg'|'rbsor', iterations, omega, tol}
    */
   constructor(g, dyeScale = 1, opts = {}) {
     this.name = 'hip-fused';

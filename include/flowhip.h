@@ -57,8 +57,13 @@ enum { FH_SEMI_LAGRANGIAN = 0, FH_MACCORMACK = 1 };
  * (pressureMultigrid.ts:219-320).  FH_SOLVER_MG_DC: mixed-precision defect
  * correction -- FP64 pressure, residual and norms, FP32 V(2,2) on the
  * correction, coarsened to <= 16 cells a side with an FP64 direct solve;
- * `iterations` = cycle cap, `tol` = relative residual target. */
-enum { FH_SOLVER_RBSOR = 1, FH_SOLVER_MG = 2, FH_SOLVER_MG_DC = 3 };
+ * `iterations` = cycle cap, `tol` = relative residual target.
+ * FH_SOLVER_MG_FCG: a defect-correction cycle, then (if the tolerance is not
+ * met) flexible conjugate gradients with one stored direction, preconditioned
+ * by the same FP32 V-cycle -- the Krylov wrapper docs/WEBGPU.md:462-465 names
+ * for boundary-limited multigrid contraction; `iterations` = cycle cap (one
+ * V-cycle each). */
+enum { FH_SOLVER_RBSOR = 1, FH_SOLVER_MG = 2, FH_SOLVER_MG_DC = 3, FH_SOLVER_MG_FCG = 4 };
 
 /* Step phases reported by fh_get_timings (milliseconds of device time). */
 enum {
@@ -140,7 +145,8 @@ int fh_phase_project(fh_engine* e, double grad_scale);
 int fh_phase_dye(fh_engine* e, int scheme, double dt, const double keep[3]);
 
 /* Relative residual (rmsRemainingDivergence / fluidDivRms, pressure.ts:
- * 102-161) before and after every cycle of the last FH_SOLVER_MG_DC solve;
+ * 102-161) before and after every cycle of the last FH_SOLVER_MG_DC or
+ * FH_SOLVER_MG_FCG solve;
  * returns the number of entries written (<= max). */
 int fh_solve_history(fh_engine* e, double* out, int max);
 
@@ -163,7 +169,9 @@ enum {
   FH_K_RESTRICT_FINE = 7,     /* residual+restrict from the finest level */
   FH_K_SMOOTH_RESTRICT_FINE = 8, /* fused pre-smooth + restrict, finest level */
   FH_K_PROLONG_SMOOTH_FINE = 9,  /* fused prolong + post-smooth, finest level */
-  FH_K_COARSE_VCYCLE = 10        /* one-workgroup sub-cycle of the coarse levels */
+  FH_K_COARSE_VCYCLE = 10,       /* one-workgroup sub-cycle of the coarse levels */
+  FH_K_FCG_DOTS = 11,            /* flexible CG: dot products (+ coefficients) */
+  FH_K_FCG_UPDATE = 12           /* flexible CG: direction, p, r and norm update */
 };
 int fh_trace_kernel(fh_engine* e, int kernel);
 int fh_trace_stats(fh_engine* e, int* count, double* total_ms);

@@ -299,6 +299,61 @@ def mg_dc_cycle(nx, ny, p, b, label):
     return p
 
 
+def mg_apply_a(nx, ny, x, label):
+    """A x of the defect-correction system (count x - sum of open neighbours
+    on fluid cells with a non-zero count, 0 elsewhere): the negated residual
+    of x against a zero right-hand side (oracle.c fo_mg_residual)."""
+    return -mg_residual(nx, ny, x, np.zeros(nx * ny), label)
+
+
+def mg_fcg_solve(nx, ny, p, div, label, scale, cycles, tol, precond=None, dc_first=True):
+    """Flexible conjugate gradients with one stored direction (Y. Notay,
+    "Flexible conjugate gradients", SIAM J. Sci. Comput. 22 (2000)), FP64,
+    on the defect-correction system A p = b, b = -scale div, preconditioned
+    by one defect-correction V-cycle from zero (precond(r) overrides it);
+    the Krylov wrapper docs/WEBGPU.md:462-465 names for multigrid whose
+    contraction is limited at Air/Solid boundaries.  Textbook form, every
+    product computed explicitly:
+        z = B r;  beta = -<z, A d'> / <d', A d'>;  d = z + beta d';
+        alpha = <d, r> / <d, A d>;  p += alpha d;  r -= alpha A d.
+    dc_first: the first cycle is a plain defect-correction cycle (p += B r,
+    r recomputed from p) and CG starts from its result; False: CG from the
+    start.  The zero-divergence guard of pressureMultigrid.ts:294-300
+    applies.  Returns (p, iterations, relative-residual history as in
+    mg_solve)."""
+    p = _f64(p).copy()
+    div, label = _f64(div), _lbl(label)
+    if fluid_div_rms(div, label) == 0:
+        return np.zeros(nx * ny), 0, np.zeros(0)
+    B = precond or (lambda r: mg_dc_cycle(nx, ny, np.zeros(nx * ny), r, label))
+    b = -scale * div
+    r = mg_residual(nx, ny, p, b, label)
+    hist = [relative_residual(nx, ny, p, div, label, scale)]
+    start = 0
+    if dc_first and cycles > 0:
+        p = p + B(r)
+        r = mg_residual(nx, ny, p, b, label)
+        hist.append(relative_residual(nx, ny, p, div, label, scale))
+        if tol > 0 and hist[-1] <= tol:
+            return p, 1, np.array(hist)
+        start = 1
+    d = q = None
+    for it in range(start, cycles):
+        z = B(r)
+        if d is None:
+            d = z
+        else:
+            d = z - (z @ q) / (d @ q) * d
+        q = mg_apply_a(nx, ny, d, label)
+        alpha = (d @ r) / (d @ q)
+        p = p + alpha * d
+        r = r - alpha * q
+        hist.append(relative_residual(nx, ny, p, div, label, scale))
+        if tol > 0 and hist[-1] <= tol:
+            return p, it + 1, np.array(hist)
+    return p, cycles, np.array(hist)
+
+
 def coarse_direct_solve(nx, ny, label, b):
     label, b = _lbl(label), _f64(b)
     e = np.empty(nx * ny)

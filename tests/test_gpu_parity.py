@@ -346,14 +346,15 @@ def test_full_size_karman_projection_property():
     assert np.sqrt(np.mean(div_after ** 2)) < 1e-4 * max(np.sqrt(np.mean(div_before ** 2)), 1)
 
 
+@pytest.mark.parametrize("solver", ["mg-dc", "mg-fcg"])
 @pytest.mark.parametrize("iterations,tol", [(60, 1e-6), (1, 0.0), (5, 0.0), (0, 1e-6)])
-def test_mg_dc_zero_divergence_guard(iterations, tol):
+def test_mg_dc_zero_divergence_guard(iterations, tol, solver):
     """pressureMultigrid.ts:294-300: zero divergence gives p = 0 and 0 cycles,
     whatever the warm start, and the next solve on the engine runs normally."""
     from flowlab_amd.engine import HipPressureSolver
     nx, ny = 96, 64
     lab, div = boundary_fixture(nx, ny)
-    sol = HipPressureSolver(nx, ny, "mg-dc")
+    sol = HipPressureSolver(nx, ny, solver)
     p = np.linspace(-1.0, 1.0, nx * ny)  # a leftover warm start
     it = sol.solve(None, p, np.zeros(nx * ny), lab, 0.02, iterations, 1.0, tol)
     assert it == 0 and not p.any()
@@ -363,14 +364,15 @@ def test_mg_dc_zero_divergence_guard(iterations, tol):
     assert O.relative_residual(nx, ny, p, div, lab, 0.02) <= 1e-6
 
 
-def test_mg_dc_stops_at_the_first_cycle_within_tolerance():
+@pytest.mark.parametrize("solver", ["mg-dc", "mg-fcg"])
+def test_mg_dc_stops_at_the_first_cycle_within_tolerance(solver):
     """Cycle count is minimal however the host schedules the cycles: every
     history entry but the last is above the tolerance, repeated warm-started
     solves included."""
     from flowlab_amd.engine import HipPressureSolver
     nx, ny = 256, 192
     lab, div = boundary_fixture(nx, ny)
-    sol = HipPressureSolver(nx, ny, "mg-dc")
+    sol = HipPressureSolver(nx, ny, solver)
     p = np.zeros(nx * ny)
     for k in range(4):
         d = div * (1.0 + 0.05 * k)
@@ -399,3 +401,68 @@ def test_step_max_face_speed_tracks_the_velocity():
     u2, v2 = 3.0 * sim.f.u, 3.0 * sim.f.v
     e.upload(u2, v2, None, None, None)
     assert e.max_face_speed() == max(np.abs(f32(u2)).max(), np.abs(f32(v2)).max())
+
+
+@pytest.mark.parametrize("closed", [False, True])
+def test_mg_fcg_reaches_1e6(closed):
+    """Flexible CG around the V-cycle: the oracle's FP64 residual of the
+    device's p is within the tolerance, the cycle count is the oracle
+    restatement's within one, and the residual the device reports (updated
+    recursively) is the true one."""
+    from flowlab_amd.engine import HipPressureSolver
+    nx, ny = 256, 192
+    if closed:
+        lab = np.zeros(nx * ny, np.uint8)
+        lab.reshape(ny, nx)[60:100, 80:120] = O.SOLID
+        div = O.lcg_fill(nx * ny, 777)
+        div[lab == O.SOLID] = 0
+        div[lab == O.FLUID] -= div[lab == O.FLUID].mean()  # compatible RHS
+    else:
+        lab, div = boundary_fixture(nx, ny)
+    sol = HipPressureSolver(nx, ny, "mg-fcg")
+    p = np.zeros(nx * ny)
+    it = sol.solve(None, p, div, lab, 0.02, 60, 1.0, 1e-6)
+    rel = O.relative_residual(nx, ny, p, div, lab, 0.02)
+    assert rel <= 1e-6 and it < 60
+    _, it_cpu, _ = O.mg_fcg_solve(nx, ny, np.zeros(nx * ny), div, lab, 0.02, 60, 1e-6)
+    assert abs(it - it_cpu) <= 1
+    hist = sol.engine.solve_history()
+    assert len(hist) == it + 1 and hist[-1] <= 1e-6
+    assert hist[-1] == pytest.approx(rel, rel=1e-3)
+    dc = HipPressureSolver(nx, ny, "mg-dc")
+    it_dc = dc.solve(None, np.zeros(nx * ny), div, lab, 0.02, 60, 1.0, 1e-6)
+    assert it < it_dc
+
+
+def test_mg_fcg_steps_agree_with_mg_dc():
+    """Whole Karman steps with either solver: both pressures meet 1e-6, so
+    the velocities agree to the level that tolerance allows."""
+    from flowlab_amd.simulation import Simulation
+    nx, ny = 256, 128
+    out = {}
+    for solver in ("mg-dc", "mg-fcg"):
+        g, scene = S.karman_street(nx, ny)
+        sim = Simulation(nx, ny, dict(scheme="macCormack", cfl_target=0.5, solver=solver,
+                                      pressure_iters=60, tol=1e-6, residual_every=0))
+        sim.reset(scene)
+        for _ in range(3):
+            sim.step()
+            assert sim.stepper.engine.solve_history()[-1] <= 1e-6
+        out[solver] = (sim.f.u.copy(), sim.f.v.copy())
+    assert rel_l2(out["mg-fcg"][0], out["mg-dc"][0]) < 1e-4
+    assert rel_l2(out["mg-fcg"][1], out["mg-dc"][1]) < 1e-4
+
+
+def test_mg_fcg_first_step_is_the_mg_dc_cycle():
+    """The first flexible-CG step takes alpha = 1: the same bits as one
+    defect-correction cycle from the same warm start."""
+    from flowlab_amd.engine import HipPressureSolver
+    nx, ny = 160, 96
+    lab, div = boundary_fixture(nx, ny)
+    p0 = O.lcg_fill(nx * ny, 5) * (lab == O.FLUID)
+    out = []
+    for solver in ("mg-dc", "mg-fcg"):
+        p = p0.copy()
+        assert HipPressureSolver(nx, ny, solver).solve(None, p, div, lab, 0.02, 1, 1.0, 0) == 1
+        out.append(p)
+    assert np.array_equal(out[0], out[1])
