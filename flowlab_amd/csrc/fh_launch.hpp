@@ -54,6 +54,8 @@ void launch_dye_patch(hipStream_t s, float* dye, const float* patch, int dnx, in
                       int j0, int pnx, int pny);
 void launch_commit_labels(hipStream_t s, double* p, float* u, float* v, float* dye,
                           const uint8_t* lbl, int nx, int ny, int dye_plane);
+// max |u|, |v| into *out (simulation.ts:451-459); u and v are read as float4
+// and must be 16-byte aligned (the engine passes hipMalloc bases).
 void launch_max_speed(hipStream_t s, const float* u, const float* v, int nx, int ny,
                       float* partials, unsigned* counter, float* out);
 
