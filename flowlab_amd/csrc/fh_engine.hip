@@ -1397,6 +1397,8 @@ int fh_get_tiles(fh_engine* e, int* shapes, int* nlevels) {
 // shape and keeps the fastest.  Only multigrid scratch (level unknowns and
 // coarse right-hand sides) is touched; labels must be set (a solve or step
 // has run).  ms_out (optional): nlevels x fh_num_tile_shapes() ms per pair.
+constexpr int kTuneRounds = 3;
+
 int fh_tune_tiles(fh_engine* e, int reps, float* ms_out) {
   GUARD("tuneTiles()");
   if (reps < 1) reps = 1;
@@ -1410,50 +1412,55 @@ int fh_tune_tiles(fh_engine* e, int reps, float* ms_out) {
   for (int l = 0; l < e->lc_dc && l < kMaxTiledLevels; l++) {
     Level& L = e->lv[l];
     Level& N = e->lv[l + 1];
-    float best = 1e30f;
+    // kTuneRounds interleaved rounds over the shapes, each shape's best
+    // round kept: one round's clock or queue hiccup cannot pick a shape.
+    float msr[kNumTileShapes];
+    for (int t = 0; t < kNumTileShapes; t++) msr[t] = 1e30f;
+    for (int round = 0; round < kTuneRounds; round++)
+      for (int t = 0; t < kNumTileShapes; t++) {
+        for (int r = -2; r < reps; r++) {
+          if (r == 0) HIPCHK(hipEventRecord(a, e->s));
+          launch_smooth_restrict(e->s, t, nullptr, L.xa, L.b, L.info, L.nx, L.ny, N.b, N.nx, N.ny,
+                                 1, nullptr);
+          launch_prolong_smooth(e->s, t, L.xa, L.x, L.b, L.info, L.nx, L.ny, N.x, N.nx, N.ny,
+                                nullptr);
+        }
+        HIPCHK(hipEventRecord(b, e->s));
+        HIPCHK(hipEventSynchronize(b));
+        LAUNCHED("tune_tiles");
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, a, b));
+        msr[t] = std::min(msr[t], ms / reps);
+      }
     int best_t = 0;
     for (int t = 0; t < kNumTileShapes; t++) {
-      for (int r = -2; r < reps; r++) {
-        if (r == 0) HIPCHK(hipEventRecord(a, e->s));
-        launch_smooth_restrict(e->s, t, nullptr, L.xa, L.b, L.info, L.nx, L.ny, N.b, N.nx, N.ny, 1,
-                               nullptr);
-        launch_prolong_smooth(e->s, t, L.xa, L.x, L.b, L.info, L.nx, L.ny, N.x, N.nx, N.ny,
-                              nullptr);
-      }
-      HIPCHK(hipEventRecord(b, e->s));
-      HIPCHK(hipEventSynchronize(b));
-      LAUNCHED("tune_tiles");
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, a, b));
-      ms /= reps;
-      if (ms_out) ms_out[l * kNumTileShapes + t] = ms;
-      if (ms < best) {
-        best = ms;
-        best_t = t;
-      }
+      if (ms_out) ms_out[l * kNumTileShapes + t] = msr[t];
+      if (msr[t] < msr[best_t]) best_t = t;
     }
     e->tile[l] = best_t;
   }
   // the one-workgroup coarse sub-cycle: its workgroup size
   {
     const int lc = e->lc_dc, nlev = e->nlev_dc;
-    float best = 1e30f;
-    for (int g = 0; g < kNumCoarseGroups; g++) {
-      for (int r = -2; r < reps; r++) {
-        if (r == 0) HIPCHK(hipEventRecord(a, e->s));
-        launch_coarse_vcycle(e->s, g, coarse_levels(e, lc, nlev), e->lv[lc].b, nullptr,
-                             e->lv[lc].x, e->M, e->cells, e->ncoarse, 1, nullptr);
+    float msg[kNumCoarseGroups];
+    for (int g = 0; g < kNumCoarseGroups; g++) msg[g] = 1e30f;
+    for (int round = 0; round < kTuneRounds; round++)
+      for (int g = 0; g < kNumCoarseGroups; g++) {
+        for (int r = -2; r < reps; r++) {
+          if (r == 0) HIPCHK(hipEventRecord(a, e->s));
+          launch_coarse_vcycle(e->s, g, coarse_levels(e, lc, nlev), e->lv[lc].b, nullptr,
+                               e->lv[lc].x, e->M, e->cells, e->ncoarse, 1, nullptr);
+        }
+        HIPCHK(hipEventRecord(b, e->s));
+        HIPCHK(hipEventSynchronize(b));
+        LAUNCHED("tune_coarse");
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, a, b));
+        msg[g] = std::min(msg[g], ms / reps);
       }
-      HIPCHK(hipEventRecord(b, e->s));
-      HIPCHK(hipEventSynchronize(b));
-      LAUNCHED("tune_coarse");
-      float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, a, b));
-      if (ms / reps < best) {
-        best = ms / reps;
-        e->coarse_group = g;
-      }
-    }
+    e->coarse_group = 0;
+    for (int g = 1; g < kNumCoarseGroups; g++)
+      if (msg[g] < msg[e->coarse_group]) e->coarse_group = g;
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);

@@ -189,8 +189,9 @@ const char* fh_tile_shape_name(int shape);
 int fh_set_tile(fh_engine* e, int level, int shape);
 int fh_get_tiles(fh_engine* e, int* shapes, int* nlevels);
 /* Autotuner: times every shape on each tiled level of this engine's grid
- * (reps launch pairs each) and keeps the fastest, then the coarse
- * sub-cycle's workgroup size likewise; needs labels set.
+ * (reps launch pairs each, best of three interleaved rounds) and keeps the
+ * fastest, then the coarse sub-cycle's workgroup size likewise; needs labels
+ * set.
  * ms_out (may be NULL): nlevels x fh_num_tile_shapes() ms. */
 int fh_tune_tiles(fh_engine* e, int reps, float* ms_out);
 /* The coarse sub-cycle's workgroup size (index 0: 1024, 1: 512, 2: 256
