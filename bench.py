@@ -342,8 +342,12 @@ def main():
     # per-launch device time: event bracket around each launch, less the
     # bracket's own cost (an empty event pair on the same stream)
     k_avg_ms = r["k_ms"] / max(r["n_launch"], 1) - r["bracket_ms"]
-    bytes_per_launch = ALGO_BYTES_PER_CELL[args.kernel] * cells
-    achieved = bytes_per_launch / (k_avg_ms * 1e-3) / 1e9 if r["n_launch"] else None
+    # kernels without a bandwidth figure (the latency-bound coarse sub-cycle)
+    # report their launch time only
+    bpc = ALGO_BYTES_PER_CELL.get(args.kernel)
+    bytes_per_launch = bpc * cells if bpc else None
+    achieved = (bytes_per_launch / (k_avg_ms * 1e-3) / 1e9
+                if r["n_launch"] and bytes_per_launch else None)
     out = {
         "metric": METRIC,
         "value": value,
