@@ -155,6 +155,7 @@ class HipEngine:
 
     def set_timing(self, on: bool):
         check(self.lib.fh_set_timing(self._e, 1 if on else 0))
+        self._timing = bool(on)
 
     def timings(self) -> dict:
         ms = np.zeros(len(hip.PHASES))
