@@ -1,10 +1,11 @@
 """Many independent scenes advanced together on one GPU (libflowhip's
 fh_batch_* entry points, include/flowhip.h).
 
-Each scene steps as Simulation.step does with solver 'mg-dc' (dt from its own
-CFL, simulation.ts:467-587), but every phase of every scene goes out in one
-persistent launch, the step can be replayed as a captured hipGraph, and the
-multigrid tile shape of each level is picked by an autotuner.
+Each scene is a single-grid engine on its own stream and steps as
+Simulation.step does with solver 'mg-fcg' (dt from its own CFL,
+simulation.ts:467-587); a batch step hands the scenes, largest first, to a
+pool of host threads, so the scenes' kernels interleave on the device.  The
+multigrid tile shapes of every scene can be set or autotuned.
 """
 from __future__ import annotations
 
@@ -67,7 +68,7 @@ class HipBatch:
                                                       hip.dptr(d)))
 
     def step(self, scheme="macCormack", cfl=0.5, dt_max=1.0, rho=1.0, tol=1e-6, max_cycles=60,
-             graph_cycles=8) -> None:
+             graph_cycles=0) -> None:
         hip.check_batch(hip.lib().fh_batch_step(self._b, hip.SCHEMES[scheme], cfl, dt_max, rho,
                                                 tol, max_cycles, graph_cycles))
 

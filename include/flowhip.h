@@ -246,13 +246,14 @@ int fh_set_warm_start(fh_engine* e, fh_unet* u);
 
 /* ---- Batched scenes ------------------------------------------------------
  * No counterpart in the reference, which steps one grid per Simulation
- * (simulation.ts:467-587); its nearest relative is the one-submit recording
- * of a fused step (stepGpu.ts:441-542).  A batch holds up to 255 independent
- * grids of any sizes (dyeScale 1, fixed at creation) and advances all of
- * them with one launch per phase: each scene takes dt from its own CFL
- * (simulation.ts:477-479) and solves its pressure by FP64 defect-correction
- * multigrid (FH_SOLVER_MG_DC) to its own convergence.  Per-scene fields are
- * uploaded and read in the reference's host layout, FP64, as fh_upload. */
+ * (simulation.ts:467-587).  A batch holds up to 255 independent grids of any
+ * sizes (dyeScale 1, fixed at creation), each a single-grid engine with its
+ * own stream.  A step hands the scenes, largest first, to a pool of host
+ * threads, so their kernels interleave on the device: each scene takes dt
+ * from its own CFL (simulation.ts:474-479) and solves its pressure with
+ * FH_SOLVER_MG_FCG to its own convergence, exactly as fh_step on that scene
+ * alone.  Per-scene fields are uploaded and read in the reference's host
+ * layout, FP64, as fh_upload. */
 typedef struct fh_batch fh_batch;
 const char* fh_batch_last_error(void);
 /* NULL on failure (fh_batch_last_error says why). */
@@ -266,10 +267,9 @@ int fh_batch_upload(fh_batch* b, int scene, const double* u, const double* v, co
 int fh_batch_set_forces(fh_batch* b, int scene, double buoyancy, double confinement,
                         const double* decay);
 /* One step of every scene: dt = min(dt_max, cfl h / max face speed),
- * scale = rho h^2/dt, gradScale = dt/(rho h).  Pressure: V-cycles until the
- * relative residual <= tol (tol <= 0: exactly max_cycles).  graph_cycles > 0
- * replays the step with that many cycles as one captured hipGraph (further
- * cycles, if needed, one graph each); 0 launches the kernels directly. */
+ * scale = rho h^2/dt, gradScale = dt/(rho h), dye keep exp(-decay dt).
+ * Pressure: V-cycles until the relative residual <= tol (tol <= 0: exactly
+ * max_cycles).  graph_cycles is accepted and ignored. */
 int fh_batch_step(fh_batch* b, int scheme, double cfl, double dt_max, double rho, double tol,
                   int max_cycles, int graph_cycles);
 /* Per scene of the last step (arrays of nscenes, any may be NULL). */
@@ -278,13 +278,14 @@ int fh_batch_read(fh_batch* b, int scene, double* u, double* v, double* p, doubl
 int fh_batch_last_step_ms(fh_batch* b, float* ms);
 long long fh_batch_cells(fh_batch* b);
 /* Multigrid tile shapes: fh_batch_num_tiles() candidates by name; level -1
- * sets every tiled level.  Results do not depend on the shape. */
+ * sets every tiled level of every scene; get_tiles reports the largest
+ * scene's.  Results do not depend on the shape. */
 int fh_batch_num_tiles(void);
 const char* fh_batch_tile_name(int shape);
 int fh_batch_set_tile(fh_batch* b, int level, int shape);
 int fh_batch_get_tiles(fh_batch* b, int* shapes, int* nlevels);
-/* Autotuner: per tiled level, times every candidate over the batch (reps
- * passes each) and keeps the fastest; ms_out gets nlevels x num_tiles ms. */
+/* Autotuner: every scene tunes its own levels (fh_tune_tiles); ms_out gets
+ * the largest scene's nlevels x num_tiles ms. */
 int fh_batch_tune(fh_batch* b, int reps, float* ms_out);
 
 #ifdef __cplusplus

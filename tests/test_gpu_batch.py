@@ -1,10 +1,10 @@
-"""Batched scenes on the GPU (fh_batch_*): one launch per phase for scenes of
-uneven size, device-side dt and convergence, a captured hipGraph step and the
-tile-shape autotuner.
+"""Batched scenes on the GPU (fh_batch_*): scenes of uneven size, each a
+single-grid engine on its own stream, stepped together from a host thread
+pool with dt from each scene's CFL.
 
-The batch runs the single-grid engine's per-cell code, so each scene must
-equal the same scene stepped alone bit for bit, and both are held to the
-FP64 oracle at the per-kernel bar (relative L2 <= 1e-5 in FP32) with every
+Each scene runs the single-grid engine's step, so it must equal the same scene
+stepped alone bit for bit however the threads interleave; both are held to
+the FP64 oracle at the per-kernel bar (relative L2 <= 1e-5 in FP32) with every
 pressure solve at 1e-6 by the oracle's own residual.
 """
 import numpy as np
@@ -17,13 +17,6 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-6
 
-# The batched kernels took a GPU memory aperture fault on their first device
-# run (k_b_maccormack_faces reported, step 0 of the first test below) and the
-# cause is not yet shown; fh_batch_step and fh_batch_tune refuse to launch
-# them until it is.  These tests stay in place for that run.
-OFF = pytest.mark.skip(reason="batched kernels switched off: an unexplained GPU memory fault "
-                              "on their first run")
-
 
 def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
@@ -32,7 +25,8 @@ def rel_l2(a, b):
 def small_scenes():
     """Uneven sizes, closed boxes with obstacles and one open outlet."""
     return [S.decaying_turbulence(64, 64, 11), S.decaying_turbulence(128, 64, 12),
-            S.karman_street(96, 48), S.decaying_turbulence(64, 128, 13)]
+            S.karman_street(96, 48), S.decaying_turbulence(64, 128, 13),
+            S.decaying_turbulence(160, 96, 14), S.karman_street(128, 64)]
 
 
 def make_batch(scenes):
@@ -42,7 +36,6 @@ def make_batch(scenes):
     return b, fields
 
 
-@OFF
 def test_batch_equals_single_engine_bitwise():
     from flowlab_amd.simulation import Simulation
     scenes = small_scenes()
@@ -50,13 +43,13 @@ def test_batch_equals_single_engine_bitwise():
     sims = []
     for g, sc in scenes:
         sim = Simulation(g.nx, g.ny, dict(scheme="macCormack", cfl_target=0.5, dt_max=1.0,
-                                         solver="mg-dc", pressure_iters=60, tol=TOL,
+                                         solver="mg-fcg", pressure_iters=60, tol=TOL,
                                          residual_every=0))
         sim.reset(sc)
         sims.append(sim)
     for n in range(3):
         print(f"step {n}", flush=True)
-        b.step(cfl=0.5, dt_max=1.0, tol=TOL, max_cycles=60, graph_cycles=0)
+        b.step(cfl=0.5, dt_max=1.0, tol=TOL, max_cycles=60)
         dt, iters, res = b.results()
         for s, sim in enumerate(sims):
             sim.step()
@@ -68,9 +61,8 @@ def test_batch_equals_single_engine_bitwise():
             assert res[s] <= TOL
 
 
-@OFF
 def test_batch_matches_oracle():
-    scenes = small_scenes()
+    scenes = small_scenes()[:4]
     b, _ = make_batch(scenes)
     prm = O.step_params(scheme="macCormack", solver="mg-dc", pressure_iters=60, tol=TOL,
                         cfl_target=0.5, dt_max=1.0, residual_every=0)
@@ -80,38 +72,37 @@ def test_batch_matches_oracle():
         o.reset(g, sc)
         osims.append(o)
     for n in range(2):
-        b.step(cfl=0.5, dt_max=1.0, tol=TOL, max_cycles=60, graph_cycles=4)
+        b.step(cfl=0.5, dt_max=1.0, tol=TOL, max_cycles=60)
         dt, iters, res = b.results()
         for s, (g, _) in enumerate(scenes):
             osims[s].step(prm)
             u, v, p, _ = b.read(s)
-            assert abs(dt[s] - prm.dt_used) <= 1e-6 * prm.dt_used
-            assert abs(iters[s] - prm.iters) <= 1, (s, iters[s], prm.iters)
+            # dt = cfl h / max|u|: the device's FP32 velocity after a solve to
+            # 1e-6 against the oracle's FP64 one, so the velocity's bar
+            assert abs(dt[s] - prm.dt_used) <= 1e-5 * prm.dt_used
+            # flexible CG against the oracle's defect correction: no more cycles
+            assert iters[s] <= prm.iters + 1, (s, iters[s], prm.iters)
             assert rel_l2(u, osims[s].u) < 1e-5, s
             assert rel_l2(v, osims[s].v) < 1e-4, s
             assert res[s] <= TOL
 
 
-@OFF
-def test_graph_replays_equal_eager_steps():
-    """hipGraph replays (with eager host work between them) give exactly the
-    eager launches' fields, step after step; 3 cycles in the graph leave most
-    scenes to the one-cycle graph, so both captured pieces are exercised."""
+def test_repeated_batches_give_the_same_bits():
+    """Thread interleaving changes nothing: two batches of the same scenes
+    agree field for field, step after step."""
     scenes = small_scenes()
     a, _ = make_batch(scenes)
-    g, _ = make_batch(scenes)
-    for n in range(4):
-        print(f"step {n}", flush=True)
-        a.step(tol=TOL, graph_cycles=0)
-        g.step(tol=TOL, graph_cycles=3)
-        ra, rg = a.results(), g.results()
-        assert np.array_equal(ra[1], rg[1]) and np.array_equal(ra[0], rg[0])
+    c, _ = make_batch(scenes)
+    for n in range(3):
+        a.step(tol=TOL)
+        c.step(tol=TOL)
+        ra, rc = a.results(), c.results()
+        assert np.array_equal(ra[1], rc[1]) and np.array_equal(ra[0], rc[0])
         for s in range(len(scenes)):
-            for x, y in zip(a.read(s)[:3], g.read(s)[:3]):
+            for x, y in zip(a.read(s)[:3], c.read(s)[:3]):
                 assert np.array_equal(x, y), (n, s)
 
 
-@OFF
 def test_tile_shapes_and_autotuner_do_not_change_results():
     from flowlab_amd.batch import tile_names
     scenes = small_scenes()[:2]
@@ -121,7 +112,7 @@ def test_tile_shapes_and_autotuner_do_not_change_results():
         b.set_tile(shape)
         assert set(b.tiles()) == {shape}
         for _ in range(2):
-            b.step(tol=TOL, graph_cycles=0)
+            b.step(tol=TOL)
         out = [b.read(s)[:3] for s in range(len(scenes))]
         if ref is None:
             ref = out
@@ -134,7 +125,7 @@ def test_tile_shapes_and_autotuner_do_not_change_results():
     assert len(rep["tiles"]) >= 1 and all(t in tile_names() for t in rep["tiles"])
     assert all(v > 0 for lvl in rep["ms"] for v in lvl.values())
     for _ in range(2):
-        b.step(tol=TOL, graph_cycles=0)
+        b.step(tol=TOL)
     for x, y in zip(ref, [b.read(s)[:3] for s in range(len(scenes))]):
         for a_, b_ in zip(x, y):
             assert np.array_equal(a_, b_)
