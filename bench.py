@@ -222,7 +222,7 @@ def run_batched(args):
         "vcycles_per_solve_mean": sum(iters) / len(iters),
         "max_relative_residual": res_max,
         "largest_scene_tiles": tune["tiles"],
-        "execution": "one engine and stream per scene, 8 host threads taking scenes largest "
+        "execution": "one engine and stream per scene, 16 host threads taking scenes largest "
                      "first",
     }
 
