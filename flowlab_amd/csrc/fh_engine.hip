@@ -398,7 +398,7 @@ int refresh_levels(fh_engine* e) {
   std::vector<int> cells;
   build_pseudo_inverse(C.nx, C.ny, lab, M, cells);
   int n = (int)cells.size();
-  if (n > 1024) {
+  if (n > kMaxDirect) {
     set_err("coarsest level too large for the direct solve");
     return FH_ERR_BAD_ARG;
   }
@@ -440,10 +440,11 @@ CoarseLevels coarse_levels(const fh_engine* e, int lc, int nlev) {
   return C;
 }
 
-// First level from which the rest of the hierarchy fits one workgroup's LDS.
+// First level from which the rest of the hierarchy fits one workgroup's LDS
+// (and its sweeps, at most kCoarseSweepCells cells of a colour per level).
 int coarse_start(const fh_engine* e, int nlev) {
   for (int l = 0; l < nlev; l++)
-    if (nlev - l <= kMaxCoarse &&
+    if (nlev - l <= kMaxCoarse && (long)((e->lv[l].nx + 1) / 2) * e->lv[l].ny <= kCoarseSweepCells &&
         coarse_vcycle_lds_bytes(coarse_levels(e, l, nlev)) <= kCoarseLds)
       return l;
   return nlev - 1;

@@ -127,6 +127,11 @@ void launch_set_int(hipStream_t s, int* p, int v);
 
 // ---- fused multigrid (fh_mgfused.hip) ----------------------------------------
 constexpr int kMaxCoarse = 12;
+// Most unknowns of the coarsest level's direct (pseudo-inverse) solve.
+constexpr int kMaxDirect = 1024;
+// Most cells of one colour on a level of the one-workgroup coarse sub-cycle
+// (a workgroup of NC threads holds kCoarseSweepCells / NC per thread).
+constexpr int kCoarseSweepCells = 4096;
 struct CoarseLevels {
   int n;
   int nx[kMaxCoarse];

@@ -75,7 +75,7 @@ size_t coarse_vcycle_lds_bytes(const CoarseLevels& L) {
     size_t n = (size_t)(L.nx[l] + 2) * (L.ny[l] + 2);
     b += 8 * n + ((n + 3) & ~(size_t)3);
   }
-  return b;
+  return b + 8 * (size_t)kMaxDirect;  // the direct solve's dense right-hand side and indices
 }
 
 void launch_cell_info(hipStream_t s, const uint8_t* lbl, int nx, int ny, uint8_t* info) {

@@ -39,51 +39,71 @@ __device__ __forceinline__ uint8_t cell_info(const uint8_t* __restrict__ lbl, in
   return (uint8_t)(0x80 | (e << 3) | (w << 4) | (n << 5) | (s << 6) | (e + w + n + s));
 }
 
-// (sum + b) / count, with count in 1..4 (1/3 rounded: within an ulp of the
-// division).
-__device__ __forceinline__ float gs_update(float sum, float b, int cnt) {
-  float r = cnt == 1 ? 1.f : cnt == 2 ? 0.5f : cnt == 3 ? (1.f / 3.f) : 0.25f;
-  return (sum + b) * r;
+// 1 / count for a sweep, count in 1..4: 1, 1/2 and 1/4 are exact powers of
+// two (exponent 127 - count/2), 1/3 is the rounded constant (within an ulp of
+// the division); (sum + b) * recip is the update.  One select, no branch.
+__device__ __forceinline__ float gs_recip(int cnt) {
+  const float p2 = __int_as_float((127 - (cnt >> 1)) << 23);
+  return cnt == 3 ? (1.f / 3.f) : p2;
 }
 
-// One colour half-sweep with omega = 1 on an LDS region of LX x LY cells,
-// updating cells of the colour whose LDS distance to the region border is
-// >= ring.  par0 = parity of LDS (0,0) in global coordinates.  zero: the
-// unknown is zero on entry (neighbour sum = 0).  Cells of one colour only
-// read cells of the other, so each thread reads all of its cells' operands
-// (fully unrolled, LX/LY are template constants) before writing any: the LDS
-// reads are in flight together instead of one cell after another.
+// Red-black Gauss-Seidel (omega = 1) on an LDS window of LX x LY cells.
+// Labels and right-hand side do not change while a tile is smoothed, so each
+// thread finds its cells once -- for each colour, K cells of the window's
+// ring-1 interior, as LDS index, b and 1/count (0: not updated -- off the
+// fluid, count 0, or a slot past the interior) -- and every half-sweep is
+// then two neighbour reads, five flops and a write per cell.
+//
+// Every half-sweep updates the whole ring-1 interior.  A temporally blocked
+// tile is exact only at LDS distance >= s from the border after s
+// half-sweeps; cells nearer the border take values that differ from the
+// global sweep, and the cells that read them are themselves nearer than s+1.
+// The exact region is the same as when the updated region shrinks by one
+// ring per half-sweep, so the tile interior gets the same bits.
 template <int LX, int LY>
-__device__ __forceinline__ void lds_half_sweep(float* sx, const float* sb, const uint8_t* si,
-                                               int ring, int color, int par0, bool zero,
-                                               int tid) {
-  constexpr int P = (LX + 1) / 2;  // pairs per row
-  constexpr int K = (P * (LY - 2) + NT - 1) / NT;
-  const int r0 = ring, r1 = LY - 1 - ring;
-  const int total = P * (r1 - r0 + 1);
-  int l[K];
-  bool act[K];
-  float nb[K], bb[K];
-  uint8_t ib[K];
+struct TileSweep {
+  static constexpr int P = (LX + 1) / 2;  // pairs per row
+  static constexpr int ITEMS = P * (LY - 2);
+  static constexpr int K = (ITEMS + NT - 1) / NT;
+  int l[2][K];
+  float b[2][K], r[2][K];
+
+  // par0 = parity of LDS (0,0) in global coordinates; colour c updates the
+  // cells of global parity (i + j) & 1 == c.
+  __device__ __forceinline__ void setup(const float* sb, const uint8_t* si, int par0, int tid) {
 #pragma unroll
-  for (int k = 0; k < K; k++) {
-    int q = tid + k * NT;
-    int lj = r0 + q / P;
-    int pc = q - (lj - r0) * P;
-    int li = 2 * pc + ((lj + par0 + color) & 1);
-    act[k] = q < total && li >= ring && li <= LX - 1 - ring;
-    l[k] = act[k] ? li + lj * LX : LX + 1;  // any interior cell: reads stay in the array
-    ib[k] = si[l[k]];
-    bb[k] = sb[l[k]];
-    nb[k] = zero ? 0.f
-                 : (((sx[l[k] + 1] + sx[l[k] - 1]) + sx[l[k] + LX]) + sx[l[k] - LX]);
-  }
+    for (int c = 0; c < 2; c++)
 #pragma unroll
-  for (int k = 0; k < K; k++) {
-    int cnt = ib[k] & 7;
-    if (act[k] && cnt != 0) sx[l[k]] = gs_update(nb[k], bb[k], cnt);
+      for (int k = 0; k < K; k++) {
+        const int q = tid + k * NT;
+        const int lj = 1 + q / P;
+        const int pc = q - (lj - 1) * P;
+        const int li = 2 * pc + ((lj + par0 + c) & 1);
+        const bool act = q < ITEMS && li >= 1 && li <= LX - 2;
+        const int lq = act ? li + lj * LX : LX + 1;  // inactive: any interior cell
+        const int cnt = act ? (si[lq] & 7) : 0;
+        l[c][k] = lq;
+        b[c][k] = sb[lq];
+        r[c][k] = cnt ? gs_recip(cnt) : 0.f;
+      }
   }
-}
+
+  // One half-sweep of colour C; zero: the unknown is zero on entry
+  // (neighbour sum 0).  All reads are issued before any write.
+  template <int C>
+  __device__ __forceinline__ void sweep(float* sx, bool zero) const {
+    float nv[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int q = l[C][k];
+      const float sum = zero ? 0.f : ((sx[q + 1] + sx[q - 1]) + sx[q + LX]) + sx[q - LX];
+      nv[k] = (sum + b[C][k]) * r[C][k];
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      if (r[C][k] != 0.f) sx[l[C][k]] = nv[k];
+  }
+};
 
 // LDS-typed pointers for the one-workgroup coarse levels, whose arrays sit at
 // runtime offsets of one dynamic LDS block: with the address space explicit
@@ -106,46 +126,78 @@ struct FastDiv {
   }
 };
 
-// Runtime-width variant for the one-workgroup coarse levels (arrays padded
-// by a ring of zero cells), in chunks of 4 cells per thread with the reads of
-// a chunk issued before its writes.
-__device__ __forceinline__ void lds_half_sweep_rt(lds_fp sx, const lds_fp sb, const lds_bp si,
-                                                  int LX, int LY, int color, bool zero, int tid,
-                                                  int nthreads) {
-  const int w = LX - 2, P = (w + 1) >> 1;
-  const int total = P * (LY - 2);
-  const FastDiv dP(P);
-  for (int base = 0; base < total; base += 4 * nthreads) {
-    int l[4];
-    bool act[4];
-    float nb[4], bb[4];
-    uint8_t ib[4];
+// The one-workgroup coarse levels' variant of TileSweep: runtime level sizes
+// (arrays padded by a ring of zero cells, pitch LX), NC threads and up to KC
+// cells per thread and colour, so a level holds at most NC * KC cells of a
+// colour (the host keeps coarse levels within that).  A wave whose slots are
+// all past the level's cells skips them, so on the small levels most of the
+// workgroup does nothing.
+template <int NC, int KC>
+struct CoarseSweep {
+  int l[2][KC];
+  float b[2][KC], r[2][KC];
+  int nk;  // slots holding a cell for this wave (wave-uniform)
+  int LX;
+
+  __device__ __forceinline__ void setup(const lds_fp sb, const lds_bp si, int lx, int ly,
+                                        int tid) {
+    LX = lx;
+    const int P = (lx - 1) >> 1;  // pairs per row of the nx = lx - 2 cells
+    const int total = P * (ly - 2);
+    const int wbase = __builtin_amdgcn_readfirstlane(tid & ~63);
+    nk = total > wbase ? min(KC, (total - wbase + NC - 1) / NC) : 0;
+    const FastDiv dP(P);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      act[k] = false;
-      ib[k] = 0;
-      l[k] = 0;
-      nb[k] = bb[k] = 0.f;
-      if (base + k * nthreads >= total) continue;  // uniform: nothing of this slot is left
-      int q = base + tid + k * nthreads;
-      int r = dP.div(q), pc = q - r * P;
-      int lj = 1 + r;
-      // padded (0,0) is global (-1,-1): global parity of (li, lj) = li + lj
-      int li = 1 + 2 * pc + (((1 + 2 * pc + lj + color) & 1) ? 1 : 0);
-      act[k] = q < total && li <= LX - 2;
-      l[k] = act[k] ? li + lj * LX : LX + 1;
-      ib[k] = si[l[k]];
-      bb[k] = sb[l[k]];
-      nb[k] = zero ? 0.f
-                   : (((sx[l[k] + 1] + sx[l[k] - 1]) + sx[l[k] + LX]) + sx[l[k] - LX]);
-    }
+    for (int c = 0; c < 2; c++)
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      int cnt = ib[k] & 7;
-      if (act[k] && cnt != 0) sx[l[k]] = gs_update(nb[k], bb[k], cnt);
-    }
+      for (int k = 0; k < KC; k++) {
+        l[c][k] = lx + 1;
+        b[c][k] = 0.f;
+        r[c][k] = 0.f;
+        if (k >= nk) continue;
+        const int q = tid + k * NC;
+        const int rw = dP.div(q), pc = q - rw * P;
+        const int lj = 1 + rw;
+        // padded (0,0) is global (-1,-1): global parity of (li, lj) = li + lj
+        const int li = 1 + 2 * pc + ((1 + lj + c) & 1);
+        const bool act = q < total && li <= lx - 2;
+        const int lq = act ? li + lj * lx : lx + 1;
+        const int cnt = act ? (si[lq] & 7) : 0;
+        l[c][k] = lq;
+        b[c][k] = sb[lq];
+        r[c][k] = cnt ? gs_recip(cnt) : 0.f;
+      }
   }
-}
+
+  template <int C>
+  __device__ __forceinline__ void sweep(lds_fp sx, bool zero) const {
+    float nv[KC];
+#pragma unroll
+    for (int k = 0; k < KC; k++) {
+      nv[k] = 0.f;
+      if (k >= nk) continue;
+      const int q = l[C][k];
+      const float sum = zero ? 0.f : ((sx[q + 1] + sx[q - 1]) + sx[q + LX]) + sx[q - LX];
+      nv[k] = (sum + b[C][k]) * r[C][k];
+    }
+#pragma unroll
+    for (int k = 0; k < KC; k++)
+      if (k < nk && r[C][k] != 0.f) sx[l[C][k]] = nv[k];
+  }
+
+  // Four half-sweeps (two red-black sweeps), colour 0 first, with a barrier
+  // after each.
+  __device__ __forceinline__ void two_sweeps(lds_fp sx, bool zero) const {
+    sweep<0>(sx, zero);
+    __syncthreads();
+    sweep<1>(sx, false);
+    __syncthreads();
+    sweep<0>(sx, false);
+    __syncthreads();
+    sweep<1>(sx, false);
+    __syncthreads();
+  }
+};
 
 // Loads an LX x LY window with global origin (gx0, gy0) into LDS (zeros off
 // the domain; x = 0 when xin is null).  Every thread first issues all of its
@@ -231,11 +283,16 @@ __device__ __forceinline__ void smooth_restrict_tile(const float* __restrict__ x
   if (!stage_tile<AX, AY>(sx, sb, si, zero ? nullptr : xin, b, info, nx, ny, gx0, gy0, tid, done))
     return;
   __syncthreads();
-  const int par0 = (gx0 + gy0) & 1;
-  for (int s = 1; s <= 4; s++) {
-    lds_half_sweep<AX, AY>(sx, sb, si, s, (s - 1) & 1, par0, zero && s == 1, tid);
-    __syncthreads();
-  }
+  TileSweep<AX, AY> sw;
+  sw.setup(sb, si, (gx0 + gy0) & 1, tid);
+  sw.template sweep<0>(sx, zero);
+  __syncthreads();
+  sw.template sweep<1>(sx, false);
+  __syncthreads();
+  sw.template sweep<0>(sx, false);
+  __syncthreads();
+  sw.template sweep<1>(sx, false);
+  __syncthreads();
   // coarse cells of this tile: children (2I+di, 2J+dj) are tile-interior
   const int CX = TX / 2, CY = TY / 2;
   for (int q = tid; q < CX * CY; q += NT) {
@@ -304,6 +361,8 @@ __device__ __forceinline__ void prolong_smooth_tile(const float* __restrict__ xi
     }
   }
   __syncthreads();
+  TileSweep<BXW, BYH> sw;
+  sw.setup(sb, si, (gx0 + gy0) & 1, tid);
   for (int q = tid; q < BXW * BYH; q += NT) {
     if (!(si[q] & 0x80)) continue;
     int lj = q / BXW, li = q - lj * BXW;
@@ -318,11 +377,14 @@ __device__ __forceinline__ void prolong_smooth_tile(const float* __restrict__ xi
     sx[q] += 0.5625f * e00 + 0.1875f * (e10 + e01) + 0.0625f * e11;
   }
   __syncthreads();
-  const int par0 = (gx0 + gy0) & 1;
-  for (int s = 1; s <= 4; s++) {
-    lds_half_sweep<BXW, BYH>(sx, sb, si, s, (s - 1) & 1, par0, false, tid);
-    __syncthreads();
-  }
+  sw.template sweep<0>(sx, false);
+  __syncthreads();
+  sw.template sweep<1>(sx, false);
+  __syncthreads();
+  sw.template sweep<0>(sx, false);
+  __syncthreads();
+  sw.template sweep<1>(sx, false);
+  __syncthreads();
   for (int q = tid; q < TX * TY; q += NT) {
     int lj = q / TX, li = q - lj * TX;
     int gi = ox + li, gj = oy + lj;
@@ -358,6 +420,13 @@ struct CLevel {
   int nx, ny, px, py;
 };
 
+// Bytes of all levels' arrays; the direct solve's dense vector follows them.
+__device__ __forceinline__ int coarse_levels_bytes(const CoarseLevels& L) {
+  int off = 0;
+  for (int k = 0; k < L.n; k++) off += clevel_bytes(L.nx[k], L.ny[k]);
+  return off;
+}
+
 __device__ __forceinline__ CLevel clevel(lds_bp smem, const CoarseLevels& L, int l) {
   int off = 0;
   for (int k = 0; k < l; k++) off += clevel_bytes(L.nx[k], L.ny[k]);
@@ -368,7 +437,7 @@ __device__ __forceinline__ CLevel clevel(lds_bp smem, const CoarseLevels& L, int
 
 // Body of the sub-cycle for a workgroup of NC threads; smem is the dynamic
 // LDS of coarse_vcycle_lds_bytes(L).
-template <int NC, int KC = 4096 / NC>
+template <int NC, int KC = kCoarseSweepCells / NC>
 __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
                                                    const float* __restrict__ b0,
                                                    const float* __restrict__ x0in,
@@ -378,6 +447,9 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
                                                    int direct, unsigned char* smem_generic) {
   const int tid = threadIdx.x;
   const lds_bp smem = (lds_bp)smem_generic;
+  // this thread's coarse unknown for the direct solve's gather, loaded now
+  // so its latency hides behind the staging
+  const int my_cell = (direct && tid < ncoarse) ? cells[tid] : 0;
   {
     // All levels' padded cells as one index range, KC per thread, loads
     // issued together before the LDS writes (one memory wait per thread).
@@ -398,7 +470,7 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
           n = padded(L.nx[l], L.ny[l]);
         }
         int nx = L.nx[l], ny = L.ny[l], px = nx + 2;
-        int lj = q / px, li = q - lj * px;
+        int lj = FastDiv(px).div(q), li = q - lj * px;
         int gi = li - 1, gj = lj - 1;
         bool in = t < total && gi >= 0 && gj >= 0 && gi < nx && gj < ny;
         int g = in ? gi + gj * nx : 0;
@@ -422,9 +494,10 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
   for (int l = 0; l < L.n - 1; l++) {
     const CLevel F = clevel(smem, L, l), G = clevel(smem, L, l + 1);
     bool zero = (l > 0) || !x0in;
-    for (int s = 1; s <= 4; s++) {
-      lds_half_sweep_rt(F.x, F.b, F.i, F.px, F.py, (s - 1) & 1, zero && s == 1, tid, NC);
-      __syncthreads();
+    {
+      CoarseSweep<NC, KC> sw;
+      sw.setup(F.b, F.i, F.px, F.py, tid);
+      sw.two_sweeps(F.x, zero);
     }
     const FastDiv dc(G.nx);
     for (int q = tid; q < G.nx * G.ny; q += NC) {
@@ -445,42 +518,46 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
     const CLevel C = clevel(smem, L, l);
     bool zero = (l > 0) || !x0in;
     if (direct) {
-      // x = M b in FP64, eight lanes per row; lane `part` takes columns
-      // part + 8m in chunks of 8, all 8 loads of a chunk in flight together.
+      // b at the coarse unknowns gathered into a dense LDS vector, with
+      // their padded LDS indices beside it (both after the level arrays),
+      // then x = M b in FP64, eight lanes per row; lane `part` takes columns
+      // part + 8m, sixteen loads of M in flight at once.
+      const lds_fp bv = (lds_fp)(smem + coarse_levels_bytes(L));
+      typedef __attribute__((address_space(3))) int* lds_ip;
+      const lds_ip bi = (lds_ip)(smem + coarse_levels_bytes(L) + 4 * kMaxDirect);
       const FastDiv dn(C.nx);
+      for (int q = tid; q < ncoarse; q += NC) {
+        const int c = q == tid ? my_cell : cells[q], cj = dn.div(c), ci = c - cj * C.nx;
+        const int k = (ci + 1) + (cj + 1) * C.px;
+        bi[q] = k;
+        bv[q] = C.b[k];
+      }
+      __syncthreads();
       for (int t = tid; t < 8 * ncoarse; t += NC) {
         int a = t >> 3, part = t & 7;
         const double* row = M + (size_t)a * ncoarse;
         double s = 0.0;
-        for (int q0 = part; q0 < ncoarse; q0 += 64) {
-          double mv[8];
-          int cv[8];
+        for (int q0 = part; q0 < ncoarse; q0 += 128) {
+          double mv[16];
 #pragma unroll
-          for (int m = 0; m < 8; m++) {
+          for (int m = 0; m < 16; m++) {
             int q = q0 + 8 * m;
             mv[m] = q < ncoarse ? row[q] : 0.0;
-            cv[m] = q < ncoarse ? cells[q] : 0;
           }
 #pragma unroll
-          for (int m = 0; m < 8; m++)
-            if (q0 + 8 * m < ncoarse) {
-              const int cj = dn.div(cv[m]), ci = cv[m] - cj * C.nx;
-              s += mv[m] * (double)C.b[(ci + 1) + (cj + 1) * C.px];
-            }
+          for (int m = 0; m < 16; m++)
+            if (q0 + 8 * m < ncoarse) s += mv[m] * (double)bv[q0 + 8 * m];
         }
         s += __shfl_xor(s, 1, 64);
         s += __shfl_xor(s, 2, 64);
         s += __shfl_xor(s, 4, 64);
-        if (part == 0) {
-          const int c = cells[a], cj = dn.div(c), ci = c - cj * C.nx;
-          C.x[(ci + 1) + (cj + 1) * C.px] = (float)s;
-        }
+        if (part == 0) C.x[bi[a]] = (float)s;
       }
     } else {
-      for (int s = 0; s < 16; s++) {
-        lds_half_sweep_rt(C.x, C.b, C.i, C.px, C.py, s & 1, zero && s == 0, tid, NC);
-        __syncthreads();
-      }
+      CoarseSweep<NC, KC> sw;
+      sw.setup(C.b, C.i, C.px, C.py, tid);
+      sw.two_sweeps(C.x, zero);
+      for (int s = 1; s < 4; s++) sw.two_sweeps(C.x, false);
     }
     __syncthreads();
   }
@@ -500,9 +577,10 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
                  0.0625f * E[(I2 + 1) + (J2 + 1) * G.px];
     }
     __syncthreads();
-    for (int s = 1; s <= 4; s++) {
-      lds_half_sweep_rt(F.x, F.b, F.i, F.px, F.py, (s - 1) & 1, false, tid, NC);
-      __syncthreads();
+    {
+      CoarseSweep<NC, KC> sw;
+      sw.setup(F.b, F.i, F.px, F.py, tid);
+      sw.two_sweeps(F.x, false);
     }
   }
   {
