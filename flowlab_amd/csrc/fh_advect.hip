@@ -130,8 +130,28 @@ __device__ __forceinline__ bool within_reach(float m, float c, int tid) {
   return fabsf(c) * vw <= kReach;
 }
 
-__device__ __forceinline__ LField window(const float (*lds)[WN], int q, int wx0, int wy0) {
-  return LField{(lds_cfp)lds[q], wx0, wy0, WW};
+template <class F>
+__device__ __forceinline__ F window_as(const float (*lds)[WN], int q, int wx0, int wy0) {
+  F f;
+  f.l = (lds_cfp)lds[q];
+  f.x0 = wx0;
+  f.y0 = wy0;
+  f.w = WW;
+  return f;
+}
+
+// Tiles clear of the domain edges.  With D <= kReach every position a face
+// or cell (i, j) of the tile samples -- the backtraces both ways, their
+// midpoints, the clamp stencil, with the half-cell offsets of the staggered
+// fields -- has its stencil origin in [i-2, i+1] x [j-2, j+1], so when the
+// tile's indices lie kEdge cells inside on every side no origin reaches the
+// clamps (below 0, above count-2) of any field: sampling without the clamp
+// (LFieldI) takes the same indices and weights.
+constexpr int kEdge = 3;
+
+__device__ __forceinline__ bool interior_tile(int x0, int y0, int nx, int ny) {
+  return x0 >= kEdge && y0 >= kEdge && x0 + TX - 1 <= nx - 1 - kEdge &&
+         y0 + TY - 1 <= ny - 1 - kEdge;
 }
 
 // advect.ts:72-112 with carrier == source (the only use: u and v advect
@@ -148,8 +168,16 @@ __global__ __launch_bounds__(256) void k_advect_faces_lds(
   const float m = stage_windows<2>(lds, f, wx0, wy0, tid);
   const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
   if (within_reach(m, c, tid)) {
-    const LField u = window(lds, 0, wx0, wy0), v = window(lds, 1, wx0, wy0);
-    for (int r = 0; r < TY / BY; r++) advect_faces_t(u, v, u, v, uo, vo, lbl, nx, ny, c, i, j + r * BY);
+    auto run = [&](auto tag) {
+      using F = decltype(tag);
+      const F u = window_as<F>(lds, 0, wx0, wy0), v = window_as<F>(lds, 1, wx0, wy0);
+      for (int r = 0; r < TY / BY; r++)
+        advect_faces_t(u, v, u, v, uo, vo, lbl, nx, ny, c, i, j + r * BY);
+    };
+    if (interior_tile(x0, y0, nx, ny))
+      run(LFieldI{});
+    else
+      run(LField{});
   } else {
     const GField u{uc, nx + 1}, v{vc, nx};
     for (int r = 0; r < TY / BY; r++) advect_faces_t(u, v, u, v, uo, vo, lbl, nx, ny, c, i, j + r * BY);
@@ -170,10 +198,17 @@ __global__ __launch_bounds__(256) void k_maccormack_faces_lds(
   const float m = stage_windows<4>(lds, f, wx0, wy0, tid);
   const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
   if (within_reach(m, c, tid)) {
-    const LField a = window(lds, 0, wx0, wy0), b = window(lds, 1, wx0, wy0),
-                 ah = window(lds, 2, wx0, wy0), bh = window(lds, 3, wx0, wy0);
-    for (int r = 0; r < TY / BY; r++)
-      maccormack_faces_t(a, b, ah, bh, uo, vo, lbl, nx, ny, c, i, j + r * BY);
+    auto run = [&](auto tag) {
+      using F = decltype(tag);
+      const F a = window_as<F>(lds, 0, wx0, wy0), b = window_as<F>(lds, 1, wx0, wy0),
+              ah = window_as<F>(lds, 2, wx0, wy0), bh = window_as<F>(lds, 3, wx0, wy0);
+      for (int r = 0; r < TY / BY; r++)
+        maccormack_faces_t(a, b, ah, bh, uo, vo, lbl, nx, ny, c, i, j + r * BY);
+    };
+    if (interior_tile(x0, y0, nx, ny))
+      run(LFieldI{});
+    else
+      run(LField{});
   } else {
     const GField a{un, nx + 1}, b{vn, nx}, ah{uh, nx + 1}, bh{vh, nx};
     for (int r = 0; r < TY / BY; r++)
@@ -198,11 +233,18 @@ __global__ __launch_bounds__(256) void k_advect_dye_lds(
   const float m = stage_windows<5>(lds, f, wx0, wy0, tid);
   const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
   if (within_reach(m, c, tid)) {
-    const LField a = window(lds, 0, wx0, wy0), b = window(lds, 1, wx0, wy0),
-                 q0 = window(lds, 2, wx0, wy0), q1 = window(lds, 3, wx0, wy0),
-                 q2 = window(lds, 4, wx0, wy0);
-    for (int r = 0; r < TY / BY; r++)
-      advect_dye_t(a, b, q0, q1, q2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i, j + r * BY);
+    auto run = [&](auto tag) {
+      using F = decltype(tag);
+      const F a = window_as<F>(lds, 0, wx0, wy0), b = window_as<F>(lds, 1, wx0, wy0),
+              q0 = window_as<F>(lds, 2, wx0, wy0), q1 = window_as<F>(lds, 3, wx0, wy0),
+              q2 = window_as<F>(lds, 4, wx0, wy0);
+      for (int r = 0; r < TY / BY; r++)
+        advect_dye_t(a, b, q0, q1, q2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i, j + r * BY);
+    };
+    if (interior_tile(x0, y0, nx, ny))
+      run(LFieldI{});
+    else
+      run(LField{});
   } else {
     const GField a{u, nx + 1}, b{v, nx}, q0{q, nx}, q1{q + plane, nx}, q2{q + 2 * plane, nx};
     for (int r = 0; r < TY / BY; r++)
@@ -228,13 +270,20 @@ __global__ __launch_bounds__(256) void k_maccormack_dye_lds(
   const float m = stage_windows<8>(lds, f, wx0, wy0, tid);
   const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
   if (within_reach(m, c, tid)) {
-    const LField a = window(lds, 0, wx0, wy0), b = window(lds, 1, wx0, wy0),
-                 n0 = window(lds, 2, wx0, wy0), n1 = window(lds, 3, wx0, wy0),
-                 n2 = window(lds, 4, wx0, wy0), h0 = window(lds, 5, wx0, wy0),
-                 h1 = window(lds, 6, wx0, wy0), h2 = window(lds, 7, wx0, wy0);
-    for (int r = 0; r < TY / BY; r++)
-      maccormack_dye_t(a, b, n0, n1, n2, h0, h1, h2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i,
-                       j + r * BY);
+    auto run = [&](auto tag) {
+      using F = decltype(tag);
+      const F a = window_as<F>(lds, 0, wx0, wy0), b = window_as<F>(lds, 1, wx0, wy0),
+              n0 = window_as<F>(lds, 2, wx0, wy0), n1 = window_as<F>(lds, 3, wx0, wy0),
+              n2 = window_as<F>(lds, 4, wx0, wy0), h0 = window_as<F>(lds, 5, wx0, wy0),
+              h1 = window_as<F>(lds, 6, wx0, wy0), h2 = window_as<F>(lds, 7, wx0, wy0);
+      for (int r = 0; r < TY / BY; r++)
+        maccormack_dye_t(a, b, n0, n1, n2, h0, h1, h2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i,
+                         j + r * BY);
+    };
+    if (interior_tile(x0, y0, nx, ny))
+      run(LFieldI{});
+    else
+      run(LField{});
   } else {
     const GField a{u, nx + 1}, b{v, nx}, n0{qn, nx}, n1{qn + plane, nx}, n2{qn + 2 * plane, nx},
         h0{qh, nx}, h1{qh + plane, nx}, h2{qh + 2 * plane, nx};

@@ -56,6 +56,18 @@ __device__ __forceinline__ Axis clamped_axis(int base, float off, int count) {
   return {i0, f};
 }
 
+// clamped_axis where the caller has shown that the clamp cannot act (pos
+// stays inside [0, count-1) by more than a cell): the same i0 and f.
+template <bool CLAMP>
+__device__ __forceinline__ Axis axis_t(int base, float off, int count) {
+  if constexpr (CLAMP) {
+    return clamped_axis(base, off, count);
+  } else {
+    const float fl = floorf(off);
+    return {base + (int)fl, off - fl};
+  }
+}
+
 // grid.ts:180-191
 __device__ __forceinline__ float bilerp(float v00, float v10, float v01, float v11, float fx,
                                         float fy) {
@@ -127,6 +139,7 @@ __device__ __forceinline__ void backtrace(const float* __restrict__ uc,
 // reads a window of the field staged in LDS, for callers that have shown
 // every index they will read lies inside the window.
 struct GField {
+  static constexpr bool kClamp = true;  // sampling clamps to the domain (grid.ts:173-177)
   const float* __restrict__ p;
   int s;  // row stride
   __device__ __forceinline__ float at(int i, int j) const { return p[i + j * s]; }
@@ -139,6 +152,7 @@ struct GField {
 typedef __attribute__((address_space(3))) const float* lds_cfp;
 
 struct LField {
+  static constexpr bool kClamp = true;
   lds_cfp l;    // window origin: field index (x0, y0)
   int x0, y0, w;  // w = window row stride
   __device__ __forceinline__ float at(int i, int j) const { return l[(i - x0) + (j - y0) * w]; }
@@ -148,6 +162,12 @@ struct LField {
   }
 };
 
+// An LDS window whose every sample position is known to stay clear of the
+// domain edges, so sampling skips the clamp (it would not act).
+struct LFieldI : LField {
+  static constexpr bool kClamp = false;
+};
+
 template <class F>
 __device__ __forceinline__ Quad gather4f(const F& q, Axis ax, Axis ay) {
   return q.quad(ax.i0, ay.i0);
@@ -155,16 +175,16 @@ __device__ __forceinline__ Quad gather4f(const F& q, Axis ax, Axis ay) {
 
 template <class F>
 __device__ __forceinline__ float sample_uf(const F& u, int nx, int ny, Pos x, Pos y) {
-  Axis ax = clamped_axis(x.i, x.f, nx + 1);
-  Axis ay = clamped_axis(y.i, y.f - 0.5f, ny);
+  Axis ax = axis_t<F::kClamp>(x.i, x.f, nx + 1);
+  Axis ay = axis_t<F::kClamp>(y.i, y.f - 0.5f, ny);
   Quad g = gather4f(u, ax, ay);
   return bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
 }
 
 template <class F>
 __device__ __forceinline__ float sample_vf(const F& v, int nx, int ny, Pos x, Pos y) {
-  Axis ax = clamped_axis(x.i, x.f - 0.5f, nx);
-  Axis ay = clamped_axis(y.i, y.f, ny + 1);
+  Axis ax = axis_t<F::kClamp>(x.i, x.f - 0.5f, nx);
+  Axis ay = axis_t<F::kClamp>(y.i, y.f, ny + 1);
   Quad g = gather4f(v, ax, ay);
   return bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
 }

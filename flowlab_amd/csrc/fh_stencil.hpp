@@ -87,8 +87,8 @@ __device__ __forceinline__ void maccormack_faces_t(const NU& un, const NV& vn, c
       float ubar = sample_uf(uh, nx, ny, rx, ry);
       float corrected = uh.at(i, j) + 0.5f * (u0 - ubar);
       backtracef(un, vn, nx, ny, X, Y, u0, v0, c, &fx, &fy);
-      Axis ax = clamped_axis(fx.i, fx.f, nx + 1);
-      Axis ay = clamped_axis(fy.i, fy.f - 0.5f, ny);
+      Axis ax = axis_t<NU::kClamp>(fx.i, fx.f, nx + 1);
+      Axis ay = axis_t<NU::kClamp>(fy.i, fy.f - 0.5f, ny);
       uo[k] = clamp_to_quad(corrected, gather4f(un, ax, ay));
     }
   }
@@ -104,8 +104,8 @@ __device__ __forceinline__ void maccormack_faces_t(const NU& un, const NV& vn, c
       float vbar = sample_vf(vh, nx, ny, rx, ry);
       float corrected = vh.at(i, j) + 0.5f * (v0 - vbar);
       backtracef(un, vn, nx, ny, X, Y, u0, v0, c, &fx, &fy);
-      Axis ax = clamped_axis(fx.i, fx.f - 0.5f, nx);
-      Axis ay = clamped_axis(fy.i, fy.f, ny + 1);
+      Axis ax = axis_t<NV::kClamp>(fx.i, fx.f - 0.5f, nx);
+      Axis ay = axis_t<NV::kClamp>(fy.i, fy.f, ny + 1);
       vo[k] = clamp_to_quad(corrected, gather4f(vn, ax, ay));
     }
   }
@@ -152,8 +152,8 @@ __device__ __forceinline__ void advect_dye_t(const CU& u, const CV& v, const Q& 
   Pos bx, by;
   backtracef(u, v, nx, ny, X, Y, u0, v0, c, &bx, &by);
   Pos dx{i, 0.5f + ds * (bx.f - X.f)}, dy{j, 0.5f + ds * (by.f - Y.f)};
-  Axis ax = clamped_axis(dx.i, dx.f - 0.5f, dnx);
-  Axis ay = clamped_axis(dy.i, dy.f - 0.5f, dny);
+  Axis ax = axis_t<Q::kClamp>(dx.i, dx.f - 0.5f, dnx);
+  Axis ay = axis_t<Q::kClamp>(dy.i, dy.f - 0.5f, dny);
   Quad g0 = gather4f(q0, ax, ay);
   Quad g1 = gather4f(q1, ax, ay);
   Quad g2 = gather4f(q2, ax, ay);
@@ -197,8 +197,8 @@ __device__ __forceinline__ void maccormack_dye_t(const CU& u, const CV& v, const
   float v0 = sample_vf(v, nx, ny, X, Y);
   Pos bx, by;
   backtracef(u, v, nx, ny, X, Y, u0, v0, -c, &bx, &by);
-  Axis ax = clamped_axis(i, ds * (bx.f - X.f), dnx);
-  Axis ay = clamped_axis(j, ds * (by.f - Y.f), dny);
+  Axis ax = axis_t<Q::kClamp>(i, ds * (bx.f - X.f), dnx);
+  Axis ay = axis_t<Q::kClamp>(j, ds * (by.f - Y.f), dny);
   Quad r0 = gather4f(h0, ax, ay);
   Quad r1 = gather4f(h1, ax, ay);
   Quad r2 = gather4f(h2, ax, ay);
@@ -206,8 +206,8 @@ __device__ __forceinline__ void maccormack_dye_t(const CU& u, const CV& v, const
   float c1 = h1.at(i, j) + 0.5f * (n1.at(i, j) - bilerp(r1.a, r1.b, r1.c, r1.d, ax.f, ay.f));
   float c2 = h2.at(i, j) + 0.5f * (n2.at(i, j) - bilerp(r2.a, r2.b, r2.c, r2.d, ax.f, ay.f));
   backtracef(u, v, nx, ny, X, Y, u0, v0, c, &bx, &by);
-  ax = clamped_axis(i, ds * (bx.f - X.f), dnx);
-  ay = clamped_axis(j, ds * (by.f - Y.f), dny);
+  ax = axis_t<Q::kClamp>(i, ds * (bx.f - X.f), dnx);
+  ay = axis_t<Q::kClamp>(j, ds * (by.f - Y.f), dny);
   qo[k] = clamp_to_quad(c0, gather4f(n0, ax, ay)) * k0;
   qo[k + plane] = clamp_to_quad(c1, gather4f(n1, ax, ay)) * k1;
   qo[k + 2 * plane] = clamp_to_quad(c2, gather4f(n2, ax, ay)) * k2;
