@@ -1,0 +1,35 @@
+// Host stand-ins for the HIP built-ins the LDS advection kernels use, so the
+// kernels' bodies run on the CPU under tests/test_advect_emu.py.  Test only:
+// nothing here is compiled into libflowhip.
+#pragma once
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+struct dim3 {
+  unsigned x = 1, y = 1, z = 1;
+  dim3(unsigned a = 1, unsigned b = 1, unsigned c = 1) : x(a), y(b), z(c) {}
+};
+extern dim3 threadIdx, blockIdx, gridDim, blockDim;
+struct EmuBarrier {};
+extern int emu_phase;
+// Each emulated thread runs the kernel twice: in phase 0 up to its first
+// barrier (the LDS staging, whose writes are idempotent), in phase 1 to the
+// end with every stage already in place.
+inline void __syncthreads() {
+  if (emu_phase == 0) throw EmuBarrier{};
+}
+typedef void* hipStream_t;
+#define __device__
+#define __host__
+#define __global__
+#define __forceinline__ inline
+#define __launch_bounds__(...)
+#define __shared__ static
+#define __restrict__
+template <class T>
+T __shfl_xor(T v, int, int) {
+  return v;
+}
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }

@@ -1,0 +1,47 @@
+"""Writes the CPU-emulation build of the LDS advection kernels (test only):
+copies the device headers and fh_advect.hip from flowlab_amd/csrc into
+`out`, with every unchecked LDS read bounds-checked against its window, every
+clamp-free sampling axis compared with the clamped one, the launches made
+serial loops and the workgroup reach test evaluated serially."""
+import os
+import re
+import shutil
+import sys
+
+CSRC = os.path.join(os.path.dirname(__file__), "..", "..", "flowlab_amd", "csrc")
+
+
+def make(out: str) -> None:
+    os.makedirs(out, exist_ok=True)
+    for f in ("fh_device.hpp", "fh_stencil.hpp", "fh_launch.hpp"):
+        shutil.copy(os.path.join(CSRC, f), out)
+    p = os.path.join(out, "fh_device.hpp")
+    dev = open(p).read()
+    subs = [
+        ("__device__ __forceinline__ float at(int i, int j) const { return l[(i - x0) + (j - y0) * w]; }",
+         "__device__ __forceinline__ float at(int i, int j) const { emu_inwin(i - x0, j - y0, 0); "
+         "return l[(i - x0) + (j - y0) * w]; }"),
+        ("    lds_cfp r = l + (i - x0) + (j - y0) * w;",
+         "    emu_inwin(i - x0, j - y0, 1);\n    lds_cfp r = l + (i - x0) + (j - y0) * w;"),
+        ("    const float fl = floorf(off);\n    return {base + (int)fl, off - fl};",
+         "    const float fl = floorf(off);\n    Axis r{base + (int)fl, off - fl};\n"
+         "    emu_axis_check(r, clamped_axis(base, off, count));\n    return r;"),
+    ]
+    for a, b in subs:
+        if dev.count(a) != 1:
+            raise RuntimeError(f"fh_device.hpp changed: {a[:50]!r} not found once")
+        dev = dev.replace(a, b)
+    open(p, "w").write(dev)
+    src = open(os.path.join(CSRC, "fh_advect.hip")).read()
+    src = re.sub(r"(\w+)<<<(.*?), (dim3\([^)]*\)), 0, s>>>\(", r"emu_launch(\1, \2, \3, ", src,
+                 flags=re.S)
+    n = src.count("within_reach(m, c, tid)")
+    if n != 4:
+        raise RuntimeError(f"fh_advect.hip changed: {n} reach tests")
+    src = src.replace("within_reach(m, c, tid)", "emu_within_reach(lds, c, kReach, WN)")
+    src += "\nint emu_win_w = fh::WW, emu_win_h = fh::WH;\n"
+    open(os.path.join(out, "fh_advect_emu.cpp"), "w").write('#include "emu.hpp"\n' + src)
+
+
+if __name__ == "__main__":
+    make(sys.argv[1])
