@@ -219,16 +219,31 @@ __device__ __forceinline__ bool stage_tile(float* sx, float* sb, uint8_t* si,
   constexpr int N = LX * LY, K = (N + NT - 1) / NT;
   float rx[K], rb[K];
   uint8_t ri[K];
+  if (gx0 >= 0 && gy0 >= 0 && gx0 + LX <= nx && gy0 + LY <= ny) {
+    // the window lies inside the domain (most tiles): no cell needs a test,
+    // and the last partial slot reads a clamped in-window index
+    const int base = gx0 + gy0 * nx;
 #pragma unroll
-  for (int k = 0; k < K; k++) {
-    int q = tid + k * NT;
-    int lj = q / LX, li = q - lj * LX;
-    int gi = gx0 + li, gj = gy0 + lj;
-    bool in = q < N && gi >= 0 && gj >= 0 && gi < nx && gj < ny;
-    int g = in ? gi + gj * nx : 0;
-    rx[k] = (in && xin) ? xin[g] : 0.f;
-    rb[k] = in ? b[g] : 0.f;
-    ri[k] = in ? info[g] : 0;
+    for (int k = 0; k < K; k++) {
+      const int q = min(tid + k * NT, N - 1);
+      const int lj = q / LX, li = q - lj * LX;
+      const int g = base + li + lj * nx;
+      rx[k] = xin ? xin[g] : 0.f;
+      rb[k] = b[g];
+      ri[k] = info[g];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      int q = tid + k * NT;
+      int lj = q / LX, li = q - lj * LX;
+      int gi = gx0 + li, gj = gy0 + lj;
+      bool in = q < N && gi >= 0 && gj >= 0 && gi < nx && gj < ny;
+      int g = in ? gi + gj * nx : 0;
+      rx[k] = (in && xin) ? xin[g] : 0.f;
+      rb[k] = in ? b[g] : 0.f;
+      ri[k] = in ? info[g] : 0;
+    }
   }
   if (stop) return false;
 #pragma unroll
