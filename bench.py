@@ -317,7 +317,7 @@ def run_cpu_baseline(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the solver runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design note, not a measured path)",
               file=sys.stderr)
         sys.exit(2)

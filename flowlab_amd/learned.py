@@ -5,7 +5,8 @@ and activations, FP32 accumulation on MFMA) runs inside libflowhip.  It sees
 the defect of the previous pressure for the new right-hand side, the solid
 and Air masks and the previous pressure, and predicts a correction; the
 defect-correction multigrid then runs its V-cycles to the tolerance exactly
-as without it, so plain multigrid stays both the fallback and the judge.
+as without it, so plain multigrid stays both the fallback and the arbiter
+of convergence.
 
 This module: the network handle (parameters, save/load), the seeded scene
 generator of the training workload, data generation from the engine's own

@@ -207,7 +207,7 @@ int fh_set_advect_staging(fh_engine* e, int on);
  * A 4-level convolutional U-Net (3x3 convs, channels 16-32-64-64, BF16
  * weights and activations, FP32 accumulation on MFMA) predicts a correction
  * to the previous pressure; the defect-correction solve then runs its
- * V-cycles to the tolerance as before, so plain multigrid stays the judge.
+ * V-cycles to the tolerance as before, so plain multigrid decides convergence.
  * Parameters: per layer weights [cout][3][3][cin] then bias [cout], FP32,
  * for the 15 layers e0a e0b e1a e1b e2a e2b b3a b3b d2a d2b d1a d1b d0a d0b
  * head.  Network input: [N][H][W][8] (channels r/s_r, solid, air, p/s_p,

@@ -3,7 +3,7 @@ from identical state.
 
 Inputs are rounded to FP32 first (the device's storage precision) and the
 oracle runs on those same values, so each diff measures the kernel's own
-arithmetic.  The bar is the north-star's per-kernel relative L2 <= 1e-5 in
+arithmetic.  The bar is a per-kernel relative L2 <= 1e-5 in
 FP32, plus the reference gputests' max|diff| < 1e-5*max|p| and residuals
 within 1% for the pressure solvers (src/gpu/redBlack.gputest.ts:155-164,
 multigrid.gputest.ts:231-238).
@@ -322,7 +322,7 @@ def test_uniform_through_flow_steady_on_gpu():
 
 
 def test_full_size_karman_projection_property():
-    """configs[1] size (2048x1024): after a fused step the solve met 1e-6 by
+    """The Karman street at 2048x1024: after a fused step the solve met 1e-6 by
     the oracle's FP64 residual, and the projected field is divergence-free to
     that level -- a size-independent check where the full oracle step would
     take too long."""

@@ -68,7 +68,7 @@ def test_unet_gradient_matches_oracle(net):
 def test_capture_train_and_warm_start():
     """Capture solves of a small scene, train a few steps (the loss falls),
     and check a learned-warm-start solve still reaches 1e-6 by the oracle's
-    FP64 residual -- the multigrid stays the judge."""
+    FP64 residual -- the multigrid still decides convergence."""
     from flowlab_amd import learned as Lrn
     u = Lrn.UNet(seed=2)
     n = 64
@@ -83,7 +83,7 @@ def test_capture_train_and_warm_start():
     res = Lrn.evaluate(u, n=n, scenes=1, warm_steps=4, seed=77)
     for k in ("zero", "previous", "learned"):
         assert res[k]["vcycles"] >= 1
-    # judge: a warm-started solve converges to 1e-6 by the oracle
+    # a warm-started solve converges to 1e-6 by the oracle
     sim = Lrn._sim(n, 123)
     for _ in range(3):
         sim.step()
