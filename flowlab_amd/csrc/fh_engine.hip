@@ -97,7 +97,8 @@ struct fh_engine {
   float* fpartials = nullptr;
   float* fmax = nullptr;
   double* h_sums = nullptr;  // pinned
-  float* h_fmax = nullptr;   // pinned
+  float* h_fmax = nullptr;   // pinned, mapped; d_fmax is its device address
+  float* d_fmax = nullptr;
   // Largest face speed of the resident velocity: every write of u/v bumps
   // vel_epoch; a step enqueues the reduction of its final velocity (and the
   // copy to h_fmax, then fmax_ev) and tags it with fmax_epoch.
@@ -937,7 +938,9 @@ fh_engine* fh_create(int nx, int ny, int dye_scale, double h) {
       hipHostMalloc((void**)&e->h_res, 24 * sizeof(double),
                     hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
       hipHostGetDevicePointer((void**)&e->d_res, e->h_res, 0) != hipSuccess ||
-      hipHostMalloc((void**)&e->h_fmax, sizeof(float), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&e->h_fmax, sizeof(float),
+                    hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&e->d_fmax, e->h_fmax, 0) != hipSuccess) {
     set_err("fh_create: hipHostMalloc failed");
     return bail();
   }
@@ -1112,11 +1115,12 @@ int fh_step(fh_engine* e, int scheme, double dt, double scale, double grad_scale
   if ((st = mark(e, 4))) return st;
   if ((st = project(e, grad_scale))) return st;
   // The next step's CFL reads this velocity (simulation.ts:474-477); the dye
-  // phase below leaves it unchanged, so its reduction and copy go here and
-  // the host's wait for them overlaps the dye kernels.
-  launch_max_speed(e->s, e->u, e->v, e->nx, e->ny, e->fpartials, e->counter, e->fmax);
+  // phase below leaves it unchanged, so its reduction goes here (the result
+  // lands in the mapped h_fmax) and the host's wait for it overlaps the dye
+  // kernels.
+  launch_max_speed(e->s, e->u, e->v, e->nx, e->ny, e->fpartials, e->counter, e->fmax,
+                   e->d_fmax);
   LAUNCHED("max_speed");
-  HIPCHK(hipMemcpyAsync(e->h_fmax, e->fmax, sizeof(float), hipMemcpyDeviceToHost, e->s));
   HIPCHK(hipEventRecord(e->fmax_ev, e->s));
   e->fmax_epoch = e->vel_epoch;
   if ((st = mark(e, 5))) return st;

@@ -57,7 +57,8 @@ void launch_commit_labels(hipStream_t s, double* p, float* u, float* v, float* d
 // max |u|, |v| into *out (simulation.ts:451-459); u and v are read as float4
 // and must be 16-byte aligned (the engine passes hipMalloc bases).
 void launch_max_speed(hipStream_t s, const float* u, const float* v, int nx, int ny,
-                      float* partials, unsigned* counter, float* out);
+                      float* partials, unsigned* counter, float* out,
+                      float* host_out = nullptr);
 
 // ---- pressure (fh_mg.hip) --------------------------------------------------
 // One colour half-sweep: x = (1-w) x + w (sum - scale*b)/count on fluid cells

@@ -126,8 +126,11 @@ __global__ __launch_bounds__(256) void k_max_speed(const float* __restrict__ u,
   block_max_store(m, partials);
 }
 
+// host_out (optional) is mapped pinned host memory that receives the same
+// value, so the host reads it after an event with no copy on the stream.
 __global__ __launch_bounds__(256) void k_finalize_max(const float* __restrict__ partials,
-                                                      int nblocks, float* __restrict__ out) {
+                                                      int nblocks, float* __restrict__ out,
+                                                      float* __restrict__ host_out) {
   __shared__ float sh[4];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   float t = 0.f;
@@ -135,7 +138,11 @@ __global__ __launch_bounds__(256) void k_finalize_max(const float* __restrict__ 
   t = wave_max(t);
   if (lane == 0) sh[wid] = t;
   __syncthreads();
-  if (tid == 0) out[0] = fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
+  if (tid == 0) {
+    const float m = fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
+    out[0] = m;
+    if (host_out) host_out[0] = m;
+  }
 }
 
 void launch_vorticity(hipStream_t s, const float* u, const float* v, float* w, int nx, int ny,
@@ -177,10 +184,10 @@ void launch_commit_labels(hipStream_t s, double* p, float* u, float* v, float* d
 }
 
 void launch_max_speed(hipStream_t s, const float* u, const float* v, int nx, int ny,
-                      float* partials, unsigned* counter, float* out) {
+                      float* partials, unsigned* counter, float* out, float* host_out) {
   k_max_speed<<<1024, 256, 0, s>>>(u, v, (long)(nx + 1) * ny, (long)nx * (ny + 1), partials,
                                    counter, out);
-  k_finalize_max<<<1, 256, 0, s>>>(partials, 1024, out);
+  k_finalize_max<<<1, 256, 0, s>>>(partials, 1024, out, host_out);
 }
 
 }  // namespace fh

@@ -229,6 +229,44 @@ def test_divergence_subtract_outflow_match_oracle():
     assert np.all(gu.reshape(ny, nx + 1)[:, nx - 1] >= 0)
 
 
+@pytest.mark.parametrize("nx,ny,edges", [(96, 64, "lrbt"), (67, 45, "lt"), (3, 3, "lrbt"),
+                                          (5, 4, "rb"), (2, 5, "lr"), (40, 2, "bt")])
+def test_projection_outflow_edges_and_max_speed(nx, ny, edges):
+    """The projection pass (gradient subtraction, outflow at Air edge cells,
+    and the largest face speed of the result, taken in one pass where nx,
+    ny >= 3) against the oracle's subtractGradient then applyOutflow, with
+    Air on the given edges (l, r, b, t) and a solid block; the reported
+    face speed is the maximum of the velocity read back, exactly."""
+    from flowlab_amd.engine import HipEngine
+    g, u, v, lab = _velocity_state(nx, ny, 11)
+    L = lab.reshape(ny, nx).copy()
+    L[:] = O.FLUID
+    if "l" in edges:
+        L[:, 0] = O.AIR
+    if "r" in edges:
+        L[:, nx - 1] = O.AIR
+    if "b" in edges:
+        L[0, :] = O.AIR
+    if "t" in edges:
+        L[ny - 1, :] = O.AIR
+    if nx >= 8 and ny >= 8:
+        L[ny // 3:ny // 2, nx // 3:nx // 2] = O.SOLID
+    lab = L.ravel()
+    p = f32(O.lcg_fill(nx * ny, 5) * 1e-2)
+    p[lab != O.FLUID] = 0
+    e = HipEngine(nx, ny)
+    e.upload(u, v, p, lab, np.zeros(3 * nx * ny))
+    gs = 0.41
+    e.phase_project(gs)
+    gu, gv = e.read_velocity()
+    ou, ov = O.subtract_gradient(nx, ny, p, u, v, lab, gs)
+    ou, ov = O.apply_outflow(nx, ny, ou, ov, lab)
+    assert rel_l2(gu, ou) < REL_L2 and rel_l2(gv, ov) < REL_L2
+    assert np.abs(gu - ou).max() <= 1e-6 * max(np.abs(ou).max(), 1.0)
+    assert np.abs(gv - ov).max() <= 1e-6 * max(np.abs(ov).max(), 1.0)
+    assert e.max_face_speed() == max(np.abs(gu).max(), np.abs(gv).max())
+
+
 @pytest.mark.parametrize("scheme", ["semiLagrangian", "macCormack"])
 def test_dye_phase_with_decay_and_patch(scheme):
     nx, ny = 64, 48
