@@ -109,10 +109,125 @@ __global__ __launch_bounds__(256) void k_conv3x3(const bf16_t* __restrict__ X, i
   }
 }
 
+// The same convolution with the workgroup's input staged in LDS first: its
+// four output rows need input rows y0-1 .. y0+4 and columns x0-1 .. x0+64
+// (zero outside the image), all cin channels, [row][col][cin].  Each input
+// pixel is then fetched from memory once per workgroup instead of once per
+// tap; the MFMA operands and their order are those of k_conv3x3, so the
+// same bits.  Dynamic LDS: kConvRows x kConvCols x cin bf16.
+constexpr int kConvRows = 6, kConvCols = 66;
+
+template <int COUT>
+__global__ __launch_bounds__(256) void k_conv3x3_lds(const bf16_t* __restrict__ X, int cin,
+                                                     const bf16_t* __restrict__ Wt, int kpad,
+                                                     const float* __restrict__ bias, int N,
+                                                     int H, int Wd, int relu,
+                                                     bf16_t* __restrict__ Y,
+                                                     float* __restrict__ Yf) {
+  extern __shared__ unsigned char conv_smem[];
+  bf16_t* sX = reinterpret_cast<bf16_t*>(conv_smem);
+  constexpr int CT = COUT / 16;  // channel tiles
+  constexpr int PT = 4;          // pixel tiles per wave
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n = blockIdx.z;
+  const int y0 = blockIdx.y * 4, y = y0 + wave;
+  const int x0 = blockIdx.x * 64;
+  const size_t plane = (size_t)H * Wd;
+  const bf16_t* Xn = X + (size_t)n * plane * cin;
+  {
+    const int c8 = cin >> 3, chunks = kConvRows * kConvCols * c8;
+    for (int q = threadIdx.x; q < chunks; q += 256) {
+      const int pix = q / c8, part = q - pix * c8;
+      const int r = pix / kConvCols, c = pix - r * kConvCols;
+      const int yy = y0 - 1 + r, xx = x0 - 1 + c;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (yy >= 0 && yy < H && xx >= 0 && xx < Wd)
+        v = reinterpret_cast<const uint4*>(Xn + ((size_t)yy * Wd + xx) * cin)[part];
+      reinterpret_cast<uint4*>(sX)[q] = v;
+    }
+  }
+  __syncthreads();
+  if (y >= H) return;
+  const int j = lane & 15, g = lane >> 4;
+  f32x4 acc[PT][CT];
+#pragma unroll
+  for (int p = 0; p < PT; p++)
+#pragma unroll
+    for (int t = 0; t < CT; t++) acc[p][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int kreal = 9 * cin;
+  for (int k0 = 0; k0 < kpad; k0 += 32) {
+    const int k = k0 + 8 * g;
+    bf16x8 b[PT];
+    if (k < kreal) {
+      const int tap = k / cin, ci = k - tap * cin;
+      const int r = wave + tap / 3, dx = tap % 3 - 1;  // staged row of input row y + tap/3 - 1
+#pragma unroll
+      for (int p = 0; p < PT; p++) {
+        const int c = 16 * p + j + dx + 1;  // staged column of x0 + 16p + j + dx
+        b[p] = __builtin_bit_cast(
+            bf16x8, *reinterpret_cast<const uint4*>(sX + (r * kConvCols + c) * cin + ci));
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < PT; p++) b[p] = zero8();
+    }
+#pragma unroll
+    for (int t = 0; t < CT; t++) {
+      bf16x8 a = load8(Wt + (size_t)(16 * t + j) * kpad + k);
+#pragma unroll
+      for (int p = 0; p < PT; p++)
+        acc[p][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[p], acc[p][t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < PT; p++) {
+    int x = x0 + 16 * p + j;
+    if (x >= Wd) continue;
+    size_t pix = (size_t)n * plane + (size_t)y * Wd + x;
+#pragma unroll
+    for (int t = 0; t < CT; t++) {
+      int co = 16 * t + 4 * g;
+      float v[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        v[r] = acc[p][t][r] + (bias ? bias[co + r] : 0.f);
+        if (relu) v[r] = fmaxf(v[r], 0.f);
+      }
+      if (Yf) {
+        if (co == 0) Yf[pix] = v[0];
+      } else {
+        uint2 packed;
+        packed.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+        packed.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+        *reinterpret_cast<uint2*>(Y + pix * COUT + co) = packed;
+      }
+    }
+  }
+}
+
 void launch_conv3x3(hipStream_t s, const bf16_t* X, int cin, const bf16_t* Wt, int kpad,
                     const float* bias, int cout, int N, int H, int W, int relu, bf16_t* Y,
                     float* Yf) {
   dim3 g((W + 63) / 64, (H + 3) / 4, N);
+  // the workgroup LDS limit of the current device, read once per process
+  static int max_lds = -1;
+  if (max_lds < 0) {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      v = 65536;
+    max_lds = v;
+  }
+  const size_t lds = (size_t)kConvRows * kConvCols * cin * sizeof(bf16_t);
+  if (cin % 8 == 0 && lds <= (size_t)max_lds && (((uintptr_t)X) & 15) == 0) {
+    switch (cout) {
+      case 16: k_conv3x3_lds<16><<<g, 256, lds, s>>>(X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); return;
+      case 32: k_conv3x3_lds<32><<<g, 256, lds, s>>>(X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); return;
+      case 48: k_conv3x3_lds<48><<<g, 256, lds, s>>>(X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); return;
+      case 64: k_conv3x3_lds<64><<<g, 256, lds, s>>>(X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); return;
+      default: break;
+    }
+  }
   switch (cout) {
     case 16: k_conv3x3<16><<<g, 256, 0, s>>>(X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); break;
     case 32: k_conv3x3<32><<<g, 256, 0, s>>>(X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); break;
@@ -245,6 +360,25 @@ __global__ void k_up_concat(const bf16_t* __restrict__ a, int Ca, const bf16_t* 
     out[i] = b[pix * Cb + (c - Ca)];
 }
 
+// The same with 8 channels (16 bytes) per thread, for Ca and Cb multiples of
+// 8 and 16-byte aligned tensors: blockIdx.y is the output row (n, y), the
+// threads of a row take (x, chunk) pairs.  Pure copies, so the same bits.
+__global__ void k_up_concat8(const bf16_t* __restrict__ a, int Ca, const bf16_t* __restrict__ b,
+                             int Cb, bf16_t* __restrict__ out, int H, int W) {
+  const int C8 = (Ca + Cb) >> 3, Ca8 = Ca >> 3;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= W * C8) return;
+  const int row = blockIdx.y, n = row / H, y = row - n * H;
+  const int x = t / C8, c8 = t - x * C8;
+  const uint4* src;
+  if (c8 < Ca8)
+    src = reinterpret_cast<const uint4*>(
+              a + (((size_t)n * (H / 2) + y / 2) * (W / 2) + x / 2) * Ca) + c8;
+  else
+    src = reinterpret_cast<const uint4*>(b + ((size_t)row * W + x) * Cb) + (c8 - Ca8);
+  reinterpret_cast<uint4*>(out + ((size_t)row * W + x) * (Ca + Cb))[c8] = *src;
+}
+
 // backward of k_up_concat: da = sum over each 2x2 block, db = slice.
 __global__ void k_up_concat_bwd_a(const bf16_t* __restrict__ dout, int Ca, int Cb,
                                   bf16_t* __restrict__ da, int N, int H, int W) {
@@ -326,6 +460,14 @@ void launch_pool2_bwd(hipStream_t s, const bf16_t* dY, bf16_t* dX, int N, int H,
 }
 void launch_up_concat(hipStream_t s, const bf16_t* a, int Ca, const bf16_t* b, int Cb, bf16_t* out,
                       int N, int H, int W) {
+  const bool vec = Ca % 8 == 0 && Cb % 8 == 0 &&
+                   (((uintptr_t)a | (uintptr_t)b | (uintptr_t)out) & 15) == 0 &&
+                   (long)W * ((Ca + Cb) / 8) < (1L << 30) && (long)N * H < 65536;
+  if (vec) {
+    const int per_row = W * ((Ca + Cb) / 8);
+    k_up_concat8<<<dim3((per_row + 255) / 256, N * H), 256, 0, s>>>(a, Ca, b, Cb, out, H, W);
+    return;
+  }
   long n = (long)N * H * W * (Ca + Cb);
   k_up_concat<<<nb(n), 256, 0, s>>>(a, Ca, b, Cb, out, N, H, W);
 }
