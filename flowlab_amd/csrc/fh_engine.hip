@@ -830,8 +830,7 @@ int forces(fh_engine* e, double dt) {
 }
 
 int project(fh_engine* e, double grad_scale) {
-  launch_subtract_gradient(e->s, e->u, e->v, e->p, e->lbl, e->nx, e->ny, grad_scale);
-  launch_outflow(e->s, e->u, e->v, e->lbl, e->nx, e->ny);
+  launch_subtract_outflow(e->s, e->u, e->v, e->p, e->lbl, e->nx, e->ny, grad_scale);
   e->vel_epoch++;
   LAUNCHED("project");
   return FH_OK;

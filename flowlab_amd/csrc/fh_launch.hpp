@@ -50,6 +50,9 @@ void launch_divergence(hipStream_t s, const float* u, const float* v, double* di
 void launch_subtract_gradient(hipStream_t s, float* u, float* v, const double* p,
                               const uint8_t* lbl, int nx, int ny, double grad_scale);
 void launch_outflow(hipStream_t s, float* u, float* v, const uint8_t* lbl, int nx, int ny);
+// launch_subtract_gradient then launch_outflow, in one pass where nx, ny >= 3.
+void launch_subtract_outflow(hipStream_t s, float* u, float* v, const double* p,
+                             const uint8_t* lbl, int nx, int ny, double grad_scale);
 void launch_dye_patch(hipStream_t s, float* dye, const float* patch, int dnx, int dny, int i0,
                       int j0, int pnx, int pny);
 void launch_commit_labels(hipStream_t s, double* p, float* u, float* v, float* dye,

@@ -51,6 +51,47 @@ __global__ __launch_bounds__(256) void k_subtract(float* __restrict__ u, float* 
   subtract_at(u, v, p, lbl, nx, ny, gs, bx * BX + threadIdx.x, by * BY + threadIdx.y);
 }
 
+// Gradient subtraction with applyOutflow (boundaries.ts:115-143) after it in
+// the same pass.  Outflow reads only the face next to an Air edge cell and
+// writes that face and the edge face; the thread that subtracts at face 1
+// (nx-1) of row j applies the row's low (high) edge right after, likewise
+// face 1 (ny-1) of column i of v.  No other thread reads or writes those
+// faces, so the values are those of the two passes in turn.  Needs nx, ny
+// >= 3, where a row's two edges touch different faces.
+__global__ __launch_bounds__(256) void k_subtract_outflow(float* __restrict__ u,
+                                                          float* __restrict__ v,
+                                                          const double* __restrict__ p,
+                                                          const uint8_t* __restrict__ lbl, int nx,
+                                                          int ny, double gs) {
+  int bx, by;
+  xcd_block(&bx, &by);
+  const int i = bx * BX + threadIdx.x, j = by * BY + threadIdx.y, su = nx + 1;
+  if (i > nx || j > ny) return;
+  subtract_at(u, v, p, lbl, nx, ny, gs, i, j);
+  if (j < ny) {
+    if (i == 1 && lbl[j * nx] == AIR) {
+      const float o = fminf(u[1 + j * su], 0.f);
+      u[1 + j * su] = o;
+      u[j * su] = o;
+    } else if (i == nx - 1 && lbl[nx - 1 + j * nx] == AIR) {
+      const float o = fmaxf(u[nx - 1 + j * su], 0.f);
+      u[nx - 1 + j * su] = o;
+      u[nx + j * su] = o;
+    }
+  }
+  if (i < nx) {
+    if (j == 1 && lbl[i] == AIR) {
+      const float o = fminf(v[i + nx], 0.f);
+      v[i + nx] = o;
+      v[i] = o;
+    } else if (j == ny - 1 && lbl[i + (ny - 1) * nx] == AIR) {
+      const float o = fmaxf(v[i + (ny - 1) * nx], 0.f);
+      v[i + (ny - 1) * nx] = o;
+      v[i + ny * nx] = o;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_outflow(float* __restrict__ u, float* __restrict__ v,
                                                  const uint8_t* __restrict__ lbl, int nx, int ny) {
   outflow_at(u, v, lbl, nx, ny, blockIdx.x * blockDim.x + threadIdx.x);
@@ -165,6 +206,17 @@ void launch_divergence(hipStream_t s, const float* u, const float* v, double* di
 void launch_subtract_gradient(hipStream_t s, float* u, float* v, const double* p,
                               const uint8_t* lbl, int nx, int ny, double grad_scale) {
   k_subtract<<<grid2(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(u, v, p, lbl, nx, ny, grad_scale);
+}
+
+void launch_subtract_outflow(hipStream_t s, float* u, float* v, const double* p,
+                             const uint8_t* lbl, int nx, int ny, double grad_scale) {
+  if (nx < 3 || ny < 3) {
+    launch_subtract_gradient(s, u, v, p, lbl, nx, ny, grad_scale);
+    launch_outflow(s, u, v, lbl, nx, ny);
+    return;
+  }
+  k_subtract_outflow<<<grid2(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(u, v, p, lbl, nx, ny,
+                                                                     grad_scale);
 }
 
 void launch_outflow(hipStream_t s, float* u, float* v, const uint8_t* lbl, int nx, int ny) {
