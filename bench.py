@@ -89,6 +89,8 @@ def parse():
     ap.add_argument("--large-n", type=int, default=8192,
                     help="side of the large single-domain plume (0 = skip)")
     ap.add_argument("--large-steps", type=int, default=4)
+    ap.add_argument("--large-learned", type=int, default=1,
+                    help="also step the large domain from the learned warm start (0 = skip)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (the known-size "
                          "copy kernel that scales hardware traffic counters)")
@@ -289,6 +291,9 @@ def run_large(args):
                           "frac": gbs / HBM_PEAK_GBS}
     e.set_timing(False)
     K = args.large_steps
+    learned = run_large_learned(args, sim, scene, K) if args.large_learned else None
+    if learned:
+        learned["solves_per_s_vs_plain"] = sum(solve_ms) / K / learned["ms_per_solve"]
     return {
         "workload": f"thermal plume {n}x{n}, closed box, 200 circular obstacles, 16 buoyant "
                     f"sources, MacCormack, {SOLVER_NAMES[args.solver]}, tol 1e-6, "
@@ -302,6 +307,49 @@ def run_large(args):
         "vcycles_per_solve": sum(iters) / K,
         "max_relative_residual": res,
         "kernels": kernels,
+        "learned_warm_start": learned,
+    }
+
+
+def run_large_learned(args, sim, scene, K):
+    """The large domain's steps again from the same reset, every pressure
+    solved from the trained U-Net warm start (inference at the full 8192^2
+    resolution, then the same solver to 1e-6): K timed steps, then K more
+    with per-solve times and cycle counts."""
+    from flowlab_amd import learned as L
+    if not os.path.exists(L.WEIGHTS):
+        return None
+    n = args.large_n
+    u = L.UNet()
+    u.load(L.WEIGHTS)
+    e = sim.stepper.engine
+    sim.reset(scene)
+    e.set_warm_start(u)
+    for _ in range(2):
+        sim.step()
+    e.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        sim.step()
+    e.synchronize()
+    elapsed = time.perf_counter() - t0
+    e.set_timing(True)
+    solve_ms, iters, res = [], [], 0.0
+    for _ in range(K):
+        sim.step()
+        solve_ms.append(e.timings()["solve"])
+        iters.append(sim.iters)
+        res = max(res, float(e.solve_history()[-1]))
+    e.set_timing(False)
+    e.set_warm_start(None)
+    return {
+        "value": n * n * K / elapsed,
+        "unit": "cell-steps/s",
+        "ms_per_step": elapsed * 1e3 / K,
+        "ms_per_solve": sum(solve_ms) / K,
+        "vcycles_per_solve": sum(iters) / K,
+        "max_relative_residual": res,
+        "weights": os.path.relpath(L.WEIGHTS, ROOT),
     }
 
 
