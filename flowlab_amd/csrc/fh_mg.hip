@@ -162,9 +162,11 @@ __global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ o
 }
 
 // Reduction grids are capped at kMaxPartials blocks (grid-stride over rows
-// beyond that), so the one-block finalize reads at most kMaxPartials/1024
-// partials per thread, all in flight together.
-constexpr int kMaxPartials = 8192;
+// beyond that); the one-block finalize reads up to kMaxPartials/1024
+// partials per thread, 8 in flight at a time.  At 8192^2 the cap leaves each
+// thread 16 rows (a cap of 8192 left it 32: the defect update there took
+// 0.643 ms, now 0.615).
+constexpr int kMaxPartials = 32768;
 
 // Grid of the reduction passes (defect update, CG dots and update, norms):
 // 64-wide blocks along x, each thread walking rows with stride gy * BY.
@@ -440,22 +442,25 @@ __global__ __launch_bounds__(1024) void k_finalize_sum4(const double* __restrict
   if (!init && done && *done) return;
   __shared__ double sh[4][16];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  constexpr int K = kMaxPartials / 1024;
-  double2 lo[K], hi[K];
+  constexpr int K = 8;
   const double2* p2 = reinterpret_cast<const double2*>(partials);
-#pragma unroll
-  for (int k = 0; k < K; k++) {
-    int b = tid + k * 1024;
-    lo[k] = b < nblocks ? p2[2 * b] : make_double2(0.0, 0.0);
-    hi[k] = b < nblocks ? p2[2 * b + 1] : make_double2(0.0, 0.0);
-  }
+  const int chunks = (nblocks + 1023) / 1024;  // block-uniform
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int c = 0; c < chunks; c += K) {
+    double2 lo[K], hi[K];
 #pragma unroll
-  for (int k = 0; k < K; k++) {
-    acc[0] += lo[k].x;
-    acc[1] += lo[k].y;
-    acc[2] += hi[k].x;
-    acc[3] += hi[k].y;
+    for (int k = 0; k < K; k++) {
+      int b = tid + (c + k) * 1024;
+      lo[k] = b < nblocks ? p2[2 * b] : make_double2(0.0, 0.0);
+      hi[k] = b < nblocks ? p2[2 * b + 1] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      acc[0] += lo[k].x;
+      acc[1] += lo[k].y;
+      acc[2] += hi[k].x;
+      acc[3] += hi[k].y;
+    }
   }
 #pragma unroll
   for (int s = 0; s < 4; s++) {

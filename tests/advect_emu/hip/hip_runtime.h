@@ -31,5 +31,12 @@ template <class T>
 T __shfl_xor(T v, int, int) {
   return v;
 }
+// The device multiply uses the low 24 bits of each operand: an operand
+// outside the signed 24-bit range would give a different product there.
+inline int __mul24(int a, int b) {
+  if (a < -(1 << 23) || a >= (1 << 23) || b < -(1 << 23) || b >= (1 << 23))
+    throw std::runtime_error("__mul24 operand outside 24 bits");
+  return a * b;
+}
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }

@@ -18,11 +18,11 @@ def make(out: str) -> None:
     p = os.path.join(out, "fh_device.hpp")
     dev = open(p).read()
     subs = [
-        ("__device__ __forceinline__ float at(int i, int j) const { return l[(i - x0) + (j - y0) * w]; }",
+        ("__device__ __forceinline__ float at(int i, int j) const { return l[off(i, j)]; }",
          "__device__ __forceinline__ float at(int i, int j) const { emu_inwin(i - x0, j - y0, 0); "
-         "return l[(i - x0) + (j - y0) * w]; }"),
-        ("    lds_cfp r = l + (i - x0) + (j - y0) * w;",
-         "    emu_inwin(i - x0, j - y0, 1);\n    lds_cfp r = l + (i - x0) + (j - y0) * w;"),
+         "return l[off(i, j)]; }"),
+        ("    lds_cfp r = l + off(i, j);",
+         "    emu_inwin(i - x0, j - y0, 1);\n    lds_cfp r = l + off(i, j);"),
         ("    const float fl = floorf(off);\n    return {base + (int)fl, off - fl};",
          "    const float fl = floorf(off);\n    Axis r{base + (int)fl, off - fl};\n"
          "    emu_axis_check(r, clamped_axis(base, off, count));\n    return r;"),
