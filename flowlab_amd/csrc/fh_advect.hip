@@ -71,10 +71,26 @@ __global__ __launch_bounds__(256) void k_maccormack_dye(
 // in its window and the kernel reads LDS unchecked; otherwise (a long
 // backtrace, or a non-finite velocity) the whole workgroup reads global
 // memory exactly as the plain kernels do.  Same bits either way.
-constexpr int TX = 64, TY = 16, AH = 2;
-constexpr int WW = TX + 2 * AH + 1, WH = TY + 2 * AH + 1, WN = WW * WH;
-constexpr int kStage = (WN + 255) / 256;
+constexpr int TX = 64, AH = 2;
+constexpr int WW = TX + 2 * AH + 1;
 constexpr float kReach = 1.25f;
+
+// A tile of TX x TY outputs (TY a multiple of BY: each thread takes TY/BY
+// rows) and its staged window of WW x WH elements per field.
+template <int TY>
+struct Tile {
+  static_assert(TY % BY == 0, "rows per thread");
+  static constexpr int WH = TY + 2 * AH + 1, WN = WW * WH;
+  static constexpr int kStage = (WN + 255) / 256;
+};
+
+// Tile heights, measured per kernel at 8192^2 (us per launch for 8 / 16 /
+// 32 rows): the MacCormack passes stage 4 and 8 fields and run more
+// workgroups per CU on the smaller window (faces 664 / 690 / 815, dye
+// 783 / 860 / 1033); the forward passes stage 2 and 5 and gain from the
+// smaller halo share (faces 515 / 454 / 437, dye 520 / 496 / 579).
+constexpr int kTyAdvectFaces = 32, kTyMacCormackFaces = 8, kTyAdvectDye = 16,
+              kTyMacCormackDye = 8;
 
 struct Win {
   const float* __restrict__ p;
@@ -90,9 +106,10 @@ __device__ __forceinline__ float nan_max(float m, float x) {
 // Stages NF windows (all global loads first, then the LDS stores) and
 // returns this thread's nan_max over what it staged of windows 0 and 1 (the
 // velocity); elements outside a field are staged as 0.
-template <int NF>
+template <int NF, int WN>
 __device__ __forceinline__ float stage_windows(float (*lds)[WN], const Win (&f)[NF], int wx0,
                                                int wy0, int tid) {
+  constexpr int kStage = (WN + 255) / 256;
   float v[NF][kStage];
 #pragma unroll
   for (int r = 0; r < kStage; r++) {
@@ -130,7 +147,7 @@ __device__ __forceinline__ bool within_reach(float m, float c, int tid) {
   return fabsf(c) * vw <= kReach;
 }
 
-template <class F>
+template <class F, int WN>
 __device__ __forceinline__ F window_as(const float (*lds)[WN], int q, int wx0, int wy0) {
   F f;
   f.l = (lds_cfp)lds[q];
@@ -149,6 +166,7 @@ __device__ __forceinline__ F window_as(const float (*lds)[WN], int q, int wx0, i
 // (LFieldI) takes the same indices and weights.
 constexpr int kEdge = 3;
 
+template <int TY>
 __device__ __forceinline__ bool interior_tile(int x0, int y0, int nx, int ny) {
   return x0 >= kEdge && y0 >= kEdge && x0 + TX - 1 <= nx - 1 - kEdge &&
          y0 + TY - 1 <= ny - 1 - kEdge;
@@ -156,9 +174,11 @@ __device__ __forceinline__ bool interior_tile(int x0, int y0, int nx, int ny) {
 
 // advect.ts:72-112 with carrier == source (the only use: u and v advect
 // themselves), so two windows.
+template <int TY>
 __global__ __launch_bounds__(256) void k_advect_faces_lds(
     const float* __restrict__ uc, const float* __restrict__ vc, float* __restrict__ uo,
     float* __restrict__ vo, const uint8_t* __restrict__ lbl, int nx, int ny, float c) {
+  constexpr int WN = Tile<TY>::WN;
   __shared__ float lds[2][WN];
   int bx, by;
   xcd_block(&bx, &by);
@@ -174,7 +194,7 @@ __global__ __launch_bounds__(256) void k_advect_faces_lds(
       for (int r = 0; r < TY / BY; r++)
         advect_faces_t(u, v, u, v, uo, vo, lbl, nx, ny, c, i, j + r * BY);
     };
-    if (interior_tile(x0, y0, nx, ny))
+    if (interior_tile<TY>(x0, y0, nx, ny))
       run(LFieldI{});
     else
       run(LField{});
@@ -185,10 +205,12 @@ __global__ __launch_bounds__(256) void k_advect_faces_lds(
 }
 
 // advect.ts:177-210 (fused reverse pass and clamp, as k_maccormack_faces).
+template <int TY>
 __global__ __launch_bounds__(256) void k_maccormack_faces_lds(
     const float* __restrict__ un, const float* __restrict__ vn, const float* __restrict__ uh,
     const float* __restrict__ vh, float* __restrict__ uo, float* __restrict__ vo,
     const uint8_t* __restrict__ lbl, int nx, int ny, float c) {
+  constexpr int WN = Tile<TY>::WN;
   __shared__ float lds[4][WN];
   int bx, by;
   xcd_block(&bx, &by);
@@ -205,7 +227,7 @@ __global__ __launch_bounds__(256) void k_maccormack_faces_lds(
       for (int r = 0; r < TY / BY; r++)
         maccormack_faces_t(a, b, ah, bh, uo, vo, lbl, nx, ny, c, i, j + r * BY);
     };
-    if (interior_tile(x0, y0, nx, ny))
+    if (interior_tile<TY>(x0, y0, nx, ny))
       run(LFieldI{});
     else
       run(LField{});
@@ -218,10 +240,12 @@ __global__ __launch_bounds__(256) void k_maccormack_faces_lds(
 
 // advect.ts:241-270, dye on the velocity grid (scale 1): windows of u, v and
 // the 3 channels.
+template <int TY>
 __global__ __launch_bounds__(256) void k_advect_dye_lds(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ q,
     float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx, int ny, float c, float k0,
     float k1, float k2) {
+  constexpr int WN = Tile<TY>::WN;
   __shared__ float lds[5][WN];
   int bx, by;
   xcd_block(&bx, &by);
@@ -241,7 +265,7 @@ __global__ __launch_bounds__(256) void k_advect_dye_lds(
       for (int r = 0; r < TY / BY; r++)
         advect_dye_t(a, b, q0, q1, q2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i, j + r * BY);
     };
-    if (interior_tile(x0, y0, nx, ny))
+    if (interior_tile<TY>(x0, y0, nx, ny))
       run(LFieldI{});
     else
       run(LField{});
@@ -254,10 +278,12 @@ __global__ __launch_bounds__(256) void k_advect_dye_lds(
 
 // advect.ts:291-311, dye on the velocity grid: windows of u, v, the 3
 // channels of q^n and of the forward pass.
+template <int TY>
 __global__ __launch_bounds__(256) void k_maccormack_dye_lds(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ qn,
     const float* __restrict__ qh, float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx,
     int ny, float c, float k0, float k1, float k2) {
+  constexpr int WN = Tile<TY>::WN;
   __shared__ float lds[8][WN];
   int bx, by;
   xcd_block(&bx, &by);
@@ -280,7 +306,7 @@ __global__ __launch_bounds__(256) void k_maccormack_dye_lds(
         maccormack_dye_t(a, b, n0, n1, n2, h0, h1, h2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i,
                          j + r * BY);
     };
-    if (interior_tile(x0, y0, nx, ny))
+    if (interior_tile<TY>(x0, y0, nx, ny))
       run(LFieldI{});
     else
       run(LField{});
@@ -293,13 +319,16 @@ __global__ __launch_bounds__(256) void k_maccormack_dye_lds(
   }
 }
 
-static inline dim3 grid_t(int w, int h) { return dim3((w + TX - 1) / TX, (h + TY - 1) / TY); }
+template <int TY>
+static inline dim3 grid_t(int w, int h) {
+  return dim3((w + TX - 1) / TX, (h + TY - 1) / TY);
+}
 
 void launch_advect_faces(hipStream_t s, const float* uc, const float* vc, const float* us,
                          const float* vs, float* uo, float* vo, const uint8_t* lbl, int nx, int ny,
                          float c, bool lds) {
   if (lds && us == uc && vs == vc)
-    k_advect_faces_lds<<<grid_t(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(uc, vc, uo, vo, lbl, nx,
+    k_advect_faces_lds<kTyAdvectFaces><<<grid_t<kTyAdvectFaces>(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(uc, vc, uo, vo, lbl, nx,
                                                                        ny, c);
   else
     k_advect_faces<<<grid2(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(uc, vc, us, vs, uo, vo, lbl,
@@ -310,7 +339,7 @@ void launch_maccormack_faces(hipStream_t s, const float* un, const float* vn, co
                              const float* vh, float* uo, float* vo, const uint8_t* lbl, int nx,
                              int ny, float c, bool lds) {
   if (lds)
-    k_maccormack_faces_lds<<<grid_t(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(un, vn, uh, vh, uo,
+    k_maccormack_faces_lds<kTyMacCormackFaces><<<grid_t<kTyMacCormackFaces>(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(un, vn, uh, vh, uo,
                                                                            vo, lbl, nx, ny, c);
   else
     k_maccormack_faces<<<grid2(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(un, vn, uh, vh, uo, vo,
@@ -321,7 +350,7 @@ void launch_advect_dye(hipStream_t s, const float* u, const float* v, const floa
                        const uint8_t* lbl, int nx, int ny, int ds, float c, float k0, float k1,
                        float k2, bool lds) {
   if (lds && ds == 1)
-    k_advect_dye_lds<<<grid_t(nx, ny), dim3(BX, BY), 0, s>>>(u, v, q, qo, lbl, nx, ny, c, k0, k1,
+    k_advect_dye_lds<kTyAdvectDye><<<grid_t<kTyAdvectDye>(nx, ny), dim3(BX, BY), 0, s>>>(u, v, q, qo, lbl, nx, ny, c, k0, k1,
                                                              k2);
   else
     k_advect_dye<<<grid2(nx * ds, ny * ds), dim3(BX, BY), 0, s>>>(u, v, q, qo, lbl, nx, ny, ds,
@@ -332,7 +361,7 @@ void launch_maccormack_dye(hipStream_t s, const float* u, const float* v, const 
                            const float* qh, float* qo, const uint8_t* lbl, int nx, int ny, int ds,
                            float c, float k0, float k1, float k2, bool lds) {
   if (lds && ds == 1)
-    k_maccormack_dye_lds<<<grid_t(nx, ny), dim3(BX, BY), 0, s>>>(u, v, qn, qh, qo, lbl, nx, ny,
+    k_maccormack_dye_lds<kTyMacCormackDye><<<grid_t<kTyMacCormackDye>(nx, ny), dim3(BX, BY), 0, s>>>(u, v, qn, qh, qo, lbl, nx, ny,
                                                                  c, k0, k1, k2);
   else
     k_maccormack_dye<<<grid2(nx * ds, ny * ds), dim3(BX, BY), 0, s>>>(u, v, qn, qh, qo, lbl, nx,

@@ -47,10 +47,13 @@ void emu_launch(K k, dim3 g, dim3 b, A... a) {
     }
 }
 
-// The workgroup's reach test over its two staged velocity windows.
+// The workgroup's reach test over its two staged velocity windows of n
+// elements each; also sets the window height the LDS reads that follow are
+// checked against (each kernel has its own tile height).
 template <class L>
 bool emu_within_reach(L lds, float c, float reach, int n) {
   __syncthreads();
+  emu_win_h = n / emu_win_w;
   float vw = 0.f;
   for (int q = 0; q < 2; q++)
     for (int e = 0; e < n; e++) {

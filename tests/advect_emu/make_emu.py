@@ -33,13 +33,13 @@ def make(out: str) -> None:
         dev = dev.replace(a, b)
     open(p, "w").write(dev)
     src = open(os.path.join(CSRC, "fh_advect.hip")).read()
-    src = re.sub(r"(\w+)<<<(.*?), (dim3\([^)]*\)), 0, s>>>\(", r"emu_launch(\1, \2, \3, ", src,
-                 flags=re.S)
+    src = re.sub(r"(\w+(?:<\w+>)?)<<<(.*?), (dim3\([^)]*\)), 0, s>>>\(", r"emu_launch(\1, \2, \3, ",
+                 src, flags=re.S)
     n = src.count("within_reach(m, c, tid)")
     if n != 4:
         raise RuntimeError(f"fh_advect.hip changed: {n} reach tests")
     src = src.replace("within_reach(m, c, tid)", "emu_within_reach(lds, c, kReach, WN)")
-    src += "\nint emu_win_w = fh::WW, emu_win_h = fh::WH;\n"
+    src += "\nint emu_win_w = fh::WW, emu_win_h = 0;\n"
     open(os.path.join(out, "fh_advect_emu.cpp"), "w").write('#include "emu.hpp"\n' + src)
 
 
