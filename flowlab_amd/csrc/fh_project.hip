@@ -33,6 +33,15 @@ __global__ __launch_bounds__(256) void k_forces(float* __restrict__ u, float* __
             bx * BX + threadIdx.x, by * BY + threadIdx.y);
 }
 
+__global__ __launch_bounds__(256) void k_buoyancy(float* __restrict__ v,
+                                                  const uint8_t* __restrict__ lbl,
+                                                  const float* __restrict__ heat, int nx, int ny,
+                                                  int ds, float gain) {
+  int bx, by;
+  xcd_block(&bx, &by);
+  buoyancy_at(v, lbl, heat, nx, ny, ds, gain, bx * BX + threadIdx.x, by * BY + threadIdx.y);
+}
+
 __global__ __launch_bounds__(256) void k_divergence(const float* __restrict__ u,
                                                     const float* __restrict__ v,
                                                     double* __restrict__ div, int nx, int ny,
@@ -194,6 +203,10 @@ void launch_vorticity(hipStream_t s, const float* u, const float* v, float* w, i
 void launch_forces(hipStream_t s, float* u, float* v, const uint8_t* lbl, const float* heat,
                    const float* w, int nx, int ny, int ds, float h, float dt, float gain,
                    float eps, StirBox stir) {
+  if (!stir.active && eps == 0.f && gain != 0.f) {
+    k_buoyancy<<<grid2(nx, ny + 1), dim3(BX, BY), 0, s>>>(v, lbl, heat, nx, ny, ds, gain);
+    return;
+  }
   k_forces<<<grid2(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(u, v, lbl, heat, w, nx, ny, ds, h, dt,
                                                           gain, eps, stir);
 }

@@ -320,6 +320,25 @@ __device__ __forceinline__ void forces_at(float* __restrict__ u, float* __restri
   }
 }
 
+// forces_at with buoyancy alone (no stir, no confinement, gain != 0): u is
+// unchanged, so only the interior v faces are touched; every load of a face
+// (its two cell labels, v and the heat stencil) is issued before the solid
+// test, and the stored value is forces_at's.
+__device__ __forceinline__ void buoyancy_at(float* __restrict__ v, const uint8_t* __restrict__ lbl,
+                                            const float* __restrict__ heat, int nx, int ny,
+                                            int ds, float gain, int i, int j) {
+  if (i >= nx || j < 1 || j > ny - 1) return;
+  const int k = i + j * nx;
+  const uint8_t below = lbl[k - nx], above = lbl[k];
+  float val = v[k];
+  Axis ax = clamped_axis(i * ds, 0.5f * ds - 0.5f, nx * ds);
+  Axis ay = clamped_axis(j * ds, -0.5f, ny * ds);
+  Quad g = gather4(heat, nx * ds, ax, ay);
+  if (below == SOLID || above == SOLID) return;
+  val += gain * bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
+  v[k] = val;
+}
+
 // divergence.ts:17-25, evaluated in FP64 from the FP32 faces.
 __device__ __forceinline__ void divergence_at(const float* __restrict__ u,
                                               const float* __restrict__ v,

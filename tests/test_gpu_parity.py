@@ -213,6 +213,24 @@ def test_forces_match_oracle():
     assert rel_l2(gu, ou) < REL_L2 and rel_l2(gv, ov) < REL_L2
 
 
+@pytest.mark.parametrize("ds", [1, 2])
+def test_buoyancy_alone_matches_oracle(ds):
+    """Buoyancy without stir or confinement (its own kernel): u untouched bit
+    for bit, v against the oracle's buoyancy on the (refined) dye grid."""
+    nx, ny = 96, 64
+    e, g, u, v, p, lab, dye = _resident(nx, ny, ds)
+    dt, w = 0.01, 2.0
+    e.set_buoyancy(w)
+    e.phase_forces(dt)
+    gu, gv = e.read_velocity()
+    assert np.array_equal(gu, u)
+    ov = O.apply_buoyancy(nx, ny, g.h, v, dye[:nx * ny * ds * ds], lab, w, dt, dscale=ds)
+    dv_g, dv_o = gv - v, ov - v
+    assert np.abs(dv_o).max() > 1e-4
+    assert rel_l2(dv_g, dv_o) < 1e-5
+    assert rel_l2(gv, ov) < REL_L2
+
+
 def test_divergence_subtract_outflow_match_oracle():
     nx, ny = 96, 64
     e, g, u, v, p, lab, dye = _resident(nx, ny)
