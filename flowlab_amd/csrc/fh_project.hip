@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void k_subtract_outflow(float* __restrict__ u,
   xcd_block(&bx, &by);
   const int i = bx * BX + threadIdx.x, j = by * BY + threadIdx.y, su = nx + 1;
   if (i > nx || j > ny) return;
-  subtract_at(u, v, p, lbl, nx, ny, gs, i, j);
+  subtract_eager_at(u, v, p, lbl, nx, ny, gs, i, j);
   if (j < ny) {
     if (i == 1 && lbl[j * nx] == AIR) {
       const float o = fminf(u[1 + j * su], 0.f);

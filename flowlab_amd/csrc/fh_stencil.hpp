@@ -369,6 +369,27 @@ __device__ __forceinline__ void subtract_at(float* __restrict__ u, float* __rest
   }
 }
 
+// subtract_at for an index inside the face grid (i <= nx, j <= ny) of a grid
+// of at least 2 x 2 cells, with every load (the three cell labels and
+// pressures it can need, u and v) issued before the solid tests; the same
+// faces and values as subtract_at.
+__device__ __forceinline__ void subtract_eager_at(float* __restrict__ u, float* __restrict__ v,
+                                                  const double* __restrict__ p,
+                                                  const uint8_t* __restrict__ lbl, int nx, int ny,
+                                                  double gs, int i, int j) {
+  const bool uf = j < ny && i >= 1 && i <= nx - 1;
+  const bool vf = i < nx && j >= 1 && j <= ny - 1;
+  if (!uf && !vf) return;
+  const int c = min(i, nx - 1) + min(j, ny - 1) * nx;  // cell (i, j), in the domain here
+  const int cw = i >= 1 ? c - 1 : c, cs = j >= 1 ? c - nx : c;  // (i-1, j), (i, j-1)
+  const uint8_t l0 = lbl[c], lw = lbl[cw], ls = lbl[cs];
+  const double p0 = p[c], pw = p[cw], ps = p[cs];
+  const int ku = i + j * (nx + 1), kv = i + j * nx;
+  const float u0 = uf ? u[ku] : 0.f, v0 = vf ? v[kv] : 0.f;
+  if (uf && lw != SOLID && l0 != SOLID) u[ku] = (float)((double)u0 - gs * (p0 - pw));
+  if (vf && ls != SOLID && l0 != SOLID) v[kv] = (float)((double)v0 - gs * (p0 - ps));
+}
+
 // boundaries.ts:115-143: index t owns u row t and v column t, replaying the
 // reference's low-edge-then-high-edge order within each.
 __device__ __forceinline__ void outflow_at(float* __restrict__ u, float* __restrict__ v,
