@@ -317,8 +317,12 @@ void unet_pack(UNet& net, hipStream_t s) {
   net.packed = true;
 }
 
-hipError_t unet_resize(UNet& net, int N, int H, int W) {
-  if (net.N == N && net.H == H && net.W == W && !net.act.empty()) return hipSuccess;
+// Activations for (N, H, W); with train, also their gradients and the head's
+// gradient buffers (inference never touches them: at 8192^2 they would hold
+// about as much memory again as the activations).
+hipError_t unet_resize(UNet& net, int N, int H, int W, bool train) {
+  if (net.N == N && net.H == H && net.W == W && !net.act.empty() && (net.trainable || !train))
+    return hipSuccess;
   for (auto p : net.act)
     if (p) (void)hipFree(p);
   for (auto p : net.grad)
@@ -326,6 +330,10 @@ hipError_t unet_resize(UNet& net, int N, int H, int W) {
   if (net.out) (void)hipFree(net.out);
   if (net.dout) (void)hipFree(net.dout);
   if (net.dhead) (void)hipFree(net.dhead);
+  net.out = net.dout = nullptr;
+  net.dhead = nullptr;
+  net.N = net.H = net.W = 0;
+  net.trainable = false;
   net.act.assign(21, nullptr);
   net.grad.assign(21, nullptr);
   net.act_elems.assign(21, 0);
@@ -335,13 +343,14 @@ hipError_t unet_resize(UNet& net, int N, int H, int W) {
     size_t n = (size_t)N * (H >> lv) * (W >> lv) * kActSpec[a][1];
     net.act_elems[a] = n;
     if ((e = hipMalloc((void**)&net.act[a], n * 2 + 64))) return e;
-    if (a > 0 && (e = hipMalloc((void**)&net.grad[a], n * 2 + 64))) return e;
+    if (train && a > 0 && (e = hipMalloc((void**)&net.grad[a], n * 2 + 64))) return e;
   }
   size_t px = (size_t)N * H * W;
-  if ((e = hipMalloc((void**)&net.out, px * 4 + 64)) ||
-      (e = hipMalloc((void**)&net.dout, px * 4 + 64)) ||
-      (e = hipMalloc((void**)&net.dhead, px * 16 * 2 + 64)))
+  if ((e = hipMalloc((void**)&net.out, px * 4 + 64))) return e;
+  if (train && ((e = hipMalloc((void**)&net.dout, px * 4 + 64)) ||
+                (e = hipMalloc((void**)&net.dhead, px * 16 * 2 + 64))))
     return e;
+  net.trainable = train;
   net.N = N;
   net.H = H;
   net.W = W;
@@ -517,7 +526,7 @@ int engine_warm_start(fh_unet* u, hipStream_t s, const float* r32, const uint8_t
                       const uint8_t* info, double* p, int H, int W, float* stats2) {
   UNet& net = u->net;
   if ((H % 8) || (W % 8)) return FH_OK;  // three 2x poolings need multiples of 8
-  UCHK(unet_resize(net, 1, H, W));
+  UCHK(unet_resize(net, 1, H, W, false));
   long n = (long)H * W;
   UCHK(stats_into(s, r32, p, true, lbl, n, u->partials, stats2));
   float c = (float)(u->c0 * ((double)H * W / u->area0));
@@ -597,7 +606,7 @@ int fh_unet_get_scale(fh_unet* u, double* c0, double* area0) {
 
 int fh_unet_forward(fh_unet* u, int N, int H, int W, const float* in, float* out) {
   if (!u || !in || !out || N < 1 || H % 8 || W % 8 || H < 8 || W < 8) return FH_ERR_BAD_ARG;
-  UCHK(unet_resize(u->net, N, H, W));
+  UCHK(unet_resize(u->net, N, H, W, false));
   long n = (long)N * H * W * kUnetIn;
   float* tmp = nullptr;
   UCHK(hipMalloc((void**)&tmp, n * 4));
@@ -616,7 +625,7 @@ int fh_unet_backward(fh_unet* u, int N, int H, int W, const float* in, const flo
   if (!u || !in || !dout || !dparams || N < 1 || H % 8 || W % 8 || H < 8 || W < 8)
     return FH_ERR_BAD_ARG;
   UNet& net = u->net;
-  UCHK(unet_resize(net, N, H, W));
+  UCHK(unet_resize(net, N, H, W, true));
   long n = (long)N * H * W * kUnetIn, px = (long)N * H * W;
   float* tmp = nullptr;
   UCHK(hipMalloc((void**)&tmp, n * 4));
@@ -698,7 +707,7 @@ int fh_unet_train(fh_unet* u, int steps, int batch, double lr, unsigned long lon
   UNet& net = u->net;
   const int H = u->H, W = u->W;
   const long hw = (long)H * W, px = hw * batch;
-  UCHK(unet_resize(net, batch, H, W));
+  UCHK(unet_resize(net, batch, H, W, true));
   if (u->idx) (void)hipFree(u->idx);
   if (u->tgt) (void)hipFree(u->tgt);
   if (u->mask) (void)hipFree(u->mask);

@@ -58,6 +58,7 @@ struct UNet {
   bool packed = false;
   // activations for the current (N, H, W)
   int N = 0, H = 0, W = 0;
+  bool trainable = false;     // grad, dout and dhead allocated
   std::vector<bf16_t*> act;   // forward tensors
   std::vector<bf16_t*> grad;  // their gradients
   std::vector<size_t> act_elems;
