@@ -164,9 +164,9 @@ __global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ o
 // Reduction grids are capped at kMaxPartials blocks (grid-stride over rows
 // beyond that); the one-block finalize reads up to kMaxPartials/1024
 // partials per thread, 8 in flight at a time.  At 8192^2 the cap leaves each
-// thread 16 rows (a cap of 8192 left it 32: the defect update there took
-// 0.643 ms, now 0.615).
-constexpr int kMaxPartials = 32768;
+// thread 8 rows; caps of 8192 / 32768 left it 32 / 16, and the defect update
+// there took 0.643 / 0.615 ms against 0.59 now.
+constexpr int kMaxPartials = 65536;
 
 // Grid of the reduction passes (defect update, CG dots and update, norms):
 // 64-wide blocks along x, each thread walking rows with stride gy * BY.
