@@ -17,7 +17,7 @@ import ctypes as C
 import numpy as np
 
 from . import hip
-from .hip import check, dptr, fptr, u8ptr
+from .hip import _sized, check, dptr, fptr, u8ptr
 
 
 class HipEngine:
@@ -64,10 +64,14 @@ class HipEngine:
 
     # ---- state ----------------------------------------------------------
     def upload(self, u=None, v=None, p=None, label=None, dye=None):
-        check(self.lib.fh_upload(self._e, dptr(u), dptr(v), dptr(p), u8ptr(label), dptr(dye)))
+        check(self.lib.fh_upload(self._e, dptr(u, self.n_u, "u"), dptr(v, self.n_v, "v"),
+                                 dptr(p, self.n_c, "p"), u8ptr(label, self.n_c),
+                                 dptr(dye, 3 * self.n_d, "dye (3 planes)")))
 
     def set_labels(self, label: np.ndarray):
-        check(self.lib.fh_set_labels(self._e, u8ptr(label)))
+        if label is None:
+            raise TypeError("set_labels: label is required")
+        check(self.lib.fh_set_labels(self._e, u8ptr(label, self.n_c)))
 
     def set_dye_patch(self, patch):
         if patch is None:
@@ -75,7 +79,8 @@ class HipEngine:
         else:
             data = np.ascontiguousarray(patch.data, np.float32)
             check(self.lib.fh_set_dye_patch(self._e, patch.i0, patch.j0, patch.nx, patch.ny,
-                                            fptr(data)))
+                                            fptr(data, 4 * patch.nx * patch.ny,
+                                                 "dye patch (4 planes)")))
 
     def set_buoyancy(self, w: float):
         check(self.lib.fh_set_buoyancy(self._e, float(w)))
@@ -93,22 +98,22 @@ class HipEngine:
     def read_velocity(self, u=None, v=None):
         u = np.empty(self.n_u) if u is None else u
         v = np.empty(self.n_v) if v is None else v
-        check(self.lib.fh_read_velocity(self._e, dptr(u), dptr(v)))
+        check(self.lib.fh_read_velocity(self._e, dptr(u, self.n_u, "u"), dptr(v, self.n_v, "v")))
         return u, v
 
     def read_dye(self, dye=None):
         dye = np.empty(3 * self.n_d) if dye is None else dye
-        check(self.lib.fh_read_dye(self._e, dptr(dye)))
+        check(self.lib.fh_read_dye(self._e, dptr(dye, 3 * self.n_d, "dye (3 planes)")))
         return dye
 
     def read_pressure(self):
         p = np.empty(self.n_c)
-        check(self.lib.fh_read_pressure(self._e, dptr(p)))
+        check(self.lib.fh_read_pressure(self._e, dptr(p, self.n_c, "p")))
         return p
 
     def read_divergence(self):
         d = np.empty(self.n_c)
-        check(self.lib.fh_read_divergence(self._e, dptr(d)))
+        check(self.lib.fh_read_divergence(self._e, dptr(d, self.n_c, "div")))
         return d
 
     def max_face_speed(self) -> float:
@@ -302,9 +307,13 @@ class HipPressureSolver:
 
     def solve(self, g, p, div, label, scale, iterations, omega, tol) -> int:
         it = C.c_int()
+        n = self.engine.n_c
+        if p is None or div is None or label is None:
+            raise TypeError("HipPressureSolver.solve: p, div and label are required")
         check(self.engine.lib.fh_solve_pressure(self.engine._e, hip.SOLVERS[self.method],
-                                                dptr(p), dptr(div), u8ptr(label), scale,
-                                                int(iterations), omega, tol, C.byref(it)))
+                                                dptr(p, n, "p"), dptr(div, n, "div"),
+                                                u8ptr(label, n), scale, int(iterations), omega,
+                                                tol, C.byref(it)))
         return it.value
 
 
@@ -318,15 +327,34 @@ class HipAdvector:
         self.engine = HipEngine(nx, ny, dye_scale, h)
 
     def velocity(self, g, scheme, u_in, v_in, u_out, v_out, label, dt):
-        check(self.engine.lib.fh_advect_velocity(self.engine._e, hip.SCHEMES[scheme], dptr(u_in),
-                                                 dptr(v_in), dptr(u_out), dptr(v_out),
-                                                 u8ptr(label), dt))
+        E = self.engine
+        if any(a is None for a in (u_in, v_in, u_out, v_out, label)):
+            raise TypeError("HipAdvector.velocity: every field is required")
+        check(E.lib.fh_advect_velocity(E._e, hip.SCHEMES[scheme], dptr(u_in, E.n_u, "uIn"),
+                                       dptr(v_in, E.n_v, "vIn"), dptr(u_out, E.n_u, "uOut"),
+                                       dptr(v_out, E.n_v, "vOut"), u8ptr(label, E.n_c), dt))
 
     def dye(self, g, scheme, u, v, q_in, q_out, label, dt, dg=None):
-        n = self.engine.n_d
-        qi = np.concatenate([np.asarray(q, np.float64) for q in q_in])
-        qo = np.empty(3 * n)
-        check(self.engine.lib.fh_advect_dye(self.engine._e, hip.SCHEMES[scheme], dptr(u), dptr(v),
-                                            dptr(qi), dptr(qo), u8ptr(label), dt))
-        for c in range(len(q_out)):
-            q_out[c][:] = qo[c * n:(c + 1) * n]
+        """Advects len(q_in) independent channels (advector.ts:104-121), three
+        per device pass; dg, when given, must be this engine's dye grid."""
+        E = self.engine
+        n = E.n_d
+        if dg is not None and (dg.nx, dg.ny) != (E.dnx, E.dny):
+            raise ValueError(f"HipAdvector.dye: dye grid {dg.nx}x{dg.ny} is not this "
+                             f"engine's {E.dnx}x{E.dny} (dye scale {E.ds})")
+        if len(q_out) < len(q_in):
+            raise ValueError("HipAdvector.dye: fewer output channels than input channels")
+        if u is None or v is None or label is None:
+            raise TypeError("HipAdvector.dye: u, v and label are required")
+        for c0 in range(0, len(q_in), 3):
+            group = q_in[c0:c0 + 3]
+            qi = np.zeros(3 * n)
+            for c, q in enumerate(group):
+                qi[c * n:(c + 1) * n] = _sized(np.asarray(q, np.float64).ravel(), n,
+                                               f"dye channel {c0 + c}")
+            qo = np.empty(3 * n)
+            check(E.lib.fh_advect_dye(E._e, hip.SCHEMES[scheme], dptr(u, E.n_u, "u"),
+                                      dptr(v, E.n_v, "v"), dptr(qi, 3 * n), dptr(qo, 3 * n),
+                                      u8ptr(label, E.n_c), dt))
+            for c in range(len(group)):
+                q_out[c0 + c][:] = qo[c * n:(c + 1) * n]

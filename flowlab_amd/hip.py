@@ -172,21 +172,29 @@ def f64(a: np.ndarray) -> np.ndarray:
     return a
 
 
-def dptr(a):
-    return None if a is None else f64(a).ctypes.data_as(_dp)
+def _sized(a, n, what):
+    """The library reads or writes exactly n elements through the pointer:
+    a shorter (or longer) array is a caller error, raised before any call."""
+    if n is not None and a.size != n:
+        raise ValueError(f"{what}: expected {n} elements, got {a.size}")
+    return a
 
 
-def u8ptr(a):
+def dptr(a, n=None, what="float64 field"):
+    return None if a is None else _sized(f64(a), n, what).ctypes.data_as(_dp)
+
+
+def u8ptr(a, n=None, what="label"):
     if a is None:
         return None
-    if not (a.dtype == np.uint8 and a.flags.c_contiguous):
+    if not (isinstance(a, np.ndarray) and a.dtype == np.uint8 and a.flags.c_contiguous):
         raise TypeError("expected a C-contiguous uint8 label array")
-    return a.ctypes.data_as(_u8p)
+    return _sized(a, n, what).ctypes.data_as(_u8p)
 
 
-def fptr(a):
+def fptr(a, n=None, what="float32 array"):
     if a is None:
         return None
-    if not (a.dtype == np.float32 and a.flags.c_contiguous):
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float32 and a.flags.c_contiguous):
         raise TypeError("expected a C-contiguous float32 array")
-    return a.ctypes.data_as(_fp)
+    return _sized(a, n, what).ctypes.data_as(_fp)

@@ -363,21 +363,25 @@ def _disks(n_cells_x: int, n_cells_y: int, h: float, cx, cy, r, value):
     return out
 
 
-def large_plume(n: int = 8192, seed: int = 20240613, sources: int = 16, obstacles: int = 200):
+def large_plume(n: int = 8192, seed: int = 20240613, sources: int = 16, obstacles: int = 200,
+                cell_scale: float = 1.0):
     """Closed n x n box (walls all round) with `obstacles` seeded circular
     solids (radius U[8, 48] cells) and `sources` buoyant heat disks (radius
     U[16, 64] cells, T ~ U[0.5, 1] in dye channel 0, smoke 1 in channel 1)
     held by one full-domain dye patch; all draws from SplitMix64(seed).
     Sources are placed in the lower half, obstacles anywhere; a source cell
-    that is also Solid stays Solid (commitLabels clears its dye)."""
+    that is also Solid stays Solid (commitLabels clears its dye).
+    cell_scale multiplies every radius in cells: the same draws at
+    cell_scale = n / 8192 give the 8192^2 scene's geometry resolved on an
+    n x n grid."""
     g = Grid(n, n, 1.0 / n)
     rng = SplitMix64(seed)
     ocx = [rng.uniform(0, 1) for _ in range(obstacles)]
     ocy = [rng.uniform(0, 1) for _ in range(obstacles)]
-    orr = [rng.uniform(8, 48) * g.h for _ in range(obstacles)]
+    orr = [rng.uniform(8, 48) * cell_scale * g.h for _ in range(obstacles)]
     scx = [rng.uniform(0.05, 0.95) for _ in range(sources)]
     scy = [rng.uniform(0.05, 0.5) for _ in range(sources)]
-    srr = [rng.uniform(16, 64) * g.h for _ in range(sources)]
+    srr = [rng.uniform(16, 64) * cell_scale * g.h for _ in range(sources)]
     sT = [rng.uniform(0.5, 1.0) for _ in range(sources)]
 
     def labels(gg: Grid, label: np.ndarray):
