@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libflowhip.so")
 ARCH = os.environ.get("FLOWHIP_ARCH", "gfx950")
 
 SOURCES = ["fh_advect.hip", "fh_project.hip", "fh_mg.hip", "fh_mgfused.hip", "fh_engine.hip",
-           "fh_unet_kernels.hip", "fh_unet.hip", "fh_batch.hip"]
+           "fh_unet_kernels.hip", "fh_unet.hip", "fh_batch.hip", "fh_noalloc.hip"]
 
 HIPCC_FLAGS = [
     "-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",

@@ -71,6 +71,9 @@ SIGNATURES = [
     ("fh_set_coarse_group", _i, [_vp, _i]),
     ("fh_set_advect_staging", _i, [_vp, _i]),
     ("fh_coarse_group_threads", _i, [_vp]),
+    ("fh_noalloc_last_error", C.c_char_p, []),
+    ("fh_noalloc_max_cells", C.c_longlong, []),
+    ("fh_solve_pressure_rbsor_noalloc", _i, [_vp, _vp, _vp, _vp, _i, _i, _d, _i, _d, _d, _ip]),
     ("fh_unet_create", _vp, [C.c_ulonglong]),
     ("fh_unet_destroy", None, [_vp]),
     ("fh_unet_last_error", C.c_char_p, []),

@@ -203,6 +203,27 @@ int fh_coarse_group_threads(fh_engine* e);
  * straight from global memory.  Results are the same bits either way. */
 int fh_set_advect_staging(fh_engine* e, int on);
 
+/* ---- allocation-free pressure solve (no reference counterpart) ----------
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * red-black SOR solve of fh_solve_pressure (FH_SOLVER_RBSOR, the same kernels
+ * and the same bits) on caller-owned DEVICE arrays -- p (FP64, warm start in,
+ * answer out), div (FP64), label (u8), nx*ny each -- on the caller's stream
+ * (a hipStream_t; NULL = the null stream).  Its scratch (FP32 iterate and
+ * right-hand side, norm partials) is one fixed device array of ~10 MB per
+ * GPU, held until the process ends; nothing is allocated or freed by the
+ * call.  Limits: nx*ny <= fh_noalloc_max_cells() (2^20) and each side
+ * <= 65536, else FH_ERR_BAD_ARG (fh_noalloc_last_error says why).  Every call
+ * on one GPU shares that GPU's scratch: calls that overlap on one GPU (other
+ * host threads, other streams) race.  The call reads the norms on the host
+ * (the zero-divergence guard, and every 12 sweeps when tol > 0), so it
+ * synchronises its stream and returns with p written. */
+const char* fh_noalloc_last_error(void);
+long long fh_noalloc_max_cells(void);
+int fh_solve_pressure_rbsor_noalloc(void* stream, double* p, const double* div,
+                                    const uint8_t* label, int nx, int ny, double scale,
+                                    int iterations, double omega, double tol, int* out_iters);
+
 /* ---- learned warm start (no reference counterpart) ---------------------
  * A 4-level convolutional U-Net (3x3 convs, channels 16-32-64-64, BF16
  * weights and activations, FP32 accumulation on MFMA) predicts a correction
