@@ -1,18 +1,28 @@
 """Benchmark of the projection-method step on one MI355X.
 
-Workload: the Karman vortex street on a 2048x1024 MAC grid (inflow u=1 with a
-seeded 1e-3 inlet perturbation, Air outlet column, free-slip walls, cylinder
-of radius 0.05H at (0.25L, 0.5H)), MacCormack advection of velocity and 3 dye
-channels, mixed-precision geometric multigrid V(2,2) with red-black
-Gauss-Seidel down to 16x8 and an FP64 direct coarsest solve, every pressure
-solve to 1e-6 relative residual, dt at CFL 0.5.
+Headline workload: the 8192x8192 thermal plume (67 M cells; closed box, 200
+seeded circular obstacles, 16 buoyant heat sources, buoyancy 1, MacCormack
+advection of velocity and 3 dye channels, dt = 0.5 h capped at CFL 0.5,
+every pressure solve to 1e-6 relative residual by one mixed-precision
+multigrid V(2,2) cycle to 16x8 followed by flexible CG preconditioned by it),
+200 steps from rest.
 
-A step is one fused fh_step over the resident fields (inputs already in HBM);
-the timed region is K steps bracketed by device synchronisation.  Prints one
-JSON line.  `roofline` is measured live: the dominant kernel's launches are
-bracketed by HIP events on the engine's own stream during the timed steps.
-`cpu_baseline` times the FP64 C oracle (one thread) on a bounded sample of the
-same workload.
+A step is one fused fh_step over the resident fields (inputs already in
+HBM); the timed region is `--warmup` untimed steps after a reset, then
+exactly `--steps` steps bracketed by device synchronisation.  The whole
+200-step workload is also run from its reset with per-step phase timing
+(`workload_run`: solve times, V-cycles and residuals over every step).
+
+`roofline` is the dominant kernel (the largest device time per step at
+8192^2) measured live: each of its launches in the timed steps is bracketed
+by HIP events on the engine's stream; achieved = algorithmic bytes per launch
+/ mean bracketed duration (nothing subtracted).  `traffic` comes from the
+committed FETCH_SIZE/WRITE_SIZE passes at this size.  `legs` run the other
+configurations at their stated step counts: the 128^2 plume on red-black SOR
+(C1), the 2048x1024 Karman street (C2), the 64-scene decaying-turbulence
+batch (C3) and the learned warm-start evaluation at 1024^2 and 2048^2 (C4).
+`cpu_baseline` times the FP64 C oracle (one thread) on one step of the
+headline workload, continued from the device's state after its 200 steps.
 """
 from __future__ import annotations
 
@@ -27,193 +37,311 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell-steps/s + ms to 1e-6 pressure residual, learned vs plain MG, 1 GPU"
 HBM_PEAK_GBS = 8000.0
+WORKLOAD_STEPS = 200
 
 # Algorithmic (compulsory) bytes per cell of one launch of each traceable
 # kernel: every array element read or written once, neighbour reuse perfect.
-#   rb_sweep_fine: one colour half-sweep = half of a full red-black sweep's
-#     13 B/cell (x 4 read + 4 write, rhs 4, label 1).
-#   maccormack_faces: u^n, v^n 8 + forward-pass u, v 8 + out u, v 8 + label 1
-#     (u^n is both the carrier and the limiter's stencil: read once).
+#   maccormack_faces: u^n, v^n 8 + forward-pass u, v 8 + out u, v 8 + label 1.
 #   advect_faces: carrier = source u, v 8 + out u, v 8 + label 1.
+#   maccormack_dye: u, v 8 + 3 channels of q^n and of the forward pass 24 +
+#     3 out 12 + label 1; advect_dye: u, v 8 + 3 in 12 + 3 out 12 + label 1.
 #   dc_update: e 4 + p 8 read + 8 write + div 8 + label 1 + r32 4; under the
-#     flexible-CG solver the traced launch (the solve's first cycle) also
-#     writes the FP64 residual r, 8 more (algo_bytes below).
-#   smooth_restrict_fine: b 4 + label 1 + x 4 written + coarse b 1 (x starts
-#     at zero, so it is not read).
+#     flexible-CG solver the traced launch also writes the FP64 residual (+8).
+#   smooth_restrict_fine: b 4 + label 1 + x 4 written + coarse b 1.
 #   prolong_smooth_fine: x 4 + b 4 + label 1 + coarse e 1 + x 4 written.
 #   fcg_dots: z 4 + previous direction 4 + r 8 + info 1.
 #   fcg_update: z 4 + previous direction 4 + p 8 + r 8 + info 1 read;
 #     direction 4 + p 8 + r 8 + r32 4 written.
+#   divergence: u, v 8 read + FP64 div 8 written.
+#   forces (buoyancy alone): v 4 read + 4 written + heat 4 + label 1.
+#   project (gradient subtraction + outflow): u, v 8 read + 8 written +
+#     FP64 p 8 + label 1.
+#   rb_sweep_fine (one colour): half of a sweep's 13 (x 4 + 4, rhs 4, label 1).
+ALGO_BYTES_PER_CELL = {
+    "maccormack_faces": 25.0, "advect_faces": 17.0, "maccormack_dye": 45.0,
+    "advect_dye": 33.0, "dc_update": 33.0, "smooth_restrict_fine": 10.0,
+    "prolong_smooth_fine": 14.0, "fcg_dots": 17.0, "fcg_update": 49.0, "divergence": 16.0,
+    "forces": 13.0, "project": 25.0, "rb_sweep_fine": 6.5,
+}
+LARGE_KERNELS = ("maccormack_dye", "maccormack_faces", "advect_dye", "advect_faces", "dc_update",
+                 "prolong_smooth_fine", "smooth_restrict_fine", "project", "divergence",
+                 "forces", "fcg_dots", "fcg_update")
 SOLVER_NAMES = {
     "mg-dc": "MG-DC V(2,2) to 16x8",
     "mg-fcg": "one MG-DC V(2,2) cycle (to 16x8), then flexible CG preconditioned by it",
 }
+PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_plume_8192.json")
+
 
 def algo_bytes(kernel, solver):
     """Algorithmic bytes per cell of one traced launch of `kernel`."""
+    b = ALGO_BYTES_PER_CELL.get(kernel)
     if kernel == "dc_update" and solver == "mg-fcg":
-        return ALGO_BYTES_PER_CELL[kernel] + 8.0
-    return ALGO_BYTES_PER_CELL.get(kernel)
-
-
-ALGO_BYTES_PER_CELL = {
-    "smooth_restrict_fine": 10.0,
-    "prolong_smooth_fine": 14.0,
-    "rb_sweep_fine": 6.5,
-    "maccormack_faces": 25.0,
-    "advect_faces": 17.0,
-    "maccormack_dye": 45.0,
-    "advect_dye": 33.0,
-    "dc_update": 33.0,
-    "fcg_dots": 17.0,
-    "fcg_update": 49.0,
-}
+        b += 8.0
+    return b
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--nx", type=int, default=2048)
-    ap.add_argument("--ny", type=int, default=1024)
+    ap.add_argument("--steps", type=int, default=WORKLOAD_STEPS,
+                    help="timed steps of the headline workload")
+    ap.add_argument("--warmup", type=int, default=0,
+                    help="untimed steps after the reset, before the timed ones")
+    ap.add_argument("--n", type=int, default=8192, help="side of the headline plume")
+    ap.add_argument("--solver", default="mg-fcg", choices=sorted(SOLVER_NAMES))
     ap.add_argument("--kernel", default=None,
-                    help="kernel traced for the roofline (default: the solver's fine-level "
-                         "update, dc_update or fcg_update)")
-    ap.add_argument("--solver", default="mg-fcg", choices=sorted(SOLVER_NAMES),
-                    help="pressure solver of the GPU legs (tolerance 1e-6 either way)")
-    ap.add_argument("--cpu-steps", type=int, default=4, help="oracle steps timed (0 = skip)")
-    ap.add_argument("--batch-scenes", type=int, default=64,
-                    help="scenes of the batched decaying-turbulence run (0 = skip)")
-    ap.add_argument("--batch-steps", type=int, default=10)
-    ap.add_argument("--large-n", type=int, default=8192,
-                    help="side of the large single-domain plume (0 = skip)")
-    ap.add_argument("--large-steps", type=int, default=4)
-    ap.add_argument("--large-learned", type=int, default=1,
-                    help="also step the large domain from the learned warm start (0 = skip)")
+                    help="kernel traced for the roofline (default: the largest device time "
+                         "per step)")
+    ap.add_argument("--legs", default="c1,c2,c3,c4,learned",
+                    help="comma list of extra legs: c1, c2, c3, c4, learned (none: '')")
+    ap.add_argument("--c2-steps", type=int, default=1000)
+    ap.add_argument("--c3-steps", type=int, default=500)
+    ap.add_argument("--c1-steps", type=int, default=200)
+    ap.add_argument("--cpu-baseline", type=int, default=1, help="0 = skip the oracle timing")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (the known-size "
                          "copy kernel that scales hardware traffic counters)")
-    args = ap.parse_args()
-    if args.kernel is None:
-        args.kernel = "fcg_update" if args.solver == "mg-fcg" else "dc_update"
-    return args
+    return ap.parse_args()
 
 
-PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_karman_2048x1024.json")
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel, nx, ny):
+def measured_traffic(kernel, n):
     """HBM bytes per launch of `kernel` from the committed counter passes
-    (FETCH_SIZE and WRITE_SIZE, scaled by a known-size copy), for the
-    2048x1024 workload they were taken on; None otherwise."""
-    if (nx, ny) != (2048, 1024) or not os.path.exists(PMC_TRAFFIC):
+    (FETCH_SIZE and WRITE_SIZE, scaled by a known-size copy) at this size."""
+    if not os.path.exists(PMC_TRAFFIC):
         return None
     with open(PMC_TRAFFIC) as f:
-        k = json.load(f)["kernels"].get(kernel)
+        d = json.load(f)
+    if d.get("grid") != [n, n]:
+        return None
+    k = d["kernels"].get(kernel)
     return None if k is None else k["traffic_bytes"]
 
 
-def run_gpu(args, unet=None, trace=True, tiles=None):
-    """The Karman workload: W warm-up steps, then K timed steps.  unet: solve
-    every pressure from the learned warm start instead of the previous p."""
-    from flowlab_amd import scenes as S
-    from flowlab_amd.simulation import Simulation
+# ---- single-grid legs ---------------------------------------------------------
 
-    nx, ny = args.nx, args.ny
-    g, scene = S.karman_street(nx, ny)
-    params = dict(scheme="macCormack", cfl_target=0.5, solver=args.solver, pressure_iters=100,
-                  tol=1e-6, residual_every=0)
+def make_sim(nx, ny, scene, params):
+    from flowlab_amd.simulation import Simulation
     sim = Simulation(nx, ny, params, readback=False)
     sim.reset(scene)
+    return sim
+
+
+def instrumented(sim, scene, steps, warmup=0):
+    """`steps` steps after a reset (and `warmup` more first) with per-step
+    phase timing: solve ms, V-cycles and the final residual of each solve."""
     e = sim.stepper.engine
-    if unet is not None:
-        e.set_warm_start(unet)
+    sim.reset(scene)
+    sim.stepper.upload(sim.f)
+    for _ in range(warmup):
+        sim.step()
+    e.set_timing(True)
+    solve_ms, iters, res, phases = [], [], 0.0, {}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+        ph = e.timings()
+        solve_ms.append(ph["solve"])
+        for k, v in ph.items():
+            phases[k] = phases.get(k, 0.0) + v
+        iters.append(sim.iters)
+        h = e.solve_history()
+        if len(h):
+            res = max(res, float(h[-1]))
+    e.synchronize()
+    elapsed = time.perf_counter() - t0
+    e.set_timing(False)
+    return dict(elapsed=elapsed, solve_ms=solve_ms, iters=iters, max_res=res,
+                phases={k: v / steps for k, v in phases.items()})
+
+
+def kernel_table(sim, cells, solver, steps=2):
+    """Each kernel of LARGE_KERNELS bracketed by events over `steps` steps:
+    mean launch ms, launches per step and algorithmic GB/s."""
+    e = sim.stepper.engine
+    out = {}
+    for k in LARGE_KERNELS:
+        e.trace_kernel(k)
+        for _ in range(steps):
+            sim.step()
+        n, ms = e.trace_stats()
+        e.trace_kernel(None)
+        if not n:
+            continue
+        avg = ms / n
+        gbs = algo_bytes(k, solver) * cells / (avg * 1e-3) / 1e9
+        out[k] = {"avg_launch_ms": avg, "launches_per_step": n / steps,
+                  "ms_per_step": ms / steps, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
+                  "algorithmic_bytes_per_cell": algo_bytes(k, solver)}
+    return out
+
+
+def run_headline(args):
+    from flowlab_amd import scenes as S
+    n = args.n
+    log(f"headline: plume {n}^2 scene")
+    g, scene = S.large_plume(n)
+    params = dict(scheme="macCormack", cfl_target=0.5, dt_max=0.5 * g.h, solver=args.solver,
+                  pressure_iters=100, tol=1e-6, residual_every=0, buoyancy=1.0)
+    sim = make_sim(n, n, scene, params)
+    e = sim.stepper.engine
+    sim.step()  # uploads the fields and builds the multigrid hierarchy
+    tiles = e.tune_tiles(reps=5)["tiles"]
+    log(f"headline: tiles {tiles}; per-kernel table")
+    kernels = kernel_table(sim, n * n, args.solver)
+    dominant = args.kernel or max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    # timed pass: reset and upload (the fields are resident before the
+    # clock starts), W untimed steps, K timed steps with the dominant
+    # kernel's launches bracketed on the engine stream
+    sim.reset(scene)
+    sim.stepper.upload(sim.f)
     for _ in range(args.warmup):
         sim.step()
-    if tiles is None:
-        tiles = e.tune_tiles(reps=10)["tiles"]  # autotuned tile shape per level
-    else:
-        for level, name in enumerate(tiles):
-            e.set_tile(name, level)
-    # Timed pass: K steps with nothing else in the loop (per-step phase
-    # timing would make the host wait for every step's end).
+    e.trace_kernel(dominant)
     e.synchronize()
+    log(f"headline: timed pass ({args.warmup} + {args.steps} steps), tracing {dominant}")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         sim.step()
     e.synchronize()
     elapsed = time.perf_counter() - t0
-    # Instrumented pass over the same K steps (the run is deterministic:
-    # reset, the same warm-up, then K steps with solve times, cycle counts,
-    # residuals and the roofline kernel's launch times recorded).
-    sim.reset(scene)
-    for _ in range(args.warmup):
-        sim.step()
-    e.set_timing(True)
-    if trace:
-        e.trace_kernel(args.kernel)
-    solve_ms, iters, hist_max = [], [], 0.0
-    for _ in range(args.steps):
-        sim.step()
-        ph = e.timings()
-        solve_ms.append(ph["solve"])
-        iters.append(sim.iters)
-        hist_max = max(hist_max, float(e.solve_history()[-1]))
-    e.synchronize()
-    n_launch, k_ms, bracket_ms = 0, 0.0, 0.0
-    if trace:
-        n_launch, k_ms = e.trace_stats()
-        bracket_ms = e.trace_overhead(200)
-        e.trace_kernel(None)
-    e.set_timing(False)
-    e.set_warm_start(None)
-    return dict(sim=sim, elapsed=elapsed, solve_ms=solve_ms, iters=iters, hist_max=hist_max,
-                n_launch=n_launch, k_ms=k_ms, bracket_ms=bracket_ms, g=g, tiles=tiles)
+    n_launch, k_ms = e.trace_stats()
+    e.trace_kernel(None)
+    log("headline: the 200-step workload, instrumented")
+    wl = instrumented(sim, scene, WORKLOAD_STEPS)
+    state = None
+    if args.cpu_baseline:
+        u, v = e.read_velocity()
+        state = dict(u=u, v=v, p=e.read_pressure(), dye=e.read_dye(), label=sim.f.label.copy(),
+                     g=g, scene=scene)
+    learned = None
+    if "learned" in args.legs:
+        learned = run_learned(sim, scene, WORKLOAD_STEPS, wl)
+    sim.stepper.engine.close()
+    return dict(n=n, elapsed=elapsed, steps=args.steps, tiles=tiles, kernels=kernels,
+                dominant=dominant, n_launch=n_launch, k_ms=k_ms, workload=wl,
+                learned=learned, state=state)
 
 
-def run_learned(args, plain):
-    """The same workload with the trained U-Net warm start (weights in
-    flowlab_amd/weights), against the plain run's solves."""
+def run_learned(sim, scene, steps, plain):
+    """The same steps from the same reset, every pressure solved from the
+    trained U-Net warm start (then the same solver to 1e-6)."""
     from flowlab_amd import learned as L
     if not os.path.exists(L.WEIGHTS):
         return None
     u = L.UNet()
     u.load(L.WEIGHTS)
-    r = run_gpu(args, unet=u, trace=False, tiles=plain["tiles"])
-    K = args.steps
-    ms_l = sum(r["solve_ms"]) / K
-    ms_p = sum(plain["solve_ms"]) / K
-    return {
-        "ms_per_solve": ms_l,
-        "vcycles_per_solve": sum(r["iters"]) / K,
-        "max_relative_residual": r["hist_max"],
-        "solves_per_s_vs_plain": ms_p / ms_l,
-        "cell_steps_per_s": args.nx * args.ny * K / r["elapsed"],
-        "weights": os.path.relpath(L.WEIGHTS, ROOT),
-    }
+    e = sim.stepper.engine
+    e.set_warm_start(u)
+    r = instrumented(sim, scene, steps)
+    e.set_warm_start(None)
+    cells = sim.g.nx * sim.g.ny
+    ms_l = sum(r["solve_ms"]) / steps
+    ms_p = sum(plain["solve_ms"]) / steps
+    return {"steps": steps, "ms_per_solve": ms_l, "vcycles_per_solve": sum(r["iters"]) / steps,
+            "max_relative_residual": r["max_res"], "solves_per_s_vs_plain": ms_p / ms_l,
+            "cell_steps_per_s_with_phase_timing": cells * steps / r["elapsed"],
+            "weights": os.path.relpath(L.WEIGHTS, ROOT)}
 
 
-def run_batched(args):
-    """64 decaying-turbulence scenes of uneven size (256^2 .. 1024^2) in one
-    batch: one engine per scene on its own stream, stepped from a host thread
-    pool so the scenes' kernels interleave; tiles autotuned per scene; dt per
-    scene at CFL 0.5; pressure by mg-fcg to 1e-6."""
+def leg_summary(cells, steps, elapsed, wl, tiles=None):
+    K = len(wl["iters"])
+    out = {"cells": cells, "steps": steps, "value": cells * steps / elapsed,
+           "unit": "cell-steps/s", "ms_per_step": elapsed * 1e3 / steps,
+           "ms_per_solve": sum(wl["solve_ms"]) / K, "vcycles_per_solve": sum(wl["iters"]) / K,
+           "max_relative_residual": wl["max_res"],
+           "phase_ms_per_step": {k: round(v, 4) for k, v in wl["phases"].items()}}
+    if tiles is not None:
+        out["mg_tiles"] = tiles
+    return out
+
+
+def run_c2(args):
+    """Karman vortex street 2048x1024 at its 1000 steps (cache-resident: the
+    working set is below the 256 MB Infinity Cache)."""
+    from flowlab_amd import scenes as S
+    nx, ny, K = 2048, 1024, args.c2_steps
+    g, scene = S.karman_street(nx, ny)
+    params = dict(scheme="macCormack", cfl_target=0.5, solver=args.solver, pressure_iters=100,
+                  tol=1e-6, residual_every=0)
+    sim = make_sim(nx, ny, scene, params)
+    e = sim.stepper.engine
+    sim.step()
+    tiles = e.tune_tiles(reps=10)["tiles"]
+    kernels = kernel_table(sim, nx * ny, args.solver)
+    sim.reset(scene)
+    sim.stepper.upload(sim.f)
+    e.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        sim.step()
+    e.synchronize()
+    elapsed = time.perf_counter() - t0
+    wl = instrumented(sim, scene, K)
+    out = leg_summary(nx * ny, K, elapsed, wl, tiles)
+    out["workload"] = ("karman vortex street 2048x1024, MacCormack, "
+                       f"{SOLVER_NAMES[args.solver]}, tol 1e-6, CFL 0.5, {K} steps")
+    out["kernels"] = kernels
+    out["note"] = "working set inside the 256 MB Infinity Cache: GB/s are cache-effective"
+    if "learned" in args.legs:
+        out["learned_warm_start"] = run_learned(sim, scene, K, wl)
+    e.close()
+    return out
+
+
+def run_c1(args):
+    """128^2 closed-box thermal plume: MacCormack, buoyancy 1, vorticity
+    confinement 0.1, red-black SOR omega 1.9 x 200 sweeps, dt = 0.5 h, 200
+    steps."""
+    from flowlab_amd import scenes as S
+    g, scene = S.thermal_plume_128()
+    K = args.c1_steps
+    params = dict(scheme="macCormack", cfl_target=1e30, dt_max=0.5 * g.h, solver="rbsor",
+                  pressure_iters=200, omega=1.9, tol=0.0, buoyancy=1.0, confinement=0.1,
+                  residual_every=0)
+    sim = make_sim(128, 128, scene, params)
+    e = sim.stepper.engine
+    for _ in range(3):
+        sim.step()
+    sim.reset(scene)
+    sim.stepper.upload(sim.f)
+    e.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        sim.step()
+    e.synchronize()
+    elapsed = time.perf_counter() - t0
+    wl = instrumented(sim, scene, K)
+    out = leg_summary(128 * 128, K, elapsed, wl)
+    out["workload"] = ("thermal plume 128x128, MacCormack, buoyancy 1, confinement 0.1, "
+                       f"red-black SOR omega 1.9 x 200, dt 0.5 h, {K} steps")
+    e.close()
+    return out
+
+
+def run_c3(args):
+    """64 decaying-turbulence scenes of uneven size (256^2 .. 1024^2), one
+    engine and stream per scene stepped from a host thread pool; dt per scene
+    at CFL 0.5; pressure by mg-fcg to 1e-6; 500 steps."""
     from flowlab_amd import scenes as S
     from flowlab_amd.batch import HipBatch
-
-    scenes = S.turbulence_batch(args.batch_scenes)
+    scenes = S.turbulence_batch(64)
     b = HipBatch([g for g, _ in scenes])
     for s, (_, sc) in enumerate(scenes):
         b.reset(s, sc)
     kw = dict(scheme="macCormack", cfl=0.5, dt_max=1.0, tol=1e-6, max_cycles=100)
     b.step(**kw)
     tune = b.tune(reps=5)
-    for _ in range(max(args.warmup - 1, 1)):
-        b.step(**kw)
-    iters, res_max = [], 0.0
+    for s, (_, sc) in enumerate(scenes):
+        b.reset(s, sc)
+    K = args.c3_steps
+    iters, res_max, t_steps = [], 0.0, 0.0
     t0 = time.perf_counter()
-    for _ in range(args.batch_steps):
+    for _ in range(K):
         b.step(**kw)
         _, it, res = b.results()
         iters.append(float(it.mean()))
@@ -221,154 +349,63 @@ def run_batched(args):
     elapsed = time.perf_counter() - t0
     cells = b.cells
     sizes = sorted({(g.nx, g.ny) for g, _ in scenes})
+    b.close()
     return {
         "workload": f"{len(scenes)} decaying-turbulence scenes, sizes {sizes} drawn by "
                     "SplitMix64(20240612), square-obstacle lattices, k^-3 velocity RMS 1, "
-                    "MacCormack, one MG-DC cycle then flexible CG to 1e-6, CFL 0.5",
-        "cells": cells,
-        "steps": args.batch_steps,
-        "value": cells * args.batch_steps / elapsed,
-        "unit": "cell-steps/s",
-        "ms_per_step": elapsed * 1e3 / args.batch_steps,
-        "vcycles_per_solve_mean": sum(iters) / len(iters),
-        "max_relative_residual": res_max,
-        "largest_scene_tiles": tune["tiles"],
+                    f"MacCormack, {SOLVER_NAMES['mg-fcg']}, tol 1e-6, CFL 0.5, {K} steps",
+        "cells": cells, "steps": K, "value": cells * K / elapsed, "unit": "cell-steps/s",
+        "ms_per_step": elapsed * 1e3 / K, "vcycles_per_solve_mean": sum(iters) / len(iters),
+        "max_relative_residual": res_max, "largest_scene_tiles": tune["tiles"],
         "execution": "one engine and stream per scene, 16 host threads taking scenes largest "
                      "first",
     }
 
 
-LARGE_KERNELS = ("dc_update", "smooth_restrict_fine", "prolong_smooth_fine", "maccormack_faces",
-                 "advect_faces", "maccormack_dye", "advect_dye")
-
-
-def run_large(args):
-    """One n x n domain (default 8192^2, 67 M cells): closed box, 200 seeded
-    circular obstacles, 16 buoyant heat sources (buoyancy 1), MacCormack,
-    MG-DC to 1e-6.  Times K steps, replays them instrumented (solve times,
-    cycles), then traces each large kernel for one step and reports its
-    algorithmic GB/s against the HBM peak."""
-    from flowlab_amd import scenes as S
-    from flowlab_amd.simulation import Simulation
-
-    n = args.large_n
-    g, scene = S.large_plume(n)
-    params = dict(scheme="macCormack", cfl_target=0.5, dt_max=0.5 * g.h, solver=args.solver,
-                  pressure_iters=100, tol=1e-6, residual_every=0, buoyancy=1.0)
-    sim = Simulation(n, n, params, readback=False)
-    sim.reset(scene)
-    e = sim.stepper.engine
-    for _ in range(2):
-        sim.step()
-    e.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.large_steps):
-        sim.step()
-    e.synchronize()
-    elapsed = time.perf_counter() - t0
-    sim.reset(scene)  # the same steps again, instrumented
-    for _ in range(2):
-        sim.step()
-    e.set_timing(True)
-    solve_ms, iters, res = [], [], 0.0
-    for _ in range(args.large_steps):
-        sim.step()
-        solve_ms.append(e.timings()["solve"])
-        iters.append(sim.iters)
-        res = max(res, float(e.solve_history()[-1]))
-    e.synchronize()
-    bracket = e.trace_overhead(200)
-    kernels = {}
-    for k in LARGE_KERNELS:
-        e.trace_kernel(k)
-        sim.step()
-        cnt, ms = e.trace_stats()
-        e.trace_kernel(None)
-        if cnt:
-            avg = ms / cnt - bracket
-            gbs = algo_bytes(k, args.solver) * n * n / (avg * 1e-3) / 1e9
-            kernels[k] = {"avg_launch_ms": avg, "launches": cnt, "achieved_GBs": gbs,
-                          "frac": gbs / HBM_PEAK_GBS}
-    e.set_timing(False)
-    K = args.large_steps
-    learned = run_large_learned(args, sim, scene, K) if args.large_learned else None
-    if learned:
-        learned["solves_per_s_vs_plain"] = sum(solve_ms) / K / learned["ms_per_solve"]
-    return {
-        "workload": f"thermal plume {n}x{n}, closed box, 200 circular obstacles, 16 buoyant "
-                    f"sources, MacCormack, {SOLVER_NAMES[args.solver]}, tol 1e-6, "
-                    "SplitMix64(20240613)",
-        "cells": n * n,
-        "steps": K,
-        "value": n * n * K / elapsed,
-        "unit": "cell-steps/s",
-        "ms_per_step": elapsed * 1e3 / K,
-        "ms_per_solve": sum(solve_ms) / K,
-        "vcycles_per_solve": sum(iters) / K,
-        "max_relative_residual": res,
-        "kernels": kernels,
-        "learned_warm_start": learned,
-    }
-
-
-def run_large_learned(args, sim, scene, K):
-    """The large domain's steps again from the same reset, every pressure
-    solved from the trained U-Net warm start (inference at the full 8192^2
-    resolution, then the same solver to 1e-6): K timed steps, then K more
-    with per-solve times and cycle counts."""
+def run_c4(args):
+    """The learned warm start against the zero and previous-pressure starts
+    at 1024^2 and 2048^2 (3 seeded scenes each, the next step's solve to
+    1e-6): V-cycles and device ms per solve."""
     from flowlab_amd import learned as L
     if not os.path.exists(L.WEIGHTS):
         return None
-    n = args.large_n
     u = L.UNet()
     u.load(L.WEIGHTS)
-    e = sim.stepper.engine
-    sim.reset(scene)
-    e.set_warm_start(u)
-    for _ in range(2):
-        sim.step()
-    e.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(K):
-        sim.step()
-    e.synchronize()
-    elapsed = time.perf_counter() - t0
-    e.set_timing(True)
-    solve_ms, iters, res = [], [], 0.0
-    for _ in range(K):
-        sim.step()
-        solve_ms.append(e.timings()["solve"])
-        iters.append(sim.iters)
-        res = max(res, float(e.solve_history()[-1]))
-    e.set_timing(False)
-    e.set_warm_start(None)
-    return {
-        "value": n * n * K / elapsed,
-        "unit": "cell-steps/s",
-        "ms_per_step": elapsed * 1e3 / K,
-        "ms_per_solve": sum(solve_ms) / K,
-        "vcycles_per_solve": sum(iters) / K,
-        "max_relative_residual": res,
-        "weights": os.path.relpath(L.WEIGHTS, ROOT),
-    }
+    out = {}
+    for n in (1024, 2048):
+        r = L.evaluate(u, n, scenes=3)
+        r["learned_solves_per_s_vs_previous"] = r["previous"]["ms"] / r["learned"]["ms"]
+        r["learned_solves_per_s_vs_zero"] = r["zero"]["ms"] / r["learned"]["ms"]
+        out[f"{n}x{n}"] = r
+    return out
 
 
-def run_cpu_baseline(args):
-    from flowlab_amd import scenes as S
+def run_cpu_baseline(state, args):
+    """One step of the headline workload by the FP64 C oracle (one thread),
+    continued from the device's fields after the 200-step run, with the
+    oracle's defect-correction multigrid to 1e-6."""
+    import numpy as np
     from oracle import pyoracle as O
-
-    nx, ny = args.nx, args.ny
-    g, scene = S.karman_street(nx, ny)
-    osim = O.OracleSim(nx, ny)
-    osim.reset(g, scene)
+    n = args.n
+    g, scene = state["g"], state["scene"]
+    osim = O.OracleSim(n, n)
+    osim.reset(g, scene)  # labels and the inlet patch
+    osim.u[:] = state["u"]
+    osim.v[:] = state["v"]
+    osim.p[:] = state["p"]
+    osim.dye[:] = state["dye"]
+    assert np.array_equal(osim.label, state["label"])
+    pt = osim.patch
     prm = O.step_params(scheme="macCormack", solver="mg-dc", pressure_iters=100, tol=1e-6,
-                        cfl_target=0.5, residual_every=0)
-    osim.step(prm)  # warm-up (the impulsive start's first solve)
+                        cfl_target=0.5, dt_max=0.5 * g.h, buoyancy=1.0, residual_every=0,
+                        patch=(pt.i0, pt.j0, pt.nx, pt.ny))
     t0 = time.perf_counter()
-    for _ in range(args.cpu_steps):
-        osim.step(prm)
-    dt = time.perf_counter() - t0
-    return nx * ny * args.cpu_steps / dt, dt
+    osim.step(prm, pt.data)
+    secs = time.perf_counter() - t0
+    return {"value": n * n / secs, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"1 step of the {n}x{n} plume (step {WORKLOAD_STEPS + 1}, continued from "
+                      f"the device's fields), FP64 C oracle, one thread, MG-DC to 1e-6 "
+                      f"({prm.iters} V-cycles, {secs:.1f} s)"}
 
 
 def main():
@@ -380,85 +417,90 @@ def main():
         sys.exit(2)
     if args.copy_mb > 0:
         import torch
-        n = args.copy_mb * (1 << 20) // 4
-        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        nb = args.copy_mb * (1 << 20) // 4
+        a = torch.ones(nb, dtype=torch.float32, device="cuda")
         b = torch.empty_like(a)
         for _ in range(3):
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    r = run_gpu(args)
-    learned = run_learned(args, r)
-    batched = run_batched(args) if args.batch_scenes > 0 else None
-    large = run_large(args) if args.large_n > 0 else None
-    nx, ny, K = args.nx, args.ny, args.steps
-    cells = nx * ny
-    value = cells * K / r["elapsed"]
-    # per-launch device time: event bracket around each launch, less the
-    # bracket's own cost (an empty event pair on the same stream)
-    k_avg_ms = r["k_ms"] / max(r["n_launch"], 1) - r["bracket_ms"]
-    # kernels without a bandwidth figure (the latency-bound coarse sub-cycle)
-    # report their launch time only
-    bpc = algo_bytes(args.kernel, args.solver)
-    bytes_per_launch = bpc * cells if bpc else None
-    achieved = (bytes_per_launch / (k_avg_ms * 1e-3) / 1e9
-                if r["n_launch"] and bytes_per_launch else None)
+    legs = {k for k in args.legs.split(",") if k}
+    h = run_headline(args)
+    n, K = h["n"], args.steps
+    cells = n * n
+    wl = h["workload"]
+    out_legs = {}
+    if "c1" in legs:
+        log("leg C1")
+        out_legs["C1_plume_128_rbsor"] = run_c1(args)
+    if "c2" in legs:
+        log("leg C2")
+        out_legs["C2_karman_2048x1024"] = run_c2(args)
+    if "c3" in legs:
+        log("leg C3")
+        out_legs["C3_turbulence_batch_64"] = run_c3(args)
+    if "c4" in legs:
+        log("leg C4")
+        out_legs["C4_learned_warm_start_eval"] = run_c4(args)
+    dom = h["dominant"]
+    bpc = algo_bytes(dom, args.solver)
+    k_avg_ms = h["k_ms"] / max(h["n_launch"], 1)
+    bytes_per_launch = bpc * cells
+    achieved = bytes_per_launch / (k_avg_ms * 1e-3) / 1e9 if h["n_launch"] else None
+    wsum = leg_summary(cells, WORKLOAD_STEPS, wl["elapsed"], wl)
+    wsum["note"] = "per-step phase timing on (the host waits for every step's end)"
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": cells * K / h["elapsed"],
         "unit": "cell-steps/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": r["elapsed"] * 1e3 / K,
+        "ms_per_step": h["elapsed"] * 1e3 / K,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 (advection, smoothing, transfers) + f64 (pressure, residual, norms)",
         "data": "synthetic",
         "config": {
-            "workload": f"karman vortex street {nx}x{ny}, MacCormack, "
-                        f"{SOLVER_NAMES[args.solver]}, tol 1e-6, CFL 0.5",
-            "grid": [nx, ny],
+            "workload": f"thermal plume {n}x{n}, closed box, 200 circular obstacles, 16 "
+                        f"buoyant sources, buoyancy 1, MacCormack, {SOLVER_NAMES[args.solver]}, "
+                        "tol 1e-6, dt 0.5 h capped at CFL 0.5, SplitMix64(20240613)",
+            "grid": [n, n],
+            "workload_steps": WORKLOAD_STEPS,
+            "timed_steps": f"{args.warmup + 1}..{args.warmup + K} after the reset",
             "solver": args.solver,
             "tol": 1e-6,
-            "mg_tiles": r["tiles"],
+            "mg_tiles": h["tiles"],
         },
         "pressure": {
-            "ms_per_solve": sum(r["solve_ms"]) / K,
-            "vcycles_per_solve": sum(r["iters"]) / K,
-            "max_relative_residual": r["hist_max"],
-            "learned_warm_start": learned,
+            "ms_per_solve": wsum["ms_per_solve"],
+            "vcycles_per_solve": wsum["vcycles_per_solve"],
+            "max_relative_residual": wsum["max_relative_residual"],
+            "over": f"all {WORKLOAD_STEPS} solves of the workload",
+            "learned_warm_start": h["learned"],
         },
+        "workload_run": wsum,
         "roofline": {
-            "kernel": args.kernel,
+            "kernel": dom,
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": None if achieved is None else achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.kernel, nx, ny),
+            "traffic": measured_traffic(dom, n),
             "traffic_source": os.path.relpath(PMC_TRAFFIC, ROOT),
             "avg_launch_ms": k_avg_ms,
-            "event_bracket_ms": r["bracket_ms"],
-            "launches": r["n_launch"],
+            "launches": h["n_launch"],
             "algorithmic_bytes_per_launch": bytes_per_launch,
+            "algorithmic_bytes_per_cell": bpc,
         },
+        "kernels": h["kernels"],
+        "legs": out_legs,
     }
-    if batched is not None:
-        out["batched"] = batched
-    if large is not None:
-        out["large"] = large
-    if args.cpu_steps > 0:
-        v, secs = run_cpu_baseline(args)
-        out["cpu_baseline"] = {
-            "value": v,
-            "unit": "cell-steps/s",
-            "cores": 1,
-            "kind": "port",
-            "sample": f"{args.cpu_steps} steps of the same {nx}x{ny} workload after 1 warm-up "
-                      f"step, FP64 C oracle, one thread ({secs:.1f} s)",
-        }
+    if args.cpu_baseline and h["state"] is not None:
+        log("cpu baseline")
+        out["cpu_baseline"] = run_cpu_baseline(h["state"], args)
     print(json.dumps(out))
 
 

@@ -595,11 +595,10 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
     int st;
     if (sync_init) {
       launch_set_int(e->s, e->done, 0);
-      launch_dc_correct(e->s, e->p, e->p2, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
+      launch_dc_correct(e->s, e->p, nullptr, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
                         e->partials, nullptr, r64);
       launch_dc_norm(e->s, e->partials, e->nx, e->ny, e->sums, nullptr, nullptr, scale, 0.0);
       LAUNCHED("dc_update");
-      swap_ptr(e->p, e->p2);
       st = read_sums(e);
       if (st != FH_OK) return st;
       if (e->h_sums[2] == 0.0) {
@@ -618,22 +617,20 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
         st = engine_warm_start(e->warm, e->s, L0.b, e->lbl, L0.info, e->p, e->ny, e->nx,
                                e->wstats);
         if (st != FH_OK) return st;
-        launch_dc_correct(e->s, e->p, e->p2, nullptr, e->div, L0.info, L0.b, e->nx, e->ny,
+        launch_dc_correct(e->s, e->p, nullptr, nullptr, e->div, L0.info, L0.b, e->nx, e->ny,
                           scale, e->partials, nullptr, r64);
         launch_dc_norm(e->s, e->partials, e->nx, e->ny, e->sums, nullptr, nullptr, scale, 0.0);
         LAUNCHED("dc_update");
-        swap_ptr(e->p, e->p2);
         st = read_sums(e);
         if (st != FH_OK) return st;
       }
       e->hist.push_back(rel_resid(e->h_sums, scale, true));
     } else {
-      launch_dc_correct(e->s, e->p, e->p2, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
+      launch_dc_correct(e->s, e->p, nullptr, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
                         e->partials, nullptr, r64);
       launch_dc_norm_init(e->s, e->partials, e->nx, e->ny, e->sums, e->d_res + 16, e->done,
                           scale);
       LAUNCHED("dc_update");
-      swap_ptr(e->p, e->p2);
       HIPCHK(hipEventRecord(e->cyc_ev[2], e->s));
     }
     // Cycles are enqueued one ahead of the host's convergence check: cycle
@@ -820,9 +817,11 @@ int forces(fh_engine* e, double dt) {
   float eps = (float)e->confinement;
   float gain = (float)(e->buoyancy * dt);
   if (eps != 0.f) launch_vorticity(e->s, e->u, e->v, e->w, e->nx, e->ny, (float)(1.0 / e->h));
-  if (e->stir.active || gain != 0.f || eps != 0.f)
+  if (e->stir.active || gain != 0.f || eps != 0.f) {
+    Trace t(e, FH_K_FORCES);
     launch_forces(e->s, e->u, e->v, e->lbl, e->dye, e->w, e->nx, e->ny, e->ds, (float)e->h,
                   (float)dt, gain, eps, e->stir);
+  }
   e->stir.active = 0;  // one frame's displacement (simulation.ts:69-72)
   e->vel_epoch++;
   LAUNCHED("forces");
@@ -830,7 +829,10 @@ int forces(fh_engine* e, double dt) {
 }
 
 int project(fh_engine* e, double grad_scale) {
-  launch_subtract_outflow(e->s, e->u, e->v, e->p, e->lbl, e->nx, e->ny, grad_scale);
+  {
+    Trace t(e, FH_K_PROJECT);
+    launch_subtract_outflow(e->s, e->u, e->v, e->p, e->lbl, e->nx, e->ny, grad_scale);
+  }
   e->vel_epoch++;
   LAUNCHED("project");
   return FH_OK;
@@ -930,7 +932,7 @@ fh_engine* fh_create(int nx, int ny, int dye_scale, double h) {
   e->lc_ref = coarse_start(e, e->nlev_ref);
   e->lc_dc = coarse_start(e, e->nlev_dc);
   size_t nblk = (size_t)((nx + 63) / 64) * ((ny + 3) / 4);
-  if (dalloc(e, &e->partials, 4 * nblk) || dalloc(e, &e->counter, 1) || dalloc(e, &e->sums, 8) ||
+  if (dalloc(e, &e->partials, 4 * nblk + 4 * kFoldBlocks) || dalloc(e, &e->counter, 1) || dalloc(e, &e->sums, 8) ||
       dalloc(e, &e->fpartials, 1024) || dalloc(e, &e->fmax, 1) || dalloc(e, &e->done, 1))
     return bail();
   if (hipHostMalloc((void**)&e->h_sums, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
@@ -1010,6 +1012,9 @@ int fh_upload(fh_engine* e, const double* u, const double* v, const double* p,
   if (label) {
     HIPCHK(hipMemcpy(e->lbl, label, n_c(e), hipMemcpyHostToDevice));
     e->labels_dirty = true;
+    // The label chain and the coarsest operator depend on the labels alone:
+    // rebuilt here, once per upload, rather than inside the next solve.
+    if ((st = refresh_levels(e))) return st;
   }
   return FH_OK;
 }
@@ -1107,7 +1112,10 @@ int fh_step(fh_engine* e, int scheme, double dt, double scale, double grad_scale
   if ((st = mark(e, 1))) return st;
   if ((st = forces(e, dt))) return st;
   if ((st = mark(e, 2))) return st;
-  launch_divergence(e->s, e->u, e->v, e->div, e->nx, e->ny, 1.0 / e->h);
+  {
+    Trace t(e, FH_K_DIVERGENCE);
+    launch_divergence(e->s, e->u, e->v, e->div, e->nx, e->ny, 1.0 / e->h);
+  }
   LAUNCHED("divergence");
   if ((st = mark(e, 3))) return st;
   if ((st = solve(e, solver, scale, iterations, omega, tol, out_iters))) return st;

@@ -64,6 +64,10 @@ void launch_max_speed(hipStream_t s, const float* u, const float* v, int nx, int
                       float* host_out = nullptr);
 
 // ---- pressure (fh_mg.hip) --------------------------------------------------
+// Reductions over more than kFoldAbove blocks fold their partials with
+// kFoldBlocks workgroups before the one-block finalize; every partials
+// buffer holds 4 * kFoldBlocks doubles beyond its producers' 4 per block.
+constexpr int kFoldAbove = 8192, kFoldBlocks = 128;
 // One colour half-sweep: x = (1-w) x + w (sum - scale*b)/count on fluid cells
 // of colour (i+j)&1 == color.  pressureRedBlack.ts:32-72.  first != 0 treats
 // x as zero on entry and writes both colours (the first sweep of a V-cycle

@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void k_dc_update(
       pN += (double)e[kN];
       pS += (double)e[kS];
     }
-    pn[k] = pc;
+    if (pn) pn[k] = pc;
     double r = 0.0;
     if (f & 0x80) {
       acc[2] += d * d;
@@ -426,6 +426,41 @@ __global__ __launch_bounds__(256) void k_norms(const float* __restrict__ p32,
   block_sum_store<4>(acc, partials);
 }
 
+// Reductions with more than kFoldAbove block partials are folded first:
+// block b of kFoldBlocks sums the partials of one fixed contiguous range of
+// producing blocks in a fixed order, written after the originals (the
+// partials buffers hold 4 kFoldBlocks doubles more), so the one-block kernels
+// below read kFoldBlocks partials instead of up to 65536 (2 MB through one
+// CU: 45 us at 8192^2) and stay deterministic.  Skips like the kernel that
+// consumes it (done set, outside the initial pass).
+__global__ __launch_bounds__(256) void k_fold_partials(const double* __restrict__ in, int n,
+                                                       double* __restrict__ out,
+                                                       const int* __restrict__ done, int init) {
+  if (!init && done && *done) return;
+  const int chunk = (n + kFoldBlocks - 1) / kFoldBlocks;
+  const int b0 = blockIdx.x * chunk, b1 = min(n, b0 + chunk);
+  const double2* p2 = reinterpret_cast<const double2*>(in);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += 256) {
+    const double2 lo = p2[2 * b], hi = p2[2 * b + 1];
+    acc[0] += lo.x;
+    acc[1] += lo.y;
+    acc[2] += hi.x;
+    acc[3] += hi.y;
+  }
+  block_sum_store<4>(acc, out);
+}
+
+// The partials a one-block reduction reads: folded first when there are many.
+static inline const double* folded(hipStream_t s, const double* partials, int* n,
+                                   const int* done, int init) {
+  if (*n <= kFoldAbove) return partials;
+  double* out = const_cast<double*>(partials) + 4 * (size_t)*n;
+  k_fold_partials<<<kFoldBlocks, 256, 0, s>>>(partials, *n, out, done, init);
+  *n = kFoldBlocks;
+  return out;
+}
+
 // One block sums the 4-slot partials in block order (deterministic) into
 // out[0..3].  With tol > 0 it also writes out[4] = the relative residual
 // sqrt(out0/out1) / (|scale| sqrt(out2/out3)) (the defect-correction norm)
@@ -497,6 +532,7 @@ __global__ __launch_bounds__(1024) void k_finalize_sum4(const double* __restrict
 }
 
 void launch_finalize_sum4(hipStream_t s, const double* partials, int nblocks, double* out) {
+  partials = folded(s, partials, &nblocks, nullptr, 1);
   k_finalize_sum4<<<1, 1024, 0, s>>>(partials, nblocks, out, nullptr, nullptr, 1.0, 0.0, 0,
                                      nullptr);
 }
@@ -563,15 +599,17 @@ void launch_dc_correct(hipStream_t s, const double* po, double* pn, const float*
 void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                     double* host_sums, int* done, double scale, double tol) {
   dim3 g = reduce_grid(nx, ny);
-  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, g.x * g.y, sums, host_sums, done, scale, tol, 0,
-                                     nullptr);
+  int n = g.x * g.y;
+  partials = folded(s, partials, &n, done, 0);
+  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, n, sums, host_sums, done, scale, tol, 0, nullptr);
 }
 
 void launch_dc_norm_init(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                          double* host_sums, int* done, double scale) {
   dim3 g = reduce_grid(nx, ny);
-  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, g.x * g.y, sums, host_sums, done, scale, 0.0, 1,
-                                     nullptr);
+  int n = g.x * g.y;
+  partials = folded(s, partials, &n, done, 1);
+  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, n, sums, host_sums, done, scale, 0.0, 1, nullptr);
 }
 
 void launch_fcg_dots(hipStream_t s, const float* z, const float* dprev, const double* r,
@@ -579,7 +617,9 @@ void launch_fcg_dots(hipStream_t s, const float* z, const float* dprev, const do
                      double* state, const int* done) {
   dim3 g = reduce_grid(nx, ny);
   k_fcg_dots<<<g, dim3(BX, BY), 0, s>>>(z, dprev, r, info, nx, ny, first, partials, done);
-  k_fcg_coef<<<1, 1024, 0, s>>>(partials, g.x * g.y, state, first, done);
+  int n = g.x * g.y;
+  const double* pp = folded(s, partials, &n, done, 0);
+  k_fcg_coef<<<1, 1024, 0, s>>>(pp, n, state, first, done);
 }
 
 void launch_fcg_update(hipStream_t s, const float* z, const float* dprev, float* dnext, double* p,
@@ -594,8 +634,9 @@ void launch_fcg_norm(hipStream_t s, const double* partials, int nx, int ny, doub
                      const double* sums0, double* host_sums, int* done, double scale,
                      double tol) {
   dim3 g = reduce_grid(nx, ny);
-  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, g.x * g.y, sums, host_sums, done, scale, tol, 0,
-                                     sums0);
+  int n = g.x * g.y;
+  partials = folded(s, partials, &n, done, 0);
+  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, n, sums, host_sums, done, scale, tol, 0, sums0);
 }
 
 void launch_norms(hipStream_t s, const float* p32, const double* p64, const double* div,

@@ -24,7 +24,8 @@ namespace {
 // each side within the launch grids of the sweep and reduction kernels.
 constexpr long kNoAllocMaxCells = 1L << 20;
 constexpr int kNoAllocMaxSide = 1 << 16;
-constexpr size_t kPartialsDoubles = 4 * 65536;  // the reduction grids' cap, 4 slots a block
+// the reduction grids' cap of 65536 blocks, 4 slots a block, and the fold area
+constexpr size_t kPartialsDoubles = 4 * (65536 + kFoldBlocks);
 
 constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 constexpr size_t kOffX = 0;

@@ -251,12 +251,19 @@ class HipStepper:
         self.readback = readback
         self._dirty = True
 
+    def upload(self, f):
+        """The re-upload invalidate() asks for (stepGpu.ts:419-433): every host
+        field, with the labels' multigrid hierarchy rebuilt.  step() does it
+        when needed; a caller that times steps on resident fields does it
+        first."""
+        self.engine.upload(f.u, f.v, f.p, f.label, np.concatenate(f.dye))
+        self._dirty = False
+
     def step(self, f, scheme, dt, scale, grad_scale, dye_keep, perf=None, iterations=100,
              omega=1.0, tol=1e-6) -> int:
         e = self.engine
         if self._dirty:
-            e.upload(f.u, f.v, f.p, f.label, np.concatenate(f.dye))
-            self._dirty = False
+            self.upload(f)
         iters = e.step(scheme, dt, scale, grad_scale, dye_keep, self.solver, iterations, omega,
                        tol)
         if self.readback:

@@ -111,7 +111,7 @@ SIGNATURES = [
 KERNELS = {"rb_sweep_fine": 1, "advect_faces": 2, "maccormack_faces": 3, "advect_dye": 4,
            "maccormack_dye": 5, "dc_update": 6, "restrict_fine": 7,
            "smooth_restrict_fine": 8, "prolong_smooth_fine": 9, "coarse_vcycle": 10,
-           "fcg_dots": 11, "fcg_update": 12}
+           "fcg_dots": 11, "fcg_update": 12, "divergence": 13, "forces": 14, "project": 15}
 
 
 class HipError(RuntimeError):

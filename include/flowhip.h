@@ -171,7 +171,10 @@ enum {
   FH_K_PROLONG_SMOOTH_FINE = 9,  /* fused prolong + post-smooth, finest level */
   FH_K_COARSE_VCYCLE = 10,       /* one-workgroup sub-cycle of the coarse levels */
   FH_K_FCG_DOTS = 11,            /* flexible CG: dot products (+ coefficients) */
-  FH_K_FCG_UPDATE = 12           /* flexible CG: direction, p, r and norm update */
+  FH_K_FCG_UPDATE = 12,          /* flexible CG: direction, p, r and norm update */
+  FH_K_DIVERGENCE = 13,          /* FP64 divergence of the FP32 faces */
+  FH_K_FORCES = 14,              /* buoyancy (+ stir, vorticity confinement) */
+  FH_K_PROJECT = 15              /* gradient subtraction + outflow */
 };
 int fh_trace_kernel(fh_engine* e, int kernel);
 int fh_trace_stats(fh_engine* e, int* count, double* total_ms);
