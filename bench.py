@@ -99,6 +99,10 @@ def parse():
     ap.add_argument("--c3-steps", type=int, default=500)
     ap.add_argument("--c1-steps", type=int, default=200)
     ap.add_argument("--cpu-baseline", type=int, default=1, help="0 = skip the oracle timing")
+    ap.add_argument("--workload-run", type=int, default=1,
+                    help="0 = skip the instrumented 200-step run (profiler passes)")
+    ap.add_argument("--table", type=int, default=1,
+                    help="0 = skip the per-kernel table (profiler passes)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (the known-size "
                          "copy kernel that scales hardware traffic counters)")
@@ -192,8 +196,9 @@ def run_headline(args):
     sim.step()  # uploads the fields and builds the multigrid hierarchy
     tiles = e.tune_tiles(reps=5)["tiles"]
     log(f"headline: tiles {tiles}; per-kernel table")
-    kernels = kernel_table(sim, n * n, args.solver)
-    dominant = args.kernel or max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    kernels = kernel_table(sim, n * n, args.solver) if args.table else {}
+    dominant = args.kernel or (max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+                               if kernels else "maccormack_dye")
     # timed pass: reset and upload (the fields are resident before the
     # clock starts), W untimed steps, K timed steps with the dominant
     # kernel's launches bracketed on the engine stream
@@ -212,7 +217,7 @@ def run_headline(args):
     n_launch, k_ms = e.trace_stats()
     e.trace_kernel(None)
     log("headline: the 200-step workload, instrumented")
-    wl = instrumented(sim, scene, WORKLOAD_STEPS)
+    wl = instrumented(sim, scene, WORKLOAD_STEPS if args.workload_run else args.steps)
     state = None
     if args.cpu_baseline:
         u, v = e.read_velocity()
@@ -447,7 +452,7 @@ def main():
     k_avg_ms = h["k_ms"] / max(h["n_launch"], 1)
     bytes_per_launch = bpc * cells
     achieved = bytes_per_launch / (k_avg_ms * 1e-3) / 1e9 if h["n_launch"] else None
-    wsum = leg_summary(cells, WORKLOAD_STEPS, wl["elapsed"], wl)
+    wsum = leg_summary(cells, len(wl["iters"]), wl["elapsed"], wl)
     wsum["note"] = "per-step phase timing on (the host waits for every step's end)"
     out = {
         "metric": METRIC,

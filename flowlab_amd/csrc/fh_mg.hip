@@ -186,6 +186,54 @@ static inline dim3 reduce_grid(int nx, int ny) {
   return dim3(gx, gy < cap ? gy : cap);
 }
 
+// The defect-correction and CG passes walk kRedRows consecutive rows per
+// thread and iteration, every stencil value of those rows loaded before any
+// arithmetic or store (consecutive rows share their north/south loads, and a
+// CU keeps enough bytes in flight to stream HBM).  Grids too small to fill
+// the chip that way (fewer than kRedMinBlocks blocks) keep one row per
+// thread with reduce_grid's more numerous blocks.  The block count is capped
+// at kMaxPartials (rows beyond it are walked with the grid's stride).
+constexpr int kRedRows = 4, kRedMinBlocks = 1024;
+
+struct RGrid {
+  dim3 g;
+  int rows;  // rows per thread and iteration: kRedRows or 1
+};
+
+static inline RGrid reduce_grid_rows(int nx, int ny) {
+  const int gx = (nx + BX - 1) / BX;
+  const int trows = (ny + kRedRows - 1) / kRedRows;
+  int gy = (trows + BY - 1) / BY;
+  if ((long)gx * gy < kRedMinBlocks) return {reduce_grid(nx, ny), 1};
+  int cap = kMaxPartials / gx;
+  if (cap < 1) cap = 1;
+  return {dim3(gx, gy < cap ? gy : cap), kRedRows};
+}
+
+// Column i of a field over the rows j0-1 .. j0+R (clamped to the domain: c),
+// and its east and west neighbours (clamped columns) over rows j0 .. j0+R-1.
+// For row j0+r: centre c[r+1], north c[r+2], south c[r], east e[r], west
+// w[r] -- at the domain edge the clamped index is the cell itself, as the
+// one-row kernels' kN/kS/kE/kW are.
+template <int R, class T>
+struct Col5 {
+  T c[R + 2], e[R], w[R];
+};
+
+template <int R, class T>
+__device__ __forceinline__ void load_col5(Col5<R, T>& o, const T* __restrict__ a, int i, int j0,
+                                          int nx, int ny) {
+  const int iE = i + 1 < nx ? i + 1 : i, iW = i > 0 ? i - 1 : i;
+#pragma unroll
+  for (int q = 0; q < R + 2; q++) o.c[q] = a[i + min(max(j0 - 1 + q, 0), ny - 1) * nx];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int row = min(j0 + r, ny - 1) * nx;
+    o.e[r] = a[iE + row];
+    o.w[r] = a[iW + row];
+  }
+}
+
 // FP64 defect-correction update, one thread per cell:
 //   pn = po + e  (e = 0 off the fluid),  r = b - (count*pn - sum pn),
 //   b = -scale*div,  r32 = (float) r  (the next V-cycle's right-hand side).
@@ -195,6 +243,7 @@ static inline dim3 reduce_grid(int nx, int ny) {
 // count), so every load of a cell is issued at once, unconditionally, from
 // clamped in-domain indices, and closed neighbours add 0 (sum order +x, -x,
 // +y, -y, as in the reference's residual).
+template <int R>
 __global__ __launch_bounds__(256) void k_dc_update(
     const double* __restrict__ po, double* __restrict__ pn, const float* __restrict__ e,
     const double* __restrict__ div, const uint8_t* __restrict__ info, float* __restrict__ r32,
@@ -205,40 +254,55 @@ __global__ __launch_bounds__(256) void k_dc_update(
   xcd_block(&bx, &by);
   const int i = bx * BX + threadIdx.x;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int j = by * BY + threadIdx.y; j < ny; j += gridDim.y * BY) {
+  for (int j0 = (by * BY + threadIdx.y) * R; j0 < ny; j0 += gridDim.y * BY * R) {
     if (i >= nx) break;
-    const int k = i + j * nx;
-    const int kE = i + 1 < nx ? k + 1 : k, kW = i > 0 ? k - 1 : k;
-    const int kN = j + 1 < ny ? k + nx : k, kS = j > 0 ? k - nx : k;
-    const int f = info[k];
-    const double d = div[k];
-    double pc = po[k], pE = po[kE], pW = po[kW], pN = po[kN], pS = po[kS];
-    if (e) {
-      pc += (double)e[k];
-      pE += (double)e[kE];
-      pW += (double)e[kW];
-      pN += (double)e[kN];
-      pS += (double)e[kS];
+    Col5<R, double> P;
+    load_col5(P, po, i, j0, nx, ny);
+    Col5<R, float> E;
+    if (e) load_col5(E, e, i, j0, nx, ny);
+    int fr[R];
+    double dr[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int k = i + min(j0 + r, ny - 1) * nx;
+      fr[r] = info[k];
+      dr[r] = div[k];
     }
-    if (pn) pn[k] = pc;
-    double r = 0.0;
-    if (f & 0x80) {
-      acc[2] += d * d;
-      acc[3] += 1.0;
-      const int cnt = f & 7;
-      if (cnt > 0) {
-        double sum = 0.0;
-        sum += (f & 0x08) ? pE : 0.0;
-        sum += (f & 0x10) ? pW : 0.0;
-        sum += (f & 0x20) ? pN : 0.0;
-        sum += (f & 0x40) ? pS : 0.0;
-        r = -scale * d - ((double)cnt * pc - sum);
-        acc[0] += r * r;
-        acc[1] += 1.0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int j = j0 + r;
+      if (j >= ny) break;
+      const int k = i + j * nx;
+      const int f = fr[r];
+      const double d = dr[r];
+      double pc = P.c[r + 1], pE = P.e[r], pW = P.w[r], pN = P.c[r + 2], pS = P.c[r];
+      if (e) {
+        pc += (double)E.c[r + 1];
+        pE += (double)E.e[r];
+        pW += (double)E.w[r];
+        pN += (double)E.c[r + 2];
+        pS += (double)E.c[r];
       }
+      if (pn) pn[k] = pc;
+      double rr = 0.0;
+      if (f & 0x80) {
+        acc[2] += d * d;
+        acc[3] += 1.0;
+        const int cnt = f & 7;
+        if (cnt > 0) {
+          double sum = 0.0;
+          sum += (f & 0x08) ? pE : 0.0;
+          sum += (f & 0x10) ? pW : 0.0;
+          sum += (f & 0x20) ? pN : 0.0;
+          sum += (f & 0x40) ? pS : 0.0;
+          rr = -scale * d - ((double)cnt * pc - sum);
+          acc[0] += rr * rr;
+          acc[1] += 1.0;
+        }
+      }
+      r32[k] = (float)rr;
+      if (r64) r64[k] = rr;
     }
-    r32[k] = (float)r;
-    if (r64) r64[k] = r;
   }
   block_sum_store<4>(acc, partials);
 }
@@ -280,6 +344,7 @@ __device__ __forceinline__ double apply_a(int f, double c, double e, double w, d
 }
 
 // Pass 1: partials of s (first: d' is not read and counts as 0).
+template <int R>
 __global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
                                                   const float* __restrict__ dprev,
                                                   const double* __restrict__ r,
@@ -291,21 +356,33 @@ __global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
   xcd_block(&bx, &by);
   const int i = bx * BX + threadIdx.x;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int j = by * BY + threadIdx.y; j < ny; j += gridDim.y * BY) {
+  for (int j0 = (by * BY + threadIdx.y) * R; j0 < ny; j0 += gridDim.y * BY * R) {
     if (i >= nx) break;
-    const int k = i + j * nx;
-    const int kE = i + 1 < nx ? k + 1 : k, kW = i > 0 ? k - 1 : k;
-    const int kN = j + 1 < ny ? k + nx : k, kS = j > 0 ? k - nx : k;
-    const int f = info[k];
-    const double zc = z[k], rc = r[k];
-    const double az = apply_a(f, zc, z[kE], z[kW], z[kN], z[kS]);
-    acc[0] += zc * rc;
-    acc[1] += zc * az;
-    if (!first) {
-      const double dc = dprev[k];
-      const double ad = apply_a(f, dc, dprev[kE], dprev[kW], dprev[kN], dprev[kS]);
-      acc[2] += zc * ad;
-      acc[3] += dc * rc;
+    Col5<R, float> Z, D;
+    load_col5(Z, z, i, j0, nx, ny);
+    if (!first) load_col5(D, dprev, i, j0, nx, ny);
+    int fr[R];
+    double rr[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      const int k = i + min(j0 + q, ny - 1) * nx;
+      fr[q] = info[k];
+      rr[q] = r[k];
+    }
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      if (j0 + q >= ny) break;
+      const int f = fr[q];
+      const double zc = Z.c[q + 1], rc = rr[q];
+      const double az = apply_a(f, zc, Z.e[q], Z.w[q], Z.c[q + 2], Z.c[q]);
+      acc[0] += zc * rc;
+      acc[1] += zc * az;
+      if (!first) {
+        const double dc = D.c[q + 1];
+        const double ad = apply_a(f, dc, D.e[q], D.w[q], D.c[q + 2], D.c[q]);
+        acc[2] += zc * ad;
+        acc[3] += dc * rc;
+      }
     }
   }
   block_sum_store<4>(acc, partials);
@@ -349,6 +426,7 @@ __global__ __launch_bounds__(1024) void k_fcg_coef(const double* __restrict__ pa
 // Pass 2: d = z + beta d';  p += alpha d;  r -= alpha (A z + beta A d');
 // r32 = (float) r;  partials {sum r^2, n} over the fluid cells with a
 // non-zero count (the divergence terms of the norm are the initial pass's).
+template <int R>
 __global__ __launch_bounds__(256) void k_fcg_update(
     const float* __restrict__ z, const float* __restrict__ dprev, float* __restrict__ dnext,
     double* __restrict__ p, double* __restrict__ r, const uint8_t* __restrict__ info,
@@ -360,32 +438,46 @@ __global__ __launch_bounds__(256) void k_fcg_update(
   const int i = bx * BX + threadIdx.x;
   const double alpha = state[0], beta = first ? 0.0 : state[1];
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int j = by * BY + threadIdx.y; j < ny; j += gridDim.y * BY) {
+  for (int j0 = (by * BY + threadIdx.y) * R; j0 < ny; j0 += gridDim.y * BY * R) {
     if (i >= nx) break;
-    const int k = i + j * nx;
-    const int kE = i + 1 < nx ? k + 1 : k, kW = i > 0 ? k - 1 : k;
-    const int kN = j + 1 < ny ? k + nx : k, kS = j > 0 ? k - nx : k;
-    const int f = info[k];
-    double dc = z[k], dE = z[kE], dW = z[kW], dN = z[kN], dS = z[kS];
-    if (!first) {
-      dc += beta * (double)dprev[k];
-      dE += beta * (double)dprev[kE];
-      dW += beta * (double)dprev[kW];
-      dN += beta * (double)dprev[kN];
-      dS += beta * (double)dprev[kS];
+    Col5<R, float> Z, D;
+    load_col5(Z, z, i, j0, nx, ny);
+    if (!first) load_col5(D, dprev, i, j0, nx, ny);
+    int fr[R];
+    double pr[R], rr[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      const int k = i + min(j0 + q, ny - 1) * nx;
+      fr[q] = info[k];
+      pr[q] = p[k];
+      rr[q] = r[k];
     }
-    // the stored direction is FP32: p, r and A d all use its rounded values
-    const float d32 = (float)dc;
-    dnext[k] = d32;
-    const double ad = apply_a(f, (double)d32, (double)(float)dE, (double)(float)dW,
-                              (double)(float)dN, (double)(float)dS);
-    p[k] += alpha * (double)d32;
-    const double rn = r[k] - alpha * ad;
-    r[k] = rn;
-    r32[k] = (float)rn;
-    if ((f & 0x80) && (f & 7)) {
-      acc[0] += rn * rn;
-      acc[1] += 1.0;
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      if (j0 + q >= ny) break;
+      const int k = i + (j0 + q) * nx;
+      const int f = fr[q];
+      double dc = Z.c[q + 1], dE = Z.e[q], dW = Z.w[q], dN = Z.c[q + 2], dS = Z.c[q];
+      if (!first) {
+        dc += beta * (double)D.c[q + 1];
+        dE += beta * (double)D.e[q];
+        dW += beta * (double)D.w[q];
+        dN += beta * (double)D.c[q + 2];
+        dS += beta * (double)D.c[q];
+      }
+      // the stored direction is FP32: p, r and A d all use its rounded values
+      const float d32 = (float)dc;
+      dnext[k] = d32;
+      const double ad = apply_a(f, (double)d32, (double)(float)dE, (double)(float)dW,
+                                (double)(float)dN, (double)(float)dS);
+      p[k] = pr[q] + alpha * (double)d32;
+      const double rn = rr[q] - alpha * ad;
+      r[k] = rn;
+      r32[k] = (float)rn;
+      if ((f & 0x80) && (f & 7)) {
+        acc[0] += rn * rn;
+        acc[1] += 1.0;
+      }
     }
   }
   block_sum_store<4>(acc, partials);
@@ -591,14 +683,18 @@ void launch_dc_update(hipStream_t s, const double* po, double* pn, const float* 
 void launch_dc_correct(hipStream_t s, const double* po, double* pn, const float* e,
                        const double* div, const uint8_t* info, float* r32, int nx, int ny,
                        double scale, double* partials, const int* done, double* r64) {
-  dim3 g = reduce_grid(nx, ny);
-  k_dc_update<<<g, dim3(BX, BY), 0, s>>>(po, pn, e, div, info, r32, r64, nx, ny, scale,
-                                         partials, done);
+  const RGrid rg = reduce_grid_rows(nx, ny);
+  if (rg.rows == kRedRows)
+    k_dc_update<kRedRows><<<rg.g, dim3(BX, BY), 0, s>>>(po, pn, e, div, info, r32, r64, nx, ny,
+                                                        scale, partials, done);
+  else
+    k_dc_update<1><<<rg.g, dim3(BX, BY), 0, s>>>(po, pn, e, div, info, r32, r64, nx, ny, scale,
+                                                 partials, done);
 }
 
 void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                     double* host_sums, int* done, double scale, double tol) {
-  dim3 g = reduce_grid(nx, ny);
+  const dim3 g = reduce_grid_rows(nx, ny).g;
   int n = g.x * g.y;
   partials = folded(s, partials, &n, done, 0);
   k_finalize_sum4<<<1, 1024, 0, s>>>(partials, n, sums, host_sums, done, scale, tol, 0, nullptr);
@@ -606,7 +702,7 @@ void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, doubl
 
 void launch_dc_norm_init(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                          double* host_sums, int* done, double scale) {
-  dim3 g = reduce_grid(nx, ny);
+  const dim3 g = reduce_grid_rows(nx, ny).g;
   int n = g.x * g.y;
   partials = folded(s, partials, &n, done, 1);
   k_finalize_sum4<<<1, 1024, 0, s>>>(partials, n, sums, host_sums, done, scale, 0.0, 1, nullptr);
@@ -615,9 +711,13 @@ void launch_dc_norm_init(hipStream_t s, const double* partials, int nx, int ny, 
 void launch_fcg_dots(hipStream_t s, const float* z, const float* dprev, const double* r,
                      const uint8_t* info, int nx, int ny, int first, double* partials,
                      double* state, const int* done) {
-  dim3 g = reduce_grid(nx, ny);
-  k_fcg_dots<<<g, dim3(BX, BY), 0, s>>>(z, dprev, r, info, nx, ny, first, partials, done);
-  int n = g.x * g.y;
+  const RGrid rg = reduce_grid_rows(nx, ny);
+  if (rg.rows == kRedRows)
+    k_fcg_dots<kRedRows><<<rg.g, dim3(BX, BY), 0, s>>>(z, dprev, r, info, nx, ny, first, partials,
+                                                       done);
+  else
+    k_fcg_dots<1><<<rg.g, dim3(BX, BY), 0, s>>>(z, dprev, r, info, nx, ny, first, partials, done);
+  int n = rg.g.x * rg.g.y;
   const double* pp = folded(s, partials, &n, done, 0);
   k_fcg_coef<<<1, 1024, 0, s>>>(pp, n, state, first, done);
 }
@@ -625,15 +725,19 @@ void launch_fcg_dots(hipStream_t s, const float* z, const float* dprev, const do
 void launch_fcg_update(hipStream_t s, const float* z, const float* dprev, float* dnext, double* p,
                        double* r, const uint8_t* info, float* r32, int nx, int ny, int first,
                        const double* state, double* partials, const int* done) {
-  dim3 g = reduce_grid(nx, ny);
-  k_fcg_update<<<g, dim3(BX, BY), 0, s>>>(z, dprev, dnext, p, r, info, r32, state, nx, ny, first,
-                                          partials, done);
+  const RGrid rg = reduce_grid_rows(nx, ny);
+  if (rg.rows == kRedRows)
+    k_fcg_update<kRedRows><<<rg.g, dim3(BX, BY), 0, s>>>(z, dprev, dnext, p, r, info, r32, state,
+                                                         nx, ny, first, partials, done);
+  else
+    k_fcg_update<1><<<rg.g, dim3(BX, BY), 0, s>>>(z, dprev, dnext, p, r, info, r32, state, nx, ny,
+                                                  first, partials, done);
 }
 
 void launch_fcg_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                      const double* sums0, double* host_sums, int* done, double scale,
                      double tol) {
-  dim3 g = reduce_grid(nx, ny);
+  const dim3 g = reduce_grid_rows(nx, ny).g;
   int n = g.x * g.y;
   partials = folded(s, partials, &n, done, 0);
   k_finalize_sum4<<<1, 1024, 0, s>>>(partials, n, sums, host_sums, done, scale, tol, 0, sums0);

@@ -320,37 +320,6 @@ __device__ __forceinline__ void forces_at(float* __restrict__ u, float* __restri
   }
 }
 
-// forces_at with buoyancy alone (no stir, no confinement, gain != 0): u is
-// unchanged, so only the interior v faces are touched; every load of a face
-// (its two cell labels, v and the heat stencil) is issued before the solid
-// test, and the stored value is forces_at's.
-__device__ __forceinline__ void buoyancy_at(float* __restrict__ v, const uint8_t* __restrict__ lbl,
-                                            const float* __restrict__ heat, int nx, int ny,
-                                            int ds, float gain, int i, int j) {
-  if (i >= nx || j < 1 || j > ny - 1) return;
-  const int k = i + j * nx;
-  const uint8_t below = lbl[k - nx], above = lbl[k];
-  float val = v[k];
-  Axis ax = clamped_axis(i * ds, 0.5f * ds - 0.5f, nx * ds);
-  Axis ay = clamped_axis(j * ds, -0.5f, ny * ds);
-  Quad g = gather4(heat, nx * ds, ax, ay);
-  if (below == SOLID || above == SOLID) return;
-  val += gain * bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
-  v[k] = val;
-}
-
-// divergence.ts:17-25, evaluated in FP64 from the FP32 faces.
-__device__ __forceinline__ void divergence_at(const float* __restrict__ u,
-                                              const float* __restrict__ v,
-                                              double* __restrict__ div, int nx, int ny,
-                                              double inv_h, int i, int j) {
-  if (i >= nx || j >= ny) return;
-  const float* ur = u + j * (nx + 1);
-  double d = ((double)ur[i + 1] - (double)ur[i]) + (double)v[i + (j + 1) * nx] -
-             (double)v[i + j * nx];
-  div[i + j * nx] = inv_h * d;
-}
-
 // subtractGradient.ts:40-61 with an FP64 pressure.
 __device__ __forceinline__ void subtract_at(float* __restrict__ u, float* __restrict__ v,
                                             const double* __restrict__ p,
@@ -370,24 +339,43 @@ __device__ __forceinline__ void subtract_at(float* __restrict__ u, float* __rest
 }
 
 // subtract_at for an index inside the face grid (i <= nx, j <= ny) of a grid
-// of at least 2 x 2 cells, with every load (the three cell labels and
-// pressures it can need, u and v) issued before the solid tests; the same
-// faces and values as subtract_at.
-__device__ __forceinline__ void subtract_eager_at(float* __restrict__ u, float* __restrict__ v,
-                                                  const double* __restrict__ p,
-                                                  const uint8_t* __restrict__ lbl, int nx, int ny,
-                                                  double gs, int i, int j) {
+// of at least 2 x 2 cells, split into its loads (the three cell labels and
+// pressures it can need, u and v: issued before any store) and its stores;
+// the same faces and values as subtract_at.
+struct SubLoad {
+  bool uw, vw;  // u / v face of this index is subtracted (interior, no solid side)
+  int ku, kv;
+  float u0, v0;
+  double p0, pw, ps;
+};
+
+__device__ __forceinline__ SubLoad subtract_load(const float* __restrict__ u,
+                                                 const float* __restrict__ v,
+                                                 const double* __restrict__ p,
+                                                 const uint8_t* __restrict__ lbl, int nx, int ny,
+                                                 int i, int j) {
+  SubLoad L;
   const bool uf = j < ny && i >= 1 && i <= nx - 1;
   const bool vf = i < nx && j >= 1 && j <= ny - 1;
-  if (!uf && !vf) return;
   const int c = min(i, nx - 1) + min(j, ny - 1) * nx;  // cell (i, j), in the domain here
   const int cw = i >= 1 ? c - 1 : c, cs = j >= 1 ? c - nx : c;  // (i-1, j), (i, j-1)
   const uint8_t l0 = lbl[c], lw = lbl[cw], ls = lbl[cs];
-  const double p0 = p[c], pw = p[cw], ps = p[cs];
-  const int ku = i + j * (nx + 1), kv = i + j * nx;
-  const float u0 = uf ? u[ku] : 0.f, v0 = vf ? v[kv] : 0.f;
-  if (uf && lw != SOLID && l0 != SOLID) u[ku] = (float)((double)u0 - gs * (p0 - pw));
-  if (vf && ls != SOLID && l0 != SOLID) v[kv] = (float)((double)v0 - gs * (p0 - ps));
+  L.p0 = p[c];
+  L.pw = p[cw];
+  L.ps = p[cs];
+  L.ku = i + j * (nx + 1);
+  L.kv = i + j * nx;
+  L.u0 = uf ? u[L.ku] : 0.f;
+  L.v0 = vf ? v[L.kv] : 0.f;
+  L.uw = uf && lw != SOLID && l0 != SOLID;
+  L.vw = vf && ls != SOLID && l0 != SOLID;
+  return L;
+}
+
+__device__ __forceinline__ void subtract_store(const SubLoad& L, float* __restrict__ u,
+                                               float* __restrict__ v, double gs) {
+  if (L.uw) u[L.ku] = (float)((double)L.u0 - gs * (L.p0 - L.pw));
+  if (L.vw) v[L.kv] = (float)((double)L.v0 - gs * (L.p0 - L.ps));
 }
 
 // boundaries.ts:115-143: index t owns u row t and v column t, replaying the
