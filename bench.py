@@ -93,8 +93,9 @@ def parse():
     ap.add_argument("--kernel", default=None,
                     help="kernel traced for the roofline (default: the largest device time "
                          "per step)")
-    ap.add_argument("--legs", default="c1,c2,c3,c4,learned",
-                    help="comma list of extra legs: c1, c2, c3, c4, learned (none: '')")
+    ap.add_argument("--legs", default="c1,c2,c3,c4,learned,node",
+                    help="comma list of extra legs: c1, c2, c3, c4, learned, node (none: '')")
+    ap.add_argument("--node-steps", type=int, default=200)
     ap.add_argument("--c2-steps", type=int, default=1000)
     ap.add_argument("--c3-steps", type=int, default=500)
     ap.add_argument("--c1-steps", type=int, default=200)
@@ -385,6 +386,51 @@ def run_c4(args):
     return out
 
 
+def run_node_c2(args):
+    """The Karman street 2048x1024 stepped through the N-API HipStepper from
+    Node (flowlab_amd/node/bench_step.js): the host upload, the one-step-stale
+    velocity readback into the host fields every step, the host CFL scan and
+    the profiler marks are all inside the timed loop, as a TypeScript host
+    pays them."""
+    import shutil
+    import subprocess
+    import tempfile
+    import numpy as np
+    from flowlab_amd import scenes as S
+    node = shutil.which("node")
+    script = os.path.join(ROOT, "flowlab_amd", "node", "bench_step.js")
+    if node is None or not os.path.exists(os.path.join(ROOT, "flowlab_amd", "node",
+                                                       "flowhip.node")):
+        return None
+    nx, ny = 2048, 1024
+    g, scene = S.karman_street(nx, ny)
+    f = S_fields(g, scene)
+    with tempfile.TemporaryDirectory() as d:
+        for name in ("u", "v", "p"):
+            np.asarray(getattr(f, name), "<f8").tofile(os.path.join(d, f"{name}.f64"))
+        np.concatenate(f.dye).astype("<f8").tofile(os.path.join(d, "dye.f64"))
+        f.label.tofile(os.path.join(d, "label.u8"))
+        case = dict(nx=nx, ny=ny, h=g.h, scheme="macCormack", solver=args.solver, tol=1e-6,
+                    cfl=0.5, dtMax=1 / 30)
+        with open(os.path.join(d, "case.json"), "w") as fh:
+            json.dump(case, fh)
+        r = subprocess.run([node, script, d, "10", str(args.node_steps)], capture_output=True,
+                           text=True, timeout=600)
+    if r.returncode != 0:
+        return {"error": r.stderr[-500:]}
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out["workload"] = (f"karman vortex street {nx}x{ny} through the N-API HipStepper from Node "
+                       f"{out.get('node')}, {SOLVER_NAMES[args.solver]}, tol 1e-6, CFL 0.5 from "
+                       "the host's one-step-stale velocity; upload, readback and host CFL scan "
+                       "inside the timed loop")
+    return out
+
+
+def S_fields(g, scene):
+    from flowlab_amd.simulation import scene_fields
+    return scene_fields(g, scene)
+
+
 def run_cpu_baseline(state, args):
     """One step of the headline workload by the FP64 C oracle (one thread),
     continued from the device's fields after the 200-step run, with the
@@ -447,6 +493,9 @@ def main():
     if "c4" in legs:
         log("leg C4")
         out_legs["C4_learned_warm_start_eval"] = run_c4(args)
+    if "node" in legs:
+        log("leg C2 through Node")
+        out_legs["C2_karman_2048x1024_node_boundary"] = run_node_c2(args)
     dom = h["dominant"]
     bpc = algo_bytes(dom, args.solver)
     k_avg_ms = h["k_ms"] / max(h["n_launch"], 1)
