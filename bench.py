@@ -426,6 +426,44 @@ def run_node_c2(args):
     return out
 
 
+def run_cpu_baseline_js(args):
+    """The reference's CPU path restated in plain JavaScript (FP64
+    Float64Array fields, one thread, oracle/js/cpuStep.js) on this host's
+    Node: the 2048x1024 Karman workload, one warm-up step, then two timed
+    steps, MG-DC to 1e-6."""
+    import shutil
+    import subprocess
+    import tempfile
+    import numpy as np
+    from flowlab_amd import scenes as S
+    node = shutil.which("node")
+    if node is None:
+        return None
+    nx, ny, warm, steps = 2048, 1024, 1, 2
+    g, scene = S.karman_street(nx, ny)
+    f = S_fields(g, scene)
+    with tempfile.TemporaryDirectory() as d:
+        for name in ("u", "v", "p"):
+            np.asarray(getattr(f, name), "<f8").tofile(os.path.join(d, f"{name}.f64"))
+        np.concatenate(f.dye).astype("<f8").tofile(os.path.join(d, "dye.f64"))
+        f.label.tofile(os.path.join(d, "label.u8"))
+        case = dict(nx=nx, ny=ny, h=g.h, dyeScale=1, scheme="macCormack", solver="mg-dc",
+                    pressureIters=100, tol=1e-6, cflTarget=0.5, dtMax=1 / 30, patch=None)
+        with open(os.path.join(d, "case.json"), "w") as fh:
+            json.dump(case, fh)
+        r = subprocess.run([node, os.path.join(ROOT, "oracle", "js", "cpuStep.js"), d, str(warm),
+                            str(steps)], capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        return {"error": r.stderr[-500:]}
+    o = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"value": o["cell_steps_per_s"], "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} steps of the {nx}x{ny} Karman workload after {warm} warm-up "
+                      f"step, the reference CPU path restated in JavaScript (Float64Array, one "
+                      f"thread) on Node {o['node']}, MG-DC to 1e-6 ({o['iters']} V-cycles, "
+                      f"{o['seconds']:.1f} s)",
+            "host_cpus": o["cpus"], "cpu_model": o["cpu_model"]}
+
+
 def S_fields(g, scene):
     from flowlab_amd.simulation import scene_fields
     return scene_fields(g, scene)
@@ -555,6 +593,8 @@ def main():
     if args.cpu_baseline and h["state"] is not None:
         log("cpu baseline")
         out["cpu_baseline"] = run_cpu_baseline(h["state"], args)
+        log("cpu baseline, JavaScript")
+        out["cpu_baseline_js"] = run_cpu_baseline_js(args)
     print(json.dumps(out))
 
 

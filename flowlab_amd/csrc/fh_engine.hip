@@ -125,7 +125,11 @@ struct fh_engine {
   bool timing = false;
   hipEvent_t ev[FH_NUM_PHASES + 1] = {};
   bool ev_valid = false;
-  std::vector<float> stage;
+  // pinned host staging of the FP32 <-> FP64 field conversions (uploads and
+  // readbacks), allocated at the first transfer with room for the largest
+  // field: DMA from pinned memory instead of the runtime's pageable path
+  float* stage = nullptr;
+  size_t stage_n = 0;
   // per-kernel tracing
   int trace_kid = 0;
   std::vector<hipEvent_t> trace_ev;
@@ -239,18 +243,33 @@ int dalloc(fh_engine* e, T** p, size_t n) {
   return FH_OK;
 }
 
+// The pinned staging buffer, sized for the largest field (u, v or 3 dye planes).
+int ensure_stage(fh_engine* e, size_t n) {
+  if (e->stage && n <= e->stage_n) return FH_OK;
+  size_t want = std::max(std::max(n_u(e), n_v(e)), 3 * n_d(e));
+  want = std::max(want, n);
+  if (e->stage) HIPCHK(hipHostFree(e->stage));
+  e->stage = nullptr;
+  e->stage_n = 0;
+  HIPCHK(hipHostMalloc((void**)&e->stage, want * sizeof(float), hipHostMallocDefault));
+  e->stage_n = want;
+  return FH_OK;
+}
+
 // f64 host -> f32 device
 int up32(fh_engine* e, float* dst, const double* src, size_t n) {
-  e->stage.resize(n);
+  int st = ensure_stage(e, n);
+  if (st != FH_OK) return st;
   for (size_t k = 0; k < n; k++) e->stage[k] = (float)src[k];
-  HIPCHK(hipMemcpyAsync(dst, e->stage.data(), n * sizeof(float), hipMemcpyHostToDevice, e->s));
+  HIPCHK(hipMemcpyAsync(dst, e->stage, n * sizeof(float), hipMemcpyHostToDevice, e->s));
   HIPCHK(hipStreamSynchronize(e->s));
   return FH_OK;
 }
 
 int down32(fh_engine* e, double* dst, const float* src, size_t n) {
-  e->stage.resize(n);
-  HIPCHK(hipMemcpyAsync(e->stage.data(), src, n * sizeof(float), hipMemcpyDeviceToHost, e->s));
+  int st = ensure_stage(e, n);
+  if (st != FH_OK) return st;
+  HIPCHK(hipMemcpyAsync(e->stage, src, n * sizeof(float), hipMemcpyDeviceToHost, e->s));
   HIPCHK(hipStreamSynchronize(e->s));
   for (size_t k = 0; k < n; k++) dst[k] = (double)e->stage[k];
   return FH_OK;
@@ -993,6 +1012,7 @@ void fh_destroy(fh_engine* e) {
   if (e->h_res) (void)hipHostFree(e->h_res);
   if (e->h_sums) (void)hipHostFree(e->h_sums);
   if (e->h_fmax) (void)hipHostFree(e->h_fmax);
+  if (e->stage) (void)hipHostFree(e->stage);
   for (int k = 0; k <= FH_NUM_PHASES; k++)
     if (e->ev[k]) (void)hipEventDestroy(e->ev[k]);
   for (hipEvent_t ev : e->trace_ev) (void)hipEventDestroy(ev);
