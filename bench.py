@@ -41,10 +41,13 @@ WORKLOAD_STEPS = 200
 
 # Algorithmic (compulsory) bytes per cell of one launch of each traceable
 # kernel: every array element read or written once, neighbour reuse perfect.
-#   maccormack_faces: u^n, v^n 8 + forward-pass u, v 8 + out u, v 8 + label 1.
-#   advect_faces: carrier = source u, v 8 + out u, v 8 + label 1.
+#   maccormack_faces: u^n, v^n 8 + forward-pass u, v 8 + out u, v 8 + label 1
+#     + the limiter stencils the forward pass recorded 1.
+#   advect_faces: carrier = source u, v 8 + out u, v 8 + label 1 + the
+#     recorded limiter stencils 1 (MacCormack's forward pass).
 #   maccormack_dye: u, v 8 + 3 channels of q^n and of the forward pass 24 +
-#     3 out 12 + label 1; advect_dye: u, v 8 + 3 in 12 + 3 out 12 + label 1.
+#     3 out 12 + label 1 + recorded stencils 1; advect_dye: u, v 8 + 3 in 12 +
+#     3 out 12 + label 1 + recorded stencils 1.
 #   dc_update: e 4 + p 8 read + 8 write + div 8 + label 1 + r32 4; under the
 #     flexible-CG solver the traced launch also writes the FP64 residual (+8).
 #   smooth_restrict_fine: b 4 + label 1 + x 4 written + coarse b 1.
@@ -58,8 +61,8 @@ WORKLOAD_STEPS = 200
 #     FP64 p 8 + label 1.
 #   rb_sweep_fine (one colour): half of a sweep's 13 (x 4 + 4, rhs 4, label 1).
 ALGO_BYTES_PER_CELL = {
-    "maccormack_faces": 25.0, "advect_faces": 17.0, "maccormack_dye": 45.0,
-    "advect_dye": 33.0, "dc_update": 33.0, "smooth_restrict_fine": 10.0,
+    "maccormack_faces": 26.0, "advect_faces": 18.0, "maccormack_dye": 46.0,
+    "advect_dye": 34.0, "dc_update": 33.0, "smooth_restrict_fine": 10.0,
     "prolong_smooth_fine": 14.0, "fcg_dots": 17.0, "fcg_update": 49.0, "divergence": 16.0,
     "forces": 13.0, "project": 25.0, "rb_sweep_fine": 6.5,
 }

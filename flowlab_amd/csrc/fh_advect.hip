@@ -177,7 +177,8 @@ __device__ __forceinline__ bool interior_tile(int x0, int y0, int nx, int ny) {
 template <int TY>
 __global__ __launch_bounds__(256) void k_advect_faces_lds(
     const float* __restrict__ uc, const float* __restrict__ vc, float* __restrict__ uo,
-    float* __restrict__ vo, const uint8_t* __restrict__ lbl, int nx, int ny, float c) {
+    float* __restrict__ vo, const uint8_t* __restrict__ lbl, int nx, int ny, float c,
+    uint8_t* __restrict__ sten) {
   constexpr int WN = Tile<TY>::WN;
   __shared__ float lds[2][WN];
   int bx, by;
@@ -192,7 +193,7 @@ __global__ __launch_bounds__(256) void k_advect_faces_lds(
       using F = decltype(tag);
       const F u = window_as<F>(lds, 0, wx0, wy0), v = window_as<F>(lds, 1, wx0, wy0);
       for (int r = 0; r < TY / BY; r++)
-        advect_faces_t(u, v, u, v, uo, vo, lbl, nx, ny, c, i, j + r * BY);
+        advect_faces_t(u, v, u, v, uo, vo, lbl, nx, ny, c, i, j + r * BY, sten);
     };
     if (interior_tile<TY>(x0, y0, nx, ny))
       run(LFieldI{});
@@ -200,7 +201,8 @@ __global__ __launch_bounds__(256) void k_advect_faces_lds(
       run(LField{});
   } else {
     const GField u{uc, nx + 1}, v{vc, nx};
-    for (int r = 0; r < TY / BY; r++) advect_faces_t(u, v, u, v, uo, vo, lbl, nx, ny, c, i, j + r * BY);
+    for (int r = 0; r < TY / BY; r++)
+      advect_faces_t(u, v, u, v, uo, vo, lbl, nx, ny, c, i, j + r * BY, sten);
   }
 }
 
@@ -209,7 +211,7 @@ template <int TY>
 __global__ __launch_bounds__(256) void k_maccormack_faces_lds(
     const float* __restrict__ un, const float* __restrict__ vn, const float* __restrict__ uh,
     const float* __restrict__ vh, float* __restrict__ uo, float* __restrict__ vo,
-    const uint8_t* __restrict__ lbl, int nx, int ny, float c) {
+    const uint8_t* __restrict__ lbl, int nx, int ny, float c, const uint8_t* __restrict__ sten) {
   constexpr int WN = Tile<TY>::WN;
   __shared__ float lds[4][WN];
   int bx, by;
@@ -217,15 +219,21 @@ __global__ __launch_bounds__(256) void k_maccormack_faces_lds(
   const int x0 = bx * TX, y0 = by * TY, wx0 = x0 - AH, wy0 = y0 - AH;
   const int tid = threadIdx.x + threadIdx.y * TX;
   const Win f[4] = {{un, nx + 1, ny}, {vn, nx, ny + 1}, {uh, nx + 1, ny}, {vh, nx, ny + 1}};
-  const float m = stage_windows<4>(lds, f, wx0, wy0, tid);
   const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
+  // the recorded limiter stencils, loaded with the windows
+  uint32_t st[TY / BY] = {};
+  if (sten)
+    for (int r = 0; r < TY / BY; r++)
+      if (i <= nx && j + r * BY <= ny) st[r] = sten[i + (j + r * BY) * (nx + 1)];
+  const float m = stage_windows<4>(lds, f, wx0, wy0, tid);
   if (within_reach(m, c, tid)) {
     auto run = [&](auto tag) {
       using F = decltype(tag);
       const F a = window_as<F>(lds, 0, wx0, wy0), b = window_as<F>(lds, 1, wx0, wy0),
               ah = window_as<F>(lds, 2, wx0, wy0), bh = window_as<F>(lds, 3, wx0, wy0);
       for (int r = 0; r < TY / BY; r++)
-        maccormack_faces_t(a, b, ah, bh, uo, vo, lbl, nx, ny, c, i, j + r * BY);
+        maccormack_faces_t(a, b, ah, bh, uo, vo, lbl, nx, ny, c, i, j + r * BY, sten != nullptr,
+                           st[r]);
     };
     if (interior_tile<TY>(x0, y0, nx, ny))
       run(LFieldI{});
@@ -244,7 +252,7 @@ template <int TY>
 __global__ __launch_bounds__(256) void k_advect_dye_lds(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ q,
     float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx, int ny, float c, float k0,
-    float k1, float k2) {
+    float k1, float k2, uint8_t* __restrict__ sten) {
   constexpr int WN = Tile<TY>::WN;
   __shared__ float lds[5][WN];
   int bx, by;
@@ -263,7 +271,7 @@ __global__ __launch_bounds__(256) void k_advect_dye_lds(
               q0 = window_as<F>(lds, 2, wx0, wy0), q1 = window_as<F>(lds, 3, wx0, wy0),
               q2 = window_as<F>(lds, 4, wx0, wy0);
       for (int r = 0; r < TY / BY; r++)
-        advect_dye_t(a, b, q0, q1, q2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i, j + r * BY);
+        advect_dye_t(a, b, q0, q1, q2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i, j + r * BY, sten);
     };
     if (interior_tile<TY>(x0, y0, nx, ny))
       run(LFieldI{});
@@ -272,7 +280,7 @@ __global__ __launch_bounds__(256) void k_advect_dye_lds(
   } else {
     const GField a{u, nx + 1}, b{v, nx}, q0{q, nx}, q1{q + plane, nx}, q2{q + 2 * plane, nx};
     for (int r = 0; r < TY / BY; r++)
-      advect_dye_t(a, b, q0, q1, q2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i, j + r * BY);
+      advect_dye_t(a, b, q0, q1, q2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i, j + r * BY, sten);
   }
 }
 
@@ -282,7 +290,7 @@ template <int TY>
 __global__ __launch_bounds__(256) void k_maccormack_dye_lds(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ qn,
     const float* __restrict__ qh, float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx,
-    int ny, float c, float k0, float k1, float k2) {
+    int ny, float c, float k0, float k1, float k2, const uint8_t* __restrict__ sten) {
   constexpr int WN = Tile<TY>::WN;
   __shared__ float lds[8][WN];
   int bx, by;
@@ -293,8 +301,13 @@ __global__ __launch_bounds__(256) void k_maccormack_dye_lds(
   const Win f[8] = {{u, nx + 1, ny},      {v, nx, ny + 1},          {qn, nx, ny},
                     {qn + plane, nx, ny}, {qn + 2 * plane, nx, ny}, {qh, nx, ny},
                     {qh + plane, nx, ny}, {qh + 2 * plane, nx, ny}};
-  const float m = stage_windows<8>(lds, f, wx0, wy0, tid);
   const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
+  // the recorded limiter stencils, loaded with the windows
+  uint32_t st[TY / BY] = {};
+  if (sten)
+    for (int r = 0; r < TY / BY; r++)
+      if (i < nx && j + r * BY < ny) st[r] = sten[i + (j + r * BY) * nx];
+  const float m = stage_windows<8>(lds, f, wx0, wy0, tid);
   if (within_reach(m, c, tid)) {
     auto run = [&](auto tag) {
       using F = decltype(tag);
@@ -304,7 +317,7 @@ __global__ __launch_bounds__(256) void k_maccormack_dye_lds(
               h1 = window_as<F>(lds, 6, wx0, wy0), h2 = window_as<F>(lds, 7, wx0, wy0);
       for (int r = 0; r < TY / BY; r++)
         maccormack_dye_t(a, b, n0, n1, n2, h0, h1, h2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i,
-                         j + r * BY);
+                         j + r * BY, sten != nullptr, st[r]);
     };
     if (interior_tile<TY>(x0, y0, nx, ny))
       run(LFieldI{});
@@ -326,10 +339,10 @@ static inline dim3 grid_t(int w, int h) {
 
 void launch_advect_faces(hipStream_t s, const float* uc, const float* vc, const float* us,
                          const float* vs, float* uo, float* vo, const uint8_t* lbl, int nx, int ny,
-                         float c, bool lds) {
+                         float c, bool lds, uint8_t* sten) {
   if (lds && us == uc && vs == vc)
     k_advect_faces_lds<kTyAdvectFaces><<<grid_t<kTyAdvectFaces>(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(uc, vc, uo, vo, lbl, nx,
-                                                                       ny, c);
+                                                                       ny, c, sten);
   else
     k_advect_faces<<<grid2(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(uc, vc, us, vs, uo, vo, lbl,
                                                                   nx, ny, c);
@@ -337,10 +350,10 @@ void launch_advect_faces(hipStream_t s, const float* uc, const float* vc, const 
 
 void launch_maccormack_faces(hipStream_t s, const float* un, const float* vn, const float* uh,
                              const float* vh, float* uo, float* vo, const uint8_t* lbl, int nx,
-                             int ny, float c, bool lds) {
+                             int ny, float c, bool lds, const uint8_t* sten) {
   if (lds)
     k_maccormack_faces_lds<kTyMacCormackFaces><<<grid_t<kTyMacCormackFaces>(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(un, vn, uh, vh, uo,
-                                                                           vo, lbl, nx, ny, c);
+                                                                           vo, lbl, nx, ny, c, sten);
   else
     k_maccormack_faces<<<grid2(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(un, vn, uh, vh, uo, vo,
                                                                       lbl, nx, ny, c);
@@ -348,10 +361,10 @@ void launch_maccormack_faces(hipStream_t s, const float* un, const float* vn, co
 
 void launch_advect_dye(hipStream_t s, const float* u, const float* v, const float* q, float* qo,
                        const uint8_t* lbl, int nx, int ny, int ds, float c, float k0, float k1,
-                       float k2, bool lds) {
+                       float k2, bool lds, uint8_t* sten) {
   if (lds && ds == 1)
     k_advect_dye_lds<kTyAdvectDye><<<grid_t<kTyAdvectDye>(nx, ny), dim3(BX, BY), 0, s>>>(u, v, q, qo, lbl, nx, ny, c, k0, k1,
-                                                             k2);
+                                                             k2, sten);
   else
     k_advect_dye<<<grid2(nx * ds, ny * ds), dim3(BX, BY), 0, s>>>(u, v, q, qo, lbl, nx, ny, ds,
                                                                   c, k0, k1, k2);
@@ -359,10 +372,10 @@ void launch_advect_dye(hipStream_t s, const float* u, const float* v, const floa
 
 void launch_maccormack_dye(hipStream_t s, const float* u, const float* v, const float* qn,
                            const float* qh, float* qo, const uint8_t* lbl, int nx, int ny, int ds,
-                           float c, float k0, float k1, float k2, bool lds) {
+                           float c, float k0, float k1, float k2, bool lds, const uint8_t* sten) {
   if (lds && ds == 1)
     k_maccormack_dye_lds<kTyMacCormackDye><<<grid_t<kTyMacCormackDye>(nx, ny), dim3(BX, BY), 0, s>>>(u, v, qn, qh, qo, lbl, nx, ny,
-                                                                 c, k0, k1, k2);
+                                                                 c, k0, k1, k2, sten);
   else
     k_maccormack_dye<<<grid2(nx * ds, ny * ds), dim3(BX, BY), 0, s>>>(u, v, qn, qh, qo, lbl, nx,
                                                                       ny, ds, c, k0, k1, k2);

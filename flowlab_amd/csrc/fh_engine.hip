@@ -72,6 +72,9 @@ struct fh_engine {
   float *u = nullptr, *v = nullptr, *u2 = nullptr, *v2 = nullptr, *uh = nullptr, *vh = nullptr;
   float *dye = nullptr, *dye2 = nullptr, *dyeh = nullptr, *w = nullptr;
   uint8_t* lbl = nullptr;
+  // MacCormack: the forward passes' stencil origins of the limiter (one byte
+  // per face-grid index / dye cell), read by the correction passes
+  uint8_t *sten_f = nullptr, *sten_d = nullptr;
   double *p = nullptr, *p2 = nullptr, *div = nullptr;
   // flexible CG (allocated by the first such solve): FP64 residual, two FP32
   // directions (ping-pong by cycle parity), {alpha, beta, <d, A d>}
@@ -815,11 +818,11 @@ int advect_velocity(fh_engine* e, int scheme, double dt) {
     {
       Trace t(e, FH_K_ADVECT_FACES);
       launch_advect_faces(e->s, e->u, e->v, e->u, e->v, e->uh, e->vh, e->lbl, e->nx, e->ny, c,
-                          e->advect_lds);
+                          e->advect_lds, e->advect_lds ? e->sten_f : nullptr);
     }
     Trace t(e, FH_K_MACCORMACK_FACES);
     launch_maccormack_faces(e->s, e->u, e->v, e->uh, e->vh, e->u2, e->v2, e->lbl, e->nx, e->ny,
-                            c, e->advect_lds);
+                            c, e->advect_lds, e->advect_lds ? e->sten_f : nullptr);
   } else {
     Trace t(e, FH_K_ADVECT_FACES);
     launch_advect_faces(e->s, e->u, e->v, e->u, e->v, e->u2, e->v2, e->lbl, e->nx, e->ny, c,
@@ -862,14 +865,16 @@ int dye_phase(fh_engine* e, int scheme, double dt, const double keep[3]) {
   float k0 = keep ? (float)keep[0] : 1.f, k1 = keep ? (float)keep[1] : 1.f,
         k2 = keep ? (float)keep[2] : 1.f;
   if (scheme == FH_MACCORMACK) {
+    // the dye LDS kernels (dye scale 1) record and read the limiter stencils
+    uint8_t* sten_d = (e->advect_lds && e->ds == 1) ? e->sten_d : nullptr;
     {
       Trace t(e, FH_K_ADVECT_DYE);
       launch_advect_dye(e->s, e->u, e->v, e->dye, e->dyeh, e->lbl, e->nx, e->ny, e->ds, c, 1.f,
-                        1.f, 1.f, e->advect_lds);
+                        1.f, 1.f, e->advect_lds, sten_d);
     }
     Trace t(e, FH_K_MACCORMACK_DYE);
     launch_maccormack_dye(e->s, e->u, e->v, e->dye, e->dyeh, e->dye2, e->lbl, e->nx, e->ny, e->ds,
-                          c, k0, k1, k2, e->advect_lds);
+                          c, k0, k1, k2, e->advect_lds, sten_d);
   } else {
     Trace t(e, FH_K_ADVECT_DYE);
     launch_advect_dye(e->s, e->u, e->v, e->dye, e->dye2, e->lbl, e->nx, e->ny, e->ds, c, k0, k1,
@@ -930,6 +935,7 @@ fh_engine* fh_create(int nx, int ny, int dye_scale, double h) {
       dalloc(e, &e->v2, nv) || dalloc(e, &e->uh, nu) || dalloc(e, &e->vh, nv) ||
       dalloc(e, &e->dye, 3 * nd) || dalloc(e, &e->dye2, 3 * nd) || dalloc(e, &e->dyeh, 3 * nd) ||
       dalloc(e, &e->w, nc) || dalloc(e, &e->lbl, nc) || dalloc(e, &e->p, nc) ||
+      dalloc(e, &e->sten_f, (size_t)(nx + 1) * (ny + 1)) || dalloc(e, &e->sten_d, nd) ||
       dalloc(e, &e->p2, nc) || dalloc(e, &e->div, nc))
     return bail();
   e->nlev_ref = level_count(nx, ny, 4);
@@ -992,6 +998,8 @@ void fh_destroy(fh_engine* e) {
   for (double* p : dbufs)
     if (p) (void)hipFree(p);
   if (e->lbl) (void)hipFree(e->lbl);
+  if (e->sten_f) (void)hipFree(e->sten_f);
+  if (e->sten_d) (void)hipFree(e->sten_d);
   if (e->cells) (void)hipFree(e->cells);
   if (e->counter) (void)hipFree(e->counter);
   if (e->done) (void)hipFree(e->done);

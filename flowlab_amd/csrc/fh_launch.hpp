@@ -19,25 +19,32 @@ struct StirBox {
 // ---- advection (fh_advect.hip) -------------------------------------------
 // lds: stage the inputs through LDS windows (faces always; dye when ds == 1;
 // advect_faces only when the carrier is the source).  Same bits either way.
+// sten: the MacCormack forward pass (advect_faces, advect_dye with lds)
+// records the limiter's stencil origins there ((nx+1) x (ny+1) bytes for
+// the faces, dnx x dny for the dye), and the correction pass given the same
+// buffer reads them instead of tracing forward again; null: not recorded /
+// traced again.  A correction may take sten only from a forward pass of the
+// same velocity, labels and c that ran with lds.
 // Semi-Lagrangian pass over every u and v face: out = src at the RK2
 // backtrace along (uc, vc) with c = dt/h.  advect.ts:72-112.
 void launch_advect_faces(hipStream_t s, const float* uc, const float* vc, const float* us,
                          const float* vs, float* uo, float* vo, const uint8_t* lbl, int nx, int ny,
-                         float c, bool lds);
+                         float c, bool lds, uint8_t* sten = nullptr);
 // MacCormack reverse pass fused with the correction and limiter.
 // advect.ts:177-210.
 void launch_maccormack_faces(hipStream_t s, const float* un, const float* vn, const float* uh,
                              const float* vh, float* uo, float* vo, const uint8_t* lbl, int nx,
-                             int ny, float c, bool lds);
+                             int ny, float c, bool lds, const uint8_t* sten = nullptr);
 // Three dye channels on one backtrace, scaled by keep[c] at the write.
 // advect.ts:241-270 (+ the decay of simulation.ts:569-576).
 void launch_advect_dye(hipStream_t s, const float* u, const float* v, const float* q, float* qo,
                        const uint8_t* lbl, int nx, int ny, int ds, float c, float k0, float k1,
-                       float k2, bool lds);
+                       float k2, bool lds, uint8_t* sten = nullptr);
 // advect.ts:291-311 for three channels, times keep[c].
 void launch_maccormack_dye(hipStream_t s, const float* u, const float* v, const float* qn,
                            const float* qh, float* qo, const uint8_t* lbl, int nx, int ny, int ds,
-                           float c, float k0, float k1, float k2, bool lds);
+                           float c, float k0, float k1, float k2, bool lds,
+                           const uint8_t* sten = nullptr);
 
 // ---- forces, divergence, projection (fh_project.hip) ----------------------
 void launch_vorticity(hipStream_t s, const float* u, const float* v, float* w, int nx, int ny,

@@ -18,16 +18,35 @@ namespace fh {
 // Bodies are generic over field accessors (GField / LField, fh_device.hpp);
 // the pointer versions read global memory and give the same bits.
 
+// MacCormack's limiter clamps to the source stencil at the forward
+// backtrace (advect.ts:200-209), the stencil the forward pass has just
+// sampled.  The forward pass can record each face's (or dye cell's) stencil
+// origin relative to its own index in 2 bits per axis (offset + 2), and the
+// correction pass reads it instead of tracing forward again.  The offsets lie
+// in [-2, 1] whenever the correction's workgroup passes its reach test (every
+// backtrace of the tile then moves at most kReach = 1.25 cells, and the
+// domain clamp only moves an origin towards the face), which is the only
+// case that reads them.
+__device__ __forceinline__ uint32_t sten2(int d0, int d1) {
+  return (uint32_t)((d0 + 2) & 3) | ((uint32_t)((d1 + 2) & 3) << 2);
+}
+__device__ __forceinline__ int sten_d0(uint32_t s) { return (int)(s & 3) - 2; }
+__device__ __forceinline__ int sten_d1(uint32_t s) { return (int)((s >> 2) & 3) - 2; }
+
 // advect.ts:72-112 -- one thread index covers u(i,j) when j<ny and v(i,j)
-// when i<nx, over (nx+1) x (ny+1).  uc/vc carry, us/vs are advected.
+// when i<nx, over (nx+1) x (ny+1).  uc/vc carry, us/vs are advected.  sten
+// (may be null): the stencil origins of both faces' samples, one byte per
+// thread index (u face in bits 0-3, v face in bits 4-7; 0 for solid faces).
 template <class CU, class CV, class SU, class SV>
 __device__ __forceinline__ void advect_faces_t(const CU& uc, const CV& vc, const SU& us,
                                                const SV& vs, float* __restrict__ uo,
                                                float* __restrict__ vo,
                                                const uint8_t* __restrict__ lbl, int nx, int ny,
-                                               float c, int i, int j) {
+                                               float c, int i, int j,
+                                               uint8_t* __restrict__ sten = nullptr) {
   if (i > nx || j > ny) return;
   Pos bx, by;
+  uint32_t st = 0;
   if (j < ny) {
     int k = i + j * (nx + 1);
     if (solid_or_outside(lbl, nx, ny, i - 1, j) || solid_or_outside(lbl, nx, ny, i, j)) {
@@ -36,7 +55,12 @@ __device__ __forceinline__ void advect_faces_t(const CU& uc, const CV& vc, const
       Pos X{i, 0.f}, Y{j, 0.5f};
       float v0 = sample_vf(vc, nx, ny, X, Y);
       backtracef(uc, vc, nx, ny, X, Y, uc.at(i, j), v0, c, &bx, &by);
-      uo[k] = sample_uf(us, nx, ny, bx, by);
+      // sample_uf, with the stencil origin kept
+      Axis ax = axis_t<SU::kClamp>(bx.i, bx.f, nx + 1);
+      Axis ay = axis_t<SU::kClamp>(by.i, by.f - 0.5f, ny);
+      Quad g = gather4f(us, ax, ay);
+      uo[k] = bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
+      st = sten2(ax.i0 - i, ay.i0 - j);
     }
   }
   if (i < nx) {
@@ -47,9 +71,15 @@ __device__ __forceinline__ void advect_faces_t(const CU& uc, const CV& vc, const
       Pos X{i, 0.5f}, Y{j, 0.f};
       float u0 = sample_uf(uc, nx, ny, X, Y);
       backtracef(uc, vc, nx, ny, X, Y, u0, vc.at(i, j), c, &bx, &by);
-      vo[k] = sample_vf(vs, nx, ny, bx, by);
+      // sample_vf, with the stencil origin kept
+      Axis ax = axis_t<SV::kClamp>(bx.i, bx.f - 0.5f, nx);
+      Axis ay = axis_t<SV::kClamp>(by.i, by.f, ny + 1);
+      Quad g = gather4f(vs, ax, ay);
+      vo[k] = bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
+      st |= sten2(ax.i0 - i, ay.i0 - j) << 4;
     }
   }
+  if (sten) sten[i + j * (nx + 1)] = (uint8_t)st;
 }
 
 __device__ __forceinline__ void advect_faces_at(const float* __restrict__ uc,
@@ -66,13 +96,17 @@ __device__ __forceinline__ void advect_faces_at(const float* __restrict__ uc,
 // advect.ts:177-210 fused: the reverse pass (carrier u^n, source uHat, -dt)
 // is evaluated for this face only, then corrected = uHat + (u^n - uBar)/2 is
 // clamped to the u^n stencil at the forward backtrace.  Each face reads only
-// its own uBar, so fusing changes no value.
+// its own uBar, so fusing changes no value.  has_sten: st is this index's
+// byte of the forward pass's stencil origins (advect_faces_t), used instead
+// of tracing forward again; the caller sets it only where the offsets are
+// known to fit.
 template <class NU, class NV, class HU, class HV>
 __device__ __forceinline__ void maccormack_faces_t(const NU& un, const NV& vn, const HU& uh,
                                                    const HV& vh, float* __restrict__ uo,
                                                    float* __restrict__ vo,
                                                    const uint8_t* __restrict__ lbl, int nx, int ny,
-                                                   float c, int i, int j) {
+                                                   float c, int i, int j, bool has_sten = false,
+                                                   uint32_t st = 0) {
   if (i > nx || j > ny) return;
   Pos rx, ry, fx, fy;
   if (j < ny) {
@@ -86,9 +120,15 @@ __device__ __forceinline__ void maccormack_faces_t(const NU& un, const NV& vn, c
       backtracef(un, vn, nx, ny, X, Y, u0, v0, -c, &rx, &ry);
       float ubar = sample_uf(uh, nx, ny, rx, ry);
       float corrected = uh.at(i, j) + 0.5f * (u0 - ubar);
-      backtracef(un, vn, nx, ny, X, Y, u0, v0, c, &fx, &fy);
-      Axis ax = axis_t<NU::kClamp>(fx.i, fx.f, nx + 1);
-      Axis ay = axis_t<NU::kClamp>(fy.i, fy.f - 0.5f, ny);
+      Axis ax, ay;
+      if (has_sten) {
+        ax = {i + sten_d0(st), 0.f};
+        ay = {j + sten_d1(st), 0.f};
+      } else {
+        backtracef(un, vn, nx, ny, X, Y, u0, v0, c, &fx, &fy);
+        ax = axis_t<NU::kClamp>(fx.i, fx.f, nx + 1);
+        ay = axis_t<NU::kClamp>(fy.i, fy.f - 0.5f, ny);
+      }
       uo[k] = clamp_to_quad(corrected, gather4f(un, ax, ay));
     }
   }
@@ -103,9 +143,15 @@ __device__ __forceinline__ void maccormack_faces_t(const NU& un, const NV& vn, c
       backtracef(un, vn, nx, ny, X, Y, u0, v0, -c, &rx, &ry);
       float vbar = sample_vf(vh, nx, ny, rx, ry);
       float corrected = vh.at(i, j) + 0.5f * (v0 - vbar);
-      backtracef(un, vn, nx, ny, X, Y, u0, v0, c, &fx, &fy);
-      Axis ax = axis_t<NV::kClamp>(fx.i, fx.f - 0.5f, nx);
-      Axis ay = axis_t<NV::kClamp>(fy.i, fy.f, ny + 1);
+      Axis ax, ay;
+      if (has_sten) {
+        ax = {i + sten_d0(st >> 4), 0.f};
+        ay = {j + sten_d1(st >> 4), 0.f};
+      } else {
+        backtracef(un, vn, nx, ny, X, Y, u0, v0, c, &fx, &fy);
+        ax = axis_t<NV::kClamp>(fx.i, fx.f - 0.5f, nx);
+        ay = axis_t<NV::kClamp>(fy.i, fy.f, ny + 1);
+      }
       vo[k] = clamp_to_quad(corrected, gather4f(vn, ax, ay));
     }
   }
@@ -129,13 +175,15 @@ __device__ __forceinline__ Pos dye_pos(int i, int ds) {
 
 // advect.ts:241-270 for three channels sharing one backtrace; out *= keep.
 // q0..q2 are the channels as accessors over the dye grid (dnx x dny); the
-// output planes are qo, qo + plane, qo + 2 plane.
+// output planes are qo, qo + plane, qo + 2 plane.  sten (may be null): one
+// byte per cell, the origin of the stencil MacCormack's limiter clamps to
+// (maccormack_dye_t's forward trace; 0 for solid cells).
 template <class CU, class CV, class Q>
 __device__ __forceinline__ void advect_dye_t(const CU& u, const CV& v, const Q& q0, const Q& q1,
                                              const Q& q2, float* __restrict__ qo,
                                              const uint8_t* __restrict__ lbl, int nx, int ny,
                                              int ds, float c, float k0, float k1, float k2, int i,
-                                             int j) {
+                                             int j, uint8_t* __restrict__ sten = nullptr) {
   int dnx = nx * ds, dny = ny * ds;
   if (i >= dnx || j >= dny) return;
   size_t plane = (size_t)dnx * dny;
@@ -144,6 +192,7 @@ __device__ __forceinline__ void advect_dye_t(const CU& u, const CV& v, const Q& 
     qo[k] = q0.at(i, j) * k0;
     qo[k + plane] = q1.at(i, j) * k1;
     qo[k + 2 * plane] = q2.at(i, j) * k2;
+    if (sten) sten[k] = 0;
     return;
   }
   Pos X = dye_pos(i, ds), Y = dye_pos(j, ds);
@@ -151,9 +200,12 @@ __device__ __forceinline__ void advect_dye_t(const CU& u, const CV& v, const Q& 
   float v0 = sample_vf(v, nx, ny, X, Y);
   Pos bx, by;
   backtracef(u, v, nx, ny, X, Y, u0, v0, c, &bx, &by);
-  Pos dx{i, 0.5f + ds * (bx.f - X.f)}, dy{j, 0.5f + ds * (by.f - Y.f)};
-  Axis ax = axis_t<Q::kClamp>(dx.i, dx.f - 0.5f, dnx);
-  Axis ay = axis_t<Q::kClamp>(dy.i, dy.f - 0.5f, dny);
+  // the sample at the backtrace: dye cell i's centre sits at i + 1/2 in
+  // dye-cell units, so the stencil axis is i + ds (bx - X) -- the axes the
+  // limiter of maccormack_dye_t clamps with, recorded in sten
+  Axis ax = axis_t<Q::kClamp>(i, ds * (bx.f - X.f), dnx);
+  Axis ay = axis_t<Q::kClamp>(j, ds * (by.f - Y.f), dny);
+  if (sten) sten[k] = (uint8_t)sten2(ax.i0 - i, ay.i0 - j);
   Quad g0 = gather4f(q0, ax, ay);
   Quad g1 = gather4f(q1, ax, ay);
   Quad g2 = gather4f(q2, ax, ay);
@@ -175,13 +227,16 @@ __device__ __forceinline__ void advect_dye_at(const float* __restrict__ u,
 }
 
 // advect.ts:291-311 for three channels, fused as for the faces; out *= keep.
+// has_sten: st is this cell's byte of the forward pass's clamp stencil
+// origins (advect_dye_t), used instead of tracing forward again.
 template <class CU, class CV, class Q>
 __device__ __forceinline__ void maccormack_dye_t(const CU& u, const CV& v, const Q& n0,
                                                  const Q& n1, const Q& n2, const Q& h0,
                                                  const Q& h1, const Q& h2, float* __restrict__ qo,
                                                  const uint8_t* __restrict__ lbl, int nx, int ny,
                                                  int ds, float c, float k0, float k1, float k2,
-                                                 int i, int j) {
+                                                 int i, int j, bool has_sten = false,
+                                                 uint32_t st = 0) {
   int dnx = nx * ds, dny = ny * ds;
   if (i >= dnx || j >= dny) return;
   size_t plane = (size_t)dnx * dny;
@@ -205,9 +260,14 @@ __device__ __forceinline__ void maccormack_dye_t(const CU& u, const CV& v, const
   float c0 = h0.at(i, j) + 0.5f * (n0.at(i, j) - bilerp(r0.a, r0.b, r0.c, r0.d, ax.f, ay.f));
   float c1 = h1.at(i, j) + 0.5f * (n1.at(i, j) - bilerp(r1.a, r1.b, r1.c, r1.d, ax.f, ay.f));
   float c2 = h2.at(i, j) + 0.5f * (n2.at(i, j) - bilerp(r2.a, r2.b, r2.c, r2.d, ax.f, ay.f));
-  backtracef(u, v, nx, ny, X, Y, u0, v0, c, &bx, &by);
-  ax = axis_t<Q::kClamp>(i, ds * (bx.f - X.f), dnx);
-  ay = axis_t<Q::kClamp>(j, ds * (by.f - Y.f), dny);
+  if (has_sten) {
+    ax = {i + sten_d0(st), 0.f};
+    ay = {j + sten_d1(st), 0.f};
+  } else {
+    backtracef(u, v, nx, ny, X, Y, u0, v0, c, &bx, &by);
+    ax = axis_t<Q::kClamp>(i, ds * (bx.f - X.f), dnx);
+    ay = axis_t<Q::kClamp>(j, ds * (by.f - Y.f), dny);
+  }
   qo[k] = clamp_to_quad(c0, gather4f(n0, ax, ay)) * k0;
   qo[k + plane] = clamp_to_quad(c1, gather4f(n1, ax, ay)) * k1;
   qo[k + 2 * plane] = clamp_to_quad(c2, gather4f(n2, ax, ay)) * k2;

@@ -60,6 +60,15 @@ int main() {
                               b1.data(), lbl.data(), nx, ny, c, true);
       fails += cmp("maccormack_faces u", a0.data(), a1.data(), nu);
       fails += cmp("maccormack_faces v", b0.data(), b1.data(), nv);
+      // the correction reading the forward pass's recorded limiter stencils
+      std::vector<uint8_t> sf((size_t)(nx + 1) * (ny + 1), 0xff);
+      std::vector<float> fa(nu), fb(nv);
+      launch_advect_faces(nullptr, u.data(), v.data(), u.data(), v.data(), fa.data(), fb.data(),
+                          lbl.data(), nx, ny, c, true, sf.data());
+      launch_maccormack_faces(nullptr, u.data(), v.data(), uh.data(), vh.data(), a1.data(),
+                              b1.data(), lbl.data(), nx, ny, c, true, sf.data());
+      fails += cmp("maccormack_faces u sten", a0.data(), a1.data(), nu);
+      fails += cmp("maccormack_faces v sten", b0.data(), b1.data(), nv);
       launch_advect_dye(nullptr, u.data(), v.data(), q.data(), o0.data(), lbl.data(), nx, ny, 1, c,
                         0.9f, 1.f, 0.7f, false);
       launch_advect_dye(nullptr, u.data(), v.data(), q.data(), o1.data(), lbl.data(), nx, ny, 1, c,
@@ -70,6 +79,13 @@ int main() {
       launch_maccormack_dye(nullptr, u.data(), v.data(), q.data(), qh.data(), o1.data(),
                             lbl.data(), nx, ny, 1, c, 0.9f, 1.f, 0.7f, true);
       fails += cmp("maccormack_dye", o0.data(), o1.data(), 3 * nc);
+      std::vector<uint8_t> sd(nc, 0xff);
+      std::vector<float> fo(3 * nc);
+      launch_advect_dye(nullptr, u.data(), v.data(), q.data(), fo.data(), lbl.data(), nx, ny, 1, c,
+                        1.f, 1.f, 1.f, true, sd.data());
+      launch_maccormack_dye(nullptr, u.data(), v.data(), q.data(), qh.data(), o1.data(),
+                            lbl.data(), nx, ny, 1, c, 0.9f, 1.f, 0.7f, true, sd.data());
+      fails += cmp("maccormack_dye sten", o0.data(), o1.data(), 3 * nc);
     }
   printf("staged blocks %ld, global blocks %ld, checked LDS reads %ld, clamp-free axes %ld\n",
          emu_fast_blocks, emu_slow_blocks, emu_lds_reads, emu_axes);
