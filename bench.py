@@ -48,8 +48,11 @@ WORKLOAD_STEPS = 200
 #   maccormack_dye: u, v 8 + 3 channels of q^n and of the forward pass 24 +
 #     3 out 12 + label 1 + recorded stencils 1; advect_dye: u, v 8 + 3 in 12 +
 #     3 out 12 + label 1 + recorded stencils 1.
-#   dc_update: e 4 + p 8 read + 8 write + div 8 + label 1 + r32 4; under the
-#     flexible-CG solver the traced launch also writes the FP64 residual (+8).
+#   dc_update: a cycle's correction pass reads e 4 + p 8 + div 8 + label 1 and
+#     writes p 8 + r32 4 (33); the solve's initial defect pass has no e and
+#     writes no p (21).  Under the flexible-CG solver both also write the FP64
+#     residual (+8).  A step's launches are one initial pass and one per
+#     cycle; the per-launch figure is their mean.
 #   smooth_restrict_fine: b 4 + label 1 + x 4 written + coarse b 1.
 #   prolong_smooth_fine: x 4 + b 4 + label 1 + coarse e 1 + x 4 written.
 #   fcg_dots: z 4 + previous direction 4 + r 8 + info 1.
@@ -76,11 +79,15 @@ SOLVER_NAMES = {
 PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_plume_8192.json")
 
 
-def algo_bytes(kernel, solver):
-    """Algorithmic bytes per cell of one traced launch of `kernel`."""
+def algo_bytes(kernel, solver, launches_per_step=2.0):
+    """Algorithmic bytes per cell of one traced launch of `kernel` (for
+    dc_update the mean over a step's initial pass and `launches_per_step` - 1
+    correction passes)."""
     b = ALGO_BYTES_PER_CELL.get(kernel)
-    if kernel == "dc_update" and solver == "mg-fcg":
-        b += 8.0
+    if kernel == "dc_update":
+        r64 = 8.0 if solver == "mg-fcg" else 0.0
+        n = max(launches_per_step, 1.0)
+        b = (21.0 + r64 + (n - 1.0) * (b + r64)) / n
     return b
 
 
@@ -110,6 +117,9 @@ def parse():
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first run 3 device-to-device copies of this many MiB (the known-size "
                          "copy kernel that scales hardware traffic counters)")
+    ap.add_argument("--pmc-summary", default=None, metavar="FETCH_CSV,WRITE_CSV",
+                    help="no GPU: fold the counter_collection.csv files of a FETCH_SIZE and a "
+                         "WRITE_SIZE pass (run with --copy-mb) into the traffic table")
     return ap.parse_args()
 
 
@@ -128,6 +138,84 @@ def measured_traffic(kernel, n):
         return None
     k = d["kernels"].get(kernel)
     return None if k is None else k["traffic_bytes"]
+
+
+# rocprofv3 kernel names (template arguments and parameter lists dropped) of
+# the kernels whose traffic the table reports
+PMC_KERNELS = {
+    "k_maccormack_dye_lds": "maccormack_dye", "k_maccormack_faces_lds": "maccormack_faces",
+    "k_advect_dye_lds": "advect_dye", "k_advect_faces_lds": "advect_faces",
+    "k_dc_update": "dc_update", "k_fcg_update": "fcg_update", "k_fcg_dots": "fcg_dots",
+    "k_smooth_restrict": "smooth_restrict_fine", "k_prolong_smooth": "prolong_smooth_fine",
+    "k_subtract_outflow": "project", "k_buoyancy": "forces", "k_divergence": "divergence",
+}
+COPY_KERNEL = "__amd_rocclr_copyBuffer"
+
+
+def pmc_summary(args):
+    """HBM traffic per launch from a FETCH_SIZE pass and a WRITE_SIZE pass
+    (KiB per dispatch) of `bench.py --copy-mb M ...`.  Each counter is scaled
+    by what it reads for the M MiB device-to-device copies (M MiB moved each
+    way); each kernel's row is its dispatches at the largest grid (the fine
+    level for the multigrid kernels).  Writes the JSON the roofline's
+    `traffic` reads and prints the table as markdown."""
+    import csv
+    import re
+    from collections import defaultdict
+    fetch_csv, write_csv = args.pmc_summary.split(",")
+
+    def load(path, counter):
+        out = defaultdict(list)
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if r["Counter_Name"] != counter:
+                    raise SystemExit(f"{path}: counter {r['Counter_Name']}, expected {counter}")
+                out[(r["Kernel_Name"], int(r["Grid_Size"]))].append(float(r["Counter_Value"]))
+        return out
+
+    F, W = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    copy_kib = args.copy_mb * 1024
+    if copy_kib <= 0:
+        raise SystemExit("--pmc-summary needs the --copy-mb the passes ran with")
+    # the copies of the known size: every copy dispatch of that many work-items or more
+    big = max(g for (k, g) in F if k.startswith(COPY_KERNEL))
+    cf = [v for (k, g), vs in F.items() if k.startswith(COPY_KERNEL) and g == big for v in vs]
+    cw = [v for (k, g), vs in W.items() if k.startswith(COPY_KERNEL) and g == big for v in vs]
+    sf, sw = copy_kib / (sum(cf) / len(cf)), copy_kib / (sum(cw) / len(cw))
+
+    def short(name):
+        return re.sub(r"\(.*", "", name.replace("void ", "").replace("fh::", ""))
+
+    pick = {}
+    for (kn, g) in F:
+        key = PMC_KERNELS.get(re.sub(r"<.*", "", short(kn)))
+        if key and (key not in pick or pick[key][1] < g):
+            pick[key] = (kn, g)
+    n = args.n
+    cells = n * n
+    out = {"grid": [n, n], "solver": args.solver, "copy_mb": args.copy_mb,
+           "copy_scale_fetch": sf, "copy_scale_write": sw, "kernels": {}}
+    rows = []
+    for key, (kn, g) in pick.items():
+        fv, wv = F[(kn, g)], W.get((kn, g), [0.0])
+        fb = sum(fv) / len(fv) * 1024 * sf
+        wb = sum(wv) / len(wv) * 1024 * sw
+        # dc_update: one initial pass and one cycle per solve (flexible CG)
+        a = algo_bytes(key, args.solver, 2.0) * cells
+        out["kernels"][key] = {"rocprof_name": short(kn), "dispatches": len(fv),
+                               "fetch_bytes": fb, "write_bytes": wb, "traffic_bytes": fb + wb,
+                               "algorithmic_bytes": a, "ratio": (fb + wb) / a}
+        rows.append((short(kn), len(fv), fb / 1e6, wb / 1e6, (fb + wb) / 1e6, a / 1e6,
+                     (fb + wb) / a))
+    with open(PMC_TRAFFIC, "w") as f:
+        json.dump(out, f, indent=1)
+    print(f"copy {args.copy_mb} MiB: fetch x{sf:.4f}, write x{sw:.4f}")
+    print("| kernel | dispatches | fetched MB | written MB | traffic MB | algorithmic MB | "
+          "traffic / algorithmic |")
+    print("|---|---|---|---|---|---|---|")
+    for r in sorted(rows, key=lambda r: -r[4]):
+        print(f"| `{r[0]}` | {r[1]} | {r[2]:.1f} | {r[3]:.1f} | {r[4]:.1f} | {r[5]:.1f} | "
+              f"{r[6]:.3f} |")
 
 
 # ---- single-grid legs ---------------------------------------------------------
@@ -181,10 +269,11 @@ def kernel_table(sim, cells, solver, steps=2):
         if not n:
             continue
         avg = ms / n
-        gbs = algo_bytes(k, solver) * cells / (avg * 1e-3) / 1e9
+        bpc = algo_bytes(k, solver, n / steps)
+        gbs = bpc * cells / (avg * 1e-3) / 1e9
         out[k] = {"avg_launch_ms": avg, "launches_per_step": n / steps,
                   "ms_per_step": ms / steps, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
-                  "algorithmic_bytes_per_cell": algo_bytes(k, solver)}
+                  "algorithmic_bytes_per_cell": bpc}
     return out
 
 
@@ -333,9 +422,9 @@ def run_c1(args):
 
 
 def run_c3(args):
-    """64 decaying-turbulence scenes of uneven size (256^2 .. 1024^2), one
-    engine and stream per scene stepped from a host thread pool; dt per scene
-    at CFL 0.5; pressure by mg-fcg to 1e-6; 500 steps."""
+    """64 decaying-turbulence scenes of uneven size (256^2 .. 1024^2), each
+    phase one launch over every scene's tiles; dt per scene at CFL 0.5;
+    pressure by mg-fcg to 1e-6 per scene; 500 steps."""
     from flowlab_amd import scenes as S
     from flowlab_amd.batch import HipBatch
     scenes = S.turbulence_batch(64)
@@ -365,9 +454,10 @@ def run_c3(args):
                     f"MacCormack, {SOLVER_NAMES['mg-fcg']}, tol 1e-6, CFL 0.5, {K} steps",
         "cells": cells, "steps": K, "value": cells * K / elapsed, "unit": "cell-steps/s",
         "ms_per_step": elapsed * 1e3 / K, "vcycles_per_solve_mean": sum(iters) / len(iters),
-        "max_relative_residual": res_max, "largest_scene_tiles": tune["tiles"],
-        "execution": "one engine and stream per scene, 16 host threads taking scenes largest "
-                     "first",
+        "max_relative_residual": res_max, "mg_tiles": tune["tiles"],
+        "execution": "one launch per phase over a (scene, block) work list of all scenes, one "
+                     "stream; per-scene convergence flags on the device, one host read per "
+                     "cycle",
     }
 
 
@@ -502,6 +592,9 @@ def run_cpu_baseline(state, args):
 
 def main():
     args = parse()
+    if args.pmc_summary:
+        pmc_summary(args)
+        return
     if args.gpus != 1:
         print(f"bench.py: the solver runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design note, not a measured path)",
@@ -538,7 +631,7 @@ def main():
         log("leg C2 through Node")
         out_legs["C2_karman_2048x1024_node_boundary"] = run_node_c2(args)
     dom = h["dominant"]
-    bpc = algo_bytes(dom, args.solver)
+    bpc = algo_bytes(dom, args.solver, h["n_launch"] / max(K, 1))
     k_avg_ms = h["k_ms"] / max(h["n_launch"], 1)
     bytes_per_launch = bpc * cells
     achieved = bytes_per_launch / (k_avg_ms * 1e-3) / 1e9 if h["n_launch"] else None

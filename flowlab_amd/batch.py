@@ -1,11 +1,10 @@
 """Many independent scenes advanced together on one GPU (libflowhip's
 fh_batch_* entry points, include/flowhip.h).
 
-Each scene is a single-grid engine on its own stream and steps as
-Simulation.step does with solver 'mg-fcg' (dt from its own CFL,
-simulation.ts:467-587); a batch step hands the scenes, largest first, to a
-pool of host threads, so the scenes' kernels interleave on the device.  The
-multigrid tile shapes of every scene can be set or autotuned.
+Each scene steps as Simulation.step does with solver 'mg-fcg' (dt from its
+own CFL, simulation.ts:467-587), bit for bit as a single-grid engine; a batch
+step launches each phase once for all scenes over a (scene, block) work list.
+The multigrid tile shape of each level index can be set or autotuned.
 """
 from __future__ import annotations
 
@@ -67,10 +66,10 @@ class HipBatch:
         hip.check_batch(hip.lib().fh_batch_set_forces(self._b, s, buoyancy, confinement,
                                                       hip.dptr(d)))
 
-    def step(self, scheme="macCormack", cfl=0.5, dt_max=1.0, rho=1.0, tol=1e-6, max_cycles=60,
-             graph_cycles=0) -> None:
+    def step(self, scheme="macCormack", cfl=0.5, dt_max=1.0, rho=1.0, tol=1e-6,
+             max_cycles=60) -> None:
         hip.check_batch(hip.lib().fh_batch_step(self._b, hip.SCHEMES[scheme], cfl, dt_max, rho,
-                                                tol, max_cycles, graph_cycles))
+                                                tol, max_cycles))
 
     def results(self):
         """(dt, V-cycles, relative residual) per scene of the last step."""

@@ -6,6 +6,7 @@
 // (a wavefront's 64 faces backtrace into a ~64+2c wide strip) and the kernels
 // are bound by HBM traffic: MacCormack velocity moves ~68 B per cell
 // (forward 2x13, correction 2x21), dye ~78 B per cell for 3 channels.
+#include "fh_batchdesc.hpp"
 #include "fh_stencil.hpp"
 
 namespace fh {
@@ -175,14 +176,12 @@ __device__ __forceinline__ bool interior_tile(int x0, int y0, int nx, int ny) {
 // advect.ts:72-112 with carrier == source (the only use: u and v advect
 // themselves), so two windows.
 template <int TY>
-__global__ __launch_bounds__(256) void k_advect_faces_lds(
+__device__ __forceinline__ void advect_faces_tile(
     const float* __restrict__ uc, const float* __restrict__ vc, float* __restrict__ uo,
     float* __restrict__ vo, const uint8_t* __restrict__ lbl, int nx, int ny, float c,
-    uint8_t* __restrict__ sten) {
+    uint8_t* __restrict__ sten, int bx, int by) {
   constexpr int WN = Tile<TY>::WN;
   __shared__ float lds[2][WN];
-  int bx, by;
-  xcd_block(&bx, &by);
   const int x0 = bx * TX, y0 = by * TY, wx0 = x0 - AH, wy0 = y0 - AH;
   const int tid = threadIdx.x + threadIdx.y * TX;
   const Win f[2] = {{uc, nx + 1, ny}, {vc, nx, ny + 1}};
@@ -206,16 +205,25 @@ __global__ __launch_bounds__(256) void k_advect_faces_lds(
   }
 }
 
-// advect.ts:177-210 (fused reverse pass and clamp, as k_maccormack_faces).
 template <int TY>
-__global__ __launch_bounds__(256) void k_maccormack_faces_lds(
-    const float* __restrict__ un, const float* __restrict__ vn, const float* __restrict__ uh,
-    const float* __restrict__ vh, float* __restrict__ uo, float* __restrict__ vo,
-    const uint8_t* __restrict__ lbl, int nx, int ny, float c, const uint8_t* __restrict__ sten) {
-  constexpr int WN = Tile<TY>::WN;
-  __shared__ float lds[4][WN];
+__global__ __launch_bounds__(256) void k_advect_faces_lds(
+    const float* __restrict__ uc, const float* __restrict__ vc, float* __restrict__ uo,
+    float* __restrict__ vo, const uint8_t* __restrict__ lbl, int nx, int ny, float c,
+    uint8_t* __restrict__ sten) {
   int bx, by;
   xcd_block(&bx, &by);
+  advect_faces_tile<TY>(uc, vc, uo, vo, lbl, nx, ny, c, sten, bx, by);
+}
+
+// advect.ts:177-210 (fused reverse pass and clamp, as k_maccormack_faces).
+template <int TY>
+__device__ __forceinline__ void maccormack_faces_tile(
+    const float* __restrict__ un, const float* __restrict__ vn, const float* __restrict__ uh,
+    const float* __restrict__ vh, float* __restrict__ uo, float* __restrict__ vo,
+    const uint8_t* __restrict__ lbl, int nx, int ny, float c, const uint8_t* __restrict__ sten,
+    int bx, int by) {
+  constexpr int WN = Tile<TY>::WN;
+  __shared__ float lds[4][WN];
   const int x0 = bx * TX, y0 = by * TY, wx0 = x0 - AH, wy0 = y0 - AH;
   const int tid = threadIdx.x + threadIdx.y * TX;
   const Win f[4] = {{un, nx + 1, ny}, {vn, nx, ny + 1}, {uh, nx + 1, ny}, {vh, nx, ny + 1}};
@@ -246,17 +254,25 @@ __global__ __launch_bounds__(256) void k_maccormack_faces_lds(
   }
 }
 
+template <int TY>
+__global__ __launch_bounds__(256) void k_maccormack_faces_lds(
+    const float* __restrict__ un, const float* __restrict__ vn, const float* __restrict__ uh,
+    const float* __restrict__ vh, float* __restrict__ uo, float* __restrict__ vo,
+    const uint8_t* __restrict__ lbl, int nx, int ny, float c, const uint8_t* __restrict__ sten) {
+  int bx, by;
+  xcd_block(&bx, &by);
+  maccormack_faces_tile<TY>(un, vn, uh, vh, uo, vo, lbl, nx, ny, c, sten, bx, by);
+}
+
 // advect.ts:241-270, dye on the velocity grid (scale 1): windows of u, v and
 // the 3 channels.
 template <int TY>
-__global__ __launch_bounds__(256) void k_advect_dye_lds(
+__device__ __forceinline__ void advect_dye_tile(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ q,
     float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx, int ny, float c, float k0,
-    float k1, float k2, uint8_t* __restrict__ sten) {
+    float k1, float k2, uint8_t* __restrict__ sten, int bx, int by) {
   constexpr int WN = Tile<TY>::WN;
   __shared__ float lds[5][WN];
-  int bx, by;
-  xcd_block(&bx, &by);
   const int x0 = bx * TX, y0 = by * TY, wx0 = x0 - AH, wy0 = y0 - AH;
   const int tid = threadIdx.x + threadIdx.y * TX;
   const size_t plane = (size_t)nx * ny;
@@ -284,17 +300,26 @@ __global__ __launch_bounds__(256) void k_advect_dye_lds(
   }
 }
 
+template <int TY>
+__global__ __launch_bounds__(256) void k_advect_dye_lds(
+    const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ q,
+    float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx, int ny, float c, float k0,
+    float k1, float k2, uint8_t* __restrict__ sten) {
+  int bx, by;
+  xcd_block(&bx, &by);
+  advect_dye_tile<TY>(u, v, q, qo, lbl, nx, ny, c, k0, k1, k2, sten, bx, by);
+}
+
 // advect.ts:291-311, dye on the velocity grid: windows of u, v, the 3
 // channels of q^n and of the forward pass.
 template <int TY>
-__global__ __launch_bounds__(256) void k_maccormack_dye_lds(
+__device__ __forceinline__ void maccormack_dye_tile(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ qn,
     const float* __restrict__ qh, float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx,
-    int ny, float c, float k0, float k1, float k2, const uint8_t* __restrict__ sten) {
+    int ny, float c, float k0, float k1, float k2, const uint8_t* __restrict__ sten, int bx,
+    int by) {
   constexpr int WN = Tile<TY>::WN;
   __shared__ float lds[8][WN];
-  int bx, by;
-  xcd_block(&bx, &by);
   const int x0 = bx * TX, y0 = by * TY, wx0 = x0 - AH, wy0 = y0 - AH;
   const int tid = threadIdx.x + threadIdx.y * TX;
   const size_t plane = (size_t)nx * ny;
@@ -330,6 +355,16 @@ __global__ __launch_bounds__(256) void k_maccormack_dye_lds(
       maccormack_dye_t(a, b, n0, n1, n2, h0, h1, h2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i,
                        j + r * BY);
   }
+}
+
+template <int TY>
+__global__ __launch_bounds__(256) void k_maccormack_dye_lds(
+    const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ qn,
+    const float* __restrict__ qh, float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx,
+    int ny, float c, float k0, float k1, float k2, const uint8_t* __restrict__ sten) {
+  int bx, by;
+  xcd_block(&bx, &by);
+  maccormack_dye_tile<TY>(u, v, qn, qh, qo, lbl, nx, ny, c, k0, k1, k2, sten, bx, by);
 }
 
 template <int TY>
@@ -379,6 +414,89 @@ void launch_maccormack_dye(hipStream_t s, const float* u, const float* v, const 
   else
     k_maccormack_dye<<<grid2(nx * ds, ny * ds), dim3(BX, BY), 0, s>>>(u, v, qn, qh, qo, lbl, nx,
                                                                       ny, ds, c, k0, k1, k2);
+}
+
+// ---- batched launches ----------------------------------------------------------
+// One launch over the tiles of every scene of the batch: each block finds its
+// scene and tile (batch_block) and runs the single-grid kernel's tile body on
+// that scene's buffers with that scene's c = dt / h.
+
+template <int TY>
+__global__ __launch_bounds__(256) void kb_advect_faces(const SceneDesc* __restrict__ D,
+                                                       BatchGrid g, const BStep* __restrict__ st,
+                                                       int q, int mac) {
+  int s, bx, by, gy, bid;
+  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
+  const SceneDesc& d = D[s];
+  advect_faces_tile<TY>(d.u[q], d.v[q], mac ? d.uh : d.u[q ^ 1], mac ? d.vh : d.v[q ^ 1], d.lbl,
+                        d.nx, d.ny, st[s].c, mac ? d.sten_f : nullptr, bx, by);
+}
+
+template <int TY>
+__global__ __launch_bounds__(256) void kb_maccormack_faces(const SceneDesc* __restrict__ D,
+                                                           BatchGrid g,
+                                                           const BStep* __restrict__ st, int q) {
+  int s, bx, by, gy, bid;
+  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
+  const SceneDesc& d = D[s];
+  maccormack_faces_tile<TY>(d.u[q], d.v[q], d.uh, d.vh, d.u[q ^ 1], d.v[q ^ 1], d.lbl, d.nx, d.ny,
+                            st[s].c, d.sten_f, bx, by);
+}
+
+// The forward MacCormack pass keeps the dye (keep 1 at its write, as
+// dye_phase passes it); the semi-Lagrangian pass applies the step's keep.
+template <int TY>
+__global__ __launch_bounds__(256) void kb_advect_dye(const SceneDesc* __restrict__ D, BatchGrid g,
+                                                     const BStep* __restrict__ st, int q,
+                                                     int mac) {
+  int s, bx, by, gy, bid;
+  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
+  const SceneDesc& d = D[s];
+  const BStep& t = st[s];
+  advect_dye_tile<TY>(d.u[q ^ 1], d.v[q ^ 1], d.dye[q], mac ? d.dyeh : d.dye[q ^ 1], d.lbl, d.nx,
+                      d.ny, t.c, mac ? 1.f : t.k0, mac ? 1.f : t.k1, mac ? 1.f : t.k2,
+                      mac ? d.sten_d : nullptr, bx, by);
+}
+
+template <int TY>
+__global__ __launch_bounds__(256) void kb_maccormack_dye(const SceneDesc* __restrict__ D,
+                                                         BatchGrid g, const BStep* __restrict__ st,
+                                                         int q) {
+  int s, bx, by, gy, bid;
+  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
+  const SceneDesc& d = D[s];
+  const BStep& t = st[s];
+  maccormack_dye_tile<TY>(d.u[q ^ 1], d.v[q ^ 1], d.dye[q], d.dyeh, d.dye[q ^ 1], d.lbl, d.nx,
+                          d.ny, t.c, t.k0, t.k1, t.k2, d.sten_d, bx, by);
+}
+
+dim3 batch_grid_advect(int which, int nx, int ny) {
+  switch (which) {
+    case kBAdvectFaces: return grid_t<kTyAdvectFaces>(nx + 1, ny + 1);
+    case kBMacCormackFaces: return grid_t<kTyMacCormackFaces>(nx + 1, ny + 1);
+    case kBAdvectDye: return grid_t<kTyAdvectDye>(nx, ny);
+    default: return grid_t<kTyMacCormackDye>(nx, ny);
+  }
+}
+
+void launch_batch_advect_faces(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
+                               const BStep* st, int q, int mac) {
+  kb_advect_faces<kTyAdvectFaces><<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, st, q, mac);
+}
+
+void launch_batch_maccormack_faces(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
+                                   const BStep* st, int q) {
+  kb_maccormack_faces<kTyMacCormackFaces><<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, st, q);
+}
+
+void launch_batch_advect_dye(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
+                             const BStep* st, int q, int mac) {
+  kb_advect_dye<kTyAdvectDye><<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, st, q, mac);
+}
+
+void launch_batch_maccormack_dye(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
+                                 const BStep* st, int q) {
+  kb_maccormack_dye<kTyMacCormackDye><<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, st, q);
 }
 
 }  // namespace fh

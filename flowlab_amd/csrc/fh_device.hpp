@@ -209,13 +209,16 @@ __device__ __forceinline__ void backtracef(const FU& uc, const FV& vc, int nx, i
 // several.  With n = 8q + r blocks, XCD x runs the q + (x < r) dispatch
 // indices x, x+8, ...; the k-th of them becomes logical block
 // x q + min(x, r) + k, a bijection onto [0, n).
-__device__ __forceinline__ void xcd_block(int* bx, int* by) {
-  const int gx = (int)gridDim.x, n = gx * (int)gridDim.y;
-  const int b = (int)blockIdx.x + (int)blockIdx.y * gx;
+__device__ __forceinline__ void xcd_map(int b, int n, int gx, int* bx, int* by) {
   const int x = b & 7, q = n >> 3, r = n & 7;
   const int l = x * q + (x < r ? x : r) + (b >> 3);
   *by = l / gx;
   *bx = l - *by * gx;
+}
+
+__device__ __forceinline__ void xcd_block(int* bx, int* by) {
+  const int gx = (int)gridDim.x;
+  xcd_map((int)blockIdx.x + (int)blockIdx.y * gx, gx * (int)gridDim.y, gx, bx, by);
 }
 
 // ---- reductions ---------------------------------------------------------
@@ -243,12 +246,13 @@ namespace fh {
 // deterministic.  (A last-block-done atomic counter serialises thousands of
 // device-scope atomics across the XCDs and was measured at ~25 ns each.)
 // Block size must be a multiple of 64 and at most 1024.
+// bid: the slot of this block's partials (its dispatch index in a
+// single-grid launch, its index within its scene's grid in a batched one).
 template <int NS>
-__device__ void block_sum_store(double (&val)[NS], double* partials) {
+__device__ void block_sum_store_at(double (&val)[NS], double* partials, int bid) {
   __shared__ double sh[NS][16];
   const int nthreads = blockDim.x * blockDim.y;
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  const int bid = blockIdx.x + blockIdx.y * gridDim.x;
   const int lane = tid & 63, wid = tid >> 6, nw = nthreads >> 6;
 #pragma unroll
   for (int s = 0; s < NS; s++) {
@@ -263,11 +267,15 @@ __device__ void block_sum_store(double (&val)[NS], double* partials) {
   }
 }
 
-__device__ inline void block_max_store(float val, float* partials) {
+template <int NS>
+__device__ void block_sum_store(double (&val)[NS], double* partials) {
+  block_sum_store_at<NS>(val, partials, blockIdx.x + blockIdx.y * gridDim.x);
+}
+
+__device__ inline void block_max_store_at(float val, float* partials, int bid) {
   __shared__ float sh[16];
   const int nthreads = blockDim.x * blockDim.y;
   const int tid = threadIdx.x + threadIdx.y * blockDim.x;
-  const int bid = blockIdx.x + blockIdx.y * gridDim.x;
   const int lane = tid & 63, wid = tid >> 6, nw = nthreads >> 6;
   float v = wave_max(val);
   if (lane == 0) sh[wid] = v;
@@ -277,6 +285,10 @@ __device__ inline void block_max_store(float val, float* partials) {
     for (int w = 0; w < nw; w++) t = fmaxf(t, sh[w]);
     partials[bid] = t;
   }
+}
+
+__device__ inline void block_max_store(float val, float* partials) {
+  block_max_store_at(val, partials, blockIdx.x + blockIdx.y * gridDim.x);
 }
 
 }  // namespace fh

@@ -14,6 +14,7 @@
 
 #include "../../include/flowhip.h"
 #include "fh_device.hpp"
+#include "fh_engine_impl.hpp"
 #include "fh_launch.hpp"
 #include "fh_mgtile.hpp"
 
@@ -34,23 +35,6 @@ thread_local std::string g_err;
 
 void set_err(const std::string& s) { g_err = s; }
 
-struct Level {
-  int nx, ny;
-  uint8_t* lbl;  // level 0 aliases the engine's label array
-  float* x;      // the level's unknown after a cycle
-  float* b;
-  float* xa;     // pre-smoothed unknown (tiled kernels read halos of x
-                 // while writing interiors, so they ping-pong x <-> xa)
-  uint8_t* info; // per-cell Fluid flag + open-neighbour count (cell_info)
-};
-
-// LDS the one-workgroup coarse sub-cycle may use.  64 KB: with gfx950's
-// 160 KB the 128x64 level of a 2048x1024 grid would fit too, but one
-// workgroup then takes longer over it (measured +31 us) than the two tiled
-// launches it replaces (17 us).
-constexpr size_t kCoarseLds = 65536;
-constexpr int kMaxTiledLevels = 16;
-
 // pressureMultigrid.ts:65-78 with a configurable coarsest side.
 int level_count(int nx, int ny, int coarsest) {
   int n = 1;
@@ -61,93 +45,6 @@ int level_count(int nx, int ny, int coarsest) {
   }
   return n;
 }
-
-}  // namespace
-
-struct fh_engine {
-  int nx, ny, ds, dnx, dny;
-  double h;
-  hipStream_t s = nullptr;
-  // fields
-  float *u = nullptr, *v = nullptr, *u2 = nullptr, *v2 = nullptr, *uh = nullptr, *vh = nullptr;
-  float *dye = nullptr, *dye2 = nullptr, *dyeh = nullptr, *w = nullptr;
-  uint8_t* lbl = nullptr;
-  // MacCormack: the forward passes' stencil origins of the limiter (one byte
-  // per face-grid index / dye cell), read by the correction passes
-  uint8_t *sten_f = nullptr, *sten_d = nullptr;
-  double *p = nullptr, *p2 = nullptr, *div = nullptr;
-  // flexible CG (allocated by the first such solve): FP64 residual, two FP32
-  // directions (ping-pong by cycle parity), {alpha, beta, <d, A d>}
-  double* rvec = nullptr;
-  float* dvec[2] = {nullptr, nullptr};
-  double* fcg = nullptr;
-  // multigrid: levels for coarsest side 4 (reference); the defect-correction
-  // hierarchy (coarsest side 16) is a prefix of it.
-  std::vector<Level> lv;
-  int nlev_ref = 0, nlev_dc = 0;
-  int lc_ref = 0, lc_dc = 0;  // first level run by the one-workgroup sub-cycle
-  int tile[kMaxTiledLevels] = {};  // tile shape per tiled level (kTileShapes)
-  int coarse_group = 0;            // coarse sub-cycle workgroup (kCoarseGroups)
-  bool advect_lds = true;          // LDS-staged advection kernels
-  double* M = nullptr;  // coarsest-level pseudo-inverse, n*n
-  int* cells = nullptr;
-  int ncoarse = 0;
-  bool labels_dirty = true;
-  // reductions
-  double* partials = nullptr;
-  unsigned* counter = nullptr;
-  double* sums = nullptr;
-  float* fpartials = nullptr;
-  float* fmax = nullptr;
-  double* h_sums = nullptr;  // pinned
-  float* h_fmax = nullptr;   // pinned, mapped; d_fmax is its device address
-  float* d_fmax = nullptr;
-  // Largest face speed of the resident velocity: every write of u/v bumps
-  // vel_epoch; a step enqueues the reduction of its final velocity (and the
-  // copy to h_fmax, then fmax_ev) and tags it with fmax_epoch.
-  long vel_epoch = 0, fmax_epoch = -1;
-  hipEvent_t fmax_ev = nullptr;
-  // defect-correction loop: device stop flag, two host result slots (one per
-  // cycle in flight) that the norm kernel writes, and the events that mark
-  // those kernels' completion
-  int* done = nullptr;
-  double* h_res = nullptr;  // mapped, coherent; d_res is its device address: two cycle
-                            // slots of 8 and the initial pass's at 16
-  double* d_res = nullptr;
-  hipEvent_t cyc_ev[3] = {};  // cycle slots 0, 1 and the initial pass (2)
-  double rho_first = -1.0;     // r_0 / r_init of the last solve (its first cycle's contraction)
-  // patch / forces
-  float* patch = nullptr;
-  int pi0 = 0, pj0 = 0, pnx = 0, pny = 0, has_patch = 0;
-  double buoyancy = 0, confinement = 0;
-  StirBox stir{};
-  // state
-  std::atomic<bool> busy{false};  // calls may come from several host threads
-  bool lost = false;
-  std::vector<double> hist;
-  bool timing = false;
-  hipEvent_t ev[FH_NUM_PHASES + 1] = {};
-  bool ev_valid = false;
-  // pinned host staging of the FP32 <-> FP64 field conversions (uploads and
-  // readbacks), allocated at the first transfer with room for the largest
-  // field: DMA from pinned memory instead of the runtime's pageable path
-  float* stage = nullptr;
-  size_t stage_n = 0;
-  // per-kernel tracing
-  int trace_kid = 0;
-  std::vector<hipEvent_t> trace_ev;
-  int trace_n = 0;
-  // learned warm start: training-sample capture and inference
-  fh_unet* capture = nullptr;
-  fh_unet* warm = nullptr;
-  double* psave = nullptr;
-  float* wstats = nullptr;
-  // device time of the last pressure solve
-  hipEvent_t solve_ev[2] = {};
-  bool solve_ev_valid = false;
-};
-
-namespace {
 
 int fail_hip(fh_engine* e, hipError_t err, const char* what) {
   char buf[256];
@@ -895,6 +792,13 @@ int mark(fh_engine* e, int phase) {
 }
 
 }  // namespace
+
+int fh::engine_batch_prepare(fh_engine* e) {
+  GUARD("HipBatch.prepare()");
+  int st;
+  if (e->labels_dirty && (st = refresh_levels(e)) != FH_OK) return st;
+  return ensure_fcg(e);
+}
 
 extern "C" {
 

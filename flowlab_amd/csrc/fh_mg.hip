@@ -5,6 +5,7 @@
 // Every level solves  count*x - sum(x over non-solid neighbours) = b  on
 // fluid cells, with Air neighbours contributing x = 0 and off-grid/Solid
 // neighbours absent (pressureMultigrid.ts:22-42).
+#include "fh_batchdesc.hpp"
 #include "fh_device.hpp"
 #include "fh_launch.hpp"
 
@@ -244,17 +245,15 @@ __device__ __forceinline__ void load_col5(Col5<R, T>& o, const T* __restrict__ a
 // clamped in-domain indices, and closed neighbours add 0 (sum order +x, -x,
 // +y, -y, as in the reference's residual).
 template <int R>
-__global__ __launch_bounds__(256) void k_dc_update(
+__device__ __forceinline__ void dc_update_body(
     const double* __restrict__ po, double* __restrict__ pn, const float* __restrict__ e,
     const double* __restrict__ div, const uint8_t* __restrict__ info, float* __restrict__ r32,
     double* __restrict__ r64, int nx, int ny, double scale, double* partials,
-    const int* __restrict__ done) {
+    const int* __restrict__ done, int bx, int by, int gy, int bid) {
   if (done && *done) return;
-  int bx, by;
-  xcd_block(&bx, &by);
   const int i = bx * BX + threadIdx.x;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int j0 = (by * BY + threadIdx.y) * R; j0 < ny; j0 += gridDim.y * BY * R) {
+  for (int j0 = (by * BY + threadIdx.y) * R; j0 < ny; j0 += gy * BY * R) {
     if (i >= nx) break;
     Col5<R, double> P;
     load_col5(P, po, i, j0, nx, ny);
@@ -304,7 +303,19 @@ __global__ __launch_bounds__(256) void k_dc_update(
       if (r64) r64[k] = rr;
     }
   }
-  block_sum_store<4>(acc, partials);
+  block_sum_store_at<4>(acc, partials, bid);
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_dc_update(
+    const double* __restrict__ po, double* __restrict__ pn, const float* __restrict__ e,
+    const double* __restrict__ div, const uint8_t* __restrict__ info, float* __restrict__ r32,
+    double* __restrict__ r64, int nx, int ny, double scale, double* partials,
+    const int* __restrict__ done) {
+  int bx, by;
+  xcd_block(&bx, &by);
+  dc_update_body<R>(po, pn, e, div, info, r32, r64, nx, ny, scale, partials, done, bx, by,
+                    gridDim.y, blockIdx.x + blockIdx.y * gridDim.x);
 }
 
 // ---- flexible conjugate gradients (FCG(1)) around the V-cycle -------------
@@ -345,18 +356,17 @@ __device__ __forceinline__ double apply_a(int f, double c, double e, double w, d
 
 // Pass 1: partials of s (first: d' is not read and counts as 0).
 template <int R>
-__global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
-                                                  const float* __restrict__ dprev,
-                                                  const double* __restrict__ r,
-                                                  const uint8_t* __restrict__ info, int nx,
-                                                  int ny, int first, double* partials,
-                                                  const int* __restrict__ done) {
+__device__ __forceinline__ void fcg_dots_body(const float* __restrict__ z,
+                                              const float* __restrict__ dprev,
+                                              const double* __restrict__ r,
+                                              const uint8_t* __restrict__ info, int nx, int ny,
+                                              int first, double* partials,
+                                              const int* __restrict__ done, int bx, int by,
+                                              int gy, int bid) {
   if (done && *done) return;
-  int bx, by;
-  xcd_block(&bx, &by);
   const int i = bx * BX + threadIdx.x;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int j0 = (by * BY + threadIdx.y) * R; j0 < ny; j0 += gridDim.y * BY * R) {
+  for (int j0 = (by * BY + threadIdx.y) * R; j0 < ny; j0 += gy * BY * R) {
     if (i >= nx) break;
     Col5<R, float> Z, D;
     load_col5(Z, z, i, j0, nx, ny);
@@ -385,15 +395,28 @@ __global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
       }
     }
   }
-  block_sum_store<4>(acc, partials);
+  block_sum_store_at<4>(acc, partials, bid);
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_fcg_dots(const float* __restrict__ z,
+                                                  const float* __restrict__ dprev,
+                                                  const double* __restrict__ r,
+                                                  const uint8_t* __restrict__ info, int nx,
+                                                  int ny, int first, double* partials,
+                                                  const int* __restrict__ done) {
+  int bx, by;
+  xcd_block(&bx, &by);
+  fcg_dots_body<R>(z, dprev, r, info, nx, ny, first, partials, done, bx, by, gridDim.y,
+                   blockIdx.x + blockIdx.y * gridDim.x);
 }
 
 // The step coefficients from pass 1's partials: state = {alpha, beta,
 // <d, A d>} (the last carried to the next iteration).  A non-positive or
 // non-finite <d, A d> (a direction the operator does not see) gives alpha 0.
-__global__ __launch_bounds__(1024) void k_fcg_coef(const double* __restrict__ partials,
-                                                   int nblocks, double* __restrict__ state,
-                                                   int first, const int* __restrict__ done) {
+__device__ __forceinline__ void fcg_coef_body(const double* __restrict__ partials, int nblocks,
+                                              double* __restrict__ state, int first,
+                                              const int* __restrict__ done) {
   if (done && *done) return;
   __shared__ double sh[4][16];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -423,22 +446,26 @@ __global__ __launch_bounds__(1024) void k_fcg_coef(const double* __restrict__ pa
   }
 }
 
+__global__ __launch_bounds__(1024) void k_fcg_coef(const double* __restrict__ partials,
+                                                   int nblocks, double* __restrict__ state,
+                                                   int first, const int* __restrict__ done) {
+  fcg_coef_body(partials, nblocks, state, first, done);
+}
+
 // Pass 2: d = z + beta d';  p += alpha d;  r -= alpha (A z + beta A d');
 // r32 = (float) r;  partials {sum r^2, n} over the fluid cells with a
 // non-zero count (the divergence terms of the norm are the initial pass's).
 template <int R>
-__global__ __launch_bounds__(256) void k_fcg_update(
+__device__ __forceinline__ void fcg_update_body(
     const float* __restrict__ z, const float* __restrict__ dprev, float* __restrict__ dnext,
     double* __restrict__ p, double* __restrict__ r, const uint8_t* __restrict__ info,
     float* __restrict__ r32, const double* __restrict__ state, int nx, int ny, int first,
-    double* partials, const int* __restrict__ done) {
+    double* partials, const int* __restrict__ done, int bx, int by, int gy, int bid) {
   if (done && *done) return;
-  int bx, by;
-  xcd_block(&bx, &by);
   const int i = bx * BX + threadIdx.x;
   const double alpha = state[0], beta = first ? 0.0 : state[1];
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int j0 = (by * BY + threadIdx.y) * R; j0 < ny; j0 += gridDim.y * BY * R) {
+  for (int j0 = (by * BY + threadIdx.y) * R; j0 < ny; j0 += gy * BY * R) {
     if (i >= nx) break;
     Col5<R, float> Z, D;
     load_col5(Z, z, i, j0, nx, ny);
@@ -480,7 +507,19 @@ __global__ __launch_bounds__(256) void k_fcg_update(
       }
     }
   }
-  block_sum_store<4>(acc, partials);
+  block_sum_store_at<4>(acc, partials, bid);
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_fcg_update(
+    const float* __restrict__ z, const float* __restrict__ dprev, float* __restrict__ dnext,
+    double* __restrict__ p, double* __restrict__ r, const uint8_t* __restrict__ info,
+    float* __restrict__ r32, const double* __restrict__ state, int nx, int ny, int first,
+    double* partials, const int* __restrict__ done) {
+  int bx, by;
+  xcd_block(&bx, &by);
+  fcg_update_body<R>(z, dprev, dnext, p, r, info, r32, state, nx, ny, first, partials, done, bx,
+                     by, gridDim.y, blockIdx.x + blockIdx.y * gridDim.x);
 }
 
 // pressure.ts:102-161 norms of a pressure held in FP32 (p32) or FP64 (p64):
@@ -560,12 +599,12 @@ static inline const double* folded(hipStream_t s, const double* partials, int* n
 // is a mapped pinned host buffer that receives the same 6 values, so the host
 // reads them after an event without a copy on the stream.  nblocks <=
 // kMaxPartials.
-__global__ __launch_bounds__(1024) void k_finalize_sum4(const double* __restrict__ partials,
-                                                        int nblocks, double* __restrict__ out,
-                                                        double* __restrict__ host_out,
-                                                        int* __restrict__ done, double scale,
-                                                        double tol, int init,
-                                                        const double* __restrict__ fixed23) {
+__device__ __forceinline__ void finalize_sum4_body(const double* __restrict__ partials,
+                                                   int nblocks, double* __restrict__ out,
+                                                   double* __restrict__ host_out,
+                                                   int* __restrict__ done, double scale,
+                                                   double tol, int init,
+                                                   const double* __restrict__ fixed23) {
   if (!init && done && *done) return;
   __shared__ double sh[4][16];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -621,6 +660,15 @@ __global__ __launch_bounds__(1024) void k_finalize_sum4(const double* __restrict
     }
     if (done && (stop || init)) *done = stop ? 1 : 0;
   }
+}
+
+__global__ __launch_bounds__(1024) void k_finalize_sum4(const double* __restrict__ partials,
+                                                        int nblocks, double* __restrict__ out,
+                                                        double* __restrict__ host_out,
+                                                        int* __restrict__ done, double scale,
+                                                        double tol, int init,
+                                                        const double* __restrict__ fixed23) {
+  finalize_sum4_body(partials, nblocks, out, host_out, done, scale, tol, init, fixed23);
 }
 
 void launch_finalize_sum4(hipStream_t s, const double* partials, int nblocks, double* out) {
@@ -749,6 +797,97 @@ void launch_norms(hipStream_t s, const float* p32, const double* p64, const doub
   dim3 g = reduce_grid(nx, ny);
   k_norms<<<g, dim3(BX, BY), 0, s>>>(p32, p64, div, lbl, nx, ny, scale, partials);
   launch_finalize_sum4(s, partials, g.x * g.y, sums);
+}
+
+// ---- batched launches ----------------------------------------------------------
+// The defect and CG passes over every scene of the work list (batch_block),
+// each scene at its own rows per thread, and the one-block reductions as one
+// block per scene.  A scene's partials land in its own buffer at the slots
+// of its single-grid launch, so its norms have the same bits.
+
+__global__ __launch_bounds__(256) void kb_dc_update(const SceneDesc* __restrict__ D, BatchGrid g,
+                                                    const BStep* __restrict__ st, int q,
+                                                    int init) {
+  int s, bx, by, gy, bid;
+  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
+  const SceneDesc& d = D[s];
+  double* pn = init ? nullptr : d.p[q ^ 1];
+  const float* e = init ? nullptr : d.lv[0].x;
+  const int* done = init ? nullptr : d.done;
+  if (d.red_rows == kRedRows)
+    dc_update_body<kRedRows>(d.p[q], pn, e, d.div, d.lv[0].info, d.lv[0].b, d.rvec, d.nx, d.ny,
+                             st[s].scale, d.partials, done, bx, by, gy, bid);
+  else
+    dc_update_body<1>(d.p[q], pn, e, d.div, d.lv[0].info, d.lv[0].b, d.rvec, d.nx, d.ny,
+                      st[s].scale, d.partials, done, bx, by, gy, bid);
+}
+
+__global__ __launch_bounds__(1024) void kb_finalize(const SceneDesc* __restrict__ D,
+                                                    const BStep* __restrict__ st, int mode,
+                                                    int slot, double tol) {
+  const SceneDesc& d = D[blockIdx.x];
+  const double scale = st[blockIdx.x].scale;
+  if (mode == 0)
+    finalize_sum4_body(d.partials, d.red_blocks, d.sums, d.d_res + 16, d.done, scale, 0.0, 1,
+                       nullptr);
+  else
+    finalize_sum4_body(d.partials, d.red_blocks, d.sums, d.d_res + 8 * slot, d.done, scale, tol,
+                       0, mode == 2 ? d.sums : nullptr);
+}
+
+__global__ __launch_bounds__(256) void kb_fcg_dots(const SceneDesc* __restrict__ D, BatchGrid g,
+                                                   int cp, int first) {
+  int s, bx, by, gy, bid;
+  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
+  const SceneDesc& d = D[s];
+  if (d.red_rows == kRedRows)
+    fcg_dots_body<kRedRows>(d.lv[0].x, d.dvec[cp], d.rvec, d.lv[0].info, d.nx, d.ny, first,
+                            d.partials, d.done, bx, by, gy, bid);
+  else
+    fcg_dots_body<1>(d.lv[0].x, d.dvec[cp], d.rvec, d.lv[0].info, d.nx, d.ny, first,
+                     d.partials, d.done, bx, by, gy, bid);
+}
+
+__global__ __launch_bounds__(1024) void kb_fcg_coef(const SceneDesc* __restrict__ D, int first) {
+  const SceneDesc& d = D[blockIdx.x];
+  fcg_coef_body(d.partials, d.red_blocks, d.fcg, first, d.done);
+}
+
+__global__ __launch_bounds__(256) void kb_fcg_update(const SceneDesc* __restrict__ D,
+                                                     BatchGrid g, int q, int cp, int first) {
+  int s, bx, by, gy, bid;
+  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
+  const SceneDesc& d = D[s];
+  if (d.red_rows == kRedRows)
+    fcg_update_body<kRedRows>(d.lv[0].x, d.dvec[cp], d.dvec[cp ^ 1], d.p[q ^ 1], d.rvec,
+                              d.lv[0].info, d.lv[0].b, d.fcg, d.nx, d.ny, first, d.partials,
+                              d.done, bx, by, gy, bid);
+  else
+    fcg_update_body<1>(d.lv[0].x, d.dvec[cp], d.dvec[cp ^ 1], d.p[q ^ 1], d.rvec, d.lv[0].info,
+                       d.lv[0].b, d.fcg, d.nx, d.ny, first, d.partials, d.done, bx, by, gy, bid);
+}
+
+dim3 batch_grid_reduce(int nx, int ny, int* rows) {
+  const RGrid rg = reduce_grid_rows(nx, ny);
+  *rows = rg.rows;
+  return rg.g;
+}
+
+void launch_batch_dc_update(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
+                            const BStep* st, int q, int init) {
+  kb_dc_update<<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, st, q, init);
+}
+
+void launch_batch_finalize(hipStream_t s, const SceneDesc* D, int nscenes, const BStep* st,
+                           int mode, int slot, double tol) {
+  kb_finalize<<<nscenes, 1024, 0, s>>>(D, st, mode, slot, tol);
+}
+
+void launch_batch_fcg(hipStream_t s, const SceneDesc* D, const BatchGrid& g, int nscenes, int q,
+                      int cp, int first) {
+  kb_fcg_dots<<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, cp, first);
+  kb_fcg_coef<<<nscenes, 1024, 0, s>>>(D, first);
+  kb_fcg_update<<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, q, cp, first);
 }
 
 }  // namespace fh

@@ -21,6 +21,7 @@
 // Every kernel takes `done`: once the defect-correction loop has converged
 // (flag set on the device by the norm kernel), cycles the host enqueued
 // ahead of that knowledge return at once and change nothing.
+#include "fh_batchdesc.hpp"
 #include "fh_mgtile.hpp"
 
 namespace fh {
@@ -143,6 +144,83 @@ void launch_coarse_vcycle(hipStream_t s, int group, const CoarseLevels& L, const
                                                  done);
       break;
   }
+}
+
+// ---- batched launches ----------------------------------------------------------
+// Level l of every scene that tiles it (l < its lc; the work list holds only
+// those), then each scene's sub-cycle as one workgroup per scene: the same
+// tile bodies on each scene's level buffers, so the same bits as its own
+// V-cycle.
+
+template <int TX, int TY>
+__global__ __launch_bounds__(NT) void kb_smooth_restrict(const SceneDesc* __restrict__ D,
+                                                         BatchGrid g, int l) {
+  int s, bx, by, gy, bid;
+  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
+  const SceneDesc& d = D[s];
+  const LevelDesc& L = d.lv[l];
+  const LevelDesc& N = d.lv[l + 1];
+  smooth_restrict_tile<TX, TY>(nullptr, L.xa, L.b, L.info, L.nx, L.ny, N.b, N.nx, N.ny, 1, d.done,
+                               bx, by);
+}
+
+template <int TX, int TY>
+__global__ __launch_bounds__(NT) void kb_prolong_smooth(const SceneDesc* __restrict__ D,
+                                                        BatchGrid g, int l) {
+  int s, bx, by, gy, bid;
+  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
+  const SceneDesc& d = D[s];
+  const LevelDesc& L = d.lv[l];
+  const LevelDesc& N = d.lv[l + 1];
+  prolong_smooth_tile<TX, TY>(L.xa, L.x, L.b, L.info, L.nx, L.ny, N.x, N.nx, N.ny, d.done, bx, by);
+}
+
+__global__ __launch_bounds__(NC) void kb_coarse_vcycle(const SceneDesc* __restrict__ D) {
+  const SceneDesc& d = D[blockIdx.x];
+  if (*d.done) return;
+  extern __shared__ unsigned char smem[];
+  coarse_vcycle_body<NC>(d.cl, d.lv[d.lc].b, nullptr, d.lv[d.lc].x, d.M, d.cells, d.ncoarse, 1,
+                         smem);
+}
+
+template <int TX, int TY>
+static void bsr(hipStream_t s, const SceneDesc* D, const BatchGrid& g, int l) {
+  kb_smooth_restrict<TX, TY><<<g.blocks, NT, 0, s>>>(D, g, l);
+}
+
+template <int TX, int TY>
+static void bps(hipStream_t s, const SceneDesc* D, const BatchGrid& g, int l) {
+  kb_prolong_smooth<TX, TY><<<g.blocks, NT, 0, s>>>(D, g, l);
+}
+
+void launch_batch_smooth_restrict(hipStream_t s, int shape, const SceneDesc* D,
+                                  const BatchGrid& g, int l) {
+  switch (shape) {
+    case 1: bsr<32, 32>(s, D, g, l); break;
+    case 2: bsr<64, 16>(s, D, g, l); break;
+    case 3: bsr<128, 16>(s, D, g, l); break;
+    case 4: bsr<32, 16>(s, D, g, l); break;
+    case 5: bsr<16, 16>(s, D, g, l); break;
+    case 6: bsr<32, 8>(s, D, g, l); break;
+    default: bsr<64, 32>(s, D, g, l); break;
+  }
+}
+
+void launch_batch_prolong_smooth(hipStream_t s, int shape, const SceneDesc* D, const BatchGrid& g,
+                                 int l) {
+  switch (shape) {
+    case 1: bps<32, 32>(s, D, g, l); break;
+    case 2: bps<64, 16>(s, D, g, l); break;
+    case 3: bps<128, 16>(s, D, g, l); break;
+    case 4: bps<32, 16>(s, D, g, l); break;
+    case 5: bps<16, 16>(s, D, g, l); break;
+    case 6: bps<32, 8>(s, D, g, l); break;
+    default: bps<64, 32>(s, D, g, l); break;
+  }
+}
+
+void launch_batch_coarse_vcycle(hipStream_t s, const SceneDesc* D, int nscenes, size_t lds) {
+  kb_coarse_vcycle<<<nscenes, NC, lds, s>>>(D);
 }
 
 }  // namespace fh

@@ -3,6 +3,7 @@
 //
 // All of these are single-pass streaming stencils: one thread per face or
 // cell, x fastest, so each wavefront touches contiguous rows.
+#include "fh_batchdesc.hpp"
 #include "fh_stencil.hpp"
 
 namespace fh {
@@ -49,12 +50,9 @@ static inline dim3 grid_rows(int w, int h) {
 // Buoyancy alone (buoyancy.ts:70-88: no stir, no confinement, gain != 0):
 // u is unchanged, so only the interior v faces j in [1, ny-1] are touched,
 // kRows faces of a column per thread; the stored value is forces_at's.
-__global__ __launch_bounds__(256) void k_buoyancy(float* __restrict__ v,
-                                                  const uint8_t* __restrict__ lbl,
-                                                  const float* __restrict__ heat, int nx, int ny,
-                                                  int ds, float gain) {
-  int bx, by;
-  xcd_block(&bx, &by);
+__device__ __forceinline__ void buoyancy_rows(float* __restrict__ v, const uint8_t* __restrict__ lbl,
+                                              const float* __restrict__ heat, int nx, int ny,
+                                              int ds, float gain, int bx, int by) {
   const int i = bx * BX + threadIdx.x, j0 = 1 + (by * BY + threadIdx.y) * kRows;
   if (i >= nx || j0 > ny - 1) return;
   const int dnx = nx * ds;
@@ -80,14 +78,21 @@ __global__ __launch_bounds__(256) void k_buoyancy(float* __restrict__ v,
   }
 }
 
-// divergence.ts:17-25, evaluated in FP64 from the FP32 faces, kRows cells of
-// a column per thread.
-__global__ __launch_bounds__(256) void k_divergence(const float* __restrict__ u,
-                                                    const float* __restrict__ v,
-                                                    double* __restrict__ div, int nx, int ny,
-                                                    double inv_h) {
+__global__ __launch_bounds__(256) void k_buoyancy(float* __restrict__ v,
+                                                  const uint8_t* __restrict__ lbl,
+                                                  const float* __restrict__ heat, int nx, int ny,
+                                                  int ds, float gain) {
   int bx, by;
   xcd_block(&bx, &by);
+  buoyancy_rows(v, lbl, heat, nx, ny, ds, gain, bx, by);
+}
+
+// divergence.ts:17-25, evaluated in FP64 from the FP32 faces, kRows cells of
+// a column per thread.
+__device__ __forceinline__ void divergence_rows(const float* __restrict__ u,
+                                                const float* __restrict__ v,
+                                                double* __restrict__ div, int nx, int ny,
+                                                double inv_h, int bx, int by) {
   const int i = bx * BX + threadIdx.x, j0 = (by * BY + threadIdx.y) * kRows;
   if (i >= nx || j0 >= ny) return;
   const int su = nx + 1;
@@ -105,6 +110,15 @@ __global__ __launch_bounds__(256) void k_divergence(const float* __restrict__ u,
     if (j0 + r < ny)
       div[i + (j0 + r) * nx] =
           inv_h * (((double)ub[r] - (double)ua[r]) + (double)vv[r + 1] - (double)vv[r]);
+}
+
+__global__ __launch_bounds__(256) void k_divergence(const float* __restrict__ u,
+                                                    const float* __restrict__ v,
+                                                    double* __restrict__ div, int nx, int ny,
+                                                    double inv_h) {
+  int bx, by;
+  xcd_block(&bx, &by);
+  divergence_rows(u, v, div, nx, ny, inv_h, bx, by);
 }
 
 __global__ __launch_bounds__(256) void k_subtract(float* __restrict__ u, float* __restrict__ v,
@@ -156,13 +170,10 @@ __device__ __forceinline__ void outflow_edges_at(float* __restrict__ u, float* _
 // kRows face rows per thread: every row's loads (SubLoad) are issued before
 // any store; then the subtraction of each row, then each row's outflow (which
 // reads back only faces this thread has just written).
-__global__ __launch_bounds__(256) void k_subtract_outflow(float* __restrict__ u,
-                                                          float* __restrict__ v,
-                                                          const double* __restrict__ p,
-                                                          const uint8_t* __restrict__ lbl, int nx,
-                                                          int ny, double gs) {
-  int bx, by;
-  xcd_block(&bx, &by);
+__device__ __forceinline__ void subtract_outflow_rows(float* __restrict__ u, float* __restrict__ v,
+                                                      const double* __restrict__ p,
+                                                      const uint8_t* __restrict__ lbl, int nx,
+                                                      int ny, double gs, int bx, int by) {
   const int i = bx * BX + threadIdx.x, j0 = (by * BY + threadIdx.y) * kRows, su = nx + 1;
   if (i > nx || j0 > ny) return;
   SubLoad L[kRows];
@@ -177,6 +188,16 @@ __global__ __launch_bounds__(256) void k_subtract_outflow(float* __restrict__ u,
     if (j > ny) break;
     outflow_edges_at(u, v, lbl, nx, ny, su, i, j);
   }
+}
+
+__global__ __launch_bounds__(256) void k_subtract_outflow(float* __restrict__ u,
+                                                          float* __restrict__ v,
+                                                          const double* __restrict__ p,
+                                                          const uint8_t* __restrict__ lbl, int nx,
+                                                          int ny, double gs) {
+  int bx, by;
+  xcd_block(&bx, &by);
+  subtract_outflow_rows(u, v, p, lbl, nx, ny, gs, bx, by);
 }
 
 
@@ -332,6 +353,131 @@ void launch_max_speed(hipStream_t s, const float* u, const float* v, int nx, int
   k_max_speed<<<1024, 256, 0, s>>>(u, v, (long)(nx + 1) * ny, (long)nx * (ny + 1), partials,
                                    counter, out);
   k_finalize_max<<<1, 256, 0, s>>>(partials, 1024, out, host_out);
+}
+
+// ---- batched launches ----------------------------------------------------------
+// One launch over every scene of the work list; each block runs the
+// single-grid kernel's body on its scene (batch_block).  The forces act on
+// the advected velocity [q ^ 1] with the heat of the dye not yet advected
+// [q], as the single-grid step orders them.
+
+__global__ __launch_bounds__(256) void kb_vorticity(const SceneDesc* __restrict__ D, BatchGrid g,
+                                                    const BStep* __restrict__ st, int q) {
+  int s, bx, by, gy, bid;
+  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
+  const SceneDesc& d = D[s];
+  vorticity_at(d.u[q ^ 1], d.v[q ^ 1], d.w, d.nx, d.ny, st[s].inv_hf, bx * BX + threadIdx.x,
+               by * BY + threadIdx.y);
+}
+
+__global__ __launch_bounds__(256) void kb_forces(const SceneDesc* __restrict__ D, BatchGrid g,
+                                                 const BStep* __restrict__ st, int q) {
+  int s, bx, by, gy, bid;
+  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
+  const SceneDesc& d = D[s];
+  const BStep& t = st[s];
+  const StirBox none{};
+  forces_at(d.u[q ^ 1], d.v[q ^ 1], d.lbl, d.dye[q], d.w, d.nx, d.ny, 1, t.hf, t.dtf, t.gain,
+            t.eps, none, bx * BX + threadIdx.x, by * BY + threadIdx.y);
+}
+
+// The single-grid step skips buoyancy when buoyancy * dt rounds to 0.
+__global__ __launch_bounds__(256) void kb_buoyancy(const SceneDesc* __restrict__ D, BatchGrid g,
+                                                   const BStep* __restrict__ st, int q) {
+  int s, bx, by, gy, bid;
+  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
+  const SceneDesc& d = D[s];
+  const float gain = st[s].gain;
+  if (gain == 0.f) return;
+  buoyancy_rows(d.v[q ^ 1], d.lbl, d.dye[q], d.nx, d.ny, 1, gain, bx, by);
+}
+
+__global__ __launch_bounds__(256) void kb_divergence(const SceneDesc* __restrict__ D, BatchGrid g,
+                                                     const BStep* __restrict__ st, int q) {
+  int s, bx, by, gy, bid;
+  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
+  const SceneDesc& d = D[s];
+  divergence_rows(d.u[q ^ 1], d.v[q ^ 1], d.div, d.nx, d.ny, st[s].inv_h, bx, by);
+}
+
+// Gradient subtraction with the pressure the solve left in p [q ^ 1].
+__global__ __launch_bounds__(256) void kb_project(const SceneDesc* __restrict__ D, BatchGrid g,
+                                                  const BStep* __restrict__ st, int q) {
+  int s, bx, by, gy, bid;
+  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
+  const SceneDesc& d = D[s];
+  subtract_outflow_rows(d.u[q ^ 1], d.v[q ^ 1], d.p[q ^ 1], d.lbl, d.nx, d.ny, st[s].grad_scale,
+                        bx, by);
+}
+
+// kBatchSpeedBlocks blocks per scene stride over its u and v faces (the
+// maximum does not depend on the order); one block per scene then reduces
+// the scene's partials into host_out[s].
+__global__ __launch_bounds__(256) void kb_max_speed(const SceneDesc* __restrict__ D, BatchGrid g,
+                                                    int q, float* __restrict__ partials) {
+  int s, bx, by, gy, bid;
+  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
+  const SceneDesc& d = D[s];
+  const long stride = (long)kBatchSpeedBlocks * 256;
+  const long t0 = (long)bid * 256 + threadIdx.x;
+  float m = 0.f;
+  auto scan = [&](const float* __restrict__ a, long n) {
+    for (long k = t0; k < n; k += stride) m = fmaxf(m, fabsf(a[k]));
+  };
+  scan(d.u[q ^ 1], (long)(d.nx + 1) * d.ny);
+  scan(d.v[q ^ 1], (long)d.nx * (d.ny + 1));
+  block_max_store_at(m, partials + (size_t)s * kBatchSpeedBlocks, bid);
+}
+
+__global__ __launch_bounds__(64) void kb_finalize_max(const float* __restrict__ partials,
+                                                      float* __restrict__ host_out) {
+  const int s = blockIdx.x;
+  float m = 0.f;
+  for (int b = threadIdx.x; b < kBatchSpeedBlocks; b += 64)
+    m = fmaxf(m, partials[(size_t)s * kBatchSpeedBlocks + b]);
+  m = wave_max(m);
+  if (threadIdx.x == 0) host_out[s] = m;
+}
+
+dim3 batch_grid_project(int which, int nx, int ny) {
+  switch (which) {
+    case kBVorticity: return grid2(nx, ny);
+    case kBForces: return grid2(nx + 1, ny + 1);
+    case kBBuoyancy: return grid_rows(nx, ny - 1);
+    case kBDivergence: return grid_rows(nx, ny);
+    default: return grid_rows(nx + 1, ny + 1);
+  }
+}
+
+void launch_batch_vorticity(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
+                            const BStep* st, int q) {
+  kb_vorticity<<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, st, q);
+}
+
+void launch_batch_forces(hipStream_t s, const SceneDesc* D, const BatchGrid& g, const BStep* st,
+                         int q) {
+  kb_forces<<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, st, q);
+}
+
+void launch_batch_buoyancy(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
+                           const BStep* st, int q) {
+  kb_buoyancy<<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, st, q);
+}
+
+void launch_batch_divergence(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
+                             const BStep* st, int q) {
+  kb_divergence<<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, st, q);
+}
+
+void launch_batch_project(hipStream_t s, const SceneDesc* D, const BatchGrid& g, const BStep* st,
+                          int q) {
+  kb_project<<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, st, q);
+}
+
+void launch_batch_max_speed(hipStream_t s, const SceneDesc* D, const BatchGrid& g, int q,
+                            int nscenes, float* partials, float* host_out) {
+  kb_max_speed<<<g.blocks, 256, 0, s>>>(D, g, q, partials);
+  kb_finalize_max<<<nscenes, 64, 0, s>>>(partials, host_out);
 }
 
 }  // namespace fh

@@ -96,7 +96,7 @@ SIGNATURES = [
     ("fh_batch_destroy", None, [_vp]),
     ("fh_batch_upload", _i, [_vp, _i, _dp, _dp, _dp, _u8p, _dp]),
     ("fh_batch_set_forces", _i, [_vp, _i, _d, _d, _dp]),
-    ("fh_batch_step", _i, [_vp, _i, _d, _d, _d, _d, _i, _i]),
+    ("fh_batch_step", _i, [_vp, _i, _d, _d, _d, _d, _i]),
     ("fh_batch_results", _i, [_vp, _dp, _ip, _dp]),
     ("fh_batch_read", _i, [_vp, _i, _dp, _dp, _dp, _dp]),
     ("fh_batch_last_step_ms", _i, [_vp, _fp]),

@@ -271,13 +271,15 @@ int fh_set_warm_start(fh_engine* e, fh_unet* u);
 /* ---- Batched scenes ------------------------------------------------------
  * No counterpart in the reference, which steps one grid per Simulation
  * (simulation.ts:467-587).  A batch holds up to 255 independent grids of any
- * sizes (dyeScale 1, fixed at creation), each a single-grid engine with its
- * own stream.  A step hands the scenes, largest first, to a pool of host
- * threads, so their kernels interleave on the device: each scene takes dt
- * from its own CFL (simulation.ts:474-479) and solves its pressure with
- * FH_SOLVER_MG_FCG to its own convergence, exactly as fh_step on that scene
- * alone.  Per-scene fields are uploaded and read in the reference's host
- * layout, FP64, as fh_upload. */
+ * sizes of at least 3 x 3 (dyeScale 1, fixed at creation), each with the
+ * buffers of a single-grid engine.  A step launches each phase once for all
+ * scenes, over a work list of (scene, block) tasks: each scene takes dt from
+ * its own CFL (simulation.ts:474-479) and solves its pressure with
+ * FH_SOLVER_MG_FCG to its own convergence, bit for bit as fh_step on that
+ * scene alone.  Per-scene fields are uploaded and read in the reference's
+ * host layout, FP64, as fh_upload.  The defect-correction grid of a scene
+ * must need at most 8192 reduction blocks (about 4 M cells); a larger one
+ * fails the step with FH_ERR_BAD_ARG. */
 typedef struct fh_batch fh_batch;
 const char* fh_batch_last_error(void);
 /* NULL on failure (fh_batch_last_error says why). */
@@ -292,24 +294,25 @@ int fh_batch_set_forces(fh_batch* b, int scene, double buoyancy, double confinem
                         const double* decay);
 /* One step of every scene: dt = min(dt_max, cfl h / max face speed),
  * scale = rho h^2/dt, gradScale = dt/(rho h), dye keep exp(-decay dt).
- * Pressure: V-cycles until the relative residual <= tol (tol <= 0: exactly
- * max_cycles).  graph_cycles is accepted and ignored. */
+ * Pressure: cycles until the scene's relative residual <= tol, at most
+ * max_cycles. */
 int fh_batch_step(fh_batch* b, int scheme, double cfl, double dt_max, double rho, double tol,
-                  int max_cycles, int graph_cycles);
+                  int max_cycles);
 /* Per scene of the last step (arrays of nscenes, any may be NULL). */
 int fh_batch_results(fh_batch* b, double* dt, int* iters, double* resid);
 int fh_batch_read(fh_batch* b, int scene, double* u, double* v, double* p, double* dye);
 int fh_batch_last_step_ms(fh_batch* b, float* ms);
 long long fh_batch_cells(fh_batch* b);
-/* Multigrid tile shapes: fh_batch_num_tiles() candidates by name; level -1
- * sets every tiled level of every scene; get_tiles reports the largest
- * scene's.  Results do not depend on the shape. */
+/* Multigrid tile shapes of the batched launches, one per level index (level
+ * l of every scene that tiles it): fh_batch_num_tiles() candidates by name;
+ * level -1 sets every level; get_tiles reports the tiled levels of the
+ * deepest scene.  Results do not depend on the shape. */
 int fh_batch_num_tiles(void);
 const char* fh_batch_tile_name(int shape);
 int fh_batch_set_tile(fh_batch* b, int level, int shape);
 int fh_batch_get_tiles(fh_batch* b, int* shapes, int* nlevels);
-/* Autotuner: every scene tunes its own levels (fh_tune_tiles); ms_out gets
- * the largest scene's nlevels x num_tiles ms. */
+/* Autotuner of the batched multigrid launches: per level index, the shape
+ * with the shortest launch pair; ms_out gets nlevels x num_tiles ms. */
 int fh_batch_tune(fh_batch* b, int reps, float* ms_out);
 
 #ifdef __cplusplus

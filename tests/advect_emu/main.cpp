@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "emu.hpp"
+#include "fh_batchdesc.hpp"
 #include "fh_launch.hpp"
 
 dim3 threadIdx, blockIdx, gridDim, blockDim;
@@ -31,8 +32,100 @@ static int cmp(const char* what, const float* a, const float* b, size_t n) {
   return bad != 0;
 }
 
-int main() {
+// The batched launches over three scenes of uneven size (work list built
+// largest scene last, offsets padded to multiples of 8) against each scene's
+// own single-grid launch: the same bits.
+static int batch_check() {
   int fails = 0;
+  const int sizes[][2] = {{67, 45}, {130, 70}, {200, 33}};
+  const float cs[] = {0.8f, 1.3f, 0.4f};
+  constexpr int NS = 3;
+  struct Scene {
+    std::vector<float> u[2], v[2], uh, vh, q[2], qh;
+    std::vector<uint8_t> lbl, sf, sd;
+  } sc[NS];
+  std::vector<SceneDesc> D(NS);
+  std::vector<BStep> st(NS);
+  for (int s = 0; s < NS; s++) {
+    const int nx = sizes[s][0], ny = sizes[s][1];
+    const size_t nu = (size_t)(nx + 1) * ny, nv = (size_t)nx * (ny + 1), nc = (size_t)nx * ny;
+    Scene& S = sc[s];
+    S.u[0] = rnd(nu, 1.f), S.v[0] = rnd(nv, 1.f), S.u[1].assign(nu, 0.f), S.v[1].assign(nv, 0.f);
+    S.uh.assign(nu, 0.f), S.vh.assign(nv, 0.f);
+    S.q[0] = rnd(3 * nc, 1.f), S.q[1].assign(3 * nc, 0.f), S.qh.assign(3 * nc, 0.f);
+    S.lbl.resize(nc);
+    std::uniform_int_distribution<int> dl(0, 9);
+    for (auto& l : S.lbl) {
+      int r = dl(rng);
+      l = r == 0 ? 2 : (r == 1 ? 1 : 0);
+    }
+    S.sf.assign((size_t)(nx + 1) * (ny + 1), 0xff), S.sd.assign(nc, 0xff);
+    SceneDesc& d = D[s];
+    memset(&d, 0, sizeof d);
+    d.nx = nx, d.ny = ny;
+    for (int k = 0; k < 2; k++)
+      d.u[k] = S.u[k].data(), d.v[k] = S.v[k].data(), d.dye[k] = S.q[k].data();
+    d.uh = S.uh.data(), d.vh = S.vh.data(), d.dyeh = S.qh.data();
+    d.lbl = S.lbl.data(), d.sten_f = S.sf.data(), d.sten_d = S.sd.data();
+    st[s] = BStep{};
+    st[s].c = cs[s], st[s].k0 = 0.9f, st[s].k1 = 1.f, st[s].k2 = 0.7f;
+  }
+  auto list = [&](int which, std::vector<BSpan>& span, std::vector<uint8_t>& chunk) {
+    span.assign(NS, BSpan{0, 0, 0, 0});
+    chunk.clear();
+    int off = 0;
+    for (int s = NS - 1; s >= 0; s--) {
+      const dim3 g = batch_grid_advect(which, sizes[s][0], sizes[s][1]);
+      const int n = (int)(g.x * g.y), padded = (n + 7) & ~7;
+      span[s] = BSpan{off, (int)g.x, (int)g.y, 0};
+      chunk.insert(chunk.end(), padded / 8, (uint8_t)s);
+      off += padded;
+    }
+    return BatchGrid{span.data(), chunk.data(), off};
+  };
+  std::vector<BSpan> sp[4];
+  std::vector<uint8_t> ch[4];
+  BatchGrid g[4];
+  for (int k = 0; k < 4; k++) g[k] = list(k, sp[k], ch[k]);
+  for (int mac = 0; mac < 2; mac++) {
+    // batched: velocity [0] -> [1], then dye [0] -> [1] along velocity [1]
+    launch_batch_advect_faces(nullptr, D.data(), g[kBAdvectFaces], st.data(), 0, mac);
+    if (mac) launch_batch_maccormack_faces(nullptr, D.data(), g[kBMacCormackFaces], st.data(), 0);
+    launch_batch_advect_dye(nullptr, D.data(), g[kBAdvectDye], st.data(), 0, mac);
+    if (mac) launch_batch_maccormack_dye(nullptr, D.data(), g[kBMacCormackDye], st.data(), 0);
+    for (int s = 0; s < NS; s++) {
+      const int nx = sizes[s][0], ny = sizes[s][1];
+      const size_t nu = (size_t)(nx + 1) * ny, nv = (size_t)nx * (ny + 1), nc = (size_t)nx * ny;
+      Scene& S = sc[s];
+      const float c = cs[s];
+      std::vector<float> a(nu), b(nv), ah(nu), bh(nv), o(3 * nc), oh(3 * nc);
+      std::vector<uint8_t> sf((size_t)(nx + 1) * (ny + 1)), sd(nc);
+      if (mac) {
+        launch_advect_faces(nullptr, S.u[0].data(), S.v[0].data(), S.u[0].data(), S.v[0].data(),
+                            ah.data(), bh.data(), S.lbl.data(), nx, ny, c, true, sf.data());
+        launch_maccormack_faces(nullptr, S.u[0].data(), S.v[0].data(), ah.data(), bh.data(),
+                                a.data(), b.data(), S.lbl.data(), nx, ny, c, true, sf.data());
+        launch_advect_dye(nullptr, S.u[1].data(), S.v[1].data(), S.q[0].data(), oh.data(),
+                          S.lbl.data(), nx, ny, 1, c, 1.f, 1.f, 1.f, true, sd.data());
+        launch_maccormack_dye(nullptr, S.u[1].data(), S.v[1].data(), S.q[0].data(), oh.data(),
+                              o.data(), S.lbl.data(), nx, ny, 1, c, 0.9f, 1.f, 0.7f, true,
+                              sd.data());
+      } else {
+        launch_advect_faces(nullptr, S.u[0].data(), S.v[0].data(), S.u[0].data(), S.v[0].data(),
+                            a.data(), b.data(), S.lbl.data(), nx, ny, c, true);
+        launch_advect_dye(nullptr, S.u[1].data(), S.v[1].data(), S.q[0].data(), o.data(),
+                          S.lbl.data(), nx, ny, 1, c, 0.9f, 1.f, 0.7f, true);
+      }
+      fails += cmp(mac ? "batch mac u" : "batch sl u", a.data(), S.u[1].data(), nu);
+      fails += cmp(mac ? "batch mac v" : "batch sl v", b.data(), S.v[1].data(), nv);
+      fails += cmp(mac ? "batch mac dye" : "batch sl dye", o.data(), S.q[1].data(), 3 * nc);
+    }
+  }
+  return fails;
+}
+
+int main() {
+  int fails = batch_check();
   const int sizes[][2] = {{67, 45}, {130, 70}, {64, 16}, {200, 33}, {300, 100}, {199, 52}, {195, 51}};
   const float cs[] = {0.3f, 1.0f, 1.249f, 1.3f, 3.7f};  // speeds up to 1: displacement up to c cells
   for (auto& sz : sizes)

@@ -13,7 +13,7 @@ CSRC = os.path.join(os.path.dirname(__file__), "..", "..", "flowlab_amd", "csrc"
 
 def make(out: str) -> None:
     os.makedirs(out, exist_ok=True)
-    for f in ("fh_device.hpp", "fh_stencil.hpp", "fh_launch.hpp"):
+    for f in ("fh_device.hpp", "fh_stencil.hpp", "fh_launch.hpp", "fh_batchdesc.hpp"):
         shutil.copy(os.path.join(CSRC, f), out)
     p = os.path.join(out, "fh_device.hpp")
     dev = open(p).read()
