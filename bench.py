@@ -48,11 +48,12 @@ WORKLOAD_STEPS = 200
 #   maccormack_dye: u, v 8 + 3 channels of q^n and of the forward pass 24 +
 #     3 out 12 + label 1 + recorded stencils 1; advect_dye: u, v 8 + 3 in 12 +
 #     3 out 12 + label 1 + recorded stencils 1.
-#   dc_update: a cycle's correction pass reads e 4 + p 8 + div 8 + label 1 and
-#     writes p 8 + r32 4 (33); the solve's initial defect pass has no e and
-#     writes no p (21).  Under the flexible-CG solver both also write the FP64
-#     residual (+8).  A step's launches are one initial pass and one per
-#     cycle; the per-launch figure is their mean.
+#   dc_update: a cycle's correction pass (the launches the trace brackets)
+#     reads e 4 + p 8 + div 8 + label 1 and writes p 8 + r32 4 (33); the
+#     solve's initial defect pass has no e and writes no p (21).  Under the
+#     flexible-CG solver both also write the FP64 residual (+8).  The counter
+#     table counts every dispatch, one initial pass and one cycle per solve
+#     under flexible CG, so its algorithmic figure is their mean.
 #   smooth_restrict_fine: b 4 + label 1 + x 4 written + coarse b 1.
 #   prolong_smooth_fine: x 4 + b 4 + label 1 + coarse e 1 + x 4 written.
 #   fcg_dots: z 4 + previous direction 4 + r 8 + info 1.
@@ -79,16 +80,21 @@ SOLVER_NAMES = {
 PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_plume_8192.json")
 
 
-def algo_bytes(kernel, solver, launches_per_step=2.0):
-    """Algorithmic bytes per cell of one traced launch of `kernel` (for
-    dc_update the mean over a step's initial pass and `launches_per_step` - 1
-    correction passes)."""
+def algo_bytes(kernel, solver):
+    """Algorithmic bytes per cell of one traced launch of `kernel`."""
     b = ALGO_BYTES_PER_CELL.get(kernel)
+    if kernel == "dc_update" and solver == "mg-fcg":
+        b += 8.0
+    return b
+
+
+def algo_bytes_all_dispatches(kernel, solver):
+    """Algorithmic bytes per cell of the mean dispatch of `kernel` in a
+    flexible-CG solve (dc_update: one initial pass, one correction pass)."""
     if kernel == "dc_update":
         r64 = 8.0 if solver == "mg-fcg" else 0.0
-        n = max(launches_per_step, 1.0)
-        b = (21.0 + r64 + (n - 1.0) * (b + r64)) / n
-    return b
+        return (21.0 + r64 + ALGO_BYTES_PER_CELL["dc_update"] + r64) / 2.0
+    return algo_bytes(kernel, solver)
 
 
 def parse():
@@ -200,8 +206,7 @@ def pmc_summary(args):
         fv, wv = F[(kn, g)], W.get((kn, g), [0.0])
         fb = sum(fv) / len(fv) * 1024 * sf
         wb = sum(wv) / len(wv) * 1024 * sw
-        # dc_update: one initial pass and one cycle per solve (flexible CG)
-        a = algo_bytes(key, args.solver, 2.0) * cells
+        a = algo_bytes_all_dispatches(key, args.solver) * cells
         out["kernels"][key] = {"rocprof_name": short(kn), "dispatches": len(fv),
                                "fetch_bytes": fb, "write_bytes": wb, "traffic_bytes": fb + wb,
                                "algorithmic_bytes": a, "ratio": (fb + wb) / a}
@@ -269,7 +274,7 @@ def kernel_table(sim, cells, solver, steps=2):
         if not n:
             continue
         avg = ms / n
-        bpc = algo_bytes(k, solver, n / steps)
+        bpc = algo_bytes(k, solver)
         gbs = bpc * cells / (avg * 1e-3) / 1e9
         out[k] = {"avg_launch_ms": avg, "launches_per_step": n / steps,
                   "ms_per_step": ms / steps, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
@@ -631,7 +636,7 @@ def main():
         log("leg C2 through Node")
         out_legs["C2_karman_2048x1024_node_boundary"] = run_node_c2(args)
     dom = h["dominant"]
-    bpc = algo_bytes(dom, args.solver, h["n_launch"] / max(K, 1))
+    bpc = algo_bytes(dom, args.solver)
     k_avg_ms = h["k_ms"] / max(h["n_launch"], 1)
     bytes_per_launch = bpc * cells
     achieved = bytes_per_launch / (k_avg_ms * 1e-3) / 1e9 if h["n_launch"] else None

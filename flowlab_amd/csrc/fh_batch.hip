@@ -188,7 +188,7 @@ int prepare(fh_batch* b) {
     d.p[0] = e->p, d.p[1] = e->p2, d.div = e->div, d.rvec = e->rvec, d.fcg = e->fcg;
     d.dvec[0] = e->dvec[0], d.dvec[1] = e->dvec[1];
     d.partials = e->partials, d.sums = e->sums, d.d_res = e->d_res, d.done = e->done;
-    const dim3 rg = batch_grid_reduce(e->nx, e->ny, &d.red_rows);
+    const dim3 rg = batch_grid_reduce(e->nx, e->ny);
     d.red_blocks = (int)(rg.x * rg.y);
     if (d.red_blocks > kFoldAbove || e->nlev_dc > kBatchMaxLevels) {
       b_err = "fh_batch_step: scene " + std::to_string(s) +
@@ -230,10 +230,7 @@ int prepare(fh_batch* b) {
     }
     lb.add(&b->g_proj[k], b->order, grids);
   }
-  for (int s = 0; s < ns; s++) {
-    int rows;
-    grids[s] = batch_grid_reduce(b->desc[s].nx, b->desc[s].ny, &rows);
-  }
+  for (int s = 0; s < ns; s++) grids[s] = batch_grid_reduce(b->desc[s].nx, b->desc[s].ny);
   lb.add(&b->g_red, b->order, grids);
   for (int s = 0; s < ns; s++) grids[s] = dim3(kBatchSpeedBlocks, 1);
   lb.add(&b->g_speed, b->order, grids);

@@ -23,7 +23,7 @@ namespace fh {
 
 constexpr int kBatchMaxScenes = 255;
 constexpr int kBatchMaxLevels = 16;
-constexpr int kBatchSpeedBlocks = 32;  // blocks of the max-speed reduction per scene
+constexpr int kBatchSpeedBlocks = 128;  // blocks of the max-speed reduction per scene
 
 struct LevelDesc {
   float *x, *xa, *b;
@@ -44,8 +44,7 @@ struct SceneDesc {
   float* dvec[2];
   double *partials, *sums, *d_res;
   int* done;
-  int red_rows;    // rows per thread of the defect/CG passes (4 or 1)
-  int red_blocks;  // their partials
+  int red_blocks;  // partials of the defect / CG passes
   int lc;          // levels [0, lc) tiled; [lc, nlev) in the one-workgroup sub-cycle
   LevelDesc lv[kBatchMaxLevels];
   CoarseLevels cl;
@@ -96,8 +95,8 @@ enum BatchAdvect { kBAdvectFaces, kBMacCormackFaces, kBAdvectDye, kBMacCormackDy
 dim3 batch_grid_advect(int which, int nx, int ny);
 enum BatchProject { kBVorticity, kBForces, kBBuoyancy, kBDivergence, kBProject };
 dim3 batch_grid_project(int which, int nx, int ny);
-// The defect-correction / CG passes' grid and rows per thread.
-dim3 batch_grid_reduce(int nx, int ny, int* rows);
+// The defect-correction / CG passes' grid.
+dim3 batch_grid_reduce(int nx, int ny);
 
 // ---- batched launchers (one launch over every scene of the work list) --------
 // q: the step's parity (SceneDesc).  mac: the MacCormack forward pass (into

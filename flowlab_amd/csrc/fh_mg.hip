@@ -190,25 +190,22 @@ static inline dim3 reduce_grid(int nx, int ny) {
 // The defect-correction and CG passes walk kRedRows consecutive rows per
 // thread and iteration, every stencil value of those rows loaded before any
 // arithmetic or store (consecutive rows share their north/south loads, and a
-// CU keeps enough bytes in flight to stream HBM).  Grids too small to fill
-// the chip that way (fewer than kRedMinBlocks blocks) keep one row per
-// thread with reduce_grid's more numerous blocks.  The block count is capped
-// at kMaxPartials (rows beyond it are walked with the grid's stride).
-constexpr int kRedRows = 4, kRedMinBlocks = 1024;
+// CU keeps enough bytes in flight to stream HBM).  The block count is capped
+// at kMaxPartials (rows beyond it are walked with the grid's stride).  Grids
+// below ~1 M cells have fewer blocks than CUs this way; one row per thread
+// over more blocks did not make such a grid's solve faster on its own (it is
+// bound by launch latency there), and in a batched launch the scenes
+// together fill the chip, where one row per thread ran the 64-scene batch's
+// CG update at 0.47 of the HBM peak.
+constexpr int kRedRows = 4;
 
-struct RGrid {
-  dim3 g;
-  int rows;  // rows per thread and iteration: kRedRows or 1
-};
-
-static inline RGrid reduce_grid_rows(int nx, int ny) {
+static inline dim3 reduce_grid_rows(int nx, int ny) {
   const int gx = (nx + BX - 1) / BX;
   const int trows = (ny + kRedRows - 1) / kRedRows;
-  int gy = (trows + BY - 1) / BY;
-  if ((long)gx * gy < kRedMinBlocks) return {reduce_grid(nx, ny), 1};
+  const int gy = (trows + BY - 1) / BY;
   int cap = kMaxPartials / gx;
   if (cap < 1) cap = 1;
-  return {dim3(gx, gy < cap ? gy : cap), kRedRows};
+  return dim3(gx, gy < cap ? gy : cap);
 }
 
 // Column i of a field over the rows j0-1 .. j0+R (clamped to the domain: c),
@@ -731,18 +728,13 @@ void launch_dc_update(hipStream_t s, const double* po, double* pn, const float* 
 void launch_dc_correct(hipStream_t s, const double* po, double* pn, const float* e,
                        const double* div, const uint8_t* info, float* r32, int nx, int ny,
                        double scale, double* partials, const int* done, double* r64) {
-  const RGrid rg = reduce_grid_rows(nx, ny);
-  if (rg.rows == kRedRows)
-    k_dc_update<kRedRows><<<rg.g, dim3(BX, BY), 0, s>>>(po, pn, e, div, info, r32, r64, nx, ny,
-                                                        scale, partials, done);
-  else
-    k_dc_update<1><<<rg.g, dim3(BX, BY), 0, s>>>(po, pn, e, div, info, r32, r64, nx, ny, scale,
-                                                 partials, done);
+  k_dc_update<kRedRows><<<reduce_grid_rows(nx, ny), dim3(BX, BY), 0, s>>>(
+      po, pn, e, div, info, r32, r64, nx, ny, scale, partials, done);
 }
 
 void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                     double* host_sums, int* done, double scale, double tol) {
-  const dim3 g = reduce_grid_rows(nx, ny).g;
+  const dim3 g = reduce_grid_rows(nx, ny);
   int n = g.x * g.y;
   partials = folded(s, partials, &n, done, 0);
   k_finalize_sum4<<<1, 1024, 0, s>>>(partials, n, sums, host_sums, done, scale, tol, 0, nullptr);
@@ -750,7 +742,7 @@ void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, doubl
 
 void launch_dc_norm_init(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                          double* host_sums, int* done, double scale) {
-  const dim3 g = reduce_grid_rows(nx, ny).g;
+  const dim3 g = reduce_grid_rows(nx, ny);
   int n = g.x * g.y;
   partials = folded(s, partials, &n, done, 1);
   k_finalize_sum4<<<1, 1024, 0, s>>>(partials, n, sums, host_sums, done, scale, 0.0, 1, nullptr);
@@ -759,13 +751,10 @@ void launch_dc_norm_init(hipStream_t s, const double* partials, int nx, int ny, 
 void launch_fcg_dots(hipStream_t s, const float* z, const float* dprev, const double* r,
                      const uint8_t* info, int nx, int ny, int first, double* partials,
                      double* state, const int* done) {
-  const RGrid rg = reduce_grid_rows(nx, ny);
-  if (rg.rows == kRedRows)
-    k_fcg_dots<kRedRows><<<rg.g, dim3(BX, BY), 0, s>>>(z, dprev, r, info, nx, ny, first, partials,
-                                                       done);
-  else
-    k_fcg_dots<1><<<rg.g, dim3(BX, BY), 0, s>>>(z, dprev, r, info, nx, ny, first, partials, done);
-  int n = rg.g.x * rg.g.y;
+  const dim3 g = reduce_grid_rows(nx, ny);
+  k_fcg_dots<kRedRows><<<g, dim3(BX, BY), 0, s>>>(z, dprev, r, info, nx, ny, first, partials,
+                                                  done);
+  int n = g.x * g.y;
   const double* pp = folded(s, partials, &n, done, 0);
   k_fcg_coef<<<1, 1024, 0, s>>>(pp, n, state, first, done);
 }
@@ -773,19 +762,14 @@ void launch_fcg_dots(hipStream_t s, const float* z, const float* dprev, const do
 void launch_fcg_update(hipStream_t s, const float* z, const float* dprev, float* dnext, double* p,
                        double* r, const uint8_t* info, float* r32, int nx, int ny, int first,
                        const double* state, double* partials, const int* done) {
-  const RGrid rg = reduce_grid_rows(nx, ny);
-  if (rg.rows == kRedRows)
-    k_fcg_update<kRedRows><<<rg.g, dim3(BX, BY), 0, s>>>(z, dprev, dnext, p, r, info, r32, state,
-                                                         nx, ny, first, partials, done);
-  else
-    k_fcg_update<1><<<rg.g, dim3(BX, BY), 0, s>>>(z, dprev, dnext, p, r, info, r32, state, nx, ny,
-                                                  first, partials, done);
+  k_fcg_update<kRedRows><<<reduce_grid_rows(nx, ny), dim3(BX, BY), 0, s>>>(
+      z, dprev, dnext, p, r, info, r32, state, nx, ny, first, partials, done);
 }
 
 void launch_fcg_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                      const double* sums0, double* host_sums, int* done, double scale,
                      double tol) {
-  const dim3 g = reduce_grid_rows(nx, ny).g;
+  const dim3 g = reduce_grid_rows(nx, ny);
   int n = g.x * g.y;
   partials = folded(s, partials, &n, done, 0);
   k_finalize_sum4<<<1, 1024, 0, s>>>(partials, n, sums, host_sums, done, scale, tol, 0, sums0);
@@ -814,12 +798,8 @@ __global__ __launch_bounds__(256) void kb_dc_update(const SceneDesc* __restrict_
   double* pn = init ? nullptr : d.p[q ^ 1];
   const float* e = init ? nullptr : d.lv[0].x;
   const int* done = init ? nullptr : d.done;
-  if (d.red_rows == kRedRows)
-    dc_update_body<kRedRows>(d.p[q], pn, e, d.div, d.lv[0].info, d.lv[0].b, d.rvec, d.nx, d.ny,
-                             st[s].scale, d.partials, done, bx, by, gy, bid);
-  else
-    dc_update_body<1>(d.p[q], pn, e, d.div, d.lv[0].info, d.lv[0].b, d.rvec, d.nx, d.ny,
-                      st[s].scale, d.partials, done, bx, by, gy, bid);
+  dc_update_body<kRedRows>(d.p[q], pn, e, d.div, d.lv[0].info, d.lv[0].b, d.rvec, d.nx, d.ny,
+                           st[s].scale, d.partials, done, bx, by, gy, bid);
 }
 
 __global__ __launch_bounds__(1024) void kb_finalize(const SceneDesc* __restrict__ D,
@@ -840,12 +820,8 @@ __global__ __launch_bounds__(256) void kb_fcg_dots(const SceneDesc* __restrict__
   int s, bx, by, gy, bid;
   if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
   const SceneDesc& d = D[s];
-  if (d.red_rows == kRedRows)
-    fcg_dots_body<kRedRows>(d.lv[0].x, d.dvec[cp], d.rvec, d.lv[0].info, d.nx, d.ny, first,
-                            d.partials, d.done, bx, by, gy, bid);
-  else
-    fcg_dots_body<1>(d.lv[0].x, d.dvec[cp], d.rvec, d.lv[0].info, d.nx, d.ny, first,
-                     d.partials, d.done, bx, by, gy, bid);
+  fcg_dots_body<kRedRows>(d.lv[0].x, d.dvec[cp], d.rvec, d.lv[0].info, d.nx, d.ny, first,
+                          d.partials, d.done, bx, by, gy, bid);
 }
 
 __global__ __launch_bounds__(1024) void kb_fcg_coef(const SceneDesc* __restrict__ D, int first) {
@@ -858,20 +834,12 @@ __global__ __launch_bounds__(256) void kb_fcg_update(const SceneDesc* __restrict
   int s, bx, by, gy, bid;
   if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
   const SceneDesc& d = D[s];
-  if (d.red_rows == kRedRows)
-    fcg_update_body<kRedRows>(d.lv[0].x, d.dvec[cp], d.dvec[cp ^ 1], d.p[q ^ 1], d.rvec,
-                              d.lv[0].info, d.lv[0].b, d.fcg, d.nx, d.ny, first, d.partials,
-                              d.done, bx, by, gy, bid);
-  else
-    fcg_update_body<1>(d.lv[0].x, d.dvec[cp], d.dvec[cp ^ 1], d.p[q ^ 1], d.rvec, d.lv[0].info,
-                       d.lv[0].b, d.fcg, d.nx, d.ny, first, d.partials, d.done, bx, by, gy, bid);
+  fcg_update_body<kRedRows>(d.lv[0].x, d.dvec[cp], d.dvec[cp ^ 1], d.p[q ^ 1], d.rvec,
+                            d.lv[0].info, d.lv[0].b, d.fcg, d.nx, d.ny, first, d.partials, d.done,
+                            bx, by, gy, bid);
 }
 
-dim3 batch_grid_reduce(int nx, int ny, int* rows) {
-  const RGrid rg = reduce_grid_rows(nx, ny);
-  *rows = rg.rows;
-  return rg.g;
-}
+dim3 batch_grid_reduce(int nx, int ny) { return reduce_grid_rows(nx, ny); }
 
 void launch_batch_dc_update(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
                             const BStep* st, int q, int init) {

@@ -421,8 +421,21 @@ __global__ __launch_bounds__(256) void kb_max_speed(const SceneDesc* __restrict_
   const long stride = (long)kBatchSpeedBlocks * 256;
   const long t0 = (long)bid * 256 + threadIdx.x;
   float m = 0.f;
+  // float4 loads over the whole float4s of each array (hipMalloc bases, so
+  // 16-byte aligned), two in flight per thread, then the tail
   auto scan = [&](const float* __restrict__ a, long n) {
-    for (long k = t0; k < n; k += stride) m = fmaxf(m, fabsf(a[k]));
+    const float4* a4 = reinterpret_cast<const float4*>(a);
+    const long n4 = n >> 2;
+    auto m4 = [](float4 x) {
+      return fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fmaxf(fabsf(x.z), fabsf(x.w)));
+    };
+    long k = t0;
+    for (; k + stride < n4; k += 2 * stride) {
+      const float4 x = a4[k], y = a4[k + stride];
+      m = fmaxf(m, fmaxf(m4(x), m4(y)));
+    }
+    if (k < n4) m = fmaxf(m, m4(a4[k]));
+    for (long t = (n4 << 2) + t0; t < n; t += stride) m = fmaxf(m, fabsf(a[t]));
   };
   scan(d.u[q ^ 1], (long)(d.nx + 1) * d.ny);
   scan(d.v[q ^ 1], (long)d.nx * (d.ny + 1));
