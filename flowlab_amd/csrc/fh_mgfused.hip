@@ -26,8 +26,8 @@
 
 namespace fh {
 
-template <int TX, int TY>
-__global__ __launch_bounds__(NT) void k_smooth_restrict(const float* __restrict__ xin,
+template <int TX, int TY, int W>
+__global__ __launch_bounds__(W) void k_smooth_restrict(const float* __restrict__ xin,
                                                         float* __restrict__ xout,
                                                         const float* __restrict__ b,
                                                         const uint8_t* __restrict__ info, int nx,
@@ -36,11 +36,11 @@ __global__ __launch_bounds__(NT) void k_smooth_restrict(const float* __restrict_
                                                         const int* __restrict__ done) {
   int bx, by;
   xcd_block(&bx, &by);
-  smooth_restrict_tile<TX, TY>(xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done, bx, by);
+  smooth_restrict_tile<TX, TY, W>(xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done, bx, by);
 }
 
-template <int TX, int TY>
-__global__ __launch_bounds__(NT) void k_prolong_smooth(const float* __restrict__ xin,
+template <int TX, int TY, int W>
+__global__ __launch_bounds__(W) void k_prolong_smooth(const float* __restrict__ xin,
                                                        float* __restrict__ xout,
                                                        const float* __restrict__ b,
                                                        const uint8_t* __restrict__ info, int nx,
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(NT) void k_prolong_smooth(const float* __restrict__
                                                        const int* __restrict__ done) {
   int bx, by;
   xcd_block(&bx, &by);
-  prolong_smooth_tile<TX, TY>(xin, xout, b, info, nx, ny, e, cnx, cny, done, bx, by);
+  prolong_smooth_tile<TX, TY, W>(xin, xout, b, info, nx, ny, e, cnx, cny, done, bx, by);
 }
 
 template <int G>
@@ -84,33 +84,31 @@ void launch_cell_info(hipStream_t s, const uint8_t* lbl, int nx, int ny, uint8_t
   k_cell_info<<<g, dim3(64, 4), 0, s>>>(lbl, nx, ny, info);
 }
 
-template <int TX, int TY>
+template <int TX, int TY, int W>
 static void sr_launch(hipStream_t s, const float* xin, float* xout, const float* b,
                       const uint8_t* info, int nx, int ny, float* bc, int cnx, int cny, int zero,
                       const int* done) {
   dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY);
-  k_smooth_restrict<TX, TY><<<g, NT, 0, s>>>(xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done);
+  k_smooth_restrict<TX, TY, W><<<g, W, 0, s>>>(xin, xout, b, info, nx, ny, bc, cnx, cny, zero,
+                                               done);
 }
 
-template <int TX, int TY>
+template <int TX, int TY, int W>
 static void ps_launch(hipStream_t s, const float* xin, float* xout, const float* b,
                       const uint8_t* info, int nx, int ny, const float* e, int cnx, int cny,
                       const int* done) {
   dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY);
-  k_prolong_smooth<TX, TY><<<g, NT, 0, s>>>(xin, xout, b, info, nx, ny, e, cnx, cny, done);
+  k_prolong_smooth<TX, TY, W><<<g, W, 0, s>>>(xin, xout, b, info, nx, ny, e, cnx, cny, done);
 }
 
 void launch_smooth_restrict(hipStream_t s, int shape, const float* xin, float* xout,
                             const float* b, const uint8_t* info, int nx, int ny, float* bc, int cnx,
                             int cny, int zero, const int* done) {
   switch (shape) {
-    case 1: sr_launch<32, 32>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
-    case 2: sr_launch<64, 16>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
-    case 3: sr_launch<128, 16>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
-    case 4: sr_launch<32, 16>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
-    case 5: sr_launch<16, 16>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
-    case 6: sr_launch<32, 8>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
-    default: sr_launch<64, 32>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
+#define FH_SR_CASE(k, tx, ty, w) \
+  case k: sr_launch<tx, ty, w>(s, xin, xout, b, info, nx, ny, bc, cnx, cny, zero, done); break;
+    FH_TILE_SHAPES(FH_SR_CASE)
+#undef FH_SR_CASE
   }
 }
 
@@ -118,13 +116,10 @@ void launch_prolong_smooth(hipStream_t s, int shape, const float* xin, float* xo
                            const float* b, const uint8_t* info, int nx, int ny, const float* e,
                            int cnx, int cny, const int* done) {
   switch (shape) {
-    case 1: ps_launch<32, 32>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
-    case 2: ps_launch<64, 16>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
-    case 3: ps_launch<128, 16>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
-    case 4: ps_launch<32, 16>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
-    case 5: ps_launch<16, 16>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
-    case 6: ps_launch<32, 8>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
-    default: ps_launch<64, 32>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
+#define FH_PS_CASE(k, tx, ty, w) \
+  case k: ps_launch<tx, ty, w>(s, xin, xout, b, info, nx, ny, e, cnx, cny, done); break;
+    FH_TILE_SHAPES(FH_PS_CASE)
+#undef FH_PS_CASE
   }
 }
 
@@ -152,27 +147,28 @@ void launch_coarse_vcycle(hipStream_t s, int group, const CoarseLevels& L, const
 // tile bodies on each scene's level buffers, so the same bits as its own
 // V-cycle.
 
-template <int TX, int TY>
-__global__ __launch_bounds__(NT) void kb_smooth_restrict(const SceneDesc* __restrict__ D,
-                                                         BatchGrid g, int l) {
-  int s, bx, by, gy, bid;
-  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
-  const SceneDesc& d = D[s];
-  const LevelDesc& L = d.lv[l];
-  const LevelDesc& N = d.lv[l + 1];
-  smooth_restrict_tile<TX, TY>(nullptr, L.xa, L.b, L.info, L.nx, L.ny, N.b, N.nx, N.ny, 1, d.done,
-                               bx, by);
-}
-
-template <int TX, int TY>
-__global__ __launch_bounds__(NT) void kb_prolong_smooth(const SceneDesc* __restrict__ D,
+template <int TX, int TY, int W>
+__global__ __launch_bounds__(W) void kb_smooth_restrict(const SceneDesc* __restrict__ D,
                                                         BatchGrid g, int l) {
   int s, bx, by, gy, bid;
   if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
   const SceneDesc& d = D[s];
   const LevelDesc& L = d.lv[l];
   const LevelDesc& N = d.lv[l + 1];
-  prolong_smooth_tile<TX, TY>(L.xa, L.x, L.b, L.info, L.nx, L.ny, N.x, N.nx, N.ny, d.done, bx, by);
+  smooth_restrict_tile<TX, TY, W>(nullptr, L.xa, L.b, L.info, L.nx, L.ny, N.b, N.nx, N.ny, 1,
+                                  d.done, bx, by);
+}
+
+template <int TX, int TY, int W>
+__global__ __launch_bounds__(W) void kb_prolong_smooth(const SceneDesc* __restrict__ D,
+                                                       BatchGrid g, int l) {
+  int s, bx, by, gy, bid;
+  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
+  const SceneDesc& d = D[s];
+  const LevelDesc& L = d.lv[l];
+  const LevelDesc& N = d.lv[l + 1];
+  prolong_smooth_tile<TX, TY, W>(L.xa, L.x, L.b, L.info, L.nx, L.ny, N.x, N.nx, N.ny, d.done, bx,
+                                 by);
 }
 
 __global__ __launch_bounds__(NC) void kb_coarse_vcycle(const SceneDesc* __restrict__ D) {
@@ -183,39 +179,33 @@ __global__ __launch_bounds__(NC) void kb_coarse_vcycle(const SceneDesc* __restri
                          smem);
 }
 
-template <int TX, int TY>
+template <int TX, int TY, int W>
 static void bsr(hipStream_t s, const SceneDesc* D, const BatchGrid& g, int l) {
-  kb_smooth_restrict<TX, TY><<<g.blocks, NT, 0, s>>>(D, g, l);
+  kb_smooth_restrict<TX, TY, W><<<g.blocks, W, 0, s>>>(D, g, l);
 }
 
-template <int TX, int TY>
+template <int TX, int TY, int W>
 static void bps(hipStream_t s, const SceneDesc* D, const BatchGrid& g, int l) {
-  kb_prolong_smooth<TX, TY><<<g.blocks, NT, 0, s>>>(D, g, l);
+  kb_prolong_smooth<TX, TY, W><<<g.blocks, W, 0, s>>>(D, g, l);
 }
 
 void launch_batch_smooth_restrict(hipStream_t s, int shape, const SceneDesc* D,
                                   const BatchGrid& g, int l) {
   switch (shape) {
-    case 1: bsr<32, 32>(s, D, g, l); break;
-    case 2: bsr<64, 16>(s, D, g, l); break;
-    case 3: bsr<128, 16>(s, D, g, l); break;
-    case 4: bsr<32, 16>(s, D, g, l); break;
-    case 5: bsr<16, 16>(s, D, g, l); break;
-    case 6: bsr<32, 8>(s, D, g, l); break;
-    default: bsr<64, 32>(s, D, g, l); break;
+#define FH_BSR_CASE(k, tx, ty, w) \
+  case k: bsr<tx, ty, w>(s, D, g, l); break;
+    FH_TILE_SHAPES(FH_BSR_CASE)
+#undef FH_BSR_CASE
   }
 }
 
 void launch_batch_prolong_smooth(hipStream_t s, int shape, const SceneDesc* D, const BatchGrid& g,
                                  int l) {
   switch (shape) {
-    case 1: bps<32, 32>(s, D, g, l); break;
-    case 2: bps<64, 16>(s, D, g, l); break;
-    case 3: bps<128, 16>(s, D, g, l); break;
-    case 4: bps<32, 16>(s, D, g, l); break;
-    case 5: bps<16, 16>(s, D, g, l); break;
-    case 6: bps<32, 8>(s, D, g, l); break;
-    default: bps<64, 32>(s, D, g, l); break;
+#define FH_BPS_CASE(k, tx, ty, w) \
+  case k: bps<tx, ty, w>(s, D, g, l); break;
+    FH_TILE_SHAPES(FH_BPS_CASE)
+#undef FH_BPS_CASE
   }
 }
 

@@ -10,17 +10,28 @@
 
 namespace fh {
 
-constexpr int NT = 256;  // threads per tile workgroup
+constexpr int NT = 256;  // threads per tile workgroup unless the shape says otherwise
 
-// Tile shapes (output cells per workgroup, both sides even) the tiled
-// smoothers are instantiated for; the autotuners choose among them per level.
+// Tile shapes (output cells per workgroup, both sides even, and the
+// workgroup's threads) the tiled smoothers are instantiated for; the
+// autotuners choose among them per level.  A larger tile recomputes less
+// halo (the 16x16 tile sweeps 2.25 cells per output cell, 64x64 1.27) but
+// holds fewer workgroups per CU; the wider workgroups split a large tile's
+// sweeps over more waves.  FH_TILE_SHAPES lists them for the launchers'
+// switches: (index, tx, ty, threads).
+#define FH_TILE_SHAPES(X)                                                            \
+  X(0, 64, 32, 256) X(1, 32, 32, 256) X(2, 64, 16, 256) X(3, 128, 16, 256)          \
+  X(4, 32, 16, 256) X(5, 16, 16, 256) X(6, 32, 8, 256) X(7, 64, 32, 512)            \
+  X(8, 32, 32, 512) X(9, 64, 64, 1024) X(10, 128, 32, 1024)
 struct TileShape {
-  int tx, ty;
+  int tx, ty, nt;
   const char* name;
 };
 constexpr TileShape kTileShapes[] = {
-    {64, 32, "64x32"}, {32, 32, "32x32"}, {64, 16, "64x16"}, {128, 16, "128x16"},
-    {32, 16, "32x16"}, {16, 16, "16x16"}, {32, 8, "32x8"}};
+    {64, 32, 256, "64x32"},       {32, 32, 256, "32x32"},      {64, 16, 256, "64x16"},
+    {128, 16, 256, "128x16"},     {32, 16, 256, "32x16"},      {16, 16, 256, "16x16"},
+    {32, 8, 256, "32x8"},         {64, 32, 512, "64x32/512"},  {32, 32, 512, "32x32/512"},
+    {64, 64, 1024, "64x64/1024"}, {128, 32, 1024, "128x32/1024"}};
 constexpr int kNumTileShapes = sizeof(kTileShapes) / sizeof(kTileShapes[0]);
 constexpr int HA = 5;    // halo of the pre-smooth+restrict tile (4 half-sweeps + residual)
 constexpr int HB = 4;    // halo of the prolong+post-smooth tile (4 half-sweeps)
@@ -60,11 +71,11 @@ __device__ __forceinline__ float gs_recip(int cnt) {
 // global sweep, and the cells that read them are themselves nearer than s+1.
 // The exact region is the same as when the updated region shrinks by one
 // ring per half-sweep, so the tile interior gets the same bits.
-template <int LX, int LY>
+template <int LX, int LY, int W = NT>
 struct TileSweep {
   static constexpr int P = (LX + 1) / 2;  // pairs per row
   static constexpr int ITEMS = P * (LY - 2);
-  static constexpr int K = (ITEMS + NT - 1) / NT;
+  static constexpr int K = (ITEMS + W - 1) / W;
   int l[2][K];
   float b[2][K], r[2][K];
 
@@ -75,7 +86,7 @@ struct TileSweep {
     for (int c = 0; c < 2; c++)
 #pragma unroll
       for (int k = 0; k < K; k++) {
-        const int q = tid + k * NT;
+        const int q = tid + k * W;
         const int lj = 1 + q / P;
         const int pc = q - (lj - 1) * P;
         const int li = 2 * pc + ((lj + par0 + c) & 1);
@@ -208,7 +219,7 @@ struct CoarseSweep {
 // The defect-correction loop's `done` flag is read alongside: returns false
 // (before any LDS write, uniformly for the workgroup) if it is set, so the
 // flag costs no memory round trip of its own.
-template <int LX, int LY>
+template <int LX, int LY, int W = NT>
 __device__ __forceinline__ bool stage_tile(float* sx, float* sb, uint8_t* si,
                                            const float* __restrict__ xin,
                                            const float* __restrict__ b,
@@ -216,7 +227,7 @@ __device__ __forceinline__ bool stage_tile(float* sx, float* sb, uint8_t* si,
                                            int gx0, int gy0, int tid,
                                            const int* __restrict__ done) {
   const int stop = done ? *done : 0;
-  constexpr int N = LX * LY, K = (N + NT - 1) / NT;
+  constexpr int N = LX * LY, K = (N + W - 1) / W;
   float rx[K], rb[K];
   uint8_t ri[K];
   if (gx0 >= 0 && gy0 >= 0 && gx0 + LX <= nx && gy0 + LY <= ny) {
@@ -225,7 +236,7 @@ __device__ __forceinline__ bool stage_tile(float* sx, float* sb, uint8_t* si,
     const int base = gx0 + gy0 * nx;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      const int q = min(tid + k * NT, N - 1);
+      const int q = min(tid + k * W, N - 1);
       const int lj = q / LX, li = q - lj * LX;
       const int g = base + li + lj * nx;
       rx[k] = xin ? xin[g] : 0.f;
@@ -235,7 +246,7 @@ __device__ __forceinline__ bool stage_tile(float* sx, float* sb, uint8_t* si,
   } else {
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      int q = tid + k * NT;
+      int q = tid + k * W;
       int lj = q / LX, li = q - lj * LX;
       int gi = gx0 + li, gj = gy0 + lj;
       bool in = q < N && gi >= 0 && gj >= 0 && gi < nx && gj < ny;
@@ -248,7 +259,7 @@ __device__ __forceinline__ bool stage_tile(float* sx, float* sb, uint8_t* si,
   if (stop) return false;
 #pragma unroll
   for (int k = 0; k < K; k++) {
-    int q = tid + k * NT;
+    int q = tid + k * W;
     if (q < N) {
       sx[q] = rx[k];
       sb[q] = rb[k];
@@ -275,7 +286,7 @@ __device__ __forceinline__ float lds_residual(XP sx, BP sb, IP si, int LX, int l
 // One TX x TY tile (tile index bx, by) per call; TX and TY even, so the
 // children of a coarse cell never straddle tiles.  A caller that runs
 // several tiles per workgroup puts a barrier between calls (LDS reuse).
-template <int TX, int TY>
+template <int TX, int TY, int W = NT>
 __device__ __forceinline__ void smooth_restrict_tile(const float* __restrict__ xin,
                                                      float* __restrict__ xout,
                                                      const float* __restrict__ b,
@@ -295,10 +306,11 @@ __device__ __forceinline__ void smooth_restrict_tile(const float* __restrict__ x
   __asm__ volatile("" : "+v"(tid));
   const int ox = bx * TX, oy = by * TY;
   const int gx0 = ox - HA, gy0 = oy - HA;
-  if (!stage_tile<AX, AY>(sx, sb, si, zero ? nullptr : xin, b, info, nx, ny, gx0, gy0, tid, done))
+  if (!stage_tile<AX, AY, W>(sx, sb, si, zero ? nullptr : xin, b, info, nx, ny, gx0, gy0, tid,
+                            done))
     return;
   __syncthreads();
-  TileSweep<AX, AY> sw;
+  TileSweep<AX, AY, W> sw;
   sw.setup(sb, si, (gx0 + gy0) & 1, tid);
   sw.template sweep<0>(sx, zero);
   __syncthreads();
@@ -310,7 +322,7 @@ __device__ __forceinline__ void smooth_restrict_tile(const float* __restrict__ x
   __syncthreads();
   // coarse cells of this tile: children (2I+di, 2J+dj) are tile-interior
   const int CX = TX / 2, CY = TY / 2;
-  for (int q = tid; q < CX * CY; q += NT) {
+  for (int q = tid; q < CX * CY; q += W) {
     int cj = q / CX, ci = q - cj * CX;
     int I = ox / 2 + ci, J = oy / 2 + cj;
     if (I >= cnx || J >= cny) continue;
@@ -324,7 +336,7 @@ __device__ __forceinline__ void smooth_restrict_tile(const float* __restrict__ x
       }
     bc[I + J * cnx] = s;
   }
-  for (int q = tid; q < TX * TY; q += NT) {
+  for (int q = tid; q < TX * TY; q += W) {
     int lj = q / TX, li = q - lj * TX;
     int gi = ox + li, gj = oy + lj;
     if (gi < nx && gj < ny) xout[gi + gj * nx] = sx[(li + HA) + (lj + HA) * AX];
@@ -334,7 +346,7 @@ __device__ __forceinline__ void smooth_restrict_tile(const float* __restrict__ x
 // ---- k_prolong_smooth --------------------------------------------------------
 // x += bilinear(e) on fluid cells (pressureMultigrid.ts:192-207), then 2
 // post-smoothing sweeps.  Reads xin, writes xout (distinct buffers).
-template <int TX, int TY>
+template <int TX, int TY, int W = NT>
 __device__ __forceinline__ void prolong_smooth_tile(const float* __restrict__ xin,
                                                     float* __restrict__ xout,
                                                     const float* __restrict__ b,
@@ -358,27 +370,27 @@ __device__ __forceinline__ void prolong_smooth_tile(const float* __restrict__ xi
   // coarse window: parents I = gi>>1 and their +-1 neighbours
   const int ex0 = (gx0 >> 1) - 1, ey0 = (gy0 >> 1) - 1;
   {
-    constexpr int K = (EX * EY + NT - 1) / NT;
+    constexpr int K = (EX * EY + W - 1) / W;
     float re[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      int q = tid + k * NT;
+      int q = tid + k * W;
       int ej = q / EX, ei = q - ej * EX;
       int I = ex0 + ei, J = ey0 + ej;
       bool in = q < EX * EY && I >= 0 && J >= 0 && I < cnx && J < cny;
       re[k] = in ? e[I + J * cnx] : 0.f;
     }
-    if (!stage_tile<BXW, BYH>(sx, sb, si, xin, b, info, nx, ny, gx0, gy0, tid, done)) return;
+    if (!stage_tile<BXW, BYH, W>(sx, sb, si, xin, b, info, nx, ny, gx0, gy0, tid, done)) return;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-      int q = tid + k * NT;
+      int q = tid + k * W;
       if (q < EX * EY) se[q] = re[k];
     }
   }
   __syncthreads();
-  TileSweep<BXW, BYH> sw;
+  TileSweep<BXW, BYH, W> sw;
   sw.setup(sb, si, (gx0 + gy0) & 1, tid);
-  for (int q = tid; q < BXW * BYH; q += NT) {
+  for (int q = tid; q < BXW * BYH; q += W) {
     if (!(si[q] & 0x80)) continue;
     int lj = q / BXW, li = q - lj * BXW;
     int gi = gx0 + li, gj = gy0 + lj;
@@ -400,7 +412,7 @@ __device__ __forceinline__ void prolong_smooth_tile(const float* __restrict__ xi
   __syncthreads();
   sw.template sweep<1>(sx, false);
   __syncthreads();
-  for (int q = tid; q < TX * TY; q += NT) {
+  for (int q = tid; q < TX * TY; q += W) {
     int lj = q / TX, li = q - lj * TX;
     int gi = ox + li, gj = oy + lj;
     if (gi < nx && gj < ny) xout[gi + gj * nx] = sx[(li + HB) + (lj + HB) * BXW];

@@ -167,21 +167,49 @@ __device__ __forceinline__ void outflow_edges_at(float* __restrict__ u, float* _
   }
 }
 
-// kRows face rows per thread: every row's loads (SubLoad) are issued before
-// any store; then the subtraction of each row, then each row's outflow (which
-// reads back only faces this thread has just written).
+// kRows face rows per thread: every row's loads are issued before any store;
+// then the subtraction of each row, then each row's outflow (which reads back
+// only faces this thread has just written).  The pressure and labels of a
+// thread's column are loaded once for rows j0-1 .. j0+kRows-1 and those of
+// the west column for rows j0 .. j0+kRows-1: row j's south cell is row j-1's
+// cell.  These are the values subtract_load reads wherever a face is
+// written (at j = ny, where the clamped south index differs, no face is).
 __device__ __forceinline__ void subtract_outflow_rows(float* __restrict__ u, float* __restrict__ v,
                                                       const double* __restrict__ p,
                                                       const uint8_t* __restrict__ lbl, int nx,
                                                       int ny, double gs, int bx, int by) {
   const int i = bx * BX + threadIdx.x, j0 = (by * BY + threadIdx.y) * kRows, su = nx + 1;
   if (i > nx || j0 > ny) return;
-  SubLoad L[kRows];
+  const int ic = min(i, nx - 1), iw = i >= 1 ? ic - 1 : ic;
+  double pc[kRows + 1], pw[kRows];
+  uint8_t lc[kRows + 1], lw[kRows];
+  float u0[kRows], v0[kRows];
 #pragma unroll
-  for (int r = 0; r < kRows; r++) L[r] = subtract_load(u, v, p, lbl, nx, ny, i, min(j0 + r, ny));
+  for (int r = 0; r <= kRows; r++) {
+    const int k = ic + min(max(j0 - 1 + r, 0), ny - 1) * nx;
+    pc[r] = p[k];
+    lc[r] = lbl[k];
+  }
 #pragma unroll
-  for (int r = 0; r < kRows; r++)
-    if (j0 + r <= ny) subtract_store(L[r], u, v, gs);
+  for (int r = 0; r < kRows; r++) {
+    const int j = min(j0 + r, ny), k = iw + min(j, ny - 1) * nx;
+    pw[r] = p[k];
+    lw[r] = lbl[k];
+    u0[r] = (j < ny && i >= 1 && i <= nx - 1) ? u[i + j * su] : 0.f;
+    v0[r] = (i < nx && j >= 1 && j <= ny - 1) ? v[i + j * nx] : 0.f;
+  }
+#pragma unroll
+  for (int r = 0; r < kRows; r++) {
+    const int j = j0 + r;
+    if (j > ny) break;
+    const bool uf = j < ny && i >= 1 && i <= nx - 1;
+    const bool vf = i < nx && j >= 1 && j <= ny - 1;
+    const uint8_t l0 = lc[r + 1];
+    if (uf && lw[r] != SOLID && l0 != SOLID)
+      u[i + j * su] = (float)((double)u0[r] - gs * (pc[r + 1] - pw[r]));
+    if (vf && lc[r] != SOLID && l0 != SOLID)
+      v[i + j * nx] = (float)((double)v0[r] - gs * (pc[r + 1] - pc[r]));
+  }
 #pragma unroll
   for (int r = 0; r < kRows; r++) {
     const int j = j0 + r;
