@@ -469,7 +469,9 @@ def run_c3(args):
 def run_c4(args):
     """The learned warm start against the zero and previous-pressure starts
     at 1024^2 and 2048^2 (3 seeded scenes each, the next step's solve to
-    1e-6): V-cycles and device ms per solve."""
+    1e-6): V-cycles and device ms per solve.  `network` is the learned start
+    applied to a zero pressure (the network's own gain over the zero start);
+    `learned` applies it to the previous pressure."""
     from flowlab_amd import learned as L
     if not os.path.exists(L.WEIGHTS):
         return None
@@ -480,6 +482,7 @@ def run_c4(args):
         r = L.evaluate(u, n, scenes=3)
         r["learned_solves_per_s_vs_previous"] = r["previous"]["ms"] / r["learned"]["ms"]
         r["learned_solves_per_s_vs_zero"] = r["zero"]["ms"] / r["learned"]["ms"]
+        r["network_vcycles_vs_zero"] = r["network"]["vcycles"] / r["zero"]["vcycles"]
         out[f"{n}x{n}"] = r
     return out
 

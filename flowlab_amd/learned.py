@@ -211,9 +211,11 @@ def evaluate(unet: UNet, n: int, scenes: int = 4, warm_steps: int = 8, seed: int
              tol: float = 1e-6):
     """For `scenes` seeded n x n scenes advanced `warm_steps` steps, solve the
     next step's pressure from (a) zero, (b) the previous pressure, (c) the
-    learned warm start, each to `tol`; returns V-cycles and device ms."""
+    learned warm start on the previous pressure, (d) the learned warm start on
+    a zero pressure (the network's own contribution, without the previous
+    pressure), each to `tol`; returns V-cycles and device ms."""
     from .engine import HipPressureSolver
-    res = {"zero": [], "previous": [], "learned": []}
+    res = {"zero": [], "previous": [], "learned": [], "network": []}
     for s in range(scenes):
         sim = _sim(n, seed + s, tol=tol)
         for _ in range(warm_steps):
@@ -227,11 +229,12 @@ def evaluate(unet: UNet, n: int, scenes: int = 4, warm_steps: int = 8, seed: int
         label = sim.f.label.copy()
         scale = sim.g.h * sim.g.h / sim.dt
         solver = HipPressureSolver(n, n, "mg-dc")
-        for name in ("zero", "previous", "learned"):
-            solver.engine.set_warm_start(unet if name == "learned" else None)
-            p = np.zeros(n * n) if name == "zero" else p_prev.copy()
+        for name in ("zero", "previous", "learned", "network"):
+            solver.engine.set_warm_start(unet if name in ("learned", "network") else None)
+            from_zero = name in ("zero", "network")
+            p = np.zeros(n * n) if from_zero else p_prev.copy()
             solver.solve(None, p, div, label, scale, 1, 1.0, 0)  # warm-up: labels, buffers
-            p = np.zeros(n * n) if name == "zero" else p_prev.copy()
+            p = np.zeros(n * n) if from_zero else p_prev.copy()
             it = solver.solve(None, p, div, label, scale, 200, 1.0, tol)
             res[name].append((it, solver.engine.last_solve_ms()))
         solver.engine.set_warm_start(None)

@@ -93,28 +93,51 @@ def test_capture_train_and_warm_start():
     p, d = e.read_pressure(), e.read_divergence()
     rel = O.relative_residual(n, n, p, d, sim.f.label, sim.g.h * sim.g.h / sim.dt)
     assert rel <= 1e-6
-    # the warm start is a residual-minimising step along the network output,
-    # so its defect is never above the previous pressure's (same inputs)
+
+
+def test_trained_network_on_held_out_scenes():
+    """The committed trained network on scenes it was not trained on (seeds
+    9101.., training used 100..499), 512 x 512 as in training, each solve to
+    1e-6 from four starts: zero, the network on a zero pressure, the previous
+    pressure (the reference's own warm start), the network on the previous
+    pressure.  Every solve reaches 1e-6 by the oracle's FP64 residual, and the
+    network's start never needs more V-cycles than the start it is applied
+    to.  No gain is asserted: measured here it needs as many V-cycles as the
+    start it corrects (the residual-minimising step along its output removes
+    too little of the error to save a cycle), and the bench's learned-start
+    figures carry the same result."""
+    import os
+    from flowlab_amd import learned as Lrn
     from flowlab_amd.engine import HipPressureSolver
-    sim2 = Lrn._sim(n, 321)
-    for _ in range(4):
-        sim2.step()
-    e2 = sim2.stepper.engine
-    p_prev = e2.read_pressure()
-    sim2.step()
-    div = e2.read_divergence()
-    label = sim2.f.label.copy()
-    scale = sim2.g.h * sim2.g.h / sim2.dt
-    solver = HipPressureSolver(n, n, "mg-dc")
-    first = {}
-    for name in ("previous", "learned"):
-        solver.engine.set_warm_start(u if name == "learned" else None)
-        p = p_prev.copy()
-        solver.solve(None, p, div, label, scale, 1, 1.0, 0)  # labels, buffers
-        p = p_prev.copy()
-        solver.solve(None, p, div, label, scale, 100, 1.0, 1e-6)
-        first[name] = float(solver.engine.solve_history()[0])
-        rel = O.relative_residual(n, n, p, div, label, scale)
-        assert rel <= 1e-6, name
-    solver.engine.set_warm_start(None)
-    assert first["learned"] <= first["previous"] * (1 + 1e-9), first
+    if not os.path.exists(Lrn.WEIGHTS):
+        pytest.skip("no trained weights")
+    u = Lrn.UNet()
+    u.load(Lrn.WEIGHTS)
+    n = 512
+    starts = (("zero", False, None), ("network", False, u), ("previous", True, None),
+              ("learned", True, u))
+    cycles = {k: [] for k, _, _ in starts}
+    for seed in (9101, 9102):
+        sim = Lrn._sim(n, seed)
+        for _ in range(8):
+            sim.step()
+        e = sim.stepper.engine
+        p_prev = e.read_pressure()
+        sim.step()
+        div = e.read_divergence()
+        label = sim.f.label.copy()
+        scale = sim.g.h * sim.g.h / sim.dt
+        solver = HipPressureSolver(n, n, "mg-dc")
+        for name, prev, warm in starts:
+            solver.engine.set_warm_start(warm)
+            p = p_prev.copy() if prev else np.zeros(n * n)
+            solver.solve(None, p, div, label, scale, 1, 1.0, 0)  # labels, buffers
+            p = p_prev.copy() if prev else np.zeros(n * n)
+            cycles[name].append(solver.solve(None, p, div, label, scale, 200, 1.0, 1e-6))
+            assert O.relative_residual(n, n, p, div, label, scale) <= 1e-6, (seed, name)
+        solver.engine.set_warm_start(None)
+    print("V-cycles", cycles)
+    for a, b in (("network", "zero"), ("learned", "previous")):
+        assert all(x <= y for x, y in zip(cycles[a], cycles[b])), (a, b, cycles)
+    # the previous pressure, for reference, saves cycles over the zero start
+    assert sum(cycles["previous"]) < sum(cycles["zero"]), cycles
