@@ -666,7 +666,31 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
   double target = tol > 0 ? tol * divrms : -1;
   launch_f64_to_f32(e->s, e->p, L0.x, N);
   int used = iterations;
-  if (solver == FH_SOLVER_RBSOR) {
+  if (solver == FH_SOLVER_RBSOR && rb_sor_lds_fits(e->nx, e->ny)) {
+    // the whole grid in one workgroup's LDS: the iterations between two
+    // convergence checks (every 12th iteration) run as one launch
+    launch_rhs32(e->s, e->div, L0.b, N, 1.0);
+    for (int k = 0; k < iterations;) {
+      const int chunk = target >= 0 ? std::min(12, iterations - k) : iterations - k;
+      {
+        Trace t(e, FH_K_RB_SWEEP_FINE);
+        launch_rb_sor_lds(e->s, L0.x, L0.b, L0.lbl, L0.nx, L0.ny, chunk, (float)omega,
+                          (float)scale);
+      }
+      LAUNCHED("rb_sor_lds");
+      k += chunk;
+      if (target >= 0 && chunk == 12) {
+        launch_norms(e->s, L0.x, nullptr, e->div, e->lbl, e->nx, e->ny, scale, e->partials,
+                     e->sums);
+        st = read_sums(e);
+        if (st != FH_OK) return st;
+        if (e->h_sums[1] > 0 && std::sqrt(e->h_sums[0] / e->h_sums[1]) <= target) {
+          used = k;
+          break;
+        }
+      }
+    }
+  } else if (solver == FH_SOLVER_RBSOR) {
     launch_rhs32(e->s, e->div, L0.b, N, 1.0);
     for (int k = 0; k < iterations; k++) {
       rb(e, 0, 0, (float)omega, (float)scale, 0);

@@ -81,6 +81,14 @@ constexpr int kFoldAbove = 8192, kFoldBlocks = 128;
 // on a zero correction).
 void launch_rb_sweep(hipStream_t s, float* x, const float* b, const uint8_t* lbl, int nx, int ny,
                      int color, float omega, float scale, int first);
+// `iters` iterations (red then black, as launch_rb_sweep with first = 0) in
+// one workgroup holding the whole grid in LDS; the same bits.  Only where
+// rb_sor_lds_fits (9 B per cell within the device's workgroup LDS).
+bool rb_sor_lds_fits(int nx, int ny);
+void launch_rb_sor_lds(hipStream_t s, float* x, const float* b, const uint8_t* lbl, int nx, int ny,
+                       int iters, float omega, float scale);
+// The current device's workgroup LDS limit in bytes (cached per device).
+int device_max_lds();
 // Coarse b = sum of child residuals, coarse x = 0.  pressureMultigrid.ts:117-178.
 void launch_residual_restrict(hipStream_t s, const float* x, const float* b, const uint8_t* lbl,
                               int nx, int ny, float* bc, float* xc, int cnx, int cny);

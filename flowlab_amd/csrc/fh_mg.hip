@@ -5,15 +5,30 @@
 // Every level solves  count*x - sum(x over non-solid neighbours) = b  on
 // fluid cells, with Air neighbours contributing x = 0 and off-grid/Solid
 // neighbours absent (pressureMultigrid.ts:22-42).
+#include <atomic>
+#include <type_traits>
+
 #include "fh_batchdesc.hpp"
 #include "fh_device.hpp"
 #include "fh_launch.hpp"
+#include "fh_mgtile.hpp"
 
 namespace fh {
 
 constexpr int BX = 64, BY = 4;
 
 static inline dim3 grid2(int w, int h) { return dim3((w + BX - 1) / BX, (h + BY - 1) / BY); }
+
+// One SOR update of a fluid cell with cnt > 0 open neighbours summing to
+// `sum` (pressureRedBlack.ts:52-66), shared by both red-black kernels.  The
+// fused multiply-adds are written out: left to contraction, the compiler
+// fused (1 - omega) old + omega pgs one way in one kernel and the other way
+// in the other, and the two kernels' bits differed.
+__device__ __forceinline__ float rb_value(float sum, int cnt, float bk, float old, float omega,
+                                          float scale) {
+  const float pgs = fmaf(-scale, bk, sum) / (float)cnt;
+  return fmaf(omega, pgs, (1.f - omega) * old);
+}
 
 // pressureRedBlack.ts:32-72.  Thread t of row j owns the cell pair
 // (2t, 2t+1) and updates the one of this colour, so every lane is busy and a
@@ -52,9 +67,76 @@ __global__ __launch_bounds__(256) void k_rb_sweep(float* __restrict__ x,
     if (!first) sum += x[k - nx];
   }
   if (cnt == 0) return;
-  float pgs = (sum - scale * b[k]) / (float)cnt;
-  float old = first ? 0.f : x[k];
-  x[k] = (1.f - omega) * old + omega * pgs;
+  x[k] = rb_value(sum, cnt, b[k], first ? 0.f : x[k], omega, scale);
+}
+
+// The whole red-black SOR solve of a grid small enough for one workgroup's
+// LDS (x and b FP32 and the cell info byte, 9 B per cell: 128 x 128 fits the
+// 160 KB): `iters` iterations of both colours in one launch, a barrier
+// between half-sweeps, instead of two dependent launches per iteration.
+// Each thread finds its cells of each colour once (LDS index and info byte
+// packed in one register, at most kSorItems per colour); a half-sweep then
+// reads every neighbour of its cells before writing any.  Each cell's update is
+// k_rb_sweep's: rb_value, the same neighbours (the info byte's open bits are
+// its solid_or_outside tests) summed in the same order, a closed neighbour
+// adding +0 (which leaves the sum unchanged: the sum starts at +0 and so is
+// never -0).  A colour reads only the other colour, so the bits are those of
+// the launch-per-half-sweep solve.
+constexpr int kSorThreads = 1024, kSorItems = 8;
+
+__global__ __launch_bounds__(kSorThreads) void k_rb_sor_lds(float* __restrict__ x,
+                                                            const float* __restrict__ b,
+                                                            const uint8_t* __restrict__ lbl,
+                                                            int nx, int ny, int iters,
+                                                            float omega, float scale) {
+  extern __shared__ unsigned char smem[];
+  const int n = nx * ny, tid = threadIdx.x;
+  float* sx = reinterpret_cast<float*>(smem);
+  float* sb = sx + n;
+  uint8_t* si = reinterpret_cast<uint8_t*>(sb + n);
+  for (int k = tid; k < n; k += kSorThreads) {
+    const int j = k / nx, i = k - j * nx;
+    sx[k] = x[k];
+    sb[k] = b[k];
+    si[k] = cell_info(lbl, nx, ny, i, j);
+  }
+  __syncthreads();
+  const int P = (nx + 1) / 2, items = P * ny;
+  uint32_t pk[2][kSorItems];  // LDS index | info << 24; 0: no cell to update
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+#pragma unroll
+    for (int r = 0; r < kSorItems; r++) {
+      const int q = tid + r * kSorThreads;
+      const int j = q / P, t = q - j * P;
+      const int i = 2 * t + ((j + c) & 1);
+      const int k = i + j * nx;
+      const int f = (q < items && i < nx) ? si[k] : 0;
+      pk[c][r] = ((f & 0x80) && (f & 7)) ? (uint32_t)k | ((uint32_t)f << 24) : 0u;
+    }
+  auto half = [&](auto cc) {
+    constexpr int c = decltype(cc)::value;
+    float nv[kSorItems];
+#pragma unroll
+    for (int r = 0; r < kSorItems; r++) {
+      const int k = (int)(pk[c][r] & 0xffffffu), f = (int)(pk[c][r] >> 24);
+      const float ve = (f & 0x08) ? sx[k + 1] : 0.f;
+      const float vw = (f & 0x10) ? sx[k - 1] : 0.f;
+      const float vn = (f & 0x20) ? sx[k + nx] : 0.f;
+      const float vs = (f & 0x40) ? sx[k - nx] : 0.f;
+      const float sum = (((0.f + ve) + vw) + vn) + vs;
+      nv[r] = f ? rb_value(sum, f & 7, sb[k], sx[k], omega, scale) : 0.f;
+    }
+#pragma unroll
+    for (int r = 0; r < kSorItems; r++)
+      if (pk[c][r]) sx[pk[c][r] & 0xffffffu] = nv[r];
+    __syncthreads();
+  };
+  for (int it = 0; it < iters; it++) {
+    half(std::integral_constant<int, 0>{});
+    half(std::integral_constant<int, 1>{});
+  }
+  for (int k = tid; k < n; k += kSorThreads) x[k] = sx[k];
 }
 
 // pressureMultigrid.ts:117-151 at one cell; 0 off the fluid.
@@ -677,6 +759,35 @@ void launch_finalize_sum4(hipStream_t s, const double* partials, int nblocks, do
 __global__ void k_set_int(int* p, int v) { *p = v; }
 
 void launch_set_int(hipStream_t s, int* p, int v) { k_set_int<<<1, 1, 0, s>>>(p, v); }
+
+int device_max_lds() {
+  // the workgroup LDS limit of the current device, read once per device;
+  // engines on other host threads may race here, so the cache is atomic
+  // (they can only store the same value)
+  constexpr int kMaxDevices = 64;
+  static std::atomic<int> limit[kMaxDevices];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = -1;
+  int v = dev >= 0 ? limit[dev].load(std::memory_order_relaxed) : 0;
+  if (v <= 0) {
+    if (dev < 0 || hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                       hipSuccess)
+      v = 65536;
+    if (dev >= 0) limit[dev].store(v, std::memory_order_relaxed);
+  }
+  return v;
+}
+
+bool rb_sor_lds_fits(int nx, int ny) {
+  return (size_t)nx * ny * 9 <= (size_t)device_max_lds() &&
+         (long)((nx + 1) / 2) * ny <= (long)kSorThreads * kSorItems;
+}
+
+void launch_rb_sor_lds(hipStream_t s, float* x, const float* b, const uint8_t* lbl, int nx, int ny,
+                       int iters, float omega, float scale) {
+  k_rb_sor_lds<<<1, kSorThreads, (size_t)nx * ny * 9, s>>>(x, b, lbl, nx, ny, iters, omega,
+                                                           scale);
+}
 
 void launch_rb_sweep(hipStream_t s, float* x, const float* b, const uint8_t* lbl, int nx, int ny,
                      int color, float omega, float scale, int first) {

@@ -12,6 +12,7 @@
 #include <atomic>
 
 #include "fh_device.hpp"
+#include "fh_launch.hpp"
 #include "fh_unet.hpp"
 
 namespace fh {
@@ -207,27 +208,11 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(const bf16_t* __restrict__ 
   }
 }
 
-constexpr int kMaxLdsDevices = 64;
-
 void launch_conv3x3(hipStream_t s, const bf16_t* X, int cin, const bf16_t* Wt, int kpad,
                     const float* bias, int cout, int N, int H, int W, int relu, bf16_t* Y,
                     float* Yf) {
   dim3 g((W + 63) / 64, (H + 3) / 4, N);
-  // the workgroup LDS limit of the current device, read once per device;
-  // engines on other host threads may race here, so the cache is atomic
-  // (they can only store the same value)
-  static std::atomic<int> lds_limit[kMaxLdsDevices];
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxLdsDevices) dev = -1;
-  int max_lds = dev >= 0 ? lds_limit[dev].load(std::memory_order_relaxed) : 0;
-  if (max_lds <= 0) {
-    int v = 0;
-    if (dev < 0 ||
-        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-      v = 65536;
-    max_lds = v;
-    if (dev >= 0) lds_limit[dev].store(v, std::memory_order_relaxed);
-  }
+  const int max_lds = device_max_lds();
   const size_t lds = (size_t)kConvRows * kConvCols * cin * sizeof(bf16_t);
   if (cin % 8 == 0 && lds <= (size_t)max_lds && (((uintptr_t)X) & 15) == 0) {
     switch (cout) {

@@ -160,7 +160,9 @@ int fh_synchronize(fh_engine* e);
  * fh_trace_stats returns the launch count and the summed device ms. */
 enum {
   FH_K_NONE = 0,
-  FH_K_RB_SWEEP_FINE = 1,     /* red-black half-sweep on the finest level */
+  FH_K_RB_SWEEP_FINE = 1,     /* red-black half-sweep on the finest level (on a grid that
+                                 fits one workgroup's LDS, one launch of several
+                                 iterations) */
   FH_K_ADVECT_FACES = 2,      /* semi-Lagrangian / MacCormack forward, faces */
   FH_K_MACCORMACK_FACES = 3,  /* MacCormack reverse+correct, faces */
   FH_K_ADVECT_DYE = 4,
