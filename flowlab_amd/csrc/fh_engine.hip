@@ -804,7 +804,8 @@ int dye_phase(fh_engine* e, int scheme, double dt, const double keep[3]) {
   LAUNCHED("advect_dye");
   swap_ptr(e->dye, e->dye2);
   if (e->has_patch) {
-    launch_dye_patch(e->s, e->dye, e->patch, e->dnx, e->dny, e->pi0, e->pj0, e->pnx, e->pny);
+    launch_dye_patch(e->s, e->dye, e->patch, e->dnx, e->dny, e->pi0, e->pj0, e->pnx, e->pny,
+                     e->patch_tiles, e->patch_ntiles);
     LAUNCHED("dye_patch");
   }
   return FH_OK;
@@ -929,6 +930,7 @@ void fh_destroy(fh_engine* e) {
   if (e->sten_f) (void)hipFree(e->sten_f);
   if (e->sten_d) (void)hipFree(e->sten_d);
   if (e->cells) (void)hipFree(e->cells);
+  if (e->patch_tiles) (void)hipFree(e->patch_tiles);
   if (e->counter) (void)hipFree(e->counter);
   if (e->done) (void)hipFree(e->done);
   if (e->psave) (void)hipFree(e->psave);
@@ -993,7 +995,10 @@ int fh_set_labels(fh_engine* e, const uint8_t* label) {
 int fh_set_dye_patch(fh_engine* e, int i0, int j0, int pnx, int pny, const float* data) {
   GUARD("HipStepper.setDyePatch()");
   if (e->patch) HIPCHK(hipFree(e->patch));
+  if (e->patch_tiles) HIPCHK(hipFree(e->patch_tiles));
   e->patch = nullptr;
+  e->patch_tiles = nullptr;
+  e->patch_ntiles = 0;
   e->has_patch = 0;
   if (!data) return FH_OK;
   if (pnx <= 0 || pny <= 0) {
@@ -1003,6 +1008,14 @@ int fh_set_dye_patch(fh_engine* e, int i0, int j0, int pnx, int pny, const float
   size_t n = (size_t)4 * pnx * pny;
   HIPCHK(hipMalloc((void**)&e->patch, n * sizeof(float)));
   HIPCHK(hipMemcpy(e->patch, data, n * sizeof(float), hipMemcpyHostToDevice));
+  std::vector<int> tiles;
+  dye_patch_tiles(data, pnx, pny, tiles);
+  if (!tiles.empty()) {
+    HIPCHK(hipMalloc((void**)&e->patch_tiles, tiles.size() * sizeof(int)));
+    HIPCHK(hipMemcpy(e->patch_tiles, tiles.data(), tiles.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+  }
+  e->patch_ntiles = (int)tiles.size();
   e->pi0 = i0;
   e->pj0 = j0;
   e->pnx = pnx;

@@ -86,6 +86,8 @@ struct fh_engine {
   // patch / forces
   float* patch = nullptr;
   int pi0 = 0, pj0 = 0, pnx = 0, pny = 0, has_patch = 0;
+  int* patch_tiles = nullptr;  // tiles of the patch with non-zero coverage
+  int patch_ntiles = 0;
   double buoyancy = 0, confinement = 0;
   fh::StirBox stir{};
   // state

@@ -60,8 +60,13 @@ void launch_outflow(hipStream_t s, float* u, float* v, const uint8_t* lbl, int n
 // launch_subtract_gradient then launch_outflow, in one pass where nx, ny >= 3.
 void launch_subtract_outflow(hipStream_t s, float* u, float* v, const double* p,
                              const uint8_t* lbl, int nx, int ny, double grad_scale);
+// The patch's BX x BY tiles (64 x 4, row-major over the patch) with any
+// non-zero coverage, from the host copy of the patch data (4 planes).
+void dye_patch_tiles(const float* data, int pnx, int pny, std::vector<int>& tiles);
+// Applies the patch on those tiles only (cells of zero coverage are left
+// unchanged either way).
 void launch_dye_patch(hipStream_t s, float* dye, const float* patch, int dnx, int dny, int i0,
-                      int j0, int pnx, int pny);
+                      int j0, int pnx, int pny, const int* tiles, int ntiles);
 void launch_commit_labels(hipStream_t s, double* p, float* u, float* v, float* dye,
                           const uint8_t* lbl, int nx, int ny, int dye_plane);
 // max |u|, |v| into *out (simulation.ts:451-459); u and v are read as float4

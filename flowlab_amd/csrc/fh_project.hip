@@ -3,6 +3,9 @@
 //
 // All of these are single-pass streaming stencils: one thread per face or
 // cell, x fastest, so each wavefront touches contiguous rows.
+#include <algorithm>
+#include <vector>
+
 #include "fh_batchdesc.hpp"
 #include "fh_stencil.hpp"
 
@@ -235,11 +238,17 @@ __global__ __launch_bounds__(256) void k_outflow(float* __restrict__ u, float* _
 }
 
 // dye.ts:70-87: dye += cov (val - dye) inside the patch, skipped at cov 0.
+// One workgroup per BX x BY patch tile with any non-zero coverage (the list
+// the host built when the patch was set), so a patch that spans the domain
+// around a few small sources costs those sources' tiles.
 __global__ __launch_bounds__(256) void k_dye_patch(float* __restrict__ dye,
                                                    const float* __restrict__ pd, int dnx, int dny,
-                                                   int i0, int j0, int pnx, int pny) {
-  int i = blockIdx.x * BX + threadIdx.x;
-  int j = blockIdx.y * BY + threadIdx.y;
+                                                   int i0, int j0, int pnx, int pny,
+                                                   const int* __restrict__ tiles) {
+  const int gx = (pnx + BX - 1) / BX, t = tiles[blockIdx.x];
+  const int ty = t / gx, tx = t - ty * gx;
+  int i = tx * BX + threadIdx.x;
+  int j = ty * BY + threadIdx.y;
   if (i >= pnx || j >= pny) return;
   int di = i0 + i, dj = j0 + j;
   if (di < 0 || dj < 0 || di >= dnx || dj >= dny) return;
@@ -365,9 +374,27 @@ void launch_outflow(hipStream_t s, float* u, float* v, const uint8_t* lbl, int n
   k_outflow<<<(n + 255) / 256, 256, 0, s>>>(u, v, lbl, nx, ny);
 }
 
+void dye_patch_tiles(const float* data, int pnx, int pny, std::vector<int>& tiles) {
+  const int gx = (pnx + BX - 1) / BX, gy = (pny + BY - 1) / BY;
+  const float* cov = data + (size_t)3 * pnx * pny;
+  tiles.clear();
+  for (int ty = 0; ty < gy; ty++)
+    for (int tx = 0; tx < gx; tx++) {
+      bool any = false;
+      for (int j = ty * BY; j < std::min(pny, (ty + 1) * BY) && !any; j++)
+        for (int i = tx * BX; i < std::min(pnx, (tx + 1) * BX); i++)
+          if (cov[i + (size_t)j * pnx] != 0.f) {
+            any = true;
+            break;
+          }
+      if (any) tiles.push_back(tx + ty * gx);
+    }
+}
+
 void launch_dye_patch(hipStream_t s, float* dye, const float* patch, int dnx, int dny, int i0,
-                      int j0, int pnx, int pny) {
-  k_dye_patch<<<grid2(pnx, pny), dim3(BX, BY), 0, s>>>(dye, patch, dnx, dny, i0, j0, pnx, pny);
+                      int j0, int pnx, int pny, const int* tiles, int ntiles) {
+  if (ntiles > 0)
+    k_dye_patch<<<ntiles, dim3(BX, BY), 0, s>>>(dye, patch, dnx, dny, i0, j0, pnx, pny, tiles);
 }
 
 void launch_commit_labels(hipStream_t s, double* p, float* u, float* v, float* dye,
