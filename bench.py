@@ -162,9 +162,10 @@ def pmc_summary(args):
     """HBM traffic per launch from a FETCH_SIZE pass and a WRITE_SIZE pass
     (KiB per dispatch) of `bench.py --copy-mb M ...`.  Each counter is scaled
     by what it reads for the M MiB device-to-device copies (M MiB moved each
-    way); each kernel's row is its dispatches at the largest grid (the fine
-    level for the multigrid kernels).  Writes the JSON the roofline's
-    `traffic` reads and prints the table as markdown."""
+    way); each kernel's row is its dispatches at its largest grid (the fine
+    level for the multigrid kernels), of the instantiation dispatched most.
+    Writes the JSON the roofline's `traffic` reads and prints the table as
+    markdown."""
     import csv
     import re
     from collections import defaultdict
@@ -192,10 +193,17 @@ def pmc_summary(args):
     def short(name):
         return re.sub(r"\(.*", "", name.replace("void ", "").replace("fh::", ""))
 
-    pick = {}
+    # per kernel: each instantiation's finest-level dispatches (its largest
+    # grid), then the instantiation the run used most (the tuned tile shape;
+    # the tuner runs every other shape only while timing them)
+    finest = {}
     for (kn, g) in F:
+        if kn not in finest or finest[kn] < g:
+            finest[kn] = g
+    pick = {}
+    for kn, g in finest.items():
         key = PMC_KERNELS.get(re.sub(r"<.*", "", short(kn)))
-        if key and (key not in pick or pick[key][1] < g):
+        if key and (key not in pick or len(F[pick[key]]) < len(F[(kn, g)])):
             pick[key] = (kn, g)
     n = args.n
     cells = n * n
