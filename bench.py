@@ -48,18 +48,19 @@ WORKLOAD_STEPS = 200
 #   maccormack_dye: u, v 8 + 3 channels of q^n and of the forward pass 24 +
 #     3 out 12 + label 1 + recorded stencils 1; advect_dye: u, v 8 + 3 in 12 +
 #     3 out 12 + label 1 + recorded stencils 1.
-#   dc_update: a cycle's correction pass (the launches the trace brackets)
-#     reads e 4 + p 8 + div 8 + label 1 and writes p 8 + r32 4 (33); the
-#     solve's initial defect pass has no e and writes no p (21).  Under the
-#     flexible-CG solver both also write the FP64 residual (+8).  The counter
-#     table counts every dispatch, one initial pass and one cycle per solve
-#     under flexible CG, so its algorithmic figure is their mean.
+#   dc_update: a cycle's correction pass reads e 4 + p 8 + div 8 + label 1
+#     and writes p 8 + r32 4 (33); under the flexible-CG solver it also
+#     writes the FP64 residual (+8).  (The solve's initial defect pass is the
+#     divergence kernel below.)
 #   smooth_restrict_fine: b 4 + label 1 + x 4 written + coarse b 1.
 #   prolong_smooth_fine: x 4 + b 4 + label 1 + coarse e 1 + x 4 written.
 #   fcg_dots: z 4 + previous direction 4 + r 8 + info 1.
 #   fcg_update: z 4 + previous direction 4 + p 8 + r 8 + info 1 read;
 #     direction 4 + p 8 + r 8 + r32 4 written.
-#   divergence: u, v 8 read + FP64 div 8 written.
+#   divergence: under the FP64 solvers formed by the solve's initial defect
+#     pass (the traced launch): u, v 8 + p 8 + label 1 read, div 8 + r32 4
+#     written, +8 for the FP64 residual under flexible CG; otherwise its own
+#     pass, u, v 8 read + div 8 written (16).
 #   forces (buoyancy alone): v 4 read + 4 written + heat 4 + label 1.
 #   project (gradient subtraction + outflow): u, v 8 read + 8 written +
 #     FP64 p 8 + label 1.
@@ -85,16 +86,11 @@ def algo_bytes(kernel, solver):
     b = ALGO_BYTES_PER_CELL.get(kernel)
     if kernel == "dc_update" and solver == "mg-fcg":
         b += 8.0
+    if kernel == "divergence" and solver in ("mg-dc", "mg-fcg"):
+        b = 29.0 + (8.0 if solver == "mg-fcg" else 0.0)
     return b
 
 
-def algo_bytes_all_dispatches(kernel, solver):
-    """Algorithmic bytes per cell of the mean dispatch of `kernel` in a
-    flexible-CG solve (dc_update: one initial pass, one correction pass)."""
-    if kernel == "dc_update":
-        r64 = 8.0 if solver == "mg-fcg" else 0.0
-        return (21.0 + r64 + ALGO_BYTES_PER_CELL["dc_update"] + r64) / 2.0
-    return algo_bytes(kernel, solver)
 
 
 def parse():
@@ -153,7 +149,7 @@ PMC_KERNELS = {
     "k_advect_dye_lds": "advect_dye", "k_advect_faces_lds": "advect_faces",
     "k_dc_update": "dc_update", "k_fcg_update": "fcg_update", "k_fcg_dots": "fcg_dots",
     "k_smooth_restrict": "smooth_restrict_fine", "k_prolong_smooth": "prolong_smooth_fine",
-    "k_subtract_outflow": "project", "k_buoyancy": "forces", "k_divergence": "divergence",
+    "k_subtract_outflow": "project", "k_buoyancy": "forces", "k_div_defect": "divergence",
 }
 COPY_KERNEL = "__amd_rocclr_copyBuffer"
 
@@ -214,7 +210,7 @@ def pmc_summary(args):
         fv, wv = F[(kn, g)], W.get((kn, g), [0.0])
         fb = sum(fv) / len(fv) * 1024 * sf
         wb = sum(wv) / len(wv) * 1024 * sw
-        a = algo_bytes_all_dispatches(key, args.solver) * cells
+        a = algo_bytes(key, args.solver) * cells
         out["kernels"][key] = {"rocprof_name": short(kn), "dispatches": len(fv),
                                "fetch_bytes": fb, "write_bytes": wb, "traffic_bytes": fb + wb,
                                "algorithmic_bytes": a, "ratio": (fb + wb) / a}

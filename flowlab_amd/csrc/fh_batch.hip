@@ -72,7 +72,7 @@ struct fh_batch {
   BStep* h_step = nullptr;  // pinned
   unsigned char* d_lists = nullptr;
   size_t lists_bytes = 0;
-  BatchGrid g_adv[4] = {}, g_proj[5] = {}, g_red = {}, g_speed = {};
+  BatchGrid g_adv[4] = {}, g_proj[kNumBatchProject] = {}, g_red = {}, g_speed = {};
   BatchGrid g_lv[kBatchMaxLevels][kNumTileShapes] = {};
   // largest face speed of each scene's velocity: partials, then mapped host floats
   float* d_fpart = nullptr;
@@ -219,7 +219,7 @@ int prepare(fh_batch* b) {
     for (int s = 0; s < ns; s++) grids[s] = batch_grid_advect(k, b->desc[s].nx, b->desc[s].ny);
     lb.add(&b->g_adv[k], b->order, grids);
   }
-  for (int k = 0; k < 5; k++) {
+  for (int k = 0; k < kNumBatchProject; k++) {
     for (int s = 0; s < ns; s++) {
       const fh_engine* e = b->e[s];
       const bool conf = (float)e->confinement != 0.f;
@@ -294,7 +294,8 @@ int vcycle(fh_batch* b) {
 // and final residuals into b->iters / b->resid as fh_step reports them.
 int solve(fh_batch* b, double tol, int max_cycles) {
   const int ns = (int)b->e.size(), q = b->q;
-  launch_batch_dc_update(b->s, b->d_desc, b->g_red, b->d_step, q, 1);
+  // the initial defect pass, forming each scene's divergence (as fh_step does)
+  launch_batch_dc_update(b->s, b->d_desc, b->g_red, b->d_step, q, 2);
   launch_batch_finalize(b->s, b->d_desc, ns, b->d_step, 0, 0, 0.0);
   BLAUNCHED("batched initial defect");
   BCHK(hipEventRecord(b->ev_init, b->s));
@@ -552,7 +553,6 @@ int fh_batch_step(fh_batch* b, int scheme, double cfl, double dt_max, double rho
     launch_batch_forces(b->s, D, b->g_proj[kBForces], S, q);
   }
   if (b->g_proj[kBBuoyancy].blocks) launch_batch_buoyancy(b->s, D, b->g_proj[kBBuoyancy], S, q);
-  launch_batch_divergence(b->s, D, b->g_proj[kBDivergence], S, q);
   BLAUNCHED("batched advection and forces");
   if ((st = solve(b, tol, max_cycles)) != FH_OK) return st;
   launch_batch_project(b->s, D, b->g_proj[kBProject], S, q);

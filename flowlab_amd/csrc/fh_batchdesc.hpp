@@ -93,7 +93,7 @@ __device__ __forceinline__ bool batch_block(const BatchGrid& g, int* s, int* bx,
 // The single-grid launchers' grids, so a scene's blocks are the same either way.
 enum BatchAdvect { kBAdvectFaces, kBMacCormackFaces, kBAdvectDye, kBMacCormackDye };
 dim3 batch_grid_advect(int which, int nx, int ny);
-enum BatchProject { kBVorticity, kBForces, kBBuoyancy, kBDivergence, kBProject };
+enum BatchProject { kBVorticity, kBForces, kBBuoyancy, kBProject, kNumBatchProject };
 dim3 batch_grid_project(int which, int nx, int ny);
 // The defect-correction / CG passes' grid.
 dim3 batch_grid_reduce(int nx, int ny);
@@ -117,8 +117,6 @@ void launch_batch_forces(hipStream_t s, const SceneDesc* D, const BatchGrid& g, 
                          int q);
 void launch_batch_buoyancy(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
                            const BStep* st, int q);
-void launch_batch_divergence(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
-                             const BStep* st, int q);
 void launch_batch_project(hipStream_t s, const SceneDesc* D, const BatchGrid& g, const BStep* st,
                           int q);
 // max |u|, |v| of velocity [q ^ 1] per scene into host_out[s] (mapped
@@ -126,8 +124,10 @@ void launch_batch_project(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
 // partials holds kBatchSpeedBlocks floats per scene.
 void launch_batch_max_speed(hipStream_t s, const SceneDesc* D, const BatchGrid& g, int q,
                             int nscenes, float* partials, float* host_out);
-// The defect pass: init (the pass before the first cycle: p [q] unchanged,
-// no correction) or the first cycle's update (p [q ^ 1] = p [q] + e).
+// The defect pass: init 1 (the pass before the first cycle: p [q]
+// unchanged, no correction), init 2 (the same, forming the divergence of
+// velocity [q ^ 1] in the pass and storing it) or init 0, the first cycle's
+// update (p [q ^ 1] = p [q] + e).
 void launch_batch_dc_update(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
                             const BStep* st, int q, int init);
 // One block per scene: the norm of the last defect / CG pass into sums and

@@ -68,6 +68,12 @@ __device__ __forceinline__ Axis axis_t(int base, float off, int count) {
   }
 }
 
+// divergence.ts:17-25 of one cell in FP64 from its four FP32 faces (west and
+// east u, south and north v), shared by every kernel that forms it.
+__device__ __forceinline__ double div_value(float uw, float ue, float vs, float vn, double inv_h) {
+  return inv_h * (((double)ue - (double)uw) + (double)vn - (double)vs);
+}
+
 // grid.ts:180-191
 __device__ __forceinline__ float bilerp(float v00, float v10, float v01, float v11, float fx,
                                         float fy) {

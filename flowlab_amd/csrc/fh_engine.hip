@@ -470,20 +470,22 @@ double rel_resid(const double* s, double scale, bool scaled_r) {
 }
 
 // PressureSolver.solve on the resident p (FP64), div (FP64), label.
+// div_fused: the divergence has not been formed; the initial defect pass of
+// the FP64 solvers forms it from the resident velocity.
 int solve_impl(fh_engine* e, int solver, double scale, int iterations, double omega,
-               double tol, int* out_iters);
+               double tol, int* out_iters, bool div_fused);
 
 // Every solve is bracketed by two events: fh_last_solve_ms reports its
 // device time (label refresh, warm start and all cycles included).
 int solve(fh_engine* e, int solver, double scale, int iterations, double omega, double tol,
-          int* out_iters) {
+          int* out_iters, bool div_fused = false) {
   e->solve_ev_valid = false;
   if (!e->solve_ev[0]) {
     HIPCHK(hipEventCreate(&e->solve_ev[0]));
     HIPCHK(hipEventCreate(&e->solve_ev[1]));
   }
   HIPCHK(hipEventRecord(e->solve_ev[0], e->s));
-  int st = solve_impl(e, solver, scale, iterations, omega, tol, out_iters);
+  int st = solve_impl(e, solver, scale, iterations, omega, tol, out_iters, div_fused);
   if (st != FH_OK) return st;
   HIPCHK(hipEventRecord(e->solve_ev[1], e->s));
   e->solve_ev_valid = true;
@@ -491,7 +493,7 @@ int solve(fh_engine* e, int solver, double scale, int iterations, double omega, 
 }
 
 int solve_impl(fh_engine* e, int solver, double scale, int iterations, double omega,
-               double tol, int* out_iters) {
+               double tol, int* out_iters, bool div_fused) {
   int N = e->nx * e->ny;
   if (e->labels_dirty) {
     int st = refresh_levels(e);
@@ -545,8 +547,14 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
       }
       e->hist.push_back(rel_resid(e->h_sums, scale, true));
     } else {
-      launch_dc_correct(e->s, e->p, nullptr, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
-                        e->partials, nullptr, r64);
+      if (div_fused) {
+        Trace t(e, FH_K_DIVERGENCE);
+        launch_div_defect(e->s, e->u, e->v, 1.0 / e->h, e->div, e->p, L0.info, L0.b, r64, e->nx,
+                          e->ny, scale, e->partials);
+      } else {
+        launch_dc_correct(e->s, e->p, nullptr, nullptr, e->div, L0.info, L0.b, e->nx, e->ny,
+                          scale, e->partials, nullptr, r64);
+      }
       launch_dc_norm_init(e->s, e->partials, e->nx, e->ny, e->sums, e->d_res + 16, e->done,
                           scale);
       LAUNCHED("dc_update");
@@ -1081,13 +1089,20 @@ int fh_step(fh_engine* e, int scheme, double dt, double scale, double grad_scale
   if ((st = mark(e, 1))) return st;
   if ((st = forces(e, dt))) return st;
   if ((st = mark(e, 2))) return st;
-  {
-    Trace t(e, FH_K_DIVERGENCE);
-    launch_divergence(e->s, e->u, e->v, e->div, e->nx, e->ny, 1.0 / e->h);
+  // The FP64 solvers' initial defect pass forms the divergence itself
+  // (unless a capture or the learned start needs the defect on the host
+  // first); the other solvers read it from its own pass.
+  const bool fuse = (solver == FH_SOLVER_MG_DC || solver == FH_SOLVER_MG_FCG) && !e->capture &&
+                    !e->warm;
+  if (!fuse) {
+    {
+      Trace t(e, FH_K_DIVERGENCE);
+      launch_divergence(e->s, e->u, e->v, e->div, e->nx, e->ny, 1.0 / e->h);
+    }
+    LAUNCHED("divergence");
   }
-  LAUNCHED("divergence");
   if ((st = mark(e, 3))) return st;
-  if ((st = solve(e, solver, scale, iterations, omega, tol, out_iters))) return st;
+  if ((st = solve(e, solver, scale, iterations, omega, tol, out_iters, fuse))) return st;
   if ((st = mark(e, 4))) return st;
   if ((st = project(e, grad_scale))) return st;
   // The next step's CFL reads this velocity (simulation.ts:474-477); the dye

@@ -123,6 +123,12 @@ void launch_dc_update(hipStream_t s, const double* po, double* pn, const float* 
 void launch_dc_correct(hipStream_t s, const double* po, double* pn, const float* e,
                        const double* div, const uint8_t* info, float* r32, int nx, int ny,
                        double scale, double* partials, const int* done, double* r64 = nullptr);
+// launch_dc_correct's initial pass (no e, no pn, done unread) with the
+// divergence (launch_divergence's values, stored to div) formed in the same
+// pass from the faces u, v.
+void launch_div_defect(hipStream_t s, const float* u, const float* v, double inv_h, double* div,
+                       const double* po, const uint8_t* info, float* r32, double* r64, int nx,
+                       int ny, double scale, double* partials);
 void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                     double* host_sums, int* done, double scale, double tol);
 // The norm of the pass before the first cycle: runs whatever *done holds and

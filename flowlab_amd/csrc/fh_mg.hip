@@ -323,12 +323,18 @@ __device__ __forceinline__ void load_col5(Col5<R, T>& o, const T* __restrict__ a
 // count), so every load of a cell is issued at once, unconditionally, from
 // clamped in-domain indices, and closed neighbours add 0 (sum order +x, -x,
 // +y, -y, as in the reference's residual).
-template <int R>
+//
+// kDiv: the solve's initial pass (no e, no pn) forming the divergence itself
+// from the faces u, v (div_value, as k_divergence does) and storing it to
+// div_out, instead of reading a divergence a separate pass wrote.
+template <int R, bool kDiv = false>
 __device__ __forceinline__ void dc_update_body(
     const double* __restrict__ po, double* __restrict__ pn, const float* __restrict__ e,
     const double* __restrict__ div, const uint8_t* __restrict__ info, float* __restrict__ r32,
     double* __restrict__ r64, int nx, int ny, double scale, double* partials,
-    const int* __restrict__ done, int bx, int by, int gy, int bid) {
+    const int* __restrict__ done, int bx, int by, int gy, int bid,
+    const float* __restrict__ u = nullptr, const float* __restrict__ v = nullptr,
+    double inv_h = 0.0, double* __restrict__ div_out = nullptr) {
   if (done && *done) return;
   const int i = bx * BX + threadIdx.x;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -342,9 +348,12 @@ __device__ __forceinline__ void dc_update_body(
     double dr[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      const int k = i + min(j0 + r, ny - 1) * nx;
+      const int jj = min(j0 + r, ny - 1), k = i + jj * nx;
       fr[r] = info[k];
-      dr[r] = div[k];
+      if constexpr (kDiv)
+        dr[r] = div_value(u[i + jj * (nx + 1)], u[i + 1 + jj * (nx + 1)], v[k], v[k + nx], inv_h);
+      else
+        dr[r] = div[k];
     }
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -353,6 +362,7 @@ __device__ __forceinline__ void dc_update_body(
       const int k = i + j * nx;
       const int f = fr[r];
       const double d = dr[r];
+      if constexpr (kDiv) div_out[k] = d;
       double pc = P.c[r + 1], pE = P.e[r], pW = P.w[r], pN = P.c[r + 2], pS = P.c[r];
       if (e) {
         pc += (double)E.c[r + 1];
@@ -395,6 +405,21 @@ __global__ __launch_bounds__(256) void k_dc_update(
   xcd_block(&bx, &by);
   dc_update_body<R>(po, pn, e, div, info, r32, r64, nx, ny, scale, partials, done, bx, by,
                     gridDim.y, blockIdx.x + blockIdx.y * gridDim.x);
+}
+
+// The solve's initial defect pass with the divergence formed in the same
+// pass (dc_update_body<R, true>): the divergence kernel and this pass's read
+// of div become one launch.
+template <int R>
+__global__ __launch_bounds__(256) void k_div_defect(
+    const float* __restrict__ u, const float* __restrict__ v, double inv_h, double* __restrict__ div,
+    const double* __restrict__ po, const uint8_t* __restrict__ info, float* __restrict__ r32,
+    double* __restrict__ r64, int nx, int ny, double scale, double* partials) {
+  int bx, by;
+  xcd_block(&bx, &by);
+  dc_update_body<R, true>(po, nullptr, nullptr, nullptr, info, r32, r64, nx, ny, scale, partials,
+                          nullptr, bx, by, gridDim.y, blockIdx.x + blockIdx.y * gridDim.x, u, v,
+                          inv_h, div);
 }
 
 // ---- flexible conjugate gradients (FCG(1)) around the V-cycle -------------
@@ -843,6 +868,13 @@ void launch_dc_correct(hipStream_t s, const double* po, double* pn, const float*
       po, pn, e, div, info, r32, r64, nx, ny, scale, partials, done);
 }
 
+void launch_div_defect(hipStream_t s, const float* u, const float* v, double inv_h, double* div,
+                       const double* po, const uint8_t* info, float* r32, double* r64, int nx,
+                       int ny, double scale, double* partials) {
+  k_div_defect<kRedRows><<<reduce_grid_rows(nx, ny), dim3(BX, BY), 0, s>>>(
+      u, v, inv_h, div, po, info, r32, r64, nx, ny, scale, partials);
+}
+
 void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                     double* host_sums, int* done, double scale, double tol) {
   const dim3 g = reduce_grid_rows(nx, ny);
@@ -906,6 +938,12 @@ __global__ __launch_bounds__(256) void kb_dc_update(const SceneDesc* __restrict_
   int s, bx, by, gy, bid;
   if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
   const SceneDesc& d = D[s];
+  if (init == 2) {  // the initial pass with the divergence of velocity [q ^ 1]
+    dc_update_body<kRedRows, true>(d.p[q], nullptr, nullptr, nullptr, d.lv[0].info, d.lv[0].b,
+                                   d.rvec, d.nx, d.ny, st[s].scale, d.partials, nullptr, bx, by,
+                                   gy, bid, d.u[q ^ 1], d.v[q ^ 1], st[s].inv_h, d.div);
+    return;
+  }
   double* pn = init ? nullptr : d.p[q ^ 1];
   const float* e = init ? nullptr : d.lv[0].x;
   const int* done = init ? nullptr : d.done;

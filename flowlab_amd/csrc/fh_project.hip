@@ -111,8 +111,7 @@ __device__ __forceinline__ void divergence_rows(const float* __restrict__ u,
 #pragma unroll
   for (int r = 0; r < kRows; r++)
     if (j0 + r < ny)
-      div[i + (j0 + r) * nx] =
-          inv_h * (((double)ub[r] - (double)ua[r]) + (double)vv[r + 1] - (double)vv[r]);
+      div[i + (j0 + r) * nx] = div_value(ua[r], ub[r], vv[r], vv[r + 1], inv_h);
 }
 
 __global__ __launch_bounds__(256) void k_divergence(const float* __restrict__ u,
@@ -447,14 +446,6 @@ __global__ __launch_bounds__(256) void kb_buoyancy(const SceneDesc* __restrict__
   buoyancy_rows(d.v[q ^ 1], d.lbl, d.dye[q], d.nx, d.ny, 1, gain, bx, by);
 }
 
-__global__ __launch_bounds__(256) void kb_divergence(const SceneDesc* __restrict__ D, BatchGrid g,
-                                                     const BStep* __restrict__ st, int q) {
-  int s, bx, by, gy, bid;
-  if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
-  const SceneDesc& d = D[s];
-  divergence_rows(d.u[q ^ 1], d.v[q ^ 1], d.div, d.nx, d.ny, st[s].inv_h, bx, by);
-}
-
 // Gradient subtraction with the pressure the solve left in p [q ^ 1].
 __global__ __launch_bounds__(256) void kb_project(const SceneDesc* __restrict__ D, BatchGrid g,
                                                   const BStep* __restrict__ st, int q) {
@@ -512,7 +503,6 @@ dim3 batch_grid_project(int which, int nx, int ny) {
     case kBVorticity: return grid2(nx, ny);
     case kBForces: return grid2(nx + 1, ny + 1);
     case kBBuoyancy: return grid_rows(nx, ny - 1);
-    case kBDivergence: return grid_rows(nx, ny);
     default: return grid_rows(nx + 1, ny + 1);
   }
 }
@@ -530,11 +520,6 @@ void launch_batch_forces(hipStream_t s, const SceneDesc* D, const BatchGrid& g, 
 void launch_batch_buoyancy(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
                            const BStep* st, int q) {
   kb_buoyancy<<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, st, q);
-}
-
-void launch_batch_divergence(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
-                             const BStep* st, int q) {
-  kb_divergence<<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, st, q);
 }
 
 void launch_batch_project(hipStream_t s, const SceneDesc* D, const BatchGrid& g, const BStep* st,
