@@ -221,7 +221,7 @@ __device__ __forceinline__ void maccormack_faces_tile(
     const float* __restrict__ un, const float* __restrict__ vn, const float* __restrict__ uh,
     const float* __restrict__ vh, float* __restrict__ uo, float* __restrict__ vo,
     const uint8_t* __restrict__ lbl, int nx, int ny, float c, const uint8_t* __restrict__ sten,
-    int bx, int by) {
+    const float* __restrict__ heat, float gain, int bx, int by) {
   constexpr int WN = Tile<TY>::WN;
   __shared__ float lds[4][WN];
   const int x0 = bx * TX, y0 = by * TY, wx0 = x0 - AH, wy0 = y0 - AH;
@@ -241,7 +241,7 @@ __device__ __forceinline__ void maccormack_faces_tile(
               ah = window_as<F>(lds, 2, wx0, wy0), bh = window_as<F>(lds, 3, wx0, wy0);
       for (int r = 0; r < TY / BY; r++)
         maccormack_faces_t(a, b, ah, bh, uo, vo, lbl, nx, ny, c, i, j + r * BY, sten != nullptr,
-                           st[r]);
+                           st[r], heat, gain);
     };
     if (interior_tile<TY>(x0, y0, nx, ny))
       run(LFieldI{});
@@ -250,7 +250,8 @@ __device__ __forceinline__ void maccormack_faces_tile(
   } else {
     const GField a{un, nx + 1}, b{vn, nx}, ah{uh, nx + 1}, bh{vh, nx};
     for (int r = 0; r < TY / BY; r++)
-      maccormack_faces_t(a, b, ah, bh, uo, vo, lbl, nx, ny, c, i, j + r * BY);
+      maccormack_faces_t(a, b, ah, bh, uo, vo, lbl, nx, ny, c, i, j + r * BY, false, 0, heat,
+                         gain);
   }
 }
 
@@ -258,10 +259,11 @@ template <int TY>
 __global__ __launch_bounds__(256) void k_maccormack_faces_lds(
     const float* __restrict__ un, const float* __restrict__ vn, const float* __restrict__ uh,
     const float* __restrict__ vh, float* __restrict__ uo, float* __restrict__ vo,
-    const uint8_t* __restrict__ lbl, int nx, int ny, float c, const uint8_t* __restrict__ sten) {
+    const uint8_t* __restrict__ lbl, int nx, int ny, float c, const uint8_t* __restrict__ sten,
+    const float* __restrict__ heat, float gain) {
   int bx, by;
   xcd_block(&bx, &by);
-  maccormack_faces_tile<TY>(un, vn, uh, vh, uo, vo, lbl, nx, ny, c, sten, bx, by);
+  maccormack_faces_tile<TY>(un, vn, uh, vh, uo, vo, lbl, nx, ny, c, sten, heat, gain, bx, by);
 }
 
 // advect.ts:241-270, dye on the velocity grid (scale 1): windows of u, v and
@@ -385,10 +387,11 @@ void launch_advect_faces(hipStream_t s, const float* uc, const float* vc, const 
 
 void launch_maccormack_faces(hipStream_t s, const float* un, const float* vn, const float* uh,
                              const float* vh, float* uo, float* vo, const uint8_t* lbl, int nx,
-                             int ny, float c, bool lds, const uint8_t* sten) {
+                             int ny, float c, bool lds, const uint8_t* sten, const float* heat,
+                             float gain) {
   if (lds)
-    k_maccormack_faces_lds<kTyMacCormackFaces><<<grid_t<kTyMacCormackFaces>(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(un, vn, uh, vh, uo,
-                                                                           vo, lbl, nx, ny, c, sten);
+    k_maccormack_faces_lds<kTyMacCormackFaces><<<grid_t<kTyMacCormackFaces>(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(
+        un, vn, uh, vh, uo, vo, lbl, nx, ny, c, sten, heat, gain);
   else
     k_maccormack_faces<<<grid2(nx + 1, ny + 1), dim3(BX, BY), 0, s>>>(un, vn, uh, vh, uo, vo,
                                                                       lbl, nx, ny, c);
@@ -439,8 +442,12 @@ __global__ __launch_bounds__(256) void kb_maccormack_faces(const SceneDesc* __re
   int s, bx, by, gy, bid;
   if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
   const SceneDesc& d = D[s];
+  // buoyancy alone (no confinement, gain != 0) is applied at the v-face
+  // write with the dye not yet advected, as the single-grid step fuses it
+  const BStep& t = st[s];
+  const bool buoy = t.eps == 0.f && t.gain != 0.f;
   maccormack_faces_tile<TY>(d.u[q], d.v[q], d.uh, d.vh, d.u[q ^ 1], d.v[q ^ 1], d.lbl, d.nx, d.ny,
-                            st[s].c, d.sten_f, bx, by);
+                            t.c, d.sten_f, buoy ? d.dye[q] : nullptr, t.gain, bx, by);
 }
 
 // The forward MacCormack pass keeps the dye (keep 1 at its write, as

@@ -552,7 +552,9 @@ int fh_batch_step(fh_batch* b, int scheme, double cfl, double dt_max, double rho
     launch_batch_vorticity(b->s, D, b->g_proj[kBVorticity], S, q);
     launch_batch_forces(b->s, D, b->g_proj[kBForces], S, q);
   }
-  if (b->g_proj[kBBuoyancy].blocks) launch_batch_buoyancy(b->s, D, b->g_proj[kBBuoyancy], S, q);
+  // MacCormack applies buoyancy alone inside its face pass (kb_maccormack_faces)
+  if (!mac && b->g_proj[kBBuoyancy].blocks)
+    launch_batch_buoyancy(b->s, D, b->g_proj[kBBuoyancy], S, q);
   BLAUNCHED("batched advection and forces");
   if ((st = solve(b, tol, max_cycles)) != FH_OK) return st;
   launch_batch_project(b->s, D, b->g_proj[kBProject], S, q);

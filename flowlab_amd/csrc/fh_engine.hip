@@ -741,7 +741,16 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
   return FH_OK;
 }
 
-int advect_velocity(fh_engine* e, int scheme, double dt) {
+// A step whose body forces are buoyancy alone (no stir, no confinement,
+// dye scale 1) applies it inside the LDS MacCormack face pass, at each open
+// interior v face's write: the value the separate buoyancy pass would give,
+// without reading and writing v again.
+bool buoyancy_fused(const fh_engine* e, int scheme, double dt) {
+  return scheme == FH_MACCORMACK && e->advect_lds && e->ds == 1 && !e->stir.active &&
+         (float)e->confinement == 0.f && (float)(e->buoyancy * dt) != 0.f;
+}
+
+int advect_velocity(fh_engine* e, int scheme, double dt, bool with_buoyancy = false) {
   float c = (float)(dt / e->h);
   if (scheme == FH_MACCORMACK) {
     {
@@ -751,7 +760,8 @@ int advect_velocity(fh_engine* e, int scheme, double dt) {
     }
     Trace t(e, FH_K_MACCORMACK_FACES);
     launch_maccormack_faces(e->s, e->u, e->v, e->uh, e->vh, e->u2, e->v2, e->lbl, e->nx, e->ny,
-                            c, e->advect_lds, e->advect_lds ? e->sten_f : nullptr);
+                            c, e->advect_lds, e->advect_lds ? e->sten_f : nullptr,
+                            with_buoyancy ? e->dye : nullptr, (float)(e->buoyancy * dt));
   } else {
     Trace t(e, FH_K_ADVECT_FACES);
     launch_advect_faces(e->s, e->u, e->v, e->u, e->v, e->u2, e->v2, e->lbl, e->nx, e->ny, c,
@@ -764,11 +774,12 @@ int advect_velocity(fh_engine* e, int scheme, double dt) {
   return FH_OK;
 }
 
-int forces(fh_engine* e, double dt) {
+// buoyancy_done: the face pass has applied buoyancy alone (buoyancy_fused).
+int forces(fh_engine* e, double dt, bool buoyancy_done = false) {
   float eps = (float)e->confinement;
   float gain = (float)(e->buoyancy * dt);
   if (eps != 0.f) launch_vorticity(e->s, e->u, e->v, e->w, e->nx, e->ny, (float)(1.0 / e->h));
-  if (e->stir.active || gain != 0.f || eps != 0.f) {
+  if (!buoyancy_done && (e->stir.active || gain != 0.f || eps != 0.f)) {
     Trace t(e, FH_K_FORCES);
     launch_forces(e->s, e->u, e->v, e->lbl, e->dye, e->w, e->nx, e->ny, e->ds, (float)e->h,
                   (float)dt, gain, eps, e->stir);
@@ -1085,9 +1096,10 @@ int fh_step(fh_engine* e, int scheme, double dt, double scale, double grad_scale
   }
   int st;
   if ((st = mark(e, 0))) return st;
-  if ((st = advect_velocity(e, scheme, dt))) return st;
+  const bool buoy = buoyancy_fused(e, scheme, dt);
+  if ((st = advect_velocity(e, scheme, dt, buoy))) return st;
   if ((st = mark(e, 1))) return st;
-  if ((st = forces(e, dt))) return st;
+  if ((st = forces(e, dt, buoy))) return st;
   if ((st = mark(e, 2))) return st;
   // The FP64 solvers' initial defect pass forms the divergence itself
   // (unless a capture or the learned start needs the defect on the host

@@ -31,10 +31,14 @@ void launch_advect_faces(hipStream_t s, const float* uc, const float* vc, const 
                          const float* vs, float* uo, float* vo, const uint8_t* lbl, int nx, int ny,
                          float c, bool lds, uint8_t* sten = nullptr);
 // MacCormack reverse pass fused with the correction and limiter.
-// advect.ts:177-210.
+// advect.ts:177-210.  heat (lds only; may be null): buoyancy alone
+// (buoyancy.ts:70-88, dye scale 1) applied at each open interior v face's
+// write, v += gain * heat at the face, as launch_forces's buoyancy pass would
+// apply it after this pass.
 void launch_maccormack_faces(hipStream_t s, const float* un, const float* vn, const float* uh,
                              const float* vh, float* uo, float* vo, const uint8_t* lbl, int nx,
-                             int ny, float c, bool lds, const uint8_t* sten = nullptr);
+                             int ny, float c, bool lds, const uint8_t* sten = nullptr,
+                             const float* heat = nullptr, float gain = 0.f);
 // Three dye channels on one backtrace, scaled by keep[c] at the write.
 // advect.ts:241-270 (+ the decay of simulation.ts:569-576).
 void launch_advect_dye(hipStream_t s, const float* u, const float* v, const float* q, float* qo,

@@ -93,6 +93,18 @@ __device__ __forceinline__ void advect_faces_at(const float* __restrict__ uc,
                  lbl, nx, ny, c, i, j);
 }
 
+// buoyancy.ts:70-88 on one open interior v face (i, j): val + gain times the
+// heat (dye channel 0 on a grid ds x finer) at the face, as k_buoyancy
+// applies it; shared so the fused and the separate pass give the same bits.
+__device__ __forceinline__ float buoyancy_value(float val, const float* __restrict__ heat, int nx,
+                                                int ny, int ds, float gain, int i, int j) {
+  const int dnx = nx * ds;
+  const Axis ax = clamped_axis(i * ds, 0.5f * ds - 0.5f, dnx);
+  const Axis ay = clamped_axis(j * ds, -0.5f, ny * ds);
+  const Quad g = gather4(heat, dnx, ax, ay);
+  return val + gain * bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
+}
+
 // advect.ts:177-210 fused: the reverse pass (carrier u^n, source uHat, -dt)
 // is evaluated for this face only, then corrected = uHat + (u^n - uBar)/2 is
 // clamped to the u^n stencil at the forward backtrace.  Each face reads only
@@ -106,7 +118,9 @@ __device__ __forceinline__ void maccormack_faces_t(const NU& un, const NV& vn, c
                                                    float* __restrict__ vo,
                                                    const uint8_t* __restrict__ lbl, int nx, int ny,
                                                    float c, int i, int j, bool has_sten = false,
-                                                   uint32_t st = 0) {
+                                                   uint32_t st = 0,
+                                                   const float* __restrict__ heat = nullptr,
+                                                   float gain = 0.f) {
   if (i > nx || j > ny) return;
   Pos rx, ry, fx, fy;
   if (j < ny) {
@@ -152,7 +166,9 @@ __device__ __forceinline__ void maccormack_faces_t(const NU& un, const NV& vn, c
         ax = axis_t<NV::kClamp>(fx.i, fx.f - 0.5f, nx);
         ay = axis_t<NV::kClamp>(fy.i, fy.f, ny + 1);
       }
-      vo[k] = clamp_to_quad(corrected, gather4f(vn, ax, ay));
+      const float vc = clamp_to_quad(corrected, gather4f(vn, ax, ay));
+      // buoyancy alone (k_buoyancy's value) on this open interior face
+      vo[k] = heat ? buoyancy_value(vc, heat, nx, ny, 1, gain, i, j) : vc;
     }
   }
 }
