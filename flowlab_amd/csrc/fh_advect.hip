@@ -106,10 +106,12 @@ __device__ __forceinline__ float nan_max(float m, float x) {
 
 // Stages NF windows (all global loads first, then the LDS stores) and
 // returns this thread's nan_max over what it staged of windows 0 and 1 (the
-// velocity); elements outside a field are staged as 0.
+// velocity); elements outside a field are staged as 0.  speed (optional):
+// the same maximum with fmaxf's treatment of NaN (ignored), as the max-speed
+// reduction takes it.
 template <int NF, int WN>
 __device__ __forceinline__ float stage_windows(float (*lds)[WN], const Win (&f)[NF], int wx0,
-                                               int wy0, int tid) {
+                                               int wy0, int tid, float* speed = nullptr) {
   constexpr int kStage = (WN + 255) / 256;
   float v[NF][kStage];
 #pragma unroll
@@ -123,7 +125,7 @@ __device__ __forceinline__ float stage_windows(float (*lds)[WN], const Win (&f)[
                     ? f[q].p[i + j * f[q].w]
                     : 0.f;
   }
-  float m = 0.f;
+  float m = 0.f, sp = 0.f;
 #pragma unroll
   for (int r = 0; r < kStage; r++) {
     const int e = tid + r * 256;
@@ -132,7 +134,9 @@ __device__ __forceinline__ float stage_windows(float (*lds)[WN], const Win (&f)[
       for (int q = 0; q < NF; q++) lds[q][e] = v[q][r];
     }
     m = nan_max(nan_max(m, v[0][r]), v[1][r]);
+    if (speed) sp = fmaxf(sp, fmaxf(fabsf(v[0][r]), fabsf(v[1][r])));
   }
+  if (speed) *speed = sp;
   return m;
 }
 
@@ -144,6 +148,28 @@ __device__ __forceinline__ bool within_reach(float m, float c, int tid) {
   for (int o = 32; o > 0; o >>= 1) m = nan_max(m, __shfl_xor(m, o, 64));
   if ((tid & 63) == 0) wm[tid >> 6] = m;
   __syncthreads();
+  const float vw = nan_max(nan_max(nan_max(wm[0], wm[1]), wm[2]), wm[3]);
+  return fabsf(c) * vw <= kReach;
+}
+
+// within_reach that also stores the workgroup's largest |u|, |v| (speed,
+// each thread's from stage_windows) to partials[bid], under the same barrier.
+// The windows of all tiles of a launch cover every u and v element, so the
+// largest of these partials is the fields' max face speed.
+__device__ __forceinline__ bool within_reach_speed(float m, float sp, float c, int tid,
+                                                   float* __restrict__ partials, int bid) {
+  __shared__ float wm[4], ws[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    m = nan_max(m, __shfl_xor(m, o, 64));
+    sp = fmaxf(sp, __shfl_xor(sp, o, 64));
+  }
+  if ((tid & 63) == 0) {
+    wm[tid >> 6] = m;
+    ws[tid >> 6] = sp;
+  }
+  __syncthreads();
+  if (tid == 0) partials[bid] = fmaxf(fmaxf(ws[0], ws[1]), fmaxf(ws[2], ws[3]));
   const float vw = nan_max(nan_max(nan_max(wm[0], wm[1]), wm[2]), wm[3]);
   return fabsf(c) * vw <= kReach;
 }
@@ -272,7 +298,8 @@ template <int TY>
 __device__ __forceinline__ void advect_dye_tile(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ q,
     float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx, int ny, float c, float k0,
-    float k1, float k2, uint8_t* __restrict__ sten, int bx, int by) {
+    float k1, float k2, uint8_t* __restrict__ sten, int bx, int by,
+    float* __restrict__ speed = nullptr, int bid = 0) {
   constexpr int WN = Tile<TY>::WN;
   __shared__ float lds[5][WN];
   const int x0 = bx * TX, y0 = by * TY, wx0 = x0 - AH, wy0 = y0 - AH;
@@ -280,9 +307,12 @@ __device__ __forceinline__ void advect_dye_tile(
   const size_t plane = (size_t)nx * ny;
   const Win f[5] = {{u, nx + 1, ny}, {v, nx, ny + 1}, {q, nx, ny}, {q + plane, nx, ny},
                     {q + 2 * plane, nx, ny}};
-  const float m = stage_windows<5>(lds, f, wx0, wy0, tid);
+  float sp = 0.f;
+  const float m = stage_windows<5>(lds, f, wx0, wy0, tid, speed ? &sp : nullptr);
   const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
-  if (within_reach(m, c, tid)) {
+  const bool reach =
+      speed ? within_reach_speed(m, sp, c, tid, speed, bid) : within_reach(m, c, tid);
+  if (reach) {
     auto run = [&](auto tag) {
       using F = decltype(tag);
       const F a = window_as<F>(lds, 0, wx0, wy0), b = window_as<F>(lds, 1, wx0, wy0),
@@ -306,10 +336,11 @@ template <int TY>
 __global__ __launch_bounds__(256) void k_advect_dye_lds(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ q,
     float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx, int ny, float c, float k0,
-    float k1, float k2, uint8_t* __restrict__ sten) {
+    float k1, float k2, uint8_t* __restrict__ sten, float* __restrict__ speed) {
   int bx, by;
   xcd_block(&bx, &by);
-  advect_dye_tile<TY>(u, v, q, qo, lbl, nx, ny, c, k0, k1, k2, sten, bx, by);
+  advect_dye_tile<TY>(u, v, q, qo, lbl, nx, ny, c, k0, k1, k2, sten, bx, by, speed,
+                      blockIdx.x + blockIdx.y * gridDim.x);
 }
 
 // advect.ts:291-311, dye on the velocity grid: windows of u, v, the 3
@@ -397,12 +428,17 @@ void launch_maccormack_faces(hipStream_t s, const float* un, const float* vn, co
                                                                       lbl, nx, ny, c);
 }
 
+int advect_dye_speed_blocks(int nx, int ny) {
+  const dim3 g = grid_t<kTyAdvectDye>(nx, ny);
+  return (int)(g.x * g.y);
+}
+
 void launch_advect_dye(hipStream_t s, const float* u, const float* v, const float* q, float* qo,
                        const uint8_t* lbl, int nx, int ny, int ds, float c, float k0, float k1,
-                       float k2, bool lds, uint8_t* sten) {
+                       float k2, bool lds, uint8_t* sten, float* speed) {
   if (lds && ds == 1)
     k_advect_dye_lds<kTyAdvectDye><<<grid_t<kTyAdvectDye>(nx, ny), dim3(BX, BY), 0, s>>>(u, v, q, qo, lbl, nx, ny, c, k0, k1,
-                                                             k2, sten);
+                                                             k2, sten, speed);
   else
     k_advect_dye<<<grid2(nx * ds, ny * ds), dim3(BX, BY), 0, s>>>(u, v, q, qo, lbl, nx, ny, ds,
                                                                   c, k0, k1, k2);

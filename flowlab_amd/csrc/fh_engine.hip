@@ -800,25 +800,45 @@ int project(fh_engine* e, double grad_scale) {
   return FH_OK;
 }
 
-int dye_phase(fh_engine* e, int scheme, double dt, const double keep[3]) {
+// The max face speed of the (unchanged) velocity from the forward dye pass's
+// staged windows: each workgroup's maximum, then their maximum into the
+// mapped h_fmax, as the separate reduction would leave it.
+bool speed_in_dye(const fh_engine* e) { return e->advect_lds && e->ds == 1; }
+
+int dye_phase(fh_engine* e, int scheme, double dt, const double keep[3], bool speed = false) {
   float c = (float)(dt / e->h);
   float k0 = keep ? (float)keep[0] : 1.f, k1 = keep ? (float)keep[1] : 1.f,
         k2 = keep ? (float)keep[2] : 1.f;
+  float* sp = (speed && speed_in_dye(e)) ? e->fpartials : nullptr;
+  auto speed_done = [&]() -> int {
+    if (!sp) return FH_OK;
+    launch_finalize_max(e->s, e->fpartials, advect_dye_speed_blocks(e->nx, e->ny), e->fmax,
+                        e->d_fmax);
+    LAUNCHED("max_speed");
+    HIPCHK(hipEventRecord(e->fmax_ev, e->s));
+    e->fmax_epoch = e->vel_epoch;
+    return FH_OK;
+  };
+  int st;
   if (scheme == FH_MACCORMACK) {
     // the dye LDS kernels (dye scale 1) record and read the limiter stencils
     uint8_t* sten_d = (e->advect_lds && e->ds == 1) ? e->sten_d : nullptr;
     {
       Trace t(e, FH_K_ADVECT_DYE);
       launch_advect_dye(e->s, e->u, e->v, e->dye, e->dyeh, e->lbl, e->nx, e->ny, e->ds, c, 1.f,
-                        1.f, 1.f, e->advect_lds, sten_d);
+                        1.f, 1.f, e->advect_lds, sten_d, sp);
     }
+    if ((st = speed_done()) != FH_OK) return st;
     Trace t(e, FH_K_MACCORMACK_DYE);
     launch_maccormack_dye(e->s, e->u, e->v, e->dye, e->dyeh, e->dye2, e->lbl, e->nx, e->ny, e->ds,
                           c, k0, k1, k2, e->advect_lds, sten_d);
   } else {
-    Trace t(e, FH_K_ADVECT_DYE);
-    launch_advect_dye(e->s, e->u, e->v, e->dye, e->dye2, e->lbl, e->nx, e->ny, e->ds, c, k0, k1,
-                      k2, e->advect_lds);
+    {
+      Trace t(e, FH_K_ADVECT_DYE);
+      launch_advect_dye(e->s, e->u, e->v, e->dye, e->dye2, e->lbl, e->nx, e->ny, e->ds, c, k0,
+                        k1, k2, e->advect_lds, nullptr, sp);
+    }
+    if ((st = speed_done()) != FH_OK) return st;
   }
   LAUNCHED("advect_dye");
   swap_ptr(e->dye, e->dye2);
@@ -906,7 +926,9 @@ fh_engine* fh_create(int nx, int ny, int dye_scale, double h) {
   e->lc_dc = coarse_start(e, e->nlev_dc);
   size_t nblk = (size_t)((nx + 63) / 64) * ((ny + 3) / 4);
   if (dalloc(e, &e->partials, 4 * nblk + 4 * kFoldBlocks) || dalloc(e, &e->counter, 1) || dalloc(e, &e->sums, 8) ||
-      dalloc(e, &e->fpartials, 1024) || dalloc(e, &e->fmax, 1) || dalloc(e, &e->done, 1))
+      dalloc(e, &e->fpartials,
+             std::max<size_t>(1024, (size_t)advect_dye_speed_blocks(nx, ny))) ||
+      dalloc(e, &e->fmax, 1) || dalloc(e, &e->done, 1))
     return bail();
   if (hipHostMalloc((void**)&e->h_sums, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&e->h_res, 24 * sizeof(double),
@@ -1118,16 +1140,20 @@ int fh_step(fh_engine* e, int scheme, double dt, double scale, double grad_scale
   if ((st = mark(e, 4))) return st;
   if ((st = project(e, grad_scale))) return st;
   // The next step's CFL reads this velocity (simulation.ts:474-477); the dye
-  // phase below leaves it unchanged, so its reduction goes here (the result
-  // lands in the mapped h_fmax) and the host's wait for it overlaps the dye
-  // kernels.
-  launch_max_speed(e->s, e->u, e->v, e->nx, e->ny, e->fpartials, e->counter, e->fmax,
-                   e->d_fmax);
-  LAUNCHED("max_speed");
-  HIPCHK(hipEventRecord(e->fmax_ev, e->s));
-  e->fmax_epoch = e->vel_epoch;
+  // phase below leaves it unchanged, so its maximum is taken from the forward
+  // dye pass's staged velocity windows (or, without the LDS dye pass, by its
+  // own reduction here), the result lands in the mapped h_fmax and the host's
+  // wait for it overlaps the remaining dye kernels.
+  const bool speed = speed_in_dye(e);
+  if (!speed) {
+    launch_max_speed(e->s, e->u, e->v, e->nx, e->ny, e->fpartials, e->counter, e->fmax,
+                     e->d_fmax);
+    LAUNCHED("max_speed");
+    HIPCHK(hipEventRecord(e->fmax_ev, e->s));
+    e->fmax_epoch = e->vel_epoch;
+  }
   if ((st = mark(e, 5))) return st;
-  if ((st = dye_phase(e, scheme, dt, keep))) return st;
+  if ((st = dye_phase(e, scheme, dt, keep, speed))) return st;
   if ((st = mark(e, 6))) return st;
   e->ev_valid = e->timing;
   return FH_OK;

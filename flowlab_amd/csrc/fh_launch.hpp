@@ -41,9 +41,13 @@ void launch_maccormack_faces(hipStream_t s, const float* un, const float* vn, co
                              const float* heat = nullptr, float gain = 0.f);
 // Three dye channels on one backtrace, scaled by keep[c] at the write.
 // advect.ts:241-270 (+ the decay of simulation.ts:569-576).
+// speed (lds with ds == 1 only; may be null): each workgroup stores the
+// largest |u|, |v| of its staged velocity window there, advect_dye_speed_blocks
+// values in all; their maximum is the max face speed (launch_finalize_max).
 void launch_advect_dye(hipStream_t s, const float* u, const float* v, const float* q, float* qo,
                        const uint8_t* lbl, int nx, int ny, int ds, float c, float k0, float k1,
-                       float k2, bool lds, uint8_t* sten = nullptr);
+                       float k2, bool lds, uint8_t* sten = nullptr, float* speed = nullptr);
+int advect_dye_speed_blocks(int nx, int ny);
 // advect.ts:291-311 for three channels, times keep[c].
 void launch_maccormack_dye(hipStream_t s, const float* u, const float* v, const float* qn,
                            const float* qh, float* qo, const uint8_t* lbl, int nx, int ny, int ds,
@@ -78,6 +82,9 @@ void launch_commit_labels(hipStream_t s, double* p, float* u, float* v, float* d
 void launch_max_speed(hipStream_t s, const float* u, const float* v, int nx, int ny,
                       float* partials, unsigned* counter, float* out,
                       float* host_out = nullptr);
+// The largest of n per-workgroup maxima into *out and host_out (optional).
+void launch_finalize_max(hipStream_t s, const float* partials, int n, float* out,
+                         float* host_out);
 
 // ---- pressure (fh_mg.hip) --------------------------------------------------
 // Reductions over more than kFoldAbove blocks fold their partials with

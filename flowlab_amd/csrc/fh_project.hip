@@ -331,6 +331,40 @@ __global__ __launch_bounds__(256) void k_finalize_max(const float* __restrict__ 
   }
 }
 
+// The maximum of many partials: 1024 threads, each taking float4s four deep
+// (partials is a hipMalloc base, so 16-byte aligned), then the tail.
+__global__ __launch_bounds__(1024) void k_finalize_max_wide(const float* __restrict__ partials,
+                                                            int n, float* __restrict__ out,
+                                                            float* __restrict__ host_out) {
+  __shared__ float sh[16];
+  const int tid = threadIdx.x;
+  const float4* p4 = reinterpret_cast<const float4*>(partials);
+  const int n4 = n >> 2;
+  float t = 0.f;
+  auto m4 = [](float4 x) { return fmaxf(fmaxf(x.x, x.y), fmaxf(x.z, x.w)); };
+  int k = tid;
+  for (; k + 3 * 1024 < n4; k += 4 * 1024) {
+    const float4 a = p4[k], b = p4[k + 1024], c = p4[k + 2048], d = p4[k + 3072];
+    t = fmaxf(t, fmaxf(fmaxf(m4(a), m4(b)), fmaxf(m4(c), m4(d))));
+  }
+  for (; k < n4; k += 1024) t = fmaxf(t, m4(p4[k]));
+  for (int b = (n4 << 2) + tid; b < n; b += 1024) t = fmaxf(t, partials[b]);
+  t = wave_max(t);
+  if ((tid & 63) == 0) sh[tid >> 6] = t;
+  __syncthreads();
+  if (tid == 0) {
+    float m = 0.f;
+    for (int w = 0; w < 16; w++) m = fmaxf(m, sh[w]);
+    out[0] = m;
+    if (host_out) host_out[0] = m;
+  }
+}
+
+void launch_finalize_max(hipStream_t s, const float* partials, int n, float* out,
+                         float* host_out) {
+  k_finalize_max_wide<<<1, 1024, 0, s>>>(partials, n, out, host_out);
+}
+
 void launch_vorticity(hipStream_t s, const float* u, const float* v, float* w, int nx, int ny,
                       float inv_h) {
   k_vorticity<<<grid2(nx, ny), dim3(BX, BY), 0, s>>>(u, v, w, nx, ny, inv_h);
