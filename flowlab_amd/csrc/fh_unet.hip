@@ -149,6 +149,10 @@ __global__ void k_head_grad(const float* __restrict__ dout, long n, bf16_t* __re
 // the correction is 0), the step m = <r0, q> / <q, q> (at least 0) minimises
 // |b - A (p_prev + m o)| over fluid cells, so the warm-started residual is
 // never larger than the previous pressure's, and the output scale drops out.
+// Blocks of k_warm_dots: 16 per CU, so the pass streams at HBM rate (512
+// left it at ~7 % of the HBM peak at 2048x1024).
+constexpr int kWarmBlocks = 4096;
+
 // k_warm_dots: block partial sums {r0.q, q.q} over fluid cells with an open
 // neighbour (info byte: bit 7 fluid, bits 3..6 open +x,-x,+y,-y, 0..2 count).
 __global__ __launch_bounds__(256) void k_warm_dots(const float* __restrict__ r0,
@@ -534,8 +538,9 @@ int engine_warm_start(fh_unet* u, hipStream_t s, const float* r32, const uint8_t
   k_prep<double><<<g, 256, 0, s>>>(r32, p, lbl, nullptr, stats2, nullptr, n, H, W, c,
                                    net.act[0], nullptr, nullptr);
   unet_forward(net, s);
-  k_warm_dots<<<512, 256, 0, s>>>(r32, net.out, info, W, n, u->partials);
-  launch_finalize_sum4(s, u->partials, 512, u->sums);
+  // enough blocks to fill the chip (the partials buffer holds kWarmBlocks)
+  k_warm_dots<<<kWarmBlocks, 256, 0, s>>>(r32, net.out, info, W, n, u->partials);
+  launch_finalize_sum4(s, u->partials, kWarmBlocks, u->sums);
   k_apply_warm<<<nblk(n), 256, 0, s>>>(p, net.out, info, u->sums, n);
   UCHK(hipGetLastError());
   return FH_OK;
@@ -558,7 +563,7 @@ fh_unet* fh_unet_create(unsigned long long seed) {
     delete u;
     return nullptr;
   }
-  (void)hipMalloc((void**)&u->partials, 4096 * 8);
+  (void)hipMalloc((void**)&u->partials, 4 * kWarmBlocks * sizeof(double));
   (void)hipMalloc((void**)&u->sums, 64);
   return u;
 }
