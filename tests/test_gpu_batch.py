@@ -1,11 +1,11 @@
-"""Batched scenes on the GPU (fh_batch_*): scenes of uneven size, each a
-single-grid engine on its own stream, stepped together from a host thread
-pool with dt from each scene's CFL.
+"""Batched scenes on the GPU (fh_batch_*): scenes of uneven size stepped
+together, each phase one launch over a (scene, block) work list, with dt from
+each scene's CFL.
 
-Each scene runs the single-grid engine's step, so it must equal the same scene
-stepped alone bit for bit however the threads interleave; both are held to
-the FP64 oracle at the per-kernel bar (relative L2 <= 1e-5 in FP32) with every
-pressure solve at 1e-6 by the oracle's own residual.
+Each scene's blocks take the indices and partials slots of that scene's own
+single-grid launch, so a scene must equal the same scene stepped alone bit for
+bit; both are held to the FP64 oracle at the per-kernel bar (relative L2 <=
+1e-5 in FP32) with every pressure solve at 1e-6 by the oracle's own residual.
 """
 import numpy as np
 import pytest
@@ -139,3 +139,69 @@ def test_batch_rejects_bad_arguments():
         b.step(cfl=-1.0)
     with pytest.raises(hip.HipError):
         b.set_tile(99)
+
+
+def heated(scene, cx=0.5, cy=0.3, sigma=0.15):
+    """The scene plus a Gaussian heat blob (dye channel 0), smoke in channel 1
+    and a uniform channel 2, so buoyancy and decay act."""
+    g0, sc = scene
+
+    def dye(g, d):
+        x = (np.arange(g.nx) + 0.5) * g.h
+        y = (np.arange(g.ny) + 0.5) * g.h
+        r2 = (x[None, :] - cx * g.nx * g.h) ** 2 + (y[:, None] - cy) ** 2
+        blob = np.exp(-r2 / (sigma * sigma)).ravel()
+        d[0][:] = blob
+        d[1][:] = 0.5 * blob
+        d[2][:] = 0.25
+    return g0, S.SceneSpec(labels=sc.labels, seed=sc.seed, dye=dye)
+
+
+# (buoyancy, confinement, per-channel decay) per scene: buoyancy alone (fused
+# into the MacCormack face pass), buoyancy with confinement (the vorticity and
+# force passes), confinement with decay, decay alone, buoyancy on an open outlet
+FORCES = [(2.0, 0.0, (0.0, 0.0, 0.0)), (2.0, 0.3, (0.0, 0.0, 0.0)),
+          (0.0, 0.3, (0.5, 1.0, 0.0)), (0.0, 0.0, (0.2, 0.2, 0.2)),
+          (1.0, 0.0, (0.0, 0.3, 0.0))]
+
+
+@pytest.mark.parametrize("scheme", ["macCormack", "semiLagrangian"])
+def test_batch_forces_equal_single_engine_bitwise(scheme):
+    """Buoyancy, vorticity confinement and dye decay set per scene step each
+    scene exactly as a single-grid engine with the same parameters does."""
+    from flowlab_amd.simulation import Simulation
+    scenes = [heated(S.decaying_turbulence(64, 64, 21)),
+              heated(S.decaying_turbulence(128, 64, 22)),
+              heated(S.decaying_turbulence(64, 96, 23)),
+              heated(S.decaying_turbulence(96, 64, 24)),
+              heated(S.karman_street(128, 64), cx=0.25, cy=0.5)]
+    b, _ = make_batch(scenes)
+    sims = []
+    for s, ((g, sc), (buoy, conf, decay)) in enumerate(zip(scenes, FORCES)):
+        b.set_forces(s, buoyancy=buoy, confinement=conf, decay=decay)
+        sim = Simulation(g.nx, g.ny, dict(scheme=scheme, cfl_target=0.5, dt_max=1.0,
+                                         solver="mg-fcg", pressure_iters=60, tol=TOL,
+                                         residual_every=0, buoyancy=buoy, confinement=conf,
+                                         dye_decay=list(decay)))
+        sim.reset(sc)
+        sims.append(sim)
+    for n in range(3):
+        print(f"{scheme} step {n}", flush=True)
+        b.step(scheme=scheme, cfl=0.5, dt_max=1.0, tol=TOL, max_cycles=60)
+        dt, iters, res = b.results()
+        for s, sim in enumerate(sims):
+            sim.step()
+            u, v, p, dye = b.read(s)
+            sdye = sim.read_dye()
+            assert dt[s] == sim.dt, s
+            assert iters[s] == sim.iters, (s, iters[s], sim.iters)
+            assert np.array_equal(u, sim.f.u) and np.array_equal(v, sim.f.v), s
+            assert np.array_equal(p, sim.stepper.engine.read_pressure()), s
+            for c in range(3):
+                assert np.array_equal(dye[c], sdye[c]), (s, c)
+            assert res[s] <= TOL
+    # the forces acted: a heated scene's velocity differs from the unforced one
+    plain, _ = make_batch(scenes[:1])
+    for n in range(3):
+        plain.step(scheme=scheme, cfl=0.5, dt_max=1.0, tol=TOL, max_cycles=60)
+    assert not np.array_equal(plain.read(0)[1], b.read(0)[1])
