@@ -200,14 +200,8 @@ int prepare(fh_batch* b) {
       const Level& L = e->lv[l];
       d.lv[l] = LevelDesc{L.x, L.xa, L.b, L.info, L.nx, L.ny};
     }
-    CoarseLevels& C = d.cl;
-    C.n = e->nlev_dc - e->lc_dc;
-    for (int k = 0; k < C.n; k++) {
-      const Level& L = e->lv[e->lc_dc + k];
-      C.nx[k] = L.nx;
-      C.ny[k] = L.ny;
-      C.info[k] = L.info;
-    }
+    d.cl = engine_coarse_levels(e, e->lc_dc, e->nlev_dc);
+    const CoarseLevels& C = d.cl;
     d.M = e->M, d.cells = e->cells, d.ncoarse = e->ncoarse;
     b->max_lc = std::max(b->max_lc, d.lc);
     b->coarse_lds = std::max(b->coarse_lds, coarse_vcycle_lds_bytes(C));

@@ -304,9 +304,15 @@ int refresh_levels(fh_engine* e) {
                           e->lv[l].nx, e->lv[l].ny);
     LAUNCHED("coarsen_labels");
   }
-  for (auto& L : e->lv) {
+  for (size_t l = 0; l < e->lv.size(); l++) {
+    Level& L = e->lv[l];
     launch_cell_info(e->s, L.lbl, L.nx, L.ny, L.info);
     LAUNCHED("cell_info");
+    if ((int)l >= e->cinfo_from) {
+      launch_pad_info(e->s, L.info, L.nx, L.ny,
+                      e->cinfo + (e->cinfo_off[l] - e->cinfo_off[e->cinfo_from]));
+      LAUNCHED("pad_info");
+    }
     HIPCHK(hipMemsetAsync(L.x, 0, (size_t)L.nx * L.ny * sizeof(float), e->s));
     HIPCHK(hipMemsetAsync(L.xa, 0, (size_t)L.nx * L.ny * sizeof(float), e->s));
   }
@@ -355,8 +361,12 @@ CoarseLevels coarse_levels(const fh_engine* e, int lc, int nlev) {
   for (int k = 0; k < C.n; k++) {
     C.nx[k] = e->lv[lc + k].nx;
     C.ny[k] = e->lv[lc + k].ny;
-    C.info[k] = e->lv[lc + k].info;
   }
+  coarse_levels_layout(C);
+  // sizing before the padded info exists (coarse_start) needs no image
+  C.image = (e->cinfo && lc >= e->cinfo_from)
+                ? e->cinfo + (e->cinfo_off[lc] - e->cinfo_off[e->cinfo_from])
+                : nullptr;
   return C;
 }
 
@@ -857,6 +867,10 @@ int mark(fh_engine* e, int phase) {
 
 }  // namespace
 
+fh::CoarseLevels fh::engine_coarse_levels(const fh_engine* e, int lc, int nlev) {
+  return coarse_levels(e, lc, nlev);
+}
+
 int fh::engine_batch_prepare(fh_engine* e) {
   GUARD("HipBatch.prepare()");
   int st;
@@ -924,6 +938,14 @@ fh_engine* fh_create(int nx, int ny, int dye_scale, double h) {
   }
   e->lc_ref = coarse_start(e, e->nlev_ref);
   e->lc_dc = coarse_start(e, e->nlev_dc);
+  // padded info of every level a sub-cycle may stage
+  e->cinfo_from = std::min(e->lc_ref, e->lc_dc);
+  e->cinfo_off.assign(e->lv.size() + 1, 0);
+  for (size_t l = 0; l < e->lv.size(); l++)
+    e->cinfo_off[l + 1] = e->cinfo_off[l] +
+                          ((int)l >= e->cinfo_from ? padded_info_bytes(e->lv[l].nx, e->lv[l].ny) : 0);
+  if (dalloc(e, &e->cinfo, (size_t)(e->cinfo_off[e->lv.size()] - e->cinfo_off[e->cinfo_from])))
+    return bail();
   size_t nblk = (size_t)((nx + 63) / 64) * ((ny + 3) / 4);
   if (dalloc(e, &e->partials, 4 * nblk + 4 * kFoldBlocks) || dalloc(e, &e->counter, 1) || dalloc(e, &e->sums, 8) ||
       dalloc(e, &e->fpartials,
@@ -971,6 +993,7 @@ void fh_destroy(fh_engine* e) {
   if (e->sten_f) (void)hipFree(e->sten_f);
   if (e->sten_d) (void)hipFree(e->sten_d);
   if (e->cells) (void)hipFree(e->cells);
+  if (e->cinfo) (void)hipFree(e->cinfo);
   if (e->patch_tiles) (void)hipFree(e->patch_tiles);
   if (e->counter) (void)hipFree(e->counter);
   if (e->done) (void)hipFree(e->done);

@@ -56,6 +56,11 @@ struct fh_engine {
   int tile[fh::kMaxTiledLevels] = {};  // tile shape per tiled level (kTileShapes)
   int coarse_group = 0;            // coarse sub-cycle workgroup (kCoarseGroups)
   bool advect_lds = true;          // LDS-staged advection kernels
+  // every level from cinfo_from on with its info padded as the coarse
+  // sub-cycle stages it (launch_pad_info); level l at byte cinfo_off[l]
+  uint8_t* cinfo = nullptr;
+  int cinfo_from = 0;
+  std::vector<int> cinfo_off;
   double* M = nullptr;  // coarsest-level pseudo-inverse, n*n
   int* cells = nullptr;
   int ncoarse = 0;
@@ -120,4 +125,6 @@ namespace fh {
 // Makes an engine ready for the batched step (fh_batch.hip): its multigrid
 // hierarchy current for its labels and the flexible-CG vectors allocated.
 int engine_batch_prepare(fh_engine* e);
+// The coarse sub-cycle's description of levels [lc, nlev) of an engine.
+CoarseLevels engine_coarse_levels(const fh_engine* e, int lc, int nlev);
 }  // namespace fh

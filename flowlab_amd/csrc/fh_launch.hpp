@@ -177,12 +177,38 @@ constexpr int kMaxDirect = 1024;
 // Most cells of one colour on a level of the one-workgroup coarse sub-cycle
 // (a workgroup of NC threads holds kCoarseSweepCells / NC per thread).
 constexpr int kCoarseSweepCells = 4096;
+// Multiplier m of q / d = mulhi(q, m) for 0 <= q, d < 2^16 (FastDiv,
+// fh_mgtile.hpp); 0 for d = 1, whose quotient is q itself.
+__host__ __device__ inline unsigned fastdiv_magic(int d) {
+  return d > 1 ? 0xFFFFFFFFu / (unsigned)d + 1u : 0u;
+}
+// Bytes of a level's info array padded by a ring of zeros (pitch nx + 2),
+// rounded up to 16.
+inline int padded_info_bytes(int nx, int ny) { return ((nx + 2) * (ny + 2) + 15) & ~15; }
+// The levels of the one-workgroup coarse sub-cycle and its LDS layout: every
+// level's unknown x (padded by a ring of zeros, pitch nx + 2), then every
+// level's right-hand side b (same padding), then every level's info bytes
+// (padded the same way, each level rounded up to 16 bytes), then the direct
+// solve's dense vector and indices.  The offsets and the division
+// multipliers of nx and of the red-black pairs per row (nx + 1) / 2 come from
+// the host (coarse_levels_layout), so the kernel neither divides nor searches
+// levels.  image: the info region exactly as it is staged (every level's
+// padded info, launch_pad_info), built once per geometry.
 struct CoarseLevels {
   int n;
   int nx[kMaxCoarse];
   int ny[kMaxCoarse];
-  const uint8_t* info[kMaxCoarse];  // per-level cell info (launch_cell_info)
+  int xo[kMaxCoarse], bo[kMaxCoarse];  // float offsets of x and b
+  int io[kMaxCoarse];                  // byte offset of info
+  unsigned mnx[kMaxCoarse], mpr[kMaxCoarse];
+  int cells;   // padded cells of all levels (floats of the x region)
+  int ibytes;  // bytes of the info region
+  const uint8_t* image;
 };
+// Fills the layout fields of L from n, nx and ny.
+void coarse_levels_layout(CoarseLevels& L);
+// out = info padded by a ring of zeros (pitch nx + 2), padded_info_bytes long.
+void launch_pad_info(hipStream_t s, const uint8_t* info, int nx, int ny, uint8_t* out);
 // info byte per cell: bit 7 fluid, bits 0..2 the count of non-solid
 // in-domain neighbours; 0 for non-fluid cells.
 void launch_cell_info(hipStream_t s, const uint8_t* lbl, int nx, int ny, uint8_t* info);

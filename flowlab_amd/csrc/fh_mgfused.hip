@@ -70,13 +70,41 @@ __global__ void k_cell_info(const uint8_t* __restrict__ lbl, int nx, int ny,
   if (i < nx && j < ny) info[i + j * nx] = cell_info(lbl, nx, ny, i, j);
 }
 
+void coarse_levels_layout(CoarseLevels& L) {
+  int cells = 0, ibytes = 0;
+  for (int l = 0; l < L.n; l++) cells += (L.nx[l] + 2) * (L.ny[l] + 2);
+  for (int l = 0, c = 0; l < L.n; l++) {
+    L.xo[l] = c;
+    L.bo[l] = cells + c;
+    L.io[l] = 8 * cells + ibytes;
+    L.mnx[l] = fastdiv_magic(L.nx[l]);
+    L.mpr[l] = fastdiv_magic((L.nx[l] + 1) >> 1);
+    c += (L.nx[l] + 2) * (L.ny[l] + 2);
+    ibytes += padded_info_bytes(L.nx[l], L.ny[l]);
+  }
+  L.cells = cells;
+  L.ibytes = ibytes;
+}
+
 size_t coarse_vcycle_lds_bytes(const CoarseLevels& L) {
   size_t b = 0;
-  for (int l = 0; l < L.n; l++) {
-    size_t n = (size_t)(L.nx[l] + 2) * (L.ny[l] + 2);
-    b += 8 * n + ((n + 3) & ~(size_t)3);
-  }
+  for (int l = 0; l < L.n; l++)
+    b += 8 * (size_t)(L.nx[l] + 2) * (L.ny[l] + 2) + padded_info_bytes(L.nx[l], L.ny[l]);
   return b + 8 * (size_t)kMaxDirect;  // the direct solve's dense right-hand side and indices
+}
+
+__global__ void k_pad_info(const uint8_t* __restrict__ info, int nx, int ny,
+                           uint8_t* __restrict__ out, int nbytes) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  if (q >= nbytes) return;
+  const int px = nx + 2, j = q / px, i = q - j * px;
+  const bool in = i >= 1 && i <= nx && j >= 1 && j <= ny;
+  out[q] = in ? info[(i - 1) + (j - 1) * nx] : 0;
+}
+
+void launch_pad_info(hipStream_t s, const uint8_t* info, int nx, int ny, uint8_t* out) {
+  const int n = padded_info_bytes(nx, ny);
+  k_pad_info<<<(n + 255) / 256, 256, 0, s>>>(info, nx, ny, out, n);
 }
 
 void launch_cell_info(hipStream_t s, const uint8_t* lbl, int nx, int ny, uint8_t* info) {

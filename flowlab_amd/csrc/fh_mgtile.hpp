@@ -130,8 +130,9 @@ typedef __attribute__((address_space(3))) uint8_t* lds_bp;
 struct FastDiv {
   unsigned m;
   int d;
-  __device__ __forceinline__ explicit FastDiv(int dd)
-      : m(dd > 1 ? 0xFFFFFFFFu / (unsigned)dd + 1u : 0u), d(dd) {}
+  __device__ __forceinline__ explicit FastDiv(int dd) : m(fastdiv_magic(dd)), d(dd) {}
+  // with the multiplier computed on the host (fastdiv_magic)
+  __device__ __forceinline__ FastDiv(unsigned mm, int dd) : m(mm), d(dd) {}
   __device__ __forceinline__ int div(int q) const {
     return d > 1 ? (int)__umulhi((unsigned)q, m) : q;
   }
@@ -150,14 +151,14 @@ struct CoarseSweep {
   int nk;  // slots holding a cell for this wave (wave-uniform)
   int LX;
 
+  // dP: division by the pairs per row, (lx - 1) / 2
   __device__ __forceinline__ void setup(const lds_fp sb, const lds_bp si, int lx, int ly,
-                                        int tid) {
+                                        FastDiv dP, int tid) {
     LX = lx;
     const int P = (lx - 1) >> 1;  // pairs per row of the nx = lx - 2 cells
     const int total = P * (ly - 2);
     const int wbase = __builtin_amdgcn_readfirstlane(tid & ~63);
     nk = total > wbase ? min(KC, (total - wbase + NC - 1) / NC) : 0;
-    const FastDiv dP(P);
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -420,46 +421,29 @@ __device__ __forceinline__ void prolong_smooth_tile(const float* __restrict__ xi
 }
 
 // ---- k_coarse_vcycle -----------------------------------------------------------
-// The whole sub-cycle from level lc down, in one workgroup's LDS.  Each
-// level's arrays are padded by a one-cell ring of zeros (pitch nx+2) so the
-// half-sweep's neighbour reads never leave the level.
+// The whole sub-cycle from level lc down, in one workgroup's LDS, laid out as
+// CoarseLevels describes (each level padded by a one-cell ring of zeros, so
+// the half-sweep's neighbour reads never leave the level).
 // The workgroup size NC is a template parameter (the autotuner picks it): a
 // larger group has more lanes per phase, a smaller one cheaper barriers.  KC
-// = padded cells staged per thread per pass, so NC*KC covers the usual
-// coarse hierarchy in one pass.
+// = cells of a colour per thread, so NC*KC covers the usual coarse hierarchy
+// in one pass.  With up to 16 waves on one CU every VALU instruction per
+// thread costs ~16 cycles of each SIMD, so the per-level offsets and
+// division multipliers come from the host and the staging is a straight copy.
 constexpr int NC = 1024;  // the batch's coarse workgroup
 constexpr int kCoarseGroups[] = {1024, 512, 256};
 constexpr int kNumCoarseGroups = 3;
 
-__device__ __forceinline__ int padded(int nx, int ny) { return (nx + 2) * (ny + 2); }
-
-__device__ __forceinline__ int clevel_bytes(int nx, int ny) {
-  const int n = padded(nx, ny);
-  return 8 * n + ((n + 3) & ~3);
-}
-
-// One coarse level's LDS arrays: x and b (pitch nx+2, padded by the zero
-// ring) and the info bytes, at the level's offset in the dynamic LDS block
-// (levels in order: x, b, info each, as coarse_vcycle_lds_bytes sizes them).
+// One coarse level's LDS arrays and sizes.
 struct CLevel {
   lds_fp x, b;
   lds_bp i;
   int nx, ny, px, py;
 };
 
-// Bytes of all levels' arrays; the direct solve's dense vector follows them.
-__device__ __forceinline__ int coarse_levels_bytes(const CoarseLevels& L) {
-  int off = 0;
-  for (int k = 0; k < L.n; k++) off += clevel_bytes(L.nx[k], L.ny[k]);
-  return off;
-}
-
 __device__ __forceinline__ CLevel clevel(lds_bp smem, const CoarseLevels& L, int l) {
-  int off = 0;
-  for (int k = 0; k < l; k++) off += clevel_bytes(L.nx[k], L.ny[k]);
-  const int nx = L.nx[l], ny = L.ny[l], n = padded(nx, ny);
-  lds_bp p = smem + off;
-  return {(lds_fp)p, (lds_fp)(p + 4 * n), p + 8 * n, nx, ny, nx + 2, ny + 2};
+  const int nx = L.nx[l], ny = L.ny[l];
+  return {(lds_fp)smem + L.xo[l], (lds_fp)smem + L.bo[l], smem + L.io[l], nx, ny, nx + 2, ny + 2};
 }
 
 // Body of the sub-cycle for a workgroup of NC threads; smem is the dynamic
@@ -474,46 +458,70 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
                                                    int direct, unsigned char* smem_generic) {
   const int tid = threadIdx.x;
   const lds_bp smem = (lds_bp)smem_generic;
-  // this thread's coarse unknown for the direct solve's gather, loaded now
-  // so its latency hides behind the staging
+  // The direct solve's operands for this thread -- its coarse unknown and
+  // the first sixteen entries of its row segment of M (row tid / 8, columns
+  // tid % 8 + 8m) -- loaded now: a barrier waits only for LDS, so the loads
+  // arrive while the levels are smoothed.
   const int my_cell = (direct && tid < ncoarse) ? cells[tid] : 0;
+  double mpre[16];
   {
-    // All levels' padded cells as one index range, KC per thread, loads
-    // issued together before the LDS writes (one memory wait per thread).
-    int total = 0;
-    for (int l = 0; l < L.n; l++) total += padded(L.nx[l], L.ny[l]);
-    for (int base = 0; base < total; base += KC * NC) {
-      float rx[KC], rb[KC];
-      uint8_t ri[KC];
-      int ox[KC], ob[KC], oi[KC];  // LDS byte offsets of the cell's x, b, info
+    const bool has = direct && tid < 8 * ncoarse;
+    const double* row = M + (size_t)(has ? tid >> 3 : 0) * ncoarse;
 #pragma unroll
-      for (int k = 0; k < KC; k++) {
-        int t = base + tid + k * NC;
-        int l = 0, q = t, off = 0, n = padded(L.nx[0], L.ny[0]);
-        while (l < L.n - 1 && q >= n) {
-          q -= n;
-          off += 8 * n + ((n + 3) & ~3);
-          l++;
-          n = padded(L.nx[l], L.ny[l]);
-        }
-        int nx = L.nx[l], ny = L.ny[l], px = nx + 2;
-        int lj = FastDiv(px).div(q), li = q - lj * px;
-        int gi = li - 1, gj = lj - 1;
-        bool in = t < total && gi >= 0 && gj >= 0 && gi < nx && gj < ny;
-        int g = in ? gi + gj * nx : 0;
-        ri[k] = in ? L.info[l][g] : 0;
-        rx[k] = (l == 0 && in && x0in) ? x0in[g] : 0.f;
-        rb[k] = (l == 0 && in) ? b0[g] : 0.f;
-        ox[k] = t < total ? off + 4 * q : -1;
-        ob[k] = off + 4 * n + 4 * q;
-        oi[k] = off + 8 * n + q;
+    for (int m = 0; m < 16; m++) {
+      const int q = (tid & 7) + 8 * m;
+      mpre[m] = (has && q < ncoarse) ? row[q] : 0.0;
+    }
+  }
+  {
+    // Staging: the info image in 16-byte chunks and level 0's right-hand
+    // side into its padded b, each pass issuing all its loads before its
+    // first LDS write (one pass for the usual hierarchy).  Every level's x is
+    // zeroed (padding ring included); every other b is written by the
+    // restriction before it is read.
+    constexpr int KI = 4;  // 16-byte chunks per thread and pass
+    typedef __attribute__((address_space(3))) unsigned* lds_up;
+    const lds_up di = (lds_up)(smem + L.io[0]);
+    const int nchunks = L.ibytes >> 4;
+    const int nx0 = L.nx[0], n0 = nx0 * L.ny[0];
+    const FastDiv d0(L.mnx[0], nx0);
+    for (int pass = 0; pass * KI * NC < nchunks || pass * KC * NC < n0; pass++) {
+      uint4 iv[KI];
+      float bv0[KC];
+      int bo0[KC];
+#pragma unroll
+      for (int k = 0; k < KI; k++) {
+        const int c = tid + (pass * KI + k) * NC;
+        iv[k] = c < nchunks ? reinterpret_cast<const uint4*>(L.image)[c] : make_uint4(0, 0, 0, 0);
       }
 #pragma unroll
       for (int k = 0; k < KC; k++) {
-        if (ox[k] < 0) continue;
-        smem[oi[k]] = ri[k];
-        *(lds_fp)(smem + ox[k]) = rx[k];
-        *(lds_fp)(smem + ob[k]) = rb[k];
+        const int q = tid + (pass * KC + k) * NC;
+        const int j = d0.div(q), i = q - j * nx0;
+        bv0[k] = q < n0 ? b0[q] : 0.f;
+        bo0[k] = q < n0 ? L.bo[0] + (i + 1) + (j + 1) * (nx0 + 2) : -1;
+      }
+      if (pass == 0)
+        for (int q = tid; q < L.cells; q += NC) ((lds_fp)smem)[q] = 0.f;
+#pragma unroll
+      for (int k = 0; k < KI; k++) {
+        const int c = tid + (pass * KI + k) * NC;
+        if (c < nchunks) {
+          di[4 * c] = iv[k].x;
+          di[4 * c + 1] = iv[k].y;
+          di[4 * c + 2] = iv[k].z;
+          di[4 * c + 3] = iv[k].w;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < KC; k++)
+        if (bo0[k] >= 0) ((lds_fp)smem)[bo0[k]] = bv0[k];
+    }
+    if (x0in) {  // warm start of level 0, over the zeros
+      __syncthreads();
+      for (int q = tid; q < n0; q += NC) {
+        const int j = d0.div(q), i = q - j * nx0;
+        ((lds_fp)smem)[L.xo[0] + (i + 1) + (j + 1) * (nx0 + 2)] = x0in[q];
       }
     }
   }
@@ -523,10 +531,10 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
     bool zero = (l > 0) || !x0in;
     {
       CoarseSweep<NC, KC> sw;
-      sw.setup(F.b, F.i, F.px, F.py, tid);
+      sw.setup(F.b, F.i, F.px, F.py, FastDiv(L.mpr[l], (F.nx + 1) >> 1), tid);
       sw.two_sweeps(F.x, zero);
     }
-    const FastDiv dc(G.nx);
+    const FastDiv dc(L.mnx[l + 1], G.nx);
     for (int q = tid; q < G.nx * G.ny; q += NC) {
       int J = dc.div(q), I = q - J * G.nx;
       float s = 0.f;
@@ -546,13 +554,14 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
     bool zero = (l > 0) || !x0in;
     if (direct) {
       // b at the coarse unknowns gathered into a dense LDS vector, with
-      // their padded LDS indices beside it (both after the level arrays),
+      // their padded LDS indices beside it (both after the info region),
       // then x = M b in FP64, eight lanes per row; lane `part` takes columns
-      // part + 8m, sixteen loads of M in flight at once.
-      const lds_fp bv = (lds_fp)(smem + coarse_levels_bytes(L));
+      // part + 8m, sixteen loads of M in flight at once (the first sixteen
+      // of this thread's first row segment already in mpre).
+      const lds_fp bv = (lds_fp)(smem + L.io[0] + L.ibytes);
       typedef __attribute__((address_space(3))) int* lds_ip;
-      const lds_ip bi = (lds_ip)(smem + coarse_levels_bytes(L) + 4 * kMaxDirect);
-      const FastDiv dn(C.nx);
+      const lds_ip bi = (lds_ip)(smem + L.io[0] + L.ibytes + 4 * kMaxDirect);
+      const FastDiv dn(L.mnx[l], C.nx);
       for (int q = tid; q < ncoarse; q += NC) {
         const int c = q == tid ? my_cell : cells[q], cj = dn.div(c), ci = c - cj * C.nx;
         const int k = (ci + 1) + (cj + 1) * C.px;
@@ -566,10 +575,15 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
         double s = 0.0;
         for (int q0 = part; q0 < ncoarse; q0 += 128) {
           double mv[16];
+          if (t == tid && q0 == part) {
 #pragma unroll
-          for (int m = 0; m < 16; m++) {
-            int q = q0 + 8 * m;
-            mv[m] = q < ncoarse ? row[q] : 0.0;
+            for (int m = 0; m < 16; m++) mv[m] = mpre[m];
+          } else {
+#pragma unroll
+            for (int m = 0; m < 16; m++) {
+              int q = q0 + 8 * m;
+              mv[m] = q < ncoarse ? row[q] : 0.0;
+            }
           }
 #pragma unroll
           for (int m = 0; m < 16; m++)
@@ -582,7 +596,7 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
       }
     } else {
       CoarseSweep<NC, KC> sw;
-      sw.setup(C.b, C.i, C.px, C.py, tid);
+      sw.setup(C.b, C.i, C.px, C.py, FastDiv(L.mpr[l], (C.nx + 1) >> 1), tid);
       sw.two_sweeps(C.x, zero);
       for (int s = 1; s < 4; s++) sw.two_sweeps(C.x, false);
     }
@@ -590,7 +604,7 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
   }
   for (int l = L.n - 2; l >= 0; l--) {
     const CLevel F = clevel(smem, L, l), G = clevel(smem, L, l + 1);
-    const FastDiv dn(F.nx);
+    const FastDiv dn(L.mnx[l], F.nx);
     for (int q = tid; q < F.nx * F.ny; q += NC) {
       int gj = dn.div(q), gi = q - gj * F.nx;
       int lq = (gi + 1) + (gj + 1) * F.px;
@@ -606,13 +620,13 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
     __syncthreads();
     {
       CoarseSweep<NC, KC> sw;
-      sw.setup(F.b, F.i, F.px, F.py, tid);
+      sw.setup(F.b, F.i, F.px, F.py, FastDiv(L.mpr[l], (F.nx + 1) >> 1), tid);
       sw.two_sweeps(F.x, false);
     }
   }
   {
     const CLevel F = clevel(smem, L, 0);
-    const FastDiv dn(F.nx);
+    const FastDiv dn(L.mnx[0], F.nx);
     for (int q = tid; q < F.nx * F.ny; q += NC) {
       int gj = dn.div(q), gi = q - gj * F.nx;
       x0out[q] = F.x[(gi + 1) + (gj + 1) * F.px];
