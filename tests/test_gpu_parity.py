@@ -370,6 +370,79 @@ def test_plume_128_whole_step_matches_oracle():
     assert sim.f.v.max() > 0.01  # the plume rises
 
 
+def _reseed(osim, sim):
+    """The oracle takes the device's current fields: velocity, dye and the
+    pressure the next solve starts from."""
+    osim.u[:] = sim.f.u
+    osim.v[:] = sim.f.v
+    osim.dye[:] = np.concatenate(sim.read_dye())
+    osim.p[:] = sim.stepper.engine.read_pressure()
+
+
+def test_plume_128_every_step_from_device_state():
+    """All 200 steps of the 128^2 plume (RB-SOR omega 1.9 x 200, buoyancy 1,
+    confinement 0.1, MacCormack): before each device step the oracle takes
+    the device's fields and steps once in FP64, and the device's step must
+    match it at the per-kernel bar.  Re-seeding every step keeps the FP32
+    rounding of earlier steps from compounding through the flow, so the bar
+    stays 1e-5 over the whole run."""
+    from flowlab_amd.simulation import Simulation
+    g, scene = S.thermal_plume_128()
+    params = dict(scheme="macCormack", cfl_target=1e30, dt_max=0.5 * g.h, solver="rbsor",
+                  pressure_iters=200, omega=1.9, tol=0.0, buoyancy=1.0, confinement=0.1,
+                  residual_every=0)
+    sim = Simulation(128, 128, params)
+    sim.reset(scene)
+    osim = O.OracleSim(128, 128)
+    osim.reset(g, scene)
+    prm = O.step_params(scheme="macCormack", solver="rbsor", pressure_iters=200, omega=1.9,
+                        tol=0.0, cfl_target=1e30, dt_max=0.5 * g.h, buoyancy=1.0,
+                        confinement=0.1, residual_every=0,
+                        patch=(osim.patch.i0, osim.patch.j0, osim.patch.nx, osim.patch.ny))
+    worst = (0.0, 0.0, 0.0)
+    for n in range(200):
+        _reseed(osim, sim)
+        sim.step()
+        osim.step(prm, osim.patch.data)
+        e = (rel_l2(sim.f.u, osim.u), rel_l2(sim.f.v, osim.v),
+             rel_l2(np.concatenate(sim.read_dye()), osim.dye))
+        worst = tuple(max(a, b) for a, b in zip(worst, e))
+        if n % 50 == 0:
+            print(f"step {n}: u {e[0]:.2e} v {e[1]:.2e} dye {e[2]:.2e}", flush=True)
+    print("worst", worst)
+    assert max(worst) < REL_L2, worst
+    assert sim.f.v.max() > 0.05  # the plume has risen
+
+
+@pytest.mark.parametrize("solver", ["mg-dc", "mg-fcg"])
+def test_karman_every_step_from_device_state(solver):
+    """Karman street 512x256, 20 steps at CFL 0.5 with every solve to 1e-6:
+    each device step against one oracle step (defect-correction multigrid)
+    from the device's fields.  The device's dt comes from its own max face
+    speed and the oracle's from the same values, so both take the same dt;
+    the two solves differ by at most their 1e-6 residuals."""
+    from flowlab_amd.simulation import Simulation
+    nx, ny = 512, 256
+    g, scene = S.karman_street(nx, ny)
+    sim = Simulation(nx, ny, dict(scheme="macCormack", cfl_target=0.5, solver=solver,
+                                  pressure_iters=60, tol=1e-6, residual_every=0))
+    sim.reset(scene)
+    osim = O.OracleSim(nx, ny)
+    osim.reset(g, scene)
+    worst = (0.0, 0.0)
+    for n in range(20):
+        _reseed(osim, sim)
+        prm = O.step_params(scheme="macCormack", solver="mg-dc", pressure_iters=60, tol=1e-6,
+                            cfl_target=0.5, residual_every=0)
+        sim.step()
+        osim.step(prm)
+        assert sim.dt == prm.dt_used, (n, sim.dt, prm.dt_used)
+        e = (rel_l2(sim.f.u, osim.u), rel_l2(sim.f.v, osim.v))
+        worst = tuple(max(a, b) for a, b in zip(worst, e))
+    print("worst", worst)
+    assert worst[0] < REL_L2 and worst[1] < REL_L2, worst
+
+
 def test_karman_whole_step_mg_dc():
     """Karman street at 512x256 with MG-DC to 1e-6: per-step parity with the
     oracle and every solve at 1e-6 by the oracle's own FP64 residual."""
