@@ -109,7 +109,16 @@ __device__ __forceinline__ float nan_max(float m, float x) {
 // velocity); elements outside a field are staged as 0.  speed (optional):
 // the same maximum with fmaxf's treatment of NaN (ignored), as the max-speed
 // reduction takes it.
-template <int NF, int WN>
+//
+// kClamped: every load unconditional, from the index clamped into the field,
+// an element outside replaced by 0 after the load -- no branch per element.
+// The batched kernels need it: their field pointers are read from the scene
+// descriptor, and with a branch per element the compiler there waited for
+// all loads in flight at almost every element (the batched MacCormack dye
+// correction ran 1.8x the single-grid kernel's time on the same scene; 1.07x
+// with kClamped).  The single-grid kernels keep the branches, which measured
+// 4-8 % faster for them at 8192^2.  Same values either way.
+template <int NF, int WN, bool kClamped = false>
 __device__ __forceinline__ float stage_windows(float (*lds)[WN], const Win (&f)[NF], int wx0,
                                                int wy0, int tid, float* speed = nullptr) {
   constexpr int kStage = (WN + 255) / 256;
@@ -120,10 +129,16 @@ __device__ __forceinline__ float stage_windows(float (*lds)[WN], const Win (&f)[
     const int b = e / WW, a = e - b * WW;
     const int i = wx0 + a, j = wy0 + b;
 #pragma unroll
-    for (int q = 0; q < NF; q++)
-      v[q][r] = (e < WN && i >= 0 && j >= 0 && i < f[q].w && j < f[q].h)
-                    ? f[q].p[i + j * f[q].w]
-                    : 0.f;
+    for (int q = 0; q < NF; q++) {
+      const bool in = e < WN && i >= 0 && j >= 0 && i < f[q].w && j < f[q].h;
+      if constexpr (kClamped) {
+        const int ci = min(max(i, 0), f[q].w - 1), cj = min(max(j, 0), f[q].h - 1);
+        const float x = f[q].p[ci + cj * f[q].w];
+        v[q][r] = in ? x : 0.f;
+      } else {
+        v[q][r] = in ? f[q].p[i + j * f[q].w] : 0.f;
+      }
+    }
   }
   float m = 0.f, sp = 0.f;
 #pragma unroll
@@ -201,7 +216,7 @@ __device__ __forceinline__ bool interior_tile(int x0, int y0, int nx, int ny) {
 
 // advect.ts:72-112 with carrier == source (the only use: u and v advect
 // themselves), so two windows.
-template <int TY>
+template <int TY, bool kBatch = false>
 __device__ __forceinline__ void advect_faces_tile(
     const float* __restrict__ uc, const float* __restrict__ vc, float* __restrict__ uo,
     float* __restrict__ vo, const uint8_t* __restrict__ lbl, int nx, int ny, float c,
@@ -211,7 +226,7 @@ __device__ __forceinline__ void advect_faces_tile(
   const int x0 = bx * TX, y0 = by * TY, wx0 = x0 - AH, wy0 = y0 - AH;
   const int tid = threadIdx.x + threadIdx.y * TX;
   const Win f[2] = {{uc, nx + 1, ny}, {vc, nx, ny + 1}};
-  const float m = stage_windows<2>(lds, f, wx0, wy0, tid);
+  const float m = stage_windows<2, Tile<TY>::WN, kBatch>(lds, f, wx0, wy0, tid);
   const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
   if (within_reach(m, c, tid)) {
     auto run = [&](auto tag) {
@@ -242,7 +257,7 @@ __global__ __launch_bounds__(256) void k_advect_faces_lds(
 }
 
 // advect.ts:177-210 (fused reverse pass and clamp, as k_maccormack_faces).
-template <int TY>
+template <int TY, bool kBatch = false>
 __device__ __forceinline__ void maccormack_faces_tile(
     const float* __restrict__ un, const float* __restrict__ vn, const float* __restrict__ uh,
     const float* __restrict__ vh, float* __restrict__ uo, float* __restrict__ vo,
@@ -259,7 +274,7 @@ __device__ __forceinline__ void maccormack_faces_tile(
   if (sten)
     for (int r = 0; r < TY / BY; r++)
       if (i <= nx && j + r * BY <= ny) st[r] = sten[i + (j + r * BY) * (nx + 1)];
-  const float m = stage_windows<4>(lds, f, wx0, wy0, tid);
+  const float m = stage_windows<4, Tile<TY>::WN, kBatch>(lds, f, wx0, wy0, tid);
   if (within_reach(m, c, tid)) {
     auto run = [&](auto tag) {
       using F = decltype(tag);
@@ -294,7 +309,7 @@ __global__ __launch_bounds__(256) void k_maccormack_faces_lds(
 
 // advect.ts:241-270, dye on the velocity grid (scale 1): windows of u, v and
 // the 3 channels.
-template <int TY>
+template <int TY, bool kBatch = false>
 __device__ __forceinline__ void advect_dye_tile(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ q,
     float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx, int ny, float c, float k0,
@@ -307,9 +322,18 @@ __device__ __forceinline__ void advect_dye_tile(
   const size_t plane = (size_t)nx * ny;
   const Win f[5] = {{u, nx + 1, ny}, {v, nx, ny + 1}, {q, nx, ny}, {q + plane, nx, ny},
                     {q + 2 * plane, nx, ny}};
-  float sp = 0.f;
-  const float m = stage_windows<5>(lds, f, wx0, wy0, tid, speed ? &sp : nullptr);
   const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
+  // batched: each row's solid test loaded with the windows, so it is issued
+  // before the first store even though the compiler cannot tell qo from lbl
+  // (a batch's scene pointers come from memory, without the kernel
+  // arguments' restrict); -1: the cell reads its label itself
+  int sol[TY / BY];
+#pragma unroll
+  for (int r = 0; r < TY / BY; r++)
+    sol[r] = !kBatch ? -1 : (i < nx && j + r * BY < ny) ? lbl[i + (j + r * BY) * nx] == SOLID : 1;
+  float sp = 0.f;
+  const float m =
+      stage_windows<5, Tile<TY>::WN, kBatch>(lds, f, wx0, wy0, tid, speed ? &sp : nullptr);
   const bool reach =
       speed ? within_reach_speed(m, sp, c, tid, speed, bid) : within_reach(m, c, tid);
   if (reach) {
@@ -319,7 +343,8 @@ __device__ __forceinline__ void advect_dye_tile(
               q0 = window_as<F>(lds, 2, wx0, wy0), q1 = window_as<F>(lds, 3, wx0, wy0),
               q2 = window_as<F>(lds, 4, wx0, wy0);
       for (int r = 0; r < TY / BY; r++)
-        advect_dye_t(a, b, q0, q1, q2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i, j + r * BY, sten);
+        advect_dye_t(a, b, q0, q1, q2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i, j + r * BY, sten,
+                     sol[r]);
     };
     if (interior_tile<TY>(x0, y0, nx, ny))
       run(LFieldI{});
@@ -345,7 +370,7 @@ __global__ __launch_bounds__(256) void k_advect_dye_lds(
 
 // advect.ts:291-311, dye on the velocity grid: windows of u, v, the 3
 // channels of q^n and of the forward pass.
-template <int TY>
+template <int TY, bool kBatch = false>
 __device__ __forceinline__ void maccormack_dye_tile(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ qn,
     const float* __restrict__ qh, float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx,
@@ -360,12 +385,17 @@ __device__ __forceinline__ void maccormack_dye_tile(
                     {qn + plane, nx, ny}, {qn + 2 * plane, nx, ny}, {qh, nx, ny},
                     {qh + plane, nx, ny}, {qh + 2 * plane, nx, ny}};
   const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
-  // the recorded limiter stencils, loaded with the windows
+  // the recorded limiter stencils, loaded with the windows, and (batched)
+  // each row's solid test, as in advect_dye_tile
   uint32_t st[TY / BY] = {};
-  if (sten)
-    for (int r = 0; r < TY / BY; r++)
-      if (i < nx && j + r * BY < ny) st[r] = sten[i + (j + r * BY) * nx];
-  const float m = stage_windows<8>(lds, f, wx0, wy0, tid);
+  int sol[TY / BY];
+#pragma unroll
+  for (int r = 0; r < TY / BY; r++) {
+    const bool in = i < nx && j + r * BY < ny;
+    if (sten && in) st[r] = sten[i + (j + r * BY) * nx];
+    sol[r] = !kBatch ? -1 : in ? lbl[i + (j + r * BY) * nx] == SOLID : 1;
+  }
+  const float m = stage_windows<8, Tile<TY>::WN, kBatch>(lds, f, wx0, wy0, tid);
   if (within_reach(m, c, tid)) {
     auto run = [&](auto tag) {
       using F = decltype(tag);
@@ -375,7 +405,7 @@ __device__ __forceinline__ void maccormack_dye_tile(
               h1 = window_as<F>(lds, 6, wx0, wy0), h2 = window_as<F>(lds, 7, wx0, wy0);
       for (int r = 0; r < TY / BY; r++)
         maccormack_dye_t(a, b, n0, n1, n2, h0, h1, h2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i,
-                         j + r * BY, sten != nullptr, st[r]);
+                         j + r * BY, sten != nullptr, st[r], sol[r]);
     };
     if (interior_tile<TY>(x0, y0, nx, ny))
       run(LFieldI{});
@@ -467,8 +497,9 @@ __global__ __launch_bounds__(256) void kb_advect_faces(const SceneDesc* __restri
   int s, bx, by, gy, bid;
   if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
   const SceneDesc& d = D[s];
-  advect_faces_tile<TY>(d.u[q], d.v[q], mac ? d.uh : d.u[q ^ 1], mac ? d.vh : d.v[q ^ 1], d.lbl,
-                        d.nx, d.ny, st[s].c, mac ? d.sten_f : nullptr, bx, by);
+  advect_faces_tile<TY, true>(gm(d.u[q]), gm(d.v[q]), mac ? gm(d.uh) : gm(d.u[q ^ 1]),
+                        mac ? gm(d.vh) : gm(d.v[q ^ 1]), gm(d.lbl), d.nx, d.ny, st[s].c,
+                        mac ? gm(d.sten_f) : nullptr, bx, by);
 }
 
 template <int TY>
@@ -482,8 +513,9 @@ __global__ __launch_bounds__(256) void kb_maccormack_faces(const SceneDesc* __re
   // write with the dye not yet advected, as the single-grid step fuses it
   const BStep& t = st[s];
   const bool buoy = t.eps == 0.f && t.gain != 0.f;
-  maccormack_faces_tile<TY>(d.u[q], d.v[q], d.uh, d.vh, d.u[q ^ 1], d.v[q ^ 1], d.lbl, d.nx, d.ny,
-                            t.c, d.sten_f, buoy ? d.dye[q] : nullptr, t.gain, bx, by);
+  maccormack_faces_tile<TY, true>(gm(d.u[q]), gm(d.v[q]), gm(d.uh), gm(d.vh), gm(d.u[q ^ 1]),
+                            gm(d.v[q ^ 1]), gm(d.lbl), d.nx, d.ny, t.c, gm(d.sten_f),
+                            buoy ? gm(d.dye[q]) : nullptr, t.gain, bx, by);
 }
 
 // The forward MacCormack pass keeps the dye (keep 1 at its write, as
@@ -496,9 +528,10 @@ __global__ __launch_bounds__(256) void kb_advect_dye(const SceneDesc* __restrict
   if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
   const SceneDesc& d = D[s];
   const BStep& t = st[s];
-  advect_dye_tile<TY>(d.u[q ^ 1], d.v[q ^ 1], d.dye[q], mac ? d.dyeh : d.dye[q ^ 1], d.lbl, d.nx,
-                      d.ny, t.c, mac ? 1.f : t.k0, mac ? 1.f : t.k1, mac ? 1.f : t.k2,
-                      mac ? d.sten_d : nullptr, bx, by);
+  advect_dye_tile<TY, true>(gm(d.u[q ^ 1]), gm(d.v[q ^ 1]), gm(d.dye[q]),
+                      mac ? gm(d.dyeh) : gm(d.dye[q ^ 1]), gm(d.lbl), d.nx, d.ny, t.c,
+                      mac ? 1.f : t.k0, mac ? 1.f : t.k1, mac ? 1.f : t.k2,
+                      mac ? gm(d.sten_d) : nullptr, bx, by);
 }
 
 template <int TY>
@@ -509,8 +542,9 @@ __global__ __launch_bounds__(256) void kb_maccormack_dye(const SceneDesc* __rest
   if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
   const SceneDesc& d = D[s];
   const BStep& t = st[s];
-  maccormack_dye_tile<TY>(d.u[q ^ 1], d.v[q ^ 1], d.dye[q], d.dyeh, d.dye[q ^ 1], d.lbl, d.nx,
-                          d.ny, t.c, t.k0, t.k1, t.k2, d.sten_d, bx, by);
+  maccormack_dye_tile<TY, true>(gm(d.u[q ^ 1]), gm(d.v[q ^ 1]), gm(d.dye[q]), gm(d.dyeh),
+                          gm(d.dye[q ^ 1]), gm(d.lbl), d.nx, d.ny, t.c, t.k0, t.k1, t.k2,
+                          gm(d.sten_d), bx, by);
 }
 
 dim3 batch_grid_advect(int which, int nx, int ny) {

@@ -18,6 +18,20 @@ namespace fh {
 
 enum : uint8_t { FLUID = 0, AIR = 1, SOLID = 2 };
 
+// A pointer read out of memory (a batch's scene descriptor, a struct of
+// levels) is generic to the compiler -- only kernel arguments are known to
+// be global memory -- so every access through it would be a flat
+// instruction, which waits on the LDS and the vector-memory counters both.
+// Every field buffer is global memory (device or mapped host), and gm()
+// tells the compiler so: the address re-enters as a global pointer through
+// an integer (a plain generic -> global -> generic cast pair is folded away
+// again), and the accesses derived from it become global instructions.
+// The conversion costs no instruction.
+template <class T>
+__device__ __forceinline__ T* gm(T* p) {
+  return (T*)(__attribute__((address_space(1))) T*)(uintptr_t)p;
+}
+
 // grid.ts:159-161: off-grid counts as solid inside a stencil.
 __device__ __forceinline__ bool solid_or_outside(const uint8_t* __restrict__ l, int nx, int ny,
                                                  int i, int j) {

@@ -939,16 +939,18 @@ __global__ __launch_bounds__(256) void kb_dc_update(const SceneDesc* __restrict_
   if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
   const SceneDesc& d = D[s];
   if (init == 2) {  // the initial pass with the divergence of velocity [q ^ 1]
-    dc_update_body<kRedRows, true>(d.p[q], nullptr, nullptr, nullptr, d.lv[0].info, d.lv[0].b,
-                                   d.rvec, d.nx, d.ny, st[s].scale, d.partials, nullptr, bx, by,
-                                   gy, bid, d.u[q ^ 1], d.v[q ^ 1], st[s].inv_h, d.div);
+    dc_update_body<kRedRows, true>(gm(d.p[q]), nullptr, nullptr, nullptr, gm(d.lv[0].info),
+                                   gm(d.lv[0].b), gm(d.rvec), d.nx, d.ny, st[s].scale,
+                                   gm(d.partials), nullptr, bx, by, gy, bid, gm(d.u[q ^ 1]),
+                                   gm(d.v[q ^ 1]), st[s].inv_h, gm(d.div));
     return;
   }
-  double* pn = init ? nullptr : d.p[q ^ 1];
-  const float* e = init ? nullptr : d.lv[0].x;
-  const int* done = init ? nullptr : d.done;
-  dc_update_body<kRedRows>(d.p[q], pn, e, d.div, d.lv[0].info, d.lv[0].b, d.rvec, d.nx, d.ny,
-                           st[s].scale, d.partials, done, bx, by, gy, bid);
+  double* pn = init ? nullptr : gm(d.p[q ^ 1]);
+  const float* e = init ? nullptr : gm(d.lv[0].x);
+  const int* done = init ? nullptr : gm(d.done);
+  dc_update_body<kRedRows>(gm(d.p[q]), pn, e, gm(d.div), gm(d.lv[0].info), gm(d.lv[0].b),
+                           gm(d.rvec), d.nx, d.ny, st[s].scale, gm(d.partials), done, bx, by, gy,
+                           bid);
 }
 
 __global__ __launch_bounds__(1024) void kb_finalize(const SceneDesc* __restrict__ D,
@@ -957,11 +959,11 @@ __global__ __launch_bounds__(1024) void kb_finalize(const SceneDesc* __restrict_
   const SceneDesc& d = D[blockIdx.x];
   const double scale = st[blockIdx.x].scale;
   if (mode == 0)
-    finalize_sum4_body(d.partials, d.red_blocks, d.sums, d.d_res + 16, d.done, scale, 0.0, 1,
-                       nullptr);
+    finalize_sum4_body(gm(d.partials), d.red_blocks, gm(d.sums), gm(d.d_res) + 16, gm(d.done),
+                       scale, 0.0, 1, nullptr);
   else
-    finalize_sum4_body(d.partials, d.red_blocks, d.sums, d.d_res + 8 * slot, d.done, scale, tol,
-                       0, mode == 2 ? d.sums : nullptr);
+    finalize_sum4_body(gm(d.partials), d.red_blocks, gm(d.sums), gm(d.d_res) + 8 * slot, gm(d.done),
+                       scale, tol, 0, mode == 2 ? gm(d.sums) : nullptr);
 }
 
 __global__ __launch_bounds__(256) void kb_fcg_dots(const SceneDesc* __restrict__ D, BatchGrid g,
@@ -969,13 +971,13 @@ __global__ __launch_bounds__(256) void kb_fcg_dots(const SceneDesc* __restrict__
   int s, bx, by, gy, bid;
   if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
   const SceneDesc& d = D[s];
-  fcg_dots_body<kRedRows>(d.lv[0].x, d.dvec[cp], d.rvec, d.lv[0].info, d.nx, d.ny, first,
-                          d.partials, d.done, bx, by, gy, bid);
+  fcg_dots_body<kRedRows>(gm(d.lv[0].x), gm(d.dvec[cp]), gm(d.rvec), gm(d.lv[0].info), d.nx, d.ny,
+                          first, gm(d.partials), gm(d.done), bx, by, gy, bid);
 }
 
 __global__ __launch_bounds__(1024) void kb_fcg_coef(const SceneDesc* __restrict__ D, int first) {
   const SceneDesc& d = D[blockIdx.x];
-  fcg_coef_body(d.partials, d.red_blocks, d.fcg, first, d.done);
+  fcg_coef_body(gm(d.partials), d.red_blocks, gm(d.fcg), first, gm(d.done));
 }
 
 __global__ __launch_bounds__(256) void kb_fcg_update(const SceneDesc* __restrict__ D,
@@ -983,9 +985,9 @@ __global__ __launch_bounds__(256) void kb_fcg_update(const SceneDesc* __restrict
   int s, bx, by, gy, bid;
   if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
   const SceneDesc& d = D[s];
-  fcg_update_body<kRedRows>(d.lv[0].x, d.dvec[cp], d.dvec[cp ^ 1], d.p[q ^ 1], d.rvec,
-                            d.lv[0].info, d.lv[0].b, d.fcg, d.nx, d.ny, first, d.partials, d.done,
-                            bx, by, gy, bid);
+  fcg_update_body<kRedRows>(gm(d.lv[0].x), gm(d.dvec[cp]), gm(d.dvec[cp ^ 1]), gm(d.p[q ^ 1]),
+                            gm(d.rvec), gm(d.lv[0].info), gm(d.lv[0].b), gm(d.fcg), d.nx, d.ny,
+                            first, gm(d.partials), gm(d.done), bx, by, gy, bid);
 }
 
 dim3 batch_grid_reduce(int nx, int ny) { return reduce_grid_rows(nx, ny); }

@@ -183,8 +183,8 @@ __global__ __launch_bounds__(W) void kb_smooth_restrict(const SceneDesc* __restr
   const SceneDesc& d = D[s];
   const LevelDesc& L = d.lv[l];
   const LevelDesc& N = d.lv[l + 1];
-  smooth_restrict_tile<TX, TY, W>(nullptr, L.xa, L.b, L.info, L.nx, L.ny, N.b, N.nx, N.ny, 1,
-                                  d.done, bx, by);
+  smooth_restrict_tile<TX, TY, W>(nullptr, gm(L.xa), gm(L.b), gm(L.info), L.nx, L.ny, gm(N.b),
+                                  N.nx, N.ny, 1, gm(d.done), bx, by);
 }
 
 template <int TX, int TY, int W>
@@ -195,16 +195,16 @@ __global__ __launch_bounds__(W) void kb_prolong_smooth(const SceneDesc* __restri
   const SceneDesc& d = D[s];
   const LevelDesc& L = d.lv[l];
   const LevelDesc& N = d.lv[l + 1];
-  prolong_smooth_tile<TX, TY, W>(L.xa, L.x, L.b, L.info, L.nx, L.ny, N.x, N.nx, N.ny, d.done, bx,
-                                 by);
+  prolong_smooth_tile<TX, TY, W>(gm(L.xa), gm(L.x), gm(L.b), gm(L.info), L.nx, L.ny, gm(N.x),
+                                 N.nx, N.ny, gm(d.done), bx, by);
 }
 
 __global__ __launch_bounds__(NC) void kb_coarse_vcycle(const SceneDesc* __restrict__ D) {
   const SceneDesc& d = D[blockIdx.x];
-  if (*d.done) return;
+  if (*gm(d.done)) return;
   extern __shared__ unsigned char smem[];
-  coarse_vcycle_body<NC>(d.cl, d.lv[d.lc].b, nullptr, d.lv[d.lc].x, d.M, d.cells, d.ncoarse, 1,
-                         smem);
+  coarse_vcycle_body<NC>(d.cl, gm(d.lv[d.lc].b), nullptr, gm(d.lv[d.lc].x), gm(d.M), gm(d.cells),
+                         d.ncoarse, 1, smem);
 }
 
 template <int TX, int TY, int W>

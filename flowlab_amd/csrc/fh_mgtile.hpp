@@ -492,7 +492,7 @@ __device__ __forceinline__ void coarse_vcycle_body(const CoarseLevels& L,
 #pragma unroll
       for (int k = 0; k < KI; k++) {
         const int c = tid + (pass * KI + k) * NC;
-        iv[k] = c < nchunks ? reinterpret_cast<const uint4*>(L.image)[c] : make_uint4(0, 0, 0, 0);
+        iv[k] = c < nchunks ? reinterpret_cast<const uint4*>(gm(L.image))[c] : make_uint4(0, 0, 0, 0);
       }
 #pragma unroll
       for (int k = 0; k < KC; k++) {

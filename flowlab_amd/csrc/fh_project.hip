@@ -454,8 +454,8 @@ __global__ __launch_bounds__(256) void kb_vorticity(const SceneDesc* __restrict_
   int s, bx, by, gy, bid;
   if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
   const SceneDesc& d = D[s];
-  vorticity_at(d.u[q ^ 1], d.v[q ^ 1], d.w, d.nx, d.ny, st[s].inv_hf, bx * BX + threadIdx.x,
-               by * BY + threadIdx.y);
+  vorticity_at(gm(d.u[q ^ 1]), gm(d.v[q ^ 1]), gm(d.w), d.nx, d.ny, st[s].inv_hf,
+               bx * BX + threadIdx.x, by * BY + threadIdx.y);
 }
 
 __global__ __launch_bounds__(256) void kb_forces(const SceneDesc* __restrict__ D, BatchGrid g,
@@ -465,8 +465,8 @@ __global__ __launch_bounds__(256) void kb_forces(const SceneDesc* __restrict__ D
   const SceneDesc& d = D[s];
   const BStep& t = st[s];
   const StirBox none{};
-  forces_at(d.u[q ^ 1], d.v[q ^ 1], d.lbl, d.dye[q], d.w, d.nx, d.ny, 1, t.hf, t.dtf, t.gain,
-            t.eps, none, bx * BX + threadIdx.x, by * BY + threadIdx.y);
+  forces_at(gm(d.u[q ^ 1]), gm(d.v[q ^ 1]), gm(d.lbl), gm(d.dye[q]), gm(d.w), d.nx, d.ny, 1, t.hf,
+            t.dtf, t.gain, t.eps, none, bx * BX + threadIdx.x, by * BY + threadIdx.y);
 }
 
 // The single-grid step skips buoyancy when buoyancy * dt rounds to 0.
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(256) void kb_buoyancy(const SceneDesc* __restrict__
   const SceneDesc& d = D[s];
   const float gain = st[s].gain;
   if (gain == 0.f) return;
-  buoyancy_rows(d.v[q ^ 1], d.lbl, d.dye[q], d.nx, d.ny, 1, gain, bx, by);
+  buoyancy_rows(gm(d.v[q ^ 1]), gm(d.lbl), gm(d.dye[q]), d.nx, d.ny, 1, gain, bx, by);
 }
 
 // Gradient subtraction with the pressure the solve left in p [q ^ 1].
@@ -486,8 +486,8 @@ __global__ __launch_bounds__(256) void kb_project(const SceneDesc* __restrict__ 
   int s, bx, by, gy, bid;
   if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
   const SceneDesc& d = D[s];
-  subtract_outflow_rows(d.u[q ^ 1], d.v[q ^ 1], d.p[q ^ 1], d.lbl, d.nx, d.ny, st[s].grad_scale,
-                        bx, by);
+  subtract_outflow_rows(gm(d.u[q ^ 1]), gm(d.v[q ^ 1]), gm(d.p[q ^ 1]), gm(d.lbl), d.nx, d.ny,
+                        st[s].grad_scale, bx, by);
 }
 
 // kBatchSpeedBlocks blocks per scene stride over its u and v faces (the
@@ -517,8 +517,8 @@ __global__ __launch_bounds__(256) void kb_max_speed(const SceneDesc* __restrict_
     if (k < n4) m = fmaxf(m, m4(a4[k]));
     for (long t = (n4 << 2) + t0; t < n; t += stride) m = fmaxf(m, fabsf(a[t]));
   };
-  scan(d.u[q ^ 1], (long)(d.nx + 1) * d.ny);
-  scan(d.v[q ^ 1], (long)d.nx * (d.ny + 1));
+  scan(gm(d.u[q ^ 1]), (long)(d.nx + 1) * d.ny);
+  scan(gm(d.v[q ^ 1]), (long)d.nx * (d.ny + 1));
   block_max_store_at(m, partials + (size_t)s * kBatchSpeedBlocks, bid);
 }
 

@@ -193,18 +193,20 @@ __device__ __forceinline__ Pos dye_pos(int i, int ds) {
 // q0..q2 are the channels as accessors over the dye grid (dnx x dny); the
 // output planes are qo, qo + plane, qo + 2 plane.  sten (may be null): one
 // byte per cell, the origin of the stencil MacCormack's limiter clamps to
-// (maccormack_dye_t's forward trace; 0 for solid cells).
+// (maccormack_dye_t's forward trace; 0 for solid cells).  solid: the cell's
+// solid_or_outside test when the caller has already loaded it (-1: read lbl).
 template <class CU, class CV, class Q>
 __device__ __forceinline__ void advect_dye_t(const CU& u, const CV& v, const Q& q0, const Q& q1,
                                              const Q& q2, float* __restrict__ qo,
                                              const uint8_t* __restrict__ lbl, int nx, int ny,
                                              int ds, float c, float k0, float k1, float k2, int i,
-                                             int j, uint8_t* __restrict__ sten = nullptr) {
+                                             int j, uint8_t* __restrict__ sten = nullptr,
+                                             int solid = -1) {
   int dnx = nx * ds, dny = ny * ds;
   if (i >= dnx || j >= dny) return;
   size_t plane = (size_t)dnx * dny;
   int k = i + j * dnx;
-  if (solid_or_outside(lbl, nx, ny, i / ds, j / ds)) {
+  if (solid >= 0 ? solid != 0 : solid_or_outside(lbl, nx, ny, i / ds, j / ds)) {
     qo[k] = q0.at(i, j) * k0;
     qo[k + plane] = q1.at(i, j) * k1;
     qo[k + 2 * plane] = q2.at(i, j) * k2;
@@ -244,7 +246,8 @@ __device__ __forceinline__ void advect_dye_at(const float* __restrict__ u,
 
 // advect.ts:291-311 for three channels, fused as for the faces; out *= keep.
 // has_sten: st is this cell's byte of the forward pass's clamp stencil
-// origins (advect_dye_t), used instead of tracing forward again.
+// origins (advect_dye_t), used instead of tracing forward again.  solid as
+// for advect_dye_t.
 template <class CU, class CV, class Q>
 __device__ __forceinline__ void maccormack_dye_t(const CU& u, const CV& v, const Q& n0,
                                                  const Q& n1, const Q& n2, const Q& h0,
@@ -252,12 +255,12 @@ __device__ __forceinline__ void maccormack_dye_t(const CU& u, const CV& v, const
                                                  const uint8_t* __restrict__ lbl, int nx, int ny,
                                                  int ds, float c, float k0, float k1, float k2,
                                                  int i, int j, bool has_sten = false,
-                                                 uint32_t st = 0) {
+                                                 uint32_t st = 0, int solid = -1) {
   int dnx = nx * ds, dny = ny * ds;
   if (i >= dnx || j >= dny) return;
   size_t plane = (size_t)dnx * dny;
   int k = i + j * dnx;
-  if (solid_or_outside(lbl, nx, ny, i / ds, j / ds)) {
+  if (solid >= 0 ? solid != 0 : solid_or_outside(lbl, nx, ny, i / ds, j / ds)) {
     qo[k] = n0.at(i, j) * k0;
     qo[k + plane] = n1.at(i, j) * k1;
     qo[k + 2 * plane] = n2.at(i, j) * k2;
