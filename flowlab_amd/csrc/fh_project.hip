@@ -197,8 +197,11 @@ __device__ __forceinline__ void subtract_outflow_rows(float* __restrict__ u, flo
     const int j = min(j0 + r, ny), k = iw + min(j, ny - 1) * nx;
     pw[r] = p[k];
     lw[r] = lbl[k];
-    u0[r] = (j < ny && i >= 1 && i <= nx - 1) ? u[i + j * su] : 0.f;
-    v0[r] = (i < nx && j >= 1 && j <= ny - 1) ? v[i + j * nx] : 0.f;
+    // the faces from indices clamped onto the interior faces, loaded
+    // unconditionally: a branch per load made each one wait for the last
+    // (the value is only used on the interior face itself)
+    u0[r] = u[min(max(i, 1), nx - 1) + min(j, ny - 1) * su];
+    v0[r] = v[min(i, nx - 1) + min(max(j, 1), ny - 1) * nx];
   }
 #pragma unroll
   for (int r = 0; r < kRows; r++) {
