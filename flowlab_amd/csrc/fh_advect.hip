@@ -226,14 +226,18 @@ __device__ __forceinline__ void advect_faces_tile(
   const int x0 = bx * TX, y0 = by * TY, wx0 = x0 - AH, wy0 = y0 - AH;
   const int tid = threadIdx.x + threadIdx.y * TX;
   const Win f[2] = {{uc, nx + 1, ny}, {vc, nx, ny + 1}};
-  const float m = stage_windows<2, Tile<TY>::WN, kBatch>(lds, f, wx0, wy0, tid);
   const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
+  // every row's face solid tests, loaded with the windows
+  int cl[TY / BY];
+#pragma unroll
+  for (int r = 0; r < TY / BY; r++) cl[r] = closed_faces(lbl, nx, ny, i, j + r * BY);
+  const float m = stage_windows<2, Tile<TY>::WN, kBatch>(lds, f, wx0, wy0, tid);
   if (within_reach(m, c, tid)) {
     auto run = [&](auto tag) {
       using F = decltype(tag);
       const F u = window_as<F>(lds, 0, wx0, wy0), v = window_as<F>(lds, 1, wx0, wy0);
       for (int r = 0; r < TY / BY; r++)
-        advect_faces_t(u, v, u, v, uo, vo, lbl, nx, ny, c, i, j + r * BY, sten);
+        advect_faces_t(u, v, u, v, uo, vo, lbl, nx, ny, c, i, j + r * BY, sten, cl[r]);
     };
     if (interior_tile<TY>(x0, y0, nx, ny))
       run(LFieldI{});
@@ -269,11 +273,15 @@ __device__ __forceinline__ void maccormack_faces_tile(
   const int tid = threadIdx.x + threadIdx.y * TX;
   const Win f[4] = {{un, nx + 1, ny}, {vn, nx, ny + 1}, {uh, nx + 1, ny}, {vh, nx, ny + 1}};
   const int i = x0 + threadIdx.x, j = y0 + threadIdx.y;
-  // the recorded limiter stencils, loaded with the windows
+  // the recorded limiter stencils and every row's face solid tests, loaded
+  // with the windows
   uint32_t st[TY / BY] = {};
-  if (sten)
-    for (int r = 0; r < TY / BY; r++)
-      if (i <= nx && j + r * BY <= ny) st[r] = sten[i + (j + r * BY) * (nx + 1)];
+  int cl[TY / BY];
+#pragma unroll
+  for (int r = 0; r < TY / BY; r++) {
+    if (sten && i <= nx && j + r * BY <= ny) st[r] = sten[i + (j + r * BY) * (nx + 1)];
+    cl[r] = closed_faces(lbl, nx, ny, i, j + r * BY);
+  }
   const float m = stage_windows<4, Tile<TY>::WN, kBatch>(lds, f, wx0, wy0, tid);
   if (within_reach(m, c, tid)) {
     auto run = [&](auto tag) {
@@ -282,7 +290,7 @@ __device__ __forceinline__ void maccormack_faces_tile(
               ah = window_as<F>(lds, 2, wx0, wy0), bh = window_as<F>(lds, 3, wx0, wy0);
       for (int r = 0; r < TY / BY; r++)
         maccormack_faces_t(a, b, ah, bh, uo, vo, lbl, nx, ny, c, i, j + r * BY, sten != nullptr,
-                           st[r], heat, gain);
+                           st[r], heat, gain, cl[r]);
     };
     if (interior_tile<TY>(x0, y0, nx, ny))
       run(LFieldI{});

@@ -33,23 +33,43 @@ __device__ __forceinline__ uint32_t sten2(int d0, int d1) {
 __device__ __forceinline__ int sten_d0(uint32_t s) { return (int)(s & 3) - 2; }
 __device__ __forceinline__ int sten_d1(uint32_t s) { return (int)((s >> 2) & 3) - 2; }
 
+// The solid tests of thread index (i, j)'s faces (bit 0: u face (i, j),
+// solid_or_outside at (i-1, j) or (i, j); bit 1: v face, at (i, j-1) or
+// (i, j)) from three label loads issued unconditionally, at indices clamped
+// into the grid: tested one by one behind their short-circuit branches,
+// each label load waited for the one before.
+__device__ __forceinline__ int closed_faces(const uint8_t* __restrict__ lbl, int nx, int ny, int i,
+                                            int j) {
+  auto at = [&](int a, int b) {
+    const uint8_t l = lbl[min(max(a, 0), nx - 1) + min(max(b, 0), ny - 1) * nx];
+    return a < 0 || b < 0 || a >= nx || b >= ny || l == SOLID;
+  };
+  const bool c = at(i, j), w = at(i - 1, j), s = at(i, j - 1);
+  return (int)(w || c) | ((int)(s || c) << 1);
+}
+
 // advect.ts:72-112 -- one thread index covers u(i,j) when j<ny and v(i,j)
 // when i<nx, over (nx+1) x (ny+1).  uc/vc carry, us/vs are advected.  sten
 // (may be null): the stencil origins of both faces' samples, one byte per
 // thread index (u face in bits 0-3, v face in bits 4-7; 0 for solid faces).
+// closed: the two faces' solid tests when the caller has loaded them
+// (closed_faces; -1: read lbl here).
 template <class CU, class CV, class SU, class SV>
 __device__ __forceinline__ void advect_faces_t(const CU& uc, const CV& vc, const SU& us,
                                                const SV& vs, float* __restrict__ uo,
                                                float* __restrict__ vo,
                                                const uint8_t* __restrict__ lbl, int nx, int ny,
                                                float c, int i, int j,
-                                               uint8_t* __restrict__ sten = nullptr) {
+                                               uint8_t* __restrict__ sten = nullptr,
+                                               int closed = -1) {
   if (i > nx || j > ny) return;
   Pos bx, by;
   uint32_t st = 0;
   if (j < ny) {
     int k = i + j * (nx + 1);
-    if (solid_or_outside(lbl, nx, ny, i - 1, j) || solid_or_outside(lbl, nx, ny, i, j)) {
+    if (closed >= 0 ? (closed & 1) != 0
+                    : solid_or_outside(lbl, nx, ny, i - 1, j) ||
+                          solid_or_outside(lbl, nx, ny, i, j)) {
       uo[k] = us.at(i, j);
     } else {
       Pos X{i, 0.f}, Y{j, 0.5f};
@@ -65,7 +85,9 @@ __device__ __forceinline__ void advect_faces_t(const CU& uc, const CV& vc, const
   }
   if (i < nx) {
     int k = i + j * nx;
-    if (solid_or_outside(lbl, nx, ny, i, j - 1) || solid_or_outside(lbl, nx, ny, i, j)) {
+    if (closed >= 0 ? (closed & 2) != 0
+                    : solid_or_outside(lbl, nx, ny, i, j - 1) ||
+                          solid_or_outside(lbl, nx, ny, i, j)) {
       vo[k] = vs.at(i, j);
     } else {
       Pos X{i, 0.5f}, Y{j, 0.f};
@@ -120,12 +142,14 @@ __device__ __forceinline__ void maccormack_faces_t(const NU& un, const NV& vn, c
                                                    float c, int i, int j, bool has_sten = false,
                                                    uint32_t st = 0,
                                                    const float* __restrict__ heat = nullptr,
-                                                   float gain = 0.f) {
+                                                   float gain = 0.f, int closed = -1) {
   if (i > nx || j > ny) return;
   Pos rx, ry, fx, fy;
   if (j < ny) {
     int k = i + j * (nx + 1);
-    if (solid_or_outside(lbl, nx, ny, i - 1, j) || solid_or_outside(lbl, nx, ny, i, j)) {
+    if (closed >= 0 ? (closed & 1) != 0
+                    : solid_or_outside(lbl, nx, ny, i - 1, j) ||
+                          solid_or_outside(lbl, nx, ny, i, j)) {
       uo[k] = un.at(i, j);
     } else {
       Pos X{i, 0.f}, Y{j, 0.5f};
@@ -148,7 +172,9 @@ __device__ __forceinline__ void maccormack_faces_t(const NU& un, const NV& vn, c
   }
   if (i < nx) {
     int k = i + j * nx;
-    if (solid_or_outside(lbl, nx, ny, i, j - 1) || solid_or_outside(lbl, nx, ny, i, j)) {
+    if (closed >= 0 ? (closed & 2) != 0
+                    : solid_or_outside(lbl, nx, ny, i, j - 1) ||
+                          solid_or_outside(lbl, nx, ny, i, j)) {
       vo[k] = vn.at(i, j);
     } else {
       Pos X{i, 0.5f}, Y{j, 0.f};
