@@ -81,12 +81,18 @@ SOLVER_NAMES = {
     "mg-fcg": "one MG-DC V(2,2) cycle (to 16x8), then flexible CG preconditioned by it",
 }
 PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_plume_8192.json")
+# start of each step's solve: the previous pressure (the reference's), or the
+# last two solutions extrapolated linearly in time (fh_set_warm_extrapolation)
+WARM_STARTS = {"previous": (0, "the previous pressure"),
+               "extrapolated": (1, "the last two solutions extrapolated linearly in time")}
 
 
-def algo_bytes(kernel, solver, buoyancy=True):
+def algo_bytes(kernel, solver, buoyancy=True, warm="previous"):
     """Algorithmic bytes per cell of one traced launch of `kernel` (buoyancy:
     the workload's forces are buoyancy alone, which the MacCormack face pass
-    applies, reading the heat, +4)."""
+    applies, reading the heat, +4; warm "extrapolated": the solve's initial
+    pass also reads the previous solution and writes the start it forms,
+    +16)."""
     b = ALGO_BYTES_PER_CELL.get(kernel)
     if kernel == "maccormack_faces" and buoyancy:
         b += 4.0
@@ -94,6 +100,8 @@ def algo_bytes(kernel, solver, buoyancy=True):
         b += 8.0
     if kernel == "divergence" and solver in ("mg-dc", "mg-fcg"):
         b = 29.0 + (8.0 if solver == "mg-fcg" else 0.0)
+        if warm == "extrapolated":
+            b += 16.0
     return b
 
 
@@ -108,6 +116,8 @@ def parse():
                     help="untimed steps after the reset, before the timed ones")
     ap.add_argument("--n", type=int, default=8192, help="side of the headline plume")
     ap.add_argument("--solver", default="mg-fcg", choices=sorted(SOLVER_NAMES))
+    ap.add_argument("--warm", default="extrapolated", choices=sorted(WARM_STARTS),
+                    help="start of each step's pressure solve (headline and C2)")
     ap.add_argument("--kernel", default=None,
                     help="kernel traced for the roofline (default: the largest device time "
                          "per step)")
@@ -216,7 +226,7 @@ def pmc_summary(args):
         fv, wv = F[(kn, g)], W.get((kn, g), [0.0])
         fb = sum(fv) / len(fv) * 1024 * sf
         wb = sum(wv) / len(wv) * 1024 * sw
-        a = algo_bytes(key, args.solver) * cells
+        a = algo_bytes(key, args.solver, warm=args.warm) * cells
         out["kernels"][key] = {"rocprof_name": short(kn), "dispatches": len(fv),
                                "fetch_bytes": fb, "write_bytes": wb, "traffic_bytes": fb + wb,
                                "algorithmic_bytes": a, "ratio": (fb + wb) / a}
@@ -270,7 +280,7 @@ def instrumented(sim, scene, steps, warmup=0):
                 phases={k: v / steps for k, v in phases.items()})
 
 
-def kernel_table(sim, cells, solver, steps=2, buoyancy=False):
+def kernel_table(sim, cells, solver, steps=2, buoyancy=False, warm="previous"):
     """Each kernel of LARGE_KERNELS bracketed by events over `steps` steps:
     mean launch ms, launches per step and algorithmic GB/s."""
     e = sim.stepper.engine
@@ -284,7 +294,7 @@ def kernel_table(sim, cells, solver, steps=2, buoyancy=False):
         if not n:
             continue
         avg = ms / n
-        bpc = algo_bytes(k, solver, buoyancy)
+        bpc = algo_bytes(k, solver, buoyancy, warm)
         gbs = bpc * cells / (avg * 1e-3) / 1e9
         out[k] = {"avg_launch_ms": avg, "launches_per_step": n / steps,
                   "ms_per_step": ms / steps, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
@@ -298,13 +308,15 @@ def run_headline(args):
     log(f"headline: plume {n}^2 scene")
     g, scene = S.large_plume(n)
     params = dict(scheme="macCormack", cfl_target=0.5, dt_max=0.5 * g.h, solver=args.solver,
-                  pressure_iters=100, tol=1e-6, residual_every=0, buoyancy=1.0)
+                  pressure_iters=100, tol=1e-6, residual_every=0, buoyancy=1.0,
+                  warm_extrapolation=WARM_STARTS[args.warm][0])
     sim = make_sim(n, n, scene, params)
     e = sim.stepper.engine
     sim.step()  # uploads the fields and builds the multigrid hierarchy
     tiles = e.tune_tiles(reps=5)["tiles"]
     log(f"headline: tiles {tiles}; per-kernel table")
-    kernels = kernel_table(sim, n * n, args.solver, buoyancy=True) if args.table else {}
+    kernels = (kernel_table(sim, n * n, args.solver, buoyancy=True, warm=args.warm)
+               if args.table else {})
     dominant = args.kernel or (max(kernels, key=lambda k: kernels[k]["ms_per_step"])
                                if kernels else "maccormack_dye")
     # timed pass: reset and upload (the fields are resident before the
@@ -326,6 +338,7 @@ def run_headline(args):
     e.trace_kernel(None)
     log("headline: the 200-step workload, instrumented")
     wl = instrumented(sim, scene, WORKLOAD_STEPS if args.workload_run else args.steps)
+    other = other_start(sim, scene, len(wl["iters"]), wl, args.warm)
     state = None
     if args.cpu_baseline:
         u, v = e.read_velocity()
@@ -337,12 +350,29 @@ def run_headline(args):
     sim.stepper.engine.close()
     return dict(n=n, elapsed=elapsed, steps=args.steps, tiles=tiles, kernels=kernels,
                 dominant=dominant, n_launch=n_launch, k_ms=k_ms, workload=wl,
-                learned=learned, state=state)
+                learned=learned, state=state, other_start=other)
+
+
+def other_start(sim, scene, steps, wl, warm):
+    """The same steps from the same reset with the other start of the solve
+    (previous pressure against the extrapolation): ms and V-cycles per solve."""
+    e = sim.stepper.engine
+    name = "previous" if warm == "extrapolated" else "extrapolated"
+    e.set_warm_extrapolation(WARM_STARTS[name][0])
+    r = instrumented(sim, scene, steps)
+    e.set_warm_extrapolation(WARM_STARTS[warm][0])
+    K = len(r["iters"])
+    ms = sum(r["solve_ms"]) / K
+    return {"start": WARM_STARTS[name][1], "steps": K, "ms_per_solve": ms,
+            "vcycles_per_solve": sum(r["iters"]) / K, "max_relative_residual": r["max_res"],
+            "ms_per_step_with_phase_timing": r["elapsed"] * 1e3 / K,
+            "solves_per_s_of_the_run_vs_this": ms / (sum(wl["solve_ms"]) / len(wl["iters"]))}
 
 
 def run_learned(sim, scene, steps, plain):
     """The same steps from the same reset, every pressure solved from the
-    trained U-Net warm start (then the same solver to 1e-6)."""
+    trained U-Net warm start applied to the run's own start (then the same
+    solver to 1e-6)."""
     from flowlab_amd import learned as L
     if not os.path.exists(L.WEIGHTS):
         return None
@@ -380,12 +410,12 @@ def run_c2(args):
     nx, ny, K = 2048, 1024, args.c2_steps
     g, scene = S.karman_street(nx, ny)
     params = dict(scheme="macCormack", cfl_target=0.5, solver=args.solver, pressure_iters=100,
-                  tol=1e-6, residual_every=0)
+                  tol=1e-6, residual_every=0, warm_extrapolation=WARM_STARTS[args.warm][0])
     sim = make_sim(nx, ny, scene, params)
     e = sim.stepper.engine
     sim.step()
     tiles = e.tune_tiles(reps=10)["tiles"]
-    kernels = kernel_table(sim, nx * ny, args.solver, buoyancy=False)
+    kernels = kernel_table(sim, nx * ny, args.solver, buoyancy=False, warm=args.warm)
     sim.reset(scene)
     sim.stepper.upload(sim.f)
     e.synchronize()
@@ -397,7 +427,9 @@ def run_c2(args):
     wl = instrumented(sim, scene, K)
     out = leg_summary(nx * ny, K, elapsed, wl, tiles)
     out["workload"] = ("karman vortex street 2048x1024, MacCormack, "
-                       f"{SOLVER_NAMES[args.solver]}, tol 1e-6, CFL 0.5, {K} steps")
+                       f"{SOLVER_NAMES[args.solver]}, tol 1e-6, CFL 0.5, {K} steps, each solve "
+                       f"from {WARM_STARTS[args.warm][1]}")
+    out["other_start"] = other_start(sim, scene, K, wl, args.warm)
     out["kernels"] = kernels
     out["note"] = "working set inside the 256 MB Infinity Cache: GB/s are cache-effective"
     if "learned" in args.legs:
@@ -649,7 +681,7 @@ def main():
         log("leg C2 through Node")
         out_legs["C2_karman_2048x1024_node_boundary"] = run_node_c2(args)
     dom = h["dominant"]
-    bpc = algo_bytes(dom, args.solver)
+    bpc = algo_bytes(dom, args.solver, warm=args.warm)
     k_avg_ms = h["k_ms"] / max(h["n_launch"], 1)
     bytes_per_launch = bpc * cells
     achieved = bytes_per_launch / (k_avg_ms * 1e-3) / 1e9 if h["n_launch"] else None
@@ -671,7 +703,9 @@ def main():
         "config": {
             "workload": f"thermal plume {n}x{n}, closed box, 200 circular obstacles, 16 "
                         f"buoyant sources, buoyancy 1, MacCormack, {SOLVER_NAMES[args.solver]}, "
-                        "tol 1e-6, dt 0.5 h capped at CFL 0.5, SplitMix64(20240613)",
+                        "tol 1e-6, dt 0.5 h capped at CFL 0.5, SplitMix64(20240613), each "
+                        f"solve from {WARM_STARTS[args.warm][1]}",
+            "warm_start": args.warm,
             "grid": [n, n],
             "workload_steps": WORKLOAD_STEPS,
             "timed_steps": f"{args.warmup + 1}..{args.warmup + K} after the reset",
@@ -684,6 +718,8 @@ def main():
             "vcycles_per_solve": wsum["vcycles_per_solve"],
             "max_relative_residual": wsum["max_relative_residual"],
             "over": f"all {WORKLOAD_STEPS} solves of the workload",
+            "start": WARM_STARTS[args.warm][1],
+            "other_start": h["other_start"],
             "learned_warm_start": h["learned"],
         },
         "workload_run": wsum,

@@ -479,31 +479,102 @@ double rel_resid(const double* s, double scale, bool scaled_r) {
   return rr / std::sqrt(s[2] / s[3]);
 }
 
+// Forgets the pressure history of the extrapolated warm start: the resident
+// p is no longer the last fh_step solve's result.
+void drop_pressure_history(fh_engine* e) {
+  e->p_dt = 0.0;
+  e->ph_n = 0;
+}
+
+// Lagrange weights of the extrapolation to tau after the newest pressure:
+// nodes 0 (p), -a (h0, a = dt of the step that produced p) and -(a + b)
+// (h1, b = dt of the step that produced h0); order = nodes used - 1.
+void extrapolation_weights(int order, double tau, double a, double b, double w[3]) {
+  w[0] = 1.0;
+  w[1] = w[2] = 0.0;
+  if (order == 1) {
+    w[0] = 1.0 + tau / a;
+    w[1] = -tau / a;
+  } else if (order == 2) {
+    w[0] = (tau + a) * (tau + a + b) / (a * (a + b));
+    w[1] = -tau * (tau + a + b) / (a * b);
+    w[2] = tau * (tau + a) / (b * (a + b));
+  }
+}
+
+// The start of an fh_step solve of step size dt: with the extrapolation on
+// and the resident p the last step's solution, the start is p extrapolated
+// over the history (order: the entries there are, at most the set order; 0
+// while the history is empty), formed into p2 by the solve's first pass.
+struct StartPlan {
+  bool on = false;
+  int order = 0;
+  Extrapolation xp{};
+};
+
+StartPlan plan_start(fh_engine* e, double dt) {
+  StartPlan sp;
+  if (e->extrap <= 0 || !(dt > 0.0) || !(e->p_dt > 0.0)) return sp;
+  sp.on = true;
+  sp.order = std::min(e->ph_n, e->extrap);
+  double w[3];
+  extrapolation_weights(sp.order, dt, e->p_dt, e->ph_dt[0], w);
+  sp.xp = Extrapolation{e->ph[0], e->ph[1], w[0], w[1], w[2]};
+  return sp;
+}
+
+// After the start is in p2: it becomes p, the old p (the last solution)
+// becomes the newest history entry, and the buffer of the entry that falls
+// out of the history becomes p2.
+void rotate_history(fh_engine* e) {
+  double* last = e->p;
+  e->p = e->p2;
+  if (e->extrap >= 2) {
+    e->p2 = e->ph[1];
+    e->ph[1] = e->ph[0];
+    e->ph_dt[1] = e->ph_dt[0];
+  } else {
+    e->p2 = e->ph[0];
+  }
+  e->ph[0] = last;
+  e->ph_dt[0] = e->p_dt;
+  e->ph_n = std::min(e->ph_n + 1, e->extrap);
+}
+
 // PressureSolver.solve on the resident p (FP64), div (FP64), label.
 // div_fused: the divergence has not been formed; the initial defect pass of
-// the FP64 solvers forms it from the resident velocity.
+// the FP64 solvers forms it from the resident velocity.  sp: the start.
 int solve_impl(fh_engine* e, int solver, double scale, int iterations, double omega,
-               double tol, int* out_iters, bool div_fused);
+               double tol, int* out_iters, bool div_fused, const StartPlan& sp);
 
 // Every solve is bracketed by two events: fh_last_solve_ms reports its
 // device time (label refresh, warm start and all cycles included).
+// step_dt > 0: the solve of an fh_step of that step size (its start may be
+// extrapolated in time, and its result joins the pressure history); 0: any
+// other solve, which ends the history.
 int solve(fh_engine* e, int solver, double scale, int iterations, double omega, double tol,
-          int* out_iters, bool div_fused = false) {
+          int* out_iters, bool div_fused = false, double step_dt = 0.0) {
   e->solve_ev_valid = false;
   if (!e->solve_ev[0]) {
     HIPCHK(hipEventCreate(&e->solve_ev[0]));
     HIPCHK(hipEventCreate(&e->solve_ev[1]));
   }
   HIPCHK(hipEventRecord(e->solve_ev[0], e->s));
-  int st = solve_impl(e, solver, scale, iterations, omega, tol, out_iters, div_fused);
-  if (st != FH_OK) return st;
+  const StartPlan sp = plan_start(e, step_dt);
+  if (!sp.on) e->ph_n = 0;
+  int st = solve_impl(e, solver, scale, iterations, omega, tol, out_iters, div_fused, sp);
+  if (st != FH_OK) {
+    drop_pressure_history(e);
+    return st;
+  }
+  e->p_dt = step_dt;
   HIPCHK(hipEventRecord(e->solve_ev[1], e->s));
   e->solve_ev_valid = true;
   return FH_OK;
 }
 
 int solve_impl(fh_engine* e, int solver, double scale, int iterations, double omega,
-               double tol, int* out_iters, bool div_fused) {
+               double tol, int* out_iters, bool div_fused, const StartPlan& sp) {
   int N = e->nx * e->ny;
   if (e->labels_dirty) {
     int st = refresh_levels(e);
@@ -511,7 +582,17 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
   }
   e->hist.clear();
   Level& L0 = e->lv[0];
-  if (solver == FH_SOLVER_MG_DC || solver == FH_SOLVER_MG_FCG) {
+  const bool fp64 = solver == FH_SOLVER_MG_DC || solver == FH_SOLVER_MG_FCG;
+  const bool sync_init = fp64 && (e->capture || e->warm);
+  // The extrapolated start: formed by the FP64 solvers' fused initial pass,
+  // otherwise by a pass of its own before anything reads p.
+  const bool start_in_pass = sp.on && fp64 && div_fused && !sync_init;
+  if (sp.on && !start_in_pass) {
+    launch_extrapolate(e->s, e->p, sp.xp, sp.order, e->p2, N);
+    LAUNCHED("extrapolate");
+    rotate_history(e);
+  }
+  if (fp64) {
     const bool fcg = solver == FH_SOLVER_MG_FCG;
     if (fcg) {
       int st0 = ensure_fcg(e);
@@ -522,7 +603,6 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
     // pressureMultigrid.ts:294-300 reads the same pass: on the device (the
     // norm sets `done`, so the cycles enqueued behind it skip themselves)
     // unless a capture or learned warm start needs the norm on the host now.
-    const bool sync_init = e->capture || e->warm;
     int st;
     if (sync_init) {
       launch_set_int(e->s, e->done, 0);
@@ -558,9 +638,13 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
       e->hist.push_back(rel_resid(e->h_sums, scale, true));
     } else {
       if (div_fused) {
-        Trace t(e, FH_K_DIVERGENCE);
-        launch_div_defect(e->s, e->u, e->v, 1.0 / e->h, e->div, e->p, L0.info, L0.b, r64, e->nx,
-                          e->ny, scale, e->partials);
+        {
+          Trace t(e, FH_K_DIVERGENCE);
+          launch_div_defect(e->s, e->u, e->v, 1.0 / e->h, e->div, e->p, L0.info, L0.b, r64,
+                            e->nx, e->ny, scale, e->partials, start_in_pass ? e->p2 : nullptr,
+                            &sp.xp, sp.order);
+        }
+        if (start_in_pass) rotate_history(e);
       } else {
         launch_dc_correct(e->s, e->p, nullptr, nullptr, e->div, L0.info, L0.b, e->nx, e->ny,
                           scale, e->partials, nullptr, r64);
@@ -986,7 +1070,8 @@ void fh_destroy(fh_engine* e) {
                     e->patch, e->fpartials, e->fmax, e->dvec[0], e->dvec[1]};
   for (float* p : fbufs)
     if (p) (void)hipFree(p);
-  double* dbufs[] = {e->p, e->p2, e->div, e->M, e->partials, e->sums, e->rvec, e->fcg};
+  double* dbufs[] = {e->p,   e->p2,  e->div,   e->M,    e->partials,
+                     e->sums, e->rvec, e->fcg, e->ph[0], e->ph[1]};
   for (double* p : dbufs)
     if (p) (void)hipFree(p);
   if (e->lbl) (void)hipFree(e->lbl);
@@ -1027,6 +1112,7 @@ int fh_upload(fh_engine* e, const double* u, const double* v, const double* p,
   GUARD("HipStepper.upload()");
   int st;
   e->vel_epoch++;
+  drop_pressure_history(e);
   if (u && (st = up32(e, e->u, u, n_u(e)))) return st;
   if (v && (st = up32(e, e->v, v, n_v(e)))) return st;
   if (dye && (st = up32(e, e->dye, dye, 3 * n_d(e)))) return st;
@@ -1047,6 +1133,7 @@ int fh_set_labels(fh_engine* e, const uint8_t* label) {
     set_err("setLabels: null label");
     return FH_ERR_BAD_ARG;
   }
+  drop_pressure_history(e);
   HIPCHK(hipMemcpy(e->lbl, label, n_c(e), hipMemcpyHostToDevice));
   launch_commit_labels(e->s, e->p, e->u, e->v, e->dye, e->lbl, e->nx, e->ny, (int)n_d(e));
   e->vel_epoch++;
@@ -1159,7 +1246,7 @@ int fh_step(fh_engine* e, int scheme, double dt, double scale, double grad_scale
     LAUNCHED("divergence");
   }
   if ((st = mark(e, 3))) return st;
-  if ((st = solve(e, solver, scale, iterations, omega, tol, out_iters, fuse))) return st;
+  if ((st = solve(e, solver, scale, iterations, omega, tol, out_iters, fuse, dt))) return st;
   if ((st = mark(e, 4))) return st;
   if ((st = project(e, grad_scale))) return st;
   // The next step's CFL reads this velocity (simulation.ts:474-477); the dye
@@ -1373,6 +1460,20 @@ int fh_set_warm_start(fh_engine* e, fh_unet* u) {
   GUARD("setWarmStart()");
   if (u && !e->wstats) HIPCHK(hipMalloc((void**)&e->wstats, 64));
   e->warm = u;
+  return FH_OK;
+}
+
+int fh_set_warm_extrapolation(fh_engine* e, int order) {
+  GUARD("setWarmExtrapolation()");
+  if (order < 0 || order > 2) {
+    set_err("setWarmExtrapolation: order must be 0, 1 or 2");
+    return FH_ERR_BAD_ARG;
+  }
+  int st;
+  for (int k = 0; k < order; k++)
+    if (!e->ph[k] && (st = dalloc(e, &e->ph[k], n_c(e)))) return st;
+  if (order != e->extrap) e->ph_n = 0;
+  e->extrap = order;
   return FH_OK;
 }
 

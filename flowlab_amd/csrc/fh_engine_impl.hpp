@@ -116,6 +116,16 @@ struct fh_engine {
   fh_unet* warm = nullptr;
   double* psave = nullptr;
   float* wstats = nullptr;
+  // warm start extrapolated in time (fh_set_warm_extrapolation): its order
+  // (0: the previous pressure, the reference's start), the pressures earlier
+  // fh_step solves ended with (ph[0] the newest) and how many are valid, the
+  // dt of the step that produced each, and the dt of the step whose solve
+  // left the resident p (0: p is not such a solution, history unusable)
+  int extrap = 0;
+  double* ph[2] = {nullptr, nullptr};
+  int ph_n = 0;
+  double ph_dt[2] = {0.0, 0.0};
+  double p_dt = 0.0;
   // device time of the last pressure solve
   hipEvent_t solve_ev[2] = {};
   bool solve_ev_valid = false;

@@ -87,6 +87,14 @@ void launch_finalize_max(hipStream_t s, const float* partials, int n, float* out
                          float* host_out);
 
 // ---- pressure (fh_mg.hip) --------------------------------------------------
+// Warm start extrapolated in time from the previous solves' pressures:
+// start = w0 p + w1 h0 + w2 h1 (h0 the newest earlier solution; terms beyond
+// the order unread), formed as fma(w2, h1, fma(w1, h0, w0 p)).
+struct Extrapolation {
+  const double* h0;
+  const double* h1;
+  double w0, w1, w2;
+};
 // Reductions over more than kFoldAbove blocks fold their partials with
 // kFoldBlocks workgroups before the one-block finalize; every partials
 // buffer holds 4 * kFoldBlocks doubles beyond its producers' 4 per block.
@@ -137,9 +145,13 @@ void launch_dc_correct(hipStream_t s, const double* po, double* pn, const float*
 // launch_dc_correct's initial pass (no e, no pn, done unread) with the
 // divergence (launch_divergence's values, stored to div) formed in the same
 // pass from the faces u, v.
+// pn non-null: the pass starts from po extrapolated at `order` over *xp
+// (order 0: po itself) and stores that start to pn (which differs from po
+// and xp's history).
 void launch_div_defect(hipStream_t s, const float* u, const float* v, double inv_h, double* div,
                        const double* po, const uint8_t* info, float* r32, double* r64, int nx,
-                       int ny, double scale, double* partials);
+                       int ny, double scale, double* partials, double* pn = nullptr,
+                       const Extrapolation* xp = nullptr, int order = 0);
 void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                     double* host_sums, int* done, double scale, double tol);
 // The norm of the pass before the first cycle: runs whatever *done holds and
@@ -168,6 +180,11 @@ void launch_fcg_norm(hipStream_t s, const double* partials, int nx, int ny, doub
 void launch_norms(hipStream_t s, const float* p32, const double* p64, const double* div,
                   const uint8_t* lbl, int nx, int ny, double scale, double* partials,
                   double* sums);
+// out = p extrapolated at the given order over n cells (order 0: a copy of
+// p).  Every array holds an even number of doubles, 16-byte aligned, and out
+// differs from the others.
+void launch_extrapolate(hipStream_t s, const double* p, const Extrapolation& xp, int order,
+                        double* out, long n);
 void launch_set_int(hipStream_t s, int* p, int v);
 
 // ---- fused multigrid (fh_mgfused.hip) ----------------------------------------

@@ -244,6 +244,27 @@ __global__ void k_f32_to_f64(const float* __restrict__ a, double* __restrict__ o
   if (k < n) out[k] = (double)a[k];
 }
 
+// Warm start extrapolated in time on its own pass (the solvers whose
+// initial pass is not dc_update_body's): out = p extrapolated over the
+// history (order 0: a copy of p).  Two cells per thread as double2; every
+// array holds an even number of doubles (the engine pads).
+template <int X>
+__global__ __launch_bounds__(256) void k_extrapolate(const double* __restrict__ p,
+                                                    Extrapolation xp, double* __restrict__ out,
+                                                    long n2) {
+  const long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (k >= n2) return;
+  double2 t = reinterpret_cast<const double2*>(p)[k];
+  if constexpr (X >= 1) {
+    const double2 a = reinterpret_cast<const double2*>(xp.h0)[k];
+    double2 b = make_double2(0.0, 0.0);
+    if constexpr (X >= 2) b = reinterpret_cast<const double2*>(xp.h1)[k];
+    t.x = extrap_value<X>(xp, t.x, a.x, b.x);
+    t.y = extrap_value<X>(xp, t.y, a.y, b.y);
+  }
+  reinterpret_cast<double2*>(out)[k] = t;
+}
+
 // Reduction grids are capped at kMaxPartials blocks (grid-stride over rows
 // beyond that); the one-block finalize reads up to kMaxPartials/1024
 // partials per thread, 8 in flight at a time.  At 8192^2 the cap leaves each
@@ -314,6 +335,17 @@ __device__ __forceinline__ void load_col5(Col5<R, T>& o, const T* __restrict__ a
   }
 }
 
+// One value of the extrapolated start: w0 t + w1 a (+ w2 b for X = 2), with
+// its multiply-adds written out so every pass that forms it gives the same
+// bits.
+template <int X>
+__device__ __forceinline__ double extrap_value(const Extrapolation& xp, double t, double a,
+                                               double b) {
+  double r = __fma_rn(xp.w1, a, xp.w0 * t);
+  if constexpr (X >= 2) r = __fma_rn(xp.w2, b, r);
+  return r;
+}
+
 // FP64 defect-correction update, one thread per cell:
 //   pn = po + e  (e = 0 off the fluid),  r = b - (count*pn - sum pn),
 //   b = -scale*div,  r32 = (float) r  (the next V-cycle's right-hand side).
@@ -324,17 +356,20 @@ __device__ __forceinline__ void load_col5(Col5<R, T>& o, const T* __restrict__ a
 // clamped in-domain indices, and closed neighbours add 0 (sum order +x, -x,
 // +y, -y, as in the reference's residual).
 //
-// kDiv: the solve's initial pass (no e, no pn) forming the divergence itself
-// from the faces u, v (div_value, as k_divergence does) and storing it to
-// div_out, instead of reading a divergence a separate pass wrote.
-template <int R, bool kDiv = false>
+// kDiv: the solve's initial pass (no e) forming the divergence itself from
+// the faces u, v (div_value, as k_divergence does) and storing it to div_out,
+// instead of reading a divergence a separate pass wrote.
+// X >= 1 (kDiv only): the pass starts from po extrapolated in time over the
+// history xp.h0 (and xp.h1 for X = 2), extrap_value at every stencil point,
+// and stores that start to pn.
+template <int R, bool kDiv = false, int X = 0>
 __device__ __forceinline__ void dc_update_body(
     const double* __restrict__ po, double* __restrict__ pn, const float* __restrict__ e,
     const double* __restrict__ div, const uint8_t* __restrict__ info, float* __restrict__ r32,
     double* __restrict__ r64, int nx, int ny, double scale, double* partials,
     const int* __restrict__ done, int bx, int by, int gy, int bid,
     const float* __restrict__ u = nullptr, const float* __restrict__ v = nullptr,
-    double inv_h = 0.0, double* __restrict__ div_out = nullptr) {
+    double inv_h = 0.0, double* __restrict__ div_out = nullptr, Extrapolation xp = {}) {
   if (done && *done) return;
   const int i = bx * BX + threadIdx.x;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -344,6 +379,19 @@ __device__ __forceinline__ void dc_update_body(
     load_col5(P, po, i, j0, nx, ny);
     Col5<R, float> E;
     if (e) load_col5(E, e, i, j0, nx, ny);
+    if constexpr (X >= 1) {
+      Col5<R, double> H0, H1;
+      load_col5(H0, xp.h0, i, j0, nx, ny);
+      if constexpr (X >= 2) load_col5(H1, xp.h1, i, j0, nx, ny);
+#pragma unroll
+      for (int q = 0; q < R + 2; q++)
+        P.c[q] = extrap_value<X>(xp, P.c[q], H0.c[q], X >= 2 ? H1.c[q] : 0.0);
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        P.e[r] = extrap_value<X>(xp, P.e[r], H0.e[r], X >= 2 ? H1.e[r] : 0.0);
+        P.w[r] = extrap_value<X>(xp, P.w[r], H0.w[r], X >= 2 ? H1.w[r] : 0.0);
+      }
+    }
     int fr[R];
     double dr[R];
 #pragma unroll
@@ -410,16 +458,19 @@ __global__ __launch_bounds__(256) void k_dc_update(
 // The solve's initial defect pass with the divergence formed in the same
 // pass (dc_update_body<R, true>): the divergence kernel and this pass's read
 // of div become one launch.
-template <int R>
+// X >= 1: from po extrapolated over xp's history, the start stored to pn
+// (pn non-null with X = 0: po copied there).
+template <int R, int X = 0>
 __global__ __launch_bounds__(256) void k_div_defect(
     const float* __restrict__ u, const float* __restrict__ v, double inv_h, double* __restrict__ div,
     const double* __restrict__ po, const uint8_t* __restrict__ info, float* __restrict__ r32,
-    double* __restrict__ r64, int nx, int ny, double scale, double* partials) {
+    double* __restrict__ r64, int nx, int ny, double scale, double* partials,
+    double* __restrict__ pn, Extrapolation xp) {
   int bx, by;
   xcd_block(&bx, &by);
-  dc_update_body<R, true>(po, nullptr, nullptr, nullptr, info, r32, r64, nx, ny, scale, partials,
-                          nullptr, bx, by, gridDim.y, blockIdx.x + blockIdx.y * gridDim.x, u, v,
-                          inv_h, div);
+  dc_update_body<R, true, X>(po, pn, nullptr, nullptr, info, r32, r64, nx, ny, scale, partials,
+                             nullptr, bx, by, gridDim.y, blockIdx.x + blockIdx.y * gridDim.x, u,
+                             v, inv_h, div, xp);
 }
 
 // ---- flexible conjugate gradients (FCG(1)) around the V-cycle -------------
@@ -853,6 +904,18 @@ void launch_f32_to_f64(hipStream_t s, const float* a, double* out, int n) {
   k_f32_to_f64<<<(n + 255) / 256, 256, 0, s>>>(a, out, n);
 }
 
+void launch_extrapolate(hipStream_t s, const double* p, const Extrapolation& xp, int order,
+                        double* out, long n) {
+  const long n2 = (n + 1) / 2;
+  const unsigned g = (unsigned)((n2 + 255) / 256);
+  if (order <= 0)
+    k_extrapolate<0><<<g, 256, 0, s>>>(p, xp, out, n2);
+  else if (order == 1)
+    k_extrapolate<1><<<g, 256, 0, s>>>(p, xp, out, n2);
+  else
+    k_extrapolate<2><<<g, 256, 0, s>>>(p, xp, out, n2);
+}
+
 void launch_dc_update(hipStream_t s, const double* po, double* pn, const float* e,
                       const double* div, const uint8_t* info, float* r32, int nx, int ny,
                       double scale, double* partials, double* sums, double* host_sums, int* done,
@@ -870,9 +933,19 @@ void launch_dc_correct(hipStream_t s, const double* po, double* pn, const float*
 
 void launch_div_defect(hipStream_t s, const float* u, const float* v, double inv_h, double* div,
                        const double* po, const uint8_t* info, float* r32, double* r64, int nx,
-                       int ny, double scale, double* partials) {
-  k_div_defect<kRedRows><<<reduce_grid_rows(nx, ny), dim3(BX, BY), 0, s>>>(
-      u, v, inv_h, div, po, info, r32, r64, nx, ny, scale, partials);
+                       int ny, double scale, double* partials, double* pn,
+                       const Extrapolation* xp, int order) {
+  const dim3 g = reduce_grid_rows(nx, ny), b(BX, BY);
+  const Extrapolation x = xp ? *xp : Extrapolation{};
+  if (!pn || order <= 0)
+    k_div_defect<kRedRows, 0><<<g, b, 0, s>>>(u, v, inv_h, div, po, info, r32, r64, nx, ny,
+                                               scale, partials, pn, x);
+  else if (order == 1)
+    k_div_defect<kRedRows, 1><<<g, b, 0, s>>>(u, v, inv_h, div, po, info, r32, r64, nx, ny,
+                                               scale, partials, pn, x);
+  else
+    k_div_defect<kRedRows, 2><<<g, b, 0, s>>>(u, v, inv_h, div, po, info, r32, r64, nx, ny,
+                                               scale, partials, pn, x);
 }
 
 void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,

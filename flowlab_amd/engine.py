@@ -178,6 +178,11 @@ class HipEngine:
         """Learned warm start for defect-correction solves (None = p_prev)."""
         check(self.lib.fh_set_warm_start(self._e, None if unet is None else unet._u))
 
+    def set_warm_extrapolation(self, order: int):
+        """Start each step's solve from the previous solutions extrapolated
+        in time (order 1 linear, 2 quadratic; 0 = the previous pressure)."""
+        check(self.lib.fh_set_warm_extrapolation(self._e, int(order)))
+
     def last_solve_ms(self) -> float:
         ms = C.c_double()
         check(self.lib.fh_last_solve_ms(self._e, C.byref(ms)))

@@ -35,6 +35,9 @@ class SimulationParams:
     residual_every: int = 1
     confinement: float = 0.0
     solver: str = "mg-dc"
+    # build-defined: start each solve from the previous solutions
+    # extrapolated in time (1 linear, 2 quadratic); 0 = the previous pressure
+    warm_extrapolation: int = 0
 
 
 def optimal_omega(nx: int, ny: Optional[int] = None) -> float:
@@ -125,6 +128,8 @@ class Simulation:
         self.stepper = HipStepper(nx, ny, s, self.g.h, solver=prm.solver, readback=readback)
         if timing:
             self.stepper.engine.set_timing(True)
+        if prm.warm_extrapolation:
+            self.stepper.engine.set_warm_extrapolation(prm.warm_extrapolation)
         self._steps = 0
         self._dye_patch: Optional[DyePatch] = None
 

@@ -269,6 +269,17 @@ int fh_set_capture(fh_engine* e, fh_unet* u);
 int fh_last_solve_ms(fh_engine* e, double* ms);
 /* NULL = start from the previous pressure (the reference's warm start). */
 int fh_set_warm_start(fh_engine* e, fh_unet* u);
+/* Warm start of fh_step's pressure solve extrapolated in time (no
+ * counterpart in the reference, which starts from the previous pressure,
+ * stepGpu.ts:19-24): order 1 (linear) or 2 (quadratic) Lagrange
+ * extrapolation, at the step's own dt, of the pressures the last order + 1
+ * fh_step solves ended with; 0 (the default) starts from the previous
+ * pressure.  The history restarts after fh_upload, fh_set_labels, any solve
+ * outside fh_step, or a failed step, and the first steps after a restart use
+ * what history there is.  The learned warm start (fh_set_warm_start), when
+ * set, corrects the extrapolated pressure.  The solve still runs to its
+ * tolerance, so only the iteration count depends on the start. */
+int fh_set_warm_extrapolation(fh_engine* e, int order);
 
 /* ---- Batched scenes ------------------------------------------------------
  * No counterpart in the reference, which steps one grid per Simulation
