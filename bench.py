@@ -509,11 +509,10 @@ def run_c3(args):
 
 
 def run_c4(args):
-    """The learned warm start against the zero and previous-pressure starts
-    at 1024^2 and 2048^2 (3 seeded scenes each, the next step's solve to
-    1e-6): V-cycles and device ms per solve.  `network` is the learned start
-    applied to a zero pressure (the network's own gain over the zero start);
-    `learned` applies it to the previous pressure."""
+    """The learned warm start at 1024^2 and 2048^2 (3 seeded scenes each, the
+    next step's solve to 1e-6), V-cycles and device ms per solve from six
+    starts: zero, previous pressure, linear extrapolation in time, and the
+    network on each (`network`, `learned`, `learned_extrapolated`)."""
     from flowlab_amd import learned as L
     if not os.path.exists(L.WEIGHTS):
         return None
@@ -522,9 +521,12 @@ def run_c4(args):
     out = {}
     for n in (1024, 2048):
         r = L.evaluate(u, n, scenes=3)
-        r["learned_solves_per_s_vs_previous"] = r["previous"]["ms"] / r["learned"]["ms"]
-        r["learned_solves_per_s_vs_zero"] = r["zero"]["ms"] / r["learned"]["ms"]
-        r["network_vcycles_vs_zero"] = r["network"]["vcycles"] / r["zero"]["vcycles"]
+        for net, base in (("network", "zero"), ("learned", "previous"),
+                          ("learned_extrapolated", "extrapolated")):
+            r[f"{net}_vcycles_vs_{base}"] = r[net]["vcycles"] / r[base]["vcycles"]
+            r[f"{net}_solves_per_s_vs_{base}"] = r[base]["ms"] / r[net]["ms"]
+        r["learned_extrapolated_solves_per_s_vs_previous"] = (
+            r["previous"]["ms"] / r["learned_extrapolated"]["ms"])
         out[f"{n}x{n}"] = r
     return out
 

@@ -748,7 +748,7 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
     }
     *out_iters = used;
     if (e->capture) {
-      st = engine_capture_hook(e->capture, e->s, nullptr, nullptr, e->psave, e->p, N, e->ny,
+      st = engine_capture_hook(e->capture, e->s, nullptr, e->lbl, e->psave, e->p, N, e->ny,
                                e->nx, false);
       if (st != FH_OK) return st;
     }

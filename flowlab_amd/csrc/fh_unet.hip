@@ -3,14 +3,19 @@
 // captured from the engine's own multigrid solves, and the warm-start step
 // used inside the defect-correction solve.
 //
-// Formulation.  Given the previous pressure p_prev and the new right-hand
-// side b, the network sees the defect r0 = b - A p_prev (normalised by its
-// RMS s_r over the fluid), the solid and Air masks, and p_prev (normalised
-// by its RMS s_p), and predicts the correction o; the warm start is
-//   p0 = p_prev + s_r * c * o      on fluid cells,
-// c = c0 * (H W / area0) scaling the trained magnitude with the grid area.
-// Training minimises the mean squared error of o against
-// (p* - p_prev) / (s_r c) over fluid cells, p* being the multigrid solution.
+// Formulation.  The solve's start p (the previous pressure, or its
+// extrapolation in time) leaves the defect r0 = b - A p.  The fine grid is
+// gathered into F x F blocks (F a power of two, coarse sides at most 128):
+// per block the sum of r0 (the coarse right-hand side: a smooth p with
+// A p = r on the fine grid has A_c p_c = sum of r over the block on the
+// coarse one), its solid and Air fractions and the fluid mean of p.  The
+// network sees those four channels (the defect over its RMS s_r, the
+// pressure over its RMS s_p) and predicts o, the coarse correction over
+// s_r c0; the correction q = s_r c0 o is interpolated bilinearly onto the
+// fine fluid cells and the start becomes p + m q with the energy-optimal
+// step m >= 0 (k_warm_dots), so it never has a larger A-norm error than p.
+// Training minimises the mean squared error of o against the fluid mean of
+// (p* - p) over each block, over s_r c0, p* being the multigrid solution.
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -31,75 +36,134 @@ __device__ __forceinline__ bf16_t f2bf_(float f) {
   return (bf16_t)(u >> 16);
 }
 
-// Per-grid statistics for the normalisation: {sum r^2, sum p^2, n} over
-// fluid cells, two-phase (block partials, then one block).
-template <class PT>
-__global__ __launch_bounds__(256) void k_stats(const float* __restrict__ r,
-                                               const PT* __restrict__ p,
-                                               const uint8_t* __restrict__ lbl, long n,
-                                               double* partials) {
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x) {
-    if (lbl[i] != FLUID) continue;
-    double rv = r[i], pv = (double)p[i];
-    acc[0] += rv * rv;
-    acc[1] += pv * pv;
-    acc[2] += 1.0;
-  }
-  block_sum_store<4>(acc, partials);
+// ---- the coarse grid the network runs on -------------------------------------
+// Each coarse cell gathers an F x F block of fine cells as float4
+// {sum of the defect r, solid cells, Air cells, sum of p over fluid cells}
+// (fluid cells = F^2 - solid - Air), built by repeated 2 x 2 block sums.
+
+// First level of the gather from the fine fields (G = 1: each fine cell as
+// it is; G = 2: 2 x 2 blocks).  With p1 non-null the pressure summed is
+// p1 - p (the captured correction), r unread.
+template <int G>
+__global__ void k_gather_first(const float* __restrict__ r, const uint8_t* __restrict__ lbl,
+                               const double* __restrict__ p, const double* __restrict__ p1, int nx,
+                               int cw, int ch, float4* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= cw || j >= ch) return;
+  float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int dy = 0; dy < G; dy++)
+#pragma unroll
+    for (int dx = 0; dx < G; dx++) {
+      const long k = (long)(G * i + dx) + (long)(G * j + dy) * nx;
+      const uint8_t l = lbl[k];
+      if (!p1) a.x += r[k];
+      a.y += l == SOLID ? 1.f : 0.f;
+      a.z += l == AIR ? 1.f : 0.f;
+      if (l == FLUID) a.w += p1 ? (float)(p1[k] - p[k]) : (float)p[k];
+    }
+  out[i + (long)j * cw] = a;
 }
 
-__global__ void k_stats_finalize(const double* __restrict__ partials, int nblocks,
-                                 float* __restrict__ out2) {
-  __shared__ double sh[3][4];
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  for (int s = 0; s < 3; s++) {
-    double t = 0.0;
-    for (int b = tid; b < nblocks; b += 256) t += partials[(size_t)b * 4 + s];
-    t = wave_sum(t);
-    if (lane == 0) sh[s][wid] = t;
+// One more 2 x 2 level: out (cw x ch) from in (2 cw x 2 ch).
+__global__ void k_gather2(const float4* __restrict__ in, int cw, int ch, float4* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= cw || j >= ch) return;
+  const long w = 2L * cw, k = 2L * i + 2L * j * w;
+  const float4 a = in[k], b = in[k + 1], c = in[k + w], d = in[k + w + 1];
+  out[i + (long)j * cw] = make_float4((a.x + b.x) + (c.x + d.x), (a.y + b.y) + (c.y + d.y),
+                                      (a.z + b.z) + (c.z + d.z), (a.w + b.w) + (c.w + d.w));
+}
+
+// The network's input scales of one coarse grid: s_r, s_p = RMS of the
+// defect sum and of the fluid-mean pressure over coarse cells holding fluid
+// (1 where zero).  One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_coarse_stats(const float4* __restrict__ c, int n,
+                                                       float area, float* __restrict__ out2) {
+  __shared__ double sh[3][16];
+  double a[3] = {0.0, 0.0, 0.0};
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const float4 v = c[k];
+    const float fl = area - v.y - v.z;
+    if (fl < 0.5f) continue;
+    const double pm = (double)v.w / fl;
+    a[0] += (double)v.x * v.x;
+    a[1] += pm * pm;
+    a[2] += 1.0;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int q = 0; q < 3; q++) {
+    const double t = wave_sum(a[q]);
+    if (lane == 0) sh[q][wid] = t;
   }
   __syncthreads();
-  if (tid == 0) {
-    double r2 = ((sh[0][0] + sh[0][1]) + sh[0][2]) + sh[0][3];
-    double p2 = ((sh[1][0] + sh[1][1]) + sh[1][2]) + sh[1][3];
-    double nn = ((sh[2][0] + sh[2][1]) + sh[2][2]) + sh[2][3];
-    double sr = nn > 0 ? sqrt(r2 / nn) : 0.0;
-    double sp = nn > 0 ? sqrt(p2 / nn) : 0.0;
+  if (threadIdx.x == 0) {
+    double t[3] = {0.0, 0.0, 0.0};
+    for (int q = 0; q < 3; q++)
+      for (int w = 0; w < 16; w++) t[q] += sh[q][w];
+    const double sr = t[2] > 0 ? sqrt(t[0] / t[2]) : 0.0, sp = t[2] > 0 ? sqrt(t[1] / t[2]) : 0.0;
     out2[0] = (float)(sr > 0 ? sr : 1.0);
     out2[1] = (float)(sp > 0 ? sp : 1.0);
   }
 }
 
-// Network input for sample b of a batch: channels {r/s_r, solid, air,
-// p_prev/s_p, 0, 0, 0, 0}; optional target (p*-p_prev)/(s_r c) and mask.
-template <class PT>
-__global__ void k_prep(const float* __restrict__ r, const PT* __restrict__ pp,
-                       const uint8_t* __restrict__ lbl, const float* __restrict__ dlt,
-                       const float* __restrict__ stats, const int* __restrict__ idx, long hw,
-                       int H, int W, float c, bf16_t* __restrict__ in, float* __restrict__ tgt,
-                       float* __restrict__ mask) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  int b = blockIdx.y;
-  if (i >= hw) return;
-  long slot = idx ? idx[b] : 0;
-  long src = slot * hw + i;
-  float sr = stats[2 * slot], sp = stats[2 * slot + 1];
-  uint8_t l = lbl[src];
-  float c0 = (l == FLUID) ? r[src] / sr : 0.f;
-  float c3 = (float)((double)pp[src] / sp);
-  uint4 v;
-  v.x = (uint32_t)f2bf_(c0) | ((uint32_t)f2bf_(l == SOLID ? 1.f : 0.f) << 16);
-  v.y = (uint32_t)f2bf_(l == AIR ? 1.f : 0.f) | ((uint32_t)f2bf_(c3) << 16);
-  v.z = 0;
-  v.w = 0;
-  long o = (long)b * hw + i;
-  *reinterpret_cast<uint4*>(in + o * kUnetIn) = v;
-  if (tgt) {
-    tgt[o] = (l == FLUID) ? dlt[src] / (sr * c) : 0.f;
-    mask[o] = (l == FLUID) ? 1.f : 0.f;
+// Normalised network input of a coarse grid: {r / s_r, solid fraction, Air
+// fraction, fluid-mean p / s_p}; written as the float4 sample x (capture)
+// and/or the bf16 input tensor (8 channels, 4 used).
+__global__ void k_coarse_input(const float4* __restrict__ c, int n, float area,
+                               const float* __restrict__ stats2, float4* __restrict__ x,
+                               bf16_t* __restrict__ in) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float4 v = c[k];
+  const float fl = area - v.y - v.z;
+  const float inv = 1.f / area;
+  float4 o;
+  o.x = v.x / stats2[0];
+  o.y = v.y * inv;
+  o.z = v.z * inv;
+  o.w = fl >= 0.5f ? (v.w / fl) / stats2[1] : 0.f;
+  if (x) x[k] = o;
+  if (in) {
+    uint4 u;
+    u.x = (uint32_t)f2bf_(o.x) | ((uint32_t)f2bf_(o.y) << 16);
+    u.y = (uint32_t)f2bf_(o.z) | ((uint32_t)f2bf_(o.w) << 16);
+    u.z = 0;
+    u.w = 0;
+    *reinterpret_cast<uint4*>(in + (long)k * kUnetIn) = u;
   }
+}
+
+// Captured target: the fluid-mean correction of each coarse cell over s_r.
+__global__ void k_coarse_target(const float4* __restrict__ c, int n, float area,
+                                const float* __restrict__ stats2, float* __restrict__ t) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const float4 v = c[k];
+  const float fl = area - v.y - v.z;
+  t[k] = fl >= 0.5f ? (v.w / fl) / stats2[0] : 0.f;
+}
+
+// Training batch b: the bf16 input from stored samples x, the target
+// t / c0 and the mask of coarse cells holding fluid.
+__global__ void k_train_prep(const float4* __restrict__ x, const float* __restrict__ t,
+                             const int* __restrict__ idx, long hw, float inv_c0,
+                             bf16_t* __restrict__ in, float* __restrict__ tgt,
+                             float* __restrict__ mask) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (i >= hw) return;
+  const long src = (long)idx[b] * hw + i, o = (long)b * hw + i;
+  const float4 v = x[src];
+  uint4 u;
+  u.x = (uint32_t)f2bf_(v.x) | ((uint32_t)f2bf_(v.y) << 16);
+  u.y = (uint32_t)f2bf_(v.z) | ((uint32_t)f2bf_(v.w) << 16);
+  u.z = 0;
+  u.w = 0;
+  *reinterpret_cast<uint4*>(in + o * kUnetIn) = u;
+  const bool fluid = 1.f - v.y - v.z > 1e-6f;
+  tgt[o] = fluid ? t[src] * inv_c0 : 0.f;
+  mask[o] = fluid ? 1.f : 0.f;
 }
 
 // Masked MSE: partial sums {sum m (o-t)^2, sum m}.
@@ -144,19 +208,42 @@ __global__ void k_head_grad(const float* __restrict__ dout, long n, bf16_t* __re
   dhead[i * 16] = f2bf_(dout[i]);
 }
 
-// The warm start is a line search along the network output o: with the
-// defect r0 = b - A p_prev and q = A o (o counted as 0 off the fluid, where
-// the correction is 0), the step m = <r0, q> / <q, q> (at least 0) minimises
-// |b - A (p_prev + m o)| over fluid cells, so the warm-started residual is
-// never larger than the previous pressure's, and the output scale drops out.
-// Blocks of k_warm_dots: 16 per CU, so the pass streams at HBM rate (512
-// left it at ~7 % of the HBM peak at 2048x1024).
+// The network's correction on the fine grid: q = s_r c0 o at the cell's
+// centre, o bilinear between coarse cell centres (clamped at the edges), on
+// fluid cells; 0 elsewhere.
+__global__ void k_prolong_out(const float* __restrict__ o, int cw, int ch, int F,
+                              const uint8_t* __restrict__ info, int nx, int ny,
+                              const float* __restrict__ stats2, float c0, float* __restrict__ q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  if (i >= nx || j >= ny) return;
+  const long k = i + (long)j * nx;
+  if (!(info[k] & 0x80)) {
+    q[k] = 0.f;
+    return;
+  }
+  const float invF = 1.f / (float)F;
+  const float x = fminf(fmaxf(((float)i + 0.5f) * invF - 0.5f, 0.f), (float)(cw - 1));
+  const float y = fminf(fmaxf(((float)j + 0.5f) * invF - 0.5f, 0.f), (float)(ch - 1));
+  const int x0 = min((int)x, cw - 1), y0 = min((int)y, ch - 1);
+  const int x1 = min(x0 + 1, cw - 1), y1 = min(y0 + 1, ch - 1);
+  const float fx = x - (float)x0, fy = y - (float)y0;
+  const float top = o[x0 + y0 * cw] + fx * (o[x1 + y0 * cw] - o[x0 + y0 * cw]);
+  const float bot = o[x0 + y1 * cw] + fx * (o[x1 + y1 * cw] - o[x0 + y1 * cw]);
+  q[k] = stats2[0] * c0 * (top + fy * (bot - top));
+}
+
+// The warm start is p + m q with the step m that minimises the energy norm
+// of the error, |p* - p - m q|_A, over m >= 0: m = <r0, q> / <q, A q> (r0 = b
+// - A p, so A (p* - p) = r0).  The learned start thus never has a larger
+// A-norm error than the start it corrects, and the output scale drops out.
+// Blocks of k_warm_dots: 16 per CU, so the pass streams at HBM rate.
 constexpr int kWarmBlocks = 4096;
 
-// k_warm_dots: block partial sums {r0.q, q.q} over fluid cells with an open
-// neighbour (info byte: bit 7 fluid, bits 3..6 open +x,-x,+y,-y, 0..2 count).
+// k_warm_dots: block partial sums {<r0, q>, <q, A q>} over fluid cells with
+// an open neighbour (info byte: bit 7 fluid, bits 3..6 open +x,-x,+y,-y,
+// 0..2 count); A q = count q - sum of open fluid neighbours' q.
 __global__ __launch_bounds__(256) void k_warm_dots(const float* __restrict__ r0,
-                                                   const float* __restrict__ o,
+                                                   const float* __restrict__ q,
                                                    const uint8_t* __restrict__ info, int nx,
                                                    long n, double* partials) {
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -166,43 +253,30 @@ __global__ __launch_bounds__(256) void k_warm_dots(const float* __restrict__ r0,
     if (!(f & 0x80) || (f & 7) == 0) continue;
     // an open neighbour is in the domain, so its index is valid
     float nb = 0.f;
-    if ((f & 0x08) && (info[k + 1] & 0x80)) nb += o[k + 1];
-    if ((f & 0x10) && (info[k - 1] & 0x80)) nb += o[k - 1];
-    if ((f & 0x20) && (info[k + nx] & 0x80)) nb += o[k + nx];
-    if ((f & 0x40) && (info[k - nx] & 0x80)) nb += o[k - nx];
-    const double q = (double)(f & 7) * (double)o[k] - (double)nb;
-    acc[0] += (double)r0[k] * q;
-    acc[1] += q * q;
+    if (f & 0x08) nb += q[k + 1];
+    if (f & 0x10) nb += q[k - 1];
+    if (f & 0x20) nb += q[k + nx];
+    if (f & 0x40) nb += q[k - nx];
+    const double aq = (double)(f & 7) * (double)q[k] - (double)nb;
+    acc[0] += (double)r0[k] * (double)q[k];
+    acc[1] += (double)q[k] * aq;
   }
   block_sum_store<4>(acc, partials);
 }
 
-// p += m o on fluid cells, m from the finalized sums {r0.q, q.q}.
-__global__ void k_apply_warm(double* __restrict__ p, const float* __restrict__ o,
-                             const uint8_t* __restrict__ info, const double* __restrict__ sums,
-                             long n) {
+// p += m q on fluid cells, m from the finalized sums {<r0, q>, <q, A q>}.
+__global__ void k_apply_warm(double* __restrict__ p, const float* __restrict__ q,
+                             const double* __restrict__ sums, long n) {
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double m = sums[1] > 0.0 ? fmax(sums[0] / sums[1], 0.0) : 0.0;
-  if (info[i] & 0x80) p[i] += m * (double)o[i];
+  p[i] += m * (double)q[i];
 }
 
 // Host input [N][H][W][8] f32 -> bf16
 __global__ void k_f32_to_bf16(const float* __restrict__ a, bf16_t* __restrict__ o, long n) {
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i < n) o[i] = f2bf_(a[i]);
-}
-
-template <class A, class B>
-__global__ void k_convert(const A* __restrict__ a, B* __restrict__ b, long n) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i < n) b[i] = (B)a[i];
-}
-
-__global__ void k_delta(const double* __restrict__ p, const double* __restrict__ p0,
-                        float* __restrict__ d, long n) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i < n) d[i] = (float)(p[i] - p0[i]);
 }
 
 inline int nblk(long n) { return (int)((n + 255) / 256); }
@@ -459,17 +533,25 @@ using namespace fh;
 struct fh_unet {
   UNet net;
   hipStream_t s = nullptr;
-  // training set (device): per sample r (f32), label (u8), p_prev (f32),
-  // delta (f32), stats {s_r, s_p}
-  int cap = 0, count = 0, H = 0, W = 0;
-  float *r = nullptr, *pp = nullptr, *dl = nullptr, *stats = nullptr;
-  uint8_t* lbl = nullptr;
+  // training set (device), coarse samples of cw x ch cells: the normalised
+  // input x (float4 per cell) and the target t (correction / s_r), and the
+  // s_r of each sample
+  int cap = 0, count = 0, cw = 0, ch = 0;
+  float4* x = nullptr;
+  float* t = nullptr;
+  float* sr = nullptr;
   double* partials = nullptr;
   double* sums = nullptr;
   int* idx = nullptr;
   float *tgt = nullptr, *mask = nullptr;
   float c0 = 1.f;
-  double area0 = 512.0 * 512.0;
+  // gather scratch for the current fine grid (two float4 ping-pong levels),
+  // the correction on the fine grid, and the capture's pending sample state
+  float4* g[2] = {nullptr, nullptr};
+  size_t g_cells = 0;
+  float* q = nullptr;
+  size_t q_cells = 0;
+  float* cstats = nullptr;  // {s_r, s_p} of the coarse grid in use
   std::string err;
 };
 
@@ -495,53 +577,119 @@ static thread_local std::string g_unet_err;
 
 namespace fh {
 
-static hipError_t stats_into(hipStream_t s, const float* r, const void* p, bool p64,
-                             const uint8_t* lbl, long n, double* partials, float* out2) {
-  const int blocks = 512;
-  if (p64)
-    k_stats<double><<<blocks, 256, 0, s>>>(r, (const double*)p, lbl, n, partials);
+// The block size F of the network's coarse grid for an nx x ny fine grid:
+// the smallest power of two with both nx / F and ny / F at most
+// kUnetCoarseSide; 0 when F does not divide the sides or the coarse sides are
+// not multiples of 8 (the network's three 2x poolings).
+int unet_coarse_factor(int nx, int ny, int* cw, int* ch) {
+  int F = 1;
+  while (nx / F > kUnetCoarseSide || ny / F > kUnetCoarseSide) F *= 2;
+  if (nx % F || ny % F || (nx / F) % 8 || (ny / F) % 8) return 0;
+  *cw = nx / F;
+  *ch = ny / F;
+  return F;
+}
+
+// Gathers the fine grid (nx x ny) into F x F blocks: the result's address
+// (in u->g) is returned through out.  p1 non-null: the pressure gathered is
+// p1 - p (r unread).
+static hipError_t gather(fh_unet* u, hipStream_t s, const float* r, const uint8_t* lbl,
+                         const double* p, const double* p1, int nx, int ny, int F,
+                         float4** out) {
+  const size_t need = F == 1 ? (size_t)nx * ny : (size_t)(nx / 2) * (ny / 2);
+  if (u->g_cells < need) {
+    for (auto& b : u->g) {
+      if (b) (void)hipFree(b);
+      b = nullptr;
+    }
+    u->g_cells = 0;
+    hipError_t e;
+    if ((e = hipMalloc((void**)&u->g[0], need * sizeof(float4))) ||
+        (e = hipMalloc((void**)&u->g[1], need * sizeof(float4))))
+      return e;
+    u->g_cells = need;
+  }
+  int cw = F == 1 ? nx : nx / 2, ch = F == 1 ? ny : ny / 2;
+  dim3 blk(256), grd((cw + 255) / 256, ch);
+  if (F == 1)
+    k_gather_first<1><<<grd, blk, 0, s>>>(r, lbl, p, p1, nx, cw, ch, u->g[0]);
   else
-    k_stats<float><<<blocks, 256, 0, s>>>(r, (const float*)p, lbl, n, partials);
-  k_stats_finalize<<<1, 256, 0, s>>>(partials, blocks, out2);
+    k_gather_first<2><<<grd, blk, 0, s>>>(r, lbl, p, p1, nx, cw, ch, u->g[0]);
+  int cur = 0;
+  for (int f = 2; f < F; f *= 2) {
+    cw /= 2;
+    ch /= 2;
+    k_gather2<<<dim3((cw + 255) / 256, ch), blk, 0, s>>>(u->g[cur], cw, ch, u->g[cur ^ 1]);
+    cur ^= 1;
+  }
+  *out = u->g[cur];
   return hipGetLastError();
 }
 
 int engine_capture_hook(fh_unet* u, hipStream_t s, const float* r32, const uint8_t* lbl,
                         const double* p_prev, const double* p_final, long n, int H, int W,
                         bool begin) {
-  if (!u || u->cap == 0 || H != u->H || W != u->W) return FH_OK;
+  (void)n;
+  int cw = 0, ch = 0;
+  const int F = unet_coarse_factor(W, H, &cw, &ch);
+  if (!u || u->cap == 0 || F == 0 || cw != u->cw || ch != u->ch) return FH_OK;
   if (u->count >= u->cap) return FH_OK;
-  long slot = u->count;
-  size_t o = (size_t)slot * n;
+  // both calls gather over the labels; the first also over the defect, the
+  // second over both pressures
+  if (!lbl || !p_prev || (begin ? !r32 : !p_final)) {
+    g_unet_err = "capture: missing field";
+    return FH_ERR_BAD_ARG;
+  }
+  const long hw = (long)cw * ch, slot = u->count;
+  const float area = (float)F * (float)F;
+  float4* c = nullptr;
   if (begin) {
-    UCHK(hipMemcpyAsync(u->r + o, r32, n * 4, hipMemcpyDeviceToDevice, s));
-    UCHK(hipMemcpyAsync(u->lbl + o, lbl, n, hipMemcpyDeviceToDevice, s));
-    k_convert<double, float><<<nblk(n), 256, 0, s>>>(p_prev, u->pp + o, n);
-    UCHK(stats_into(s, r32, p_prev, true, lbl, n, u->partials, u->stats + 2 * slot));
+    UCHK(gather(u, s, r32, lbl, p_prev, nullptr, W, H, F, &c));
+    k_coarse_stats<<<1, 1024, 0, s>>>(c, (int)hw, area, u->sr + 2 * slot);
+    k_coarse_input<<<nblk(hw), 256, 0, s>>>(c, (int)hw, area, u->sr + 2 * slot,
+                                            u->x + slot * hw, nullptr);
   } else {
-    k_delta<<<nblk(n), 256, 0, s>>>(p_final, p_prev, u->dl + o, n);
-    UCHK(hipGetLastError());
+    UCHK(gather(u, s, nullptr, lbl, p_prev, p_final, W, H, F, &c));
+    k_coarse_target<<<nblk(hw), 256, 0, s>>>(c, (int)hw, area, u->sr + 2 * slot,
+                                             u->t + slot * hw);
     u->count++;
   }
+  UCHK(hipGetLastError());
   return FH_OK;
 }
 
 int engine_warm_start(fh_unet* u, hipStream_t s, const float* r32, const uint8_t* lbl,
                       const uint8_t* info, double* p, int H, int W, float* stats2) {
+  (void)stats2;
   UNet& net = u->net;
-  if ((H % 8) || (W % 8)) return FH_OK;  // three 2x poolings need multiples of 8
-  UCHK(unet_resize(net, 1, H, W, false));
-  long n = (long)H * W;
-  UCHK(stats_into(s, r32, p, true, lbl, n, u->partials, stats2));
-  float c = (float)(u->c0 * ((double)H * W / u->area0));
-  dim3 g(nblk(n), 1);
-  k_prep<double><<<g, 256, 0, s>>>(r32, p, lbl, nullptr, stats2, nullptr, n, H, W, c,
-                                   net.act[0], nullptr, nullptr);
+  int cw = 0, ch = 0;
+  const int F = unet_coarse_factor(W, H, &cw, &ch);
+  if (F == 0) return FH_OK;  // no coarse grid the network can take: the start stays
+  if (!r32 || !lbl || !info || !p) {
+    g_unet_err = "warm start: missing field";
+    return FH_ERR_BAD_ARG;
+  }
+  const long n = (long)H * W, hw = (long)cw * ch;
+  UCHK(unet_resize(net, 1, ch, cw, false));
+  if (u->q_cells < (size_t)n) {
+    if (u->q) (void)hipFree(u->q);
+    u->q = nullptr;
+    u->q_cells = 0;
+    UCHK(hipMalloc((void**)&u->q, (size_t)n * sizeof(float)));
+    u->q_cells = n;
+  }
+  float4* c = nullptr;
+  UCHK(gather(u, s, r32, lbl, p, nullptr, W, H, F, &c));
+  const float area = (float)F * (float)F;
+  k_coarse_stats<<<1, 1024, 0, s>>>(c, (int)hw, area, u->cstats);
+  k_coarse_input<<<nblk(hw), 256, 0, s>>>(c, (int)hw, area, u->cstats, nullptr, net.act[0]);
   unet_forward(net, s);
+  k_prolong_out<<<dim3((W + 255) / 256, H), 256, 0, s>>>(net.out, cw, ch, F, info, W, H,
+                                                         u->cstats, u->c0, u->q);
   // enough blocks to fill the chip (the partials buffer holds kWarmBlocks)
-  k_warm_dots<<<kWarmBlocks, 256, 0, s>>>(r32, net.out, info, W, n, u->partials);
+  k_warm_dots<<<kWarmBlocks, 256, 0, s>>>(r32, u->q, info, W, n, u->partials);
   launch_finalize_sum4(s, u->partials, kWarmBlocks, u->sums);
-  k_apply_warm<<<nblk(n), 256, 0, s>>>(p, net.out, info, u->sums, n);
+  k_apply_warm<<<nblk(n), 256, 0, s>>>(p, u->q, u->sums, n);
   UCHK(hipGetLastError());
   return FH_OK;
 }
@@ -554,17 +702,25 @@ const char* fh_unet_last_error(void) { return g_unet_err.c_str(); }
 
 int fh_unet_num_params(void) { return (int)unet_param_count(); }
 
+int fh_unet_coarse_grid(int nx, int ny, int* cw, int* ch) {
+  if (!cw || !ch || nx < 1 || ny < 1) return 0;
+  return unet_coarse_factor(nx, ny, cw, ch);
+}
+
 fh_unet* fh_unet_create(unsigned long long seed) {
   fh_unet* u = new fh_unet();
-  if (hipStreamCreate(&u->s) != hipSuccess || unet_init(u->net, seed, u->s) != hipSuccess) {
+  if (hipStreamCreate(&u->s) != hipSuccess || unet_init(u->net, seed, u->s) != hipSuccess ||
+      hipMalloc((void**)&u->partials, 4 * kWarmBlocks * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&u->sums, 64) != hipSuccess ||
+      hipMalloc((void**)&u->cstats, 64) != hipSuccess) {
     g_unet_err = "fh_unet_create: allocation failed";
     unet_free(u->net);
+    for (void* p : {(void*)u->partials, (void*)u->sums, (void*)u->cstats})
+      if (p) (void)hipFree(p);
     if (u->s) (void)hipStreamDestroy(u->s);
     delete u;
     return nullptr;
   }
-  (void)hipMalloc((void**)&u->partials, 4 * kWarmBlocks * sizeof(double));
-  (void)hipMalloc((void**)&u->sums, 64);
   return u;
 }
 
@@ -572,9 +728,9 @@ void fh_unet_destroy(fh_unet* u) {
   if (!u) return;
   (void)hipStreamSynchronize(u->s);
   unet_free(u->net);
-  for (void* p : {(void*)u->r, (void*)u->pp, (void*)u->dl, (void*)u->stats, (void*)u->lbl,
-                  (void*)u->partials, (void*)u->sums, (void*)u->idx, (void*)u->tgt,
-                  (void*)u->mask})
+  for (void* p : {(void*)u->x, (void*)u->t, (void*)u->sr, (void*)u->partials, (void*)u->sums,
+                  (void*)u->idx, (void*)u->tgt, (void*)u->mask, (void*)u->g[0], (void*)u->g[1],
+                  (void*)u->q, (void*)u->cstats})
     if (p) (void)hipFree(p);
   (void)hipStreamDestroy(u->s);
   delete u;
@@ -595,17 +751,15 @@ int fh_unet_set_params(fh_unet* u, const float* in, int n) {
   return FH_OK;
 }
 
-int fh_unet_set_scale(fh_unet* u, double c0, double area0) {
-  if (!u || !(c0 > 0) || !(area0 > 0)) return FH_ERR_BAD_ARG;
+int fh_unet_set_scale(fh_unet* u, double c0) {
+  if (!u || !(c0 > 0)) return FH_ERR_BAD_ARG;
   u->c0 = (float)c0;
-  u->area0 = area0;
   return FH_OK;
 }
 
-int fh_unet_get_scale(fh_unet* u, double* c0, double* area0) {
-  if (!u || !c0 || !area0) return FH_ERR_BAD_ARG;
+int fh_unet_get_scale(fh_unet* u, double* c0) {
+  if (!u || !c0) return FH_ERR_BAD_ARG;
   *c0 = u->c0;
-  *area0 = u->area0;
   return FH_OK;
 }
 
@@ -649,58 +803,50 @@ int fh_unet_backward(fh_unet* u, int N, int H, int W, const float* in, const flo
 }
 
 int fh_unet_dataset_reserve(fh_unet* u, int capacity, int H, int W) {
-  if (!u || capacity < 0 || H % 8 || W % 8) return FH_ERR_BAD_ARG;
+  if (!u || capacity < 0 || H < 8 || W < 8 || H % 8 || W % 8 || H > kUnetCoarseSide ||
+      W > kUnetCoarseSide)
+    return FH_ERR_BAD_ARG;
   UCHK(hipStreamSynchronize(u->s));
-  for (void** p : {(void**)&u->r, (void**)&u->pp, (void**)&u->dl, (void**)&u->stats,
-                   (void**)&u->lbl})
+  for (void** p : {(void**)&u->x, (void**)&u->t, (void**)&u->sr})
     if (*p) {
       UCHK(hipFree(*p));
       *p = nullptr;
     }
   u->cap = capacity;
   u->count = 0;
-  u->H = H;
-  u->W = W;
+  u->ch = H;
+  u->cw = W;
   if (capacity == 0) return FH_OK;
   size_t n = (size_t)capacity * H * W;
-  UCHK(hipMalloc((void**)&u->r, n * 4));
-  UCHK(hipMalloc((void**)&u->pp, n * 4));
-  UCHK(hipMalloc((void**)&u->dl, n * 4));
-  UCHK(hipMalloc((void**)&u->lbl, n));
-  UCHK(hipMalloc((void**)&u->stats, (size_t)capacity * 2 * 4));
+  UCHK(hipMalloc((void**)&u->x, n * sizeof(float4)));
+  UCHK(hipMalloc((void**)&u->t, n * 4));
+  UCHK(hipMalloc((void**)&u->sr, (size_t)capacity * 2 * 4));
   return FH_OK;
 }
 
 int fh_unet_dataset_size(fh_unet* u) { return u ? u->count : 0; }
 
-// RMS over the first samples of |p* - p_prev| / s_r: the output scale c0.
+// c0 = RMS over the first samples' coarse cells holding fluid of the target
+// (correction / s_r).
 int fh_unet_calibrate(fh_unet* u, int samples) {
   if (!u || u->count == 0) return FH_ERR_BAD_ARG;
   UCHK(hipStreamSynchronize(u->s));
-  long n = (long)u->H * u->W;
-  int m = samples < u->count ? samples : u->count;
-  std::vector<float> d(n), st(2);
-  std::vector<uint8_t> l(n);
-  double acc = 0;
-  int used = 0;
+  const long hw = (long)u->cw * u->ch;
+  const int m = samples < u->count ? samples : u->count;
+  std::vector<float4> x(hw);
+  std::vector<float> t(hw);
+  double acc = 0, cnt = 0;
   for (int k = 0; k < m; k++) {
-    UCHK(hipMemcpy(d.data(), u->dl + (size_t)k * n, n * 4, hipMemcpyDeviceToHost));
-    UCHK(hipMemcpy(l.data(), u->lbl + (size_t)k * n, n, hipMemcpyDeviceToHost));
-    UCHK(hipMemcpy(st.data(), u->stats + 2 * k, 8, hipMemcpyDeviceToHost));
-    double s2 = 0, cnt = 0;
-    for (long i = 0; i < n; i++)
-      if (l[i] == FLUID) {
-        s2 += (double)d[i] * d[i];
+    UCHK(hipMemcpy(x.data(), u->x + (size_t)k * hw, hw * sizeof(float4), hipMemcpyDeviceToHost));
+    UCHK(hipMemcpy(t.data(), u->t + (size_t)k * hw, hw * 4, hipMemcpyDeviceToHost));
+    for (long i = 0; i < hw; i++)
+      if (1.f - x[i].y - x[i].z > 1e-6f) {
+        acc += (double)t[i] * t[i];
         cnt += 1;
       }
-    if (cnt > 0 && st[0] > 0) {
-      acc += std::sqrt(s2 / cnt) / st[0];
-      used++;
-    }
   }
-  if (used == 0) return FH_ERR_BAD_ARG;
-  u->c0 = (float)(acc / used);
-  u->area0 = (double)u->H * u->W;
+  if (cnt == 0 || acc == 0) return FH_ERR_BAD_ARG;
+  u->c0 = (float)std::sqrt(acc / cnt);
   return FH_OK;
 }
 
@@ -710,7 +856,7 @@ int fh_unet_train(fh_unet* u, int steps, int batch, double lr, unsigned long lon
                   double* loss_out) {
   if (!u || u->count < 1 || batch < 1 || steps < 0) return FH_ERR_BAD_ARG;
   UNet& net = u->net;
-  const int H = u->H, W = u->W;
+  const int H = u->ch, W = u->cw;
   const long hw = (long)H * W, px = hw * batch;
   UCHK(unet_resize(net, batch, H, W, true));
   if (u->idx) (void)hipFree(u->idx);
@@ -726,7 +872,7 @@ int fh_unet_train(fh_unet* u, int steps, int batch, double lr, unsigned long lon
   for (int k = 0; k < u->count; k++) order[k] = k;
   size_t pos = order.size();
   std::vector<int> hidx(batch);
-  float c = (float)(u->c0 * ((double)hw / u->area0));
+  const float inv_c0 = 1.f / u->c0;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   for (int st = 0; st < steps; st++) {
     for (int b = 0; b < batch; b++) {
@@ -741,8 +887,8 @@ int fh_unet_train(fh_unet* u, int steps, int batch, double lr, unsigned long lon
     }
     UCHK(hipMemcpyAsync(u->idx, hidx.data(), batch * 4, hipMemcpyHostToDevice, u->s));
     dim3 g(nblk(hw), batch);
-    k_prep<float><<<g, 256, 0, u->s>>>(u->r, u->pp, u->lbl, u->dl, u->stats, u->idx, hw, H, W, c,
-                                       net.act[0], u->tgt, u->mask);
+    k_train_prep<<<g, 256, 0, u->s>>>(u->x, u->t, u->idx, hw, inv_c0, net.act[0], u->tgt,
+                                      u->mask);
     unet_forward(net, u->s);
     k_loss<<<512, 256, 0, u->s>>>(net.out, u->tgt, u->mask, px, u->partials);
     launch_finalize_sum4(u->s, u->partials, 512, u->sums);

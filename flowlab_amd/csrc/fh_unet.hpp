@@ -45,6 +45,9 @@ struct ConvLayer {
 };
 
 constexpr int kUnetIn = 8;
+// The learned warm start runs the network on the fine grid gathered into
+// blocks so that neither coarse side exceeds this (unet_coarse_factor).
+constexpr int kUnetCoarseSide = 128;
 constexpr int kUnetLayers = 15;
 // layer indices
 enum { E0A, E0B, E1A, E1B, E2A, E2B, B3A, B3B, D2A, D2B, D1A, D1B, D0A, D0B, HEAD };

@@ -80,8 +80,9 @@ SIGNATURES = [
     ("fh_unet_num_params", _i, []),
     ("fh_unet_get_params", _i, [_vp, _fp, _i]),
     ("fh_unet_set_params", _i, [_vp, _fp, _i]),
-    ("fh_unet_set_scale", _i, [_vp, _d, _d]),
-    ("fh_unet_get_scale", _i, [_vp, _dp, _dp]),
+    ("fh_unet_coarse_grid", _i, [_i, _i, _ip, _ip]),
+    ("fh_unet_set_scale", _i, [_vp, _d]),
+    ("fh_unet_get_scale", _i, [_vp, _dp]),
     ("fh_unet_forward", _i, [_vp, _i, _i, _i, _fp, _fp]),
     ("fh_unet_backward", _i, [_vp, _i, _i, _i, _fp, _fp, _fp]),
     ("fh_unet_dataset_reserve", _i, [_vp, _i, _i, _i]),

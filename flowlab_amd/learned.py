@@ -1,17 +1,18 @@
 """Learned warm start for the pressure solve (no reference counterpart).
 
 A 4-level convolutional U-Net (3x3 convs, channels 16-32-64-64, BF16 weights
-and activations, FP32 accumulation on MFMA) runs inside libflowhip.  It sees
-the defect of the previous pressure for the new right-hand side, the solid
-and Air masks and the previous pressure, and predicts a correction; the
+and activations, FP32 accumulation on MFMA) runs inside libflowhip on a
+coarse view of the grid: the defect of the solve's start summed over F x F
+blocks (coarse sides at most 128), the blocks' solid and Air fractions and
+their fluid-mean start pressure.  Its output, interpolated onto the fine
+fluid cells, is added to the start with the energy-optimal step; the
 defect-correction multigrid then runs its V-cycles to the tolerance exactly
-as without it, so plain multigrid stays both the fallback and the arbiter
-of convergence.
+as without it, so plain multigrid stays both the fallback and the arbiter of
+convergence.
 
 This module: the network handle (parameters, save/load), the seeded scene
 generator of the training workload, data generation from the engine's own
-multigrid solves, training, and the zero / previous-pressure / learned
-comparison.
+multigrid solves, training, and the comparison of starts.
 """
 from __future__ import annotations
 
@@ -67,25 +68,24 @@ class UNet:
         _ucheck(self.lib.fh_unet_set_params(self._u, fptr(p), self.n))
 
     @property
-    def scale(self):
-        c0, a0 = C.c_double(), C.c_double()
-        _ucheck(self.lib.fh_unet_get_scale(self._u, C.byref(c0), C.byref(a0)))
-        return c0.value, a0.value
+    def scale(self) -> float:
+        """c0: the coarse correction is s_r * c0 * the network's output."""
+        c0 = C.c_double()
+        _ucheck(self.lib.fh_unet_get_scale(self._u, C.byref(c0)))
+        return c0.value
 
     @scale.setter
-    def scale(self, v):
-        _ucheck(self.lib.fh_unet_set_scale(self._u, float(v[0]), float(v[1])))
+    def scale(self, c0: float):
+        _ucheck(self.lib.fh_unet_set_scale(self._u, float(c0)))
 
     def save(self, path: str = WEIGHTS, meta: dict | None = None):
         os.makedirs(os.path.dirname(path), exist_ok=True)
-        c0, a0 = self.scale
-        np.savez(path, params=self.params, c0=c0, area0=a0,
-                 meta=json.dumps(meta or {}))
+        np.savez(path, params=self.params, c0=self.scale, meta=json.dumps(meta or {}))
 
     def load(self, path: str = WEIGHTS):
         d = np.load(path, allow_pickle=False)
         self.params = d["params"]
-        self.scale = (float(d["c0"]), float(d["area0"]))
+        self.scale = float(d["c0"])
         return json.loads(str(d["meta"]))
 
     # ---- raw network (tests) ---------------------------------------------
@@ -106,6 +106,7 @@ class UNet:
 
     # ---- training ----------------------------------------------------------
     def reserve(self, capacity: int, H: int, W: int):
+        """Room for `capacity` samples of an H x W coarse grid (coarse_grid)."""
         _ucheck(self.lib.fh_unet_dataset_reserve(self._u, capacity, H, W))
 
     @property
@@ -120,6 +121,14 @@ class UNet:
         _ucheck(self.lib.fh_unet_train(self._u, steps, batch, lr, seed,
                                        loss.ctypes.data_as(C.POINTER(C.c_double))))
         return loss
+
+
+def coarse_grid(nx: int, ny: int):
+    """(F, coarse width, coarse height) of the grid the network runs on for
+    an nx x ny grid; F = 0 when the network cannot take it."""
+    cw, ch = C.c_int(), C.c_int()
+    F = hip.lib().fh_unet_coarse_grid(nx, ny, C.byref(cw), C.byref(ch))
+    return F, cw.value, ch.value
 
 
 # ---- the training / evaluation workload ----------------------------------
@@ -175,66 +184,92 @@ def random_scene(n: int, seed: int):
                           dye_source=dye_source), rng.uniform(0.5, 2.0)
 
 
-def _sim(n, seed, tol=1e-6):
+def _sim(n, seed, tol=1e-6, extrapolation=0):
     from .simulation import Simulation
     g, scene, w = random_scene(n, seed)
     sim = Simulation(n, n, dict(scheme="macCormack", cfl_target=0.5, dt_max=0.5 / n * 4,
                                 solver="mg-dc", pressure_iters=100, tol=tol, buoyancy=w,
-                                residual_every=0), readback=False)
+                                residual_every=0, warm_extrapolation=extrapolation),
+                     readback=False)
     sim.reset(scene)
     return sim
 
 
 def generate(unet: UNet, samples: int, n: int = 512, steps_per_scene: int = 50, seed: int = 100,
-             log=print):
+             zero_every: int = 5, log=print):
     """Fill unet's training set from plain multigrid solves (tol 1e-8) of
-    seeded random n x n scenes, `steps_per_scene` steps each."""
-    unet.reserve(samples, n, n)
+    seeded random n x n scenes, `steps_per_scene` steps each: each step's
+    solve, from the previous pressure in even-numbered scenes and from the
+    linear extrapolation in time in odd-numbered ones, and every
+    `zero_every`-th step also the same right-hand side solved from zero."""
+    from .engine import HipPressureSolver
+    F, cw, ch = coarse_grid(n, n)
+    if not F:
+        raise ValueError(f"no coarse grid for {n} x {n}")
+    unet.reserve(samples, ch, cw)
+    zero = HipPressureSolver(n, n, "mg-dc")
+    zero.engine.set_capture(unet)
     k = 0
     t0 = time.perf_counter()
     while unet.samples < samples:
-        sim = _sim(n, seed + k, tol=1e-8)
+        sim = _sim(n, seed + k, tol=1e-8, extrapolation=k % 2)
         sim.stepper.engine.set_capture(unet)
-        for _ in range(steps_per_scene):
+        for st in range(steps_per_scene):
             if unet.samples >= samples:
                 break
             sim.step()
+            if zero_every and st % zero_every == zero_every - 1 and unet.samples < samples:
+                e = sim.stepper.engine
+                zero.solve(None, np.zeros(n * n), e.read_divergence(), sim.f.label,
+                           sim.g.h * sim.g.h / sim.dt, 100, 1.0, 1e-8)
         sim.stepper.engine.set_capture(None)
         k += 1
         if log and k % 20 == 0:
             log(f"generate: {unet.samples}/{samples} samples from {k} scenes "
                 f"({time.perf_counter() - t0:.1f} s)")
+    zero.engine.set_capture(None)
     return k
+
+
+START_NAMES = ("zero", "network", "previous", "learned", "extrapolated", "learned_extrapolated")
 
 
 def evaluate(unet: UNet, n: int, scenes: int = 4, warm_steps: int = 8, seed: int = 9000,
              tol: float = 1e-6):
     """For `scenes` seeded n x n scenes advanced `warm_steps` steps, solve the
-    next step's pressure from (a) zero, (b) the previous pressure, (c) the
-    learned warm start on the previous pressure, (d) the learned warm start on
-    a zero pressure (the network's own contribution, without the previous
-    pressure), each to `tol`; returns V-cycles and device ms."""
+    next step's pressure to `tol` from six starts: zero; the network on a
+    zero pressure (its own contribution, without the previous pressure); the
+    previous pressure (the reference's warm start); the network on the
+    previous pressure; the last two pressures extrapolated linearly in time
+    at the steps' own dt; the network on that extrapolation.  Returns
+    V-cycles and device ms per start."""
     from .engine import HipPressureSolver
-    res = {"zero": [], "previous": [], "learned": [], "network": []}
+    res = {k: [] for k in START_NAMES}
     for s in range(scenes):
         sim = _sim(n, seed + s, tol=tol)
-        for _ in range(warm_steps):
+        for _ in range(warm_steps - 1):
             sim.step()
         e = sim.stepper.engine
-        # the next step's right-hand side: advance once more with the
-        # solve's input captured from the engine's own buffers
-        p_prev = e.read_pressure()
+        p_a = e.read_pressure()
         sim.step()
+        dt_a = sim.dt
+        p_b = e.read_pressure()
+        # the next step's right-hand side, captured from the engine's buffers
+        sim.step()
+        dt_b = sim.dt
         div = e.read_divergence()
         label = sim.f.label.copy()
         scale = sim.g.h * sim.g.h / sim.dt
+        starts = {"zero": np.zeros(n * n), "previous": p_b,
+                  "extrapolated": p_b + (dt_b / dt_a) * (p_b - p_a)}
         solver = HipPressureSolver(n, n, "mg-dc")
-        for name in ("zero", "previous", "learned", "network"):
-            solver.engine.set_warm_start(unet if name in ("learned", "network") else None)
-            from_zero = name in ("zero", "network")
-            p = np.zeros(n * n) if from_zero else p_prev.copy()
+        for name in START_NAMES:
+            base = {"network": "zero", "learned": "previous",
+                    "learned_extrapolated": "extrapolated"}.get(name, name)
+            solver.engine.set_warm_start(unet if base != name else None)
+            p = starts[base].copy()
             solver.solve(None, p, div, label, scale, 1, 1.0, 0)  # warm-up: labels, buffers
-            p = np.zeros(n * n) if from_zero else p_prev.copy()
+            p = starts[base].copy()
             it = solver.solve(None, p, div, label, scale, 200, 1.0, tol)
             res[name].append((it, solver.engine.last_solve_ms()))
         solver.engine.set_warm_start(None)

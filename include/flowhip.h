@@ -231,33 +231,45 @@ int fh_solve_pressure_rbsor_noalloc(void* stream, double* p, const double* div,
 
 /* ---- learned warm start (no reference counterpart) ---------------------
  * A 4-level convolutional U-Net (3x3 convs, channels 16-32-64-64, BF16
- * weights and activations, FP32 accumulation on MFMA) predicts a correction
- * to the previous pressure; the defect-correction solve then runs its
- * V-cycles to the tolerance as before, so plain multigrid decides convergence.
+ * weights and activations, FP32 accumulation on MFMA) predicts a coarse
+ * correction to the start of the solve (the previous pressure, or its
+ * extrapolation in time); the defect-correction solve then runs its
+ * V-cycles to the tolerance as before, so plain multigrid decides
+ * convergence.  The network runs on the fine grid gathered into F x F blocks
+ * (fh_unet_coarse_grid), its output is interpolated bilinearly onto the fine
+ * fluid cells, and the start moves along it by the energy-optimal step
+ * (never increasing the error's A-norm).
  * Parameters: per layer weights [cout][3][3][cin] then bias [cout], FP32,
  * for the 15 layers e0a e0b e1a e1b e2a e2b b3a b3b d2a d2b d1a d1b d0a d0b
- * head.  Network input: [N][H][W][8] (channels r/s_r, solid, air, p/s_p,
- * then zeros); output [N][H][W].  H and W must be multiples of 8. */
+ * head.  Network input: [N][H][W][8] (channels: defect block sum / s_r,
+ * solid fraction, Air fraction, fluid-mean pressure / s_p, then zeros);
+ * output [N][H][W].  H and W must be multiples of 8. */
 typedef struct fh_unet fh_unet;
 fh_unet* fh_unet_create(unsigned long long seed);
 void fh_unet_destroy(fh_unet* u);
 const char* fh_unet_last_error(void);
 int fh_unet_num_params(void);
+/* The block size F of the coarse grid the network runs on for an nx x ny
+ * grid (the smallest power of two leaving both sides at most 128) and the
+ * coarse sides; 0 when F does not divide the sides into multiples of 8 (the
+ * learned start then leaves the start unchanged). */
+int fh_unet_coarse_grid(int nx, int ny, int* cw, int* ch);
 int fh_unet_get_params(fh_unet* u, float* out, int n);
 int fh_unet_set_params(fh_unet* u, const float* in, int n);
-/* Output scale: warm start = p_prev + s_r * c0 * (H W / area0) * net. */
-int fh_unet_set_scale(fh_unet* u, double c0, double area0);
-int fh_unet_get_scale(fh_unet* u, double* c0, double* area0);
+/* Output scale: the coarse correction is s_r * c0 * net. */
+int fh_unet_set_scale(fh_unet* u, double c0);
+int fh_unet_get_scale(fh_unet* u, double* c0);
 int fh_unet_forward(fh_unet* u, int N, int H, int W, const float* in, float* out);
 /* Parameter gradient of sum(dout .* net(in)). */
 int fh_unet_backward(fh_unet* u, int N, int H, int W, const float* in, const float* dout,
                      float* dparams);
-/* Training set on the device: every FH_SOLVER_MG_DC solve of an engine with
- * capture on (fh_set_capture) appends (defect, labels, p_prev, p* - p_prev)
- * until `capacity` samples of H x W are held. */
+/* Training set on the device: every FP64 multigrid solve of an engine with
+ * capture on (fh_set_capture) whose coarse grid is H x W (coarse sides)
+ * appends (the network input of its start, the block-mean correction
+ * p* - p over s_r) until `capacity` samples are held. */
 int fh_unet_dataset_reserve(fh_unet* u, int capacity, int H, int W);
 int fh_unet_dataset_size(fh_unet* u);
-/* Sets c0 from the captured samples' RMS correction / defect ratio. */
+/* Sets c0 to the RMS of the captured targets (correction / s_r). */
 int fh_unet_calibrate(fh_unet* u, int samples);
 /* `steps` Adam steps (lr; beta 0.9/0.999) on batches of `batch`; per-step
  * masked MSE in loss_out (may be NULL). */

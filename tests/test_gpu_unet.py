@@ -75,13 +75,12 @@ def test_capture_train_and_warm_start():
     Lrn.generate(u, samples=24, n=n, steps_per_scene=12, seed=5, log=None)
     assert u.samples == 24
     u.calibrate(8)
-    c0, a0 = u.scale
-    assert c0 > 0 and a0 == n * n
+    assert u.scale > 0
     loss = u.train(steps=40, batch=4, lr=2e-3, seed=3)
     assert np.isfinite(loss).all()
     assert loss[-10:].mean() < loss[:5].mean()
     res = Lrn.evaluate(u, n=n, scenes=1, warm_steps=4, seed=77)
-    for k in ("zero", "previous", "learned"):
+    for k in Lrn.START_NAMES:
         assert res[k]["vcycles"] >= 1
     # a warm-started solve converges to 1e-6 by the oracle
     sim = Lrn._sim(n, 123)
@@ -97,15 +96,16 @@ def test_capture_train_and_warm_start():
 
 def test_trained_network_on_held_out_scenes():
     """The committed trained network on scenes it was not trained on (seeds
-    9101.., training used 100..499), 512 x 512 as in training, each solve to
-    1e-6 from four starts: zero, the network on a zero pressure, the previous
-    pressure (the reference's own warm start), the network on the previous
-    pressure.  Every solve reaches 1e-6 by the oracle's FP64 residual, and the
-    network's start never needs more V-cycles than the start it is applied
-    to.  No gain is asserted: measured here it needs as many V-cycles as the
-    start it corrects (the residual-minimising step along its output removes
-    too little of the error to save a cycle), and the bench's learned-start
-    figures carry the same result."""
+    9101-9104; training used 100..433), 512 x 512 as in training, the next
+    step's solve to 1e-6 from six starts: zero, the network on a zero
+    pressure, the previous pressure (the reference's own warm start), the
+    network on it, the last two pressures extrapolated linearly in time, and
+    the network on that.  Every solve reaches 1e-6 by the oracle's FP64
+    residual.  The gain asserted is the network's on the previous pressure:
+    fewer V-cycles in all over the four scenes.  (On the extrapolation it
+    saves about a third of a cycle per solve, from zero nothing; the device
+    time of its inference exceeds what the saved cycles take, see the bench's
+    C4 leg.)"""
     import os
     from flowlab_amd import learned as Lrn
     from flowlab_amd.engine import HipPressureSolver
@@ -114,30 +114,34 @@ def test_trained_network_on_held_out_scenes():
     u = Lrn.UNet()
     u.load(Lrn.WEIGHTS)
     n = 512
-    starts = (("zero", False, None), ("network", False, u), ("previous", True, None),
-              ("learned", True, u))
-    cycles = {k: [] for k, _, _ in starts}
-    for seed in (9101, 9102):
+    cycles = {k: [] for k in Lrn.START_NAMES}
+    for seed in (9101, 9102, 9103, 9104):
         sim = Lrn._sim(n, seed)
-        for _ in range(8):
+        for _ in range(7):
             sim.step()
         e = sim.stepper.engine
-        p_prev = e.read_pressure()
+        p_a = e.read_pressure()
+        sim.step()
+        dt_a = sim.dt
+        p_b = e.read_pressure()
         sim.step()
         div = e.read_divergence()
         label = sim.f.label.copy()
         scale = sim.g.h * sim.g.h / sim.dt
+        starts = {"zero": np.zeros(n * n), "previous": p_b,
+                  "extrapolated": p_b + (sim.dt / dt_a) * (p_b - p_a)}
         solver = HipPressureSolver(n, n, "mg-dc")
-        for name, prev, warm in starts:
-            solver.engine.set_warm_start(warm)
-            p = p_prev.copy() if prev else np.zeros(n * n)
+        for name in Lrn.START_NAMES:
+            base = {"network": "zero", "learned": "previous",
+                    "learned_extrapolated": "extrapolated"}.get(name, name)
+            solver.engine.set_warm_start(u if base != name else None)
+            p = starts[base].copy()
             solver.solve(None, p, div, label, scale, 1, 1.0, 0)  # labels, buffers
-            p = p_prev.copy() if prev else np.zeros(n * n)
+            p = starts[base].copy()
             cycles[name].append(solver.solve(None, p, div, label, scale, 200, 1.0, 1e-6))
             assert O.relative_residual(n, n, p, div, label, scale) <= 1e-6, (seed, name)
         solver.engine.set_warm_start(None)
     print("V-cycles", cycles)
-    for a, b in (("network", "zero"), ("learned", "previous")):
-        assert all(x <= y for x, y in zip(cycles[a], cycles[b])), (a, b, cycles)
-    # the previous pressure, for reference, saves cycles over the zero start
-    assert sum(cycles["previous"]) < sum(cycles["zero"]), cycles
+    assert sum(cycles["learned"]) < sum(cycles["previous"]), cycles
+    # the starts themselves, for reference
+    assert sum(cycles["extrapolated"]) < sum(cycles["previous"]) < sum(cycles["zero"]), cycles
