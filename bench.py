@@ -60,8 +60,8 @@ WORKLOAD_STEPS = 200
 #     direction 4 + p 8 + r 8 + r32 4 written.
 #   divergence: under the FP64 solvers formed by the solve's initial defect
 #     pass (the traced launch): u, v 8 + p 8 + label 1 read, div 8 + r32 4
-#     written, +8 for the FP64 residual under flexible CG; otherwise its own
-#     pass, u, v 8 read + div 8 written (16).
+#     written (the first cycle's update stores the FP64 residual flexible CG
+#     keeps); otherwise its own pass, u, v 8 read + div 8 written (16).
 #   forces (buoyancy alone, where the face pass does not apply it): v 4 read
 #     + 4 written + heat 4 + label 1.
 #   project (gradient subtraction + outflow): u, v 8 read + 8 written +
@@ -99,7 +99,7 @@ def algo_bytes(kernel, solver, buoyancy=True, warm="previous"):
     if kernel == "dc_update" and solver == "mg-fcg":
         b += 8.0
     if kernel == "divergence" and solver in ("mg-dc", "mg-fcg"):
-        b = 29.0 + (8.0 if solver == "mg-fcg" else 0.0)
+        b = 29.0
         if warm == "extrapolated":
             b += 16.0
     return b
@@ -556,7 +556,7 @@ def run_node_c2(args):
         np.concatenate(f.dye).astype("<f8").tofile(os.path.join(d, "dye.f64"))
         f.label.tofile(os.path.join(d, "label.u8"))
         case = dict(nx=nx, ny=ny, h=g.h, scheme="macCormack", solver=args.solver, tol=1e-6,
-                    cfl=0.5, dtMax=1 / 30)
+                    cfl=0.5, dtMax=1 / 30, warm=WARM_STARTS[args.warm][0])
         with open(os.path.join(d, "case.json"), "w") as fh:
             json.dump(case, fh)
         r = subprocess.run([node, script, d, "10", str(args.node_steps)], capture_output=True,
@@ -566,8 +566,9 @@ def run_node_c2(args):
     out = json.loads(r.stdout.strip().splitlines()[-1])
     out["workload"] = (f"karman vortex street {nx}x{ny} through the N-API HipStepper from Node "
                        f"{out.get('node')}, {SOLVER_NAMES[args.solver]}, tol 1e-6, CFL 0.5 from "
-                       "the host's one-step-stale velocity; upload, readback and host CFL scan "
-                       "inside the timed loop")
+                       "the host's one-step-stale velocity, each solve from "
+                       f"{WARM_STARTS[args.warm][1]}; upload, readback and host CFL scan inside "
+                       "the timed loop")
     return out
 
 

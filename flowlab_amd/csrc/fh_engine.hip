@@ -598,7 +598,10 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
       int st0 = ensure_fcg(e);
       if (st0 != FH_OK) return st0;
     }
-    double* r64 = fcg ? e->rvec : nullptr;  // CG keeps the FP64 residual
+    // CG keeps the FP64 residual; the passes before the first cycle need not
+    // store it, since that cycle is a defect-correction cycle, whose update
+    // pass stores the residual of its result before any CG step reads it
+    double* r64 = fcg ? e->rvec : nullptr;
     // r = b - A p (no update) + norms.  The zero-divergence guard of
     // pressureMultigrid.ts:294-300 reads the same pass: on the device (the
     // norm sets `done`, so the cycles enqueued behind it skip themselves)
@@ -607,7 +610,7 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
     if (sync_init) {
       launch_set_int(e->s, e->done, 0);
       launch_dc_correct(e->s, e->p, nullptr, nullptr, e->div, L0.info, L0.b, e->nx, e->ny, scale,
-                        e->partials, nullptr, r64);
+                        e->partials, nullptr, nullptr);
       launch_dc_norm(e->s, e->partials, e->nx, e->ny, e->sums, nullptr, nullptr, scale, 0.0);
       LAUNCHED("dc_update");
       st = read_sums(e);
@@ -629,7 +632,7 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
                                e->wstats);
         if (st != FH_OK) return st;
         launch_dc_correct(e->s, e->p, nullptr, nullptr, e->div, L0.info, L0.b, e->nx, e->ny,
-                          scale, e->partials, nullptr, r64);
+                          scale, e->partials, nullptr, nullptr);
         launch_dc_norm(e->s, e->partials, e->nx, e->ny, e->sums, nullptr, nullptr, scale, 0.0);
         LAUNCHED("dc_update");
         st = read_sums(e);
@@ -640,14 +643,14 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
       if (div_fused) {
         {
           Trace t(e, FH_K_DIVERGENCE);
-          launch_div_defect(e->s, e->u, e->v, 1.0 / e->h, e->div, e->p, L0.info, L0.b, r64,
+          launch_div_defect(e->s, e->u, e->v, 1.0 / e->h, e->div, e->p, L0.info, L0.b, nullptr,
                             e->nx, e->ny, scale, e->partials, start_in_pass ? e->p2 : nullptr,
                             &sp.xp, sp.order);
         }
         if (start_in_pass) rotate_history(e);
       } else {
         launch_dc_correct(e->s, e->p, nullptr, nullptr, e->div, L0.info, L0.b, e->nx, e->ny,
-                          scale, e->partials, nullptr, r64);
+                          scale, e->partials, nullptr, nullptr);
       }
       launch_dc_norm_init(e->s, e->partials, e->nx, e->ny, e->sums, e->d_res + 16, e->done,
                           scale);

@@ -1013,7 +1013,7 @@ __global__ __launch_bounds__(256) void kb_dc_update(const SceneDesc* __restrict_
   const SceneDesc& d = D[s];
   if (init == 2) {  // the initial pass with the divergence of velocity [q ^ 1]
     dc_update_body<kRedRows, true>(gm(d.p[q]), nullptr, nullptr, nullptr, gm(d.lv[0].info),
-                                   gm(d.lv[0].b), gm(d.rvec), d.nx, d.ny, st[s].scale,
+                                   gm(d.lv[0].b), nullptr, d.nx, d.ny, st[s].scale,
                                    gm(d.partials), nullptr, bx, by, gy, bid, gm(d.u[q ^ 1]),
                                    gm(d.v[q ^ 1]), st[s].inv_h, gm(d.div));
     return;
@@ -1022,8 +1022,8 @@ __global__ __launch_bounds__(256) void kb_dc_update(const SceneDesc* __restrict_
   const float* e = init ? nullptr : gm(d.lv[0].x);
   const int* done = init ? nullptr : gm(d.done);
   dc_update_body<kRedRows>(gm(d.p[q]), pn, e, gm(d.div), gm(d.lv[0].info), gm(d.lv[0].b),
-                           gm(d.rvec), d.nx, d.ny, st[s].scale, gm(d.partials), done, bx, by, gy,
-                           bid);
+                           init ? nullptr : gm(d.rvec), d.nx, d.ny, st[s].scale, gm(d.partials),
+                           done, bx, by, gy, bid);
 }
 
 __global__ __launch_bounds__(1024) void kb_finalize(const SceneDesc* __restrict__ D,

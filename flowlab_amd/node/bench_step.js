@@ -8,7 +8,9 @@
  *   node bench_step.js <dir> <warmup> <steps>
  *
  * <dir> holds a scene as written by the Python side (case.json with nx, ny,
- * h, scheme, solver, tol, cfl, dtMax; u.f64 v.f64 p.f64 label.u8 dye.f64).
+ * h, scheme, solver, tol, cfl, dtMax, warm (the extrapolation order of the
+ * solve's start, 0 = the previous pressure); u.f64 v.f64 p.f64 label.u8
+ * dye.f64).
  * dt follows Simulation.step (simulation.ts:474-479) from the host's
  * one-step-stale velocity, as the reference's fused path computes it.
  * Prints one JSON line.
@@ -54,7 +56,8 @@ async function main() {
     label: new Uint8Array(fs.readFileSync(path.join(dir, 'label.u8'))),
     dye: [0, 1, 2].map((k) => dye.slice(k * n, (k + 1) * n)),
   };
-  const st = new HipStepper(g, 1, { solver: c.solver, iterations: 100, tol: c.tol });
+  const st = new HipStepper(g, 1, { solver: c.solver, iterations: 100, tol: c.tol,
+                                   warmExtrapolation: c.warm || 0 });
   const keep = [1, 1, 1];
   const perf = new Phases();
   const one = async () => {

@@ -220,7 +220,8 @@ static napi_value SetScalar(napi_env env, napi_callback_info info, int which) {
   double v = get_num(env, a[1]);
   int st = which == 0 ? fh_set_buoyancy(h->e, v)
          : which == 1 ? fh_set_confinement(h->e, v)
-                      : fh_set_timing(h->e, v != 0.0);
+         : which == 2 ? fh_set_timing(h->e, v != 0.0)
+                      : fh_set_warm_extrapolation(h->e, (int)v);
   if (st) return throw_status(env, st);
   return NULL;
 }
@@ -236,6 +237,12 @@ static napi_value SetConfinement(napi_env env, napi_callback_info info) {
 /* setTiming(e, on): per-phase device timing of every step (fh_set_timing) */
 static napi_value SetTiming(napi_env env, napi_callback_info info) {
   return SetScalar(env, info, 2);
+}
+
+/* setWarmExtrapolation(e, order): each step's solve starts from the last
+ * order + 1 solutions extrapolated in time (fh_set_warm_extrapolation) */
+static napi_value SetWarmExtrapolation(napi_env env, napi_callback_info info) {
+  return SetScalar(env, info, 3);
 }
 
 /* getTimings(e) -> Float64Array of FH_NUM_PHASES device ms of the last step
@@ -567,6 +574,7 @@ static napi_value Init(napi_env env, napi_value exports) {
   EXPORT("setConfinement", SetConfinement);
   EXPORT("setStir", SetStir);
   EXPORT("setTiming", SetTiming);
+  EXPORT("setWarmExtrapolation", SetWarmExtrapolation);
   EXPORT("getTimings", GetTimings);
   EXPORT("maxFaceSpeed", MaxFaceSpeed);
   EXPORT("step", Step);

@@ -47,7 +47,10 @@ class HipStepper {
    * @param {{nx:number, ny:number, h:number}} g  the velocity grid
    * @param {number} dyeScale whole-number dye refinement
    * @param {object} opts {solver: 'mg-dc'|'mg-fcg'|'mg'|'rbsor', iterations, omega, tol,
-   *   deviceTiming: per-phase device timestamps of every step (default true)}
+   *   deviceTiming: per-phase device timestamps of every step (default true),
+   *   warmExtrapolation: 0 (default: each solve starts from the previous
+   *   pressure, as stepGpu.ts does), 1 or 2 (from the last solutions
+   *   extrapolated linearly or quadratically in time)}
    */
   constructor(g, dyeScale = 1, opts = {}) {
     this.name = 'hip-fused';
@@ -75,6 +78,7 @@ class HipStepper {
     this.deviceMs = null;
     this.deviceTiming = opts.deviceTiming !== false;
     if (this.deviceTiming) addon.setTiming(this.e, 1);
+    if (opts.warmExtrapolation) addon.setWarmExtrapolation(this.e, opts.warmExtrapolation);
   }
 
   /** GpuStep.step: resolves with the solver's iteration count.  Like the

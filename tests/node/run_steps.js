@@ -72,6 +72,7 @@ async function main() {
   const f = fields();
   const st = new HipStepper(g, c.dyeScale, {
     solver: c.solver, iterations: c.iterations, omega: c.omega, tol: c.tol,
+    warmExtrapolation: c.warm || 0,
   });
   let patch = null;
   if (c.patch) {

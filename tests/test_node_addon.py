@@ -28,7 +28,7 @@ pytestmark = pytest.mark.skipif(NODE is None or not os.path.exists(
 EXPORTS = {"deviceCount", "create", "upload", "setLabels", "setDyePatch", "setBuoyancy",
            "setConfinement", "setStir", "setTiming", "getTimings", "maxFaceSpeed", "step",
            "solvePressure", "readVelocity", "readDye", "readPressure", "advectVelocity",
-           "advectDye"}
+           "advectDye", "setWarmExtrapolation"}
 
 
 def test_addon_loads_and_exports():
@@ -121,3 +121,45 @@ def test_node_stepper_matches_oracle(tmp_path, f32):
     stale1 = np.fromfile(tmp_path / "stale_u_1.f64", "<f8")
     assert np.array_equal(inv_u, stale1)
     assert rel(inv_u, us[0]) < tol_u
+
+
+@pytest.mark.gpu
+def test_node_stepper_extrapolated_start_matches_python_engine(tmp_path):
+    """HipStepper({warmExtrapolation: 1}) from Node and the Python engine with
+    the same setting step the same scene through the same library: the same
+    V-cycles and the same velocity bits after 6 steps, and fewer V-cycles in
+    all than the previous-pressure start."""
+    from flowlab_amd import scenes as S
+    from flowlab_amd.engine import HipEngine
+    from oracle import pyoracle as O
+    nx, ny, steps = 128, 64, 6
+    g, scene = S.karman_street(nx, ny)
+    osim = O.OracleSim(nx, ny)
+    osim.reset(g, scene)
+    dt = 0.5 * g.h
+    for name, arr in (("u", osim.u), ("v", osim.v), ("p", osim.p), ("dye", osim.dye)):
+        np.asarray(arr, "<f8").tofile(tmp_path / f"{name}.f64")
+    osim.label.tofile(tmp_path / "label.u8")
+    case = dict(nx=nx, ny=ny, h=g.h, dyeScale=1, scheme="macCormack", solver="mg-fcg",
+                iterations=60, omega=1.0, tol=1e-6, steps=steps, dt=dt, buoyancy=0.0,
+                confinement=0.0, keep=[1, 1, 1], patch=None, f32=False, warm=1)
+    (tmp_path / "case.json").write_text(json.dumps(case))
+    r = subprocess.run([NODE, DRIVER, str(tmp_path)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    res = json.loads((tmp_path / "result.json").read_text())
+    its = {}
+    for order in (1, 0):
+        e = HipEngine(nx, ny, 1, g.h)
+        e.upload(osim.u, osim.v, osim.p, osim.label, osim.dye)
+        e.set_warm_extrapolation(order)
+        its[order] = [e.step("macCormack", dt, g.h * g.h / dt, dt / g.h, (1.0, 1.0, 1.0),
+                             "mg-fcg", 60, 1.0, 1e-6) for _ in range(steps)]
+        if order == 1:
+            u, v = e.read_velocity()
+        e.close()
+    assert res["iters"] == its[1], (res["iters"], its)
+    assert np.array_equal(np.fromfile(tmp_path / "out_u.f64", "<f8"), u)
+    assert np.array_equal(np.fromfile(tmp_path / "out_v.f64", "<f8"), v)
+    assert sum(its[1]) < sum(its[0]), its
+
