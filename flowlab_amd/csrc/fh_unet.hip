@@ -826,6 +826,17 @@ int fh_unet_dataset_reserve(fh_unet* u, int capacity, int H, int W) {
 
 int fh_unet_dataset_size(fh_unet* u) { return u ? u->count : 0; }
 
+int fh_unet_dataset_get(fh_unet* u, int index, float* x, float* t, float* scales) {
+  if (!u || index < 0 || index >= u->count) return FH_ERR_BAD_ARG;
+  UCHK(hipStreamSynchronize(u->s));
+  UCHK(hipDeviceSynchronize());
+  const size_t hw = (size_t)u->cw * u->ch;
+  if (x) UCHK(hipMemcpy(x, u->x + index * hw, hw * sizeof(float4), hipMemcpyDeviceToHost));
+  if (t) UCHK(hipMemcpy(t, u->t + index * hw, hw * 4, hipMemcpyDeviceToHost));
+  if (scales) UCHK(hipMemcpy(scales, u->sr + 2 * (size_t)index, 8, hipMemcpyDeviceToHost));
+  return FH_OK;
+}
+
 // c0 = RMS over the first samples' coarse cells holding fluid of the target
 // (correction / s_r).
 int fh_unet_calibrate(fh_unet* u, int samples) {

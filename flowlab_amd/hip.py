@@ -87,6 +87,7 @@ SIGNATURES = [
     ("fh_unet_backward", _i, [_vp, _i, _i, _i, _fp, _fp, _fp]),
     ("fh_unet_dataset_reserve", _i, [_vp, _i, _i, _i]),
     ("fh_unet_dataset_size", _i, [_vp]),
+    ("fh_unet_dataset_get", _i, [_vp, _i, _fp, _fp, _fp]),
     ("fh_unet_calibrate", _i, [_vp, _i]),
     ("fh_unet_train", _i, [_vp, _i, _i, _d, C.c_ulonglong, _dp]),
     ("fh_set_capture", _i, [_vp, _vp]),

@@ -108,10 +108,23 @@ class UNet:
     def reserve(self, capacity: int, H: int, W: int):
         """Room for `capacity` samples of an H x W coarse grid (coarse_grid)."""
         _ucheck(self.lib.fh_unet_dataset_reserve(self._u, capacity, H, W))
+        self._shape = (H, W)
 
     @property
     def samples(self) -> int:
         return self.lib.fh_unet_dataset_size(self._u)
+
+    def sample(self, index: int):
+        """Captured sample `index`: (x [H][W][4], t [H][W], (s_r, s_p))."""
+        n = self.lib.fh_unet_dataset_size(self._u)
+        if not 0 <= index < n:
+            raise IndexError(index)
+        H, W = self._shape
+        x = np.empty((H, W, 4), np.float32)
+        t = np.empty((H, W), np.float32)
+        sc = np.empty(2, np.float32)
+        _ucheck(self.lib.fh_unet_dataset_get(self._u, index, fptr(x), fptr(t), fptr(sc)))
+        return x, t, (float(sc[0]), float(sc[1]))
 
     def calibrate(self, samples: int = 64):
         _ucheck(self.lib.fh_unet_calibrate(self._u, samples))

@@ -269,6 +269,11 @@ int fh_unet_backward(fh_unet* u, int N, int H, int W, const float* in, const flo
  * p* - p over s_r) until `capacity` samples are held. */
 int fh_unet_dataset_reserve(fh_unet* u, int capacity, int H, int W);
 int fh_unet_dataset_size(fh_unet* u);
+/* Captured sample `index`: the network input x (4 floats per coarse cell:
+ * defect block sum / s_r, solid fraction, Air fraction, fluid-mean pressure
+ * / s_p), the target t (block-mean correction / s_r) and {s_r, s_p}; any
+ * output may be NULL. */
+int fh_unet_dataset_get(fh_unet* u, int index, float* x, float* t, float* scales);
 /* Sets c0 to the RMS of the captured targets (correction / s_r). */
 int fh_unet_calibrate(fh_unet* u, int samples);
 /* `steps` Adam steps (lr; beta 0.9/0.999) on batches of `batch`; per-step

@@ -94,6 +94,86 @@ def test_capture_train_and_warm_start():
     assert rel <= 1e-6
 
 
+def _block(a, F):
+    H, W = a.shape
+    return a.reshape(H // F, F, W // F, F).sum(axis=(1, 3))
+
+
+def test_learned_start_arithmetic_matches_host():
+    """One solve at 256^2 with both the capture and the learned start on:
+    the captured sample (the network's coarse input and target, F = 2) equals
+    numpy's gather of the same start, and the device's residual right after
+    the learned start equals the oracle's residual of the host's own start:
+    p + m q with q the network's output (run through the forward entry
+    point on that input) interpolated bilinearly onto the fluid cells, and m
+    the energy-optimal step."""
+    import os
+    from flowlab_amd import learned as Lrn
+    from flowlab_amd.engine import HipPressureSolver
+    u = Lrn.UNet(seed=5)
+    if os.path.exists(Lrn.WEIGHTS):
+        u.load(Lrn.WEIGHTS)
+    n = 256
+    F, cw, ch = Lrn.coarse_grid(n, n)
+    assert (F, cw, ch) == (2, 128, 128)
+    sim = Lrn._sim(n, 4242)
+    for _ in range(5):
+        sim.step()
+    e = sim.stepper.engine
+    p_prev = e.read_pressure()
+    sim.step()
+    div = e.read_divergence()
+    label = sim.f.label.copy()
+    scale = sim.g.h * sim.g.h / sim.dt
+    u.reserve(1, ch, cw)
+    solver = HipPressureSolver(n, n, "mg-dc")
+    solver.engine.set_capture(u)
+    solver.engine.set_warm_start(u)
+    p = p_prev.copy()
+    solver.solve(None, p, div, label, scale, 100, 1.0, 1e-6)
+    r_dev = float(solver.engine.solve_history()[0])
+    assert O.relative_residual(n, n, p, div, label, scale) <= 1e-6
+    x, t, (s_r, s_p) = u.sample(0)
+    # the coarse input and target
+    r32 = O.mg_residual(n, n, p_prev, -scale * div, label).astype(np.float32)
+    lab = label.reshape(n, n)
+    fluid = (lab == O.FLUID)
+    rs = _block(r32.reshape(n, n).astype(np.float64), F)
+    cnt = _block(fluid.astype(np.float64), F)
+    has = cnt > 0
+    assert abs(np.sqrt(np.mean(rs[has] ** 2)) / s_r - 1) < 1e-6
+    assert np.allclose(x[..., 0] * s_r, rs, rtol=1e-5, atol=1e-6 * np.abs(rs).max())
+    assert np.array_equal(x[..., 1], _block((lab == O.SOLID).astype(np.float32), F) / F ** 2)
+    assert np.array_equal(x[..., 2], _block((lab == O.AIR).astype(np.float32), F) / F ** 2)
+    pm = np.where(has, _block(np.where(fluid, p_prev.reshape(n, n), 0), F) / np.maximum(cnt, 1),
+                  0)
+    assert abs(np.sqrt(np.mean(pm[has] ** 2)) / s_p - 1) < 1e-5
+    assert np.allclose(x[..., 3] * s_p, pm, rtol=1e-5, atol=1e-6 * np.abs(pm).max())
+    dm = np.where(has, _block(np.where(fluid, (p - p_prev).reshape(n, n), 0), F)
+                  / np.maximum(cnt, 1), 0)
+    assert np.allclose(t * s_r, dm, rtol=1e-4, atol=1e-5 * np.abs(dm).max())
+    # the start the device solved from
+    xin = np.zeros((1, ch, cw, 8), np.float32)
+    xin[0, ..., :4] = x
+    o = u.forward(xin)[0].astype(np.float64)
+    yc = np.clip((np.arange(n) + 0.5) / F - 0.5, 0, ch - 1)
+    xc = np.clip((np.arange(n) + 0.5) / F - 0.5, 0, cw - 1)
+    y0, x0 = np.minimum(yc.astype(int), ch - 1), np.minimum(xc.astype(int), cw - 1)
+    y1, x1 = np.minimum(y0 + 1, ch - 1), np.minimum(x0 + 1, cw - 1)
+    fy, fx = (yc - y0)[:, None], (xc - x0)[None, :]
+    top = o[y0][:, x0] + fx * (o[y0][:, x1] - o[y0][:, x0])
+    bot = o[y1][:, x0] + fx * (o[y1][:, x1] - o[y1][:, x0])
+    q = (s_r * u.scale * (top + fy * (bot - top)) * fluid).ravel()
+    aq = O.mg_apply_a(n, n, q, label)
+    m = max(float(r32.astype(np.float64) @ q) / float(q @ aq), 0.0)
+    r_host = O.relative_residual(n, n, p_prev + m * q, div, label, scale)
+    r_start = O.relative_residual(n, n, p_prev, div, label, scale)
+    print(f"residual: start {r_start:.4e}, learned start device {r_dev:.4e} host {r_host:.4e}, "
+          f"m {m:.4f}")
+    assert m > 0
+    assert abs(r_dev - r_host) < 1e-4 * r_host
+
+
 def test_trained_network_on_held_out_scenes():
     """The committed trained network on scenes it was not trained on (seeds
     9101-9104; training used 100..433), 512 x 512 as in training, the next
