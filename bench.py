@@ -3,9 +3,11 @@
 Headline workload: the 8192x8192 thermal plume (67 M cells; closed box, 200
 seeded circular obstacles, 16 buoyant heat sources, buoyancy 1, MacCormack
 advection of velocity and 3 dye channels, dt = 0.5 h capped at CFL 0.5,
-every pressure solve to 1e-6 relative residual by one mixed-precision
-multigrid V(2,2) cycle to 16x8 followed by flexible CG preconditioned by it),
-200 steps from rest.
+every pressure solve to 1e-6 relative residual by mixed-precision
+defect-correction multigrid, V(2,2) cycles to 16x8, started from the last two
+solutions extrapolated linearly in time), 200 steps from rest.  (The Karman
+legs run flexible CG around the same V-cycle, faster there; `--solver` and
+`--c2-solver` choose.)
 
 A step is one fused fh_step over the resident fields (inputs already in
 HBM); the timed region is `--warmup` untimed steps after a reset, then
@@ -115,7 +117,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=0,
                     help="untimed steps after the reset, before the timed ones")
     ap.add_argument("--n", type=int, default=8192, help="side of the headline plume")
-    ap.add_argument("--solver", default="mg-fcg", choices=sorted(SOLVER_NAMES))
+    ap.add_argument("--solver", default="mg-dc", choices=sorted(SOLVER_NAMES),
+                    help="pressure solver of the headline plume")
+    ap.add_argument("--c2-solver", default="mg-fcg", choices=sorted(SOLVER_NAMES),
+                    help="pressure solver of the Karman street legs (C2 and through Node)")
     ap.add_argument("--warm", default="extrapolated", choices=sorted(WARM_STARTS),
                     help="start of each step's pressure solve (headline and C2)")
     ap.add_argument("--kernel", default=None,
@@ -409,13 +414,13 @@ def run_c2(args):
     from flowlab_amd import scenes as S
     nx, ny, K = 2048, 1024, args.c2_steps
     g, scene = S.karman_street(nx, ny)
-    params = dict(scheme="macCormack", cfl_target=0.5, solver=args.solver, pressure_iters=100,
+    params = dict(scheme="macCormack", cfl_target=0.5, solver=args.c2_solver, pressure_iters=100,
                   tol=1e-6, residual_every=0, warm_extrapolation=WARM_STARTS[args.warm][0])
     sim = make_sim(nx, ny, scene, params)
     e = sim.stepper.engine
     sim.step()
     tiles = e.tune_tiles(reps=10)["tiles"]
-    kernels = kernel_table(sim, nx * ny, args.solver, buoyancy=False, warm=args.warm)
+    kernels = kernel_table(sim, nx * ny, args.c2_solver, buoyancy=False, warm=args.warm)
     sim.reset(scene)
     sim.stepper.upload(sim.f)
     e.synchronize()
@@ -427,7 +432,7 @@ def run_c2(args):
     wl = instrumented(sim, scene, K)
     out = leg_summary(nx * ny, K, elapsed, wl, tiles)
     out["workload"] = ("karman vortex street 2048x1024, MacCormack, "
-                       f"{SOLVER_NAMES[args.solver]}, tol 1e-6, CFL 0.5, {K} steps, each solve "
+                       f"{SOLVER_NAMES[args.c2_solver]}, tol 1e-6, CFL 0.5, {K} steps, each solve "
                        f"from {WARM_STARTS[args.warm][1]}")
     out["other_start"] = other_start(sim, scene, K, wl, args.warm)
     out["kernels"] = kernels
@@ -555,7 +560,7 @@ def run_node_c2(args):
             np.asarray(getattr(f, name), "<f8").tofile(os.path.join(d, f"{name}.f64"))
         np.concatenate(f.dye).astype("<f8").tofile(os.path.join(d, "dye.f64"))
         f.label.tofile(os.path.join(d, "label.u8"))
-        case = dict(nx=nx, ny=ny, h=g.h, scheme="macCormack", solver=args.solver, tol=1e-6,
+        case = dict(nx=nx, ny=ny, h=g.h, scheme="macCormack", solver=args.c2_solver, tol=1e-6,
                     cfl=0.5, dtMax=1 / 30, warm=WARM_STARTS[args.warm][0])
         with open(os.path.join(d, "case.json"), "w") as fh:
             json.dump(case, fh)
@@ -565,7 +570,7 @@ def run_node_c2(args):
         return {"error": r.stderr[-500:]}
     out = json.loads(r.stdout.strip().splitlines()[-1])
     out["workload"] = (f"karman vortex street {nx}x{ny} through the N-API HipStepper from Node "
-                       f"{out.get('node')}, {SOLVER_NAMES[args.solver]}, tol 1e-6, CFL 0.5 from "
+                       f"{out.get('node')}, {SOLVER_NAMES[args.c2_solver]}, tol 1e-6, CFL 0.5 from "
                        "the host's one-step-stale velocity, each solve from "
                        f"{WARM_STARTS[args.warm][1]}; upload, readback and host CFL scan inside "
                        "the timed loop")
