@@ -583,7 +583,9 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
   e->hist.clear();
   Level& L0 = e->lv[0];
   const bool fp64 = solver == FH_SOLVER_MG_DC || solver == FH_SOLVER_MG_FCG;
-  const bool sync_init = fp64 && (e->capture || e->warm);
+  // a training capture reads the initial pass on the host; everything else
+  // (the learned start included) runs on the device behind the norm kernel
+  const bool sync_init = fp64 && e->capture;
   // The extrapolated start: formed by the FP64 solvers' fused initial pass,
   // otherwise by a pass of its own before anything reads p.
   const bool start_in_pass = sp.on && fp64 && div_fused && !sync_init;
@@ -605,7 +607,7 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
     // r = b - A p (no update) + norms.  The zero-divergence guard of
     // pressureMultigrid.ts:294-300 reads the same pass: on the device (the
     // norm sets `done`, so the cycles enqueued behind it skip themselves)
-    // unless a capture or learned warm start needs the norm on the host now.
+    // unless a training capture needs the norm on the host now.
     int st;
     if (sync_init) {
       launch_set_int(e->s, e->done, 0);
@@ -655,6 +657,19 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
       launch_dc_norm_init(e->s, e->partials, e->nx, e->ny, e->sums, e->d_res + 16, e->done,
                           scale);
       LAUNCHED("dc_update");
+      if (e->warm) {
+        // learned warm start: p += its correction of the start, then the
+        // defect and its norm again (the norm the loop reads is the start's
+        // after the correction; a zero divergence leaves p unchanged here)
+        st = engine_warm_start(e->warm, e->s, L0.b, e->lbl, L0.info, e->p, e->ny, e->nx,
+                               e->wstats);
+        if (st != FH_OK) return st;
+        launch_dc_correct(e->s, e->p, nullptr, nullptr, e->div, L0.info, L0.b, e->nx, e->ny,
+                          scale, e->partials, nullptr, nullptr);
+        launch_dc_norm_init(e->s, e->partials, e->nx, e->ny, e->sums, e->d_res + 16, e->done,
+                            scale);
+        LAUNCHED("dc_update");
+      }
       HIPCHK(hipEventRecord(e->cyc_ev[2], e->s));
     }
     // Cycles are enqueued one ahead of the host's convergence check: cycle
@@ -1237,10 +1252,9 @@ int fh_step(fh_engine* e, int scheme, double dt, double scale, double grad_scale
   if ((st = forces(e, dt, buoy))) return st;
   if ((st = mark(e, 2))) return st;
   // The FP64 solvers' initial defect pass forms the divergence itself
-  // (unless a capture or the learned start needs the defect on the host
-  // first); the other solvers read it from its own pass.
-  const bool fuse = (solver == FH_SOLVER_MG_DC || solver == FH_SOLVER_MG_FCG) && !e->capture &&
-                    !e->warm;
+  // (unless a training capture needs the defect on the host first); the
+  // other solvers read it from its own pass.
+  const bool fuse = (solver == FH_SOLVER_MG_DC || solver == FH_SOLVER_MG_FCG) && !e->capture;
   if (!fuse) {
     {
       Trace t(e, FH_K_DIVERGENCE);

@@ -16,6 +16,7 @@
 // step m >= 0 (k_warm_dots), so it never has a larger A-norm error than p.
 // Training minimises the mean squared error of o against the fluid mean of
 // (p* - p) over each block, over s_r c0, p* being the multigrid solution.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -78,16 +79,25 @@ __global__ void k_gather2(const float4* __restrict__ in, int cw, int ch, float4*
 // The network's input scales of one coarse grid: s_r, s_p = RMS of the
 // defect sum and of the fluid-mean pressure over coarse cells holding fluid
 // (1 where zero).  One workgroup of 1024 threads.
+// A coarse grid holds at most kUnetCoarseSide^2 = 16 x 1024 cells: each
+// thread loads its (up to) 16 cells at once, then sums them.
 __global__ __launch_bounds__(1024) void k_coarse_stats(const float4* __restrict__ c, int n,
                                                        float area, float* __restrict__ out2) {
+  constexpr int kPer = kUnetCoarseSide * kUnetCoarseSide / 1024;
   __shared__ double sh[3][16];
   double a[3] = {0.0, 0.0, 0.0};
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
-    const float4 v = c[k];
-    const float fl = area - v.y - v.z;
+  float4 v[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int k = threadIdx.x + q * 1024;
+    v[q] = k < n ? c[k] : make_float4(0.f, 0.f, area, 0.f);
+  }
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const float fl = area - v[q].y - v[q].z;
     if (fl < 0.5f) continue;
-    const double pm = (double)v.w / fl;
-    a[0] += (double)v.x * v.x;
+    const double pm = (double)v[q].w / fl;
+    a[0] += (double)v[q].x * v[q].x;
     a[1] += pm * pm;
     a[2] += 1.0;
   }
@@ -686,9 +696,11 @@ int engine_warm_start(fh_unet* u, hipStream_t s, const float* r32, const uint8_t
   unet_forward(net, s);
   k_prolong_out<<<dim3((W + 255) / 256, H), 256, 0, s>>>(net.out, cw, ch, F, info, W, H,
                                                          u->cstats, u->c0, u->q);
-  // enough blocks to fill the chip (the partials buffer holds kWarmBlocks)
-  k_warm_dots<<<kWarmBlocks, 256, 0, s>>>(r32, u->q, info, W, n, u->partials);
-  launch_finalize_sum4(s, u->partials, kWarmBlocks, u->sums);
+  // enough blocks to fill the chip, four cells per thread at least (the
+  // partials buffer holds kWarmBlocks)
+  const int dots_blocks = (int)std::min<long>(kWarmBlocks, std::max<long>(256, n / 1024));
+  k_warm_dots<<<dots_blocks, 256, 0, s>>>(r32, u->q, info, W, n, u->partials);
+  launch_finalize_sum4(s, u->partials, dots_blocks, u->sums);
   k_apply_warm<<<nblk(n), 256, 0, s>>>(p, u->q, u->sums, n);
   UCHK(hipGetLastError());
   return FH_OK;

@@ -9,6 +9,7 @@
 // V_MFMA_F32_16X16X32_BF16 multiplies a 16x32 weight block by a 32x16 patch
 // block; the D layout (4 consecutive channels of one pixel per lane) makes
 // the epilogue store 8 contiguous bytes per lane.
+#include <algorithm>
 #include <atomic>
 
 #include "fh_device.hpp"
@@ -208,10 +209,158 @@ __global__ __launch_bounds__(256) void k_conv3x3_lds(const bf16_t* __restrict__ 
   }
 }
 
+// The same convolution for images too small to fill the chip (the learned
+// warm start's coarse grid and its deeper levels): a workgroup takes one
+// output row of up to 16 * PT pixels and all COUT channels, and its four
+// waves split the reduction over K (wave w takes the 32-wide k-steps w,
+// w + 4, ...), each with its own loads in flight; waves 1..3 leave their
+// partial sums in LDS and wave 0 adds them in wave order, then applies the
+// bias and activation.  Four times the workgroups of k_conv3x3 and a quarter
+// of each wave's chain of dependent weight loads, which on a grid of a few
+// dozen workgroups (one or two per XCD, every weight load an L2 miss) is
+// what bounds it.  Same operands as k_conv3x3; the partial sums are added in
+// another order (FP32).  Dynamic LDS: 3 x PT x COUT/16 x 64 f32x4.
+template <int COUT, int PT>
+__global__ __launch_bounds__(256) void k_conv3x3_ksplit(const bf16_t* __restrict__ X, int cin,
+                                                        const bf16_t* __restrict__ Wt, int kpad,
+                                                        const float* __restrict__ bias, int N,
+                                                        int H, int Wd, int relu,
+                                                        bf16_t* __restrict__ Y,
+                                                        float* __restrict__ Yf) {
+  extern __shared__ unsigned char ks_smem[];
+  f32x4* red = reinterpret_cast<f32x4*>(ks_smem);
+  constexpr int CT = COUT / 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n = blockIdx.z, y = blockIdx.y, x0 = blockIdx.x * 16 * PT;
+  const int j = lane & 15, g = lane >> 4;
+  f32x4 acc[PT][CT];
+#pragma unroll
+  for (int p = 0; p < PT; p++)
+#pragma unroll
+    for (int t = 0; t < CT; t++) acc[p][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int kreal = 9 * cin;
+  const size_t plane = (size_t)H * Wd;
+  const bf16_t* Xn = X + (size_t)n * plane * cin;
+  // two k-steps per trip, every load of both issued before their MFMAs
+  for (int k0 = 32 * wave; k0 < kpad; k0 += 256) {
+    bf16x8 a[2][CT], b[2][PT];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int k = k0 + 128 * h + 8 * g;
+      const bool live = k0 + 128 * h < kpad;
+      if (live && k < kreal) {
+        const int tap = k / cin, ci = k - tap * cin;
+        const int yy = y + tap / 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+        for (int p = 0; p < PT; p++) {
+          const int xx = x0 + 16 * p + j + dx;
+          b[h][p] = (yy >= 0 && yy < H && xx >= 0 && xx < Wd)
+                        ? load8(Xn + ((size_t)yy * Wd + xx) * cin + ci)
+                        : zero8();
+        }
+      } else {
+#pragma unroll
+        for (int p = 0; p < PT; p++) b[h][p] = zero8();
+      }
+#pragma unroll
+      for (int t = 0; t < CT; t++)
+        a[h][t] = live ? load8(Wt + (size_t)(16 * t + j) * kpad + k) : zero8();
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+      for (int t = 0; t < CT; t++)
+#pragma unroll
+        for (int p = 0; p < PT; p++)
+          acc[p][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[h][t], b[h][p], acc[p][t], 0, 0, 0);
+  }
+  if (wave > 0) {
+#pragma unroll
+    for (int p = 0; p < PT; p++)
+#pragma unroll
+      for (int t = 0; t < CT; t++) red[(((wave - 1) * PT + p) * CT + t) * 64 + lane] = acc[p][t];
+  }
+  __syncthreads();
+  if (wave > 0) return;
+#pragma unroll
+  for (int w = 0; w < 3; w++)
+#pragma unroll
+    for (int p = 0; p < PT; p++)
+#pragma unroll
+      for (int t = 0; t < CT; t++) acc[p][t] += red[((w * PT + p) * CT + t) * 64 + lane];
+#pragma unroll
+  for (int p = 0; p < PT; p++) {
+    const int x = x0 + 16 * p + j;
+    if (x >= Wd) continue;
+    const size_t pix = (size_t)n * plane + (size_t)y * Wd + x;
+#pragma unroll
+    for (int t = 0; t < CT; t++) {
+      const int co = 16 * t + 4 * g;
+      float v[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        v[r] = acc[p][t][r] + (bias ? bias[co + r] : 0.f);
+        if (relu) v[r] = fmaxf(v[r], 0.f);
+      }
+      if (Yf) {
+        if (co == 0) Yf[pix] = v[0];
+      } else {
+        uint2 packed;
+        packed.x = (uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16);
+        packed.y = (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16);
+        *reinterpret_cast<uint2*>(Y + pix * COUT + co) = packed;
+      }
+    }
+  }
+}
+
+// Workgroups of k_conv3x3 below which the small-image variant runs.
+constexpr int kConvSmallGrid = 512;
+
+template <int COUT>
+static bool launch_conv_small(hipStream_t s, const bf16_t* X, int cin, const bf16_t* Wt, int kpad,
+                              const float* bias, int N, int H, int W, int relu, bf16_t* Y,
+                              float* Yf) {
+  constexpr int CT = COUT / 16;
+  // pixel tiles per row: as many as the row needs, at most 16 accumulator
+  // tiles per lane (PT * CT)
+  constexpr int kMaxPT = 16 / CT >= 4 ? 4 : 16 / CT >= 2 ? 2 : 1;
+  const int PT = std::min(W <= 16 ? 1 : W <= 32 ? 2 : 4, kMaxPT);
+  const dim3 g((W + 16 * PT - 1) / (16 * PT), H, N);
+  const size_t lds = (size_t)3 * PT * CT * 64 * sizeof(f32x4);
+  if (lds > (size_t)device_max_lds()) return false;
+  if (PT == 1) {
+    k_conv3x3_ksplit<COUT, 1><<<g, 256, lds, s>>>(X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf);
+  } else if (PT == 2) {
+    if constexpr (kMaxPT >= 2)
+      k_conv3x3_ksplit<COUT, 2><<<g, 256, lds, s>>>(X, cin, Wt, kpad, bias, N, H, W, relu, Y,
+                                                    Yf);
+  } else {
+    if constexpr (kMaxPT >= 4)
+      k_conv3x3_ksplit<COUT, 4><<<g, 256, lds, s>>>(X, cin, Wt, kpad, bias, N, H, W, relu, Y,
+                                                    Yf);
+  }
+  return true;
+}
+
 void launch_conv3x3(hipStream_t s, const bf16_t* X, int cin, const bf16_t* Wt, int kpad,
                     const float* bias, int cout, int N, int H, int W, int relu, bf16_t* Y,
                     float* Yf) {
   dim3 g((W + 63) / 64, (H + 3) / 4, N);
+  if ((long)g.x * g.y * g.z < kConvSmallGrid && cin % 8 == 0 &&
+      (((uintptr_t)X) & 15) == 0) {
+    bool done = false;
+    switch (cout) {
+      case 16: done = launch_conv_small<16>(s, X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); break;
+      case 32: done = launch_conv_small<32>(s, X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); break;
+      case 48: done = launch_conv_small<48>(s, X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); break;
+      case 64: done = launch_conv_small<64>(s, X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); break;
+      case 96: done = launch_conv_small<96>(s, X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); break;
+      case 128: done = launch_conv_small<128>(s, X, cin, Wt, kpad, bias, N, H, W, relu, Y, Yf); break;
+      default: break;
+    }
+    if (done) return;
+  }
   const int max_lds = device_max_lds();
   const size_t lds = (size_t)kConvRows * kConvCols * cin * sizeof(bf16_t);
   if (cin % 8 == 0 && lds <= (size_t)max_lds && (((uintptr_t)X) & 15) == 0) {
