@@ -481,6 +481,7 @@ def run_c3(args):
     from flowlab_amd.batch import HipBatch
     scenes = S.turbulence_batch(64)
     b = HipBatch([g for g, _ in scenes])
+    b.set_warm_extrapolation(WARM_STARTS[args.warm][0])
     for s, (_, sc) in enumerate(scenes):
         b.reset(s, sc)
     kw = dict(scheme="macCormack", cfl=0.5, dt_max=1.0, tol=1e-6, max_cycles=100)
@@ -503,7 +504,8 @@ def run_c3(args):
     return {
         "workload": f"{len(scenes)} decaying-turbulence scenes, sizes {sizes} drawn by "
                     "SplitMix64(20240612), square-obstacle lattices, k^-3 velocity RMS 1, "
-                    f"MacCormack, {SOLVER_NAMES['mg-fcg']}, tol 1e-6, CFL 0.5, {K} steps",
+                    f"MacCormack, {SOLVER_NAMES['mg-fcg']}, tol 1e-6, CFL 0.5, {K} steps, each "
+                    f"solve from {WARM_STARTS[args.warm][1]}",
         "cells": cells, "steps": K, "value": cells * K / elapsed, "unit": "cell-steps/s",
         "ms_per_step": elapsed * 1e3 / K, "vcycles_per_solve_mean": sum(iters) / len(iters),
         "max_relative_residual": res_max, "mg_tiles": tune["tiles"],

@@ -66,6 +66,11 @@ class HipBatch:
         hip.check_batch(hip.lib().fh_batch_set_forces(self._b, s, buoyancy, confinement,
                                                       hip.dptr(d)))
 
+    def set_warm_extrapolation(self, order: int) -> None:
+        """Start every scene's solve from its last solutions extrapolated in
+        time (1 linear, 2 quadratic; 0 = the previous pressure)."""
+        hip.check_batch(hip.lib().fh_batch_set_warm_extrapolation(self._b, int(order)))
+
     def step(self, scheme="macCormack", cfl=0.5, dt_max=1.0, rho=1.0, tol=1e-6,
              max_cycles=60) -> None:
         hip.check_batch(hip.lib().fh_batch_step(self._b, hip.SCHEMES[scheme], cfl, dt_max, rho,

@@ -185,7 +185,7 @@ int prepare(fh_batch* b) {
     d.uh = e->uh, d.vh = e->vh;
     d.dye[0] = e->dye, d.dye[1] = e->dye2, d.dyeh = e->dyeh, d.w = e->w;
     d.lbl = e->lbl, d.sten_f = e->sten_f, d.sten_d = e->sten_d;
-    d.p[0] = e->p, d.p[1] = e->p2, d.div = e->div, d.rvec = e->rvec, d.fcg = e->fcg;
+    d.div = e->div, d.rvec = e->rvec, d.fcg = e->fcg;
     d.dvec[0] = e->dvec[0], d.dvec[1] = e->dvec[1];
     d.partials = e->partials, d.sums = e->sums, d.d_res = e->d_res, d.done = e->done;
     const dim3 rg = batch_grid_reduce(e->nx, e->ny);
@@ -314,7 +314,7 @@ int solve(fh_batch* b, double tol, int max_cycles) {
       launch_batch_dc_update(b->s, b->d_desc, b->g_red, b->d_step, q, 0);
       launch_batch_finalize(b->s, b->d_desc, ns, b->d_step, 1, 0, tol);
     } else {
-      launch_batch_fcg(b->s, b->d_desc, b->g_red, ns, q, c & 1, c == 1 ? 1 : 0);
+      launch_batch_fcg(b->s, b->d_desc, b->g_red, b->d_step, ns, c & 1, c == 1 ? 1 : 0);
       launch_batch_finalize(b->s, b->d_desc, ns, b->d_step, 2, c & 1, tol);
     }
     BLAUNCHED("batched cycle");
@@ -350,7 +350,7 @@ int solve(fh_batch* b, double tol, int max_cycles) {
   }
   for (int s = 0; s < ns; s++)
     if (zero[s])
-      BCHK(hipMemsetAsync(b->desc[s].p[q ^ 1], 0, (size_t)b->desc[s].nx * b->desc[s].ny * 8,
+      BCHK(hipMemsetAsync(b->h_step[s].pr, 0, (size_t)b->desc[s].nx * b->desc[s].ny * 8,
                           b->s));
   return FH_OK;
 }
@@ -534,6 +534,22 @@ int fh_batch_step(fh_batch* b, int scheme, double cfl, double dt_max, double rho
     t.scale = rho * P.h * P.h / dt;
     t.grad_scale = dt / (rho * P.h);
     t.inv_h = 1.0 / e->h;
+    // the pressure buffers as fh_step would use them (the start p or its
+    // extrapolation in time into p2, the result where fh_step's first
+    // cycle puts it)
+    const StartPlan sp = plan_start(b->e[s], dt);
+    fh_engine* es = b->e[s];
+    t.p0 = es->p;
+    t.xp = sp.xp;
+    t.xo = sp.order;
+    if (sp.on) {
+      t.ps = es->p2;
+      t.pr = es->extrap >= 2 ? es->ph[1] : es->ph[0];
+    } else {
+      t.ps = nullptr;
+      t.pr = es->p2;
+      es->ph_n = 0;
+    }
     b->dt[s] = dt;
   }
   BCHK(hipMemcpyAsync(b->d_step, b->h_step, ns * sizeof(BStep), hipMemcpyHostToDevice, b->s));
@@ -550,7 +566,10 @@ int fh_batch_step(fh_batch* b, int scheme, double cfl, double dt_max, double rho
   if (!mac && b->g_proj[kBBuoyancy].blocks)
     launch_batch_buoyancy(b->s, D, b->g_proj[kBBuoyancy], S, q);
   BLAUNCHED("batched advection and forces");
-  if ((st = solve(b, tol, max_cycles)) != FH_OK) return st;
+  if ((st = solve(b, tol, max_cycles)) != FH_OK) {
+    for (fh_engine* e : b->e) drop_pressure_history(e);
+    return st;
+  }
   launch_batch_project(b->s, D, b->g_proj[kBProject], S, q);
   // the next step's CFL reads this velocity; the dye phase leaves it alone
   launch_batch_max_speed(b->s, D, b->g_speed, q, ns, b->d_fpart, b->d_umax);
@@ -560,17 +579,47 @@ int fh_batch_step(fh_batch* b, int scheme, double cfl, double dt_max, double rho
   launch_batch_advect_dye(b->s, D, b->g_adv[kBAdvectDye], S, q, mac ? 1 : 0);
   if (mac) launch_batch_maccormack_dye(b->s, D, b->g_adv[kBMacCormackDye], S, q);
   BLAUNCHED("batched dye");
-  // the engines' own view: the step's results are their current fields
-  for (fh_engine* e : b->e) {
+  // the engines' own view: the step's results are their current fields,
+  // and with the extrapolated start the history rotates as fh_step's does
+  for (int s = 0; s < ns; s++) {
+    fh_engine* e = b->e[s];
+    const BStep& t = b->h_step[s];
     swap_ptr(e->u, e->u2);
     swap_ptr(e->v, e->v2);
-    swap_ptr(e->p, e->p2);
     swap_ptr(e->dye, e->dye2);
+    if (t.ps) {
+      double* last = e->p;
+      e->p = t.pr;
+      e->p2 = t.ps;
+      if (e->extrap >= 2) {
+        e->ph[1] = e->ph[0];
+        e->ph_dt[1] = e->ph_dt[0];
+      }
+      e->ph[0] = last;
+      e->ph_dt[0] = e->p_dt;
+      e->ph_n = std::min(e->ph_n + 1, e->extrap);
+    } else {
+      swap_ptr(e->p, e->p2);
+    }
+    e->p_dt = b->dt[s];
     e->vel_epoch++;
   }
   b->q ^= 1;
   b->last_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0)
                    .count();
+  return FH_OK;
+}
+
+int fh_batch_set_warm_extrapolation(fh_batch* b, int order) {
+  BGUARD("HipBatch.setWarmExtrapolation()");
+  BCHK(hipStreamSynchronize(b->s));
+  for (fh_engine* e : b->e) {
+    const int st = fh_set_warm_extrapolation(e, order);
+    if (st != FH_OK) {
+      b_err = fh_last_error();
+      return st;
+    }
+  }
   return FH_OK;
 }
 

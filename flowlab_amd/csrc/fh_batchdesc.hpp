@@ -32,15 +32,16 @@ struct LevelDesc {
 };
 
 // A scene's buffers.  Pairs that the single-grid step swaps once per step
-// (u, v, p, dye) are held as [2] and indexed by the step's parity q: the
-// step reads [q] and leaves its result in [q ^ 1].
+// (u, v, dye) are held as [2] and indexed by the step's parity q: the step
+// reads [q] and leaves its result in [q ^ 1].  The pressure buffers of a
+// step are in its BStep (they rotate with the extrapolated start's history).
 struct SceneDesc {
   int nx, ny;
   float *u[2], *v[2], *uh, *vh;
   float *dye[2], *dyeh, *w;
   const uint8_t* lbl;
   uint8_t *sten_f, *sten_d;
-  double *p[2], *div, *rvec, *fcg;
+  double *div, *rvec, *fcg;
   float* dvec[2];
   double *partials, *sums, *d_res;
   int* done;
@@ -61,6 +62,16 @@ struct BStep {
   float gain, eps;     // buoyancy * dt, confinement
   float hf, dtf, inv_hf;
   double scale, grad_scale, inv_h;
+  // The step's pressure buffers, as fh_step uses them: the initial pass
+  // reads p0 (the last solution) and, with the extrapolated start
+  // (plan_start), the history in xp, and stores the start it forms to ps;
+  // the first cycle's update reads the start (ps, or p0 when ps is null) and
+  // writes pr, where the solve's result stays.
+  const double* p0;
+  double* ps;
+  double* pr;
+  Extrapolation xp;
+  int xo;  // order of the extrapolated start (ps non-null)
 };
 
 struct BSpan {
@@ -124,10 +135,10 @@ void launch_batch_project(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
 // partials holds kBatchSpeedBlocks floats per scene.
 void launch_batch_max_speed(hipStream_t s, const SceneDesc* D, const BatchGrid& g, int q,
                             int nscenes, float* partials, float* host_out);
-// The defect pass: init 1 (the pass before the first cycle: p [q]
-// unchanged, no correction), init 2 (the same, forming the divergence of
-// velocity [q ^ 1] in the pass and storing it) or init 0, the first cycle's
-// update (p [q ^ 1] = p [q] + e).
+// The defect pass: init 1 (the pass before the first cycle: p0 unchanged,
+// no correction), init 2 (the same, forming the divergence of velocity
+// [q ^ 1] in the pass and storing it, and the extrapolated start when the
+// step has one) or init 0, the first cycle's update (pr = start + e).
 void launch_batch_dc_update(hipStream_t s, const SceneDesc* D, const BatchGrid& g,
                             const BStep* st, int q, int init);
 // One block per scene: the norm of the last defect / CG pass into sums and
@@ -135,9 +146,9 @@ void launch_batch_dc_update(hipStream_t s, const SceneDesc* D, const BatchGrid& 
 // pass, 1 a defect-correction cycle, 2 a CG cycle.
 void launch_batch_finalize(hipStream_t s, const SceneDesc* D, int nscenes, const BStep* st,
                            int mode, int slot, double tol);
-// Flexible CG of cycle c (cp = c & 1, first = c == 1) on p [q ^ 1].
-void launch_batch_fcg(hipStream_t s, const SceneDesc* D, const BatchGrid& g, int nscenes,
-                      int q, int cp, int first);
+// Flexible CG of cycle c (cp = c & 1, first = c == 1) on each scene's pr.
+void launch_batch_fcg(hipStream_t s, const SceneDesc* D, const BatchGrid& g, const BStep* st,
+                      int nscenes, int cp, int first);
 // Multigrid level l of every scene with l < lc, in tile shape `shape`.
 void launch_batch_smooth_restrict(hipStream_t s, int shape, const SceneDesc* D,
                                   const BatchGrid& g, int l);

@@ -292,6 +292,58 @@ void build_pseudo_inverse(int nx, int ny, const std::vector<uint8_t>& lab, std::
   }
 }
 
+// Forgets the pressure history of the extrapolated warm start: the resident
+// p is no longer the last fh_step solve's result.
+void drop_pressure_history(fh_engine* e) {
+  e->p_dt = 0.0;
+  e->ph_n = 0;
+}
+
+// Lagrange weights of the extrapolation to tau after the newest pressure:
+// nodes 0 (p), -a (h0, a = dt of the step that produced p) and -(a + b)
+// (h1, b = dt of the step that produced h0); order = nodes used - 1.
+void extrapolation_weights(int order, double tau, double a, double b, double w[3]) {
+  w[0] = 1.0;
+  w[1] = w[2] = 0.0;
+  if (order == 1) {
+    w[0] = 1.0 + tau / a;
+    w[1] = -tau / a;
+  } else if (order == 2) {
+    w[0] = (tau + a) * (tau + a + b) / (a * (a + b));
+    w[1] = -tau * (tau + a + b) / (a * b);
+    w[2] = tau * (tau + a) / (b * (a + b));
+  }
+}
+
+StartPlan plan_start(fh_engine* e, double dt) {
+  StartPlan sp;
+  if (e->extrap <= 0 || !(dt > 0.0) || !(e->p_dt > 0.0)) return sp;
+  sp.on = true;
+  sp.order = std::min(e->ph_n, e->extrap);
+  double w[3];
+  extrapolation_weights(sp.order, dt, e->p_dt, e->ph_dt[0], w);
+  sp.xp = Extrapolation{e->ph[0], e->ph[1], w[0], w[1], w[2]};
+  return sp;
+}
+
+// After the start is in p2: it becomes p, the old p (the last solution)
+// becomes the newest history entry, and the buffer of the entry that falls
+// out of the history becomes p2.
+void rotate_history(fh_engine* e) {
+  double* last = e->p;
+  e->p = e->p2;
+  if (e->extrap >= 2) {
+    e->p2 = e->ph[1];
+    e->ph[1] = e->ph[0];
+    e->ph_dt[1] = e->ph_dt[0];
+  } else {
+    e->p2 = e->ph[0];
+  }
+  e->ph[0] = last;
+  e->ph_dt[0] = e->p_dt;
+  e->ph_n = std::min(e->ph_n + 1, e->extrap);
+}
+
 }  // namespace fh
 
 namespace {
@@ -477,68 +529,6 @@ double rel_resid(const double* s, double scale, bool scaled_r) {
   double rr = std::sqrt(s[0] / s[1]);
   if (scaled_r) rr /= std::fabs(scale);
   return rr / std::sqrt(s[2] / s[3]);
-}
-
-// Forgets the pressure history of the extrapolated warm start: the resident
-// p is no longer the last fh_step solve's result.
-void drop_pressure_history(fh_engine* e) {
-  e->p_dt = 0.0;
-  e->ph_n = 0;
-}
-
-// Lagrange weights of the extrapolation to tau after the newest pressure:
-// nodes 0 (p), -a (h0, a = dt of the step that produced p) and -(a + b)
-// (h1, b = dt of the step that produced h0); order = nodes used - 1.
-void extrapolation_weights(int order, double tau, double a, double b, double w[3]) {
-  w[0] = 1.0;
-  w[1] = w[2] = 0.0;
-  if (order == 1) {
-    w[0] = 1.0 + tau / a;
-    w[1] = -tau / a;
-  } else if (order == 2) {
-    w[0] = (tau + a) * (tau + a + b) / (a * (a + b));
-    w[1] = -tau * (tau + a + b) / (a * b);
-    w[2] = tau * (tau + a) / (b * (a + b));
-  }
-}
-
-// The start of an fh_step solve of step size dt: with the extrapolation on
-// and the resident p the last step's solution, the start is p extrapolated
-// over the history (order: the entries there are, at most the set order; 0
-// while the history is empty), formed into p2 by the solve's first pass.
-struct StartPlan {
-  bool on = false;
-  int order = 0;
-  Extrapolation xp{};
-};
-
-StartPlan plan_start(fh_engine* e, double dt) {
-  StartPlan sp;
-  if (e->extrap <= 0 || !(dt > 0.0) || !(e->p_dt > 0.0)) return sp;
-  sp.on = true;
-  sp.order = std::min(e->ph_n, e->extrap);
-  double w[3];
-  extrapolation_weights(sp.order, dt, e->p_dt, e->ph_dt[0], w);
-  sp.xp = Extrapolation{e->ph[0], e->ph[1], w[0], w[1], w[2]};
-  return sp;
-}
-
-// After the start is in p2: it becomes p, the old p (the last solution)
-// becomes the newest history entry, and the buffer of the entry that falls
-// out of the history becomes p2.
-void rotate_history(fh_engine* e) {
-  double* last = e->p;
-  e->p = e->p2;
-  if (e->extrap >= 2) {
-    e->p2 = e->ph[1];
-    e->ph[1] = e->ph[0];
-    e->ph_dt[1] = e->ph_dt[0];
-  } else {
-    e->p2 = e->ph[0];
-  }
-  e->ph[0] = last;
-  e->ph_dt[0] = e->p_dt;
-  e->ph_n = std::min(e->ph_n + 1, e->extrap);
 }
 
 // PressureSolver.solve on the resident p (FP64), div (FP64), label.

@@ -135,6 +135,24 @@ namespace fh {
 // Makes an engine ready for the batched step (fh_batch.hip): its multigrid
 // hierarchy current for its labels and the flexible-CG vectors allocated.
 int engine_batch_prepare(fh_engine* e);
+
+// The start of an fh_step solve of step size dt: with the extrapolation on
+// (fh_set_warm_extrapolation) and the resident p the last step's solution,
+// the start is p extrapolated over the history (order: the entries there
+// are, at most the set order; 0 while the history is empty), formed into p2
+// by the solve's first pass; off, the start is p itself.
+struct StartPlan {
+  bool on = false;
+  int order = 0;
+  Extrapolation xp{};
+};
+StartPlan plan_start(fh_engine* e, double dt);
+// After the start is in p2: it becomes p, the old p (the last solution)
+// becomes the newest history entry, and the buffer of the entry that falls
+// out of the history becomes p2.
+void rotate_history(fh_engine* e);
+// The resident p is no longer the last fh_step solve's result.
+void drop_pressure_history(fh_engine* e);
 // The coarse sub-cycle's description of levels [lc, nlev) of an engine.
 CoarseLevels engine_coarse_levels(const fh_engine* e, int lc, int nlev);
 }  // namespace fh

@@ -1011,18 +1011,36 @@ __global__ __launch_bounds__(256) void kb_dc_update(const SceneDesc* __restrict_
   int s, bx, by, gy, bid;
   if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
   const SceneDesc& d = D[s];
+  const BStep& t = st[s];
   if (init == 2) {  // the initial pass with the divergence of velocity [q ^ 1]
-    dc_update_body<kRedRows, true>(gm(d.p[q]), nullptr, nullptr, nullptr, gm(d.lv[0].info),
-                                   gm(d.lv[0].b), nullptr, d.nx, d.ny, st[s].scale,
-                                   gm(d.partials), nullptr, bx, by, gy, bid, gm(d.u[q ^ 1]),
-                                   gm(d.v[q ^ 1]), st[s].inv_h, gm(d.div));
+    Extrapolation x = t.xp;
+    x.h0 = x.h0 ? gm(x.h0) : nullptr;
+    x.h1 = x.h1 ? gm(x.h1) : nullptr;
+    double* ps = t.ps ? gm(t.ps) : nullptr;
+    const int X = ps ? t.xo : 0;
+    if (X == 1)
+      dc_update_body<kRedRows, true, 1>(gm(t.p0), ps, nullptr, nullptr, gm(d.lv[0].info),
+                                        gm(d.lv[0].b), nullptr, d.nx, d.ny, t.scale,
+                                        gm(d.partials), nullptr, bx, by, gy, bid, gm(d.u[q ^ 1]),
+                                        gm(d.v[q ^ 1]), t.inv_h, gm(d.div), x);
+    else if (X == 2)
+      dc_update_body<kRedRows, true, 2>(gm(t.p0), ps, nullptr, nullptr, gm(d.lv[0].info),
+                                        gm(d.lv[0].b), nullptr, d.nx, d.ny, t.scale,
+                                        gm(d.partials), nullptr, bx, by, gy, bid, gm(d.u[q ^ 1]),
+                                        gm(d.v[q ^ 1]), t.inv_h, gm(d.div), x);
+    else
+      dc_update_body<kRedRows, true, 0>(gm(t.p0), ps, nullptr, nullptr, gm(d.lv[0].info),
+                                        gm(d.lv[0].b), nullptr, d.nx, d.ny, t.scale,
+                                        gm(d.partials), nullptr, bx, by, gy, bid, gm(d.u[q ^ 1]),
+                                        gm(d.v[q ^ 1]), t.inv_h, gm(d.div), x);
     return;
   }
-  double* pn = init ? nullptr : gm(d.p[q ^ 1]);
+  const double* start = t.ps ? t.ps : t.p0;
+  double* pn = init ? nullptr : gm(t.pr);
   const float* e = init ? nullptr : gm(d.lv[0].x);
   const int* done = init ? nullptr : gm(d.done);
-  dc_update_body<kRedRows>(gm(d.p[q]), pn, e, gm(d.div), gm(d.lv[0].info), gm(d.lv[0].b),
-                           init ? nullptr : gm(d.rvec), d.nx, d.ny, st[s].scale, gm(d.partials),
+  dc_update_body<kRedRows>(gm(start), pn, e, gm(d.div), gm(d.lv[0].info), gm(d.lv[0].b),
+                           init ? nullptr : gm(d.rvec), d.nx, d.ny, t.scale, gm(d.partials),
                            done, bx, by, gy, bid);
 }
 
@@ -1054,11 +1072,12 @@ __global__ __launch_bounds__(1024) void kb_fcg_coef(const SceneDesc* __restrict_
 }
 
 __global__ __launch_bounds__(256) void kb_fcg_update(const SceneDesc* __restrict__ D,
-                                                     BatchGrid g, int q, int cp, int first) {
+                                                     BatchGrid g, const BStep* __restrict__ st,
+                                                     int cp, int first) {
   int s, bx, by, gy, bid;
   if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
   const SceneDesc& d = D[s];
-  fcg_update_body<kRedRows>(gm(d.lv[0].x), gm(d.dvec[cp]), gm(d.dvec[cp ^ 1]), gm(d.p[q ^ 1]),
+  fcg_update_body<kRedRows>(gm(d.lv[0].x), gm(d.dvec[cp]), gm(d.dvec[cp ^ 1]), gm(st[s].pr),
                             gm(d.rvec), gm(d.lv[0].info), gm(d.lv[0].b), gm(d.fcg), d.nx, d.ny,
                             first, gm(d.partials), gm(d.done), bx, by, gy, bid);
 }
@@ -1075,11 +1094,11 @@ void launch_batch_finalize(hipStream_t s, const SceneDesc* D, int nscenes, const
   kb_finalize<<<nscenes, 1024, 0, s>>>(D, st, mode, slot, tol);
 }
 
-void launch_batch_fcg(hipStream_t s, const SceneDesc* D, const BatchGrid& g, int nscenes, int q,
-                      int cp, int first) {
+void launch_batch_fcg(hipStream_t s, const SceneDesc* D, const BatchGrid& g, const BStep* st,
+                      int nscenes, int cp, int first) {
   kb_fcg_dots<<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, cp, first);
   kb_fcg_coef<<<nscenes, 1024, 0, s>>>(D, first);
-  kb_fcg_update<<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, q, cp, first);
+  kb_fcg_update<<<g.blocks, dim3(BX, BY), 0, s>>>(D, g, st, cp, first);
 }
 
 }  // namespace fh

@@ -483,13 +483,13 @@ __global__ __launch_bounds__(256) void kb_buoyancy(const SceneDesc* __restrict__
   buoyancy_rows(gm(d.v[q ^ 1]), gm(d.lbl), gm(d.dye[q]), d.nx, d.ny, 1, gain, bx, by);
 }
 
-// Gradient subtraction with the pressure the solve left in p [q ^ 1].
+// Gradient subtraction with the pressure the solve left in the step's pr.
 __global__ __launch_bounds__(256) void kb_project(const SceneDesc* __restrict__ D, BatchGrid g,
                                                   const BStep* __restrict__ st, int q) {
   int s, bx, by, gy, bid;
   if (!batch_block(g, &s, &bx, &by, &gy, &bid)) return;
   const SceneDesc& d = D[s];
-  subtract_outflow_rows(gm(d.u[q ^ 1]), gm(d.v[q ^ 1]), gm(d.p[q ^ 1]), gm(d.lbl), d.nx, d.ny,
+  subtract_outflow_rows(gm(d.u[q ^ 1]), gm(d.v[q ^ 1]), gm(st[s].pr), gm(d.lbl), d.nx, d.ny,
                         st[s].grad_scale, bx, by);
 }
 

@@ -101,6 +101,7 @@ SIGNATURES = [
     ("fh_batch_set_forces", _i, [_vp, _i, _d, _d, _dp]),
     ("fh_batch_step", _i, [_vp, _i, _d, _d, _d, _d, _i]),
     ("fh_batch_results", _i, [_vp, _dp, _ip, _dp]),
+    ("fh_batch_set_warm_extrapolation", _i, [_vp, _i]),
     ("fh_batch_read", _i, [_vp, _i, _dp, _dp, _dp, _dp]),
     ("fh_batch_last_step_ms", _i, [_vp, _fp]),
     ("fh_batch_cells", C.c_longlong, [_vp]),

@@ -330,6 +330,10 @@ int fh_batch_step(fh_batch* b, int scheme, double cfl, double dt_max, double rho
                   int max_cycles);
 /* Per scene of the last step (arrays of nscenes, any may be NULL). */
 int fh_batch_results(fh_batch* b, double* dt, int* iters, double* resid);
+/* Every scene's solve starts from its last solutions extrapolated in time,
+ * as fh_set_warm_extrapolation makes fh_step's (0: the previous pressure);
+ * each scene keeps its own history, restarted by its fh_batch_upload. */
+int fh_batch_set_warm_extrapolation(fh_batch* b, int order);
 int fh_batch_read(fh_batch* b, int scene, double* u, double* v, double* p, double* dye);
 int fh_batch_last_step_ms(fh_batch* b, float* ms);
 long long fh_batch_cells(fh_batch* b);
