@@ -51,10 +51,13 @@ WORKLOAD_STEPS = 200
 #   maccormack_dye: u, v 8 + 3 channels of q^n and of the forward pass 24 +
 #     3 out 12 + label 1 + recorded stencils 1; advect_dye: u, v 8 + 3 in 12 +
 #     3 out 12 + label 1 + recorded stencils 1.
-#   dc_update: a cycle's correction pass reads e 4 + p 8 + div 8 + label 1
-#     and writes p 8 + r32 4 (33); under the flexible-CG solver it also
-#     writes the FP64 residual (+8).  (The solve's initial defect pass is the
-#     divergence kernel below.)
+#   dc_update: a defect-correction cycle's update, fused with the finest
+#     prolongation + post-smoothing (the engine's default): the pre-smoothed
+#     x 4 + b 4 + label 1 + the coarse correction 1 + p 8 + div 8 read, p 8 +
+#     r32 4 written (38); as its own pass (FUSED_UPDATE False) e 4 + p 8 +
+#     div 8 + label 1 read, p 8 + r32 4 written (33); under the flexible-CG
+#     solver either also writes the FP64 residual (+8).  (The solve's initial
+#     defect pass is the divergence kernel below.)
 #   smooth_restrict_fine: b 4 + label 1 + x 4 written + coarse b 1.
 #   prolong_smooth_fine: x 4 + b 4 + label 1 + coarse e 1 + x 4 written.
 #   fcg_dots: z 4 + previous direction 4 + r 8 + info 1.
@@ -83,6 +86,9 @@ SOLVER_NAMES = {
     "mg-fcg": "one MG-DC V(2,2) cycle (to 16x8), then flexible CG preconditioned by it",
 }
 PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_plume_8192.json")
+# the engines fuse each defect-correction cycle's finest prolongation +
+# smoothing with its update (fh_set_fused_update), traced as dc_update
+FUSED_UPDATE = True
 # start of each step's solve: the previous pressure (the reference's), or the
 # last two solutions extrapolated linearly in time (fh_set_warm_extrapolation)
 WARM_STARTS = {"previous": (0, "the previous pressure"),
@@ -98,6 +104,8 @@ def algo_bytes(kernel, solver, buoyancy=True, warm="previous"):
     b = ALGO_BYTES_PER_CELL.get(kernel)
     if kernel == "maccormack_faces" and buoyancy:
         b += 4.0
+    if kernel == "dc_update" and FUSED_UPDATE:
+        b = 38.0
     if kernel == "dc_update" and solver == "mg-fcg":
         b += 8.0
     if kernel == "divergence" and solver in ("mg-dc", "mg-fcg"):
@@ -168,7 +176,7 @@ def measured_traffic(kernel, n):
 PMC_KERNELS = {
     "k_maccormack_dye_lds": "maccormack_dye", "k_maccormack_faces_lds": "maccormack_faces",
     "k_advect_dye_lds": "advect_dye", "k_advect_faces_lds": "advect_faces",
-    "k_dc_update": "dc_update", "k_fcg_update": "fcg_update", "k_fcg_dots": "fcg_dots",
+    "k_prolong_smooth_dc": "dc_update", "k_fcg_update": "fcg_update", "k_fcg_dots": "fcg_dots",
     "k_smooth_restrict": "smooth_restrict_fine", "k_prolong_smooth": "prolong_smooth_fine",
     "k_subtract_outflow": "project", "k_buoyancy": "forces", "k_div_defect": "divergence",
 }
@@ -218,8 +226,13 @@ def pmc_summary(args):
         if kn not in finest or finest[kn] < g:
             finest[kn] = g
     pick = {}
+    # with the fused update the defect-correction solve runs no unfused
+    # prolongation on the finest level: its coarser dispatches are no row
+    skip = {"prolong_smooth_fine"} if FUSED_UPDATE and args.solver == "mg-dc" else set()
     for kn, g in finest.items():
         key = PMC_KERNELS.get(re.sub(r"<.*", "", short(kn)))
+        if key in skip:
+            continue
         if key and (key not in pick or len(F[pick[key]]) < len(F[(kn, g)])):
             pick[key] = (kn, g)
     n = args.n

@@ -437,7 +437,8 @@ int coarse_start(const fh_engine* e, int nlev) {
 // prolong+post-smooth kernels, levels from lc in one workgroup.  warm: level
 // 0 starts from its stored x (the reference mode's pressure warm start);
 // otherwise every level's unknown starts at zero.
-int vcycle(fh_engine* e, int nlev, bool dc, bool warm, const int* done) {
+int vcycle(fh_engine* e, int nlev, bool dc, bool warm, const int* done,
+           bool skip_fine_prolong = false) {
   int lc = dc ? e->lc_dc : e->lc_ref;
   for (int l = 0; l < lc; l++) {
     Level& L = e->lv[l];
@@ -456,7 +457,7 @@ int vcycle(fh_engine* e, int nlev, bool dc, bool warm, const int* done) {
                          dc ? 1 : 0, done);
   }
   LAUNCHED("coarse_vcycle");
-  for (int l = lc - 1; l >= 0; l--) {
+  for (int l = lc - 1; l >= (skip_fine_prolong ? 1 : 0); l--) {
     Level& L = e->lv[l];
     Level& N = e->lv[l + 1];
     Trace t(e, l == 0 ? FH_K_PROLONG_SMOOTH_FINE : -1);
@@ -472,16 +473,37 @@ int vcycle(fh_engine* e, int nlev, bool dc, bool warm, const int* done) {
 // then cycle event c & 1.
 int enqueue_dc_cycle(fh_engine* e, int c, double scale, double tol, double* r64 = nullptr) {
   Level& L0 = e->lv[0];
-  int st = vcycle(e, e->nlev_dc, true, false, e->done);
+  // the finest level's prolongation fused with the update when it is tiled
+  // and its tiles' partials fit the reduction buffer
+  const bool fused = e->fuse_update && e->lc_dc >= 1 &&
+                     (size_t)prolong_smooth_dc_tiles(e->tile[0], e->nx, e->ny) <= e->partials_n;
+  int st = vcycle(e, e->nlev_dc, true, false, e->done, fused);
   if (st != FH_OK) return st;
-  {
-    Trace t(e, FH_K_DC_UPDATE);
-    launch_dc_correct(e->s, e->p, e->p2, L0.x, e->div, L0.info, L0.b, e->nx, e->ny, scale,
-                      e->partials, e->done, r64);
+  if (fused) {
+    const Level& L1 = e->lv[1];
+    int n;
+    {
+      Trace t(e, FH_K_DC_UPDATE);
+      n = launch_prolong_smooth_dc(e->s, e->tile[0], L0.xa, L0.b, L0.info, L0.nx, L0.ny, L1.x,
+                                   L1.nx, L1.ny, e->done, e->p, e->p2, e->div, e->b2, r64, scale,
+                                   e->partials);
+    }
+    launch_dc_norm_n(e->s, e->partials, n, e->sums, e->d_res + 8 * (c & 1), e->done, scale, tol);
+    LAUNCHED("prolong_smooth_dc");
+    // the new residual is the next cycle's right-hand side
+    swap_ptr(L0.b, e->b2);
   }
-  launch_dc_norm(e->s, e->partials, e->nx, e->ny, e->sums, e->d_res + 8 * (c & 1), e->done, scale,
-                 tol);
-  LAUNCHED("dc_update");
+  e->last_dc_fused = fused;
+  if (!fused) {
+    {
+      Trace t(e, FH_K_DC_UPDATE);
+      launch_dc_correct(e->s, e->p, e->p2, L0.x, e->div, L0.info, L0.b, e->nx, e->ny, scale,
+                        e->partials, e->done, r64);
+    }
+    launch_dc_norm(e->s, e->partials, e->nx, e->ny, e->sums, e->d_res + 8 * (c & 1), e->done,
+                   scale, tol);
+    LAUNCHED("dc_update");
+  }
   swap_ptr(e->p, e->p2);
   HIPCHK(hipEventRecord(e->cyc_ev[c & 1], e->s));
   return FH_OK;
@@ -737,7 +759,10 @@ int solve_impl(fh_engine* e, int solver, double scale, int iterations, double om
           // p/p2 were swapped for the skipped cycle's no-op update (a CG
           // step, c > 0 of FH_SOLVER_MG_FCG, updates p in place), and its
           // kernels' trace records time only their early return
-          if (!(fcg && c > 0)) swap_ptr(e->p, e->p2);
+          if (!(fcg && c > 0)) {
+            swap_ptr(e->p, e->p2);
+            if (e->last_dc_fused) swap_ptr(L0.b, e->b2);
+          }
           e->trace_n = trace_mark;
           stopped = true;
           break;
@@ -1039,7 +1064,8 @@ fh_engine* fh_create(int nx, int ny, int dye_scale, double h) {
   if (dalloc(e, &e->cinfo, (size_t)(e->cinfo_off[e->lv.size()] - e->cinfo_off[e->cinfo_from])))
     return bail();
   size_t nblk = (size_t)((nx + 63) / 64) * ((ny + 3) / 4);
-  if (dalloc(e, &e->partials, 4 * nblk + 4 * kFoldBlocks) || dalloc(e, &e->counter, 1) || dalloc(e, &e->sums, 8) ||
+  e->partials_n = nblk;
+  if (dalloc(e, &e->partials, 4 * nblk + 4 * kFoldBlocks) || dalloc(e, &e->b2, nc) || dalloc(e, &e->counter, 1) || dalloc(e, &e->sums, 8) ||
       dalloc(e, &e->fpartials,
              std::max<size_t>(1024, (size_t)advect_dye_speed_blocks(nx, ny))) ||
       dalloc(e, &e->fmax, 1) || dalloc(e, &e->done, 1))
@@ -1074,7 +1100,7 @@ fh_engine* fh_create(int nx, int ny, int dye_scale, double h) {
 void fh_destroy(fh_engine* e) {
   if (!e) return;
   if (e->s) (void)hipStreamSynchronize(e->s);
-  float* fbufs[] = {e->u, e->v, e->u2, e->v2, e->uh, e->vh, e->dye, e->dye2, e->dyeh, e->w,
+  float* fbufs[] = {e->u, e->v, e->u2, e->v2, e->uh, e->vh, e->dye, e->dye2, e->dyeh, e->w, e->b2,
                     e->patch, e->fpartials, e->fmax, e->dvec[0], e->dvec[1]};
   for (float* p : fbufs)
     if (p) (void)hipFree(p);
@@ -1470,6 +1496,12 @@ int fh_set_warm_start(fh_engine* e, fh_unet* u) {
   return FH_OK;
 }
 
+int fh_set_fused_update(fh_engine* e, int on) {
+  GUARD("setFusedUpdate()");
+  e->fuse_update = on != 0;
+  return FH_OK;
+}
+
 int fh_set_warm_extrapolation(fh_engine* e, int order) {
   GUARD("setWarmExtrapolation()");
   if (order < 0 || order > 2) {
@@ -1587,8 +1619,16 @@ int fh_tune_tiles(fh_engine* e, int reps, float* ms_out) {
           if (r == 0) HIPCHK(hipEventRecord(a, e->s));
           launch_smooth_restrict(e->s, t, nullptr, L.xa, L.b, L.info, L.nx, L.ny, N.b, N.nx, N.ny,
                                  1, nullptr);
-          launch_prolong_smooth(e->s, t, L.xa, L.x, L.b, L.info, L.nx, L.ny, N.x, N.nx, N.ny,
-                                nullptr);
+          // the finest level's defect-correction cycles end in the fused
+          // pass (its outputs, p2, b2 and the partials, are scratch here)
+          if (l == 0 && e->fuse_update &&
+              (size_t)prolong_smooth_dc_tiles(t, L.nx, L.ny) <= e->partials_n)
+            launch_prolong_smooth_dc(e->s, t, L.xa, L.b, L.info, L.nx, L.ny, N.x, N.nx, N.ny,
+                                     nullptr, e->p, e->p2, e->div, e->b2, nullptr, 1.0,
+                                     e->partials);
+          else
+            launch_prolong_smooth(e->s, t, L.xa, L.x, L.b, L.info, L.nx, L.ny, N.x, N.nx, N.ny,
+                                  nullptr);
         }
         HIPCHK(hipEventRecord(b, e->s));
         HIPCHK(hipEventSynchronize(b));

@@ -56,6 +56,14 @@ struct fh_engine {
   int tile[fh::kMaxTiledLevels] = {};  // tile shape per tiled level (kTileShapes)
   int coarse_group = 0;            // coarse sub-cycle workgroup (kCoarseGroups)
   bool advect_lds = true;          // LDS-staged advection kernels
+  // a defect-correction cycle's finest prolongation + post-smoothing and its
+  // update pass run as one kernel (launch_prolong_smooth_dc); its residual
+  // goes to b2, then b2 and the finest level's b swap
+  bool fuse_update = true;
+  float* b2 = nullptr;
+  // the last enqueued defect-correction cycle ran fused (a cycle the device
+  // skipped has its b swap undone with its p swap)
+  bool last_dc_fused = false;
   // every level from cinfo_from on with its info padded as the coarse
   // sub-cycle stages it (launch_pad_info); level l at byte cinfo_off[l]
   uint8_t* cinfo = nullptr;
@@ -65,8 +73,9 @@ struct fh_engine {
   int* cells = nullptr;
   int ncoarse = 0;
   bool labels_dirty = true;
-  // reductions
+  // reductions (room for partials_n 4-slot partials and their fold)
   double* partials = nullptr;
+  size_t partials_n = 0;
   unsigned* counter = nullptr;
   double* sums = nullptr;
   float* fpartials = nullptr;

@@ -154,6 +154,10 @@ void launch_div_defect(hipStream_t s, const float* u, const float* v, double inv
                        const Extrapolation* xp = nullptr, int order = 0);
 void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, double* sums,
                     double* host_sums, int* done, double scale, double tol);
+// launch_dc_norm over n partials of a pass with another grid (the fused
+// prolongation + update: one per tile); partials must hold 4 n + 4 kFoldBlocks.
+void launch_dc_norm_n(hipStream_t s, const double* partials, int n, double* sums,
+                      double* host_sums, int* done, double scale, double tol);
 // The norm of the pass before the first cycle: runs whatever *done holds and
 // sets *done (and sums[5]) to 1 when the divergence is zero, 0 otherwise
 // (pressureMultigrid.ts:294-300), so the cycles enqueued behind it skip
@@ -240,6 +244,17 @@ void launch_smooth_restrict(hipStream_t s, int shape, const float* xin, float* x
 void launch_prolong_smooth(hipStream_t s, int shape, const float* xin, float* xout,
                            const float* b, const uint8_t* info, int nx, int ny, const float* e,
                            int cnx, int cny, const int* done);
+// The finest level's prolongation + post-smoothing fused with the
+// defect-correction update after it (prolong_smooth_dc_tile): pn = po + e,
+// the residual of pn to rout (FP32; must differ from b) and r64 (may be
+// null), and one norm partial per tile into partials; returns the number of
+// partials (launch_dc_norm_n's n), prolong_smooth_dc_tiles(shape, nx, ny).
+int launch_prolong_smooth_dc(hipStream_t s, int shape, const float* xin, const float* b,
+                             const uint8_t* info, int nx, int ny, const float* e, int cnx,
+                             int cny, const int* done, const double* po, double* pn,
+                             const double* div, float* rout, double* r64, double scale,
+                             double* partials);
+int prolong_smooth_dc_tiles(int shape, int nx, int ny);
 // The sub-cycle of levels L[0..n) in one workgroup: b0 is the first level's
 // right-hand side, x0in its warm start (NULL = zero), x0out its result.
 // direct: FP64 dense solve on the last level, else 8 red-black pairs.

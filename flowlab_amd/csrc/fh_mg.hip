@@ -420,22 +420,7 @@ __device__ __forceinline__ void dc_update_body(
         pS += (double)E.c[r];
       }
       if (pn) pn[k] = pc;
-      double rr = 0.0;
-      if (f & 0x80) {
-        acc[2] += d * d;
-        acc[3] += 1.0;
-        const int cnt = f & 7;
-        if (cnt > 0) {
-          double sum = 0.0;
-          sum += (f & 0x08) ? pE : 0.0;
-          sum += (f & 0x10) ? pW : 0.0;
-          sum += (f & 0x20) ? pN : 0.0;
-          sum += (f & 0x40) ? pS : 0.0;
-          rr = -scale * d - ((double)cnt * pc - sum);
-          acc[0] += rr * rr;
-          acc[1] += 1.0;
-        }
-      }
+      const double rr = dc_cell(f, d, pc, pE, pW, pN, pS, scale, acc);
       r32[k] = (float)rr;
       if (r64) r64[k] = rr;
     }
@@ -952,6 +937,12 @@ void launch_dc_norm(hipStream_t s, const double* partials, int nx, int ny, doubl
                     double* host_sums, int* done, double scale, double tol) {
   const dim3 g = reduce_grid_rows(nx, ny);
   int n = g.x * g.y;
+  partials = folded(s, partials, &n, done, 0);
+  k_finalize_sum4<<<1, 1024, 0, s>>>(partials, n, sums, host_sums, done, scale, tol, 0, nullptr);
+}
+
+void launch_dc_norm_n(hipStream_t s, const double* partials, int n, double* sums,
+                      double* host_sums, int* done, double scale, double tol) {
   partials = folded(s, partials, &n, done, 0);
   k_finalize_sum4<<<1, 1024, 0, s>>>(partials, n, sums, host_sums, done, scale, tol, 0, nullptr);
 }

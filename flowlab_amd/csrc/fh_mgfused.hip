@@ -52,6 +52,19 @@ __global__ __launch_bounds__(W) void k_prolong_smooth(const float* __restrict__ 
   prolong_smooth_tile<TX, TY, W>(xin, xout, b, info, nx, ny, e, cnx, cny, done, bx, by);
 }
 
+template <int TX, int TY, int W>
+__global__ __launch_bounds__(W) void k_prolong_smooth_dc(
+    const float* __restrict__ xin, const float* __restrict__ b, const uint8_t* __restrict__ info,
+    int nx, int ny, const float* __restrict__ e, int cnx, int cny, const int* __restrict__ done,
+    const double* __restrict__ po, double* __restrict__ pn, const double* __restrict__ div,
+    float* __restrict__ rout, double* __restrict__ r64, double scale,
+    double* __restrict__ partials) {
+  int bx, by;
+  xcd_block(&bx, &by);
+  prolong_smooth_dc_tile<TX, TY, W>(xin, b, info, nx, ny, e, cnx, cny, done, bx, by, po, pn, div,
+                                    rout, r64, scale, partials, bx + by * (int)gridDim.x);
+}
+
 template <int G>
 __global__ __launch_bounds__(G) void k_coarse_vcycle(CoarseLevels L, const float* __restrict__ b0,
                                                      const float* __restrict__ x0in,
@@ -127,6 +140,38 @@ static void ps_launch(hipStream_t s, const float* xin, float* xout, const float*
                       const int* done) {
   dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY);
   k_prolong_smooth<TX, TY, W><<<g, W, 0, s>>>(xin, xout, b, info, nx, ny, e, cnx, cny, done);
+}
+
+template <int TX, int TY, int W>
+static int psdc_launch(hipStream_t s, const float* xin, const float* b, const uint8_t* info,
+                       int nx, int ny, const float* e, int cnx, int cny, const int* done,
+                       const double* po, double* pn, const double* div, float* rout, double* r64,
+                       double scale, double* partials) {
+  dim3 g((nx + TX - 1) / TX, (ny + TY - 1) / TY);
+  k_prolong_smooth_dc<TX, TY, W><<<g, W, 0, s>>>(xin, b, info, nx, ny, e, cnx, cny, done, po, pn,
+                                                 div, rout, r64, scale, partials);
+  return (int)(g.x * g.y);
+}
+
+int prolong_smooth_dc_tiles(int shape, int nx, int ny) {
+  const TileShape& t = kTileShapes[shape];
+  return ((nx + t.tx - 1) / t.tx) * ((ny + t.ty - 1) / t.ty);
+}
+
+int launch_prolong_smooth_dc(hipStream_t s, int shape, const float* xin, const float* b,
+                             const uint8_t* info, int nx, int ny, const float* e, int cnx,
+                             int cny, const int* done, const double* po, double* pn,
+                             const double* div, float* rout, double* r64, double scale,
+                             double* partials) {
+  switch (shape) {
+#define FH_PSDC_CASE(k, tx, ty, w)                                                             \
+  case k:                                                                                      \
+    return psdc_launch<tx, ty, w>(s, xin, b, info, nx, ny, e, cnx, cny, done, po, pn, div, rout, \
+                                  r64, scale, partials);
+    FH_TILE_SHAPES(FH_PSDC_CASE)
+#undef FH_PSDC_CASE
+  }
+  return 0;
 }
 
 void launch_smooth_restrict(hipStream_t s, int shape, const float* xin, float* xout,

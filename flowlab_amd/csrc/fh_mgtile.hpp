@@ -58,6 +58,36 @@ __device__ __forceinline__ float gs_recip(int cnt) {
   return cnt == 3 ? (1.f / 3.f) : p2;
 }
 
+// The defect-correction update's arithmetic at one cell with info byte f,
+// given the updated pressure pc and its four neighbours' and the divergence
+// d: returns r = b - A p (b = -scale d; 0 off the fluid or without an open
+// neighbour, neighbours summed +x, -x, +y, -y) and adds the norm terms
+// {r^2, cells with such an r, d^2, fluid cells} to acc.  Contraction is off,
+// so every pass that forms it (the update pass, and the finest prolongation
+// tile fused with it) gives the same bits.
+__device__ __forceinline__ double dc_cell(int f, double d, double pc, double pE, double pW,
+                                          double pN, double pS, double scale,
+                                          double (&acc)[4]) {
+#pragma clang fp contract(off)
+  double rr = 0.0;
+  if (f & 0x80) {
+    acc[2] += d * d;
+    acc[3] += 1.0;
+    const int cnt = f & 7;
+    if (cnt > 0) {
+      double sum = 0.0;
+      sum += (f & 0x08) ? pE : 0.0;
+      sum += (f & 0x10) ? pW : 0.0;
+      sum += (f & 0x20) ? pN : 0.0;
+      sum += (f & 0x40) ? pS : 0.0;
+      rr = -scale * d - ((double)cnt * pc - sum);
+      acc[0] += rr * rr;
+      acc[1] += 1.0;
+    }
+  }
+  return rr;
+}
+
 // Red-black Gauss-Seidel (omega = 1) on an LDS window of LX x LY cells.
 // Labels and right-hand side do not change while a tile is smoothed, so each
 // thread finds its cells once -- for each colour, K cells of the window's
@@ -346,28 +376,23 @@ __device__ __forceinline__ void smooth_restrict_tile(const float* __restrict__ x
 
 // ---- k_prolong_smooth --------------------------------------------------------
 // x += bilinear(e) on fluid cells (pressureMultigrid.ts:192-207), then 2
-// post-smoothing sweeps.  Reads xin, writes xout (distinct buffers).
-template <int TX, int TY, int W = NT>
-__device__ __forceinline__ void prolong_smooth_tile(const float* __restrict__ xin,
-                                                    float* __restrict__ xout,
+// post-smoothing sweeps, on a TX x TY tile (origin ox, oy) staged with a halo
+// of H cells: the LDS windows sx (x), sb (b), si (info) of (TX + 2H) x
+// (TY + 2H) cells and se of the coarse e around them.  Returns false
+// (uniformly, before any LDS write) when `done` is set.  Afterwards every
+// window cell at distance >= 4 from the border holds the global sweep's value
+// (the tile interior at HB = 4, one more ring at H = 5).
+template <int TX, int TY, int W, int H>
+__device__ __forceinline__ bool prolong_smooth_core(float* sx, float* sb, uint8_t* si, float* se,
+                                                    const float* __restrict__ xin,
                                                     const float* __restrict__ b,
                                                     const uint8_t* __restrict__ info, int nx,
                                                     int ny, const float* __restrict__ e, int cnx,
-                                                    int cny, const int* __restrict__ done, int bx,
-                                                    int by) {
-  constexpr int BXW = TX + 2 * HB, BYH = TY + 2 * HB;
+                                                    int cny, const int* __restrict__ done, int ox,
+                                                    int oy, int tid) {
+  constexpr int BXW = TX + 2 * H, BYH = TY + 2 * H;
   constexpr int EX = BXW / 2 + 3, EY = BYH / 2 + 3;
-  __shared__ float sx[BXW * BYH];
-  __shared__ float sb[BXW * BYH];
-  __shared__ uint8_t si[BXW * BYH];
-  __shared__ float se[EX * EY];
-  // Opaque per call: keeps the compiler from hoisting tid-only index
-  // arithmetic out of a caller's persistent task loop (which would hold
-  // all of it in registers across tasks).
-  int tid = threadIdx.x;
-  __asm__ volatile("" : "+v"(tid));
-  const int ox = bx * TX, oy = by * TY;
-  const int gx0 = ox - HB, gy0 = oy - HB;
+  const int gx0 = ox - H, gy0 = oy - H;
   // coarse window: parents I = gi>>1 and their +-1 neighbours
   const int ex0 = (gx0 >> 1) - 1, ey0 = (gy0 >> 1) - 1;
   {
@@ -381,7 +406,8 @@ __device__ __forceinline__ void prolong_smooth_tile(const float* __restrict__ xi
       bool in = q < EX * EY && I >= 0 && J >= 0 && I < cnx && J < cny;
       re[k] = in ? e[I + J * cnx] : 0.f;
     }
-    if (!stage_tile<BXW, BYH, W>(sx, sb, si, xin, b, info, nx, ny, gx0, gy0, tid, done)) return;
+    if (!stage_tile<BXW, BYH, W>(sx, sb, si, xin, b, info, nx, ny, gx0, gy0, tid, done))
+      return false;
 #pragma unroll
     for (int k = 0; k < K; k++) {
       int q = tid + k * W;
@@ -413,11 +439,107 @@ __device__ __forceinline__ void prolong_smooth_tile(const float* __restrict__ xi
   __syncthreads();
   sw.template sweep<1>(sx, false);
   __syncthreads();
+  return true;
+}
+
+// Prolongation + post-smoothing of one tile; reads xin, writes xout (distinct
+// buffers).
+template <int TX, int TY, int W = NT>
+__device__ __forceinline__ void prolong_smooth_tile(const float* __restrict__ xin,
+                                                    float* __restrict__ xout,
+                                                    const float* __restrict__ b,
+                                                    const uint8_t* __restrict__ info, int nx,
+                                                    int ny, const float* __restrict__ e, int cnx,
+                                                    int cny, const int* __restrict__ done, int bx,
+                                                    int by) {
+  constexpr int BXW = TX + 2 * HB, BYH = TY + 2 * HB;
+  constexpr int EX = BXW / 2 + 3, EY = BYH / 2 + 3;
+  __shared__ float sx[BXW * BYH];
+  __shared__ float sb[BXW * BYH];
+  __shared__ uint8_t si[BXW * BYH];
+  __shared__ float se[EX * EY];
+  // Opaque per call: keeps the compiler from hoisting tid-only index
+  // arithmetic out of a caller's persistent task loop (which would hold
+  // all of it in registers across tasks).
+  int tid = threadIdx.x;
+  __asm__ volatile("" : "+v"(tid));
+  const int ox = bx * TX, oy = by * TY;
+  if (!prolong_smooth_core<TX, TY, W, HB>(sx, sb, si, se, xin, b, info, nx, ny, e, cnx, cny,
+                                          done, ox, oy, tid))
+    return;
   for (int q = tid; q < TX * TY; q += W) {
     int lj = q / TX, li = q - lj * TX;
     int gi = ox + li, gj = oy + lj;
     if (gi < nx && gj < ny) xout[gi + gj * nx] = sx[(li + HB) + (lj + HB) * BXW];
   }
+}
+
+// The finest level's prolongation + post-smoothing fused with the
+// defect-correction update that follows it (dc_cell at every tile cell, the
+// correction e taken from the tile one ring beyond its interior, hence the
+// halo HB + 1): pn = po + e, r = b - A pn with b = -scale div, the FP32 r to
+// rout (which must differ from b, whose halos other tiles still read), the
+// FP64 r to r64 when given, and the tile's norm terms to partials[bid].  The
+// correction itself is not stored: the next cycle starts from zero.
+template <int TX, int TY, int W = NT>
+__device__ __forceinline__ void prolong_smooth_dc_tile(
+    const float* __restrict__ xin, const float* __restrict__ b, const uint8_t* __restrict__ info,
+    int nx, int ny, const float* __restrict__ e, int cnx, int cny, const int* __restrict__ done,
+    int bx, int by, const double* __restrict__ po, double* __restrict__ pn,
+    const double* __restrict__ div, float* __restrict__ rout, double* __restrict__ r64,
+    double scale, double* __restrict__ partials, int bid) {
+  constexpr int H = HB + 1;
+  constexpr int BXW = TX + 2 * H, BYH = TY + 2 * H;
+  constexpr int EX = BXW / 2 + 3, EY = BYH / 2 + 3;
+  __shared__ float sx[BXW * BYH];
+  __shared__ float sb[BXW * BYH];
+  __shared__ uint8_t si[BXW * BYH];
+  __shared__ float se[EX * EY];
+  int tid = threadIdx.x;
+  __asm__ volatile("" : "+v"(tid));
+  const int ox = bx * TX, oy = by * TY;
+  if (!prolong_smooth_core<TX, TY, W, H>(sx, sb, si, se, xin, b, info, nx, ny, e, cnx, cny, done,
+                                         ox, oy, tid))
+    return;
+  // each thread takes R consecutive rows of one tile column: the column's
+  // pressure over rows -1 .. R (north and south shared between rows) and
+  // its east and west neighbours, every load issued before the arithmetic;
+  // indices clamped into the domain as the update pass clamps them (a closed
+  // neighbour's value is not used)
+  static_assert(W % TX == 0 && TY % (W / TX) == 0, "tile rows per thread");
+  constexpr int R = TY / (W / TX);
+  const int li = tid % TX, lj0 = (tid / TX) * R;
+  const int gi = ox + li;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (gi < nx) {
+    const int iE = gi + 1 < nx ? gi + 1 : gi, iW = gi > 0 ? gi - 1 : gi;
+    double pc[R + 2], pe[R], pw[R], dv[R];
+#pragma unroll
+    for (int r = 0; r < R + 2; r++) pc[r] = po[gi + min(max(oy + lj0 - 1 + r, 0), ny - 1) * nx];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int row = min(oy + lj0 + r, ny - 1) * nx;
+      pe[r] = po[iE + row];
+      pw[r] = po[iW + row];
+      dv[r] = div[gi + row];
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int gj = oy + lj0 + r;
+      if (gj >= ny) break;
+      const int k = gi + gj * nx, l = (li + H) + (lj0 + r + H) * BXW;
+      const double c = pc[r + 1] + (double)sx[l];
+      const double pE = pe[r] + (double)sx[l + 1];
+      const double pW = pw[r] + (double)sx[l - 1];
+      const double pN = pc[r + 2] + (double)sx[l + BXW];
+      const double pS = pc[r] + (double)sx[l - BXW];
+      pn[k] = c;
+      const double rr = dc_cell(si[l], dv[r], c, pE, pW, pN, pS, scale, acc);
+      rout[k] = (float)rr;
+      if (r64) r64[k] = rr;
+    }
+  }
+  block_sum_store_at<4>(acc, partials, bid);
 }
 
 // ---- k_coarse_vcycle -----------------------------------------------------------

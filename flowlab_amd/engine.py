@@ -183,6 +183,11 @@ class HipEngine:
         in time (order 1 linear, 2 quadratic; 0 = the previous pressure)."""
         check(self.lib.fh_set_warm_extrapolation(self._e, int(order)))
 
+    def set_fused_update(self, on: bool):
+        """Defect-correction cycles run the finest prolongation + smoothing
+        and the FP64 update as one kernel (default) or as two passes."""
+        check(self.lib.fh_set_fused_update(self._e, 1 if on else 0))
+
     def last_solve_ms(self) -> float:
         ms = C.c_double()
         check(self.lib.fh_last_solve_ms(self._e, C.byref(ms)))

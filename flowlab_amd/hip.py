@@ -94,6 +94,7 @@ SIGNATURES = [
     ("fh_last_solve_ms", _i, [_vp, _dp]),
     ("fh_set_warm_start", _i, [_vp, _vp]),
     ("fh_set_warm_extrapolation", _i, [_vp, _i]),
+    ("fh_set_fused_update", _i, [_vp, _i]),
     ("fh_batch_last_error", C.c_char_p, []),
     ("fh_batch_create", _vp, [_i, _ip, _ip, _dp]),
     ("fh_batch_destroy", None, [_vp]),

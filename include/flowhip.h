@@ -297,6 +297,12 @@ int fh_set_warm_start(fh_engine* e, fh_unet* u);
  * set, corrects the extrapolated pressure.  The solve still runs to its
  * tolerance, so only the iteration count depends on the start. */
 int fh_set_warm_extrapolation(fh_engine* e, int order);
+/* On (the default): each defect-correction cycle runs its finest-level
+ * prolongation + post-smoothing and the FP64 update after it as one kernel
+ * (the same pressure and residual bits as the two passes; the residual
+ * norm's partial sums are grouped per tile).  Off: the two passes, for
+ * comparison.  No counterpart in the reference. */
+int fh_set_fused_update(fh_engine* e, int on);
 
 /* ---- Batched scenes ------------------------------------------------------
  * No counterpart in the reference, which steps one grid per Simulation

@@ -121,6 +121,38 @@ def test_mg_dc_reaches_1e6(closed):
     assert len(hist) == it + 1 and hist[-1] <= 1e-6
 
 
+@pytest.mark.parametrize("solver", ["mg-dc", "mg-fcg"])
+@pytest.mark.parametrize("shape", [(256, 192), (197, 131)])
+def test_fused_update_gives_the_same_bits(solver, shape):
+    """Defect-correction cycles whose finest prolongation + post-smoothing
+    and FP64 update run as one kernel give the same pressure bits and cycle
+    counts as the two passes, on every tile shape of the finest level (odd
+    sizes: partial tiles at two edges), and reach 1e-6 by the oracle."""
+    from flowlab_amd.engine import HipPressureSolver
+    nx, ny = shape
+    lab, div = boundary_fixture(nx, ny)
+    sol = HipPressureSolver(nx, ny, solver)
+    e = sol.engine
+    p0 = np.zeros(nx * ny)
+    sol.solve(None, p0.copy(), div, lab, 0.02, 1, 1.0, 0)  # labels, hierarchy
+    assert len(e.tiles()) >= 1, "the finest level must be tiled for the fused pass"
+    ref = {}
+    for fused in (False, True):
+        e.set_fused_update(fused)
+        for t in range(len(e.tile_names())):
+            e.set_tile(t, level=0)
+            p = p0.copy()
+            it = sol.solve(None, p, div, lab, 0.02, 60, 1.0, 1e-6)
+            if not fused:
+                ref[t] = (p, it)
+                continue
+            rp, rit = ref[t]
+            assert it == rit, (t, it, rit)
+            assert np.array_equal(p, rp), t
+    assert O.relative_residual(nx, ny, p, div, lab, 0.02) <= 1e-6
+    e.set_fused_update(True)
+
+
 def _velocity_state(nx, ny, seed=12345):
     g = S.Grid(nx, ny, 1.0 / ny)
     u = np.zeros((nx + 1) * ny)
