@@ -48,9 +48,12 @@ WORKLOAD_STEPS = 200
 #     as the body force it applies it too, reading the heat (+4).
 #   advect_faces: carrier = source u, v 8 + out u, v 8 + label 1 + the
 #     recorded limiter stencils 1 (MacCormack's forward pass).
-#   maccormack_dye: u, v 8 + 3 channels of q^n and of the forward pass 24 +
-#     3 out 12 + label 1 + recorded stencils 1; advect_dye: u, v 8 + 3 in 12 +
-#     3 out 12 + label 1 + recorded stencils 1.
+#   maccormack_dye: as one pass (the engine's default at dye scale 1: the
+#     forward-pass samples computed in LDS, DYE_ONE_PASS) u, v 8 + 3 channels
+#     of q^n 12 + label 1 + 3 out 12 (33); as the correction pass of two,
+#     u, v 8 + 3 channels of q^n and of the forward pass 24 + 3 out 12 +
+#     label 1 + recorded stencils 1 (46); advect_dye (the forward pass of two):
+#     u, v 8 + 3 in 12 + 3 out 12 + label 1 + recorded stencils 1.
 #   dc_update: a defect-correction cycle's update, fused with the finest
 #     prolongation + post-smoothing (the engine's default): the pre-smoothed
 #     x 4 + b 4 + label 1 + the coarse correction 1 + p 8 + div 8 read, p 8 +
@@ -89,6 +92,9 @@ PMC_TRAFFIC = os.path.join(ROOT, "profiles", "pmc_traffic_plume_8192.json")
 # the engines fuse each defect-correction cycle's finest prolongation +
 # smoothing with its update (fh_set_fused_update), traced as dc_update
 FUSED_UPDATE = True
+# the engines run the MacCormack dye as one pass (fh_set_dye_one_pass), traced
+# as maccormack_dye
+DYE_ONE_PASS = True
 # start of each step's solve: the previous pressure (the reference's), or the
 # last two solutions extrapolated linearly in time (fh_set_warm_extrapolation)
 WARM_STARTS = {"previous": (0, "the previous pressure"),
@@ -104,6 +110,8 @@ def algo_bytes(kernel, solver, buoyancy=True, warm="previous"):
     b = ALGO_BYTES_PER_CELL.get(kernel)
     if kernel == "maccormack_faces" and buoyancy:
         b += 4.0
+    if kernel == "maccormack_dye" and DYE_ONE_PASS:
+        b = 33.0
     if kernel == "dc_update" and FUSED_UPDATE:
         b = 38.0
     if kernel == "dc_update" and solver == "mg-fcg":
@@ -174,7 +182,8 @@ def measured_traffic(kernel, n):
 # rocprofv3 kernel names (template arguments and parameter lists dropped) of
 # the kernels whose traffic the table reports
 PMC_KERNELS = {
-    "k_maccormack_dye_lds": "maccormack_dye", "k_maccormack_faces_lds": "maccormack_faces",
+    "k_maccormack_dye_fused": "maccormack_dye", "k_maccormack_dye_lds": "maccormack_dye",
+    "k_maccormack_faces_lds": "maccormack_faces",
     "k_advect_dye_lds": "advect_dye", "k_advect_faces_lds": "advect_faces",
     "k_prolong_smooth_dc": "dc_update", "k_fcg_update": "fcg_update", "k_fcg_dots": "fcg_dots",
     "k_smooth_restrict": "smooth_restrict_fine", "k_prolong_smooth": "prolong_smooth_fine",

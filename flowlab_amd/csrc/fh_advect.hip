@@ -6,6 +6,7 @@
 // (a wavefront's 64 faces backtrace into a ~64+2c wide strip) and the kernels
 // are bound by HBM traffic: MacCormack velocity moves ~68 B per cell
 // (forward 2x13, correction 2x21), dye ~78 B per cell for 3 channels.
+#include <algorithm>
 #include "fh_batchdesc.hpp"
 #include "fh_stencil.hpp"
 
@@ -118,32 +119,59 @@ __device__ __forceinline__ float nan_max(float m, float x) {
 // correction ran 1.8x the single-grid kernel's time on the same scene; 1.07x
 // with kClamped).  The single-grid kernels keep the branches, which measured
 // 4-8 % faster for them at 8192^2.  Same values either way.
-template <int NF, int WN, bool kClamped = false>
+//
+// Windows that lie inside every field (all tiles but those along the domain
+// edges; a workgroup-uniform test) take neither bounds tests nor branches:
+// each load is the field's base plus a 32-bit byte offset, shared by the
+// fields of one row stride, and the last slot's spare lanes load the
+// window's last element (in bounds; not stored, and a window element for the
+// maxima).  The edge tiles, and fields of 2^30 elements or more, stage as
+// above.  W: the window's row length; NT: the workgroup's threads.
+template <int NF, int WN, bool kClamped = false, int W = WW, int NT = 256>
 __device__ __forceinline__ float stage_windows(float (*lds)[WN], const Win (&f)[NF], int wx0,
                                                int wy0, int tid, float* speed = nullptr) {
-  constexpr int kStage = (WN + 255) / 256;
+  constexpr int kStage = (WN + NT - 1) / NT, WH = WN / W;
+  static_assert(WH * W == WN, "window shape");
   float v[NF][kStage];
+  bool inside = wx0 >= 0 && wy0 >= 0;
 #pragma unroll
-  for (int r = 0; r < kStage; r++) {
-    const int e = tid + r * 256;
-    const int b = e / WW, a = e - b * WW;
-    const int i = wx0 + a, j = wy0 + b;
+  for (int q = 0; q < NF; q++)
+    inside = inside && wx0 + W <= f[q].w && wy0 + WH <= f[q].h &&
+             (long long)f[q].w * f[q].h < (1ll << 30);
+  if (inside) {
 #pragma unroll
-    for (int q = 0; q < NF; q++) {
-      const bool in = e < WN && i >= 0 && j >= 0 && i < f[q].w && j < f[q].h;
-      if constexpr (kClamped) {
-        const int ci = min(max(i, 0), f[q].w - 1), cj = min(max(j, 0), f[q].h - 1);
-        const float x = f[q].p[ci + cj * f[q].w];
-        v[q][r] = in ? x : 0.f;
-      } else {
-        v[q][r] = in ? f[q].p[i + j * f[q].w] : 0.f;
+    for (int r = 0; r < kStage; r++) {
+      const int e = (r + 1) * NT <= WN ? tid + r * NT : min(tid + r * NT, WN - 1);
+      const int b = e / W, a = e - b * W;
+#pragma unroll
+      for (int q = 0; q < NF; q++) {
+        const uint32_t off = (uint32_t)((wx0 + a) + (wy0 + b) * f[q].w) * 4u;
+        v[q][r] = *(const float*)((const char*)f[q].p + off);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < kStage; r++) {
+      const int e = tid + r * NT;
+      const int b = e / W, a = e - b * W;
+      const int i = wx0 + a, j = wy0 + b;
+#pragma unroll
+      for (int q = 0; q < NF; q++) {
+        const bool in = e < WN && i >= 0 && j >= 0 && i < f[q].w && j < f[q].h;
+        if constexpr (kClamped) {
+          const int ci = min(max(i, 0), f[q].w - 1), cj = min(max(j, 0), f[q].h - 1);
+          const float x = f[q].p[ci + cj * f[q].w];
+          v[q][r] = in ? x : 0.f;
+        } else {
+          v[q][r] = in ? f[q].p[i + j * f[q].w] : 0.f;
+        }
       }
     }
   }
   float m = 0.f, sp = 0.f;
 #pragma unroll
   for (int r = 0; r < kStage; r++) {
-    const int e = tid + r * 256;
+    const int e = tid + r * NT;
     if (e < WN) {
 #pragma unroll
       for (int q = 0; q < NF; q++) lds[q][e] = v[q][r];
@@ -157,13 +185,16 @@ __device__ __forceinline__ float stage_windows(float (*lds)[WN], const Win (&f)[
 
 // Workgroup-uniform reach test: every thread passes its staged maximum, the
 // barrier that publishes the windows also publishes the four waves' maxima.
+template <int NW = 4>
 __device__ __forceinline__ bool within_reach(float m, float c, int tid) {
-  __shared__ float wm[4];
+  __shared__ float wm[NW];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = nan_max(m, __shfl_xor(m, o, 64));
   if ((tid & 63) == 0) wm[tid >> 6] = m;
   __syncthreads();
-  const float vw = nan_max(nan_max(nan_max(wm[0], wm[1]), wm[2]), wm[3]);
+  float vw = wm[0];
+#pragma unroll
+  for (int k = 1; k < NW; k++) vw = nan_max(vw, wm[k]);
   return fabsf(c) * vw <= kReach;
 }
 
@@ -171,9 +202,10 @@ __device__ __forceinline__ bool within_reach(float m, float c, int tid) {
 // each thread's from stage_windows) to partials[bid], under the same barrier.
 // The windows of all tiles of a launch cover every u and v element, so the
 // largest of these partials is the fields' max face speed.
+template <int NW = 4>
 __device__ __forceinline__ bool within_reach_speed(float m, float sp, float c, int tid,
                                                    float* __restrict__ partials, int bid) {
-  __shared__ float wm[4], ws[4];
+  __shared__ float wm[NW], ws[NW];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     m = nan_max(m, __shfl_xor(m, o, 64));
@@ -184,18 +216,26 @@ __device__ __forceinline__ bool within_reach_speed(float m, float sp, float c, i
     ws[tid >> 6] = sp;
   }
   __syncthreads();
-  if (tid == 0) partials[bid] = fmaxf(fmaxf(ws[0], ws[1]), fmaxf(ws[2], ws[3]));
-  const float vw = nan_max(nan_max(nan_max(wm[0], wm[1]), wm[2]), wm[3]);
+  if (tid == 0) {
+    float s = ws[0];
+#pragma unroll
+    for (int k = 1; k < NW; k++) s = fmaxf(s, ws[k]);
+    partials[bid] = s;
+  }
+  float vw = wm[0];
+#pragma unroll
+  for (int k = 1; k < NW; k++) vw = nan_max(vw, wm[k]);
   return fabsf(c) * vw <= kReach;
 }
 
 template <class F, int WN>
-__device__ __forceinline__ F window_as(const float (*lds)[WN], int q, int wx0, int wy0) {
+__device__ __forceinline__ F window_as(const float (*lds)[WN], int q, int wx0, int wy0,
+                                       int w = WW) {
   F f;
   f.l = (lds_cfp)lds[q];
   f.x0 = wx0;
   f.y0 = wy0;
-  f.w = WW;
+  f.w = w;
   return f;
 }
 
@@ -438,6 +478,144 @@ __global__ __launch_bounds__(256) void k_maccormack_dye_lds(
   maccormack_dye_tile<TY>(u, v, qn, qh, qo, lbl, nx, ny, c, k0, k1, k2, sten, bx, by);
 }
 
+// ---- MacCormack dye in one pass ----------------------------------------------------
+// advect.ts:241-270 then 291-311 (dye scale 1) in one kernel.  The two-pass
+// form moves 80 B per cell (34 + 46): the forward pass writes qHat and the
+// limiter stencils, the correction reads them back with q^n and the velocity
+// again.  Here a workgroup stages u, v and the three channels of q^n over
+// its tile and a halo of AF = 4 cells, computes the forward pass itself on
+// the tile plus a ring of two cells (every qHat the tile's reverse traces
+// sample), keeps those in LDS with the tile's stencil bytes, and corrects
+// its tile: 33 B per cell (u, v 8, q^n 12, label 1, out 12), at a third
+// more forward-pass arithmetic for the ring.  Each value is computed by the
+// same functions on the same staged values, so the outputs are the two
+// passes' bits.
+//
+// Reach: with D <= kReach, a dye cell i's samples (the backtrace both ways,
+// its midpoint, the clamp stencil, for u, v and q alike) read columns
+// i - 2 .. i + 2 (u faces i - 1 .. i + 2), and the domain clamps only move
+// them towards i; the ring's cells x0 - 2 .. x0 + TX + 1 so read
+// x0 - 4 .. x0 + TX + 3, the window's TX + 2 AF columns, and likewise rows.
+// The bound is the two-pass kernels' (kReach) on this window.  A workgroup over it writes nothing and marks the launch (fail =
+// epoch); the two-pass kernels then run after it over every tile
+// (k_dye_forward_rerun, k_dye_correct_rerun: each workgroup returns at once
+// unless the mark is this launch's), so every output is the two passes'
+// either way.  The max-speed partials come from the staged velocity windows
+// of every tile, as in the forward kernel.
+constexpr int AF = 4, WWF = TX + 2 * AF, BYF = 8, kTyDyeFused = 16;
+
+// 5 windows, the ring's 3 channels and the tile's stencil bytes: 51.9 KB,
+// three workgroups of 512 threads per CU.
+template <int TY>
+struct FusedTile {
+  static_assert(TY % BYF == 0, "rows per thread");
+  static constexpr int WH = TY + 2 * AF, WN = WWF * WH;
+  // the forward-pass ring: cells (x0 - 2 .. x0 + TX + 1, y0 - 2 .. y0 + TY + 1)
+  static constexpr int RW = TX + 4, RH = TY + 4, RN = RW * RH;
+};
+
+// The ring's forward-pass samples and the correction's reads all keep
+// their stencil origins clear of the domain clamps (interior_tile's
+// argument, for the ring's cells).
+template <int TY>
+__device__ __forceinline__ bool interior_ring(int x0, int y0, int nx, int ny) {
+  return x0 - 2 >= kEdge && y0 - 2 >= kEdge && x0 + TX + 1 <= nx - 1 - kEdge &&
+         y0 + TY + 1 <= ny - 1 - kEdge;
+}
+
+template <int TY>
+__global__ __launch_bounds__(TX * BYF) void k_maccormack_dye_fused(
+    const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ q,
+    float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx, int ny, float c, float k0,
+    float k1, float k2, float* __restrict__ speed, int* __restrict__ fail, int epoch) {
+  using T = FusedTile<TY>;
+  constexpr int NT = TX * BYF, NW = NT / 64;
+  __shared__ float lds[5][T::WN];
+  __shared__ float hat[3][T::RN];
+  __shared__ uint8_t hst[TX * TY];
+  int bx, by;
+  xcd_block(&bx, &by);
+  const int x0 = bx * TX, y0 = by * TY, wx0 = x0 - AF, wy0 = y0 - AF, rx0 = x0 - 2,
+            ry0 = y0 - 2;
+  const int tid = threadIdx.x + threadIdx.y * TX;
+  const size_t plane = (size_t)nx * ny;
+  const Win f[5] = {{u, nx + 1, ny}, {v, nx, ny + 1}, {q, nx, ny}, {q + plane, nx, ny},
+                    {q + 2 * plane, nx, ny}};
+  float sp = 0.f;
+  const float m = stage_windows<5, T::WN, false, WWF, NT>(lds, f, wx0, wy0, tid,
+                                                           speed ? &sp : nullptr);
+  const int bid = (int)(blockIdx.x + blockIdx.y * gridDim.x);
+  const bool reach = speed ? within_reach_speed<NW>(m, sp, c, tid, speed, bid)
+                           : within_reach<NW>(m, c, tid);
+  if (!reach) {
+    if (tid == 0) atomicExch(fail, epoch);
+    return;
+  }
+  auto run = [&](auto tag) {
+    using F = decltype(tag);
+    const F a = window_as<F>(lds, 0, wx0, wy0, WWF), b = window_as<F>(lds, 1, wx0, wy0, WWF),
+            n0 = window_as<F>(lds, 2, wx0, wy0, WWF), n1 = window_as<F>(lds, 3, wx0, wy0, WWF),
+            n2 = window_as<F>(lds, 4, wx0, wy0, WWF);
+    // the forward pass (keep 1, as the two-pass forward kernel runs it) on
+    // the ring's cells inside the grid; the others are never read
+    for (int e = tid; e < T::RN; e += NT) {
+      const int rb = e / T::RW, ra = e - rb * T::RW;
+      const int i = rx0 + ra, j = ry0 + rb;
+      if (i < 0 || j < 0 || i >= nx || j >= ny) continue;
+      float o[3];
+      const uint32_t st =
+          advect_dye_cell(a, b, n0, n1, n2, lbl, nx, ny, 1, c, 1.f, 1.f, 1.f, i, j, -1, o);
+      if (ra >= 2 && ra < TX + 2 && rb >= 2 && rb < TY + 2)
+        hst[(rb - 2) * TX + (ra - 2)] = (uint8_t)st;
+      hat[0][e] = o[0];
+      hat[1][e] = o[1];
+      hat[2][e] = o[2];
+    }
+    __syncthreads();
+    const F h0 = window_as<F>(hat, 0, rx0, ry0, T::RW), h1 = window_as<F>(hat, 1, rx0, ry0, T::RW),
+            h2 = window_as<F>(hat, 2, rx0, ry0, T::RW);
+    const int i = x0 + (int)threadIdx.x;
+    for (int r = 0; r < TY / BYF; r++) {
+      const int j = y0 + (int)threadIdx.y + r * BYF;
+      const uint32_t st = (i < nx && j < ny) ? hst[(j - y0) * TX + (i - x0)] : 0u;
+      maccormack_dye_t(a, b, n0, n1, n2, h0, h1, h2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i, j,
+                       true, st);
+    }
+  };
+  if (interior_ring<TY>(x0, y0, nx, ny))
+    run(LFieldI{});
+  else
+    run(LField{});
+}
+
+// The two-pass kernels over every tile, for a launch that k_maccormack_dye_fused
+// marked (fail == epoch); otherwise each workgroup returns at once.  A small
+// grid strides over the tiles (n of them, gx per row).
+template <int TY>
+__global__ __launch_bounds__(256) void k_dye_forward_rerun(
+    const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ q,
+    float* __restrict__ qh, const uint8_t* __restrict__ lbl, int nx, int ny, float c,
+    uint8_t* __restrict__ sten, const int* __restrict__ fail, int epoch, int gx, int n) {
+  if (*fail != epoch) return;
+  for (int t = (int)blockIdx.x; t < n; t += (int)gridDim.x) {
+    __syncthreads();  // the previous tile's reads of the windows are done
+    advect_dye_tile<TY>(u, v, q, qh, lbl, nx, ny, c, 1.f, 1.f, 1.f, sten, t % gx, t / gx);
+  }
+}
+
+template <int TY>
+__global__ __launch_bounds__(256) void k_dye_correct_rerun(
+    const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ qn,
+    const float* __restrict__ qh, float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx,
+    int ny, float c, float k0, float k1, float k2, const uint8_t* __restrict__ sten,
+    const int* __restrict__ fail, int epoch, int gx, int n) {
+  if (*fail != epoch) return;
+  for (int t = (int)blockIdx.x; t < n; t += (int)gridDim.x) {
+    __syncthreads();
+    maccormack_dye_tile<TY>(u, v, qn, qh, qo, lbl, nx, ny, c, k0, k1, k2, sten, t % gx, t / gx);
+  }
+}
+
 template <int TY>
 static inline dim3 grid_t(int w, int h) {
   return dim3((w + TX - 1) / TX, (h + TY - 1) / TY);
@@ -491,6 +669,35 @@ void launch_maccormack_dye(hipStream_t s, const float* u, const float* v, const 
   else
     k_maccormack_dye<<<grid2(nx * ds, ny * ds), dim3(BX, BY), 0, s>>>(u, v, qn, qh, qo, lbl, nx,
                                                                       ny, ds, c, k0, k1, k2);
+}
+
+int maccormack_dye_fused_blocks(int nx, int ny) {
+  const dim3 g = grid_t<kTyDyeFused>(nx, ny);
+  return (int)(g.x * g.y);
+}
+
+// The two-pass kernels after a one-pass launch run with a small grid that
+// strides over the tiles; unless that launch was marked, each of its
+// workgroups returns at once.
+constexpr int kRerunGroups = 1024;
+
+void launch_maccormack_dye_fused(hipStream_t s, const float* u, const float* v, const float* qn,
+                                 float* qo, const uint8_t* lbl, int nx, int ny, float c, float k0,
+                                 float k1, float k2, float* speed, int* fail, int epoch) {
+  k_maccormack_dye_fused<kTyDyeFused><<<grid_t<kTyDyeFused>(nx, ny), dim3(BX, BYF), 0, s>>>(
+      u, v, qn, qo, lbl, nx, ny, c, k0, k1, k2, speed, fail, epoch);
+}
+
+void launch_maccormack_dye_rerun(hipStream_t s, const float* u, const float* v, const float* qn,
+                                 float* qh, float* qo, const uint8_t* lbl, int nx, int ny, float c,
+                                 float k0, float k1, float k2, uint8_t* sten, const int* fail,
+                                 int epoch) {
+  const dim3 gf = grid_t<kTyAdvectDye>(nx, ny), gc = grid_t<kTyMacCormackDye>(nx, ny);
+  const int nf = (int)(gf.x * gf.y), nc = (int)(gc.x * gc.y);
+  k_dye_forward_rerun<kTyAdvectDye><<<std::min(nf, kRerunGroups), dim3(BX, BY), 0, s>>>(
+      u, v, qn, qh, lbl, nx, ny, c, sten, fail, epoch, (int)gf.x, nf);
+  k_dye_correct_rerun<kTyMacCormackDye><<<std::min(nc, kRerunGroups), dim3(BX, BY), 0, s>>>(
+      u, v, qn, qh, qo, lbl, nx, ny, c, k0, k1, k2, sten, fail, epoch, (int)gc.x, nc);
 }
 
 // ---- batched launches ----------------------------------------------------------

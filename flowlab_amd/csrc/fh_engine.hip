@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <atomic>
 #include <cmath>
 #include <cstdio>
@@ -877,6 +878,13 @@ bool buoyancy_fused(const fh_engine* e, int scheme, double dt) {
          (float)e->confinement == 0.f && (float)(e->buoyancy * dt) != 0.f;
 }
 
+// The epoch of the next one-pass MacCormack dye launch: every launch's
+// differs from the mark any earlier launch can have left (epochs start at 1).
+int next_one_pass_epoch(fh_engine* e) {
+  e->one_pass_epoch = e->one_pass_epoch == INT_MAX ? 1 : e->one_pass_epoch + 1;
+  return e->one_pass_epoch;
+}
+
 int advect_velocity(fh_engine* e, int scheme, double dt, bool with_buoyancy = false) {
   float c = (float)(dt / e->h);
   if (scheme == FH_MACCORMACK) {
@@ -937,17 +945,30 @@ int dye_phase(fh_engine* e, int scheme, double dt, const double keep[3], bool sp
   float k0 = keep ? (float)keep[0] : 1.f, k1 = keep ? (float)keep[1] : 1.f,
         k2 = keep ? (float)keep[2] : 1.f;
   float* sp = (speed && speed_in_dye(e)) ? e->fpartials : nullptr;
+  const bool fused = scheme == FH_MACCORMACK && e->advect_lds && e->ds == 1 && e->dye_one_pass;
   auto speed_done = [&]() -> int {
     if (!sp) return FH_OK;
-    launch_finalize_max(e->s, e->fpartials, advect_dye_speed_blocks(e->nx, e->ny), e->fmax,
-                        e->d_fmax);
+    launch_finalize_max(e->s, e->fpartials,
+                        fused ? maccormack_dye_fused_blocks(e->nx, e->ny)
+                              : advect_dye_speed_blocks(e->nx, e->ny),
+                        e->fmax, e->d_fmax);
     LAUNCHED("max_speed");
     HIPCHK(hipEventRecord(e->fmax_ev, e->s));
     e->fmax_epoch = e->vel_epoch;
     return FH_OK;
   };
   int st;
-  if (scheme == FH_MACCORMACK) {
+  if (fused) {
+    const int epoch = next_one_pass_epoch(e);
+    {
+      Trace t(e, FH_K_MACCORMACK_DYE);
+      launch_maccormack_dye_fused(e->s, e->u, e->v, e->dye, e->dye2, e->lbl, e->nx, e->ny, c, k0,
+                                  k1, k2, sp, e->one_pass_fail, epoch);
+    }
+    launch_maccormack_dye_rerun(e->s, e->u, e->v, e->dye, e->dyeh, e->dye2, e->lbl, e->nx, e->ny,
+                                c, k0, k1, k2, e->sten_d, e->one_pass_fail, epoch);
+    if ((st = speed_done()) != FH_OK) return st;
+  } else if (scheme == FH_MACCORMACK) {
     // the dye LDS kernels (dye scale 1) record and read the limiter stencils
     uint8_t* sten_d = (e->advect_lds && e->ds == 1) ? e->sten_d : nullptr;
     {
@@ -1035,7 +1056,7 @@ fh_engine* fh_create(int nx, int ny, int dye_scale, double h) {
       dalloc(e, &e->dye, 3 * nd) || dalloc(e, &e->dye2, 3 * nd) || dalloc(e, &e->dyeh, 3 * nd) ||
       dalloc(e, &e->w, nc) || dalloc(e, &e->lbl, nc) || dalloc(e, &e->p, nc) ||
       dalloc(e, &e->sten_f, (size_t)(nx + 1) * (ny + 1)) || dalloc(e, &e->sten_d, nd) ||
-      dalloc(e, &e->p2, nc) || dalloc(e, &e->div, nc))
+      dalloc(e, &e->one_pass_fail, 1) || dalloc(e, &e->p2, nc) || dalloc(e, &e->div, nc))
     return bail();
   e->nlev_ref = level_count(nx, ny, 4);
   e->nlev_dc = level_count(nx, ny, 16);
@@ -1111,6 +1132,7 @@ void fh_destroy(fh_engine* e) {
   if (e->lbl) (void)hipFree(e->lbl);
   if (e->sten_f) (void)hipFree(e->sten_f);
   if (e->sten_d) (void)hipFree(e->sten_d);
+  if (e->one_pass_fail) (void)hipFree(e->one_pass_fail);
   if (e->cells) (void)hipFree(e->cells);
   if (e->cinfo) (void)hipFree(e->cinfo);
   if (e->patch_tiles) (void)hipFree(e->patch_tiles);
@@ -1688,6 +1710,12 @@ int fh_set_coarse_group(fh_engine* e, int group) {
 int fh_set_advect_staging(fh_engine* e, int on) {
   GUARD("setAdvectStaging()");
   e->advect_lds = on != 0;
+  return FH_OK;
+}
+
+int fh_set_dye_one_pass(fh_engine* e, int on) {
+  GUARD("setDyeOnePass()");
+  e->dye_one_pass = on != 0;
   return FH_OK;
 }
 

@@ -56,6 +56,9 @@ struct fh_engine {
   int tile[fh::kMaxTiledLevels] = {};  // tile shape per tiled level (kTileShapes)
   int coarse_group = 0;            // coarse sub-cycle workgroup (kCoarseGroups)
   bool advect_lds = true;          // LDS-staged advection kernels
+  bool dye_one_pass = true;        // MacCormack dye (scale 1) in one pass
+  int* one_pass_fail = nullptr;    // the one-pass kernel's reach mark (a launch's epoch)
+  int one_pass_epoch = 0;          // its launch count
   // a defect-correction cycle's finest prolongation + post-smoothing and its
   // update pass run as one kernel (launch_prolong_smooth_dc); its residual
   // goes to b2, then b2 and the finest level's b swap

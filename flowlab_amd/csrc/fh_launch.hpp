@@ -53,6 +53,22 @@ void launch_maccormack_dye(hipStream_t s, const float* u, const float* v, const 
                            const float* qh, float* qo, const uint8_t* lbl, int nx, int ny, int ds,
                            float c, float k0, float k1, float k2, bool lds,
                            const uint8_t* sten = nullptr);
+// MacCormack dye in one pass (dye scale 1): each tile computes the
+// forward-pass values its correction samples itself, in LDS.  A tile whose
+// velocity exceeds the staged reach writes nothing and marks the launch
+// (fail = epoch; epoch must differ from every earlier launch's on fail);
+// the matching *_rerun launches, enqueued right after, then run the two-pass
+// kernels over every tile (the forward pass into qh and the stencils into
+// sten), and otherwise return at once.  speed (may be null):
+// max-speed partials, maccormack_dye_fused_blocks of them.
+void launch_maccormack_dye_fused(hipStream_t s, const float* u, const float* v, const float* qn,
+                                 float* qo, const uint8_t* lbl, int nx, int ny, float c, float k0,
+                                 float k1, float k2, float* speed, int* fail, int epoch);
+void launch_maccormack_dye_rerun(hipStream_t s, const float* u, const float* v, const float* qn,
+                                 float* qh, float* qo, const uint8_t* lbl, int nx, int ny, float c,
+                                 float k0, float k1, float k2, uint8_t* sten, const int* fail,
+                                 int epoch);
+int maccormack_dye_fused_blocks(int nx, int ny);
 
 // ---- forces, divergence, projection (fh_project.hip) ----------------------
 void launch_vorticity(hipStream_t s, const float* u, const float* v, float* w, int nx, int ny,

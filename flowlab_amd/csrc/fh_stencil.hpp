@@ -215,29 +215,26 @@ __device__ __forceinline__ Pos dye_pos(int i, int ds) {
   return {i / ds, ((float)(i % ds) + 0.5f) / (float)ds};
 }
 
-// advect.ts:241-270 for three channels sharing one backtrace; out *= keep.
-// q0..q2 are the channels as accessors over the dye grid (dnx x dny); the
-// output planes are qo, qo + plane, qo + 2 plane.  sten (may be null): one
-// byte per cell, the origin of the stencil MacCormack's limiter clamps to
-// (maccormack_dye_t's forward trace; 0 for solid cells).  solid: the cell's
-// solid_or_outside test when the caller has already loaded it (-1: read lbl).
+// advect.ts:241-270 for three channels sharing one backtrace, at dye cell
+// (i, j) inside the dye grid: out = the three samples times keep; returns
+// the origin of the stencil MacCormack's limiter clamps to, relative to
+// (i, j) (sten2; maccormack_dye_t's forward trace; 0 for solid cells).  q0..q2
+// are the channels as accessors over the dye grid (dnx x dny).  solid: the
+// cell's solid_or_outside test when the caller has already loaded it (-1:
+// read lbl).
 template <class CU, class CV, class Q>
-__device__ __forceinline__ void advect_dye_t(const CU& u, const CV& v, const Q& q0, const Q& q1,
-                                             const Q& q2, float* __restrict__ qo,
-                                             const uint8_t* __restrict__ lbl, int nx, int ny,
-                                             int ds, float c, float k0, float k1, float k2, int i,
-                                             int j, uint8_t* __restrict__ sten = nullptr,
-                                             int solid = -1) {
-  int dnx = nx * ds, dny = ny * ds;
-  if (i >= dnx || j >= dny) return;
-  size_t plane = (size_t)dnx * dny;
-  int k = i + j * dnx;
+__device__ __forceinline__ uint32_t advect_dye_cell(const CU& u, const CV& v, const Q& q0,
+                                                    const Q& q1, const Q& q2,
+                                                    const uint8_t* __restrict__ lbl, int nx,
+                                                    int ny, int ds, float c, float k0, float k1,
+                                                    float k2, int i, int j, int solid,
+                                                    float (&out)[3]) {
+  const int dnx = nx * ds, dny = ny * ds;
   if (solid >= 0 ? solid != 0 : solid_or_outside(lbl, nx, ny, i / ds, j / ds)) {
-    qo[k] = q0.at(i, j) * k0;
-    qo[k + plane] = q1.at(i, j) * k1;
-    qo[k + 2 * plane] = q2.at(i, j) * k2;
-    if (sten) sten[k] = 0;
-    return;
+    out[0] = q0.at(i, j) * k0;
+    out[1] = q1.at(i, j) * k1;
+    out[2] = q2.at(i, j) * k2;
+    return 0;
   }
   Pos X = dye_pos(i, ds), Y = dye_pos(j, ds);
   float u0 = sample_uf(u, nx, ny, X, Y);
@@ -246,16 +243,38 @@ __device__ __forceinline__ void advect_dye_t(const CU& u, const CV& v, const Q& 
   backtracef(u, v, nx, ny, X, Y, u0, v0, c, &bx, &by);
   // the sample at the backtrace: dye cell i's centre sits at i + 1/2 in
   // dye-cell units, so the stencil axis is i + ds (bx - X) -- the axes the
-  // limiter of maccormack_dye_t clamps with, recorded in sten
+  // limiter of maccormack_dye_t clamps with, returned as the stencil byte
   Axis ax = axis_t<Q::kClamp>(i, ds * (bx.f - X.f), dnx);
   Axis ay = axis_t<Q::kClamp>(j, ds * (by.f - Y.f), dny);
-  if (sten) sten[k] = (uint8_t)sten2(ax.i0 - i, ay.i0 - j);
   Quad g0 = gather4f(q0, ax, ay);
   Quad g1 = gather4f(q1, ax, ay);
   Quad g2 = gather4f(q2, ax, ay);
-  qo[k] = bilerp(g0.a, g0.b, g0.c, g0.d, ax.f, ay.f) * k0;
-  qo[k + plane] = bilerp(g1.a, g1.b, g1.c, g1.d, ax.f, ay.f) * k1;
-  qo[k + 2 * plane] = bilerp(g2.a, g2.b, g2.c, g2.d, ax.f, ay.f) * k2;
+  out[0] = bilerp(g0.a, g0.b, g0.c, g0.d, ax.f, ay.f) * k0;
+  out[1] = bilerp(g1.a, g1.b, g1.c, g1.d, ax.f, ay.f) * k1;
+  out[2] = bilerp(g2.a, g2.b, g2.c, g2.d, ax.f, ay.f) * k2;
+  return sten2(ax.i0 - i, ay.i0 - j);
+}
+
+// advect_dye_cell stored: the output planes are qo, qo + plane, qo + 2 plane;
+// sten (may be null): the stencil bytes, one per cell.
+template <class CU, class CV, class Q>
+__device__ __forceinline__ void advect_dye_t(const CU& u, const CV& v, const Q& q0, const Q& q1,
+                                             const Q& q2, float* __restrict__ qo,
+                                             const uint8_t* __restrict__ lbl, int nx, int ny,
+                                             int ds, float c, float k0, float k1, float k2, int i,
+                                             int j, uint8_t* __restrict__ sten = nullptr,
+                                             int solid = -1) {
+  const int dnx = nx * ds, dny = ny * ds;
+  if (i >= dnx || j >= dny) return;
+  const size_t plane = (size_t)dnx * dny;
+  const int k = i + j * dnx;
+  float o[3];
+  const uint32_t st = advect_dye_cell(u, v, q0, q1, q2, lbl, nx, ny, ds, c, k0, k1, k2, i, j,
+                                      solid, o);
+  qo[k] = o[0];
+  qo[k + plane] = o[1];
+  qo[k + 2 * plane] = o[2];
+  if (sten) sten[k] = (uint8_t)st;
 }
 
 __device__ __forceinline__ void advect_dye_at(const float* __restrict__ u,

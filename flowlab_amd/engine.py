@@ -224,6 +224,11 @@ class HipEngine:
         """LDS-staged advection kernels (default) or plain global gathers."""
         check(self.lib.fh_set_advect_staging(self._e, int(bool(on))))
 
+    def set_dye_one_pass(self, on: bool):
+        """MacCormack dye advection in one kernel (default) or as its two
+        passes."""
+        check(self.lib.fh_set_dye_one_pass(self._e, int(bool(on))))
+
     def coarse_group_threads(self) -> int:
         return int(self.lib.fh_coarse_group_threads(self._e))
 

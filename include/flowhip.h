@@ -207,6 +207,13 @@ int fh_coarse_group_threads(fh_engine* e);
  * (the default) stages each workgroup's input windows through LDS; 0 gathers
  * straight from global memory.  Results are the same bits either way. */
 int fh_set_advect_staging(fh_engine* e, int on);
+/* MacCormack dye at dye scale 1 with staging on (advect.ts:241-270 then
+ * 291-311): on != 0 (the default) runs the forward pass and the correction
+ * as one kernel, which computes the forward-pass samples a tile's
+ * correction reads in LDS instead of storing and re-reading them; 0 runs
+ * the two passes.  Results are the same bits either way.  No counterpart
+ * in the reference. */
+int fh_set_dye_one_pass(fh_engine* e, int on);
 
 /* ---- allocation-free pressure solve (no reference counterpart) ----------
  * For callers that cannot allocate device memory (code under HIP graph
