@@ -12,12 +12,18 @@ struct dim3 {
 };
 extern dim3 threadIdx, blockIdx, gridDim, blockDim;
 struct EmuBarrier {};
-extern int emu_phase;
-// Each emulated thread runs the kernel twice: in phase 0 up to its first
-// barrier (the LDS staging, whose writes are idempotent), in phase 1 to the
-// end with every stage already in place.
+extern int emu_phase, emu_barriers;
+// Each emulated thread runs the kernel once per phase: in phase p up to its
+// (p+1)-th barrier, so that everything the workgroup writes to LDS before
+// that barrier is in place when any thread reads past it (the kernels' LDS
+// writes between barriers are idempotent), and in the last phase to the end.
 inline void __syncthreads() {
-  if (emu_phase == 0) throw EmuBarrier{};
+  if (++emu_barriers > emu_phase) throw EmuBarrier{};
+}
+inline int atomicExch(int* p, int v) {
+  const int o = *p;
+  *p = v;
+  return o;
 }
 typedef void* hipStream_t;
 #define __device__

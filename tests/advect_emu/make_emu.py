@@ -19,10 +19,10 @@ def make(out: str) -> None:
     dev = open(p).read()
     subs = [
         ("__device__ __forceinline__ float at(int i, int j) const { return l[off(i, j)]; }",
-         "__device__ __forceinline__ float at(int i, int j) const { emu_inwin(i - x0, j - y0, 0); "
-         "return l[off(i, j)]; }"),
+         "__device__ __forceinline__ float at(int i, int j) const { emu_inwin(i - x0, j - y0, 0, "
+         "w); return l[off(i, j)]; }"),
         ("    lds_cfp r = l + off(i, j);",
-         "    emu_inwin(i - x0, j - y0, 1);\n    lds_cfp r = l + off(i, j);"),
+         "    emu_inwin(i - x0, j - y0, 1, w);\n    lds_cfp r = l + off(i, j);"),
         ("    const float fl = floorf(off);\n    return {base + (int)fl, off - fl};",
          "    const float fl = floorf(off);\n    Axis r{base + (int)fl, off - fl};\n"
          "    emu_axis_check(r, clamped_axis(base, off, count));\n    return r;"),
@@ -35,11 +35,23 @@ def make(out: str) -> None:
     src = open(os.path.join(CSRC, "fh_advect.hip")).read()
     src = re.sub(r"(\w+(?:<\w+>)?)<<<(.*?), (dim3\([^)]*\)), 0, s>>>\(", r"emu_launch(\1, \2, \3, ",
                  src, flags=re.S)
-    n = src.count("within_reach(m, c, tid)")
-    if n != 4:
-        raise RuntimeError(f"fh_advect.hip changed: {n} reach tests")
-    src = src.replace("within_reach(m, c, tid)", "emu_within_reach(lds, c, kReach, WN)")
-    src += "\nint emu_win_w = fh::WW, emu_win_h = 0;\n"
+    subs = [
+        # the two-pass tile bodies
+        ("within_reach(m, c, tid)", "emu_within_reach(lds, c, kReach, WN, WW)", 4),
+        ("speed ? within_reach_speed(m, sp, c, tid, speed, bid) : "
+         "emu_within_reach(lds, c, kReach, WN, WW)",
+         "emu_within_reach(lds, c, kReach, WN, WW)", 1),
+        # the one-pass dye kernel: its windows, and its ring before it is read
+        ("speed ? within_reach_speed<NW>(m, sp, c, tid, speed, bid)\n"
+         "                           : within_reach<NW>(m, c, tid)",
+         "emu_within_reach(lds, c, kReach, T::WN, WWF)", 1),
+        ("    const F h0 = window_as<F>(hat, 0,",
+         "    emu_win_h(T::RW) = T::RH;\n    const F h0 = window_as<F>(hat, 0,", 1),
+    ]
+    for a, b, k in subs:
+        if src.count(a) != k:
+            raise RuntimeError(f"fh_advect.hip changed: {a[:60]!r} found {src.count(a)} times")
+        src = src.replace(a, b)
     open(os.path.join(out, "fh_advect_emu.cpp"), "w").write('#include "emu.hpp"\n' + src)
 
 

@@ -5,7 +5,10 @@ tiles include the interior margin exactly, at CFL numbers below and above the
 reach limit.  The emulation also aborts on any LDS read outside its window
 and on any clamp-free sampling axis (interior tiles) that differs from the
 clamped one, so it shows on the CPU that the unchecked reads stay in bounds
-and that dropping the clamp changes no index or weight."""
+and that dropping the clamp changes no index or weight.  The one-pass
+MacCormack dye kernel (its staged windows and its forward-pass ring, each
+read checked against its own extent) runs against the two global-gather
+passes too, at CFL numbers on both sides of its reach test."""
 import os
 import re
 import subprocess
@@ -34,3 +37,7 @@ def test_lds_advection_matches_global_gathers(tmp_path):
     m = re.search(r"staged blocks (\d+), global blocks (\d+), checked LDS reads (\d+), "
                   r"clamp-free axes (\d+)", r.stdout)
     assert m and all(int(g) > 0 for g in m.groups()), r.stdout
+    # some one-pass dye launches went over the reach (the two-pass kernels
+    # redid them), the others did not
+    m = re.search(r"one-pass dye launches marked over the reach (\d+)", r.stdout)
+    assert m and 0 < int(m.group(1)) < 35, r.stdout
