@@ -541,6 +541,21 @@ __global__ __launch_bounds__(TX * BYF) void k_maccormack_dye_fused(
   const size_t plane = (size_t)nx * ny;
   const Win f[5] = {{u, nx + 1, ny}, {v, nx, ny + 1}, {q, nx, ny}, {q + plane, nx, ny},
                     {q + 2 * plane, nx, ny}};
+  // the solid tests of this thread's ring cells and tile rows, loaded with
+  // the windows (-1: outside the grid, never computed)
+  constexpr int RI = (T::RN + NT - 1) / NT, R = TY / BYF;
+  int rsol[RI], tsol[R];
+#pragma unroll
+  for (int k = 0; k < RI; k++) {
+    const int e = tid + k * NT, rb = e / T::RW, ra = e - rb * T::RW;
+    const int i = rx0 + ra, j = ry0 + rb;
+    rsol[k] = (e < T::RN && i >= 0 && j >= 0 && i < nx && j < ny) ? lbl[i + j * nx] == SOLID : -1;
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = x0 + (int)threadIdx.x, j = y0 + (int)threadIdx.y + r * BYF;
+    tsol[r] = (i < nx && j < ny) ? lbl[i + j * nx] == SOLID : 1;
+  }
   float sp = 0.f;
   const float m = stage_windows<5, T::WN, false, WWF, NT>(lds, f, wx0, wy0, tid,
                                                            speed ? &sp : nullptr);
@@ -558,13 +573,14 @@ __global__ __launch_bounds__(TX * BYF) void k_maccormack_dye_fused(
             n2 = window_as<F>(lds, 4, wx0, wy0, WWF);
     // the forward pass (keep 1, as the two-pass forward kernel runs it) on
     // the ring's cells inside the grid; the others are never read
-    for (int e = tid; e < T::RN; e += NT) {
-      const int rb = e / T::RW, ra = e - rb * T::RW;
+#pragma unroll
+    for (int k = 0; k < RI; k++) {
+      if (rsol[k] < 0) continue;
+      const int e = tid + k * NT, rb = e / T::RW, ra = e - rb * T::RW;
       const int i = rx0 + ra, j = ry0 + rb;
-      if (i < 0 || j < 0 || i >= nx || j >= ny) continue;
       float o[3];
       const uint32_t st =
-          advect_dye_cell(a, b, n0, n1, n2, lbl, nx, ny, 1, c, 1.f, 1.f, 1.f, i, j, -1, o);
+          advect_dye_cell(a, b, n0, n1, n2, lbl, nx, ny, 1, c, 1.f, 1.f, 1.f, i, j, rsol[k], o);
       if (ra >= 2 && ra < TX + 2 && rb >= 2 && rb < TY + 2)
         hst[(rb - 2) * TX + (ra - 2)] = (uint8_t)st;
       hat[0][e] = o[0];
@@ -575,11 +591,11 @@ __global__ __launch_bounds__(TX * BYF) void k_maccormack_dye_fused(
     const F h0 = window_as<F>(hat, 0, rx0, ry0, T::RW), h1 = window_as<F>(hat, 1, rx0, ry0, T::RW),
             h2 = window_as<F>(hat, 2, rx0, ry0, T::RW);
     const int i = x0 + (int)threadIdx.x;
-    for (int r = 0; r < TY / BYF; r++) {
+    for (int r = 0; r < R; r++) {
       const int j = y0 + (int)threadIdx.y + r * BYF;
       const uint32_t st = (i < nx && j < ny) ? hst[(j - y0) * TX + (i - x0)] : 0u;
       maccormack_dye_t(a, b, n0, n1, n2, h0, h1, h2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i, j,
-                       true, st);
+                       true, st, tsol[r]);
     }
   };
   if (interior_ring<TY>(x0, y0, nx, ny))
