@@ -485,9 +485,9 @@ __global__ __launch_bounds__(256) void k_maccormack_dye_lds(
 // again.  Here a workgroup stages u, v and the three channels of q^n over
 // its tile and a halo of AF = 4 cells, computes the forward pass itself on
 // the tile plus a ring of two cells (every qHat the tile's reverse traces
-// sample), keeps those in LDS with the tile's stencil bytes, and corrects
-// its tile: 33 B per cell (u, v 8, q^n 12, label 1, out 12), at a third
-// more forward-pass arithmetic for the ring.  Each value is computed by the
+// sample), keeps those in LDS (the tile's own centre velocities and stencil
+// bytes in registers), and corrects its tile: 33 B per cell (u, v 8, q^n 12,
+// label 1, out 12), at a third more forward-pass arithmetic for the ring.  Each value is computed by the
 // same functions on the same staged values, so the outputs are the two
 // passes' bits.
 //
@@ -504,8 +504,8 @@ __global__ __launch_bounds__(256) void k_maccormack_dye_lds(
 // of every tile, as in the forward kernel.
 constexpr int AF = 4, WWF = TX + 2 * AF, BYF = 8, kTyDyeFused = 16;
 
-// 5 windows, the ring's 3 channels and the tile's stencil bytes: 51.9 KB,
-// three workgroups of 512 threads per CU.
+// 5 windows and the ring's 3 channels: 50.9 KB, three workgroups of 512
+// threads per CU.
 template <int TY>
 struct FusedTile {
   static_assert(TY % BYF == 0, "rows per thread");
@@ -523,16 +523,53 @@ __device__ __forceinline__ bool interior_ring(int x0, int y0, int nx, int ny) {
          y0 + TY + 1 <= ny - 1 - kEdge;
 }
 
+// Cells of the forward-pass ring of a TX x TY tile, by thread: k < R are the
+// thread's own tile cells (threadIdx.x, threadIdx.y + k BYF), so that their
+// centre velocity and stencil byte carry over to the correction in
+// registers; k = R is one of the ring's 4 (RW + TY) cells outside the tile
+// (threads below that count): two rows above, two below, two columns either
+// side.  (ra, rb): the cell in ring coordinates; false when the thread has
+// none.
+template <int TY>
+__device__ __forceinline__ bool ring_cell(int k, int* ra, int* rb) {
+  using T = FusedTile<TY>;
+  constexpr int R = TY / BYF;
+  const int tid = threadIdx.x + threadIdx.y * TX;
+  if (k < R) {
+    *ra = (int)threadIdx.x + 2;
+    *rb = (int)threadIdx.y + k * BYF + 2;
+    return true;
+  }
+  if (tid < 2 * T::RW) {
+    *rb = tid / T::RW;
+    *ra = tid - *rb * T::RW;
+    return true;
+  }
+  if (tid < 4 * T::RW) {
+    const int t = tid - 2 * T::RW;
+    *rb = TY + 2 + t / T::RW;
+    *ra = t % T::RW;
+    return true;
+  }
+  if (tid < 4 * T::RW + 4 * TY) {
+    const int t = tid - 4 * T::RW, side = t & 3;
+    *rb = 2 + (t >> 2);
+    *ra = side < 2 ? side : TX + side;
+    return true;
+  }
+  return false;
+}
+
 template <int TY>
 __global__ __launch_bounds__(TX * BYF) void k_maccormack_dye_fused(
     const float* __restrict__ u, const float* __restrict__ v, const float* __restrict__ q,
     float* __restrict__ qo, const uint8_t* __restrict__ lbl, int nx, int ny, float c, float k0,
     float k1, float k2, float* __restrict__ speed, int* __restrict__ fail, int epoch) {
   using T = FusedTile<TY>;
-  constexpr int NT = TX * BYF, NW = NT / 64;
+  constexpr int NT = TX * BYF, NW = NT / 64, R = TY / BYF;
+  static_assert(4 * T::RW + 4 * TY <= NT, "one ring cell outside the tile per thread");
   __shared__ float lds[5][T::WN];
   __shared__ float hat[3][T::RN];
-  __shared__ uint8_t hst[TX * TY];
   int bx, by;
   xcd_block(&bx, &by);
   const int x0 = bx * TX, y0 = by * TY, wx0 = x0 - AF, wy0 = y0 - AF, rx0 = x0 - 2,
@@ -541,20 +578,14 @@ __global__ __launch_bounds__(TX * BYF) void k_maccormack_dye_fused(
   const size_t plane = (size_t)nx * ny;
   const Win f[5] = {{u, nx + 1, ny}, {v, nx, ny + 1}, {q, nx, ny}, {q + plane, nx, ny},
                     {q + 2 * plane, nx, ny}};
-  // the solid tests of this thread's ring cells and tile rows, loaded with
-  // the windows (-1: outside the grid, never computed)
-  constexpr int RI = (T::RN + NT - 1) / NT, R = TY / BYF;
-  int rsol[RI], tsol[R];
+  // this thread's ring cells (ring_cell) and their solid tests, loaded with
+  // the windows (-1: none, or outside the grid)
+  int ra[R + 1], rb[R + 1], sol[R + 1];
 #pragma unroll
-  for (int k = 0; k < RI; k++) {
-    const int e = tid + k * NT, rb = e / T::RW, ra = e - rb * T::RW;
-    const int i = rx0 + ra, j = ry0 + rb;
-    rsol[k] = (e < T::RN && i >= 0 && j >= 0 && i < nx && j < ny) ? lbl[i + j * nx] == SOLID : -1;
-  }
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    const int i = x0 + (int)threadIdx.x, j = y0 + (int)threadIdx.y + r * BYF;
-    tsol[r] = (i < nx && j < ny) ? lbl[i + j * nx] == SOLID : 1;
+  for (int k = 0; k <= R; k++) {
+    const bool has = ring_cell<TY>(k, &ra[k], &rb[k]);
+    const int i = rx0 + ra[k], j = ry0 + rb[k];
+    sol[k] = (has && i >= 0 && j >= 0 && i < nx && j < ny) ? lbl[i + j * nx] == SOLID : -1;
   }
   float sp = 0.f;
   const float m = stage_windows<5, T::WN, false, WWF, NT>(lds, f, wx0, wy0, tid,
@@ -573,16 +604,17 @@ __global__ __launch_bounds__(TX * BYF) void k_maccormack_dye_fused(
             n2 = window_as<F>(lds, 4, wx0, wy0, WWF);
     // the forward pass (keep 1, as the two-pass forward kernel runs it) on
     // the ring's cells inside the grid; the others are never read
+    uint32_t st[R] = {};
+    float uv0[R][2] = {};
 #pragma unroll
-    for (int k = 0; k < RI; k++) {
-      if (rsol[k] < 0) continue;
-      const int e = tid + k * NT, rb = e / T::RW, ra = e - rb * T::RW;
-      const int i = rx0 + ra, j = ry0 + rb;
+    for (int k = 0; k <= R; k++) {
+      if (sol[k] < 0) continue;
       float o[3];
-      const uint32_t st =
-          advect_dye_cell(a, b, n0, n1, n2, lbl, nx, ny, 1, c, 1.f, 1.f, 1.f, i, j, rsol[k], o);
-      if (ra >= 2 && ra < TX + 2 && rb >= 2 && rb < TY + 2)
-        hst[(rb - 2) * TX + (ra - 2)] = (uint8_t)st;
+      const uint32_t s = advect_dye_cell(a, b, n0, n1, n2, lbl, nx, ny, 1, c, 1.f, 1.f, 1.f,
+                                         rx0 + ra[k], ry0 + rb[k], sol[k], o,
+                                         k < R ? uv0[k < R ? k : 0] : nullptr);
+      if (k < R) st[k < R ? k : 0] = s;
+      const int e = rb[k] * T::RW + ra[k];
       hat[0][e] = o[0];
       hat[1][e] = o[1];
       hat[2][e] = o[2];
@@ -591,12 +623,11 @@ __global__ __launch_bounds__(TX * BYF) void k_maccormack_dye_fused(
     const F h0 = window_as<F>(hat, 0, rx0, ry0, T::RW), h1 = window_as<F>(hat, 1, rx0, ry0, T::RW),
             h2 = window_as<F>(hat, 2, rx0, ry0, T::RW);
     const int i = x0 + (int)threadIdx.x;
-    for (int r = 0; r < R; r++) {
-      const int j = y0 + (int)threadIdx.y + r * BYF;
-      const uint32_t st = (i < nx && j < ny) ? hst[(j - y0) * TX + (i - x0)] : 0u;
-      maccormack_dye_t(a, b, n0, n1, n2, h0, h1, h2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i, j,
-                       true, st, tsol[r]);
-    }
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      maccormack_dye_t(a, b, n0, n1, n2, h0, h1, h2, qo, lbl, nx, ny, 1, c, k0, k1, k2, i,
+                       y0 + (int)threadIdx.y + r * BYF, true, st[r], sol[r] < 0 ? 1 : sol[r],
+                       uv0[r]);
   };
   if (interior_ring<TY>(x0, y0, nx, ny))
     run(LFieldI{});

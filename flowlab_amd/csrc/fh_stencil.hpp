@@ -221,14 +221,15 @@ __device__ __forceinline__ Pos dye_pos(int i, int ds) {
 // (i, j) (sten2; maccormack_dye_t's forward trace; 0 for solid cells).  q0..q2
 // are the channels as accessors over the dye grid (dnx x dny).  solid: the
 // cell's solid_or_outside test when the caller has already loaded it (-1:
-// read lbl).
+// read lbl).  uv0 (may be null): the velocity sampled at the cell centre, u
+// then v, for an open cell (maccormack_dye_t samples the same).
 template <class CU, class CV, class Q>
 __device__ __forceinline__ uint32_t advect_dye_cell(const CU& u, const CV& v, const Q& q0,
                                                     const Q& q1, const Q& q2,
                                                     const uint8_t* __restrict__ lbl, int nx,
                                                     int ny, int ds, float c, float k0, float k1,
                                                     float k2, int i, int j, int solid,
-                                                    float (&out)[3]) {
+                                                    float (&out)[3], float* uv0 = nullptr) {
   const int dnx = nx * ds, dny = ny * ds;
   if (solid >= 0 ? solid != 0 : solid_or_outside(lbl, nx, ny, i / ds, j / ds)) {
     out[0] = q0.at(i, j) * k0;
@@ -239,6 +240,10 @@ __device__ __forceinline__ uint32_t advect_dye_cell(const CU& u, const CV& v, co
   Pos X = dye_pos(i, ds), Y = dye_pos(j, ds);
   float u0 = sample_uf(u, nx, ny, X, Y);
   float v0 = sample_vf(v, nx, ny, X, Y);
+  if (uv0) {
+    uv0[0] = u0;
+    uv0[1] = v0;
+  }
   Pos bx, by;
   backtracef(u, v, nx, ny, X, Y, u0, v0, c, &bx, &by);
   // the sample at the backtrace: dye cell i's centre sits at i + 1/2 in
@@ -292,7 +297,8 @@ __device__ __forceinline__ void advect_dye_at(const float* __restrict__ u,
 // advect.ts:291-311 for three channels, fused as for the faces; out *= keep.
 // has_sten: st is this cell's byte of the forward pass's clamp stencil
 // origins (advect_dye_t), used instead of tracing forward again.  solid as
-// for advect_dye_t.
+// for advect_dye_t.  uv0 (may be null): the centre velocity advect_dye_cell
+// sampled for this cell, used instead of sampling it again.
 template <class CU, class CV, class Q>
 __device__ __forceinline__ void maccormack_dye_t(const CU& u, const CV& v, const Q& n0,
                                                  const Q& n1, const Q& n2, const Q& h0,
@@ -300,7 +306,8 @@ __device__ __forceinline__ void maccormack_dye_t(const CU& u, const CV& v, const
                                                  const uint8_t* __restrict__ lbl, int nx, int ny,
                                                  int ds, float c, float k0, float k1, float k2,
                                                  int i, int j, bool has_sten = false,
-                                                 uint32_t st = 0, int solid = -1) {
+                                                 uint32_t st = 0, int solid = -1,
+                                                 const float* uv0 = nullptr) {
   int dnx = nx * ds, dny = ny * ds;
   if (i >= dnx || j >= dny) return;
   size_t plane = (size_t)dnx * dny;
@@ -312,8 +319,8 @@ __device__ __forceinline__ void maccormack_dye_t(const CU& u, const CV& v, const
     return;
   }
   Pos X = dye_pos(i, ds), Y = dye_pos(j, ds);
-  float u0 = sample_uf(u, nx, ny, X, Y);
-  float v0 = sample_vf(v, nx, ny, X, Y);
+  const float u0 = uv0 ? uv0[0] : sample_uf(u, nx, ny, X, Y);
+  const float v0 = uv0 ? uv0[1] : sample_vf(v, nx, ny, X, Y);
   Pos bx, by;
   backtracef(u, v, nx, ny, X, Y, u0, v0, -c, &bx, &by);
   Axis ax = axis_t<Q::kClamp>(i, ds * (bx.f - X.f), dnx);
