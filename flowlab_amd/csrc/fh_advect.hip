@@ -502,7 +502,7 @@ __global__ __launch_bounds__(256) void k_maccormack_dye_lds(
 // unless the mark is this launch's), so every output is the two passes'
 // either way.  The max-speed partials come from the staged velocity windows
 // of every tile, as in the forward kernel.
-constexpr int AF = 4, WWF = TX + 2 * AF, BYF = 8, kTyDyeFused = 16;
+constexpr int AF = 4, WWF = TX + 2 * AF, BYF = 8, kTyDyeFused = 16, kTyFacesFused = 16;
 
 // 5 windows and the ring's 3 channels: 50.9 KB, three workgroups of 512
 // threads per CU.
@@ -663,6 +663,111 @@ __global__ __launch_bounds__(256) void k_dye_correct_rerun(
   }
 }
 
+// ---- MacCormack velocity in one pass -----------------------------------------------
+// advect.ts:72-112 then 177-210 in one kernel, as k_maccormack_dye_fused does
+// for the dye: a workgroup stages u^n and v^n over its thread indices and a
+// halo of AF, runs the forward pass on them plus a ring of two (the tile's
+// reverse traces sample uHat, vHat there), keeps those in LDS (its own
+// indices' cross samples and stencil bytes in registers), and corrects its
+// tile, applying buoyancy alone on the open v faces as
+// k_maccormack_faces_lds does: 21 B per thread index (u, v 8 + out 8 +
+// label 1 + heat 4) instead of the two passes' 48.  A face (i, j)'s samples
+// read columns and rows i - 2 .. i + 2, j - 2 .. j + 2 when D <= kReach, so
+// the window covers the ring's.  Same bits as the two passes; over the
+// reach, the launch is marked and the two-pass kernels redo every tile
+// (k_faces_forward_rerun, k_faces_correct_rerun).
+template <int TY>
+__global__ __launch_bounds__(TX * BYF) void k_maccormack_faces_fused(
+    const float* __restrict__ un, const float* __restrict__ vn, float* __restrict__ uo,
+    float* __restrict__ vo, const uint8_t* __restrict__ lbl, int nx, int ny, float c,
+    const float* __restrict__ heat, float gain, int* __restrict__ fail, int epoch) {
+  using T = FusedTile<TY>;
+  constexpr int NT = TX * BYF, NW = NT / 64, R = TY / BYF;
+  static_assert(4 * T::RW + 4 * TY <= NT, "one ring index outside the tile per thread");
+  __shared__ float lds[2][T::WN];
+  __shared__ float hat[2][T::RN];
+  int bx, by;
+  xcd_block(&bx, &by);
+  const int x0 = bx * TX, y0 = by * TY, wx0 = x0 - AF, wy0 = y0 - AF, rx0 = x0 - 2,
+            ry0 = y0 - 2;
+  const int tid = threadIdx.x + threadIdx.y * TX;
+  const Win f[2] = {{un, nx + 1, ny}, {vn, nx, ny + 1}};
+  // this thread's ring indices (ring_cell) and their face solid tests,
+  // loaded with the windows (-1: none, or outside (nx+1) x (ny+1))
+  int ra[R + 1], rb[R + 1], cl[R + 1];
+#pragma unroll
+  for (int k = 0; k <= R; k++) {
+    const bool has = ring_cell<TY>(k, &ra[k], &rb[k]);
+    const int i = rx0 + ra[k], j = ry0 + rb[k];
+    cl[k] = (has && i >= 0 && j >= 0 && i <= nx && j <= ny) ? closed_faces(lbl, nx, ny, i, j)
+                                                            : -1;
+  }
+  const float m = stage_windows<2, T::WN, false, WWF, NT>(lds, f, wx0, wy0, tid);
+  if (!within_reach<NW>(m, c, tid)) {
+    if (tid == 0) atomicExch(fail, epoch);
+    return;
+  }
+  auto run = [&](auto tag) {
+    using F = decltype(tag);
+    const F a = window_as<F>(lds, 0, wx0, wy0, WWF), b = window_as<F>(lds, 1, wx0, wy0, WWF);
+    // the forward pass on the ring's indices; a face that does not exist
+    // (u at j = ny, v at i = nx) stays 0 and is never sampled
+    uint32_t st[R] = {};
+    float cross[R][2] = {};
+#pragma unroll
+    for (int k = 0; k <= R; k++) {
+      if (cl[k] < 0) continue;
+      float uu = 0.f, vv = 0.f;
+      const uint32_t s = advect_faces_cell(a, b, a, b, lbl, nx, ny, c, rx0 + ra[k], ry0 + rb[k],
+                                           cl[k], &uu, &vv, k < R ? cross[k < R ? k : 0] : nullptr);
+      if (k < R) st[k < R ? k : 0] = s;
+      const int e = rb[k] * T::RW + ra[k];
+      hat[0][e] = uu;
+      hat[1][e] = vv;
+    }
+    __syncthreads();
+    const F ha = window_as<F>(hat, 0, rx0, ry0, T::RW), hb = window_as<F>(hat, 1, rx0, ry0, T::RW);
+    const int i = x0 + (int)threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      maccormack_faces_t(a, b, ha, hb, uo, vo, lbl, nx, ny, c, i, y0 + (int)threadIdx.y + r * BYF,
+                         true, st[r], heat, gain, cl[r] < 0 ? 3 : cl[r], cross[r]);
+  };
+  if (interior_ring<TY>(x0, y0, nx, ny))
+    run(LFieldI{});
+  else
+    run(LField{});
+}
+
+// The two-pass velocity kernels over every tile, for a launch that
+// k_maccormack_faces_fused marked; otherwise each workgroup returns at once.
+template <int TY>
+__global__ __launch_bounds__(256) void k_faces_forward_rerun(
+    const float* __restrict__ u, const float* __restrict__ v, float* __restrict__ uh,
+    float* __restrict__ vh, const uint8_t* __restrict__ lbl, int nx, int ny, float c,
+    uint8_t* __restrict__ sten, const int* __restrict__ fail, int epoch, int gx, int n) {
+  if (*fail != epoch) return;
+  for (int t = (int)blockIdx.x; t < n; t += (int)gridDim.x) {
+    __syncthreads();
+    advect_faces_tile<TY>(u, v, uh, vh, lbl, nx, ny, c, sten, t % gx, t / gx);
+  }
+}
+
+template <int TY>
+__global__ __launch_bounds__(256) void k_faces_correct_rerun(
+    const float* __restrict__ un, const float* __restrict__ vn, const float* __restrict__ uh,
+    const float* __restrict__ vh, float* __restrict__ uo, float* __restrict__ vo,
+    const uint8_t* __restrict__ lbl, int nx, int ny, float c, const uint8_t* __restrict__ sten,
+    const float* __restrict__ heat, float gain, const int* __restrict__ fail, int epoch, int gx,
+    int n) {
+  if (*fail != epoch) return;
+  for (int t = (int)blockIdx.x; t < n; t += (int)gridDim.x) {
+    __syncthreads();
+    maccormack_faces_tile<TY>(un, vn, uh, vh, uo, vo, lbl, nx, ny, c, sten, heat, gain, t % gx,
+                              t / gx);
+  }
+}
+
 template <int TY>
 static inline dim3 grid_t(int w, int h) {
   return dim3((w + TX - 1) / TX, (h + TY - 1) / TY);
@@ -745,6 +850,27 @@ void launch_maccormack_dye_rerun(hipStream_t s, const float* u, const float* v, 
       u, v, qn, qh, lbl, nx, ny, c, sten, fail, epoch, (int)gf.x, nf);
   k_dye_correct_rerun<kTyMacCormackDye><<<std::min(nc, kRerunGroups), dim3(BX, BY), 0, s>>>(
       u, v, qn, qh, qo, lbl, nx, ny, c, k0, k1, k2, sten, fail, epoch, (int)gc.x, nc);
+}
+
+void launch_maccormack_faces_fused(hipStream_t s, const float* un, const float* vn, float* uo,
+                                   float* vo, const uint8_t* lbl, int nx, int ny, float c,
+                                   const float* heat, float gain, int* fail, int epoch) {
+  const dim3 g = grid_t<kTyFacesFused>(nx + 1, ny + 1);
+  k_maccormack_faces_fused<kTyFacesFused><<<g, dim3(BX, BYF), 0, s>>>(un, vn, uo, vo, lbl, nx, ny, c,
+                                                                    heat, gain, fail, epoch);
+}
+
+void launch_maccormack_faces_rerun(hipStream_t s, const float* un, const float* vn, float* uh,
+                                   float* vh, float* uo, float* vo, const uint8_t* lbl, int nx,
+                                   int ny, float c, uint8_t* sten, const float* heat, float gain,
+                                   const int* fail, int epoch) {
+  const dim3 gf = grid_t<kTyAdvectFaces>(nx + 1, ny + 1),
+             gc = grid_t<kTyMacCormackFaces>(nx + 1, ny + 1);
+  const int nf = (int)(gf.x * gf.y), nc = (int)(gc.x * gc.y);
+  k_faces_forward_rerun<kTyAdvectFaces><<<std::min(nf, kRerunGroups), dim3(BX, BY), 0, s>>>(
+      un, vn, uh, vh, lbl, nx, ny, c, sten, fail, epoch, (int)gf.x, nf);
+  k_faces_correct_rerun<kTyMacCormackFaces><<<std::min(nc, kRerunGroups), dim3(BX, BY), 0, s>>>(
+      un, vn, uh, vh, uo, vo, lbl, nx, ny, c, sten, heat, gain, fail, epoch, (int)gc.x, nc);
 }
 
 // ---- batched launches ----------------------------------------------------------

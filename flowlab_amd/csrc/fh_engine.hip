@@ -887,7 +887,18 @@ int next_one_pass_epoch(fh_engine* e) {
 
 int advect_velocity(fh_engine* e, int scheme, double dt, bool with_buoyancy = false) {
   float c = (float)(dt / e->h);
-  if (scheme == FH_MACCORMACK) {
+  if (scheme == FH_MACCORMACK && e->advect_lds && e->faces_one_pass) {
+    const int epoch = next_one_pass_epoch(e);
+    const float* heat = with_buoyancy ? e->dye : nullptr;
+    const float gain = (float)(e->buoyancy * dt);
+    {
+      Trace t(e, FH_K_MACCORMACK_FACES);
+      launch_maccormack_faces_fused(e->s, e->u, e->v, e->u2, e->v2, e->lbl, e->nx, e->ny, c, heat,
+                                    gain, e->one_pass_fail, epoch);
+    }
+    launch_maccormack_faces_rerun(e->s, e->u, e->v, e->uh, e->vh, e->u2, e->v2, e->lbl, e->nx,
+                                  e->ny, c, e->sten_f, heat, gain, e->one_pass_fail, epoch);
+  } else if (scheme == FH_MACCORMACK) {
     {
       Trace t(e, FH_K_ADVECT_FACES);
       launch_advect_faces(e->s, e->u, e->v, e->u, e->v, e->uh, e->vh, e->lbl, e->nx, e->ny, c,
@@ -1710,6 +1721,12 @@ int fh_set_coarse_group(fh_engine* e, int group) {
 int fh_set_advect_staging(fh_engine* e, int on) {
   GUARD("setAdvectStaging()");
   e->advect_lds = on != 0;
+  return FH_OK;
+}
+
+int fh_set_faces_one_pass(fh_engine* e, int on) {
+  GUARD("setFacesOnePass()");
+  e->faces_one_pass = on != 0;
   return FH_OK;
 }
 

@@ -57,6 +57,7 @@ struct fh_engine {
   int coarse_group = 0;            // coarse sub-cycle workgroup (kCoarseGroups)
   bool advect_lds = true;          // LDS-staged advection kernels
   bool dye_one_pass = true;        // MacCormack dye (scale 1) in one pass
+  bool faces_one_pass = true;      // MacCormack velocity in one pass
   int* one_pass_fail = nullptr;    // the one-pass kernel's reach mark (a launch's epoch)
   int one_pass_epoch = 0;          // its launch count
   // a defect-correction cycle's finest prolongation + post-smoothing and its

@@ -69,6 +69,16 @@ void launch_maccormack_dye_rerun(hipStream_t s, const float* u, const float* v, 
                                  float k0, float k1, float k2, uint8_t* sten, const int* fail,
                                  int epoch);
 int maccormack_dye_fused_blocks(int nx, int ny);
+// The same for the velocity (advect.ts:72-112 then 177-210; heat, gain:
+// buoyancy alone, as launch_maccormack_faces applies it), the forward pass
+// of the two-pass kernels into uh, vh.
+void launch_maccormack_faces_fused(hipStream_t s, const float* un, const float* vn, float* uo,
+                                   float* vo, const uint8_t* lbl, int nx, int ny, float c,
+                                   const float* heat, float gain, int* fail, int epoch);
+void launch_maccormack_faces_rerun(hipStream_t s, const float* un, const float* vn, float* uh,
+                                   float* vh, float* uo, float* vo, const uint8_t* lbl, int nx,
+                                   int ny, float c, uint8_t* sten, const float* heat, float gain,
+                                   const int* fail, int epoch);
 
 // ---- forces, divergence, projection (fh_project.hip) ----------------------
 void launch_vorticity(hipStream_t s, const float* u, const float* v, float* w, int nx, int ny,

@@ -49,11 +49,63 @@ __device__ __forceinline__ int closed_faces(const uint8_t* __restrict__ lbl, int
 }
 
 // advect.ts:72-112 -- one thread index covers u(i,j) when j<ny and v(i,j)
-// when i<nx, over (nx+1) x (ny+1).  uc/vc carry, us/vs are advected.  sten
-// (may be null): the stencil origins of both faces' samples, one byte per
-// thread index (u face in bits 0-3, v face in bits 4-7; 0 for solid faces).
-// closed: the two faces' solid tests when the caller has loaded them
-// (closed_faces; -1: read lbl here).
+// when i<nx, over (nx+1) x (ny+1).  uc/vc carry, us/vs are advected.  The
+// values go to *uo_v and *vo_v; returns the stencil origins of both faces'
+// samples as one byte (u face in bits 0-3, v face in bits 4-7; 0 for solid
+// faces).  closed: the two faces' solid tests when the caller has loaded
+// them (closed_faces; -1: read lbl here).  cross (may be null): the carrier
+// sampled across at each open face, v at the u face then u at the v face
+// (maccormack_faces_t samples the same).
+template <class CU, class CV, class SU, class SV>
+__device__ __forceinline__ uint32_t advect_faces_cell(const CU& uc, const CV& vc, const SU& us,
+                                                      const SV& vs,
+                                                      const uint8_t* __restrict__ lbl, int nx,
+                                                      int ny, float c, int i, int j, int closed,
+                                                      float* uo_v, float* vo_v,
+                                                      float* cross = nullptr) {
+  Pos bx, by;
+  uint32_t st = 0;
+  if (j < ny) {
+    if (closed >= 0 ? (closed & 1) != 0
+                    : solid_or_outside(lbl, nx, ny, i - 1, j) ||
+                          solid_or_outside(lbl, nx, ny, i, j)) {
+      *uo_v = us.at(i, j);
+    } else {
+      Pos X{i, 0.f}, Y{j, 0.5f};
+      float v0 = sample_vf(vc, nx, ny, X, Y);
+      if (cross) cross[0] = v0;
+      backtracef(uc, vc, nx, ny, X, Y, uc.at(i, j), v0, c, &bx, &by);
+      // sample_uf, with the stencil origin kept
+      Axis ax = axis_t<SU::kClamp>(bx.i, bx.f, nx + 1);
+      Axis ay = axis_t<SU::kClamp>(by.i, by.f - 0.5f, ny);
+      Quad g = gather4f(us, ax, ay);
+      *uo_v = bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
+      st = sten2(ax.i0 - i, ay.i0 - j);
+    }
+  }
+  if (i < nx) {
+    if (closed >= 0 ? (closed & 2) != 0
+                    : solid_or_outside(lbl, nx, ny, i, j - 1) ||
+                          solid_or_outside(lbl, nx, ny, i, j)) {
+      *vo_v = vs.at(i, j);
+    } else {
+      Pos X{i, 0.5f}, Y{j, 0.f};
+      float u0 = sample_uf(uc, nx, ny, X, Y);
+      if (cross) cross[1] = u0;
+      backtracef(uc, vc, nx, ny, X, Y, u0, vc.at(i, j), c, &bx, &by);
+      // sample_vf, with the stencil origin kept
+      Axis ax = axis_t<SV::kClamp>(bx.i, bx.f - 0.5f, nx);
+      Axis ay = axis_t<SV::kClamp>(by.i, by.f, ny + 1);
+      Quad g = gather4f(vs, ax, ay);
+      *vo_v = bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
+      st |= sten2(ax.i0 - i, ay.i0 - j) << 4;
+    }
+  }
+  return st;
+}
+
+// advect_faces_cell stored: u(i, j) into uo when j < ny, v(i, j) into vo
+// when i < nx, the stencil byte into sten[i + j (nx + 1)] (sten may be null).
 template <class CU, class CV, class SU, class SV>
 __device__ __forceinline__ void advect_faces_t(const CU& uc, const CV& vc, const SU& us,
                                                const SV& vs, float* __restrict__ uo,
@@ -63,44 +115,10 @@ __device__ __forceinline__ void advect_faces_t(const CU& uc, const CV& vc, const
                                                uint8_t* __restrict__ sten = nullptr,
                                                int closed = -1) {
   if (i > nx || j > ny) return;
-  Pos bx, by;
-  uint32_t st = 0;
-  if (j < ny) {
-    int k = i + j * (nx + 1);
-    if (closed >= 0 ? (closed & 1) != 0
-                    : solid_or_outside(lbl, nx, ny, i - 1, j) ||
-                          solid_or_outside(lbl, nx, ny, i, j)) {
-      uo[k] = us.at(i, j);
-    } else {
-      Pos X{i, 0.f}, Y{j, 0.5f};
-      float v0 = sample_vf(vc, nx, ny, X, Y);
-      backtracef(uc, vc, nx, ny, X, Y, uc.at(i, j), v0, c, &bx, &by);
-      // sample_uf, with the stencil origin kept
-      Axis ax = axis_t<SU::kClamp>(bx.i, bx.f, nx + 1);
-      Axis ay = axis_t<SU::kClamp>(by.i, by.f - 0.5f, ny);
-      Quad g = gather4f(us, ax, ay);
-      uo[k] = bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
-      st = sten2(ax.i0 - i, ay.i0 - j);
-    }
-  }
-  if (i < nx) {
-    int k = i + j * nx;
-    if (closed >= 0 ? (closed & 2) != 0
-                    : solid_or_outside(lbl, nx, ny, i, j - 1) ||
-                          solid_or_outside(lbl, nx, ny, i, j)) {
-      vo[k] = vs.at(i, j);
-    } else {
-      Pos X{i, 0.5f}, Y{j, 0.f};
-      float u0 = sample_uf(uc, nx, ny, X, Y);
-      backtracef(uc, vc, nx, ny, X, Y, u0, vc.at(i, j), c, &bx, &by);
-      // sample_vf, with the stencil origin kept
-      Axis ax = axis_t<SV::kClamp>(bx.i, bx.f - 0.5f, nx);
-      Axis ay = axis_t<SV::kClamp>(by.i, by.f, ny + 1);
-      Quad g = gather4f(vs, ax, ay);
-      vo[k] = bilerp(g.a, g.b, g.c, g.d, ax.f, ay.f);
-      st |= sten2(ax.i0 - i, ay.i0 - j) << 4;
-    }
-  }
+  float a = 0.f, b = 0.f;
+  const uint32_t st = advect_faces_cell(uc, vc, us, vs, lbl, nx, ny, c, i, j, closed, &a, &b);
+  if (j < ny) uo[i + j * (nx + 1)] = a;
+  if (i < nx) vo[i + j * nx] = b;
   if (sten) sten[i + j * (nx + 1)] = (uint8_t)st;
 }
 
@@ -133,7 +151,8 @@ __device__ __forceinline__ float buoyancy_value(float val, const float* __restri
 // its own uBar, so fusing changes no value.  has_sten: st is this index's
 // byte of the forward pass's stencil origins (advect_faces_t), used instead
 // of tracing forward again; the caller sets it only where the offsets are
-// known to fit.
+// known to fit.  cross (may be null): advect_faces_cell's cross samples of
+// this index, used instead of sampling them again.
 template <class NU, class NV, class HU, class HV>
 __device__ __forceinline__ void maccormack_faces_t(const NU& un, const NV& vn, const HU& uh,
                                                    const HV& vh, float* __restrict__ uo,
@@ -142,7 +161,8 @@ __device__ __forceinline__ void maccormack_faces_t(const NU& un, const NV& vn, c
                                                    float c, int i, int j, bool has_sten = false,
                                                    uint32_t st = 0,
                                                    const float* __restrict__ heat = nullptr,
-                                                   float gain = 0.f, int closed = -1) {
+                                                   float gain = 0.f, int closed = -1,
+                                                   const float* cross = nullptr) {
   if (i > nx || j > ny) return;
   Pos rx, ry, fx, fy;
   if (j < ny) {
@@ -154,7 +174,7 @@ __device__ __forceinline__ void maccormack_faces_t(const NU& un, const NV& vn, c
     } else {
       Pos X{i, 0.f}, Y{j, 0.5f};
       float u0 = un.at(i, j);
-      float v0 = sample_vf(vn, nx, ny, X, Y);
+      float v0 = cross ? cross[0] : sample_vf(vn, nx, ny, X, Y);
       backtracef(un, vn, nx, ny, X, Y, u0, v0, -c, &rx, &ry);
       float ubar = sample_uf(uh, nx, ny, rx, ry);
       float corrected = uh.at(i, j) + 0.5f * (u0 - ubar);
@@ -179,7 +199,7 @@ __device__ __forceinline__ void maccormack_faces_t(const NU& un, const NV& vn, c
     } else {
       Pos X{i, 0.5f}, Y{j, 0.f};
       float v0 = vn.at(i, j);
-      float u0 = sample_uf(un, nx, ny, X, Y);
+      float u0 = cross ? cross[1] : sample_uf(un, nx, ny, X, Y);
       backtracef(un, vn, nx, ny, X, Y, u0, v0, -c, &rx, &ry);
       float vbar = sample_vf(vh, nx, ny, rx, ry);
       float corrected = vh.at(i, j) + 0.5f * (v0 - vbar);

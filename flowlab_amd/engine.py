@@ -229,6 +229,11 @@ class HipEngine:
         passes."""
         check(self.lib.fh_set_dye_one_pass(self._e, int(bool(on))))
 
+    def set_faces_one_pass(self, on: bool):
+        """MacCormack velocity advection in one kernel (default) or as its
+        two passes."""
+        check(self.lib.fh_set_faces_one_pass(self._e, int(bool(on))))
+
     def coarse_group_threads(self) -> int:
         return int(self.lib.fh_coarse_group_threads(self._e))
 

@@ -71,6 +71,7 @@ SIGNATURES = [
     ("fh_set_coarse_group", _i, [_vp, _i]),
     ("fh_set_advect_staging", _i, [_vp, _i]),
     ("fh_set_dye_one_pass", _i, [_vp, _i]),
+    ("fh_set_faces_one_pass", _i, [_vp, _i]),
     ("fh_coarse_group_threads", _i, [_vp]),
     ("fh_noalloc_last_error", C.c_char_p, []),
     ("fh_noalloc_max_cells", C.c_longlong, []),

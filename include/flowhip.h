@@ -214,6 +214,8 @@ int fh_set_advect_staging(fh_engine* e, int on);
  * the two passes.  Results are the same bits either way.  No counterpart
  * in the reference. */
 int fh_set_dye_one_pass(fh_engine* e, int on);
+/* The same for the MacCormack velocity (advect.ts:72-112 then 177-210). */
+int fh_set_faces_one_pass(fh_engine* e, int on);
 
 /* ---- allocation-free pressure solve (no reference counterpart) ----------
  * For callers that cannot allocate device memory (code under HIP graph

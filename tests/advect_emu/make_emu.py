@@ -47,6 +47,11 @@ def make(out: str) -> None:
          "emu_within_reach(lds, c, kReach, T::WN, WWF)", 1),
         ("    const F h0 = window_as<F>(hat, 0,",
          "    emu_win_h(T::RW) = T::RH;\n    const F h0 = window_as<F>(hat, 0,", 1),
+        # the one-pass velocity kernel likewise
+        ("  if (!within_reach<NW>(m, c, tid)) {",
+         "  if (!emu_within_reach(lds, c, kReach, T::WN, WWF)) {", 1),
+        ("    const F ha = window_as<F>(hat, 0,",
+         "    emu_win_h(T::RW) = T::RH;\n    const F ha = window_as<F>(hat, 0,", 1),
     ]
     for a, b, k in subs:
         if src.count(a) != k:

@@ -49,13 +49,13 @@ def test_staged_advection_matches_global_gathers(scheme, cfl, nx, ny, ds):
 
 @pytest.mark.parametrize("nx,ny", [(300, 170), (64, 16), (517, 263)])
 @pytest.mark.parametrize("fast_band", [False, True])
-def test_one_pass_dye_matches_two_passes(nx, ny, fast_band):
-    """MacCormack dye in one kernel (fh_set_dye_one_pass) against its two
-    LDS-staged passes and against the plain global gathers: the same bits
-    (the velocity too, which runs the same kernels either way).  With a band
-    of fast flow, the tiles whose windows hold it exceed the staged reach, so
-    the one-pass kernel marks its launch and the two-pass kernels redo every
-    tile after it; the other tiles pass."""
+def test_one_pass_maccormack_matches_two_passes(nx, ny, fast_band):
+    """MacCormack velocity and dye, each in one kernel
+    (fh_set_faces_one_pass, fh_set_dye_one_pass), against their two
+    LDS-staged passes and against the plain global gathers: the same bits.
+    With a band of fast flow, the tiles whose windows hold it exceed the
+    staged reach, so the one-pass kernel marks its launch and the two-pass
+    kernels redo every tile after it; the other tiles pass."""
     from flowlab_amd.engine import HipAdvector
     adv = HipAdvector(nx, ny)
     u, v, lbl, q = _fields(nx, ny, 1, seed=nx + 3 * ny)
@@ -66,6 +66,7 @@ def test_one_pass_dye_matches_two_passes(nx, ny, fast_band):
     for staged, one in ((False, False), (True, False), (True, True)):
         adv.engine.set_advect_staging(staged)
         adv.engine.set_dye_one_pass(one)
+        adv.engine.set_faces_one_pass(one)
         outs.append(_advect(adv, staged, "macCormack", u, v, lbl, q, dt))
     for k in (1, 2):
         for x, y in zip(outs[0][:2] + tuple(outs[0][2]), outs[k][:2] + tuple(outs[k][2])):
@@ -95,6 +96,7 @@ def _steps(scene_name, one, steps):
         sim.stepper.invalidate()
     dye0 = np.concatenate([d.copy() for d in sim.f.dye])
     sim.stepper.engine.set_dye_one_pass(one)
+    sim.stepper.engine.set_faces_one_pass(one)
     dts = []
     for _ in range(steps):
         sim.step()
@@ -106,11 +108,11 @@ def _steps(scene_name, one, steps):
 
 @pytest.mark.parametrize("scene", ["karman", "plume"])
 def test_one_pass_steps_match_two_passes(scene):
-    """Twelve whole steps with the MacCormack dye in one pass or in two: u,
-    v, p, dye and every dt the same bits.  The Karman street at 330x170 with
-    a random dye field runs at CFL 0.5, so every dt comes from the max face
-    speed that the dye pass reports; the plume at 330x330 (buoyancy) reads
-    the advected heat in the next step's velocity pass."""
+    """Twelve whole steps with the MacCormack velocity and dye in one pass
+    each or in two: u, v, p, dye and every dt the same bits.  The Karman
+    street at 330x170 with a random dye field runs at CFL 0.5, so every dt
+    comes from the max face speed that the dye pass reports; the plume at
+    330x330 applies buoyancy in the velocity pass."""
     a = _steps(scene, False, 12)
     b = _steps(scene, True, 12)
     assert a[4] == b[4]
