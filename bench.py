@@ -43,9 +43,12 @@ WORKLOAD_STEPS = 200
 
 # Algorithmic (compulsory) bytes per cell of one launch of each traceable
 # kernel: every array element read or written once, neighbour reuse perfect.
-#   maccormack_faces: u^n, v^n 8 + forward-pass u, v 8 + out u, v 8 + label 1
-#     + the limiter stencils the forward pass recorded 1; with buoyancy alone
-#     as the body force it applies it too, reading the heat (+4).
+#   maccormack_faces: as one pass (the engine's default: the forward-pass
+#     samples computed in LDS, FACES_ONE_PASS) u^n, v^n 8 + out u, v 8 +
+#     label 1 (17); as the correction pass of two, u^n, v^n 8 + forward-pass
+#     u, v 8 + out u, v 8 + label 1 + the limiter stencils the forward pass
+#     recorded 1 (26); with buoyancy alone as the body force either applies
+#     it too, reading the heat (+4).
 #   advect_faces: carrier = source u, v 8 + out u, v 8 + label 1 + the
 #     recorded limiter stencils 1 (MacCormack's forward pass).
 #   maccormack_dye: as one pass (the engine's default at dye scale 1: the
@@ -95,6 +98,9 @@ FUSED_UPDATE = True
 # the engines run the MacCormack dye as one pass (fh_set_dye_one_pass), traced
 # as maccormack_dye
 DYE_ONE_PASS = True
+# and the MacCormack velocity as one pass (fh_set_faces_one_pass), traced as
+# maccormack_faces
+FACES_ONE_PASS = True
 # start of each step's solve: the previous pressure (the reference's), or the
 # last two solutions extrapolated linearly in time (fh_set_warm_extrapolation)
 WARM_STARTS = {"previous": (0, "the previous pressure"),
@@ -108,6 +114,8 @@ def algo_bytes(kernel, solver, buoyancy=True, warm="previous"):
     pass also reads the previous solution and writes the start it forms,
     +16)."""
     b = ALGO_BYTES_PER_CELL.get(kernel)
+    if kernel == "maccormack_faces" and FACES_ONE_PASS:
+        b = 17.0
     if kernel == "maccormack_faces" and buoyancy:
         b += 4.0
     if kernel == "maccormack_dye" and DYE_ONE_PASS:
@@ -183,7 +191,7 @@ def measured_traffic(kernel, n):
 # the kernels whose traffic the table reports
 PMC_KERNELS = {
     "k_maccormack_dye_fused": "maccormack_dye", "k_maccormack_dye_lds": "maccormack_dye",
-    "k_maccormack_faces_lds": "maccormack_faces",
+    "k_maccormack_faces_fused": "maccormack_faces", "k_maccormack_faces_lds": "maccormack_faces",
     "k_advect_dye_lds": "advect_dye", "k_advect_faces_lds": "advect_faces",
     "k_prolong_smooth_dc": "dc_update", "k_fcg_update": "fcg_update", "k_fcg_dots": "fcg_dots",
     "k_smooth_restrict": "smooth_restrict_fine", "k_prolong_smooth": "prolong_smooth_fine",

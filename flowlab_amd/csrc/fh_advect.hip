@@ -381,7 +381,7 @@ __device__ __forceinline__ void advect_dye_tile(
     sol[r] = !kBatch ? -1 : (i < nx && j + r * BY < ny) ? lbl[i + (j + r * BY) * nx] == SOLID : 1;
   float sp = 0.f;
   const float m =
-      stage_windows<5, Tile<TY>::WN, kBatch>(lds, f, wx0, wy0, tid, speed ? &sp : nullptr);
+      stage_windows<5, Tile<TY>::WN, kBatch>(lds, f, wx0, wy0, tid, &sp);
   const bool reach =
       speed ? within_reach_speed(m, sp, c, tid, speed, bid) : within_reach(m, c, tid);
   if (reach) {
@@ -588,8 +588,7 @@ __global__ __launch_bounds__(TX * BYF) void k_maccormack_dye_fused(
     sol[k] = (has && i >= 0 && j >= 0 && i < nx && j < ny) ? lbl[i + j * nx] == SOLID : -1;
   }
   float sp = 0.f;
-  const float m = stage_windows<5, T::WN, false, WWF, NT>(lds, f, wx0, wy0, tid,
-                                                           speed ? &sp : nullptr);
+  const float m = stage_windows<5, T::WN, false, WWF, NT>(lds, f, wx0, wy0, tid, &sp);
   const int bid = (int)(blockIdx.x + blockIdx.y * gridDim.x);
   const bool reach = speed ? within_reach_speed<NW>(m, sp, c, tid, speed, bid)
                            : within_reach<NW>(m, c, tid);
