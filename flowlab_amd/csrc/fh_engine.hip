@@ -1097,9 +1097,12 @@ fh_engine* fh_create(int nx, int ny, int dye_scale, double h) {
     return bail();
   size_t nblk = (size_t)((nx + 63) / 64) * ((ny + 3) / 4);
   e->partials_n = nblk;
-  if (dalloc(e, &e->partials, 4 * nblk + 4 * kFoldBlocks) || dalloc(e, &e->b2, nc) || dalloc(e, &e->counter, 1) || dalloc(e, &e->sums, 8) ||
-      dalloc(e, &e->fpartials,
-             std::max<size_t>(1024, (size_t)advect_dye_speed_blocks(nx, ny))) ||
+  // the max-speed partials of either forward dye pass (two-pass or one-pass)
+  const size_t nspeed = (size_t)std::max(advect_dye_speed_blocks(nx, ny),
+                                         maccormack_dye_fused_blocks(nx, ny));
+  if (dalloc(e, &e->partials, 4 * nblk + 4 * kFoldBlocks) || dalloc(e, &e->b2, nc) ||
+      dalloc(e, &e->counter, 1) || dalloc(e, &e->sums, 8) ||
+      dalloc(e, &e->fpartials, std::max<size_t>(1024, nspeed)) ||
       dalloc(e, &e->fmax, 1) || dalloc(e, &e->done, 1))
     return bail();
   if (hipHostMalloc((void**)&e->h_sums, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess ||
