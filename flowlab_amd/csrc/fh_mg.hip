@@ -114,22 +114,37 @@ __global__ __launch_bounds__(kSorThreads) void k_rb_sor_lds(float* __restrict__ 
       const int f = (q < items && i < nx) ? si[k] : 0;
       pk[c][r] = ((f & 0x80) && (f & 7)) ? (uint32_t)k | ((uint32_t)f << 24) : 0u;
     }
+  // A cell's own value changes only when its one owner (this thread, this
+  // slot) updates it, so the owner keeps it and the cell's b in registers:
+  // a half-sweep reads only the four neighbours from LDS.  Each neighbour
+  // read is unconditional, from the cell itself when that side is closed
+  // (always in the window), and the closed side's +0 is selected after.
+  float xo[2][kSorItems], bo[2][kSorItems];
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+#pragma unroll
+    for (int r = 0; r < kSorItems; r++) {
+      const int k = (int)(pk[c][r] & 0xffffffu);
+      xo[c][r] = sx[k];
+      bo[c][r] = sb[k];
+    }
   auto half = [&](auto cc) {
     constexpr int c = decltype(cc)::value;
-    float nv[kSorItems];
 #pragma unroll
     for (int r = 0; r < kSorItems; r++) {
       const int k = (int)(pk[c][r] & 0xffffffu), f = (int)(pk[c][r] >> 24);
-      const float ve = (f & 0x08) ? sx[k + 1] : 0.f;
-      const float vw = (f & 0x10) ? sx[k - 1] : 0.f;
-      const float vn = (f & 0x20) ? sx[k + nx] : 0.f;
-      const float vs = (f & 0x40) ? sx[k - nx] : 0.f;
-      const float sum = (((0.f + ve) + vw) + vn) + vs;
-      nv[r] = f ? rb_value(sum, f & 7, sb[k], sx[k], omega, scale) : 0.f;
+      const float ve = sx[(f & 0x08) ? k + 1 : k];
+      const float vw = sx[(f & 0x10) ? k - 1 : k];
+      const float vn = sx[(f & 0x20) ? k + nx : k];
+      const float vs = sx[(f & 0x40) ? k - nx : k];
+      const float sum = (((0.f + ((f & 0x08) ? ve : 0.f)) + ((f & 0x10) ? vw : 0.f)) +
+                         ((f & 0x20) ? vn : 0.f)) +
+                        ((f & 0x40) ? vs : 0.f);
+      if (f) xo[c][r] = rb_value(sum, f & 7, bo[c][r], xo[c][r], omega, scale);
     }
 #pragma unroll
     for (int r = 0; r < kSorItems; r++)
-      if (pk[c][r]) sx[pk[c][r] & 0xffffffu] = nv[r];
+      if (pk[c][r]) sx[pk[c][r] & 0xffffffu] = xo[c][r];
     __syncthreads();
   };
   for (int it = 0; it < iters; it++) {
